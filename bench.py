@@ -1,0 +1,169 @@
+"""Benchmark of the DQMC sweep engine on one MI355X.
+
+A step is one single-direction sweep of every chain of the batch (one
+k_sweep launch).  The default workload is the batched configuration: 256
+independent chains of the 12x12 repulsive Hubbard model (t=1, U=4, mu=0,
+beta=6, 60 slices, stabilization pace 10, checkerboard break-up, FP64),
+seeds 0..255.  Beside it the same chain shape runs alone (the plain,
+one-chain-per-launch path) so the batched/plain ratio is reported, and the
+oracle (one chain per process, FP64 CPU restatement of the reference) is
+timed on the host cores for the CPU baseline.
+
+Prints ONE JSON line.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dqmc-framework_amd"))
+
+import dqmc_amd as D  # noqa: E402
+
+METRIC = "chain-sweeps/s batched vs plain FP64 GPU at max|dG|<=1e-6; p50/p99 job latency"
+
+WORKLOADS = {
+    "batched-12x12": dict(chains=256, side=12, time_size=60, beta=6.0, pace=10, model=D.REPULSIVE,
+                          checkerboard=True, U=4.0, mu=0.0, delay=16,
+                          desc="256 chains x 12x12 repulsive Hubbard t=1 U=4 mu=0 beta=6 L=60 p=10 checkerboard FP64"),
+    "plain-8x8": dict(chains=1, side=8, time_size=160, beta=8.0, pace=10, model=D.ATTRACTIVE,
+                      checkerboard=True, U=4.0, mu=-0.5, delay=16,
+                      desc="1 chain 8x8 attractive Hubbard t=1 |U|=4 mu=-0.5 beta=8 L=160 p=10 checkerboard k=16 FP64"),
+    "plain-4x4": dict(chains=1, side=4, time_size=80, beta=4.0, pace=10, model=D.REPULSIVE,
+                      checkerboard=False, U=4.0, mu=0.0, delay=16,
+                      desc="1 chain 4x4 repulsive Hubbard t=1 U=4 mu=0 beta=4 L=80 p=10 dense FP64"),
+}
+
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def chain_params(w, chains=None, seed0=0):
+    n = w["chains"] if chains is None else chains
+    return [D.make_params(w["side"], w["time_size"], w["beta"], pace=w["pace"], model=w["model"],
+                          checkerboard=w["checkerboard"], delay=w["delay"], U=w["U"], mu=w["mu"], seed=seed0 + c)
+            for c in range(n)]
+
+
+def algorithmic_flops_per_chain_sweep(w, accepted_per_chain_sweep):
+    """Per chain-sweep FLOPs (both spins; attractive counts one spin):
+    wrap 8LN^3 dense / 64LN^2 checkerboard, slice products 4LN^3 / 32LN^2,
+    updates 4 A N^2, stabilization 29.3 n_s N^3 (pivoted-QR UDV, both spins)."""
+    N = w["side"] ** 2
+    L = w["time_size"]
+    ns = -(-L // w["pace"])
+    spins = 1 if w["model"] == D.ATTRACTIVE else 2
+    if w["checkerboard"]:
+        wrap, prod = 64.0 * L * N * N, 32.0 * L * N * N
+    else:
+        wrap, prod = 8.0 * L * N ** 3, 4.0 * L * N ** 3
+    upd = 4.0 * accepted_per_chain_sweep * N * N
+    stab = 29.3 * ns * N ** 3
+    return (wrap + prod + upd + stab) * spins / 2.0
+
+
+def run_gpu(w, steps, warmup, chains=None):
+    ctx = D.Context(0)
+    b = D.Batch(ctx, chain_params(w, chains))
+    b.fields_random()
+    b.init()
+    for _ in range(warmup):
+        b.sweep(1)
+    b.synchronize()
+    n = b.n
+    acc0 = 0
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.sweep(1)
+        kernel_ms.append(b.last_sweep_ms())
+        acc0 += sum(b.info(c).accepted for c in range(min(n, 8)))
+    b.synchronize()
+    wall = time.perf_counter() - t0
+    acc_per_chain_sweep = acc0 / (min(n, 8) * steps)
+    wrap_err = max(b.info(c).wrap_error for c in range(n))
+    statuses = [b.info(c).status for c in range(n)]
+    b.close()
+    ctx.close()
+    return dict(wall=wall, kernel_ms=kernel_ms, chains=n, accepted=acc_per_chain_sweep, wrap_error=wrap_err,
+                status=max(statuses))
+
+
+def cpu_baseline(w, seconds=12.0, max_procs=16):
+    exe = os.path.join(ROOT, "oracle", "_build", "refcpu")
+    if not os.path.exists(exe):
+        subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+    procs = max(1, min(max_procs, os.cpu_count() or 1))
+    args = [exe, "--side", str(w["side"]), "--slices", str(w["time_size"]), "--beta", str(w["beta"]),
+            "--pace", str(w["pace"]), "--model", str(w["model"]), "--cb", str(int(w["checkerboard"])),
+            "--U", str(w["U"]), "--mu", str(w["mu"]), "--warm", "100000", "--bins", "0",
+            "--max-seconds", str(seconds)]
+    ps = [subprocess.Popen(args + ["--seed", str(c)], stdout=subprocess.PIPE) for c in range(procs)]
+    total = 0.0
+    for p in ps:
+        out, _ = p.communicate()
+        total += json.loads(out.decode().strip().splitlines()[-1])["sweeps_per_s"]
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": total, "unit": "chain-sweeps/s", "cores": procs, "kind": "port",
+            "sample": f"{procs} oracle processes (one chain each, {w['side']}x{w['side']}, L={w['time_size']}), "
+                      f"~{seconds:.0f} s of sweeps each, on {model or 'host CPU'}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--workload", default="batched-12x12", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-plain", action="store_true", help="skip the single-chain comparison run")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: chains are independent and would be sharded by chain across GPUs, "
+              f"but only the single-GPU figure is measured here (asked for --gpus {args.gpus}).", file=sys.stderr)
+        sys.exit(2)
+    w = WORKLOADS[args.workload]
+    print(f"[bench] {w['desc']}: warmup {args.warmup}, steps {args.steps}", file=sys.stderr, flush=True)
+    r = run_gpu(w, args.steps, args.warmup)
+    chain_sweeps = r["chains"] * args.steps
+    value = chain_sweeps / r["wall"]
+    ms_per_step = 1000.0 * r["wall"] / args.steps
+    kms = sum(r["kernel_ms"]) / len(r["kernel_ms"])
+    flops_launch = algorithmic_flops_per_chain_sweep(w, r["accepted"]) * r["chains"]
+    achieved = flops_launch / (kms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC, "value": value, "unit": "chain-sweeps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": w["desc"], "chains": r["chains"], "side": w["side"], "time_size": w["time_size"],
+                   "beta": w["beta"], "pace": w["pace"], "checkerboard": w["checkerboard"], "delay": w["delay"]},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "k_sweep", "kernel_ms": kms, "algorithmic_gflop_per_launch": flops_launch / 1e9},
+        "accepted_per_chain_sweep": r["accepted"], "max_wrap_error": r["wrap_error"],
+    }
+    if not args.no_plain and r["chains"] > 1:
+        print("[bench] plain path: the same chain alone, one chain per launch", file=sys.stderr, flush=True)
+        p = run_gpu(w, max(2, args.steps // 4), 1, chains=1)
+        plain = p["chains"] * max(2, args.steps // 4) / p["wall"]
+        out["plain_chain_sweeps_per_s"] = plain
+        out["batched_over_plain"] = value / plain
+    if not args.no_cpu:
+        print("[bench] CPU baseline (oracle, one chain per process)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(w)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

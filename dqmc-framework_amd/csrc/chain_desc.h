@@ -1,0 +1,62 @@
+// Per-chain descriptor and mutable state shared by the host runtime and the
+// HIP kernels.  Plain C layout: one array of ChainDesc lives in device memory
+// and every workgroup of a sweep launch reads the descriptor of its chain.
+#pragma once
+#include <stdint.h>
+
+#define DQMC_NT 256          // threads per workgroup (4 wavefronts of 64)
+#define DQMC_NMAX 512        // largest lattice (sites) a chain may have
+#define DQMC_KMAX 64         // largest delayed-update depth
+
+enum {
+    DQMC_FLAG_MEASURE = 1,   // accumulate equal-time observables after each sweep
+    DQMC_FLAG_RECORD = 2,    // keep G, sign per slice and accept bits of the last sweep
+    DQMC_FLAG_FROZEN = 4,    // propagate and measure without field updates (fixed configuration)
+};
+
+// number of accumulated equal-time quantities, in this order:
+// sign, filling, double occupancy, kinetic energy, momentum distribution,
+// local spin correlation, SDW structure factor, CDW structure factor, s-wave pairing
+#define DQMC_NUM_OBS 9
+
+struct ChainState {
+    uint64_t rng;        // minstd_rand0 state
+    double sign;         // current configuration sign
+    double wrap_error;   // max |G_fresh - G_wrapped| seen so far
+    double sign_top;     // sign recorded at slice L of the running backward sweep
+    int pos;             // 0: at slice 0 (next sweep goes up), L: at slice L (next goes down)
+    int n_right;         // length of the right UDV stack (left length = n_stack - n_right)
+    int status;          // 0 ok; nonzero: numerical failure code
+    int accepted;        // accepted flips in the last launch (for reporting)
+};
+
+struct ChainDesc {
+    int N, L, side, pace, n_stack, model, cb, kdelay, nspin, precision;
+    int launch_sweeps;   // > 0: sweeps of this chain in the current launch (overrides the launch argument)
+    int launch_flags;    // >= 0: flags of this chain in the current launch (overrides the launch argument)
+    double hop_t;
+    double ev[2];        // exp(+alpha), exp(-alpha): exp(alpha*s) for s=+1, s=-1
+    double dm1[2][2];    // [spin][s<0]: exp(-2 sigma alpha s) - 1 entering ratio and gamma
+    double pref[2];      // attractive ratio prefactor exp(2 alpha s) for s=+1, s=-1 (1 for repulsive)
+    double P[3];         // checkerboard plaquette entries e^{dtau mu/2}{ch*ch, ch*sh, sh*sh}
+    double Pi[3];        // inverse plaquette entries e^{-dtau mu/2}{ch*ch, -ch*sh, sh*sh}
+    // device pointers (all inside the batch arena)
+    double* G;           // nspin x N*N   current equal-time Green's function
+    double* Gw;          // nspin x N*N   product / fresh-G workspace
+    double* Tmp;         // nspin x N*N   slice-product accumulated between stabilizations
+    double* stkU;        // nspin x n_stack x N*N
+    double* stkT;        // nspin x n_stack x N*N   (cumulative V^T of each level)
+    double* stkD;        // nspin x n_stack x N
+    double* W;           // 3 x N*N scratch for the stabilization
+    double* X;           // nspin x N*kdelay   delayed-update columns
+    double* Y;           // nspin x N*kdelay   delayed-update rows
+    const double* expK;  // N*N (dense only)
+    const double* invExpK;
+    const double* cosq;  // N: Re e^{-i Q r_d} for displacement index d
+    int8_t* field;       // L x N, (t,i) at t*N+i
+    ChainState* st;
+    double* acc;         // DQMC_NUM_OBS sums
+    double* recG;        // L x nspin x N*N (RECORD)
+    double* recSign;     // L (RECORD)
+    uint8_t* recAcc;     // L x N (RECORD)
+};

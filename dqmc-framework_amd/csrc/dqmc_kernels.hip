@@ -1,0 +1,1000 @@
+// HIP kernels of the DQMC engine for gfx950 (MI355X).
+//
+// Execution model: one 256-thread workgroup owns one Markov chain for a whole
+// launch.  A launch sweeps every chain of a batch (chains may differ in N, L
+// and model) through n single-direction sweeps: slice wraps, delayed-update
+// Metropolis, slice-product accumulation, UDV-stack stabilization and the
+// equal-time measurements all run inside the workgroup, with the chain's
+// state (G, stacks, fields, random state) resident in HBM between launches.
+// Per-chain small vectors (diagonals, scalings, field row, pivots) live in LDS.
+//
+// Reference semantics followed (file:line of dqmc-framework):
+//   sweeps        src/dqmc_walker.cpp:246-412
+//   Metropolis    src/dqmc_walker.cpp:173-197, src/model/repulsive_hubbard.cpp:112-155,
+//                 src/model/attractive_hubbard.cpp:112-157
+//   B products    src/model/repulsive_hubbard.cpp:158-243, src/model/model_base.cpp:7-35,
+//                 src/checkerboard/square.cpp:66-226
+//   stacks        src/svd_stack.cpp:34-87 (UDV by pivoted QR instead of SVD; the
+//                 Green's function is independent of the factorization)
+//   recompute     include/utils/numerical_stable.hpp:132-152, :186-239
+//   measurements  src/measure/measure_methods.cpp:22-255
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "chain_desc.h"
+
+#define NT DQMC_NT
+#define NMAX DQMC_NMAX
+
+namespace dqmc {
+
+// ---------------------------------------------------------------------------
+// LDS layout of one workgroup
+// ---------------------------------------------------------------------------
+struct Lds {
+    double As[16][64];         // GEMM A tile (k-major)
+    double Bs[16][64];         // GEMM B tile (k-major)
+    double du[NMAX], dd[NMAX]; // diagonals of the current G (delayed updates, measurements)
+    double scp[NMAX], scm[NMAX];   // exp(+alpha s_i), exp(-alpha s_i) of the current field row
+    double v0[NMAX], v1[NMAX], v2[NMAX], v3[NMAX];  // stabilization / QR scratch vectors
+    int perm[NMAX];
+    int frow[NMAX];            // field row of the current slice
+    double red[16];
+    double gam[2];
+    int ivar[4];
+};
+
+__device__ __forceinline__ int tid() { return threadIdx.x; }
+__device__ __forceinline__ int lane() { return threadIdx.x & 63; }
+__device__ __forceinline__ int wave() { return threadIdx.x >> 6; }
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+// argmax with smallest index on ties
+__device__ __forceinline__ void wave_argmax(double& v, int& idx) {
+    for (int o = 32; o > 0; o >>= 1) {
+        double ov = __shfl_xor(v, o);
+        int oi = __shfl_xor(idx, o);
+        if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+    }
+}
+// sum over the workgroup of n values at once (4 per LDS pass); all threads get the result
+__device__ void wg_sum8(double* v, int n, Lds& s) {
+    for (int q = 0; q < n; ++q) v[q] = wave_sum(v[q]);
+    for (int base = 0; base < n; base += 4) {
+        const int m = (n - base) < 4 ? (n - base) : 4;
+        if (lane() == 0)
+            for (int q = 0; q < m; ++q) s.red[q * 4 + wave()] = v[base + q];
+        __syncthreads();
+        for (int q = 0; q < m; ++q) v[base + q] = s.red[q * 4 + 0] + s.red[q * 4 + 1] + s.red[q * 4 + 2] + s.red[q * 4 + 3];
+        __syncthreads();
+    }
+}
+__device__ double wg_max(double v, Lds& s) {
+    v = wave_max(v);
+    if (lane() == 0) s.red[wave()] = v;
+    __syncthreads();
+    double r = fmax(fmax(s.red[0], s.red[1]), fmax(s.red[2], s.red[3]));
+    __syncthreads();
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Random numbers: minstd_rand0 and the canonical double of
+// std::bernoulli_distribution (two engine draws per decision, C++ standard
+// [rand.util.canonical] with k = 2).  Evaluated without contraction so that
+// the decision bits equal those of the CPU generator.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t minstd_next(uint64_t x) {
+    const uint64_t p = 16807ull * x;             // < 2^46
+    uint64_t r = (p & 0x7fffffffull) + (p >> 31); // p mod (2^31-1), one fold
+    if (r >= 0x7fffffffull) r -= 0x7fffffffull;
+    return r;
+}
+__device__ __noinline__ double canonical(uint64_t& x) {
+#pragma clang fp contract(off)
+    const double R = 2147483646.0;
+    x = minstd_next(x);
+    const double a = (double)(x - 1);
+    x = minstd_next(x);
+    const double b = (double)(x - 1);
+    double sum = 0.0, tmp = 1.0;
+    sum = sum + a * tmp;
+    tmp = tmp * R;
+    sum = sum + b * tmp;
+    tmp = tmp * R;
+    double ret = sum / tmp;
+    if (ret >= 1.0) ret = 0x1.fffffffffffffp-1;
+    return ret;
+}
+
+// ---------------------------------------------------------------------------
+// Workgroup GEMM (FP64):  C = rs .* (opA(A) diag(ms) opB(B)) .* cs + beta*C
+// column-major; 64x64 output tiles, 4x4 register micro-tiles per thread,
+// 16-deep k-slabs staged through LDS.  rs/ms/cs may be null (=1).
+// C must not alias A or B.
+// ---------------------------------------------------------------------------
+__device__ void wg_gemm(int M, int Nc, int K, const double* A, int lda, bool tA, const double* B, int ldb, bool tB,
+                        double* C, int ldc, const double* rs, const double* ms, const double* cs, double beta, Lds& s) {
+    const int tx = tid() & 15, ty = tid() >> 4;
+    const int tm = (M + 63) >> 6, tn = (Nc + 63) >> 6;
+    for (int tile = 0; tile < tm * tn; ++tile) {
+        const int r0 = (tile % tm) << 6, c0 = (tile / tm) << 6;
+        double acc[4][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[r][c] = 0.0;
+        for (int k0 = 0; k0 < K; k0 += 16) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int e = tid() + NT * q;
+                int m, k;
+                if (!tA) { m = e & 63; k = e >> 6; } else { k = e & 15; m = e >> 4; }
+                const int gm = r0 + m, gk = k0 + k;
+                double v = 0.0;
+                if (gm < M && gk < K) {
+                    v = tA ? A[(size_t)gm * lda + gk] : A[(size_t)gk * lda + gm];
+                    if (ms) v *= ms[gk];
+                }
+                s.As[k][m] = v;
+                int n;
+                if (!tB) { k = e & 15; n = e >> 4; } else { n = e & 63; k = e >> 6; }
+                const int gn = c0 + n, gk2 = k0 + k;
+                double w = 0.0;
+                if (gn < Nc && gk2 < K) w = tB ? B[(size_t)gk2 * ldb + gn] : B[(size_t)gn * ldb + gk2];
+                s.Bs[k][n] = w;
+            }
+            __syncthreads();
+#pragma unroll 4
+            for (int k = 0; k < 16; ++k) {
+                double a[4], b[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) a[r] = s.As[k][tx + 16 * r];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) b[c] = s.Bs[k][ty + 16 * c];
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) acc[r][c] += a[r] * b[c];
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int gn = c0 + ty + 16 * c;
+            if (gn >= Nc) continue;
+            const double csc = cs ? cs[gn] : 1.0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gm = r0 + tx + 16 * r;
+                if (gm >= M) continue;
+                double v = acc[r][c] * csc;
+                if (rs) v *= rs[gm];
+                double* cp = C + (size_t)gn * ldc + gm;
+                *cp = (beta != 0.0) ? v + beta * (*cp) : v;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// The same contract on the matrix cores.  Each of the 4 wavefronts owns a
+// 32x32 quarter of the 64x64 tile as 2x2 blocks of v_mfma_f64_16x16x4f64.
+// Operand lanes: A[row = lane%16][k = lane/16], B[k = lane/16][col = lane%16];
+// result register i of lane l holds D[l/16 + 4i][l%16] (FP64 16x16x4 layout).
+// With lowp the tile is rounded to FP32 and multiplied by
+// v_mfma_f32_16x16x4f32 with FP32 accumulation (result D[4(l/16)+i][l%16]).
+// ---------------------------------------------------------------------------
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+template <bool lowp>
+__device__ void wg_gemm_mfma(int M, int Nc, int K, const double* A, int lda, bool tA, const double* B, int ldb, bool tB,
+                             double* C, int ldc, const double* rs, const double* ms, const double* cs, double beta,
+                             Lds& s) {
+    const int l = lane(), w = wave();
+    const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
+    const int tm = (M + 63) >> 6, tn = (Nc + 63) >> 6;
+    float* Af = reinterpret_cast<float*>(&s.As[0][0]);  // [16][64] floats in lowp mode
+    float* Bf = reinterpret_cast<float*>(&s.Bs[0][0]);
+    for (int tile = 0; tile < tm * tn; ++tile) {
+        const int r0 = (tile % tm) << 6, c0 = (tile / tm) << 6;
+        dbl4 acc[2][2];
+        flt4 accf[2][2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+                accf[a][b] = flt4{0.f, 0.f, 0.f, 0.f};
+            }
+        for (int k0 = 0; k0 < K; k0 += 16) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int e = tid() + NT * q;
+                int m, k;
+                if (!tA) { m = e & 63; k = e >> 6; } else { k = e & 15; m = e >> 4; }
+                const int gm = r0 + m, gk = k0 + k;
+                double v = 0.0;
+                if (gm < M && gk < K) {
+                    v = tA ? A[(size_t)gm * lda + gk] : A[(size_t)gk * lda + gm];
+                    if (ms) v *= ms[gk];
+                }
+                if (lowp) Af[k * 64 + m] = (float)v; else s.As[k][m] = v;
+                int n;
+                if (!tB) { k = e & 15; n = e >> 4; } else { n = e & 63; k = e >> 6; }
+                const int gn = c0 + n, gk2 = k0 + k;
+                double x = 0.0;
+                if (gn < Nc && gk2 < K) x = tB ? B[(size_t)gk2 * ldb + gn] : B[(size_t)gn * ldb + gk2];
+                if (lowp) Bf[k * 64 + n] = (float)x; else s.Bs[k][n] = x;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const int k = kk * 4 + (l >> 4);
+                if (lowp) {
+                    const float a0 = Af[k * 64 + wr + (l & 15)], a1 = Af[k * 64 + wr + 16 + (l & 15)];
+                    const float b0 = Bf[k * 64 + wc + (l & 15)], b1 = Bf[k * 64 + wc + 16 + (l & 15)];
+                    accf[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, accf[0][0], 0, 0, 0);
+                    accf[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, accf[0][1], 0, 0, 0);
+                    accf[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, accf[1][0], 0, 0, 0);
+                    accf[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, accf[1][1], 0, 0, 0);
+                } else {
+                    const double a0 = s.As[k][wr + (l & 15)], a1 = s.As[k][wr + 16 + (l & 15)];
+                    const double b0 = s.Bs[k][wc + (l & 15)], b1 = s.Bs[k][wc + 16 + (l & 15)];
+                    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+                    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+                    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+                    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int row = r0 + wr + rb * 16 + (lowp ? 4 * (l >> 4) + i : (l >> 4) + 4 * i);
+                    const int col = c0 + wc + cb * 16 + (l & 15);
+                    if (row >= M || col >= Nc) continue;
+                    double v = lowp ? (double)accf[rb][cb][i] : acc[rb][cb][i];
+                    if (cs) v *= cs[col];
+                    if (rs) v *= rs[row];
+                    double* cp = C + (size_t)col * ldc + row;
+                    *cp = (beta != 0.0) ? v + beta * (*cp) : v;
+                }
+    }
+    __syncthreads();
+}
+
+// GEMMs of the engine: FP64 on the matrix cores for everything that stays
+// double precision; the slice products and wraps drop to FP32 MFMA when the
+// chain runs in mixed precision.
+__device__ __forceinline__ void gemm64(int M, int Nc, int K, const double* A, int lda, bool tA, const double* B,
+                                       int ldb, bool tB, double* C, int ldc, const double* rs, const double* ms,
+                                       const double* cs, double beta, Lds& s) {
+    wg_gemm_mfma<false>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, s);
+}
+__device__ __forceinline__ void gemmB(const ChainDesc& d, int M, int Nc, int K, const double* A, int lda, bool tA,
+                                      const double* B, int ldb, bool tB, double* C, int ldc, const double* rs,
+                                      const double* ms, const double* cs, Lds& s) {
+    if (d.precision == 1) wg_gemm_mfma<true>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, 0.0, s);
+    else wg_gemm_mfma<false>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, 0.0, s);
+}
+
+__device__ void wg_copy(double* dst, const double* src, int n2) {
+    for (int e = tid(); e < n2; e += NT) dst[e] = src[e];
+    __syncthreads();
+}
+__device__ void wg_identity(double* M, int N) {
+    for (int e = tid(); e < N * N; e += NT) M[e] = ((e % N) == (e / N)) ? 1.0 : 0.0;
+    __syncthreads();
+}
+__device__ void wg_scale_rows(double* M, int N, const double* sc) {
+    for (int e = tid(); e < N * N; e += NT) M[e] *= sc[e % N];
+    __syncthreads();
+}
+__device__ void wg_scale_cols(double* M, int N, const double* sc) {
+    for (int e = tid(); e < N * N; e += NT) M[e] *= sc[e / N];
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Checkerboard: one sublattice pass of 4x4 plaquette rotations (in place).
+// Plaquette at (x,y) with x,y = start mod 2 acts on the sites
+// {(x,y), (x+1,y), (x,y+1), (x+1,y+1)}; P = [[a,b,b,c],[b,a,c,b],[b,c,a,b],[c,b,b,a]].
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void plaq_sites(int p, int side, int start, int idx[4]) {
+    const int h = side >> 1;
+    const int x = start + 2 * (p % h), y = start + 2 * (p / h);
+    const int x1 = (x + 1) % side, y1 = (y + 1) % side;
+    idx[0] = x + side * y; idx[1] = x1 + side * y; idx[2] = x + side * y1; idx[3] = x1 + side * y1;
+}
+__device__ __forceinline__ void plaq_apply(const double P[3], const double v[4], double o[4]) {
+    const double a = P[0], b = P[1], c = P[2];
+    o[0] = a * v[0] + b * v[1] + b * v[2] + c * v[3];
+    o[1] = b * v[0] + a * v[1] + c * v[2] + b * v[3];
+    o[2] = b * v[0] + c * v[1] + a * v[2] + b * v[3];
+    o[3] = c * v[0] + b * v[1] + b * v[2] + a * v[3];
+}
+__device__ void wg_cb_left(double* M, int N, int side, const double P[3], int start) {
+    const int np = N >> 2;
+    for (int e = tid(); e < np * N; e += NT) {
+        const int p = e % np, col = e / np;
+        int idx[4]; plaq_sites(p, side, start, idx);
+        double* c = M + (size_t)col * N;
+        double v[4] = {c[idx[0]], c[idx[1]], c[idx[2]], c[idx[3]]}, o[4];
+        plaq_apply(P, v, o);
+        c[idx[0]] = o[0]; c[idx[1]] = o[1]; c[idx[2]] = o[2]; c[idx[3]] = o[3];
+    }
+    __syncthreads();
+}
+__device__ void wg_cb_right(double* M, int N, int side, const double P[3], int start) {
+    const int np = N >> 2;
+    for (int e = tid(); e < np * N; e += NT) {
+        const int row = e % N, p = e / N;
+        int idx[4]; plaq_sites(p, side, start, idx);
+        double v[4] = {M[(size_t)idx[0] * N + row], M[(size_t)idx[1] * N + row], M[(size_t)idx[2] * N + row],
+                       M[(size_t)idx[3] * N + row]}, o[4];
+        plaq_apply(P, v, o);
+        M[(size_t)idx[0] * N + row] = o[0]; M[(size_t)idx[1] * N + row] = o[1];
+        M[(size_t)idx[2] * N + row] = o[2]; M[(size_t)idx[3] * N + row] = o[3];
+    }
+    __syncthreads();
+}
+
+// Field-row scalings exp(+alpha s_i) / exp(-alpha s_i) into LDS.
+__device__ void load_field_row(const ChainDesc& d, int e, Lds& s) {
+    for (int i = tid(); i < d.N; i += NT) {
+        const int f = d.field[(size_t)e * d.N + i];
+        s.frow[i] = f;
+        s.scp[i] = f > 0 ? d.ev[0] : d.ev[1];
+        s.scm[i] = f > 0 ? d.ev[1] : d.ev[0];
+    }
+    __syncthreads();
+}
+// after Metropolis flipped entries of the LDS field row
+__device__ void refresh_scalings(const ChainDesc& d, Lds& s) {
+    for (int i = tid(); i < d.N; i += NT) {
+        const int f = s.frow[i];
+        s.scp[i] = f > 0 ? d.ev[0] : d.ev[1];
+        s.scm[i] = f > 0 ? d.ev[1] : d.ev[0];
+    }
+    __syncthreads();
+}
+// exp(+sigma_eff alpha s) for spin index sp (0 up, 1 down); attractive ignores the spin
+__device__ __forceinline__ const double* vpos(const ChainDesc& d, int sp, Lds& s) {
+    return (d.model == 1 || sp == 0) ? s.scp : s.scm;
+}
+__device__ __forceinline__ const double* vneg(const ChainDesc& d, int sp, Lds& s) {
+    return (d.model == 1 || sp == 0) ? s.scm : s.scp;
+}
+
+// B(t) products on an N x N matrix M (in place; W is N*N scratch for the dense path).
+// op: 0 B_left, 1 B_right, 2 invB_left, 3 invB_right, 4 transB_left
+__device__ void wg_bmul(const ChainDesc& d, int op, int sp, double* M, double* W, Lds& s) {
+    const int N = d.N;
+    const double* ep = vpos(d, sp, s);
+    const double* en = vneg(d, sp, s);
+    if (d.cb) {
+        switch (op) {
+            case 0: wg_cb_left(M, N, d.side, d.P, 1); wg_cb_left(M, N, d.side, d.P, 0); wg_scale_rows(M, N, ep); break;
+            case 1: wg_scale_cols(M, N, ep); wg_cb_right(M, N, d.side, d.P, 0); wg_cb_right(M, N, d.side, d.P, 1); break;
+            case 2: wg_scale_rows(M, N, en); wg_cb_left(M, N, d.side, d.Pi, 0); wg_cb_left(M, N, d.side, d.Pi, 1); break;
+            case 3: wg_cb_right(M, N, d.side, d.Pi, 1); wg_cb_right(M, N, d.side, d.Pi, 0); wg_scale_cols(M, N, en); break;
+            case 4: wg_scale_rows(M, N, ep); wg_cb_left(M, N, d.side, d.P, 0); wg_cb_left(M, N, d.side, d.P, 1); break;
+        }
+        return;
+    }
+    switch (op) {
+        case 0: gemmB(d, N, N, N, d.expK, N, false, M, N, false, W, N, ep, nullptr, nullptr, s); break;
+        case 1: gemmB(d, N, N, N, M, N, false, d.expK, N, false, W, N, nullptr, ep, nullptr, s); break;
+        case 2: gemmB(d, N, N, N, d.invExpK, N, false, M, N, false, W, N, nullptr, en, nullptr, s); break;
+        case 3: gemmB(d, N, N, N, M, N, false, d.invExpK, N, false, W, N, nullptr, nullptr, en, s); break;
+        case 4: gemmB(d, N, N, N, d.expK, N, true, M, N, false, W, N, nullptr, ep, nullptr, s); break;
+    }
+    wg_copy(M, W, N * N);
+}
+
+// G <- B(t) G B(t)^-1 (up) or G <- B(t)^-1 G B(t) (down) for spin index sp
+__device__ void wg_wrap(const ChainDesc& d, int sp, bool up, double* G, double* W, Lds& s) {
+    const int N = d.N;
+    if (!d.cb) {
+        const double* ep = vpos(d, sp, s);
+        const double* en = vneg(d, sp, s);
+        if (up) {
+            gemmB(d, N, N, N, d.expK, N, false, G, N, false, W, N, ep, nullptr, nullptr, s);
+            gemmB(d, N, N, N, W, N, false, d.invExpK, N, false, G, N, nullptr, nullptr, en, s);
+        } else {
+            gemmB(d, N, N, N, G, N, false, d.expK, N, false, W, N, nullptr, ep, nullptr, s);
+            gemmB(d, N, N, N, d.invExpK, N, false, W, N, false, G, N, nullptr, en, nullptr, s);
+        }
+        return;
+    }
+    if (up) { wg_bmul(d, 0, sp, G, W, s); wg_bmul(d, 3, sp, G, W, s); }
+    else { wg_bmul(d, 1, sp, G, W, s); wg_bmul(d, 2, sp, G, W, s); }
+}
+
+// ---------------------------------------------------------------------------
+// Gaussian elimination with partial pivoting: A X = B, X overwrites B (N x nrhs).
+// A is destroyed.  Returns the sign of det(A) (for nrhs = 0 too).
+// ---------------------------------------------------------------------------
+__device__ double wg_lu_solve(double* A, double* B, int N, int nrhs, Lds& s) {
+    double detsign = 1.0;
+    for (int k = 0; k < N; ++k) {
+        if (wave() == 0) {
+            double best = -1.0; int bi = k;
+            for (int i = k + lane(); i < N; i += 64) {
+                const double v = fabs(A[(size_t)k * N + i]);
+                if (v > best || (v == best && i < bi)) { best = v; bi = i; }
+            }
+            wave_argmax(best, bi);
+            if (lane() == 0) s.ivar[0] = bi;
+        }
+        __syncthreads();
+        const int p = s.ivar[0];
+        if (p != k) {
+            for (int c = tid(); c < N + nrhs; c += NT) {
+                double* col = c < N ? A + (size_t)c * N : B + (size_t)(c - N) * N;
+                if (c < k) continue;  // left part of A is no longer read
+                const double t = col[k]; col[k] = col[p]; col[p] = t;
+            }
+            detsign = -detsign;
+            __syncthreads();
+        }
+        const double piv = A[(size_t)k * N + k];
+        if (piv < 0) detsign = -detsign;
+        const int nr = N - k - 1;
+        if (nr > 0) {
+            const int ncols = (N - k - 1) + nrhs;
+            for (int e = tid(); e < nr * ncols; e += NT) {
+                const int i = k + 1 + e % nr, c = e / nr;
+                const double l = A[(size_t)k * N + i] / piv;
+                double* col = c < (N - k - 1) ? A + (size_t)(k + 1 + c) * N : B + (size_t)(c - (N - k - 1)) * N;
+                col[i] -= l * col[k];
+            }
+        }
+        __syncthreads();
+    }
+    for (int k = N - 1; k >= 0 && nrhs > 0; --k) {
+        const double piv = A[(size_t)k * N + k];
+        for (int c = tid(); c < nrhs; c += NT) B[(size_t)c * N + k] /= piv;
+        __syncthreads();
+        for (int e = tid(); e < k * nrhs; e += NT) {
+            const int i = e % k, c = e / k;
+            B[(size_t)c * N + i] -= A[(size_t)k * N + i] * B[(size_t)c * N + k];
+        }
+        __syncthreads();
+    }
+    return detsign;
+}
+
+// ---------------------------------------------------------------------------
+// Householder QR with column pivoting of M (N x N, destroyed):
+//   M P = Q R,  U = Q (explicit), D = |diag R|, Tl = D^-1 R P^T  (dense N x N)
+// (LAPACK dgeqp3-style norm downdating with recomputation on cancellation.)
+// ---------------------------------------------------------------------------
+__device__ void wg_qrp_udv(double* M, int N, double* U, double* D, double* Tl, Lds& s) {
+    double* vn1 = s.v0;  // partial column norms
+    double* vn2 = s.v1;  // reference norms for the cancellation test
+    double* tau = s.v2;
+    const double tol3z = 1.4901161193847656e-08;  // sqrt(eps)
+    // initial column norms: one wave per column
+    for (int j = wave(); j < N; j += 4) {
+        double acc = 0.0;
+        for (int i = lane(); i < N; i += 64) { const double v = M[(size_t)j * N + i]; acc += v * v; }
+        acc = wave_sum(acc);
+        if (lane() == 0) { vn1[j] = sqrt(acc); vn2[j] = vn1[j]; s.perm[j] = j; }
+    }
+    __syncthreads();
+    for (int k = 0; k < N; ++k) {
+        if (wave() == 0) {
+            double best = -1.0; int bi = k;
+            for (int j = k + lane(); j < N; j += 64)
+                if (vn1[j] > best || (vn1[j] == best && j < bi)) { best = vn1[j]; bi = j; }
+            wave_argmax(best, bi);
+            if (lane() == 0) s.ivar[0] = bi;
+        }
+        __syncthreads();
+        const int p = s.ivar[0];
+        if (p != k) {
+            for (int i = tid(); i < N; i += NT) {
+                const double t = M[(size_t)k * N + i];
+                M[(size_t)k * N + i] = M[(size_t)p * N + i];
+                M[(size_t)p * N + i] = t;
+            }
+            if (tid() == 0) {
+                double t = vn1[k]; vn1[k] = vn1[p]; vn1[p] = t;
+                t = vn2[k]; vn2[k] = vn2[p]; vn2[p] = t;
+                int q = s.perm[k]; s.perm[k] = s.perm[p]; s.perm[p] = q;
+            }
+            __syncthreads();
+        }
+        // Householder reflector of M(k:N, k): wave 0 computes the norm
+        double* mk = M + (size_t)k * N;
+        if (wave() == 0) {
+            double acc = 0.0;
+            for (int i = k + 1 + lane(); i < N; i += 64) acc += mk[i] * mk[i];
+            acc = wave_sum(acc);
+            if (lane() == 0) {
+                const double x0 = mk[k];
+                const double nrm = sqrt(x0 * x0 + acc);
+                if (nrm == 0.0) { tau[k] = 0.0; s.red[8] = 1.0; s.red[9] = 0.0; }
+                else {
+                    const double alpha = x0 > 0 ? -nrm : nrm;
+                    const double v0 = x0 - alpha;
+                    tau[k] = -v0 / alpha;
+                    s.red[8] = 1.0 / v0;  // scale of the sub-diagonal part of v
+                    s.red[9] = alpha;
+                }
+            }
+        }
+        __syncthreads();
+        const double inv_v0 = s.red[8];
+        const double tk = tau[k];
+        if (tk != 0.0)
+            for (int i = k + 1 + tid(); i < N; i += NT) mk[i] *= inv_v0;
+        __syncthreads();
+        if (tid() == 0 && tk != 0.0) mk[k] = s.red[9];
+        // apply H = I - tau v v^T to the trailing columns; one wave per column
+        for (int j = k + 1 + wave(); j < N; j += 4) {
+            double* mj = M + (size_t)j * N;
+            double acc = 0.0;
+            if (tk != 0.0) {
+                for (int i = k + 1 + lane(); i < N; i += 64) acc += mk[i] * mj[i];
+                acc = wave_sum(acc);
+                acc = tk * (mj[k] + acc);
+                for (int i = k + 1 + lane(); i < N; i += 64) mj[i] -= acc * mk[i];
+            }
+            // row k element and norm downdate (lane 0)
+            double rkj = mj[k];
+            if (lane() == 0) {
+                rkj = mj[k] - acc;
+                mj[k] = rkj;
+            }
+            rkj = __shfl(rkj, 0);
+            bool recompute = false;
+            if (lane() == 0 && vn1[j] != 0.0) {
+                double temp = fabs(rkj) / vn1[j];
+                temp = 1.0 - temp * temp;
+                temp = temp < 0.0 ? 0.0 : temp;
+                const double r = vn1[j] / vn2[j];
+                const double temp2 = temp * r * r;
+                if (temp2 <= tol3z) recompute = true;
+                else vn1[j] = vn1[j] * sqrt(temp);
+            }
+            recompute = __shfl((int)recompute, 0) != 0;
+            if (recompute) {
+                double a2 = 0.0;
+                for (int i = k + 1 + lane(); i < N; i += 64) a2 += mj[i] * mj[i];
+                a2 = wave_sum(a2);
+                if (lane() == 0) { vn1[j] = sqrt(a2); vn2[j] = vn1[j]; }
+            }
+        }
+        __syncthreads();
+    }
+    // D and Tl = D^-1 R P^T
+    for (int k = tid(); k < N; k += NT) D[k] = fabs(M[(size_t)k * N + k]);
+    __syncthreads();
+    for (int e = tid(); e < N * N; e += NT) {
+        const int r = e % N, c = e / N;  // Tl(r, perm[c]) = R(r, c) / D(r)
+        Tl[(size_t)s.perm[c] * N + r] = (c >= r) ? M[(size_t)c * N + r] / D[r] : 0.0;
+    }
+    // explicit Q by backward accumulation of the reflectors
+    for (int e = tid(); e < N * N; e += NT) U[e] = ((e % N) == (e / N)) ? 1.0 : 0.0;
+    __syncthreads();
+    for (int k = N - 1; k >= 0; --k) {
+        const double tk = tau[k];
+        if (tk == 0.0) continue;
+        const double* vk = M + (size_t)k * N;
+        for (int j = k + wave(); j < N; j += 4) {
+            double* uj = U + (size_t)j * N;
+            double acc = 0.0;
+            for (int i = k + 1 + lane(); i < N; i += 64) acc += vk[i] * uj[i];
+            acc = wave_sum(acc);
+            acc = tk * (uj[k] + acc);
+            for (int i = k + 1 + lane(); i < N; i += 64) uj[i] -= acc * vk[i];
+            if (lane() == 0) uj[k] -= acc;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// UDV stacks.  The left and right stacks of a spin share one array of
+// n_stack slots: the right stack fills slots [0, n_right) upward, the left
+// stack fills [n_right, n_stack) downward, so a pop of one and the push of
+// the other always meet at the same slot.  Each slot keeps U, D and the
+// cumulative T (= V^T) of the stack product up to that level.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double* slotU(const ChainDesc& d, int sp, int k) {
+    return d.stkU + ((size_t)sp * d.n_stack + k) * d.N * d.N;
+}
+__device__ __forceinline__ double* slotT(const ChainDesc& d, int sp, int k) {
+    return d.stkT + ((size_t)sp * d.n_stack + k) * d.N * d.N;
+}
+__device__ __forceinline__ double* slotD(const ChainDesc& d, int sp, int k) {
+    return d.stkD + ((size_t)sp * d.n_stack + k) * d.N;
+}
+
+// push A onto the stack whose current top is slot `below` (-1 if empty); result at `slot`
+__device__ void wg_push(const ChainDesc& d, int sp, int slot, int below, const double* A, Lds& s) {
+    const int N = d.N;
+    double* W0 = d.W;
+    double* W1 = d.W + (size_t)N * N;
+    if (below < 0) wg_copy(W0, A, N * N);
+    else gemm64(N, N, N, A, N, false, slotU(d, sp, below), N, false, W0, N, nullptr, nullptr, slotD(d, sp, below), 0.0, s);
+    wg_qrp_udv(W0, N, slotU(d, sp, slot), slotD(d, sp, slot), W1, s);
+    __syncthreads();
+    if (below < 0) wg_copy(slotT(d, sp, slot), W1, N * N);
+    else gemm64(N, N, N, W1, N, false, slotT(d, sp, below), N, false, slotT(d, sp, slot), N, nullptr, nullptr, nullptr, 0.0, s);
+}
+
+// G = (1 + B_left B_right)^-1 from the stack tops (lslot/rslot = -1 when that stack is empty)
+__device__ void wg_fresh_green(const ChainDesc& d, int sp, int lslot, int rslot, double* Gout, Lds& s) {
+    const int N = d.N;
+    double* W0 = d.W;
+    double* W1 = d.W + (size_t)N * N;
+    if (lslot >= 0 && rslot >= 0) {
+        const double* UL = slotU(d, sp, lslot); const double* TL = slotT(d, sp, lslot); const double* DL = slotD(d, sp, lslot);
+        const double* UR = slotU(d, sp, rslot); const double* TR = slotT(d, sp, rslot); const double* DR = slotD(d, sp, rslot);
+        // v0 = 1/dlmax, v1 = dlmin, v2 = 1/drmax, v3 = drmin  (include/utils/numerical_stable.hpp:57-71)
+        for (int i = tid(); i < N; i += NT) {
+            const double dl = DL[i], dr = DR[i];
+            s.v0[i] = dl >= 1.0 ? 1.0 / dl : 1.0; s.v1[i] = dl >= 1.0 ? 1.0 : dl;
+            s.v2[i] = dr >= 1.0 ? 1.0 / dr : 1.0; s.v3[i] = dr >= 1.0 ? 1.0 : dr;
+        }
+        __syncthreads();
+        // W0 = dlmax^-1 (UL^T UR) drmax^-1 + dlmin (TL TR^T) drmin
+        gemm64(N, N, N, UL, N, true, UR, N, false, W0, N, s.v0, nullptr, s.v2, 0.0, s);
+        gemm64(N, N, N, TL, N, false, TR, N, true, W0, N, s.v1, nullptr, s.v3, 1.0, s);
+        // W1 = dlmax^-1 UL^T
+        for (int e = tid(); e < N * N; e += NT) { const int r = e % N, c = e / N; W1[e] = s.v0[r] * UL[(size_t)r * N + c]; }
+        __syncthreads();
+        wg_lu_solve(W0, W1, N, N, s);
+        // G = UR drmax^-1 W1
+        gemm64(N, N, N, UR, N, false, W1, N, false, Gout, N, nullptr, s.v2, nullptr, 0.0, s);
+        return;
+    }
+    // one stack only: X = (Db U^T + Ds T)^-1 Db U^T   (include/utils/numerical_stable.hpp:132-152)
+    const int k = lslot >= 0 ? lslot : rslot;
+    const double* U = slotU(d, sp, k); const double* T = slotT(d, sp, k); const double* D = slotD(d, sp, k);
+    for (int i = tid(); i < N; i += NT) {
+        const double x = D[i];
+        s.v0[i] = x > 1.0 ? 1.0 / x : 1.0;
+        s.v1[i] = x > 1.0 ? 1.0 : x;
+    }
+    __syncthreads();
+    for (int e = tid(); e < N * N; e += NT) {
+        const int r = e % N, c = e / N;
+        const double ut = s.v0[r] * U[(size_t)r * N + c];
+        W1[e] = ut;
+        W0[e] = ut + s.v1[r] * T[e];
+    }
+    __syncthreads();
+    wg_lu_solve(W0, W1, N, N, s);
+    if (lslot >= 0) wg_copy(Gout, W1, N * N);
+    else {  // the right stack holds the transposed product: G = X^T
+        for (int e = tid(); e < N * N; e += NT) { const int r = e % N, c = e / N; Gout[e] = W1[(size_t)r * N + c]; }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Metropolis over the sites of one slice with delayed rank-k updates.
+// G_cur = G0 + X Y^T; the diagonal of G_cur is kept in LDS; thread 0 draws
+// the decisions (it can walk over rejected sites without synchronization).
+// ---------------------------------------------------------------------------
+__device__ void wg_flush(const ChainDesc& d, int cnt, Lds& s) {
+    const int N = d.N;
+    for (int sp = 0; sp < d.nspin; ++sp)
+        gemm64(N, N, cnt, d.X + (size_t)sp * N * d.kdelay, N, false, d.Y + (size_t)sp * N * d.kdelay, N, true,
+                d.G + (size_t)sp * N * N, N, nullptr, nullptr, nullptr, 1.0, s);
+}
+
+__device__ void wg_metropolis(const ChainDesc& d, int e, uint64_t& rng, double& sign, bool record, int& naccept, Lds& s) {
+    const int N = d.N;
+    const int ns = d.nspin;
+    double* dgs[2] = {s.du, ns == 2 ? s.dd : s.du};
+    for (int i = tid(); i < N; i += NT) {
+        s.du[i] = d.G[(size_t)i * N + i];
+        if (ns == 2) s.dd[i] = d.G[(size_t)N * N + (size_t)i * N + i];
+    }
+    if (tid() == 0) s.ivar[1] = 0;
+    __syncthreads();
+    int cnt = 0;
+    int i0 = 0;
+    while (true) {
+        if (tid() == 0) {
+            int j = i0;
+            for (; j < N; ++j) {
+                const int f = s.frow[j];
+                const int si = f > 0 ? 0 : 1;
+                const double du = dgs[0][j], dd = dgs[1][j];
+                const double r = d.pref[si] * (1.0 + (1.0 - du) * d.dm1[0][si]) * (1.0 + (1.0 - dd) * d.dm1[1][si]);
+                const double p = fmin(1.0, fabs(r));
+                const bool ok = canonical(rng) < p;
+                if (record) d.recAcc[(size_t)e * N + j] = ok ? 1 : 0;
+                if (ok) {
+                    sign = (r >= 0) ? sign : -sign;
+                    s.gam[0] = d.dm1[0][si] / (1.0 + (1.0 - du) * d.dm1[0][si]);
+                    s.gam[1] = d.dm1[1][si] / (1.0 + (1.0 - dd) * d.dm1[1][si]);
+                    s.frow[j] = -f;
+                    d.field[(size_t)e * N + j] = (int8_t)(-f);
+                    break;
+                }
+            }
+            s.ivar[0] = j;
+        }
+        __syncthreads();
+        const int j = s.ivar[0];
+        if (j >= N) break;
+        ++naccept;
+        for (int sp = 0; sp < ns; ++sp) {
+            const double g = s.gam[sp];
+            const double* G0 = d.G + (size_t)sp * N * N;
+            double* X = d.X + (size_t)sp * N * d.kdelay;
+            double* Y = d.Y + (size_t)sp * N * d.kdelay;
+            double* dg = dgs[sp];
+            for (int a = tid(); a < N; a += NT) {
+                double x = G0[(size_t)j * N + a];
+                double y = G0[(size_t)a * N + j] - (a == j ? 1.0 : 0.0);
+                for (int m = 0; m < cnt; ++m) {
+                    x += X[(size_t)m * N + a] * Y[(size_t)m * N + j];
+                    y += X[(size_t)m * N + j] * Y[(size_t)m * N + a];
+                }
+                X[(size_t)cnt * N + a] = g * x;
+                Y[(size_t)cnt * N + a] = y;
+                dg[a] += g * x * y;
+            }
+        }
+        ++cnt;
+        __syncthreads();
+        if (cnt == d.kdelay) { wg_flush(d, cnt, s); cnt = 0; }
+        i0 = j + 1;
+        if (i0 >= N) break;
+    }
+    if (cnt > 0) wg_flush(d, cnt, s);
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Equal-time measurements of one slice (src/measure/measure_methods.cpp:22-255)
+// ---------------------------------------------------------------------------
+__device__ void wg_measure(const ChainDesc& d, double cs, Lds& s) {
+    const int N = d.N, side = d.side;
+    const double* Gu = d.G;
+    const double* Gd = d.nspin == 2 ? d.G + (size_t)N * N : d.G;
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // tr, docc, kin, loc, md, sdw, cdw, sw
+    for (int a = tid(); a < N; a += NT) {
+        const double du = Gu[(size_t)a * N + a], dd = Gd[(size_t)a * N + a];
+        s.du[a] = du; s.dd[a] = dd;
+        v[0] += du + dd;
+        v[1] += (1 - du) * (1 - dd);
+        v[3] += du + dd - 2 * du * dd;
+        const int x = a % side, y = a / side;
+        const int jx = (x + 1) % side + side * y, jy = x + side * ((y + 1) % side);
+        v[2] += Gu[(size_t)jx * N + a] + Gd[(size_t)jx * N + a] + Gu[(size_t)jy * N + a] + Gd[(size_t)jy * N + a];
+    }
+    __syncthreads();
+    for (int p = tid(); p < N * N; p += NT) {
+        const int i = p % N, j = p / N;
+        const int xi = i % side, yi = i / side, xj = j % side, yj = j / side;
+        const int dsp = (xj - xi + side) % side + side * ((yj - yi + side) % side);
+        const double ff = d.cosq[dsp];
+        const double gu_ij = Gu[(size_t)j * N + i], gd_ij = Gd[(size_t)j * N + i];
+        const double gu_ji = Gu[(size_t)i * N + j], gd_ji = Gd[(size_t)i * N + j];
+        const double dij = (i == j) ? 1.0 : 0.0;
+        const double guc_ij = dij - gu_ji, gdc_ij = dij - gd_ji;
+        const double guc_ii = 1.0 - s.du[i], gdc_ii = 1.0 - s.dd[i];
+        const double guc_jj = 1.0 - s.du[j], gdc_jj = 1.0 - s.dd[j];
+        v[4] += (gu_ji + gd_ji) * ff;
+        const double common = guc_ii * guc_jj + guc_ij * gu_ij + gdc_ii * gdc_jj + gdc_ij * gd_ij;
+        const double cross = gdc_ii * guc_jj + guc_ii * gdc_jj;
+        v[5] += ff * (common - cross);
+        v[6] += ff * (common + cross);
+        v[7] += guc_ij * gdc_ij;
+    }
+    wg_sum8(v, 8, s);
+    if (tid() == 0) {
+        const double Nd = (double)N;
+        double* acc = d.acc;
+        acc[0] += cs;
+        acc[1] += cs * (2 - v[0] / Nd);
+        acc[2] += cs * v[1] / Nd;
+        acc[3] += cs * (2 * d.hop_t) * v[2] / Nd;
+        acc[4] += cs * (1 - 0.5 * v[4] / Nd);
+        acc[5] += cs * v[3] / Nd;
+        acc[6] += cs * v[5] / (Nd * Nd);
+        acc[7] += cs * v[6] / (Nd * Nd);
+        acc[8] += cs * v[7] / Nd;
+    }
+    __syncthreads();
+}
+
+__device__ void record_slot(const ChainDesc& d, int slot, double sign, bool with_sign) {
+    const int N = d.N;
+    const size_t n2 = (size_t)N * N;
+    for (size_t e = tid(); e < n2 * d.nspin; e += NT) d.recG[(size_t)slot * d.nspin * n2 + e] = d.G[e];
+    if (with_sign && tid() == 0) d.recSign[slot] = sign;
+    __syncthreads();
+}
+
+// fresh G for both spins into Gw, wrap error against G, then G <- Gw
+__device__ void stabilize_green(const ChainDesc& d, int lslot, int rslot, bool track_error, ChainState& st, Lds& s) {
+    const int N = d.N;
+    const size_t n2 = (size_t)N * N;
+    double err = 0.0;
+    for (int sp = 0; sp < d.nspin; ++sp) {
+        double* Gf = d.Gw + sp * n2;
+        wg_fresh_green(d, sp, lslot, rslot, Gf, s);
+        const double* G = d.G + sp * n2;
+        for (size_t e = tid(); e < n2; e += NT) err = fmax(err, fabs(Gf[e] - G[e]));
+    }
+    err = wg_max(err, s);
+    if (track_error && tid() == 0 && err > st.wrap_error) st.wrap_error = err;
+    wg_copy(d.G, d.Gw, (int)(n2 * d.nspin));
+}
+
+// ---------------------------------------------------------------------------
+// sweeps (src/dqmc_walker.cpp:246-325 and :334-412)
+// ---------------------------------------------------------------------------
+__device__ void sweep_up(const ChainDesc& d, ChainState& st, uint64_t& rng, double& sign, int flags, int& nacc, Lds& s) {
+    const int N = d.N, L = d.L;
+    const size_t n2 = (size_t)N * N;
+    const bool rec = flags & DQMC_FLAG_RECORD, meas = flags & DQMC_FLAG_MEASURE;
+    for (int sp = 0; sp < d.nspin; ++sp) wg_identity(d.Tmp + sp * n2, N);
+    for (int t = 1; t <= L; ++t) {
+        const int e = t - 1;
+        load_field_row(d, e, s);
+        for (int sp = 0; sp < d.nspin; ++sp) wg_wrap(d, sp, true, d.G + sp * n2, d.Gw + sp * n2, s);
+        if (!(flags & DQMC_FLAG_FROZEN)) wg_metropolis(d, e, rng, sign, rec, nacc, s);
+        refresh_scalings(d, s);
+        for (int sp = 0; sp < d.nspin; ++sp) wg_bmul(d, 0, sp, d.Tmp + sp * n2, d.Gw + sp * n2, s);
+        if (t % d.pace == 0 || t == L) {
+            st.n_right -= 1;
+            const int slot = st.n_right;
+            for (int sp = 0; sp < d.nspin; ++sp)
+                wg_push(d, sp, slot, slot + 1 < d.n_stack ? slot + 1 : -1, d.Tmp + sp * n2, s);
+            stabilize_green(d, slot, slot > 0 ? slot - 1 : -1, true, st, s);
+            for (int sp = 0; sp < d.nspin; ++sp) wg_identity(d.Tmp + sp * n2, N);
+        }
+        if (rec) record_slot(d, e, sign, true);
+        if (meas) wg_measure(d, sign, s);
+    }
+    st.pos = L;
+}
+
+__device__ void sweep_down(const ChainDesc& d, ChainState& st, uint64_t& rng, double& sign, int flags, int& nacc, Lds& s) {
+    const int N = d.N, L = d.L;
+    const size_t n2 = (size_t)N * N;
+    const bool rec = flags & DQMC_FLAG_RECORD, meas = flags & DQMC_FLAG_MEASURE;
+    for (int sp = 0; sp < d.nspin; ++sp) wg_identity(d.Tmp + sp * n2, N);
+    double sign_top = sign;
+    for (int t = L; t >= 1; --t) {
+        const int e = t - 1;
+        if (t % d.pace == 0 && t != L) {
+            const int slot = st.n_right;
+            for (int sp = 0; sp < d.nspin; ++sp)
+                wg_push(d, sp, slot, slot > 0 ? slot - 1 : -1, d.Tmp + sp * n2, s);
+            st.n_right += 1;
+            stabilize_green(d, slot + 1 < d.n_stack ? slot + 1 : -1, slot, true, st, s);
+            for (int sp = 0; sp < d.nspin; ++sp) wg_identity(d.Tmp + sp * n2, N);
+        }
+        load_field_row(d, e, s);
+        if (!(flags & DQMC_FLAG_FROZEN)) wg_metropolis(d, e, rng, sign, rec, nacc, s);
+        if (rec) record_slot(d, e, sign, true);
+        if (t == L) sign_top = sign;
+        else if (meas) wg_measure(d, sign, s);
+        refresh_scalings(d, s);
+        for (int sp = 0; sp < d.nspin; ++sp) wg_bmul(d, 4, sp, d.Tmp + sp * n2, d.Gw + sp * n2, s);
+        for (int sp = 0; sp < d.nspin; ++sp) wg_wrap(d, sp, false, d.G + sp * n2, d.Gw + sp * n2, s);
+    }
+    {
+        const int slot = st.n_right;
+        for (int sp = 0; sp < d.nspin; ++sp) wg_push(d, sp, slot, slot > 0 ? slot - 1 : -1, d.Tmp + sp * n2, s);
+        st.n_right += 1;
+        stabilize_green(d, -1, slot, false, st, s);
+    }
+    if (rec) record_slot(d, L - 1, sign, false);
+    if (meas) wg_measure(d, sign_top, s);
+    st.pos = 0;
+}
+
+__global__ void __launch_bounds__(NT) k_init(const ChainDesc* __restrict__ descs) {
+    __shared__ Lds s;
+    const ChainDesc& d = descs[blockIdx.x];
+    ChainState st = *d.st;
+    const int N = d.N, L = d.L;
+    const size_t n2 = (size_t)N * N;
+    for (int sp = 0; sp < d.nspin; ++sp) wg_identity(d.Tmp + sp * n2, N);
+    st.n_right = 0;
+    for (int t = L; t >= 1; --t) {   // src/dqmc_walker.cpp:118-133
+        load_field_row(d, t - 1, s);
+        for (int sp = 0; sp < d.nspin; ++sp) wg_bmul(d, 4, sp, d.Tmp + sp * n2, d.Gw + sp * n2, s);
+        if ((t - 1) % d.pace == 0) {
+            const int slot = st.n_right;
+            for (int sp = 0; sp < d.nspin; ++sp) wg_push(d, sp, slot, slot > 0 ? slot - 1 : -1, d.Tmp + sp * n2, s);
+            st.n_right += 1;
+            for (int sp = 0; sp < d.nspin; ++sp) wg_identity(d.Tmp + sp * n2, N);
+        }
+    }
+    for (int sp = 0; sp < d.nspin; ++sp) wg_fresh_green(d, sp, -1, st.n_right - 1, d.G + sp * n2, s);
+    // sign of det(G_up) det(G_dn)  (src/dqmc_walker.cpp:162)
+    double sg = 1.0;
+    for (int sp = 0; sp < d.nspin; ++sp) {
+        wg_copy(d.Gw, d.G + sp * n2, (int)n2);
+        const double ds = wg_lu_solve(d.Gw, nullptr, N, 0, s);
+        sg *= ds;
+        if (d.nspin == 1) sg *= ds;
+    }
+    if (tid() == 0) {
+        st.sign = sg >= 0 ? 1.0 : -1.0;
+        st.wrap_error = 0.0;
+        st.pos = 0;
+        st.status = 0;
+        st.accepted = 0;
+        *d.st = st;
+    }
+}
+
+__global__ void __launch_bounds__(NT) k_sweep(const ChainDesc* __restrict__ descs, int n_sweeps, int flags) {
+    __shared__ Lds s;
+    const ChainDesc& d = descs[blockIdx.x];
+    ChainState st = *d.st;
+    uint64_t rng = st.rng;
+    double sign = st.sign;
+    int nacc = 0;
+    if (d.launch_sweeps > 0) n_sweeps = d.launch_sweeps;
+    if (d.launch_flags >= 0) flags = d.launch_flags;
+    for (int k = 0; k < n_sweeps; ++k) {
+        if (st.pos == 0) sweep_up(d, st, rng, sign, flags, nacc, s);
+        else sweep_down(d, st, rng, sign, flags, nacc, s);
+    }
+    if (tid() == 0) {
+        st.rng = rng;
+        st.sign = sign;
+        st.accepted = nacc;
+        *d.st = st;
+    }
+}
+
+// one workgroup runs one GEMM variant (0 vector FP64, 1 MFMA FP64, 2 MFMA FP32)
+__global__ void __launch_bounds__(NT) k_gemm(int variant, int M, int Nc, int K, const double* A, int lda, int tA,
+                                             const double* B, int ldb, int tB, double* C, int ldc, const double* rs,
+                                             const double* ms, const double* cs, double beta) {
+    __shared__ Lds s;
+    if (variant == 0) wg_gemm(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, s);
+    else if (variant == 1) wg_gemm_mfma<false>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, s);
+    else wg_gemm_mfma<true>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, s);
+}
+
+}  // namespace dqmc
+
+extern "C" {
+hipError_t dqmc_launch_gemm(int variant, int M, int Nc, int K, const double* A, int lda, int tA, const double* B,
+                            int ldb, int tB, double* C, int ldc, const double* rs, const double* ms, const double* cs,
+                            double beta, hipStream_t stream) {
+    hipLaunchKernelGGL(dqmc::k_gemm, dim3(1), dim3(NT), 0, stream, variant, M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc,
+                       rs, ms, cs, beta);
+    return hipGetLastError();
+}
+hipError_t dqmc_launch_init(const ChainDesc* d_descs, int n_chains, hipStream_t stream) {
+    hipLaunchKernelGGL(dqmc::k_init, dim3(n_chains), dim3(NT), 0, stream, d_descs);
+    return hipGetLastError();
+}
+hipError_t dqmc_launch_sweep(const ChainDesc* d_descs, int n_chains, int n_sweeps, int flags, hipStream_t stream) {
+    hipLaunchKernelGGL(dqmc::k_sweep, dim3(n_chains), dim3(NT), 0, stream, d_descs, n_sweeps, flags);
+    return hipGetLastError();
+}
+}

@@ -1,0 +1,597 @@
+// Host runtime of the DQMC engine: contexts, chain batches, the device arena
+// of each batch, host-side chain setup (lattice tables, exp(-dtau K),
+// checkerboard and HS constants, random fields) and the C-ABI of dqmc_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "runtime_internal.h"
+
+extern "C" hipError_t dqmc_launch_init(const ChainDesc* d_descs, int n_chains, hipStream_t stream);
+extern "C" hipError_t dqmc_launch_sweep(const ChainDesc* d_descs, int n_chains, int n_sweeps, int flags,
+                                        hipStream_t stream);
+
+extern "C" hipError_t dqmc_launch_gemm(int variant, int M, int Nc, int K, const double* A, int lda, int tA,
+                                       const double* B, int ldb, int tB, double* C, int ldc, const double* rs,
+                                       const double* ms, const double* cs, double beta, hipStream_t stream);
+
+namespace {
+
+// minstd_rand0 and the bernoulli draw of libstdc++ (see the device twin in dqmc_kernels.hip)
+struct Minstd {
+    uint64_t x = 1;
+    void seed(long long s) {
+        x = ((uint64_t)(int64_t)s) % 2147483647ULL;
+        if (x == 0) x = 1;
+    }
+    uint64_t next() { x = (16807ULL * x) % 2147483647ULL; return x; }
+};
+double canonical(Minstd& g) {
+#pragma clang fp contract(off)
+    const double R = 2147483646.0;
+    double sum = 0.0, tmp = 1.0;
+    for (int k = 0; k < 2; ++k) {
+        sum = sum + (double)(g.next() - 1) * tmp;
+        tmp = tmp * R;
+    }
+    double r = sum / tmp;
+    if (r >= 1.0) r = std::nextafter(1.0, 0.0);
+    return r;
+}
+
+
+bool is_finite(double x) { return std::isfinite(x); }
+
+// exp(-s K) of K = -t H + mu I on the periodic square lattice, with H the
+// hopping matrix of src/lattice/square.cpp:164-177 (nearest neighbours in +x
+// and +y, added symmetrically).  K is diagonal in plane waves, eps_k =
+// -2t(cos kx + cos ky) + mu, so exp(-s K)(i,j) depends on the displacement only:
+//   f(d) = 1/N sum_k exp(-s eps_k) cos(k . r_d).
+void expk_closed_form(int side, double t, double mu, double s, std::vector<double>& out) {
+    const int N = side * side;
+    std::vector<double> f(N, 0.0);
+    std::vector<double> ck(side);
+    for (int a = 0; a < side; ++a) ck[a] = std::cos(2.0 * M_PI * a / side);
+    for (int dy = 0; dy < side; ++dy)
+        for (int dx = 0; dx < side; ++dx) {
+            long double acc = 0.0L;
+            for (int ky = 0; ky < side; ++ky)
+                for (int kx = 0; kx < side; ++kx) {
+                    const double eps = -2.0 * t * (ck[kx] + ck[ky]) + mu;
+                    const double ph = 2.0 * M_PI * ((double)(kx * dx) / side + (double)(ky * dy) / side);
+                    acc += (long double)std::exp(-s * eps) * (long double)std::cos(ph);
+                }
+            f[dx + side * dy] = (double)(acc / N);
+        }
+    out.assign((size_t)N * N, 0.0);
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < N; ++i) {
+            const int dx = ((j % side) - (i % side) + side) % side;
+            const int dy = ((j / side) - (i / side) + side) % side;
+            out[(size_t)j * N + i] = f[dx + side * dy];
+        }
+}
+
+
+
+}  // namespace
+
+Layout chain_layout(int N, int L, int n_stack, int nspin, int kdelay, bool dense, bool record) {
+    Layout l{};
+    size_t o = 0;
+    const size_t n2 = (size_t)N * N * sizeof(double);
+    auto take = [&](size_t bytes) { size_t r = o; o += align256(bytes); return r; };
+    l.G = take(nspin * n2);
+    l.Gw = take(nspin * n2);
+    l.Tmp = take(nspin * n2);
+    l.stkU = take((size_t)nspin * n_stack * n2);
+    l.stkT = take((size_t)nspin * n_stack * n2);
+    l.stkD = take((size_t)nspin * n_stack * N * sizeof(double));
+    l.W = take(3 * n2);
+    l.X = take((size_t)nspin * N * kdelay * sizeof(double));
+    l.Y = take((size_t)nspin * N * kdelay * sizeof(double));
+    l.expK = dense ? take(n2) : o;
+    l.invExpK = dense ? take(n2) : o;
+    l.cosq = take(N * sizeof(double));
+    l.field = take((size_t)L * N);
+    l.st = take(sizeof(ChainState));
+    l.acc = take(16 * sizeof(double));
+    l.recG = record ? take((size_t)L * nspin * n2) : o;
+    l.recSign = record ? take(L * sizeof(double)) : o;
+    l.recAcc = record ? take((size_t)L * N) : o;
+    l.total = o;
+    return l;
+}
+
+int validate_params(const dqmc_chain_params& p, std::string& why) {
+    const int N = p.side * p.side;
+    if (p.side < 2 || N > DQMC_NMAX) { why = "side must satisfy 2 <= side and side*side <= 512"; return DQMC_ERR_ARG; }
+    if (p.time_size < 1) { why = "time_size must be >= 1"; return DQMC_ERR_ARG; }
+    if (p.stabilization_pace < 1) { why = "stabilization_pace must be >= 1"; return DQMC_ERR_ARG; }
+    if (p.model != DQMC_MODEL_REPULSIVE && p.model != DQMC_MODEL_ATTRACTIVE) { why = "unknown model"; return DQMC_ERR_ARG; }
+    if (p.checkerboard && (p.side % 2 != 0)) { why = "checkerboard break-up needs an even side length"; return DQMC_ERR_ARG; }
+    if (p.delay < 1 || p.delay > DQMC_KMAX) { why = "delay must be in [1, 64]"; return DQMC_ERR_ARG; }
+    if (p.precision != DQMC_PREC_FP64 && p.precision != DQMC_PREC_MIXED) { why = "unsupported precision mode"; return DQMC_ERR_ARG; }
+    if (!is_finite(p.beta) || !is_finite(p.hopping_t) || !is_finite(p.onsite_u) || !is_finite(p.chemical_potential)) {
+        why = "non-finite model parameter"; return DQMC_ERR_NUMERIC;
+    }
+    if (p.beta <= 0.0) { why = "beta must be > 0"; return DQMC_ERR_ARG; }
+    if (p.hopping_t < 0.0) { why = "hopping_t must be >= 0"; return DQMC_ERR_ARG; }
+    if (p.model == DQMC_MODEL_REPULSIVE && p.onsite_u < 0.0) { why = "repulsive model needs onsite_u >= 0"; return DQMC_ERR_ARG; }
+    const int nk = (p.side / 2 + 1) * (p.side / 2 + 2) / 2;
+    if (p.momentum_index < -1 || p.momentum_index >= nk) { why = "momentum_index out of range"; return DQMC_ERR_ARG; }
+    return DQMC_OK;
+}
+
+
+
+
+
+
+extern "C" {
+
+int dqmc_context_create(int device, dqmc_ctx_t* out) {
+    if (!out) return DQMC_ERR_ARG;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return DQMC_ERR_HIP;
+    if (hipSetDevice(device) != hipSuccess) return DQMC_ERR_HIP;
+    dqmc_context* c = new dqmc_context();
+    c->device = device;
+    *out = c;
+    return DQMC_OK;
+}
+
+int dqmc_context_destroy(dqmc_ctx_t ctx) {
+    delete ctx;
+    return DQMC_OK;
+}
+
+const char* dqmc_context_error(dqmc_ctx_t ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int dqmc_batch_footprint(int n_chains, const dqmc_chain_params* params, int record, size_t* bytes) {
+    if (n_chains < 1 || !params || !bytes) return DQMC_ERR_ARG;
+    size_t total = align256(sizeof(ChainDesc) * (size_t)n_chains);
+    for (int c = 0; c < n_chains; ++c) {
+        const dqmc_chain_params& p = params[c];
+        std::string why;
+        const int rc = validate_params(p, why);
+        if (rc != DQMC_OK) return rc;
+        const int N = p.side * p.side;
+        const int n_stack = (p.time_size + p.stabilization_pace - 1) / p.stabilization_pace;
+        const int nspin = p.model == DQMC_MODEL_ATTRACTIVE ? 1 : 2;
+        total += chain_layout(N, p.time_size, n_stack, nspin, p.delay, !p.checkerboard, record != 0).total;
+    }
+    *bytes = total;
+    return DQMC_OK;
+}
+
+}  // extern "C"
+
+// Host-side setup of a batch: validation, per-chain constants and tables,
+// arena layout.  No device work.
+int batch_prepare(dqmc_context* ctx, int n_chains, const dqmc_chain_params* params, bool record, dqmc_batch* b) {
+    b->ctx = ctx;
+    b->n = n_chains;
+    b->record = record;
+    b->chains.resize(n_chains);
+    b->layouts.resize(n_chains);
+    b->base.resize(n_chains);
+    b->descs.resize(n_chains);
+    size_t off = align256(sizeof(ChainDesc) * n_chains);  // the descriptor array heads the arena
+    for (int c = 0; c < n_chains; ++c) {
+        HostChain& h = b->chains[c];
+        h.p = params[c];
+        std::string why;
+        const int rc = validate_params(h.p, why);
+        if (rc != DQMC_OK) {
+            ctx->err = "chain " + std::to_string(c) + ": " + why;
+            return rc;
+        }
+        const dqmc_chain_params& p = h.p;
+        h.N = p.side * p.side;
+        h.nspin = p.model == DQMC_MODEL_ATTRACTIVE ? 1 : 2;
+        h.n_stack = (p.time_size + p.stabilization_pace - 1) / p.stabilization_pace;
+        h.kdelay = p.delay;
+        h.dtau = p.beta / p.time_size;
+        const double U = std::fabs(p.onsite_u);
+        h.alpha = std::acosh(std::exp(0.5 * h.dtau * U));
+        h.fields.assign((size_t)p.time_size * h.N, 1);
+        if (!p.checkerboard) {
+            expk_closed_form(p.side, p.hopping_t, p.chemical_potential, h.dtau, h.expK);
+            expk_closed_form(p.side, p.hopping_t, p.chemical_potential, -h.dtau, h.invExpK);
+        }
+        // Re exp(-i Q r_d) of the measured momentum (src/lattice/square.cpp:42-66, :148-161)
+        const int side = p.side;
+        std::vector<double> kx, ky;
+        for (int i = (int)std::ceil(side / 2.0); i <= side; ++i)
+            for (int j = (int)std::ceil(side / 2.0); j <= i; ++j) {
+                kx.push_back((double)i / side * 2 * M_PI - M_PI);
+                ky.push_back((double)j / side * 2 * M_PI - M_PI);
+            }
+        const int q = p.momentum_index < 0 ? (int)kx.size() - 1 : p.momentum_index;
+        h.cosq.resize(h.N);
+        for (int d = 0; d < h.N; ++d) h.cosq[d] = std::cos((-(d % side)) * kx[q] - (d / side) * ky[q]);
+        b->layouts[c] = chain_layout(h.N, p.time_size, h.n_stack, h.nspin, h.kdelay, !p.checkerboard, b->record);
+        b->base[c] = off;
+        off += b->layouts[c].total;
+    }
+    b->arena_bytes = off;
+    return DQMC_OK;
+}
+
+// Bind a prepared batch to its arena (b->arena, b->stream set): zero it, build
+// the descriptors and upload constants and descriptors, all stream-ordered.
+int batch_bind(dqmc_batch* b) {
+    dqmc_context* ctx = b->ctx;
+    b->d_descs = (ChainDesc*)b->arena;
+    HIPCHK(ctx, hipMemsetAsync(b->arena, 0, b->arena_bytes, b->stream));
+    for (int c = 0; c < b->n; ++c) {
+        HostChain& h = b->chains[c];
+        const Layout& l = b->layouts[c];
+        char* base = b->arena + b->base[c];
+        ChainDesc& d = b->descs[c];
+        std::memset(&d, 0, sizeof(d));
+        const dqmc_chain_params& p = h.p;
+        d.launch_sweeps = 0;
+        d.launch_flags = -1;
+        d.N = h.N; d.L = p.time_size; d.side = p.side; d.pace = p.stabilization_pace; d.n_stack = h.n_stack;
+        d.model = p.model; d.cb = p.checkerboard; d.kdelay = h.kdelay; d.nspin = h.nspin; d.precision = p.precision;
+        d.hop_t = p.hopping_t;
+        // exp(+spin alpha s) of src/model/repulsive_hubbard.cpp:173 for spin*s = +1 / -1
+        d.ev[0] = std::exp(+1.0 * h.alpha);
+        d.ev[1] = std::exp(-1.0 * h.alpha);
+        // exp(-2 sigma alpha s) - 1 (ratio and gamma, src/model/repulsive_hubbard.cpp:120-147)
+        const double em = std::exp(-2 * h.alpha * 1.0) - 1, ep = std::exp(-2 * h.alpha * -1.0) - 1;
+        if (p.model == DQMC_MODEL_REPULSIVE) {
+            d.dm1[0][0] = em; d.dm1[0][1] = ep;   // up: exp(-2 alpha s) - 1
+            d.dm1[1][0] = std::exp(+2 * h.alpha * 1.0) - 1;
+            d.dm1[1][1] = std::exp(+2 * h.alpha * -1.0) - 1;  // down: exp(+2 alpha s) - 1
+            d.pref[0] = d.pref[1] = 1.0;
+        } else {  // src/model/attractive_hubbard.cpp:120-124
+            d.dm1[0][0] = d.dm1[1][0] = em;
+            d.dm1[0][1] = d.dm1[1][1] = ep;
+            d.pref[0] = std::exp(2 * h.alpha * 1.0);
+            d.pref[1] = std::exp(2 * h.alpha * -1.0);
+        }
+        // plaquettes (src/checkerboard/square.cpp:34-49)
+        {
+            double ch = std::cosh(h.dtau * p.hopping_t), sh = std::sinh(h.dtau * p.hopping_t);
+            double f = std::exp(0.5 * h.dtau * p.chemical_potential);
+            d.P[0] = f * (ch * ch); d.P[1] = f * (ch * sh); d.P[2] = f * (sh * sh);
+            ch = std::cosh(-h.dtau * p.hopping_t); sh = std::sinh(-h.dtau * p.hopping_t);
+            f = std::exp(-0.5 * h.dtau * p.chemical_potential);
+            d.Pi[0] = f * (ch * ch); d.Pi[1] = f * (ch * sh); d.Pi[2] = f * (sh * sh);
+        }
+        d.G = (double*)(base + l.G); d.Gw = (double*)(base + l.Gw); d.Tmp = (double*)(base + l.Tmp);
+        d.stkU = (double*)(base + l.stkU); d.stkT = (double*)(base + l.stkT); d.stkD = (double*)(base + l.stkD);
+        d.W = (double*)(base + l.W); d.X = (double*)(base + l.X); d.Y = (double*)(base + l.Y);
+        d.expK = p.checkerboard ? nullptr : (const double*)(base + l.expK);
+        d.invExpK = p.checkerboard ? nullptr : (const double*)(base + l.invExpK);
+        d.cosq = (const double*)(base + l.cosq);
+        d.field = (int8_t*)(base + l.field);
+        d.st = (ChainState*)(base + l.st);
+        d.acc = (double*)(base + l.acc);
+        d.recG = b->record ? (double*)(base + l.recG) : nullptr;
+        d.recSign = b->record ? (double*)(base + l.recSign) : nullptr;
+        d.recAcc = b->record ? (uint8_t*)(base + l.recAcc) : nullptr;
+        if (!p.checkerboard) {
+            HIPCHK(ctx, hipMemcpyAsync((void*)d.expK, h.expK.data(), h.expK.size() * 8, hipMemcpyHostToDevice, b->stream));
+            HIPCHK(ctx, hipMemcpyAsync((void*)d.invExpK, h.invExpK.data(), h.invExpK.size() * 8, hipMemcpyHostToDevice, b->stream));
+        }
+        HIPCHK(ctx, hipMemcpyAsync((void*)d.cosq, h.cosq.data(), h.cosq.size() * 8, hipMemcpyHostToDevice, b->stream));
+    }
+    HIPCHK(ctx, hipMemcpyAsync(b->d_descs, b->descs.data(), sizeof(ChainDesc) * b->n, hipMemcpyHostToDevice, b->stream));
+    return DQMC_OK;
+}
+
+extern "C" {
+
+int dqmc_batch_create(dqmc_ctx_t ctx, int n_chains, const dqmc_chain_params* params, int record, dqmc_batch_t* out) {
+    if (!ctx || !params || !out || n_chains < 1) return DQMC_ERR_ARG;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    dqmc_batch* b = new dqmc_batch();
+    int rc = batch_prepare(ctx, n_chains, params, record != 0, b);
+    if (rc != DQMC_OK) { delete b; return rc; }
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { ctx->err = hipGetErrorString(e); delete b; return DQMC_ERR_HIP; }
+    b->owns_stream = true;
+    e = hipMalloc((void**)&b->arena, b->arena_bytes);
+    if (e != hipSuccess) {
+        ctx->err = std::string("arena of ") + std::to_string(b->arena_bytes) + " bytes: " + hipGetErrorString(e);
+        hipStreamDestroy(b->stream);
+        delete b;
+        return DQMC_ERR_MEMORY;
+    }
+    b->owns_arena = true;
+    rc = batch_bind(b);
+    if (rc == DQMC_OK) {
+        e = hipStreamSynchronize(b->stream);
+        if (e != hipSuccess) { ctx->err = hipGetErrorString(e); rc = DQMC_ERR_HIP; }
+    }
+    if (rc != DQMC_OK) { dqmc_batch_destroy(b); return rc; }
+    *out = b;
+    return DQMC_OK;
+}
+
+int dqmc_batch_destroy(dqmc_batch_t b) {
+    if (!b) return DQMC_ERR_ARG;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->ev0) hipEventDestroy(b->ev0);
+    if (b->ev1) hipEventDestroy(b->ev1);
+    if (b->arena && b->owns_arena) hipFree(b->arena);
+    if (b->stream && b->owns_stream) hipStreamDestroy(b->stream);
+    delete b;
+    return DQMC_OK;
+}
+
+int dqmc_batch_size(dqmc_batch_t b) { return b ? b->n : DQMC_ERR_ARG; }
+
+static int check_chain(dqmc_batch_t b, int c) {
+    if (!b) return DQMC_ERR_ARG;
+    if (c < 0 || c >= b->n) { b->ctx->err = "chain index out of range"; return DQMC_ERR_ARG; }
+    return DQMC_OK;
+}
+
+static int upload_state(dqmc_batch_t b, int c, uint64_t rng) {
+    ChainState st{};
+    st.rng = rng;
+    st.sign = 1.0;
+    HIPCHK(b->ctx, hipMemcpyAsync(b->descs[c].st, &st, sizeof(st), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
+    return DQMC_OK;
+}
+
+int dqmc_fields_random(dqmc_batch_t b) {
+    if (!b) return DQMC_ERR_ARG;
+    HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
+    for (int c = 0; c < b->n; ++c) {
+        HostChain& h = b->chains[c];
+        Minstd g;
+        g.seed(h.p.seed);
+        // t-outer, i-inner, one bernoulli(0.5) each (src/model/repulsive_hubbard.cpp:92-97)
+        for (size_t e = 0; e < h.fields.size(); ++e) h.fields[e] = canonical(g) < 0.5 ? 1 : -1;
+        HIPCHK(b->ctx, hipMemcpyAsync(b->descs[c].field, h.fields.data(), h.fields.size(), hipMemcpyHostToDevice, b->stream));
+        const int rc = upload_state(b, c, g.x);
+        if (rc) return rc;
+    }
+    HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
+    b->initialized = false;
+    return DQMC_OK;
+}
+
+int dqmc_set_fields(dqmc_batch_t b, int c, const int8_t* f) {
+    int rc = check_chain(b, c);
+    if (rc) return rc;
+    if (!f) return DQMC_ERR_ARG;
+    HostChain& h = b->chains[c];
+    for (size_t e = 0; e < h.fields.size(); ++e)
+        if (f[e] != 1 && f[e] != -1) { b->ctx->err = "fields must be +1 or -1"; return DQMC_ERR_ARG; }
+    std::memcpy(h.fields.data(), f, h.fields.size());
+    HIPCHK(b->ctx, hipMemcpyAsync(b->descs[c].field, h.fields.data(), h.fields.size(), hipMemcpyHostToDevice, b->stream));
+    // keep the engine state if the chain already has one; otherwise seed it
+    ChainState st{};
+    HIPCHK(b->ctx, hipMemcpyAsync(&st, b->descs[c].st, sizeof(st), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
+    if (st.rng == 0) {
+        Minstd g; g.seed(h.p.seed);
+        rc = upload_state(b, c, g.x);
+        if (rc) return rc;
+    }
+    HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
+    b->initialized = false;
+    return DQMC_OK;
+}
+
+int dqmc_set_rng_state(dqmc_batch_t b, int c, unsigned long long state) {
+    int rc = check_chain(b, c);
+    if (rc) return rc;
+    if (state == 0 || state >= 2147483647ULL) { b->ctx->err = "engine state must be in [1, 2^31-2]"; return DQMC_ERR_ARG; }
+    HIPCHK(b->ctx, hipMemcpyAsync(&b->descs[c].st->rng, &state, sizeof(uint64_t), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
+    return DQMC_OK;
+}
+
+int dqmc_get_fields(dqmc_batch_t b, int c, int8_t* f) {
+    int rc = check_chain(b, c);
+    if (rc) return rc;
+    HostChain& h = b->chains[c];
+    HIPCHK(b->ctx, hipMemcpyAsync(f, b->descs[c].field, h.fields.size(), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
+    return DQMC_OK;
+}
+
+int dqmc_set_expk(dqmc_batch_t b, int c, const double* expK, const double* invExpK) {
+    int rc = check_chain(b, c);
+    if (rc) return rc;
+    HostChain& h = b->chains[c];
+    if (h.p.checkerboard) { b->ctx->err = "chain uses the checkerboard break-up"; return DQMC_ERR_STATE; }
+    const size_t n2 = (size_t)h.N * h.N;
+    for (size_t e = 0; e < n2; ++e)
+        if (!is_finite(expK[e]) || !is_finite(invExpK[e])) { b->ctx->err = "non-finite exp(K)"; return DQMC_ERR_NUMERIC; }
+    h.expK.assign(expK, expK + n2);
+    h.invExpK.assign(invExpK, invExpK + n2);
+    HIPCHK(b->ctx, hipMemcpyAsync((void*)b->descs[c].expK, expK, n2 * 8, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(b->ctx, hipMemcpyAsync((void*)b->descs[c].invExpK, invExpK, n2 * 8, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
+    b->initialized = false;
+    return DQMC_OK;
+}
+
+int dqmc_get_expk(dqmc_batch_t b, int c, double* expK, double* invExpK) {
+    int rc = check_chain(b, c);
+    if (rc) return rc;
+    HostChain& h = b->chains[c];
+    if (h.p.checkerboard) { b->ctx->err = "chain uses the checkerboard break-up"; return DQMC_ERR_STATE; }
+    std::memcpy(expK, h.expK.data(), h.expK.size() * 8);
+    std::memcpy(invExpK, h.invExpK.data(), h.invExpK.size() * 8);
+    return DQMC_OK;
+}
+
+int dqmc_init(dqmc_batch_t b) {
+    if (!b) return DQMC_ERR_ARG;
+    HIPCHK(b->ctx, hipSetDevice(b->ctx->device));
+    for (int c = 0; c < b->n; ++c) {
+        HIPCHK(b->ctx, hipMemsetAsync(b->descs[c].acc, 0, 16 * sizeof(double), b->stream));
+        b->chains[c].counts = 0;
+        b->chains[c].sweeps = 0;
+    }
+    HIPCHK(b->ctx, dqmc_launch_init(b->d_descs, b->n, b->stream));
+    HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
+    b->initialized = true;
+    return DQMC_OK;
+}
+
+int dqmc_sweep(dqmc_batch_t b, int n_sweeps, int flags) {
+    if (!b || n_sweeps < 0) return DQMC_ERR_ARG;
+    if (!b->initialized) { b->ctx->err = "dqmc_sweep before dqmc_init"; return DQMC_ERR_STATE; }
+    if ((flags & DQMC_SWEEP_RECORD) && !b->record) { b->ctx->err = "batch created without record buffers"; return DQMC_ERR_STATE; }
+    if (n_sweeps == 0) return DQMC_OK;
+    HIPCHK(b->ctx, hipSetDevice(b->ctx->device));
+    if (!b->ev0) {
+        HIPCHK(b->ctx, hipEventCreate(&b->ev0));
+        HIPCHK(b->ctx, hipEventCreate(&b->ev1));
+    }
+    HIPCHK(b->ctx, hipEventRecord(b->ev0, b->stream));
+    HIPCHK(b->ctx, dqmc_launch_sweep(b->d_descs, b->n, n_sweeps, flags, b->stream));
+    HIPCHK(b->ctx, hipEventRecord(b->ev1, b->stream));
+    b->timed = true;
+    for (auto& h : b->chains) {
+        h.sweeps += n_sweeps;
+        if (flags & DQMC_SWEEP_MEASURE) h.counts += (double)n_sweeps * h.p.time_size;
+    }
+    return DQMC_OK;
+}
+
+int dqmc_last_sweep_ms(dqmc_batch_t b, float* ms) {
+    if (!b || !ms) return DQMC_ERR_ARG;
+    if (!b->timed) { b->ctx->err = "no sweep launched yet"; return DQMC_ERR_STATE; }
+    HIPCHK(b->ctx, hipEventSynchronize(b->ev1));
+    HIPCHK(b->ctx, hipEventElapsedTime(ms, b->ev0, b->ev1));
+    return DQMC_OK;
+}
+
+int dqmc_synchronize(dqmc_batch_t b) {
+    if (!b) return DQMC_ERR_ARG;
+    HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
+    return DQMC_OK;
+}
+
+int dqmc_get_green(dqmc_batch_t b, int c, int spin, double* out) {
+    int rc = check_chain(b, c);
+    if (rc) return rc;
+    if (spin != 1 && spin != -1) return DQMC_ERR_ARG;
+    const HostChain& h = b->chains[c];
+    const int sp = (spin == 1 || h.nspin == 1) ? 0 : 1;
+    const size_t n2 = (size_t)h.N * h.N;
+    HIPCHK(b->ctx, hipMemcpyAsync(out, b->descs[c].G + sp * n2, n2 * 8, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
+    return DQMC_OK;
+}
+
+int dqmc_get_slice_green(dqmc_batch_t b, int c, int slot, int spin, double* out) {
+    int rc = check_chain(b, c);
+    if (rc) return rc;
+    const HostChain& h = b->chains[c];
+    if (!b->record) { b->ctx->err = "batch created without record buffers"; return DQMC_ERR_STATE; }
+    if (slot < 0 || slot >= h.p.time_size || (spin != 1 && spin != -1)) return DQMC_ERR_ARG;
+    const int sp = (spin == 1 || h.nspin == 1) ? 0 : 1;
+    const size_t n2 = (size_t)h.N * h.N;
+    HIPCHK(b->ctx, hipMemcpyAsync(out, b->descs[c].recG + ((size_t)slot * h.nspin + sp) * n2, n2 * 8,
+                                  hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
+    return DQMC_OK;
+}
+
+int dqmc_get_slice_signs(dqmc_batch_t b, int c, double* out) {
+    int rc = check_chain(b, c);
+    if (rc) return rc;
+    if (!b->record) { b->ctx->err = "batch created without record buffers"; return DQMC_ERR_STATE; }
+    HIPCHK(b->ctx, hipMemcpyAsync(out, b->descs[c].recSign, b->chains[c].p.time_size * 8, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
+    return DQMC_OK;
+}
+
+int dqmc_get_accepts(dqmc_batch_t b, int c, uint8_t* out) {
+    int rc = check_chain(b, c);
+    if (rc) return rc;
+    if (!b->record) { b->ctx->err = "batch created without record buffers"; return DQMC_ERR_STATE; }
+    const HostChain& h = b->chains[c];
+    HIPCHK(b->ctx, hipMemcpyAsync(out, b->descs[c].recAcc, (size_t)h.p.time_size * h.N, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
+    return DQMC_OK;
+}
+
+int dqmc_get_info(dqmc_batch_t b, int c, dqmc_chain_info* out) {
+    int rc = check_chain(b, c);
+    if (rc) return rc;
+    ChainState st{};
+    HIPCHK(b->ctx, hipMemcpyAsync(&st, b->descs[c].st, sizeof(st), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
+    out->sign = st.sign;
+    out->wrap_error = st.wrap_error;
+    out->rng = st.rng;
+    out->pos = st.pos;
+    out->status = st.status;
+    out->accepted = st.accepted;
+    out->sweeps = b->chains[c].sweeps;
+    return DQMC_OK;
+}
+
+int dqmc_get_measure(dqmc_batch_t b, int c, double* sums, double* counts) {
+    int rc = check_chain(b, c);
+    if (rc) return rc;
+    double acc[16];
+    HIPCHK(b->ctx, hipMemcpyAsync(acc, b->descs[c].acc, sizeof(acc), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
+    for (int o = 0; o < DQMC_NUM_OBSERVABLES; ++o) {
+        sums[o] = acc[o];
+        counts[o] = b->chains[c].counts;
+    }
+    return DQMC_OK;
+}
+
+int dqmc_clear_measure(dqmc_batch_t b) {
+    if (!b) return DQMC_ERR_ARG;
+    for (int c = 0; c < b->n; ++c) {
+        HIPCHK(b->ctx, hipMemsetAsync(b->descs[c].acc, 0, 16 * sizeof(double), b->stream));
+        b->chains[c].counts = 0;
+    }
+    HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
+    return DQMC_OK;
+}
+
+int dqmc_k_gemm(dqmc_ctx_t ctx, int variant, int M, int Nc, int K, int tA, int tB, const double* A, const double* B,
+                double* C, double beta, const double* rs, const double* ms, const double* cs) {
+    if (!ctx || !A || !B || !C || M < 1 || Nc < 1 || K < 1 || variant < 0 || variant > 2) return DQMC_ERR_ARG;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const size_t na = (size_t)M * K, nb = (size_t)K * Nc, nc = (size_t)M * Nc;
+    double *dA = nullptr, *dB = nullptr, *dC = nullptr, *dv = nullptr;
+    const size_t nv = (size_t)M + K + Nc;
+    hipError_t e = hipMalloc((void**)&dA, na * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&dB, nb * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&dC, nc * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&dv, nv * 8);
+    if (e == hipSuccess) e = hipMemcpy(dA, A, na * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dB, B, nb * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dC, C, nc * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess && rs) e = hipMemcpy(dv, rs, (size_t)M * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess && ms) e = hipMemcpy(dv + M, ms, (size_t)K * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess && cs) e = hipMemcpy(dv + M + K, cs, (size_t)Nc * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = dqmc_launch_gemm(variant, M, Nc, K, dA, tA ? K : M, tA, dB, tB ? Nc : K, tB, dC, M, rs ? dv : nullptr,
+                             ms ? dv + M : nullptr, cs ? dv + M + K : nullptr, beta, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(C, dC, nc * 8, hipMemcpyDeviceToHost);
+    if (dA) hipFree(dA);
+    if (dB) hipFree(dB);
+    if (dC) hipFree(dC);
+    if (dv) hipFree(dv);
+    if (e != hipSuccess) { ctx->err = std::string("dqmc_k_gemm: ") + hipGetErrorString(e); return DQMC_ERR_HIP; }
+    return DQMC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,500 @@
+// Continuous-batching job server of the DQMC engine.
+//
+// Jobs (one Markov chain each: lattice, beta, U, mu, sweeps, seed) arrive at
+// any time through dqmc_server_submit.  A scheduler thread admits them by
+// device-memory footprint into a pool allocated once (so the serving loop
+// never calls hipMalloc/hipFree), keeps every admitted chain resident (G,
+// UDV stacks, fields, random state) and composes sweep launches over the
+// resident chains.  Chains are grouped into lanes by lattice size; each lane
+// has its own stream, so launches of small and large chains overlap on the
+// device.  Within a launch every chain receives as many sweeps as fit a
+// target launch time (from an online cost model), cut at its warm-up/bin
+// boundaries, so chains of very different cost progress together and a
+// newly admitted chain joins at the next launch of its lane.
+//
+// The per-chain computation is the one of the batch API (same kernels, same
+// state between launches), so a job's result does not depend on what else
+// is resident.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "runtime_internal.h"
+
+namespace {
+
+struct Minstd {
+    uint64_t x = 1;
+    void seed(long long s) {
+        x = ((uint64_t)(int64_t)s) % 2147483647ULL;
+        if (x == 0) x = 1;
+    }
+    uint64_t next() { x = (16807ULL * x) % 2147483647ULL; return x; }
+};
+double canonical(Minstd& g) {
+#pragma clang fp contract(off)
+    const double R = 2147483646.0;
+    double sum = 0.0, tmp = 1.0;
+    for (int k = 0; k < 2; ++k) {
+        sum = sum + (double)(g.next() - 1) * tmp;
+        tmp = tmp * R;
+    }
+    double r = sum / tmp;
+    if (r >= 1.0) r = std::nextafter(1.0, 0.0);
+    return r;
+}
+
+// first-fit allocator over one device pool (offsets, 256-byte granules)
+struct Pool {
+    size_t size = 0;
+    std::map<size_t, size_t> free_;  // offset -> length
+    void init(size_t n) { size = n; free_.clear(); free_[0] = n; }
+    bool alloc(size_t n, size_t& off) {
+        n = align256(n);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= n) {
+                off = it->first;
+                const size_t rest = it->second - n;
+                free_.erase(it);
+                if (rest) free_[off + n] = rest;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t n) {
+        n = align256(n);
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) { it->second += nx->second; free_.erase(nx); }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) { pv->second += it->second; free_.erase(it); }
+        }
+    }
+};
+
+double now_s(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+struct Job {
+    long long id = 0;
+    dqmc_job spec{};
+    size_t bytes = 0, off = 0;
+    int lane = 0;
+    std::unique_ptr<dqmc_batch> b;
+    ChainState st0{};
+    bool needs_init = true, cancel = false;
+    long long done = 0, total = 0;
+    int in_bin = 0, launched = 0, launched_flags = 0;
+    double cost = 1.0;  // FLOP per chain-sweep (model)
+    double t_submit = 0, t_start = 0;
+    std::vector<std::vector<double>> bins;
+};
+
+struct Lane {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev = nullptr;
+    ChainDesc* d_descs = nullptr;
+    ChainDesc* d_init = nullptr;
+    std::vector<ChainDesc> h_descs, h_init;
+    std::vector<Job*> in_flight;
+    std::deque<Job*> queue;
+    bool busy = false;
+    double launch_flop = 0;
+    std::chrono::steady_clock::time_point t_launch;
+};
+
+struct dqmc_server {
+    dqmc_context* ctx = nullptr;
+    size_t budget = 0, used = 0, peak = 0;
+    int max_active = 0;
+    char* pool_base = nullptr;
+    Pool pool;
+    std::vector<Lane> lanes;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::vector<Job*> active;
+    std::vector<dqmc_job_result> results;
+    std::thread worker;
+    std::atomic<bool> stop{false};
+    long long next_id = 1, submitted = 0, completed = 0, chain_sweeps = 0, launches = 0;
+    std::chrono::steady_clock::time_point t0;
+    double rate = 2.0e10;        // FLOP/s one chain achieves (updated online)
+    double target_s = 0.02;      // target launch duration
+    int status = DQMC_OK;
+    std::string err;
+};
+
+namespace {
+
+int lane_of(const dqmc_server* s, int N) {
+    const int cls = N <= 64 ? 0 : (N <= 144 ? 1 : (N <= 256 ? 2 : 3));
+    return std::min(cls, (int)s->lanes.size() - 1);
+}
+
+double chain_sweep_flop(const dqmc_chain_params& p) {
+    const double N = (double)p.side * p.side, L = p.time_size;
+    const double ns = std::ceil(L / p.stabilization_pace);
+    const double spins = p.model == DQMC_MODEL_ATTRACTIVE ? 1.0 : 2.0;
+    const double prop = p.checkerboard ? 96.0 * L * N * N : 12.0 * L * N * N * N;
+    const double upd = 4.0 * (0.3 * N * L) * N * N;
+    return spins / 2.0 * (prop + upd + 29.3 * ns * N * N * N);
+}
+
+void finish_job(dqmc_server* s, Job* j, int status, hipStream_t stream) {
+    dqmc_job_result r{};
+    r.id = j->id;
+    r.status = status;
+    r.t_submit = j->t_submit;
+    r.t_start = j->t_start;
+    r.t_finish = now_s(s->t0);
+    r.sweeps = j->done;
+    r.n_bins = (int)j->bins.size();
+    r.wrap_error = 0.0;
+    if (j->b) {
+        ChainState st{};
+        if (hipMemcpyAsync(&st, j->b->descs[0].st, sizeof(st), hipMemcpyDeviceToHost, stream) == hipSuccess &&
+            hipStreamSynchronize(stream) == hipSuccess)
+            r.wrap_error = st.wrap_error;
+    }
+    const size_t nb = j->bins.size();
+    for (int o = 0; o < DQMC_NUM_OBSERVABLES; ++o) {
+        double m = 0.0, m2 = 0.0;
+        for (auto& v : j->bins) { m += v[o]; m2 += v[o] * v[o]; }
+        if (nb > 0) { m /= nb; m2 /= nb; }
+        r.mean[o] = m;
+        r.error[o] = nb > 1 ? std::sqrt(std::max(0.0, m2 - m * m)) / std::sqrt((double)nb - 1) : 0.0;
+    }
+    if (j->b) {
+        s->pool.release(j->off, j->bytes);
+        s->used -= j->bytes;
+        j->b.reset();
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->results.push_back(r);
+    s->completed += 1;
+    s->active.erase(std::remove(s->active.begin(), s->active.end(), j), s->active.end());
+    delete j;
+    s->cv.notify_all();
+}
+
+// place a queued job into the pool and stage its state on the lane stream
+bool admit(dqmc_server* s, Lane& ln, Job* j) {
+    size_t off = 0;
+    if ((int)s->active.size() >= s->max_active) return false;
+    if (s->used + j->bytes > s->budget || !s->pool.alloc(j->bytes, off)) return false;
+    j->off = off;
+    j->b.reset(new dqmc_batch());
+    dqmc_batch* b = j->b.get();
+    if (batch_prepare(s->ctx, 1, &j->spec.chain, false, b) != DQMC_OK) {
+        s->pool.release(off, j->bytes);
+        return false;
+    }
+    b->arena = s->pool_base + off;
+    b->stream = ln.stream;
+    if (batch_bind(b) != DQMC_OK) { s->pool.release(off, j->bytes); j->b.reset(); return false; }
+    HostChain& h = b->chains[0];
+    Minstd g;
+    g.seed(h.p.seed);
+    for (size_t e = 0; e < h.fields.size(); ++e) h.fields[e] = canonical(g) < 0.5 ? 1 : -1;
+    j->st0 = ChainState{};
+    j->st0.rng = g.x;
+    j->st0.sign = 1.0;
+    hipMemcpyAsync(b->descs[0].field, h.fields.data(), h.fields.size(), hipMemcpyHostToDevice, ln.stream);
+    hipMemcpyAsync(b->descs[0].st, &j->st0, sizeof(ChainState), hipMemcpyHostToDevice, ln.stream);
+    s->used += j->bytes;
+    s->peak = std::max(s->peak, s->used);
+    j->needs_init = true;
+    j->t_start = now_s(s->t0);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->active.push_back(j);
+    }
+    return true;
+}
+
+void complete_lane(dqmc_server* s, Lane& ln) {
+    const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - ln.t_launch).count();
+    if (dt > 0 && ln.launch_flop > 0) {
+        // the slowest chain bounds a launch; track the per-chain rate it implies
+        double worst = 0.0;
+        for (Job* j : ln.in_flight) worst = std::max(worst, j->launched * j->cost);
+        const double r = worst / dt;
+        s->rate = 0.8 * s->rate + 0.2 * r;
+    }
+    for (Job* j : ln.in_flight) {
+        j->done += j->launched;
+        s->chain_sweeps += j->launched;
+        if (j->launched_flags & DQMC_FLAG_MEASURE) {
+            j->in_bin += j->launched;
+            if (j->in_bin >= j->spec.bin_size) {
+                double acc[16];
+                hipMemcpyAsync(acc, j->b->descs[0].acc, sizeof(acc), hipMemcpyDeviceToHost, ln.stream);
+                hipStreamSynchronize(ln.stream);
+                hipMemsetAsync(j->b->descs[0].acc, 0, sizeof(acc), ln.stream);
+                const double cnt = (double)j->in_bin * j->spec.chain.time_size;
+                std::vector<double> v(DQMC_NUM_OBSERVABLES);
+                const double sg = acc[0] / cnt;  // MeasureHandler::normalize_stats
+                for (int o = 1; o < DQMC_NUM_OBSERVABLES; ++o) v[o] = acc[o] / (cnt * sg);
+                v[0] = std::fabs(sg);
+                j->bins.push_back(v);
+                j->in_bin = 0;
+            }
+        }
+        j->launched = 0;
+    }
+    std::vector<Job*> fin;
+    for (Job* j : ln.in_flight)
+        if (j->done >= j->total || j->cancel) fin.push_back(j);
+    ln.in_flight.clear();
+    for (Job* j : fin) finish_job(s, j, j->cancel && j->done < j->total ? 1 : DQMC_OK, ln.stream);
+}
+
+bool launch_lane(dqmc_server* s, Lane& ln, int lane_index) {
+    std::vector<Job*> jobs;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (Job* j : s->active)
+            if (j->lane == lane_index) jobs.push_back(j);
+    }
+    std::vector<Job*> cancelled;
+    for (Job* j : jobs)
+        if (j->cancel) cancelled.push_back(j);
+    for (Job* j : cancelled) {
+        jobs.erase(std::remove(jobs.begin(), jobs.end(), j), jobs.end());
+        finish_job(s, j, 1, ln.stream);
+    }
+    if (jobs.empty()) return false;
+    // initialize newly admitted chains first (stream-ordered before the sweeps)
+    ln.h_init.clear();
+    for (Job* j : jobs)
+        if (j->needs_init) { ln.h_init.push_back(j->b->descs[0]); j->needs_init = false; }
+    if (!ln.h_init.empty()) {
+        hipMemcpyAsync(ln.d_init, ln.h_init.data(), sizeof(ChainDesc) * ln.h_init.size(), hipMemcpyHostToDevice, ln.stream);
+        if (dqmc_launch_init(ln.d_init, (int)ln.h_init.size(), ln.stream) != hipSuccess) s->status = DQMC_ERR_HIP;
+    }
+    ln.h_descs.clear();
+    ln.launch_flop = 0;
+    for (Job* j : jobs) {
+        const dqmc_job& sp = j->spec;
+        int phase_left, flags;
+        if (j->done < sp.warmup_sweeps) { phase_left = (int)(sp.warmup_sweeps - j->done); flags = 0; }
+        else { phase_left = std::min(sp.bin_size - j->in_bin, (int)(j->total - j->done)); flags = DQMC_FLAG_MEASURE; }
+        const double est = j->cost / s->rate;  // seconds per sweep
+        int n = (int)std::floor(s->target_s / est);
+        n = std::max(1, std::min(n, phase_left));
+        j->launched = n;
+        j->launched_flags = flags;
+        ChainDesc d = j->b->descs[0];
+        d.launch_sweeps = n;
+        d.launch_flags = flags;
+        ln.h_descs.push_back(d);
+        ln.launch_flop += n * j->cost;
+        ln.in_flight.push_back(j);
+    }
+    hipMemcpyAsync(ln.d_descs, ln.h_descs.data(), sizeof(ChainDesc) * ln.h_descs.size(), hipMemcpyHostToDevice, ln.stream);
+    if (dqmc_launch_sweep(ln.d_descs, (int)ln.h_descs.size(), 1, 0, ln.stream) != hipSuccess) s->status = DQMC_ERR_HIP;
+    hipEventRecord(ln.ev, ln.stream);
+    ln.t_launch = std::chrono::steady_clock::now();
+    ln.busy = true;
+    s->launches += 1;
+    return true;
+}
+
+void scheduler(dqmc_server* s) {
+    hipSetDevice(s->ctx->device);
+    while (true) {
+        bool progressed = false;
+        for (size_t li = 0; li < s->lanes.size(); ++li) {
+            Lane& ln = s->lanes[li];
+            if (ln.busy) {
+                if (hipEventQuery(ln.ev) != hipSuccess) continue;
+                ln.busy = false;
+                complete_lane(s, ln);
+                progressed = true;
+            }
+            // admit queued jobs of this lane while they fit (FIFO within the lane)
+            while (true) {
+                Job* j = nullptr;
+                {
+                    std::lock_guard<std::mutex> lk(s->mu);
+                    if (ln.queue.empty()) break;
+                    j = ln.queue.front();
+                }
+                if (j->cancel) {
+                    { std::lock_guard<std::mutex> lk(s->mu); ln.queue.pop_front(); }
+                    finish_job(s, j, 1, ln.stream);
+                    continue;
+                }
+                if (!admit(s, ln, j)) break;
+                std::lock_guard<std::mutex> lk(s->mu);
+                ln.queue.pop_front();
+            }
+            if (launch_lane(s, ln, (int)li)) progressed = true;
+        }
+        if (!progressed) {
+            std::unique_lock<std::mutex> lk(s->mu);
+            bool idle = s->active.empty();
+            for (auto& ln : s->lanes) idle = idle && ln.queue.empty() && !ln.busy;
+            if (s->stop && idle) break;
+            s->cv.wait_for(lk, std::chrono::microseconds(idle ? 2000 : 100));
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int dqmc_server_create(dqmc_ctx_t ctx, size_t hbm_budget, int max_active, int n_lanes, dqmc_server_t* out) {
+    if (!ctx || !out || hbm_budget < 1024 || max_active < 1 || n_lanes < 1 || n_lanes > 4) return DQMC_ERR_ARG;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    dqmc_server* s = new dqmc_server();
+    s->ctx = ctx;
+    s->budget = align256(hbm_budget);
+    s->max_active = max_active;
+    hipError_t e = hipMalloc((void**)&s->pool_base, s->budget);
+    if (e != hipSuccess) { ctx->err = std::string("server pool: ") + hipGetErrorString(e); delete s; return DQMC_ERR_MEMORY; }
+    s->pool.init(s->budget);
+    s->lanes.resize(n_lanes);
+    for (auto& ln : s->lanes) {
+        e = hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ln.ev, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipMalloc((void**)&ln.d_descs, sizeof(ChainDesc) * max_active);
+        if (e == hipSuccess) e = hipMalloc((void**)&ln.d_init, sizeof(ChainDesc) * max_active);
+        if (e != hipSuccess) { ctx->err = hipGetErrorString(e); dqmc_server_destroy(s); return DQMC_ERR_HIP; }
+    }
+    s->t0 = std::chrono::steady_clock::now();
+    s->worker = std::thread(scheduler, s);
+    *out = s;
+    return DQMC_OK;
+}
+
+int dqmc_server_submit(dqmc_server_t s, const dqmc_job* job, long long* id) {
+    if (!s || !job) return DQMC_ERR_ARG;
+    std::string why;
+    int rc = validate_params(job->chain, why);
+    if (rc == DQMC_OK && (job->warmup_sweeps < 0 || job->measure_sweeps < 0 || job->bin_size < 1 ||
+                          (job->measure_sweeps % job->bin_size) != 0)) {
+        why = "sweep counts: warmup >= 0, measure >= 0 and a multiple of bin_size >= 1";
+        rc = DQMC_ERR_ARG;
+    }
+    size_t bytes = 0;
+    if (rc == DQMC_OK) rc = dqmc_batch_footprint(1, &job->chain, 0, &bytes);
+    if (rc == DQMC_OK && bytes > s->budget) { why = "job footprint exceeds the server's device budget"; rc = DQMC_ERR_MEMORY; }
+    if (rc != DQMC_OK) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->err = why;
+        return rc;
+    }
+    Job* j = new Job();
+    j->spec = *job;
+    j->bytes = bytes;
+    j->total = (long long)job->warmup_sweeps + job->measure_sweeps;
+    j->cost = chain_sweep_flop(job->chain);
+    j->lane = lane_of(s, job->chain.side * job->chain.side);
+    std::lock_guard<std::mutex> lk(s->mu);
+    j->id = s->next_id++;
+    j->t_submit = now_s(s->t0);
+    s->lanes[j->lane].queue.push_back(j);
+    s->submitted += 1;
+    if (id) *id = j->id;
+    s->cv.notify_all();
+    return DQMC_OK;
+}
+
+int dqmc_server_cancel(dqmc_server_t s, long long id) {
+    if (!s) return DQMC_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (Job* j : s->active)
+        if (j->id == id) { j->cancel = true; return DQMC_OK; }
+    for (auto& ln : s->lanes)
+        for (Job* j : ln.queue)
+            if (j->id == id) { j->cancel = true; return DQMC_OK; }
+    return DQMC_ERR_ARG;
+}
+
+int dqmc_server_poll(dqmc_server_t s, dqmc_job_result* out, int max_results, int* n_out) {
+    if (!s || !n_out || (max_results > 0 && !out)) return DQMC_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    const int n = std::min((int)s->results.size(), std::max(0, max_results));
+    for (int k = 0; k < n; ++k) out[k] = s->results[k];
+    s->results.erase(s->results.begin(), s->results.begin() + n);
+    *n_out = n;
+    return s->status;
+}
+
+int dqmc_server_wait_idle(dqmc_server_t s, double timeout_s) {
+    if (!s) return DQMC_ERR_ARG;
+    const auto until = std::chrono::steady_clock::now() + std::chrono::duration<double>(timeout_s);
+    std::unique_lock<std::mutex> lk(s->mu);
+    while (true) {
+        bool idle = s->active.empty();
+        for (auto& ln : s->lanes) idle = idle && ln.queue.empty();
+        if (idle) return DQMC_OK;
+        if (std::chrono::steady_clock::now() >= until) return DQMC_ERR_STATE;
+        s->cv.wait_for(lk, std::chrono::milliseconds(5));
+    }
+}
+
+int dqmc_server_get_stats(dqmc_server_t s, dqmc_server_stats* out) {
+    if (!s || !out) return DQMC_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    out->submitted = s->submitted;
+    out->completed = s->completed;
+    out->active = (long long)s->active.size();
+    long long q = 0;
+    for (auto& ln : s->lanes) q += (long long)ln.queue.size();
+    out->queued = q;
+    out->chain_sweeps = s->chain_sweeps;
+    out->launches = s->launches;
+    out->bytes_in_use = s->used;
+    out->bytes_peak = s->peak;
+    out->budget = s->budget;
+    out->elapsed_s = now_s(s->t0);
+    return DQMC_OK;
+}
+
+const char* dqmc_server_error(dqmc_server_t s) { return s ? s->err.c_str() : "null server"; }
+
+int dqmc_server_destroy(dqmc_server_t s) {
+    if (!s) return DQMC_ERR_ARG;
+    if (s->worker.joinable()) {
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            for (Job* j : s->active) j->cancel = true;
+            for (auto& ln : s->lanes)
+                for (Job* j : ln.queue) j->cancel = true;
+            s->stop = true;
+        }
+        s->cv.notify_all();
+        s->worker.join();
+    }
+    for (auto& ln : s->lanes) {
+        if (ln.stream) hipStreamSynchronize(ln.stream);
+        if (ln.d_descs) hipFree(ln.d_descs);
+        if (ln.d_init) hipFree(ln.d_init);
+        if (ln.ev) hipEventDestroy(ln.ev);
+        if (ln.stream) hipStreamDestroy(ln.stream);
+    }
+    if (s->pool_base) hipFree(s->pool_base);
+    delete s;
+    return DQMC_OK;
+}
+
+}  // extern "C"

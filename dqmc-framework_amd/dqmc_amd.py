@@ -1,0 +1,332 @@
+"""Python binding (ctypes) of the MI355X DQMC engine's C-ABI (include/dqmc_amd.h).
+
+The engine itself is native: HIP kernels for gfx950 plus a C++ host runtime
+in lib/libdqmc_amd.so.  This module only marshals arguments.  It fails
+loudly when the library has not been built; there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libdqmc_amd.so")
+
+OK, ERR_ARG, ERR_HIP, ERR_MEMORY, ERR_STATE, ERR_NUMERIC = 0, -1, -2, -3, -4, -5
+REPULSIVE, ATTRACTIVE = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+SWEEP_MEASURE, SWEEP_RECORD, SWEEP_FROZEN = 1, 2, 4
+OBS_NAMES = ["sign", "filling_number", "double_occupancy", "kinetic_energy", "momentum_distribution",
+             "local_spin_corr", "spin_density_structure_factor", "charge_density_structure_factor",
+             "s_wave_pairing_corr"]
+
+EXPORTS = ["dqmc_context_create", "dqmc_context_destroy", "dqmc_context_error", "dqmc_batch_footprint",
+           "dqmc_batch_create", "dqmc_batch_destroy", "dqmc_batch_size", "dqmc_fields_random", "dqmc_set_fields",
+           "dqmc_get_fields", "dqmc_set_rng_state", "dqmc_set_expk", "dqmc_get_expk", "dqmc_init", "dqmc_sweep",
+           "dqmc_synchronize", "dqmc_last_sweep_ms", "dqmc_get_green", "dqmc_get_slice_green", "dqmc_get_slice_signs",
+           "dqmc_get_accepts", "dqmc_get_info", "dqmc_get_measure", "dqmc_clear_measure", "dqmc_k_gemm",
+           "dqmc_server_create", "dqmc_server_submit", "dqmc_server_cancel", "dqmc_server_poll",
+           "dqmc_server_wait_idle", "dqmc_server_get_stats", "dqmc_server_error", "dqmc_server_destroy"]
+
+
+class ChainParams(C.Structure):
+    _fields_ = [("model", C.c_int), ("side", C.c_int), ("time_size", C.c_int), ("stabilization_pace", C.c_int),
+                ("checkerboard", C.c_int), ("delay", C.c_int), ("momentum_index", C.c_int),
+                ("precision", C.c_int), ("beta", C.c_double), ("hopping_t", C.c_double),
+                ("onsite_u", C.c_double), ("chemical_potential", C.c_double), ("seed", C.c_longlong)]
+
+
+class ChainInfo(C.Structure):
+    _fields_ = [("sign", C.c_double), ("wrap_error", C.c_double), ("rng", C.c_ulonglong), ("pos", C.c_int),
+                ("status", C.c_int), ("accepted", C.c_longlong), ("sweeps", C.c_longlong)]
+
+
+class Job(C.Structure):
+    _fields_ = [("chain", ChainParams), ("warmup_sweeps", C.c_int), ("measure_sweeps", C.c_int),
+                ("bin_size", C.c_int)]
+
+
+class JobResult(C.Structure):
+    _fields_ = [("id", C.c_longlong), ("status", C.c_int), ("n_bins", C.c_int), ("t_submit", C.c_double),
+                ("t_start", C.c_double), ("t_finish", C.c_double), ("sweeps", C.c_longlong),
+                ("wrap_error", C.c_double), ("mean", C.c_double * 9), ("error", C.c_double * 9)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_longlong), ("completed", C.c_longlong), ("active", C.c_longlong),
+                ("queued", C.c_longlong), ("chain_sweeps", C.c_longlong), ("launches", C.c_longlong),
+                ("bytes_in_use", C.c_size_t), ("bytes_peak", C.c_size_t), ("budget", C.c_size_t),
+                ("elapsed_s", C.c_double)]
+
+
+class DqmcError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load lib/libdqmc_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DqmcError(f"{LIB_PATH} is missing: build it with `make -C {HERE}` (hipcc, gfx950)")
+        L = C.CDLL(LIB_PATH)
+        vp, ip = C.c_void_p, C.c_int
+        L.dqmc_context_create.argtypes = [ip, C.POINTER(vp)]
+        L.dqmc_context_destroy.argtypes = [vp]
+        L.dqmc_context_error.argtypes = [vp]
+        L.dqmc_context_error.restype = C.c_char_p
+        L.dqmc_batch_footprint.argtypes = [ip, C.POINTER(ChainParams), ip, C.POINTER(C.c_size_t)]
+        L.dqmc_batch_create.argtypes = [vp, ip, C.POINTER(ChainParams), ip, C.POINTER(vp)]
+        for n in ["dqmc_batch_destroy", "dqmc_batch_size", "dqmc_fields_random", "dqmc_init", "dqmc_synchronize",
+                  "dqmc_clear_measure"]:
+            getattr(L, n).argtypes = [vp]
+        L.dqmc_set_fields.argtypes = [vp, ip, vp]
+        L.dqmc_get_fields.argtypes = [vp, ip, vp]
+        L.dqmc_set_rng_state.argtypes = [vp, ip, C.c_ulonglong]
+        L.dqmc_set_expk.argtypes = [vp, ip, vp, vp]
+        L.dqmc_get_expk.argtypes = [vp, ip, vp, vp]
+        L.dqmc_sweep.argtypes = [vp, ip, ip]
+        L.dqmc_last_sweep_ms.argtypes = [vp, C.POINTER(C.c_float)]
+        L.dqmc_get_green.argtypes = [vp, ip, ip, vp]
+        L.dqmc_get_slice_green.argtypes = [vp, ip, ip, ip, vp]
+        L.dqmc_get_slice_signs.argtypes = [vp, ip, vp]
+        L.dqmc_get_accepts.argtypes = [vp, ip, vp]
+        L.dqmc_get_info.argtypes = [vp, ip, C.POINTER(ChainInfo)]
+        L.dqmc_get_measure.argtypes = [vp, ip, vp, vp]
+        L.dqmc_k_gemm.argtypes = [vp, ip, ip, ip, ip, ip, ip, vp, vp, vp, C.c_double, vp, vp, vp]
+        L.dqmc_server_create.argtypes = [vp, C.c_size_t, ip, ip, C.POINTER(vp)]
+        L.dqmc_server_submit.argtypes = [vp, C.POINTER(Job), C.POINTER(C.c_longlong)]
+        L.dqmc_server_cancel.argtypes = [vp, C.c_longlong]
+        L.dqmc_server_poll.argtypes = [vp, C.POINTER(JobResult), ip, C.POINTER(ip)]
+        L.dqmc_server_wait_idle.argtypes = [vp, C.c_double]
+        L.dqmc_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
+        L.dqmc_server_error.argtypes = [vp]
+        L.dqmc_server_error.restype = C.c_char_p
+        L.dqmc_server_destroy.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def make_params(side, time_size, beta, pace=10, model=REPULSIVE, checkerboard=False, delay=16, momentum=-1,
+                t=1.0, U=4.0, mu=0.0, seed=42, precision=PREC_FP64):
+    """One chain's parameters; attractive chains take |U|."""
+    return ChainParams(int(model), int(side), int(time_size), int(pace), int(bool(checkerboard)), int(delay),
+                       int(momentum), int(precision), float(beta), float(t), float(U), float(mu), int(seed))
+
+
+def footprint(params, record=False):
+    arr = (ChainParams * len(params))(*params)
+    out = C.c_size_t(0)
+    rc = lib().dqmc_batch_footprint(len(params), arr, int(record), C.byref(out))
+    if rc != OK:
+        raise DqmcError(f"dqmc_batch_footprint failed ({rc})")
+    return out.value
+
+
+class Context:
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        rc = lib().dqmc_context_create(device, C.byref(self.h))
+        if rc != OK:
+            raise DqmcError(f"dqmc_context_create({device}) failed ({rc})")
+
+    def gemm(self, variant, A, B, C=None, transA=False, transB=False, beta=0.0, rs=None, ms=None, cs=None):
+        """Run one workgroup GEMM on the device (test entry point)."""
+        A = np.asfortranarray(A, dtype=np.float64); B = np.asfortranarray(B, dtype=np.float64)
+        M = A.shape[1] if transA else A.shape[0]
+        K = A.shape[0] if transA else A.shape[1]
+        Nc = B.shape[0] if transB else B.shape[1]
+        C = np.zeros((M, Nc), order="F") if C is None else np.array(C, dtype=np.float64, order="F", copy=True)
+        vec = [None if v is None else np.ascontiguousarray(v, dtype=np.float64) for v in (rs, ms, cs)]
+        ptr = [None if v is None else _p(v) for v in vec]
+        self.check(lib().dqmc_k_gemm(self.h, variant, M, Nc, K, int(transA), int(transB), _p(A), _p(B), _p(C),
+                                     float(beta), ptr[0], ptr[1], ptr[2]), "dqmc_k_gemm")
+        return C
+
+    def error(self):
+        return lib().dqmc_context_error(self.h).decode()
+
+    def check(self, rc, what):
+        if rc != OK:
+            raise DqmcError(f"{what} failed ({rc}): {self.error()}")
+
+    def close(self):
+        if self.h:
+            lib().dqmc_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Batch:
+    """A batch of independent chains resident on one GPU."""
+
+    def __init__(self, ctx, params, record=False):
+        self.ctx = ctx
+        self.params = list(params)
+        self.n = len(self.params)
+        arr = (ChainParams * self.n)(*self.params)
+        self.h = C.c_void_p()
+        ctx.check(lib().dqmc_batch_create(ctx.h, self.n, arr, int(record), C.byref(self.h)), "dqmc_batch_create")
+        self.N = [p.side * p.side for p in self.params]
+        self.L = [p.time_size for p in self.params]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().dqmc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def fields_random(self):
+        self.ctx.check(lib().dqmc_fields_random(self.h), "dqmc_fields_random")
+
+    def set_fields(self, c, f):
+        f = np.ascontiguousarray(f, dtype=np.int8).reshape(self.L[c], self.N[c])
+        self.ctx.check(lib().dqmc_set_fields(self.h, c, _p(f)), "dqmc_set_fields")
+
+    def fields(self, c):
+        f = np.zeros((self.L[c], self.N[c]), np.int8)
+        self.ctx.check(lib().dqmc_get_fields(self.h, c, _p(f)), "dqmc_get_fields")
+        return f
+
+    def set_rng_state(self, c, x):
+        self.ctx.check(lib().dqmc_set_rng_state(self.h, c, int(x)), "dqmc_set_rng_state")
+
+    def set_expk(self, c, expK, invExpK):
+        a = np.asfortranarray(expK, dtype=np.float64); b = np.asfortranarray(invExpK, dtype=np.float64)
+        self.ctx.check(lib().dqmc_set_expk(self.h, c, _p(a), _p(b)), "dqmc_set_expk")
+
+    def expk(self, c):
+        n = self.N[c]
+        a = np.zeros((n, n), order="F"); b = np.zeros((n, n), order="F")
+        self.ctx.check(lib().dqmc_get_expk(self.h, c, _p(a), _p(b)), "dqmc_get_expk")
+        return a, b
+
+    def init(self):
+        self.ctx.check(lib().dqmc_init(self.h), "dqmc_init")
+
+    def sweep(self, n, measure=False, record=False, frozen=False):
+        flags = (SWEEP_MEASURE if measure else 0) | (SWEEP_RECORD if record else 0) | (SWEEP_FROZEN if frozen else 0)
+        self.ctx.check(lib().dqmc_sweep(self.h, int(n), flags), "dqmc_sweep")
+
+    def last_sweep_ms(self):
+        ms = C.c_float(0)
+        self.ctx.check(lib().dqmc_last_sweep_ms(self.h, C.byref(ms)), "dqmc_last_sweep_ms")
+        return ms.value
+
+    def synchronize(self):
+        self.ctx.check(lib().dqmc_synchronize(self.h), "dqmc_synchronize")
+
+    def green(self, c, spin):
+        n = self.N[c]
+        g = np.zeros((n, n), order="F")
+        self.ctx.check(lib().dqmc_get_green(self.h, c, spin, _p(g)), "dqmc_get_green")
+        return g
+
+    def slice_green(self, c, slot, spin):
+        n = self.N[c]
+        g = np.zeros((n, n), order="F")
+        self.ctx.check(lib().dqmc_get_slice_green(self.h, c, slot, spin, _p(g)), "dqmc_get_slice_green")
+        return g
+
+    def slice_signs(self, c):
+        s = np.zeros(self.L[c])
+        self.ctx.check(lib().dqmc_get_slice_signs(self.h, c, _p(s)), "dqmc_get_slice_signs")
+        return s
+
+    def accepts(self, c):
+        a = np.zeros((self.L[c], self.N[c]), np.uint8)
+        self.ctx.check(lib().dqmc_get_accepts(self.h, c, _p(a)), "dqmc_get_accepts")
+        return a
+
+    def info(self, c):
+        i = ChainInfo()
+        self.ctx.check(lib().dqmc_get_info(self.h, c, C.byref(i)), "dqmc_get_info")
+        return i
+
+    def measure_accum(self, c):
+        s = np.zeros(9); n = np.zeros(9)
+        self.ctx.check(lib().dqmc_get_measure(self.h, c, _p(s), _p(n)), "dqmc_get_measure")
+        return s, n
+
+    def clear_measure(self):
+        self.ctx.check(lib().dqmc_clear_measure(self.h), "dqmc_clear_measure")
+
+
+def normalize_bin(sums, counts):
+    """MeasureHandler::normalize_stats (src/measure/measure_handler.cpp:170-189)."""
+    sums = np.asarray(sums, dtype=np.float64); counts = np.asarray(counts, dtype=np.float64)
+    sign = sums[0] / counts[0]
+    out = sums / (counts * sign)
+    out[0] = abs(sign)
+    return out
+
+
+def bin_stats(bins):
+    """Observable::analyse (include/measure/observable.h:169-193)."""
+    b = np.asarray(bins, dtype=np.float64)
+    nb = b.shape[0]
+    mean = b.mean(axis=0)
+    err = np.sqrt(np.maximum((b ** 2).mean(axis=0) - mean ** 2, 0.0)) / np.sqrt(max(nb - 1, 1))
+    return mean, err
+
+
+def result_dict(r):
+    return {"id": r.id, "status": r.status, "n_bins": r.n_bins, "t_submit": r.t_submit, "t_start": r.t_start,
+            "t_finish": r.t_finish, "latency": r.t_finish - r.t_submit, "sweeps": r.sweeps,
+            "wrap_error": r.wrap_error, "mean": list(r.mean), "error": list(r.error)}
+
+
+class Server:
+    """Continuous-batching job server (dqmc_server_* of the C-ABI)."""
+
+    def __init__(self, ctx, hbm_budget_bytes, max_active=4096, lanes=3):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        ctx.check(lib().dqmc_server_create(ctx.h, int(hbm_budget_bytes), int(max_active), int(lanes),
+                                           C.byref(self.h)), "dqmc_server_create")
+
+    def submit(self, params, warmup, measure, bin_size):
+        j = Job(params, int(warmup), int(measure), int(bin_size))
+        jid = C.c_longlong(0)
+        rc = lib().dqmc_server_submit(self.h, C.byref(j), C.byref(jid))
+        if rc != OK:
+            raise DqmcError(f"dqmc_server_submit failed ({rc}): {lib().dqmc_server_error(self.h).decode()}")
+        return jid.value
+
+    def cancel(self, jid):
+        return lib().dqmc_server_cancel(self.h, int(jid)) == OK
+
+    def poll(self, max_results=256):
+        buf = (JobResult * max_results)()
+        n = C.c_int(0)
+        rc = lib().dqmc_server_poll(self.h, buf, max_results, C.byref(n))
+        if rc != OK:
+            raise DqmcError(f"server failure ({rc})")
+        return [result_dict(buf[k]) for k in range(n.value)]
+
+    def wait_idle(self, timeout_s):
+        return lib().dqmc_server_wait_idle(self.h, float(timeout_s)) == OK
+
+    def stats(self):
+        st = ServerStats()
+        lib().dqmc_server_get_stats(self.h, C.byref(st))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().dqmc_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

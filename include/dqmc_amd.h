@@ -1,0 +1,214 @@
+/*
+ * dqmc_amd.h — C-ABI of the MI355X DQMC engine (plain pointers, sizes and
+ * int status codes; no C++ or torch types).
+ *
+ * The boundary replaces the reference's equal-time sweep plugin calls of
+ * JefferyWangSH/dqmc-framework:
+ *   chain setup      Model::ModelBase::set_model_params / initial
+ *                      (include/model/model_base.h:111-130),
+ *                    Lattice::Square (include/lattice/square.h, src/lattice/square.cpp:16-196),
+ *                    CheckerBoard::Square::initial (src/checkerboard/square.cpp:28-50),
+ *                    DqmcWalker::set_physical_params / set_stabilization_pace
+ *                      (src/dqmc_walker.cpp:25-38)
+ *   random fields    Model::*Hubbard::set_bosonic_fields_to_random (src/model/repulsive_hubbard.cpp:86-99)
+ *                    with Utils::Random::set_seed (src/random.cpp:8-10)
+ *   init             DqmcInitializer::initial_dqmc → DqmcWalker::initial_svd_stacks,
+ *                    initial_greens_functions, initial_config_sign (src/dqmc_walker.cpp:109-163)
+ *   sweeps           DqmcWalker::sweep_from_0_to_beta / sweep_from_beta_to_0
+ *                      (src/dqmc_walker.cpp:246-412), driven as in Dqmc::thermalize / measure
+ *                      (src/dqmc.cpp:48-151)
+ *   measurement      MeasureHandler::equaltime_measure (src/measure/measure_handler.cpp:136-150)
+ *                    with the methods of src/measure/measure_methods.cpp:22-255
+ *   accessors        DqmcWalker::GreenttUp/Dn, GreenttUp/Dn(t), ConfigSign(t), WrapError
+ *                      (include/dqmc_walker.h:136-164)
+ *
+ * Conventions: matrices are column-major N x N doubles (the element layout
+ * of Eigen::MatrixXd); fields are int8 (+1/-1) with element (t, i) at t*N+i;
+ * spin is +1 (up) or -1 (down).  Calls on one batch are issued on that
+ * batch's HIP stream; getters synchronize it.  One context per thread.
+ */
+#ifndef DQMC_AMD_H
+#define DQMC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define DQMC_OK 0
+#define DQMC_ERR_ARG -1      /* invalid argument (sizes, indices, unsupported combination) */
+#define DQMC_ERR_HIP -2      /* a HIP runtime call failed (message in dqmc_context_error) */
+#define DQMC_ERR_MEMORY -3   /* device allocation failed */
+#define DQMC_ERR_STATE -4    /* call out of order (e.g. sweep before init) */
+#define DQMC_ERR_NUMERIC -5  /* non-finite input or a numerical failure on the device */
+
+/* models (Model::RepulsiveHubbard / Model::AttractiveHubbard) */
+#define DQMC_MODEL_REPULSIVE 0
+#define DQMC_MODEL_ATTRACTIVE 1
+
+/* arithmetic of the slice products and wraps: FP64 throughout, or FP32 matrix
+ * cores with FP32 accumulation for the B-matrix products and wraps while the
+ * UDV stacks, the (1 + UDV)^-1 recompute, the delayed updates and the
+ * Metropolis ratio stay FP64 */
+#define DQMC_PREC_FP64 0
+#define DQMC_PREC_MIXED 1
+
+/* sweep flags */
+#define DQMC_SWEEP_MEASURE 1 /* equal-time measurement after every sweep (Dqmc::measure) */
+#define DQMC_SWEEP_RECORD 2  /* keep per-slice G, sign and accept bits of the last sweep */
+#define DQMC_SWEEP_FROZEN 4  /* no field updates: propagate, stabilize and measure a fixed
+                                configuration (as the reference's sweep_for_dynamic_greens does) */
+
+/* equal-time observables, index order of dqmc_get_measure */
+#define DQMC_OBS_SIGN 0
+#define DQMC_OBS_FILLING 1
+#define DQMC_OBS_DOUBLE_OCCUPANCY 2
+#define DQMC_OBS_KINETIC_ENERGY 3
+#define DQMC_OBS_MOMENTUM_DISTRIBUTION 4
+#define DQMC_OBS_LOCAL_SPIN_CORR 5
+#define DQMC_OBS_SDW 6
+#define DQMC_OBS_CDW 7
+#define DQMC_OBS_SWAVE 8
+#define DQMC_NUM_OBSERVABLES 9
+
+typedef struct dqmc_context* dqmc_ctx_t;
+typedef struct dqmc_batch* dqmc_batch_t;
+
+/* One Markov chain.  Mirrors the TOML keys of src/dqmc_initializer.cpp:27-338:
+ * Model.type/params, Lattice.cell (square, side x side), CheckerBoard.whether_or_not,
+ * MonteCarlo.beta/time_size/stabilization_pace, Lattice.momentum. */
+typedef struct {
+    int model;               /* DQMC_MODEL_* ; attractive takes |U| (src/model/attractive_hubbard.cpp:37) */
+    int side;                /* square lattice side length, N = side*side, 2 <= side, N <= 512 */
+    int time_size;           /* L (dtau = beta / L, src/dqmc_walker.cpp:30) */
+    int stabilization_pace;  /* p */
+    int checkerboard;        /* 0 dense exp(-dtau K), 1 checkerboard break-up (even side only) */
+    int delay;               /* delayed-update depth k (1..64); 1 applies every flip at once */
+    int momentum_index;      /* k-star index for momentum_distribution/SDW/CDW; -1 = M point (pi,pi) */
+    int precision;           /* DQMC_PREC_FP64 or DQMC_PREC_MIXED */
+    double beta;
+    double hopping_t;
+    double onsite_u;
+    double chemical_potential;
+    long long seed;          /* Utils::Random::set_seed(seed) before the random fields */
+} dqmc_chain_params;
+
+typedef struct {
+    double sign;             /* DqmcWalker::ConfigSign() */
+    double wrap_error;       /* DqmcWalker::WrapError() */
+    unsigned long long rng;  /* engine state (minstd_rand0) */
+    int pos;                 /* 0: at slice 0, time_size: at slice beta */
+    int status;              /* 0 ok, else numerical failure */
+    long long accepted;      /* accepted flips in the last sweep launch */
+    long long sweeps;        /* single-direction sweeps done since init */
+} dqmc_chain_info;
+
+/* context: device selection, error string */
+int dqmc_context_create(int device, dqmc_ctx_t* out);
+int dqmc_context_destroy(dqmc_ctx_t ctx);
+const char* dqmc_context_error(dqmc_ctx_t ctx);
+
+/* device bytes a batch of these chains needs (admission by footprint) */
+int dqmc_batch_footprint(int n_chains, const dqmc_chain_params* params, int record, size_t* bytes);
+
+/* create a batch of independent chains resident on the context's device */
+int dqmc_batch_create(dqmc_ctx_t ctx, int n_chains, const dqmc_chain_params* params, int record, dqmc_batch_t* out);
+int dqmc_batch_destroy(dqmc_batch_t b);
+int dqmc_batch_size(dqmc_batch_t b);
+
+/* fields: random (seed, reference draw order), or explicit */
+int dqmc_fields_random(dqmc_batch_t b);
+int dqmc_set_fields(dqmc_batch_t b, int chain, const int8_t* fields);
+int dqmc_get_fields(dqmc_batch_t b, int chain, int8_t* fields);
+/* engine state of a chain's random stream (continues after the field draws) */
+int dqmc_set_rng_state(dqmc_batch_t b, int chain, unsigned long long state);
+
+/* dense exp(-dtau K) / exp(+dtau K) of a chain (override / inspect) */
+int dqmc_set_expk(dqmc_batch_t b, int chain, const double* expK, const double* invExpK);
+int dqmc_get_expk(dqmc_batch_t b, int chain, double* expK, double* invExpK);
+
+/* stacks, G(0) and sign for the current fields */
+int dqmc_init(dqmc_batch_t b);
+
+/* n single-direction sweeps per chain, alternating 0->beta and beta->0 */
+int dqmc_sweep(dqmc_batch_t b, int n_sweeps, int flags);
+int dqmc_synchronize(dqmc_batch_t b);
+/* device time of the last dqmc_sweep launch (HIP events on the batch stream) */
+int dqmc_last_sweep_ms(dqmc_batch_t b, float* ms);
+
+/* accessors */
+int dqmc_get_green(dqmc_batch_t b, int chain, int spin, double* out);
+int dqmc_get_slice_green(dqmc_batch_t b, int chain, int slot, int spin, double* out);
+int dqmc_get_slice_signs(dqmc_batch_t b, int chain, double* out);
+int dqmc_get_accepts(dqmc_batch_t b, int chain, uint8_t* out);
+int dqmc_get_info(dqmc_batch_t b, int chain, dqmc_chain_info* out);
+
+/* equal-time accumulators: sums and counts per observable (normalize per bin as
+ * MeasureHandler::normalize_stats, src/measure/measure_handler.cpp:170-189) */
+int dqmc_get_measure(dqmc_batch_t b, int chain, double* sums, double* counts);
+int dqmc_clear_measure(dqmc_batch_t b);
+
+/* ------------------------------------------------------------------------
+ * Serving: continuous batching of DQMC jobs.  No counterpart in the
+ * reference (which runs one fixed chain per MPI rank, src/dqmc_main.cpp);
+ * each job runs the reference's schedule (Dqmc::thermalize then
+ * Dqmc::measure, src/dqmc.cpp:73-151) on one chain.
+ * ------------------------------------------------------------------------ */
+typedef struct dqmc_server* dqmc_server_t;
+
+typedef struct {
+    dqmc_chain_params chain;
+    int warmup_sweeps;    /* unmeasured single-direction sweeps */
+    int measure_sweeps;   /* measured sweeps, a multiple of bin_size */
+    int bin_size;         /* measured sweeps per bin */
+} dqmc_job;
+
+typedef struct {
+    long long id;
+    int status;           /* 0 done, 1 cancelled, < 0 error code */
+    int n_bins;
+    double t_submit, t_start, t_finish;  /* seconds since the server started */
+    long long sweeps;
+    double wrap_error;
+    double mean[DQMC_NUM_OBSERVABLES];   /* Observable::analyse over the bins */
+    double error[DQMC_NUM_OBSERVABLES];
+} dqmc_job_result;
+
+typedef struct {
+    long long submitted, completed, active, queued, chain_sweeps, launches;
+    size_t bytes_in_use, bytes_peak, budget;
+    double elapsed_s;
+} dqmc_server_stats;
+
+/* hbm_budget: device bytes for resident chains (one pool, allocated once);
+ * max_active: resident chains at most; n_lanes: 1..4 size-class streams */
+int dqmc_server_create(dqmc_ctx_t ctx, size_t hbm_budget, int max_active, int n_lanes, dqmc_server_t* out);
+int dqmc_server_submit(dqmc_server_t s, const dqmc_job* job, long long* id);
+int dqmc_server_cancel(dqmc_server_t s, long long id);
+int dqmc_server_poll(dqmc_server_t s, dqmc_job_result* out, int max_results, int* n_out);
+int dqmc_server_wait_idle(dqmc_server_t s, double timeout_s);
+int dqmc_server_get_stats(dqmc_server_t s, dqmc_server_stats* out);
+const char* dqmc_server_error(dqmc_server_t s);
+int dqmc_server_destroy(dqmc_server_t s);
+
+/* ------------------------------------------------------------------------
+ * Single-kernel entry points (host buffers in, host buffers out).
+ * ------------------------------------------------------------------------ */
+
+/* Workgroup GEMM of the B-matrix products (ModelBase::mult_expK_*,
+ * src/model/model_base.cpp:7-35, and the products of
+ * include/utils/numerical_stable.hpp:222-238):
+ *   C = diag(rs) op(A) diag(ms) op(B) diag(cs) + beta C,  column-major,
+ *   op(A) M x K, op(B) K x Nc; rs/ms/cs may be NULL.
+ * variant 0: vector FP64, 1: MFMA FP64, 2: MFMA FP32 (inputs rounded to FP32). */
+int dqmc_k_gemm(dqmc_ctx_t ctx, int variant, int M, int Nc, int K, int transA, int transB, const double* A,
+                const double* B, double* C, double beta, const double* rs, const double* ms, const double* cs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DQMC_AMD_H */

@@ -1,0 +1,43 @@
+"""Workgroup GEMMs of the engine (vector FP64, MFMA FP64, MFMA FP32) against
+an FP64 numpy product of the same operands, through dqmc_k_gemm."""
+import numpy as np
+import pytest
+
+import dqmc_amd as D
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(16, 16, 16), (64, 64, 64), (100, 36, 144), (144, 144, 144), (7, 65, 3), (256, 256, 256), (400, 400, 400)]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return D.Context(0)
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("shape", SHAPES, ids=[f"{m}x{n}x{k}" for m, n, k in SHAPES])
+@pytest.mark.parametrize("tA,tB", [(False, False), (True, False), (False, True)])
+def test_gemm_variants(ctx, variant, shape, tA, tB):
+    print(f"[gpu-gemm] variant {variant} shape {shape} tA {tA} tB {tB}", flush=True)
+    M, Nc, K = shape
+    rng = np.random.default_rng(M * 7 + Nc * 3 + K + variant)
+    A = rng.standard_normal((K, M) if tA else (M, K))
+    B = rng.standard_normal((Nc, K) if tB else (K, Nc))
+    C0 = rng.standard_normal((M, Nc))
+    rs, ms, cs = rng.uniform(0.5, 2, M), rng.uniform(0.5, 2, K), rng.uniform(0.5, 2, Nc)
+    opA = A.T if tA else A
+    opB = B.T if tB else B
+    ref = rs[:, None] * (opA @ np.diag(ms) @ opB) * cs[None, :] + 0.5 * C0
+    got = ctx.gemm(variant, A, B, C0, transA=tA, transB=tB, beta=0.5, rs=rs, ms=ms, cs=cs)
+    scale = np.abs(rs[:, None]) * (np.abs(opA) @ np.diag(ms) @ np.abs(opB)) * cs[None, :] + 0.5 * np.abs(C0)
+    err = np.max(np.abs(got - ref) / scale)
+    tol = 1e-14 * K if variant < 2 else 2e-6 * np.sqrt(K)
+    assert err < tol, err
+
+
+def test_gemm_plain(ctx):
+    rng = np.random.default_rng(0)
+    A = rng.standard_normal((80, 80)); B = rng.standard_normal((80, 80))
+    for v in (0, 1):
+        assert np.allclose(ctx.gemm(v, A, B), A @ B, atol=1e-12)

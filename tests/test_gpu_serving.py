@@ -1,0 +1,104 @@
+"""Continuous-batching server: admission by footprint, jobs joining while
+others run, cancellation, and that every job's result is the same alone and
+in the mix (the per-chain computation does not depend on co-resident chains)."""
+import time
+
+import numpy as np
+import pytest
+
+import dqmc_amd as D
+import serve
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return D.Context(0)
+
+
+def batch_reference(ctx, job):
+    """The same schedule through the batch API: warm-up, then bins of measured sweeps."""
+    b = D.Batch(ctx, [job["params"]])
+    b.fields_random()
+    b.init()
+    b.sweep(job["warmup"])
+    bins = []
+    for _ in range(job["measure"] // job["bin_size"]):
+        b.clear_measure()
+        b.sweep(job["bin_size"], measure=True)
+        s, c = b.measure_accum(0)
+        bins.append(D.normalize_bin(s, c))
+    mean, err = D.bin_stats(bins) if len(bins) > 1 else (bins[0], np.zeros(9))
+    return mean, err
+
+
+def small_jobs(n, seed=1):
+    jobs = serve.generate_jobs(n, seed=seed, scale=0.06)
+    # keep the test quick: lattices up to 8x8 and at most 80 slices
+    out = []
+    for j in jobs:
+        if j["side"] <= 8 and j["L"] <= 80:
+            out.append(j)
+    return out
+
+
+def test_server_matches_batch_api_and_is_mix_independent(ctx):
+    jobs = small_jobs(60)[:12]
+    assert len(jobs) >= 6
+    print(f"[gpu-serve] {len(jobs)} jobs", flush=True)
+    # alone: each job through its own server
+    alone = {}
+    for j in jobs[:3]:
+        summary, res = serve.run(ctx, [j], rate=0, budget_bytes=1 << 30)
+        alone[j["id"]] = res[j["id"]]
+        mean, err = batch_reference(ctx, j)
+        assert np.array_equal(np.array(res[j["id"]]["mean"]), mean), j["id"]
+    # in the mix: all jobs, Poisson arrivals, jobs join while others run
+    summary, res = serve.run(ctx, jobs, rate=40.0, budget_bytes=1 << 30)
+    assert summary["failed"] == 0
+    for jid, r in alone.items():
+        assert res[jid]["mean"] == r["mean"]
+        assert res[jid]["error"] == r["error"]
+        assert res[jid]["sweeps"] == r["sweeps"]
+    assert summary["p99_latency_s"] >= summary["p50_latency_s"] > 0
+
+
+def test_admission_by_footprint_and_rejection(ctx):
+    jobs = small_jobs(40, seed=3)[:6]
+    sizes = [D.footprint([j["params"]]) for j in jobs]
+    budget = max(sizes) * 2 + 4096   # room for about two resident chains
+    summary, res = serve.run(ctx, jobs, rate=0, budget_bytes=budget)
+    assert summary["failed"] == 0 and len(res) == len(jobs)
+    assert summary["peak_bytes"] <= budget
+    srv = D.Server(ctx, 1 << 20)
+    big = D.make_params(20, 200, 10.0, pace=10)
+    with pytest.raises(D.DqmcError):
+        srv.submit(big, 10, 10, 10)
+    srv.close()
+
+
+def test_cancel_frees_slot_and_leaves_others_unchanged(ctx):
+    jobs = small_jobs(40, seed=5)[:5]
+    ref = {}
+    for j in jobs:
+        ref[j["id"]] = batch_reference(ctx, j)[0]
+    srv = D.Server(ctx, 1 << 30)
+    ids = {}
+    for j in jobs:
+        ids[srv.submit(j["params"], j["warmup"], j["measure"], j["bin_size"])] = j["id"]
+    victim = list(ids)[2]
+    assert srv.cancel(victim)
+    assert srv.wait_idle(600)
+    got = {}
+    for r in srv.poll():
+        got[ids[r["id"]]] = r
+    assert got[ids[victim]]["status"] in (0, 1)
+    for sid, jid in ids.items():
+        if sid == victim:
+            continue
+        assert got[jid]["status"] == 0
+        assert np.array_equal(np.array(got[jid]["mean"]), ref[jid])
+    st = srv.stats()
+    assert st["bytes_in_use"] == 0
+    srv.close()
