@@ -92,6 +92,28 @@ def run_gpu(w, steps, warmup, chains=None):
                 status=max(statuses))
 
 
+def phase_breakdown(w, chains, sweeps=2):
+    """Device time per sweep phase (ms per chain-sweep), from the engine's phase clock."""
+    ctx = D.Context(0)
+    b = D.Batch(ctx, chain_params(w, chains))
+    b.fields_random()
+    b.init()
+    b.sweep(1)
+    b.sweep(sweeps, profile=True)
+    b.synchronize()
+    n = min(b.n, 8)
+    tot = [0.0] * 6
+    for c in range(n):
+        ph = b.info(c).phase_seconds
+        for q in range(6):
+            tot[q] += ph[q]
+    out = {name: 1000.0 * tot[q] / (n * sweeps) for q, name in enumerate(D.PHASES)}
+    out["launch_ms_per_sweep"] = b.last_sweep_ms() / sweeps
+    b.close()
+    ctx.close()
+    return out
+
+
 def cpu_baseline(w, seconds=12.0, max_procs=16):
     exe = os.path.join(ROOT, "oracle", "_build", "refcpu")
     if not os.path.exists(exe):
@@ -128,6 +150,7 @@ def main():
     ap.add_argument("--workload", default="batched-12x12", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the single-chain comparison run")
+    ap.add_argument("--phases", action="store_true", help="also report device time per sweep phase")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: chains are independent and would be sharded by chain across GPUs, "
@@ -159,6 +182,10 @@ def main():
         plain = p["chains"] * max(2, args.steps // 4) / p["wall"]
         out["plain_chain_sweeps_per_s"] = plain
         out["batched_over_plain"] = value / plain
+    if args.phases:
+        print("[bench] phase breakdown", file=sys.stderr, flush=True)
+        out["phases_ms_per_chain_sweep"] = {"batched": phase_breakdown(w, r["chains"]),
+                                            "single": phase_breakdown(w, 1)}
     if not args.no_cpu:
         print("[bench] CPU baseline (oracle, one chain per process)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(w)

@@ -12,7 +12,12 @@ enum {
     DQMC_FLAG_MEASURE = 1,   // accumulate equal-time observables after each sweep
     DQMC_FLAG_RECORD = 2,    // keep G, sign per slice and accept bits of the last sweep
     DQMC_FLAG_FROZEN = 4,    // propagate and measure without field updates (fixed configuration)
+    DQMC_FLAG_PROFILE = 8,   // accumulate per-phase device time into ChainState::phase
 };
+
+// phases timed with DQMC_FLAG_PROFILE (100 MHz device wall clock ticks)
+enum { DQMC_PH_WRAP = 0, DQMC_PH_METROPOLIS, DQMC_PH_PRODUCT, DQMC_PH_PUSH, DQMC_PH_FRESH, DQMC_PH_MEASURE,
+       DQMC_PH_COUNT };
 
 // number of accumulated equal-time quantities, in this order:
 // sign, filling, double occupancy, kinetic energy, momentum distribution,
@@ -28,6 +33,7 @@ struct ChainState {
     int n_right;         // length of the right UDV stack (left length = n_stack - n_right)
     int status;          // 0 ok; nonzero: numerical failure code
     int accepted;        // accepted flips in the last launch (for reporting)
+    unsigned long long phase[8];  // device ticks per phase (DQMC_FLAG_PROFILE)
 };
 
 struct ChainDesc {

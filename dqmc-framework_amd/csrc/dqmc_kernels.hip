@@ -771,10 +771,10 @@ __device__ void wg_metropolis(const ChainDesc& d, int e, uint64_t& rng, double& 
 // ---------------------------------------------------------------------------
 // Equal-time measurements of one slice (src/measure/measure_methods.cpp:22-255)
 // ---------------------------------------------------------------------------
-__device__ void wg_measure(const ChainDesc& d, double cs, Lds& s) {
+__device__ void wg_measure_from(const ChainDesc& d, const double* G, double cs, Lds& s) {
     const int N = d.N, side = d.side;
-    const double* Gu = d.G;
-    const double* Gd = d.nspin == 2 ? d.G + (size_t)N * N : d.G;
+    const double* Gu = G;
+    const double* Gd = d.nspin == 2 ? G + (size_t)N * N : G;
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // tr, docc, kin, loc, md, sdw, cdw, sw
     for (int a = tid(); a < N; a += NT) {
         const double du = Gu[(size_t)a * N + a], dd = Gd[(size_t)a * N + a];
@@ -822,6 +822,8 @@ __device__ void wg_measure(const ChainDesc& d, double cs, Lds& s) {
     __syncthreads();
 }
 
+__device__ void wg_measure(const ChainDesc& d, double cs, Lds& s) { wg_measure_from(d, d.G, cs, s); }
+
 __device__ void record_slot(const ChainDesc& d, int slot, double sign, bool with_sign) {
     const int N = d.N;
     const size_t n2 = (size_t)N * N;
@@ -846,6 +848,19 @@ __device__ void stabilize_green(const ChainDesc& d, int lslot, int rslot, bool t
     wg_copy(d.G, d.Gw, (int)(n2 * d.nspin));
 }
 
+// per-phase device time (thread 0 reads the 100 MHz wall clock at barriers)
+struct PhaseClock {
+    bool on;
+    unsigned long long t;
+    __device__ explicit PhaseClock(bool enabled) : on(enabled), t(enabled ? wall_clock64() : 0) {}
+    __device__ void lap(ChainState& st, int ph) {
+        if (!on) return;
+        const unsigned long long n = wall_clock64();
+        if (tid() == 0) st.phase[ph] += n - t;
+        t = n;
+    }
+};
+
 // ---------------------------------------------------------------------------
 // sweeps (src/dqmc_walker.cpp:246-325 and :334-412)
 // ---------------------------------------------------------------------------
@@ -853,24 +868,31 @@ __device__ void sweep_up(const ChainDesc& d, ChainState& st, uint64_t& rng, doub
     const int N = d.N, L = d.L;
     const size_t n2 = (size_t)N * N;
     const bool rec = flags & DQMC_FLAG_RECORD, meas = flags & DQMC_FLAG_MEASURE;
+    PhaseClock clk(flags & DQMC_FLAG_PROFILE);
     for (int sp = 0; sp < d.nspin; ++sp) wg_identity(d.Tmp + sp * n2, N);
     for (int t = 1; t <= L; ++t) {
         const int e = t - 1;
         load_field_row(d, e, s);
         for (int sp = 0; sp < d.nspin; ++sp) wg_wrap(d, sp, true, d.G + sp * n2, d.Gw + sp * n2, s);
+        clk.lap(st, DQMC_PH_WRAP);
         if (!(flags & DQMC_FLAG_FROZEN)) wg_metropolis(d, e, rng, sign, rec, nacc, s);
+        clk.lap(st, DQMC_PH_METROPOLIS);
         refresh_scalings(d, s);
         for (int sp = 0; sp < d.nspin; ++sp) wg_bmul(d, 0, sp, d.Tmp + sp * n2, d.Gw + sp * n2, s);
+        clk.lap(st, DQMC_PH_PRODUCT);
         if (t % d.pace == 0 || t == L) {
             st.n_right -= 1;
             const int slot = st.n_right;
             for (int sp = 0; sp < d.nspin; ++sp)
                 wg_push(d, sp, slot, slot + 1 < d.n_stack ? slot + 1 : -1, d.Tmp + sp * n2, s);
+            clk.lap(st, DQMC_PH_PUSH);
             stabilize_green(d, slot, slot > 0 ? slot - 1 : -1, true, st, s);
             for (int sp = 0; sp < d.nspin; ++sp) wg_identity(d.Tmp + sp * n2, N);
+            clk.lap(st, DQMC_PH_FRESH);
         }
         if (rec) record_slot(d, e, sign, true);
         if (meas) wg_measure(d, sign, s);
+        clk.lap(st, DQMC_PH_MEASURE);
     }
     st.pos = L;
 }
@@ -879,6 +901,7 @@ __device__ void sweep_down(const ChainDesc& d, ChainState& st, uint64_t& rng, do
     const int N = d.N, L = d.L;
     const size_t n2 = (size_t)N * N;
     const bool rec = flags & DQMC_FLAG_RECORD, meas = flags & DQMC_FLAG_MEASURE;
+    PhaseClock clk(flags & DQMC_FLAG_PROFILE);
     for (int sp = 0; sp < d.nspin; ++sp) wg_identity(d.Tmp + sp * n2, N);
     double sign_top = sign;
     for (int t = L; t >= 1; --t) {
@@ -888,17 +911,23 @@ __device__ void sweep_down(const ChainDesc& d, ChainState& st, uint64_t& rng, do
             for (int sp = 0; sp < d.nspin; ++sp)
                 wg_push(d, sp, slot, slot > 0 ? slot - 1 : -1, d.Tmp + sp * n2, s);
             st.n_right += 1;
+            clk.lap(st, DQMC_PH_PUSH);
             stabilize_green(d, slot + 1 < d.n_stack ? slot + 1 : -1, slot, true, st, s);
             for (int sp = 0; sp < d.nspin; ++sp) wg_identity(d.Tmp + sp * n2, N);
+            clk.lap(st, DQMC_PH_FRESH);
         }
         load_field_row(d, e, s);
         if (!(flags & DQMC_FLAG_FROZEN)) wg_metropolis(d, e, rng, sign, rec, nacc, s);
+        clk.lap(st, DQMC_PH_METROPOLIS);
         if (rec) record_slot(d, e, sign, true);
         if (t == L) sign_top = sign;
         else if (meas) wg_measure(d, sign, s);
+        clk.lap(st, DQMC_PH_MEASURE);
         refresh_scalings(d, s);
         for (int sp = 0; sp < d.nspin; ++sp) wg_bmul(d, 4, sp, d.Tmp + sp * n2, d.Gw + sp * n2, s);
+        clk.lap(st, DQMC_PH_PRODUCT);
         for (int sp = 0; sp < d.nspin; ++sp) wg_wrap(d, sp, false, d.G + sp * n2, d.Gw + sp * n2, s);
+        clk.lap(st, DQMC_PH_WRAP);
     }
     {
         const int slot = st.n_right;
@@ -944,6 +973,7 @@ __global__ void __launch_bounds__(NT) k_init(const ChainDesc* __restrict__ descs
         st.pos = 0;
         st.status = 0;
         st.accepted = 0;
+        for (int q = 0; q < 8; ++q) st.phase[q] = 0;
         *d.st = st;
     }
 }
@@ -969,6 +999,15 @@ __global__ void __launch_bounds__(NT) k_sweep(const ChainDesc* __restrict__ desc
     }
 }
 
+// MeasureHandler::equaltime_measure over the slices kept by a RECORD sweep
+// (src/measure/measure_handler.cpp:136-150: every method loops over the L stored slices)
+__global__ void __launch_bounds__(NT) k_measure_recorded(const ChainDesc* __restrict__ descs) {
+    __shared__ Lds s;
+    const ChainDesc& d = descs[blockIdx.x];
+    const size_t slot_size = (size_t)d.nspin * d.N * d.N;
+    for (int slot = 0; slot < d.L; ++slot) wg_measure_from(d, d.recG + slot * slot_size, d.recSign[slot], s);
+}
+
 // one workgroup runs one GEMM variant (0 vector FP64, 1 MFMA FP64, 2 MFMA FP32)
 __global__ void __launch_bounds__(NT) k_gemm(int variant, int M, int Nc, int K, const double* A, int lda, int tA,
                                              const double* B, int ldb, int tB, double* C, int ldc, const double* rs,
@@ -982,6 +1021,10 @@ __global__ void __launch_bounds__(NT) k_gemm(int variant, int M, int Nc, int K, 
 }  // namespace dqmc
 
 extern "C" {
+hipError_t dqmc_launch_measure_recorded(const ChainDesc* d_descs, int n_chains, hipStream_t stream) {
+    hipLaunchKernelGGL(dqmc::k_measure_recorded, dim3(n_chains), dim3(NT), 0, stream, d_descs);
+    return hipGetLastError();
+}
 hipError_t dqmc_launch_gemm(int variant, int M, int Nc, int K, const double* A, int lda, int tA, const double* B,
                             int ldb, int tB, double* C, int ldc, const double* rs, const double* ms, const double* cs,
                             double beta, hipStream_t stream) {

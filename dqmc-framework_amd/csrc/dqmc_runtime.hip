@@ -322,6 +322,8 @@ int dqmc_batch_destroy(dqmc_batch_t b) {
     if (b->stream) hipStreamSynchronize(b->stream);
     if (b->ev0) hipEventDestroy(b->ev0);
     if (b->ev1) hipEventDestroy(b->ev1);
+    if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) hipGraphDestroy(b->graph);
     if (b->arena && b->owns_arena) hipFree(b->arena);
     if (b->stream && b->owns_stream) hipStreamDestroy(b->stream);
     delete b;
@@ -329,6 +331,12 @@ int dqmc_batch_destroy(dqmc_batch_t b) {
 }
 
 int dqmc_batch_size(dqmc_batch_t b) { return b ? b->n : DQMC_ERR_ARG; }
+
+int dqmc_batch_use_graph(dqmc_batch_t b, int enable) {
+    if (!b) return DQMC_ERR_ARG;
+    b->use_graph = enable != 0;
+    return DQMC_OK;
+}
 
 static int check_chain(dqmc_batch_t b, int c) {
     if (!b) return DQMC_ERR_ARG;
@@ -455,8 +463,20 @@ int dqmc_sweep(dqmc_batch_t b, int n_sweeps, int flags) {
         HIPCHK(b->ctx, hipEventCreate(&b->ev0));
         HIPCHK(b->ctx, hipEventCreate(&b->ev1));
     }
+    if (b->use_graph && (b->graph_n != n_sweeps || b->graph_flags != flags)) {
+        if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
+        if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
+        HIPCHK(b->ctx, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t le = dqmc_launch_sweep(b->d_descs, b->n, n_sweeps, flags, b->stream);
+        HIPCHK(b->ctx, hipStreamEndCapture(b->stream, &b->graph));
+        HIPCHK(b->ctx, le);
+        HIPCHK(b->ctx, hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+        b->graph_n = n_sweeps;
+        b->graph_flags = flags;
+    }
     HIPCHK(b->ctx, hipEventRecord(b->ev0, b->stream));
-    HIPCHK(b->ctx, dqmc_launch_sweep(b->d_descs, b->n, n_sweeps, flags, b->stream));
+    if (b->use_graph) HIPCHK(b->ctx, hipGraphLaunch(b->graph_exec, b->stream));
+    else HIPCHK(b->ctx, dqmc_launch_sweep(b->d_descs, b->n, n_sweeps, flags, b->stream));
     HIPCHK(b->ctx, hipEventRecord(b->ev1, b->stream));
     b->timed = true;
     for (auto& h : b->chains) {
@@ -538,6 +558,7 @@ int dqmc_get_info(dqmc_batch_t b, int c, dqmc_chain_info* out) {
     out->status = st.status;
     out->accepted = st.accepted;
     out->sweeps = b->chains[c].sweeps;
+    for (int q = 0; q < 6; ++q) out->phase_seconds[q] = (double)st.phase[q] * 1e-8;  // 100 MHz device wall clock
     return DQMC_OK;
 }
 
@@ -551,6 +572,15 @@ int dqmc_get_measure(dqmc_batch_t b, int c, double* sums, double* counts) {
         sums[o] = acc[o];
         counts[o] = b->chains[c].counts;
     }
+    return DQMC_OK;
+}
+
+int dqmc_measure_recorded(dqmc_batch_t b) {
+    if (!b) return DQMC_ERR_ARG;
+    if (!b->record) { b->ctx->err = "batch created without record buffers"; return DQMC_ERR_STATE; }
+    if (!b->initialized) { b->ctx->err = "measurement before dqmc_init"; return DQMC_ERR_STATE; }
+    HIPCHK(b->ctx, dqmc_launch_measure_recorded(b->d_descs, b->n, b->stream));
+    for (auto& h : b->chains) h.counts += (double)h.p.time_size;
     return DQMC_OK;
 }
 
@@ -591,6 +621,33 @@ int dqmc_k_gemm(dqmc_ctx_t ctx, int variant, int M, int Nc, int K, int tA, int t
     if (dC) hipFree(dC);
     if (dv) hipFree(dv);
     if (e != hipSuccess) { ctx->err = std::string("dqmc_k_gemm: ") + hipGetErrorString(e); return DQMC_ERR_HIP; }
+    return DQMC_OK;
+}
+
+// Delayed-update depth autotuner: times `sweeps` single-direction sweeps of a
+// batch of `n_chains` chains of the given shape for each candidate depth (after
+// one untimed sweep) with HIP events, and returns the fastest.
+int dqmc_autotune_delay(dqmc_ctx_t ctx, const dqmc_chain_params* shape, int n_chains, const int* candidates,
+                        int n_candidates, int sweeps, int* best, float* ms_out) {
+    if (!ctx || !shape || !candidates || !best || n_chains < 1 || n_candidates < 1 || sweeps < 1) return DQMC_ERR_ARG;
+    float best_ms = -1.0f;
+    for (int c = 0; c < n_candidates; ++c) {
+        std::vector<dqmc_chain_params> ps(n_chains, *shape);
+        for (int k = 0; k < n_chains; ++k) { ps[k].delay = candidates[c]; ps[k].seed = shape->seed + k; }
+        dqmc_batch_t b = nullptr;
+        int rc = dqmc_batch_create(ctx, n_chains, ps.data(), 0, &b);
+        if (rc != DQMC_OK) return rc;
+        float ms = 0.0f;
+        rc = dqmc_fields_random(b);
+        if (rc == DQMC_OK) rc = dqmc_init(b);
+        if (rc == DQMC_OK) rc = dqmc_sweep(b, 1, 0);
+        if (rc == DQMC_OK) rc = dqmc_sweep(b, sweeps, 0);
+        if (rc == DQMC_OK) rc = dqmc_last_sweep_ms(b, &ms);
+        dqmc_batch_destroy(b);
+        if (rc != DQMC_OK) return rc;
+        if (ms_out) ms_out[c] = ms;
+        if (best_ms < 0.0f || ms < best_ms) { best_ms = ms; *best = candidates[c]; }
+    }
     return DQMC_OK;
 }
 

@@ -10,6 +10,7 @@
 #include "chain_desc.h"
 
 extern "C" hipError_t dqmc_launch_init(const ChainDesc* d_descs, int n_chains, hipStream_t stream);
+extern "C" hipError_t dqmc_launch_measure_recorded(const ChainDesc* d_descs, int n_chains, hipStream_t stream);
 extern "C" hipError_t dqmc_launch_sweep(const ChainDesc* d_descs, int n_chains, int n_sweeps, int flags,
                                         hipStream_t stream);
 
@@ -48,6 +49,11 @@ struct dqmc_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;  // bracket the last sweep launch on the batch stream
     bool timed = false;
     bool owns_arena = false, owns_stream = false;
+    // sweep launch replayed from a captured HIP graph (per n_sweeps/flags)
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_n = -1, graph_flags = -1;
 };
 
 #define HIPCHK(ctx, call)                                                           \

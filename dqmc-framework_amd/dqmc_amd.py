@@ -15,16 +15,18 @@ LIB_PATH = os.path.join(HERE, "lib", "libdqmc_amd.so")
 OK, ERR_ARG, ERR_HIP, ERR_MEMORY, ERR_STATE, ERR_NUMERIC = 0, -1, -2, -3, -4, -5
 REPULSIVE, ATTRACTIVE = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
-SWEEP_MEASURE, SWEEP_RECORD, SWEEP_FROZEN = 1, 2, 4
+SWEEP_MEASURE, SWEEP_RECORD, SWEEP_FROZEN, SWEEP_PROFILE = 1, 2, 4, 8
+PHASES = ["wrap", "metropolis", "product", "push", "fresh", "measure"]
 OBS_NAMES = ["sign", "filling_number", "double_occupancy", "kinetic_energy", "momentum_distribution",
              "local_spin_corr", "spin_density_structure_factor", "charge_density_structure_factor",
              "s_wave_pairing_corr"]
 
 EXPORTS = ["dqmc_context_create", "dqmc_context_destroy", "dqmc_context_error", "dqmc_batch_footprint",
-           "dqmc_batch_create", "dqmc_batch_destroy", "dqmc_batch_size", "dqmc_fields_random", "dqmc_set_fields",
+           "dqmc_batch_create", "dqmc_batch_destroy", "dqmc_batch_size", "dqmc_batch_use_graph", "dqmc_fields_random", "dqmc_set_fields",
            "dqmc_get_fields", "dqmc_set_rng_state", "dqmc_set_expk", "dqmc_get_expk", "dqmc_init", "dqmc_sweep",
            "dqmc_synchronize", "dqmc_last_sweep_ms", "dqmc_get_green", "dqmc_get_slice_green", "dqmc_get_slice_signs",
-           "dqmc_get_accepts", "dqmc_get_info", "dqmc_get_measure", "dqmc_clear_measure", "dqmc_k_gemm",
+           "dqmc_get_accepts", "dqmc_get_info", "dqmc_get_measure", "dqmc_measure_recorded", "dqmc_clear_measure",
+           "dqmc_k_gemm", "dqmc_autotune_delay",
            "dqmc_server_create", "dqmc_server_submit", "dqmc_server_cancel", "dqmc_server_poll",
            "dqmc_server_wait_idle", "dqmc_server_get_stats", "dqmc_server_error", "dqmc_server_destroy"]
 
@@ -38,7 +40,8 @@ class ChainParams(C.Structure):
 
 class ChainInfo(C.Structure):
     _fields_ = [("sign", C.c_double), ("wrap_error", C.c_double), ("rng", C.c_ulonglong), ("pos", C.c_int),
-                ("status", C.c_int), ("accepted", C.c_longlong), ("sweeps", C.c_longlong)]
+                ("status", C.c_int), ("accepted", C.c_longlong), ("sweeps", C.c_longlong),
+                ("phase_seconds", C.c_double * 6)]
 
 
 class Job(C.Structure):
@@ -81,8 +84,9 @@ def lib():
         L.dqmc_batch_footprint.argtypes = [ip, C.POINTER(ChainParams), ip, C.POINTER(C.c_size_t)]
         L.dqmc_batch_create.argtypes = [vp, ip, C.POINTER(ChainParams), ip, C.POINTER(vp)]
         for n in ["dqmc_batch_destroy", "dqmc_batch_size", "dqmc_fields_random", "dqmc_init", "dqmc_synchronize",
-                  "dqmc_clear_measure"]:
+                  "dqmc_clear_measure", "dqmc_measure_recorded"]:
             getattr(L, n).argtypes = [vp]
+        L.dqmc_batch_use_graph.argtypes = [vp, ip]
         L.dqmc_set_fields.argtypes = [vp, ip, vp]
         L.dqmc_get_fields.argtypes = [vp, ip, vp]
         L.dqmc_set_rng_state.argtypes = [vp, ip, C.c_ulonglong]
@@ -97,6 +101,7 @@ def lib():
         L.dqmc_get_info.argtypes = [vp, ip, C.POINTER(ChainInfo)]
         L.dqmc_get_measure.argtypes = [vp, ip, vp, vp]
         L.dqmc_k_gemm.argtypes = [vp, ip, ip, ip, ip, ip, ip, vp, vp, vp, C.c_double, vp, vp, vp]
+        L.dqmc_autotune_delay.argtypes = [vp, C.POINTER(ChainParams), ip, vp, ip, ip, C.POINTER(ip), vp]
         L.dqmc_server_create.argtypes = [vp, C.c_size_t, ip, ip, C.POINTER(vp)]
         L.dqmc_server_submit.argtypes = [vp, C.POINTER(Job), C.POINTER(C.c_longlong)]
         L.dqmc_server_cancel.argtypes = [vp, C.c_longlong]
@@ -150,6 +155,15 @@ class Context:
                                      float(beta), ptr[0], ptr[1], ptr[2]), "dqmc_k_gemm")
         return C
 
+    def autotune_delay(self, params, n_chains, candidates=(8, 16, 32, 64), sweeps=2):
+        """Fastest delayed-update depth for chains of this shape (times every candidate)."""
+        cand = np.ascontiguousarray(candidates, dtype=np.int32)
+        ms = np.zeros(len(cand), dtype=np.float32)
+        best = C.c_int(0)
+        self.check(lib().dqmc_autotune_delay(self.h, C.byref(params), int(n_chains), _p(cand), len(cand), int(sweeps),
+                                             C.byref(best), _p(ms)), "dqmc_autotune_delay")
+        return best.value, dict(zip([int(c) for c in cand], [float(x) for x in ms]))
+
     def error(self):
         return lib().dqmc_context_error(self.h).decode()
 
@@ -187,6 +201,9 @@ class Batch:
     def __del__(self):
         self.close()
 
+    def use_graph(self, enable=True):
+        self.ctx.check(lib().dqmc_batch_use_graph(self.h, int(enable)), "dqmc_batch_use_graph")
+
     def fields_random(self):
         self.ctx.check(lib().dqmc_fields_random(self.h), "dqmc_fields_random")
 
@@ -215,8 +232,9 @@ class Batch:
     def init(self):
         self.ctx.check(lib().dqmc_init(self.h), "dqmc_init")
 
-    def sweep(self, n, measure=False, record=False, frozen=False):
-        flags = (SWEEP_MEASURE if measure else 0) | (SWEEP_RECORD if record else 0) | (SWEEP_FROZEN if frozen else 0)
+    def sweep(self, n, measure=False, record=False, frozen=False, profile=False):
+        flags = ((SWEEP_MEASURE if measure else 0) | (SWEEP_RECORD if record else 0) | (SWEEP_FROZEN if frozen else 0)
+                 | (SWEEP_PROFILE if profile else 0))
         self.ctx.check(lib().dqmc_sweep(self.h, int(n), flags), "dqmc_sweep")
 
     def last_sweep_ms(self):
@@ -258,6 +276,9 @@ class Batch:
         s = np.zeros(9); n = np.zeros(9)
         self.ctx.check(lib().dqmc_get_measure(self.h, c, _p(s), _p(n)), "dqmc_get_measure")
         return s, n
+
+    def measure_recorded(self):
+        self.ctx.check(lib().dqmc_measure_recorded(self.h), "dqmc_measure_recorded")
 
     def clear_measure(self):
         self.ctx.check(lib().dqmc_clear_measure(self.h), "dqmc_clear_measure")

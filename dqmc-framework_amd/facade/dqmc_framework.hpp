@@ -1,0 +1,446 @@
+// dqmc_framework.hpp — C++ host mirror of dqmc-framework's plugin interface
+// for the equal-time sweep, running on the MI355X engine through the C-ABI
+// of include/dqmc_amd.h.
+//
+// Class names, method names, argument meaning and call order follow the
+// reference so that a driver written against it reads the same:
+//   Lattice::Square                 include/lattice/square.h, src/lattice/square.cpp
+//   Model::RepulsiveHubbard /
+//   Model::AttractiveHubbard        include/model/*_hubbard.h (set_model_params, link)
+//   CheckerBoard::Square            include/checkerboard/square.h
+//   QuantumMonteCarlo::DqmcWalker   include/dqmc_walker.h:107-184
+//   Measure::MeasureHandler         include/measure/measure_handler.h:60-118
+//   QuantumMonteCarlo::DqmcInitializer::initial_modules / initial_dqmc
+//                                   include/dqmc_initializer.h:47-78
+//   QuantumMonteCarlo::Dqmc::thermalize / measure / analyse   include/dqmc.h:61-73
+//   Utils::Random::set_seed         include/random.h:12-17
+// Matrices are column-major std::vector<double> (Eigen::MatrixXd layout).
+// Errors: the reference asserts or prints and exits; here a failing engine
+// call throws std::runtime_error with the engine's message.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <random>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/dqmc_amd.h"
+
+namespace Utils {
+// The reference's global engine (std::default_random_engine, src/random.cpp:6).
+class Random {
+   public:
+    static std::default_random_engine& Engine() {
+        static std::default_random_engine e(0);
+        return e;
+    }
+    static void set_seed(const int seed) { Engine().seed(seed); }
+    // state of the engine (minstd_rand0 keeps one integer)
+    static unsigned long long state() {
+        std::ostringstream os;
+        os << Engine();
+        return std::stoull(os.str());
+    }
+    static void set_state(unsigned long long x) {
+        std::istringstream is(std::to_string(x));
+        is >> Engine();
+    }
+};
+}  // namespace Utils
+
+namespace dqmc_detail {
+inline void check(int rc, dqmc_ctx_t ctx, const char* what) {
+    if (rc != DQMC_OK) throw std::runtime_error(std::string(what) + ": " + (ctx ? dqmc_context_error(ctx) : "") +
+                                                " (status " + std::to_string(rc) + ")");
+}
+inline dqmc_ctx_t context() {
+    static dqmc_ctx_t ctx = [] {
+        dqmc_ctx_t c = nullptr;
+        if (dqmc_context_create(0, &c) != DQMC_OK) throw std::runtime_error("dqmc_context_create(0) failed");
+        return c;
+    }();
+    return ctx;
+}
+}  // namespace dqmc_detail
+
+namespace Lattice {
+using LatticeIntVec = std::vector<int>;
+
+class LatticeBase {
+   protected:
+    bool m_initial_status{false};
+    int m_space_dim{}, m_side_length{}, m_space_size{}, m_coordination_number{}, m_num_k_stars{};
+
+   public:
+    virtual ~LatticeBase() = default;
+    virtual void set_lattice_params(const LatticeIntVec& side_length_vec) = 0;
+    virtual void initial() = 0;
+    bool InitialStatus() const { return m_initial_status; }
+    int SpaceDim() const { return m_space_dim; }
+    int SpaceSize() const { return m_space_size; }
+    int SideLength() const { return m_side_length; }
+    int CoordinationNumber() const { return m_coordination_number; }
+    int kStarsNum() const { return m_num_k_stars; }
+};
+
+// 2-D periodic square lattice (src/lattice/square.cpp:16-196)
+class Square : public LatticeBase {
+    int m_gamma{}, m_x{}, m_m{};
+
+   public:
+    void set_lattice_params(const LatticeIntVec& v) override {
+        if (v.size() != 2 || v[0] != v[1] || v[0] < 2) throw std::invalid_argument("square lattice needs {L, L}, L >= 2");
+        m_space_dim = 2;
+        m_coordination_number = 4;
+        m_side_length = v[0];
+        m_space_size = v[0] * v[1];
+    }
+    void initial() override {
+        const int h = m_side_length / 2;
+        m_num_k_stars = (h + 1) * (h + 2) / 2;
+        m_gamma = 0;
+        m_x = m_num_k_stars - h - 1;
+        m_m = m_num_k_stars - 1;
+        m_initial_status = true;
+    }
+    int GammaPointIndex() const { return m_gamma; }
+    int XPointIndex() const { return m_x; }
+    int MPointIndex() const { return m_m; }
+    int NearestNeighbour(int i, int dir) const {
+        const int L = m_side_length, x = i % L, y = i / L;
+        switch (dir) {
+            case 0: return (x + 1) % L + L * y;
+            case 1: return x + L * ((y + 1) % L);
+            case 2: return (x - 1 + L) % L + L * y;
+            default: return x + L * ((y - 1 + L) % L);
+        }
+    }
+    int Displacement(int i, int j) const {
+        const int L = m_side_length;
+        return (j % L - i % L + L) % L + L * ((j / L - i / L + L) % L);
+    }
+};
+}  // namespace Lattice
+
+namespace QuantumMonteCarlo {
+class DqmcWalker;
+class DqmcInitializer;
+}  // namespace QuantumMonteCarlo
+
+namespace CheckerBoard {
+class CheckerBoardBase {
+   public:
+    virtual ~CheckerBoardBase() = default;
+};
+// The break-up itself runs on the device; this object records that the model
+// is linked to it (src/model/model_base.cpp:48-55).
+class Square : public CheckerBoardBase {
+   public:
+    template <class L, class M, class W>
+    void set_checkerboard_params(const L& lattice, const M&, const W&) {
+        if (lattice.SideLength() % 2 != 0) throw std::invalid_argument("checkerboard break-up needs an even side length");
+    }
+    void initial() {}
+};
+}  // namespace CheckerBoard
+
+namespace Model {
+class ModelBase {
+   protected:
+    int m_model_type{DQMC_MODEL_REPULSIVE};
+    double m_hopping_t{1.0}, m_onsite_u{4.0}, m_chemical_potential{0.0};
+    bool m_checkerboard{false};
+    int m_space_size{}, m_time_size{};
+    std::vector<int8_t> m_bosonic_field;  // (t, i) at t * N + i
+    friend class QuantumMonteCarlo::DqmcWalker;
+    friend class QuantumMonteCarlo::DqmcInitializer;
+
+   public:
+    virtual ~ModelBase() = default;
+    virtual void set_model_params(double hopping_t, double onsite_u, double chemical_potential) {
+        if (hopping_t < 0.0 || onsite_u < 0.0) throw std::invalid_argument("hopping_t and onsite_u must be >= 0");
+        m_hopping_t = hopping_t;
+        m_onsite_u = onsite_u;
+        m_chemical_potential = chemical_potential;
+    }
+    double HoppingT() const { return m_hopping_t; }
+    double ChemicalPotential() const { return m_chemical_potential; }
+    double OnSiteU() const { return m_onsite_u; }
+    void link() { m_checkerboard = false; }
+    void link(const CheckerBoard::CheckerBoardBase&) { m_checkerboard = true; }
+    // src/model/repulsive_hubbard.cpp:86-99: t-outer, i-inner, bernoulli(0.5) on the global engine
+    void set_bosonic_fields_to_random() {
+        std::bernoulli_distribution b(0.5);
+        for (auto& f : m_bosonic_field) f = b(Utils::Random::Engine()) ? +1 : -1;
+    }
+    const std::vector<int8_t>& BosonicFields() const { return m_bosonic_field; }
+};
+class RepulsiveHubbard : public ModelBase {
+   public:
+    RepulsiveHubbard() { m_model_type = DQMC_MODEL_REPULSIVE; }
+};
+class AttractiveHubbard : public ModelBase {
+   public:
+    AttractiveHubbard() { m_model_type = DQMC_MODEL_ATTRACTIVE; }
+};
+}  // namespace Model
+
+namespace Measure {
+class MeasureHandler;
+}
+
+namespace QuantumMonteCarlo {
+
+// The walker owns one chain of the engine (a batch of one) once initialized.
+class DqmcWalker {
+    int m_space_size{}, m_time_size{}, m_stabilization_pace{10}, m_delay{16};
+    double m_beta{}, m_time_interval{};
+    bool m_is_equaltime{false};
+    dqmc_batch_t m_batch{nullptr};
+    friend class DqmcInitializer;
+    friend class Measure::MeasureHandler;
+
+    std::vector<double> fetch(int slot, int spin) const {
+        std::vector<double> g((size_t)m_space_size * m_space_size);
+        const int rc = slot < 0 ? dqmc_get_green(m_batch, 0, spin, g.data())
+                                : dqmc_get_slice_green(m_batch, 0, slot, spin, g.data());
+        dqmc_detail::check(rc, dqmc_detail::context(), "DqmcWalker::Greentt");
+        return g;
+    }
+    void sweep_once() {
+        dqmc_detail::check(dqmc_sweep(m_batch, 1, m_is_equaltime ? DQMC_SWEEP_RECORD : 0), dqmc_detail::context(),
+                           "DqmcWalker::sweep");
+    }
+
+   public:
+    ~DqmcWalker() {
+        if (m_batch) dqmc_batch_destroy(m_batch);
+    }
+    void set_physical_params(double beta, int time_size) {
+        if (beta <= 0.0 || time_size < 1) throw std::invalid_argument("beta > 0 and time_size >= 1");
+        m_beta = beta;
+        m_time_size = time_size;
+        m_time_interval = beta / time_size;
+    }
+    void set_stabilization_pace(int p) {
+        if (p < 1) throw std::invalid_argument("stabilization pace must be >= 1");
+        m_stabilization_pace = p;
+    }
+    // delayed-update depth of the engine (no counterpart in the reference)
+    void set_delay(int k) { m_delay = k; }
+    int TimeSize() const { return m_time_size; }
+    double Beta() const { return m_beta; }
+    double TimeInterval() const { return m_time_interval; }
+    int StabilizationPace() const { return m_stabilization_pace; }
+
+    void sweep_from_0_to_beta(Model::ModelBase&) {
+        dqmc_chain_info i{};
+        dqmc_get_info(m_batch, 0, &i);
+        if (i.pos != 0) throw std::logic_error("sweep_from_0_to_beta: walker is not at slice 0");
+        sweep_once();
+    }
+    void sweep_from_beta_to_0(Model::ModelBase&) {
+        dqmc_chain_info i{};
+        dqmc_get_info(m_batch, 0, &i);
+        if (i.pos != m_time_size) throw std::logic_error("sweep_from_beta_to_0: walker is not at slice beta");
+        sweep_once();
+    }
+    // accessors (include/dqmc_walker.h:136-164); matrices are copied from the device
+    std::vector<double> GreenttUp() const { return fetch(-1, +1); }
+    std::vector<double> GreenttDn() const { return fetch(-1, -1); }
+    std::vector<double> GreenttUp(int t) const { return fetch(t, +1); }
+    std::vector<double> GreenttDn(int t) const { return fetch(t, -1); }
+    double ConfigSign() const {
+        dqmc_chain_info i{};
+        dqmc_get_info(m_batch, 0, &i);
+        return i.sign;
+    }
+    std::vector<double> vecConfigSign() const {
+        std::vector<double> s(m_time_size);
+        dqmc_detail::check(dqmc_get_slice_signs(m_batch, 0, s.data()), dqmc_detail::context(), "vecConfigSign");
+        return s;
+    }
+    double WrapError() const {
+        dqmc_chain_info i{};
+        dqmc_get_info(m_batch, 0, &i);
+        return i.wrap_error;
+    }
+};
+}  // namespace QuantumMonteCarlo
+
+namespace Measure {
+using ObsList = std::vector<std::string>;
+
+// names of the equal-time observables (src/measure/observable_handler.cpp:11-20) in engine order
+inline const std::vector<std::string>& EqualTimeNames() {
+    static const std::vector<std::string> n = {"equaltime_sign", "filling_number", "double_occupancy",
+                                               "kinetic_energy", "momentum_distribution", "local_spin_corr",
+                                               "spin_density_structure_factor", "charge_density_structure_factor",
+                                               "s_wave_pairing_corr"};
+    return n;
+}
+
+class MeasureHandler {
+    int m_sweeps_warmup{}, m_bin_num{}, m_bin_size{}, m_sweeps_between_bins{};
+    ObsList m_obs_list{};
+    int m_momentum{-1};
+    bool m_is_equaltime{false};
+    dqmc_batch_t m_batch{nullptr};
+    std::vector<std::vector<double>> m_bins;
+    std::vector<double> m_mean, m_err;
+    friend class QuantumMonteCarlo::DqmcInitializer;
+
+   public:
+    void set_measure_params(int sweeps_warmup, int bin_num, int bin_size, int sweeps_between_bins) {
+        if (sweeps_warmup < 0 || bin_num < 0 || bin_size < 0) throw std::invalid_argument("negative measuring params");
+        m_sweeps_warmup = sweeps_warmup;
+        m_bin_num = bin_num;
+        m_bin_size = bin_size;
+        m_sweeps_between_bins = sweeps_between_bins;
+    }
+    void set_observables(ObsList obs) {
+        for (const auto& o : obs) {
+            bool known = false;
+            for (const auto& n : EqualTimeNames()) known = known || (o == n);
+            if (!known) throw std::invalid_argument("unsupported observable '" + o + "' (equal-time only)");
+        }
+        m_obs_list = std::move(obs);
+        m_is_equaltime = !m_obs_list.empty();
+    }
+    void set_measured_momentum(int momentum_index) { m_momentum = momentum_index; }
+    bool isWarmUp() const { return m_sweeps_warmup != 0; }
+    bool isEqualTime() const { return m_is_equaltime; }
+    bool isDynamic() const { return false; }
+    int WarmUpSweeps() const { return m_sweeps_warmup; }
+    int SweepsBetweenBins() const { return m_sweeps_between_bins; }
+    int BinsNum() const { return m_bin_num; }
+    int BinsSize() const { return m_bin_size; }
+    int Momentum() const { return m_momentum; }
+
+    // src/measure/measure_handler.cpp:136-150: every method over the L stored slices
+    void equaltime_measure(const QuantumMonteCarlo::DqmcWalker&, const Model::ModelBase&, const Lattice::LatticeBase&) {
+        dqmc_detail::check(dqmc_measure_recorded(m_batch), dqmc_detail::context(), "equaltime_measure");
+    }
+    // :170-240 — normalize by counts and <sign>, keep |<sign>|, store the bin
+    void normalize_stats() {
+        double sums[DQMC_NUM_OBSERVABLES], counts[DQMC_NUM_OBSERVABLES];
+        dqmc_detail::check(dqmc_get_measure(m_batch, 0, sums, counts), dqmc_detail::context(), "normalize_stats");
+        std::vector<double> v(DQMC_NUM_OBSERVABLES);
+        const double sg = sums[0] / counts[0];
+        for (int o = 1; o < DQMC_NUM_OBSERVABLES; ++o) v[o] = sums[o] / (counts[o] * sg);
+        v[0] = std::fabs(sg);
+        m_bins.push_back(v);
+    }
+    void write_stats_to_bins(int) {}
+    void clear_temporary() { dqmc_detail::check(dqmc_clear_measure(m_batch), dqmc_detail::context(), "clear_temporary"); }
+    // include/measure/observable.h:169-193
+    void analyse_stats() {
+        const size_t nb = m_bins.size();
+        m_mean.assign(DQMC_NUM_OBSERVABLES, 0.0);
+        m_err.assign(DQMC_NUM_OBSERVABLES, 0.0);
+        for (int o = 0; o < DQMC_NUM_OBSERVABLES; ++o) {
+            double m = 0.0, m2 = 0.0;
+            for (auto& b : m_bins) { m += b[o]; m2 += b[o] * b[o]; }
+            m /= nb; m2 /= nb;
+            m_mean[o] = m;
+            m_err[o] = nb > 1 ? std::sqrt(std::max(0.0, m2 - m * m)) / std::sqrt((double)nb - 1) : 0.0;
+        }
+    }
+    double mean(const std::string& name) const { return m_mean.at(index(name)); }
+    double error(const std::string& name) const { return m_err.at(index(name)); }
+    static int index(const std::string& name) {
+        const auto& n = EqualTimeNames();
+        for (size_t k = 0; k < n.size(); ++k)
+            if (n[k] == name) return (int)k;
+        throw std::invalid_argument("unknown observable " + name);
+    }
+};
+}  // namespace Measure
+
+namespace QuantumMonteCarlo {
+
+class DqmcInitializer {
+   public:
+    // src/dqmc_initializer.cpp:341-392
+    static void initial_modules(Lattice::LatticeBase& lattice, Model::ModelBase& model, DqmcWalker& walker,
+                                Measure::MeasureHandler& meas, CheckerBoard::CheckerBoardBase* checkerboard = nullptr) {
+        if (!lattice.InitialStatus()) lattice.initial();
+        walker.m_space_size = lattice.SpaceSize();
+        walker.m_is_equaltime = meas.isEqualTime();
+        model.m_space_size = lattice.SpaceSize();
+        model.m_time_size = walker.TimeSize();
+        model.m_bosonic_field.assign((size_t)walker.TimeSize() * lattice.SpaceSize(), 1);
+        if (checkerboard) model.link(*checkerboard); else model.link();
+    }
+    // src/dqmc_initializer.cpp:395-408: stacks, G(0), sign — the chain moves to the device here
+    static void initial_dqmc(Lattice::LatticeBase& lattice, Model::ModelBase& model, DqmcWalker& walker,
+                             Measure::MeasureHandler& meas) {
+        dqmc_ctx_t ctx = dqmc_detail::context();
+        dqmc_chain_params p{};
+        p.model = model.m_model_type;
+        p.side = lattice.SideLength();
+        p.time_size = walker.TimeSize();
+        p.stabilization_pace = walker.StabilizationPace();
+        p.checkerboard = model.m_checkerboard ? 1 : 0;
+        p.delay = walker.m_delay;
+        p.momentum_index = meas.Momentum();
+        p.precision = DQMC_PREC_FP64;
+        p.beta = walker.Beta();
+        p.hopping_t = model.HoppingT();
+        p.onsite_u = model.OnSiteU();
+        p.chemical_potential = model.ChemicalPotential();
+        p.seed = 1;
+        if (walker.m_batch) dqmc_batch_destroy(walker.m_batch);
+        walker.m_batch = nullptr;
+        dqmc_detail::check(dqmc_batch_create(ctx, 1, &p, walker.m_is_equaltime ? 1 : 0, &walker.m_batch), ctx,
+                           "dqmc_batch_create");
+        dqmc_detail::check(dqmc_set_fields(walker.m_batch, 0, model.m_bosonic_field.data()), ctx, "dqmc_set_fields");
+        // the device continues the global engine's stream (Metropolis draws follow the field draws)
+        dqmc_detail::check(dqmc_set_rng_state(walker.m_batch, 0, Utils::Random::state()), ctx, "dqmc_set_rng_state");
+        dqmc_detail::check(dqmc_init(walker.m_batch), ctx, "dqmc_init");
+        meas.m_batch = walker.m_batch;
+    }
+};
+
+class Dqmc {
+   public:
+    // src/dqmc.cpp:48-70
+    static void sweep_forth_and_back(DqmcWalker& walker, Model::ModelBase& model, Lattice::LatticeBase& lattice,
+                                     Measure::MeasureHandler& meas) {
+        walker.sweep_from_0_to_beta(model);
+        if (meas.isEqualTime()) meas.equaltime_measure(walker, model, lattice);
+        walker.sweep_from_beta_to_0(model);
+        if (meas.isEqualTime()) meas.equaltime_measure(walker, model, lattice);
+    }
+    // src/dqmc.cpp:73-105
+    static void thermalize(DqmcWalker& walker, Model::ModelBase& model, Lattice::LatticeBase&,
+                           Measure::MeasureHandler& meas) {
+        for (int s = 1; s <= meas.WarmUpSweeps() / 2; ++s) {
+            walker.sweep_from_0_to_beta(model);
+            walker.sweep_from_beta_to_0(model);
+        }
+    }
+    // src/dqmc.cpp:108-151
+    static void measure(DqmcWalker& walker, Model::ModelBase& model, Lattice::LatticeBase& lattice,
+                        Measure::MeasureHandler& meas) {
+        if (!meas.isEqualTime()) return;
+        for (int bin = 0; bin < meas.BinsNum(); ++bin) {
+            for (int s = 1; s <= meas.BinsSize() / 2; ++s) sweep_forth_and_back(walker, model, lattice, meas);
+            meas.normalize_stats();
+            meas.write_stats_to_bins(bin);
+            meas.clear_temporary();
+            for (int s = 0; s < meas.SweepsBetweenBins() / 2; ++s) {
+                walker.sweep_from_0_to_beta(model);
+                walker.sweep_from_beta_to_0(model);
+            }
+        }
+    }
+    static void analyse(Measure::MeasureHandler& meas) { meas.analyse_stats(); }
+};
+
+}  // namespace QuantumMonteCarlo

@@ -1,0 +1,65 @@
+// A driver written the way the reference's main drives its modules
+// (src/dqmc_main.cpp:110-200: set_seed, initial_modules, random fields,
+// initial_dqmc, thermalize, measure, analyse), on the MI355X engine through
+// the C++ facade.  Prints one JSON line with the equal-time observables.
+//
+//   reference_flow side time_size beta pace model(0|1) checkerboard(0|1) U mu seed warmup bins bin_size between
+#include <cstdio>
+#include <cstdlib>
+#include <memory>
+
+#include "dqmc_framework.hpp"
+
+int main(int argc, char** argv) {
+    if (argc != 14) {
+        std::fprintf(stderr, "usage: %s side time_size beta pace model cb U mu seed warmup bins bin_size between\n", argv[0]);
+        return 2;
+    }
+    const int side = std::atoi(argv[1]), time_size = std::atoi(argv[2]);
+    const double beta = std::atof(argv[3]);
+    const int pace = std::atoi(argv[4]), model_type = std::atoi(argv[5]), cb = std::atoi(argv[6]);
+    const double U = std::atof(argv[7]), mu = std::atof(argv[8]);
+    const int seed = std::atoi(argv[9]), warmup = std::atoi(argv[10]), bins = std::atoi(argv[11]);
+    const int bin_size = std::atoi(argv[12]), between = std::atoi(argv[13]);
+
+    Utils::Random::set_seed(seed);
+
+    Lattice::Square lattice;
+    lattice.set_lattice_params({side, side});
+    lattice.initial();
+
+    std::unique_ptr<Model::ModelBase> model;
+    if (model_type == 1) model = std::make_unique<Model::AttractiveHubbard>();
+    else model = std::make_unique<Model::RepulsiveHubbard>();
+    model->set_model_params(1.0, U, mu);
+
+    QuantumMonteCarlo::DqmcWalker walker;
+    walker.set_physical_params(beta, time_size);
+    walker.set_stabilization_pace(pace);
+
+    Measure::MeasureHandler meas;
+    meas.set_measure_params(warmup, bins, bin_size, between);
+    meas.set_observables({"filling_number", "double_occupancy", "kinetic_energy", "momentum_distribution",
+                          "local_spin_corr", "spin_density_structure_factor", "charge_density_structure_factor",
+                          "s_wave_pairing_corr"});
+    meas.set_measured_momentum(lattice.MPointIndex());
+
+    CheckerBoard::Square checkerboard;
+    if (cb) checkerboard.set_checkerboard_params(lattice, *model, walker);
+    QuantumMonteCarlo::DqmcInitializer::initial_modules(lattice, *model, walker, meas, cb ? &checkerboard : nullptr);
+    model->set_bosonic_fields_to_random();
+    QuantumMonteCarlo::DqmcInitializer::initial_dqmc(lattice, *model, walker, meas);
+
+    QuantumMonteCarlo::Dqmc::thermalize(walker, *model, lattice, meas);
+    QuantumMonteCarlo::Dqmc::measure(walker, *model, lattice, meas);
+    QuantumMonteCarlo::Dqmc::analyse(meas);
+
+    const auto g = walker.GreenttUp();
+    double tr = 0.0;
+    for (int i = 0; i < side * side; ++i) tr += g[(size_t)i * side * side + i];
+    std::printf("{\"wrap_error\": %.6e, \"sign\": %.1f, \"trace_g_up\": %.15f", walker.WrapError(), walker.ConfigSign(), tr);
+    for (const auto& name : Measure::EqualTimeNames())
+        std::printf(", \"%s\": [%.15f, %.15f]", name.c_str(), meas.mean(name), meas.error(name));
+    std::printf("}\n");
+    return 0;
+}

@@ -61,6 +61,7 @@ extern "C" {
 #define DQMC_SWEEP_RECORD 2  /* keep per-slice G, sign and accept bits of the last sweep */
 #define DQMC_SWEEP_FROZEN 4  /* no field updates: propagate, stabilize and measure a fixed
                                 configuration (as the reference's sweep_for_dynamic_greens does) */
+#define DQMC_SWEEP_PROFILE 8 /* accumulate device time per sweep phase (dqmc_chain_info.phase_seconds) */
 
 /* equal-time observables, index order of dqmc_get_measure */
 #define DQMC_OBS_SIGN 0
@@ -104,6 +105,8 @@ typedef struct {
     int status;              /* 0 ok, else numerical failure */
     long long accepted;      /* accepted flips in the last sweep launch */
     long long sweeps;        /* single-direction sweeps done since init */
+    double phase_seconds[6]; /* wrap, Metropolis, slice product, stack push, fresh G, measurement
+                                (accumulated over DQMC_SWEEP_PROFILE sweeps since init) */
 } dqmc_chain_info;
 
 /* context: device selection, error string */
@@ -118,6 +121,9 @@ int dqmc_batch_footprint(int n_chains, const dqmc_chain_params* params, int reco
 int dqmc_batch_create(dqmc_ctx_t ctx, int n_chains, const dqmc_chain_params* params, int record, dqmc_batch_t* out);
 int dqmc_batch_destroy(dqmc_batch_t b);
 int dqmc_batch_size(dqmc_batch_t b);
+/* replay the persistent sweep launch from a captured HIP graph (re-captured
+ * when n_sweeps or flags change); no counterpart in the reference */
+int dqmc_batch_use_graph(dqmc_batch_t b, int enable);
 
 /* fields: random (seed, reference draw order), or explicit */
 int dqmc_fields_random(dqmc_batch_t b);
@@ -149,7 +155,17 @@ int dqmc_get_info(dqmc_batch_t b, int chain, dqmc_chain_info* out);
 /* equal-time accumulators: sums and counts per observable (normalize per bin as
  * MeasureHandler::normalize_stats, src/measure/measure_handler.cpp:170-189) */
 int dqmc_get_measure(dqmc_batch_t b, int chain, double* sums, double* counts);
+/* MeasureHandler::equaltime_measure (src/measure/measure_handler.cpp:136-150) on the
+ * slices kept by the last DQMC_SWEEP_RECORD sweep (batch created with record = 1) */
+int dqmc_measure_recorded(dqmc_batch_t b);
 int dqmc_clear_measure(dqmc_batch_t b);
+
+/* Delayed-update depth autotuner (no counterpart in the reference, which
+ * applies every flip at once): times `sweeps` sweeps of `n_chains` chains of
+ * this shape for each candidate depth and returns the fastest in *best
+ * (ms_out[c] = device time of candidate c, may be NULL). */
+int dqmc_autotune_delay(dqmc_ctx_t ctx, const dqmc_chain_params* shape, int n_chains, const int* candidates,
+                        int n_candidates, int sweeps, int* best, float* ms_out);
 
 /* ------------------------------------------------------------------------
  * Serving: continuous batching of DQMC jobs.  No counterpart in the
