@@ -35,6 +35,10 @@ WORKLOADS = {
     "plain-4x4": dict(chains=1, side=4, time_size=80, beta=4.0, pace=10, model=D.REPULSIVE,
                       checkerboard=False, U=4.0, mu=0.0, delay=16,
                       desc="1 chain 4x4 repulsive Hubbard t=1 U=4 mu=0 beta=4 L=80 p=10 dense FP64"),
+    "mixed-16x16": dict(chains=64, side=16, time_size=200, beta=10.0, pace=8, model=D.REPULSIVE,
+                        checkerboard=False, U=6.0, mu=0.0, delay=16, precision=D.PREC_MIXED,
+                        desc="64 chains x 16x16 repulsive Hubbard t=1 U=6 beta=10 L=200 p=8 dense, FP32 MFMA "
+                             "products/wraps + FP64 stacks"),
 }
 
 PEAK_FP64_TFLOPS = 78.6
@@ -43,8 +47,10 @@ PEAK_HBM_GBS = 8000.0
 
 def chain_params(w, chains=None, seed0=0):
     n = w["chains"] if chains is None else chains
+    mus = w.get("mus", [w["mu"]])
     return [D.make_params(w["side"], w["time_size"], w["beta"], pace=w["pace"], model=w["model"],
-                          checkerboard=w["checkerboard"], delay=w["delay"], U=w["U"], mu=w["mu"], seed=seed0 + c)
+                          checkerboard=w["checkerboard"], delay=w["delay"], U=w["U"], mu=mus[c * len(mus) // n],
+                          seed=seed0 + c, precision=w.get("precision", D.PREC_FP64))
             for c in range(n)]
 
 
@@ -151,6 +157,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the single-chain comparison run")
     ap.add_argument("--phases", action="store_true", help="also report device time per sweep phase")
+    ap.add_argument("--serve-jobs", type=int, default=48, help="jobs of the serving sample (0 skips it)")
+    ap.add_argument("--serve-scale", type=float, default=0.05, help="sweep-count scale of the serving sample")
+    ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrivals per second")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: chains are independent and would be sharded by chain across GPUs, "
@@ -168,7 +177,7 @@ def main():
     out = {
         "metric": METRIC, "value": value, "unit": "chain-sweeps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f32+f64" if w.get("precision") == D.PREC_MIXED else "f64", "data": "synthetic",
         "config": {"workload": w["desc"], "chains": r["chains"], "side": w["side"], "time_size": w["time_size"],
                    "beta": w["beta"], "pace": w["pace"], "checkerboard": w["checkerboard"], "delay": w["delay"]},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
@@ -186,6 +195,16 @@ def main():
         print("[bench] phase breakdown", file=sys.stderr, flush=True)
         out["phases_ms_per_chain_sweep"] = {"batched": phase_breakdown(w, r["chains"]),
                                             "single": phase_breakdown(w, 1)}
+    if args.serve_jobs > 0:
+        import serve
+        print(f"[bench] serving sample: {args.serve_jobs} jobs at {args.serve_rate}/s", file=sys.stderr, flush=True)
+        jobs = serve.generate_jobs(args.serve_jobs, scale=args.serve_scale)
+        ctx = D.Context(0)
+        summary, _ = serve.run(ctx, jobs, args.serve_rate, 64 << 30, log=sys.stderr)
+        ctx.close()
+        summary["job_mix"] = (f"side in {{4..20}}, beta U[2,12], dtau in {{0.05,0.1}}, U U[-8,8], mu U[-1,1], "
+                              f"sweeps U[50,500] x {args.serve_scale}, seed = job id, mixed precision, checkerboard")
+        out["serving"] = summary
     if not args.no_cpu:
         print("[bench] CPU baseline (oracle, one chain per process)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(w)

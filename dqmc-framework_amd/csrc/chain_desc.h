@@ -54,6 +54,7 @@ struct ChainDesc {
     double* stkT;        // nspin x n_stack x N*N   (cumulative V^T of each level)
     double* stkD;        // nspin x n_stack x N
     double* W;           // 3 x N*N scratch for the stabilization
+    double* Wq;          // 2 x 16 x N scratch of the blocked QR (GEMM panel and T factors)
     double* X;           // nspin x N*kdelay   delayed-update columns
     double* Y;           // nspin x N*kdelay   delayed-update rows
     const double* expK;  // N*N (dense only)

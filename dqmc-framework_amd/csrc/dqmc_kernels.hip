@@ -31,6 +31,9 @@ namespace dqmc {
 // ---------------------------------------------------------------------------
 // LDS layout of one workgroup
 // ---------------------------------------------------------------------------
+#define NB 16                    // panel width of the blocked LU and QR
+#define PANEL_DOUBLES 12288      // 96 KB: an LU/QR panel (up to NMAX x NB) plus its NB x NB blocks
+
 struct Lds {
     double As[16][64];         // GEMM A tile (k-major)
     double Bs[16][64];         // GEMM B tile (k-major)
@@ -39,6 +42,7 @@ struct Lds {
     double v0[NMAX], v1[NMAX], v2[NMAX], v3[NMAX];  // stabilization / QR scratch vectors
     int perm[NMAX];
     int frow[NMAX];            // field row of the current slice
+    double panel[PANEL_DOUBLES];  // LU / QR panel, or the delayed-update panels X, Y
     double red[16];
     double gam[2];
     int ivar[4];
@@ -199,7 +203,7 @@ typedef float flt4 __attribute__((ext_vector_type(4)));
 template <bool lowp>
 __device__ void wg_gemm_mfma(int M, int Nc, int K, const double* A, int lda, bool tA, const double* B, int ldb, bool tB,
                              double* C, int ldc, const double* rs, const double* ms, const double* cs, double beta,
-                             Lds& s) {
+                             Lds& s, double alpha = 1.0) {
     const int l = lane(), w = wave();
     const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
     const int tm = (M + 63) >> 6, tn = (Nc + 63) >> 6;
@@ -267,7 +271,7 @@ __device__ void wg_gemm_mfma(int M, int Nc, int K, const double* A, int lda, boo
                     const int row = r0 + wr + rb * 16 + (lowp ? 4 * (l >> 4) + i : (l >> 4) + 4 * i);
                     const int col = c0 + wc + cb * 16 + (l & 15);
                     if (row >= M || col >= Nc) continue;
-                    double v = lowp ? (double)accf[rb][cb][i] : acc[rb][cb][i];
+                    double v = (lowp ? (double)accf[rb][cb][i] : acc[rb][cb][i]) * alpha;
                     if (cs) v *= cs[col];
                     if (rs) v *= rs[row];
                     double* cp = C + (size_t)col * ldc + row;
@@ -282,8 +286,8 @@ __device__ void wg_gemm_mfma(int M, int Nc, int K, const double* A, int lda, boo
 // chain runs in mixed precision.
 __device__ __forceinline__ void gemm64(int M, int Nc, int K, const double* A, int lda, bool tA, const double* B,
                                        int ldb, bool tB, double* C, int ldc, const double* rs, const double* ms,
-                                       const double* cs, double beta, Lds& s) {
-    wg_gemm_mfma<false>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, s);
+                                       const double* cs, double beta, Lds& s, double alpha = 1.0) {
+    wg_gemm_mfma<false>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, s, alpha);
 }
 __device__ __forceinline__ void gemmB(const ChainDesc& d, int M, int Nc, int K, const double* A, int lda, bool tA,
                                       const double* B, int ldb, bool tB, double* C, int ldc, const double* rs,
@@ -426,187 +430,297 @@ __device__ void wg_wrap(const ChainDesc& d, int sp, bool up, double* G, double* 
 }
 
 // ---------------------------------------------------------------------------
-// Gaussian elimination with partial pivoting: A X = B, X overwrites B (N x nrhs).
-// A is destroyed.  Returns the sign of det(A) (for nrhs = 0 too).
+// Blocked LU with partial pivoting: A X = B, X overwrites B (N x nrhs); A is
+// destroyed.  Panels of NB columns are factored in LDS (pivot search, swaps
+// and rank-1 updates never leave the CU); the row interchanges and L11^-1 are
+// applied to the columns right of the panel and to B one column per thread;
+// the trailing matrix and B are updated with one FP64 MFMA GEMM each
+// (right-looking, Golub & Van Loan Alg. 3.4.2).  Back substitution runs by
+// NB-row blocks with a GEMM for the rows above.  Returns the sign of det(A).
 // ---------------------------------------------------------------------------
 __device__ double wg_lu_solve(double* A, double* B, int N, int nrhs, Lds& s) {
     double detsign = 1.0;
-    for (int k = 0; k < N; ++k) {
-        if (wave() == 0) {
-            double best = -1.0; int bi = k;
-            for (int i = k + lane(); i < N; i += 64) {
-                const double v = fabs(A[(size_t)k * N + i]);
-                if (v > best || (v == best && i < bi)) { best = v; bi = i; }
-            }
-            wave_argmax(best, bi);
-            if (lane() == 0) s.ivar[0] = bi;
+    double* P = s.panel;                      // m x jb, ld = m
+    double* T11 = s.panel + NMAX * NB;        // NB x NB
+    for (int k0 = 0; k0 < N; k0 += NB) {
+        const int jb = min(NB, N - k0), m = N - k0;
+        for (int e = tid(); e < m * jb; e += NT) {
+            const int r = e % m, c = e / m;
+            P[c * m + r] = A[(size_t)(k0 + c) * N + k0 + r];
         }
         __syncthreads();
-        const int p = s.ivar[0];
-        if (p != k) {
-            for (int c = tid(); c < N + nrhs; c += NT) {
-                double* col = c < N ? A + (size_t)c * N : B + (size_t)(c - N) * N;
-                if (c < k) continue;  // left part of A is no longer read
-                const double t = col[k]; col[k] = col[p]; col[p] = t;
+        for (int j = 0; j < jb; ++j) {
+            if (wave() == 0) {
+                double best = -1.0; int bi = j;
+                for (int r = j + lane(); r < m; r += 64) {
+                    const double v = fabs(P[j * m + r]);
+                    if (v > best || (v == best && r < bi)) { best = v; bi = r; }
+                }
+                wave_argmax(best, bi);
+                if (lane() == 0) { s.ivar[0] = bi; s.perm[k0 + j] = k0 + bi; }
             }
-            detsign = -detsign;
+            __syncthreads();
+            const int p = s.ivar[0];
+            if (p != j) {
+                for (int c = tid(); c < jb; c += NT) {
+                    const double t = P[c * m + j]; P[c * m + j] = P[c * m + p]; P[c * m + p] = t;
+                }
+                detsign = -detsign;
+                __syncthreads();
+            }
+            const double piv = P[j * m + j];
+            if (piv < 0) detsign = -detsign;
+            const int nr = m - j - 1, nc = jb - j - 1;
+            for (int e = tid(); e < nr * nc; e += NT) {
+                const int r = j + 1 + e % nr, c = j + 1 + e / nr;
+                P[c * m + r] -= (P[j * m + r] / piv) * P[c * m + j];
+            }
+            __syncthreads();
+            for (int r = j + 1 + tid(); r < m; r += NT) P[j * m + r] /= piv;
             __syncthreads();
         }
-        const double piv = A[(size_t)k * N + k];
-        if (piv < 0) detsign = -detsign;
-        const int nr = N - k - 1;
-        if (nr > 0) {
-            const int ncols = (N - k - 1) + nrhs;
-            for (int e = tid(); e < nr * ncols; e += NT) {
-                const int i = k + 1 + e % nr, c = e / nr;
-                const double l = A[(size_t)k * N + i] / piv;
-                double* col = c < (N - k - 1) ? A + (size_t)(k + 1 + c) * N : B + (size_t)(c - (N - k - 1)) * N;
-                col[i] -= l * col[k];
+        for (int e = tid(); e < m * jb; e += NT) {
+            const int r = e % m, c = e / m;
+            A[(size_t)(k0 + c) * N + k0 + r] = P[c * m + r];
+        }
+        // interchanges + L11^-1 on the columns right of the panel and on B
+        const int nright = N - k0 - jb;
+        for (int c = tid(); c < nright + nrhs; c += NT) {
+            double* col = c < nright ? A + (size_t)(k0 + jb + c) * N : B + (size_t)(c - nright) * N;
+            for (int j = 0; j < jb; ++j) {
+                const int pr = s.perm[k0 + j];
+                if (pr != k0 + j) { const double t = col[k0 + j]; col[k0 + j] = col[pr]; col[pr] = t; }
             }
+            double x[NB];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) x[j] = (j < jb) ? col[k0 + j] : 0.0;
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+#pragma unroll
+                for (int i = j + 1; i < NB; ++i)
+                    if (i < jb) x[i] -= P[j * m + i] * x[j];
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+                if (j < jb) col[k0 + j] = x[j];
+        }
+        __syncthreads();
+        if (jb < m) {
+            if (nright > 0)
+                gemm64(m - jb, nright, jb, P + jb, m, false, A + (size_t)(k0 + jb) * N + k0, N, false,
+                       A + (size_t)(k0 + jb) * N + k0 + jb, N, nullptr, nullptr, nullptr, 1.0, s, -1.0);
+            if (nrhs > 0)
+                gemm64(m - jb, nrhs, jb, P + jb, m, false, B + k0, N, false, B + k0 + jb, N, nullptr, nullptr, nullptr,
+                       1.0, s, -1.0);
         }
         __syncthreads();
     }
-    for (int k = N - 1; k >= 0 && nrhs > 0; --k) {
-        const double piv = A[(size_t)k * N + k];
-        for (int c = tid(); c < nrhs; c += NT) B[(size_t)c * N + k] /= piv;
-        __syncthreads();
-        for (int e = tid(); e < k * nrhs; e += NT) {
-            const int i = e % k, c = e / k;
-            B[(size_t)c * N + i] -= A[(size_t)k * N + i] * B[(size_t)c * N + k];
+    if (nrhs == 0) return detsign;
+    for (int k0 = ((N - 1) / NB) * NB; k0 >= 0; k0 -= NB) {
+        const int jb = min(NB, N - k0);
+        for (int e = tid(); e < jb * jb; e += NT) {
+            const int r = e % jb, c = e / jb;
+            T11[c * NB + r] = A[(size_t)(k0 + c) * N + k0 + r];
         }
         __syncthreads();
+        for (int c = tid(); c < nrhs; c += NT) {
+            double* col = B + (size_t)c * N;
+            double x[NB];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) x[j] = (j < jb) ? col[k0 + j] : 0.0;
+#pragma unroll
+            for (int j = NB - 1; j >= 0; --j) {
+                if (j < jb) {
+                    x[j] /= T11[j * NB + j];
+#pragma unroll
+                    for (int i = 0; i < j; ++i) x[i] -= T11[j * NB + i] * x[j];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+                if (j < jb) col[k0 + j] = x[j];
+        }
+        __syncthreads();
+        if (k0 > 0)
+            gemm64(k0, nrhs, jb, A + (size_t)k0 * N, N, false, B + k0, N, false, B, N, nullptr, nullptr, nullptr, 1.0, s,
+                   -1.0);
     }
     return detsign;
 }
 
 // ---------------------------------------------------------------------------
-// Householder QR with column pivoting of M (N x N, destroyed):
-//   M P = Q R,  U = Q (explicit), D = |diag R|, Tl = D^-1 R P^T  (dense N x N)
-// (LAPACK dgeqp3-style norm downdating with recomputation on cancellation.)
+// UDV by pre-pivoted blocked Householder QR of M (N x N):
+//   columns sorted once by decreasing norm (P), then M P = Q R without
+//   further pivoting (Bauer, SciPost Phys. Core 2, 011 (2020));
+//   U = Q (explicit), D = |diag R|, Tl = D^-1 R P^T (dense N x N).
+// Panels of NB columns are factored in LDS; the block reflector
+// I - V T V^T (LAPACK dlarft, forward/columnwise) updates the trailing
+// columns and forms Q with FP64 MFMA GEMMs.
+// Buffers: M is overwritten by the reflectors V (unit lower, explicit),
+// Mp (N x N) receives M P and then R, Wq holds NB x N of GEMM scratch and
+// the NB x NB T factor of every panel.
 // ---------------------------------------------------------------------------
-__device__ void wg_qrp_udv(double* M, int N, double* U, double* D, double* Tl, Lds& s) {
-    double* vn1 = s.v0;  // partial column norms
-    double* vn2 = s.v1;  // reference norms for the cancellation test
-    double* tau = s.v2;
-    const double tol3z = 1.4901161193847656e-08;  // sqrt(eps)
-    // initial column norms: one wave per column
+__device__ void wg_qr_udv(double* M, double* Mp, int N, double* U, double* D, double* Tl, double* Wq, Lds& s) {
+    double* P = s.panel;                  // m x jb panel, ld = m
+    double* Tb = s.panel + NMAX * NB;     // NB x NB T of the current panel (ld NB)
+    double* Gm = Tb + NB * NB;            // NB x NB Gram matrix V^T V
+    double* Wsm = Wq;                     // NB x N GEMM scratch (ld NB)
+    double* Tg = Wq + (size_t)NB * N;     // T factors of all panels, NB x NB each
+    double* tau = s.v1;
+    // column norms and the sorting permutation
     for (int j = wave(); j < N; j += 4) {
         double acc = 0.0;
         for (int i = lane(); i < N; i += 64) { const double v = M[(size_t)j * N + i]; acc += v * v; }
         acc = wave_sum(acc);
-        if (lane() == 0) { vn1[j] = sqrt(acc); vn2[j] = vn1[j]; s.perm[j] = j; }
+        if (lane() == 0) s.v0[j] = acc;
     }
     __syncthreads();
-    for (int k = 0; k < N; ++k) {
-        if (wave() == 0) {
-            double best = -1.0; int bi = k;
-            for (int j = k + lane(); j < N; j += 64)
-                if (vn1[j] > best || (vn1[j] == best && j < bi)) { best = vn1[j]; bi = j; }
-            wave_argmax(best, bi);
-            if (lane() == 0) s.ivar[0] = bi;
+    for (int j = tid(); j < N; j += NT) {
+        const double nj = s.v0[j];
+        int r = 0;
+        for (int i = 0; i < N; ++i) {
+            const double ni = s.v0[i];
+            r += (ni > nj) || (ni == nj && i < j);
+        }
+        s.perm[r] = j;
+    }
+    __syncthreads();
+    for (int e = tid(); e < N * N; e += NT) {
+        const int r = e % N, c = e / N;
+        Mp[e] = M[(size_t)s.perm[c] * N + r];
+    }
+    __syncthreads();
+    for (int k0 = 0; k0 < N; k0 += NB) {
+        const int jb = min(NB, N - k0), m = N - k0;
+        for (int e = tid(); e < m * jb; e += NT) {
+            const int r = e % m, c = e / m;
+            P[c * m + r] = Mp[(size_t)(k0 + c) * N + k0 + r];
         }
         __syncthreads();
-        const int p = s.ivar[0];
-        if (p != k) {
-            for (int i = tid(); i < N; i += NT) {
-                const double t = M[(size_t)k * N + i];
-                M[(size_t)k * N + i] = M[(size_t)p * N + i];
-                M[(size_t)p * N + i] = t;
-            }
-            if (tid() == 0) {
-                double t = vn1[k]; vn1[k] = vn1[p]; vn1[p] = t;
-                t = vn2[k]; vn2[k] = vn2[p]; vn2[p] = t;
-                int q = s.perm[k]; s.perm[k] = s.perm[p]; s.perm[p] = q;
-            }
-            __syncthreads();
-        }
-        // Householder reflector of M(k:N, k): wave 0 computes the norm
-        double* mk = M + (size_t)k * N;
-        if (wave() == 0) {
+        for (int j = 0; j < jb; ++j) {
             double acc = 0.0;
-            for (int i = k + 1 + lane(); i < N; i += 64) acc += mk[i] * mk[i];
-            acc = wave_sum(acc);
-            if (lane() == 0) {
-                const double x0 = mk[k];
+            for (int r = j + 1 + tid(); r < m; r += NT) acc += P[j * m + r] * P[j * m + r];
+            wg_sum8(&acc, 1, s);
+            if (tid() == 0) {
+                const double x0 = P[j * m + j];
                 const double nrm = sqrt(x0 * x0 + acc);
-                if (nrm == 0.0) { tau[k] = 0.0; s.red[8] = 1.0; s.red[9] = 0.0; }
+                if (nrm == 0.0) { tau[k0 + j] = 0.0; s.red[8] = 1.0; s.red[9] = x0; }
                 else {
                     const double alpha = x0 > 0 ? -nrm : nrm;
                     const double v0 = x0 - alpha;
-                    tau[k] = -v0 / alpha;
-                    s.red[8] = 1.0 / v0;  // scale of the sub-diagonal part of v
+                    tau[k0 + j] = -v0 / alpha;
+                    s.red[8] = 1.0 / v0;
                     s.red[9] = alpha;
                 }
             }
+            __syncthreads();
+            const double tk = tau[k0 + j];
+            const double inv_v0 = s.red[8];
+            if (tk != 0.0)
+                for (int r = j + 1 + tid(); r < m; r += NT) P[j * m + r] *= inv_v0;
+            __syncthreads();
+            if (tid() == 0) P[j * m + j] = s.red[9];
+            // apply H_j to the remaining panel columns, one wave per column
+            for (int c = j + 1 + wave(); c < jb; c += 4) {
+                double w = 0.0;
+                for (int r = j + 1 + lane(); r < m; r += 64) w += P[j * m + r] * P[c * m + r];
+                w = wave_sum(w);
+                w = tk * (P[c * m + j] + w);
+                for (int r = j + 1 + lane(); r < m; r += 64) P[c * m + r] -= w * P[j * m + r];
+                if (lane() == 0) P[c * m + j] -= w;
+            }
+            __syncthreads();
+        }
+        // Gram matrix of the reflectors (V unit lower: V(j,j) = 1, V(r,l) = P(r,l) for r > l)
+        for (int q = wave(); q < jb * jb; q += 4) {
+            const int l = q % jb, j = q / jb;
+            if (l >= j) continue;
+            double g = 0.0;
+            for (int r = j + 1 + lane(); r < m; r += 64) g += P[l * m + r] * P[j * m + r];
+            g = wave_sum(g);
+            if (lane() == 0) Gm[j * NB + l] = P[l * m + j] + g;
         }
         __syncthreads();
-        const double inv_v0 = s.red[8];
-        const double tk = tau[k];
-        if (tk != 0.0)
-            for (int i = k + 1 + tid(); i < N; i += NT) mk[i] *= inv_v0;
-        __syncthreads();
-        if (tid() == 0 && tk != 0.0) mk[k] = s.red[9];
-        // apply H = I - tau v v^T to the trailing columns; one wave per column
-        for (int j = k + 1 + wave(); j < N; j += 4) {
-            double* mj = M + (size_t)j * N;
-            double acc = 0.0;
-            if (tk != 0.0) {
-                for (int i = k + 1 + lane(); i < N; i += 64) acc += mk[i] * mj[i];
-                acc = wave_sum(acc);
-                acc = tk * (mj[k] + acc);
-                for (int i = k + 1 + lane(); i < N; i += 64) mj[i] -= acc * mk[i];
-            }
-            // row k element and norm downdate (lane 0)
-            double rkj = mj[k];
-            if (lane() == 0) {
-                rkj = mj[k] - acc;
-                mj[k] = rkj;
-            }
-            rkj = __shfl(rkj, 0);
-            bool recompute = false;
-            if (lane() == 0 && vn1[j] != 0.0) {
-                double temp = fabs(rkj) / vn1[j];
-                temp = 1.0 - temp * temp;
-                temp = temp < 0.0 ? 0.0 : temp;
-                const double r = vn1[j] / vn2[j];
-                const double temp2 = temp * r * r;
-                if (temp2 <= tol3z) recompute = true;
-                else vn1[j] = vn1[j] * sqrt(temp);
-            }
-            recompute = __shfl((int)recompute, 0) != 0;
-            if (recompute) {
-                double a2 = 0.0;
-                for (int i = k + 1 + lane(); i < N; i += 64) a2 += mj[i] * mj[i];
-                a2 = wave_sum(a2);
-                if (lane() == 0) { vn1[j] = sqrt(a2); vn2[j] = vn1[j]; }
+        if (tid() == 0) {
+            for (int e = 0; e < NB * NB; ++e) Tb[e] = 0.0;
+            for (int j = 0; j < jb; ++j) {
+                const double tj = tau[k0 + j];
+                for (int i = 0; i < j; ++i) {
+                    double acc = 0.0;
+                    for (int l = i; l < j; ++l) acc += Tb[l * NB + i] * Gm[j * NB + l];
+                    Tb[j * NB + i] = -tj * acc;
+                }
+                Tb[j * NB + j] = tj;
+                for (int i = j + 1; i < NB; ++i) Tb[j * NB + i] = 0.0;
             }
         }
         __syncthreads();
+        // R to Mp, explicit V to M, T to Tg
+        for (int e = tid(); e < m * jb; e += NT) {
+            const int r = e % m, c = e / m;
+            const double v = P[c * m + r];
+            Mp[(size_t)(k0 + c) * N + k0 + r] = (r <= c) ? v : 0.0;
+            M[(size_t)(k0 + c) * N + k0 + r] = (r == c) ? 1.0 : (r > c ? v : 0.0);
+        }
+        for (int e = tid(); e < NB * NB; e += NT) Tg[(size_t)(k0 / NB) * NB * NB + e] = Tb[e];
+        __syncthreads();
+        // trailing columns: A2 <- (I - V T^T V^T) A2
+        const int n2 = N - k0 - jb;
+        if (n2 > 0) {
+            const double* V = M + (size_t)k0 * N + k0;
+            double* A2 = Mp + (size_t)(k0 + jb) * N + k0;
+            gemm64(jb, n2, m, V, N, true, A2, N, false, Wsm, NB, nullptr, nullptr, nullptr, 0.0, s);
+            for (int c = tid(); c < n2; c += NT) {
+                double x[NB], y[NB];
+#pragma unroll
+                for (int i = 0; i < NB; ++i) x[i] = (i < jb) ? Wsm[(size_t)c * NB + i] : 0.0;
+#pragma unroll
+                for (int i = 0; i < NB; ++i) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int l = 0; l <= i; ++l) acc += Tb[i * NB + l] * x[l];   // (T^T)(i,l) = T(l,i)
+                    y[i] = acc;
+                }
+#pragma unroll
+                for (int i = 0; i < NB; ++i)
+                    if (i < jb) Wsm[(size_t)c * NB + i] = y[i];
+            }
+            __syncthreads();
+            gemm64(m, n2, jb, V, N, false, Wsm, NB, false, A2, N, nullptr, nullptr, nullptr, 1.0, s, -1.0);
+        }
     }
     // D and Tl = D^-1 R P^T
-    for (int k = tid(); k < N; k += NT) D[k] = fabs(M[(size_t)k * N + k]);
+    for (int k = tid(); k < N; k += NT) D[k] = fabs(Mp[(size_t)k * N + k]);
     __syncthreads();
     for (int e = tid(); e < N * N; e += NT) {
-        const int r = e % N, c = e / N;  // Tl(r, perm[c]) = R(r, c) / D(r)
-        Tl[(size_t)s.perm[c] * N + r] = (c >= r) ? M[(size_t)c * N + r] / D[r] : 0.0;
+        const int r = e % N, c = e / N;
+        Tl[(size_t)s.perm[c] * N + r] = (c >= r) ? Mp[(size_t)c * N + r] / D[r] : 0.0;
     }
-    // explicit Q by backward accumulation of the reflectors
+    // Q = H_0 H_1 ... : U <- (I - V T V^T) U, panels from the last
     for (int e = tid(); e < N * N; e += NT) U[e] = ((e % N) == (e / N)) ? 1.0 : 0.0;
     __syncthreads();
-    for (int k = N - 1; k >= 0; --k) {
-        const double tk = tau[k];
-        if (tk == 0.0) continue;
-        const double* vk = M + (size_t)k * N;
-        for (int j = k + wave(); j < N; j += 4) {
-            double* uj = U + (size_t)j * N;
-            double acc = 0.0;
-            for (int i = k + 1 + lane(); i < N; i += 64) acc += vk[i] * uj[i];
-            acc = wave_sum(acc);
-            acc = tk * (uj[k] + acc);
-            for (int i = k + 1 + lane(); i < N; i += 64) uj[i] -= acc * vk[i];
-            if (lane() == 0) uj[k] -= acc;
+    for (int k0 = ((N - 1) / NB) * NB; k0 >= 0; k0 -= NB) {
+        const int jb = min(NB, N - k0), m = N - k0;
+        const double* V = M + (size_t)k0 * N + k0;
+        const double* T = Tg + (size_t)(k0 / NB) * NB * NB;
+        double* U2 = U + (size_t)k0 * N + k0;
+        gemm64(jb, m, m, V, N, true, U2, N, false, Wsm, NB, nullptr, nullptr, nullptr, 0.0, s);
+        for (int c = tid(); c < m; c += NT) {
+            double x[NB], y[NB];
+#pragma unroll
+            for (int i = 0; i < NB; ++i) x[i] = (i < jb) ? Wsm[(size_t)c * NB + i] : 0.0;
+#pragma unroll
+            for (int i = 0; i < NB; ++i) {
+                double acc = 0.0;
+#pragma unroll
+                for (int l = i; l < NB; ++l) acc += T[l * NB + i] * x[l];   // T(i,l), upper
+                y[i] = acc;
+            }
+#pragma unroll
+            for (int i = 0; i < NB; ++i)
+                if (i < jb) Wsm[(size_t)c * NB + i] = y[i];
         }
         __syncthreads();
+        gemm64(m, m, jb, V, N, false, Wsm, NB, false, U2, N, nullptr, nullptr, nullptr, 1.0, s, -1.0);
     }
 }
 
@@ -634,7 +748,7 @@ __device__ void wg_push(const ChainDesc& d, int sp, int slot, int below, const d
     double* W1 = d.W + (size_t)N * N;
     if (below < 0) wg_copy(W0, A, N * N);
     else gemm64(N, N, N, A, N, false, slotU(d, sp, below), N, false, W0, N, nullptr, nullptr, slotD(d, sp, below), 0.0, s);
-    wg_qrp_udv(W0, N, slotU(d, sp, slot), slotD(d, sp, slot), W1, s);
+    wg_qr_udv(W0, d.W + 2 * (size_t)N * N, N, slotU(d, sp, slot), slotD(d, sp, slot), W1, d.Wq, s);
     __syncthreads();
     if (below < 0) wg_copy(slotT(d, sp, slot), W1, N * N);
     else gemm64(N, N, N, W1, N, false, slotT(d, sp, below), N, false, slotT(d, sp, slot), N, nullptr, nullptr, nullptr, 0.0, s);
@@ -695,22 +809,30 @@ __device__ void wg_fresh_green(const ChainDesc& d, int sp, int lslot, int rslot,
 // G_cur = G0 + X Y^T; the diagonal of G_cur is kept in LDS; thread 0 draws
 // the decisions (it can walk over rejected sites without synchronization).
 // ---------------------------------------------------------------------------
-__device__ void wg_flush(const ChainDesc& d, int cnt, Lds& s) {
+// delayed-update depth that fits the LDS panels (X and Y for every spin)
+__device__ __forceinline__ int delay_depth(const ChainDesc& d) {
+    const int cap = PANEL_DOUBLES / (2 * d.nspin * d.N);
+    return min(d.kdelay, cap);
+}
+
+__device__ void wg_flush(const ChainDesc& d, int cnt, int k, Lds& s) {
     const int N = d.N;
-    for (int sp = 0; sp < d.nspin; ++sp)
-        gemm64(N, N, cnt, d.X + (size_t)sp * N * d.kdelay, N, false, d.Y + (size_t)sp * N * d.kdelay, N, true,
-                d.G + (size_t)sp * N * N, N, nullptr, nullptr, nullptr, 1.0, s);
+    for (int sp = 0; sp < d.nspin; ++sp) {
+        const double* X = s.panel + (size_t)sp * 2 * N * k;
+        const double* Y = X + (size_t)N * k;
+        gemm64(N, N, cnt, X, N, false, Y, N, true, d.G + (size_t)sp * N * N, N, nullptr, nullptr, nullptr, 1.0, s);
+    }
 }
 
 __device__ void wg_metropolis(const ChainDesc& d, int e, uint64_t& rng, double& sign, bool record, int& naccept, Lds& s) {
     const int N = d.N;
     const int ns = d.nspin;
+    const int k = delay_depth(d);
     double* dgs[2] = {s.du, ns == 2 ? s.dd : s.du};
     for (int i = tid(); i < N; i += NT) {
         s.du[i] = d.G[(size_t)i * N + i];
         if (ns == 2) s.dd[i] = d.G[(size_t)N * N + (size_t)i * N + i];
     }
-    if (tid() == 0) s.ivar[1] = 0;
     __syncthreads();
     int cnt = 0;
     int i0 = 0;
@@ -743,28 +865,28 @@ __device__ void wg_metropolis(const ChainDesc& d, int e, uint64_t& rng, double& 
         for (int sp = 0; sp < ns; ++sp) {
             const double g = s.gam[sp];
             const double* G0 = d.G + (size_t)sp * N * N;
-            double* X = d.X + (size_t)sp * N * d.kdelay;
-            double* Y = d.Y + (size_t)sp * N * d.kdelay;
+            double* X = s.panel + (size_t)sp * 2 * N * k;
+            double* Y = X + (size_t)N * k;
             double* dg = dgs[sp];
             for (int a = tid(); a < N; a += NT) {
                 double x = G0[(size_t)j * N + a];
                 double y = G0[(size_t)a * N + j] - (a == j ? 1.0 : 0.0);
                 for (int m = 0; m < cnt; ++m) {
-                    x += X[(size_t)m * N + a] * Y[(size_t)m * N + j];
-                    y += X[(size_t)m * N + j] * Y[(size_t)m * N + a];
+                    x += X[m * N + a] * Y[m * N + j];
+                    y += X[m * N + j] * Y[m * N + a];
                 }
-                X[(size_t)cnt * N + a] = g * x;
-                Y[(size_t)cnt * N + a] = y;
+                X[cnt * N + a] = g * x;
+                Y[cnt * N + a] = y;
                 dg[a] += g * x * y;
             }
         }
         ++cnt;
         __syncthreads();
-        if (cnt == d.kdelay) { wg_flush(d, cnt, s); cnt = 0; }
+        if (cnt == k) { wg_flush(d, cnt, k, s); cnt = 0; }
         i0 = j + 1;
         if (i0 >= N) break;
     }
-    if (cnt > 0) wg_flush(d, cnt, s);
+    if (cnt > 0) wg_flush(d, cnt, k, s);
     __syncthreads();
 }
 

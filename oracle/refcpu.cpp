@@ -376,6 +376,57 @@ static void qrp_udv(const Mat& Ain, Mat& U, vector<double>& d, Mat& V) {
         for (int c = r; c < n; ++c) V(perm[c], r) = A(r, c) / d[r];
 }
 
+// Pre-pivoted Householder QR: columns sorted once by decreasing norm, then
+// QR without pivoting (Bauer, SciPost Phys. Core 2, 011 (2020)): A P = Q R,
+// U = Q, d = |diag R|, V^T = diag(d)^-1 R P^T.  The GPU stacks use this
+// factorization with blocked Householder panels.
+static void qr_prepivot_udv(const Mat& Ain, Mat& U, vector<double>& d, Mat& V) {
+    const int n = Ain.n;
+    vector<double> cn(n);
+    for (int j = 0; j < n; ++j) { double s = 0.0; for (int i = 0; i < n; ++i) s += Ain(i, j) * Ain(i, j); cn[j] = s; }
+    vector<int> perm(n);
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return cn[a] > cn[b]; });
+    Mat A(n);
+    for (int c = 0; c < n; ++c)
+        for (int i = 0; i < n; ++i) A(i, c) = Ain(i, perm[c]);
+    vector<double> tau(n, 0.0);
+    for (int k = 0; k < n; ++k) {
+        double norm2 = 0.0;
+        for (int i = k; i < n; ++i) norm2 += A(i, k) * A(i, k);
+        const double nrm = std::sqrt(norm2);
+        if (nrm == 0.0) continue;
+        const double alpha = (A(k, k) > 0) ? -nrm : nrm;
+        const double v0 = A(k, k) - alpha;
+        for (int i = k + 1; i < n; ++i) A(i, k) /= v0;
+        tau[k] = -v0 / alpha;
+        A(k, k) = alpha;
+        for (int j = k + 1; j < n; ++j) {
+            double s = A(k, j);
+            for (int i = k + 1; i < n; ++i) s += A(i, k) * A(i, j);
+            s *= tau[k];
+            A(k, j) -= s;
+            for (int i = k + 1; i < n; ++i) A(i, j) -= s * A(i, k);
+        }
+    }
+    U = Mat::Identity(n);
+    for (int k = n - 1; k >= 0; --k) {
+        if (tau[k] == 0.0) continue;
+        for (int j = k; j < n; ++j) {
+            double s = U(k, j);
+            for (int i = k + 1; i < n; ++i) s += A(i, k) * U(i, j);
+            s *= tau[k];
+            U(k, j) -= s;
+            for (int i = k + 1; i < n; ++i) U(i, j) -= s * A(i, k);
+        }
+    }
+    d.assign(n, 0.0);
+    for (int k = 0; k < n; ++k) d[k] = std::fabs(A(k, k));
+    V = Mat(n);
+    for (int r = 0; r < n; ++r)
+        for (int c = r; c < n; ++c) V(perm[c], r) = A(r, c) / d[r];
+}
+
 // Matrix exponential, Higham 2005 scaling & squaring with Pade degree
 // 3/5/7/9/13 selected by the 1-norm (the method of Eigen's MatrixExponential,
 // used at src/model/repulsive_hubbard.cpp:64-65).
@@ -422,7 +473,7 @@ static Mat expm(const Mat& Ain) {
 // ----------------------------------------------------------------------------
 // Utils::SvdStack (src/svd_stack.cpp:15-87)
 // ----------------------------------------------------------------------------
-enum Factorization { kSVD = 0, kQRP = 1 };
+enum Factorization { kSVD = 0, kQRP = 1, kQRPre = 2 };
 
 struct SvdStack {
     int dim = 0, len = 0, fact = kSVD;
@@ -434,7 +485,8 @@ struct SvdStack {
     void clear() { len = 0; }
     void decompose(const Mat& M, int slot) {
         if (fact == kSVD) svd(M, U[slot], S[slot], V[slot]);
-        else qrp_udv(M, U[slot], S[slot], V[slot]);
+        else if (fact == kQRP) qrp_udv(M, U[slot], S[slot], V[slot]);
+        else qr_prepivot_udv(M, U[slot], S[slot], V[slot]);
     }
     void push(const Mat& m) {                                                  // :34-61
         if (len == 0) decompose(m, 0);
@@ -975,11 +1027,13 @@ void rc_expm(int n, const double* a, double* out) {
     Mat A(n); std::memcpy(A.a.data(), a, sizeof(double) * n * n);
     Mat E = expm(A); std::memcpy(out, E.a.data(), sizeof(double) * n * n);
 }
-// factorization: 0 svd, 1 qrp.  out: U, s, V
+// factorization: 0 svd, 1 column-pivoted QR, 2 pre-pivoted QR.  out: U, s, V
 void rc_udv(int n, int factorization, const double* a, double* U, double* s, double* V) {
     Mat A(n); std::memcpy(A.a.data(), a, sizeof(double) * n * n);
     Mat Um, Vm; vector<double> sv;
-    if (factorization == 0) svd(A, Um, sv, Vm); else qrp_udv(A, Um, sv, Vm);
+    if (factorization == 0) svd(A, Um, sv, Vm);
+    else if (factorization == 1) qrp_udv(A, Um, sv, Vm);
+    else qr_prepivot_udv(A, Um, sv, Vm);
     std::memcpy(U, Um.a.data(), sizeof(double) * n * n);
     std::memcpy(V, Vm.a.data(), sizeof(double) * n * n);
     std::memcpy(s, sv.data(), sizeof(double) * n);

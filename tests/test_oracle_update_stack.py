@@ -53,7 +53,7 @@ def test_ratio_and_rank1_vs_brute_force(model):
         assert np.max(np.abs(c.green(-1) - np.linalg.inv(np.eye(n) + Md2))) < 1e-12
 
 
-@pytest.mark.parametrize("fact", [0, 1])
+@pytest.mark.parametrize("fact", [0, 1, 2])
 def test_udv_reconstruction(fact):
     rng = np.random.default_rng(5)
     for n in (4, 16, 33):
@@ -67,7 +67,7 @@ def test_udv_reconstruction(fact):
             assert np.allclose(s, np.linalg.svd(A, compute_uv=False), rtol=1e-12)
 
 
-@pytest.mark.parametrize("fact", [0, 1])
+@pytest.mark.parametrize("fact", [0, 1, 2])
 def test_udv_graded_matrix(fact):
     # column-graded input X diag(d), the shape met in the stacks ((A U) D): every
     # column, however small, is reproduced to relative accuracy
@@ -130,11 +130,12 @@ def test_stabilized_green_matches_direct_every_slice(model, cb):
 
 
 @pytest.mark.parametrize("model", [0, 1])
-def test_green_invariant_under_factorization(model):
-    # SVD stacks (the reference) and pivoted-QR stacks give the same G to rounding
+@pytest.mark.parametrize("fact", [1, 2])
+def test_green_invariant_under_factorization(model, fact):
+    # SVD stacks (the reference) and column-pivoted / pre-pivoted QR stacks give the same G to rounding
     kw = dict(pace=10, model=model, U=4.0, mu=0.0)
     a = O.Chain(4, 80, 4.0, factorization=0, **kw)
-    b = O.Chain(4, 80, 4.0, factorization=1, **kw)
+    b = O.Chain(4, 80, 4.0, factorization=fact, **kw)
     for c in (a, b):
         c.seed(42); c.fields_random(); c.init()
     assert np.max(np.abs(a.green(1) - b.green(1))) < 1e-11
@@ -144,3 +145,16 @@ def test_green_invariant_under_factorization(model):
         for slot in range(0, 80, 7):
             assert np.max(np.abs(a.slice_green(slot, 1) - b.slice_green(slot, 1))) < 1e-9
     assert np.max(np.abs(a.green(-1) - b.green(-1))) < 1e-10
+
+
+def test_prepivot_green_at_low_temperature():
+    # beta = 12, U = 8: scales spanning ~1e20; SVD and pre-pivoted QR stacks still agree
+    kw = dict(pace=10, model=1, U=8.0, mu=0.3)
+    a = O.Chain(6, 120, 12.0, factorization=0, **kw)
+    b = O.Chain(6, 120, 12.0, factorization=2, **kw)
+    for c in (a, b):
+        c.seed(5); c.fields_random(); c.init()
+    assert np.max(np.abs(a.green(1) - b.green(1))) < 1e-10
+    a.sweep_forward(); b.sweep_forward()
+    assert np.array_equal(a.fields(), b.fields())
+    assert np.max(np.abs(a.green(1) - b.green(1))) < 1e-10
