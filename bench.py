@@ -27,7 +27,7 @@ METRIC = "chain-sweeps/s batched vs plain FP64 GPU at max|dG|<=1e-6; p50/p99 job
 
 WORKLOADS = {
     "batched-12x12": dict(chains=256, side=12, time_size=60, beta=6.0, pace=10, model=D.REPULSIVE,
-                          checkerboard=True, U=4.0, mu=0.0, delay=16,
+                          checkerboard=True, U=4.0, mu=0.0, delay=32,
                           desc="256 chains x 12x12 repulsive Hubbard t=1 U=4 mu=0 beta=6 L=60 p=10 checkerboard FP64"),
     "plain-8x8": dict(chains=1, side=8, time_size=160, beta=8.0, pace=10, model=D.ATTRACTIVE,
                       checkerboard=True, U=4.0, mu=-0.5, delay=16,
@@ -36,7 +36,8 @@ WORKLOADS = {
                       checkerboard=False, U=4.0, mu=0.0, delay=16,
                       desc="1 chain 4x4 repulsive Hubbard t=1 U=4 mu=0 beta=4 L=80 p=10 dense FP64"),
     "mixed-16x16": dict(chains=64, side=16, time_size=200, beta=10.0, pace=8, model=D.REPULSIVE,
-                        checkerboard=False, U=6.0, mu=0.0, delay=16, precision=D.PREC_MIXED,
+                        checkerboard=False, U=6.0, mu=0.0, delay=32, precision=D.PREC_MIXED,
+                        mus=[0.0, -0.2, -0.4, -0.6],
                         desc="64 chains x 16x16 repulsive Hubbard t=1 U=6 beta=10 L=200 p=8 dense, FP32 MFMA "
                              "products/wraps + FP64 stacks"),
 }
@@ -108,10 +109,11 @@ def phase_breakdown(w, chains, sweeps=2):
     b.sweep(sweeps, profile=True)
     b.synchronize()
     n = min(b.n, 8)
-    tot = [0.0] * 6
+    nph = len(D.PHASES)
+    tot = [0.0] * nph
     for c in range(n):
         ph = b.info(c).phase_seconds
-        for q in range(6):
+        for q in range(nph):
             tot[q] += ph[q]
     out = {name: 1000.0 * tot[q] / (n * sweeps) for q, name in enumerate(D.PHASES)}
     out["launch_ms_per_sweep"] = b.last_sweep_ms() / sweeps

@@ -7,6 +7,7 @@
 #define DQMC_NT 256          // threads per workgroup (4 wavefronts of 64)
 #define DQMC_NMAX 512        // largest lattice (sites) a chain may have
 #define DQMC_KMAX 64         // largest delayed-update depth
+#define DQMC_PANEL_DOUBLES 12288   // LDS panel of a workgroup (96 KB)
 
 enum {
     DQMC_FLAG_MEASURE = 1,   // accumulate equal-time observables after each sweep
@@ -17,6 +18,8 @@ enum {
 
 // phases timed with DQMC_FLAG_PROFILE (100 MHz device wall clock ticks)
 enum { DQMC_PH_WRAP = 0, DQMC_PH_METROPOLIS, DQMC_PH_PRODUCT, DQMC_PH_PUSH, DQMC_PH_FRESH, DQMC_PH_MEASURE,
+       DQMC_PH_WALK,     // part of METROPOLIS: the in-block decisions and C updates (one wavefront)
+       DQMC_PH_PANEL,    // part of PUSH and FRESH: QR and LU panel factorizations in LDS
        DQMC_PH_COUNT };
 
 // number of accumulated equal-time quantities, in this order:
@@ -54,7 +57,7 @@ struct ChainDesc {
     double* stkT;        // nspin x n_stack x N*N   (cumulative V^T of each level)
     double* stkD;        // nspin x n_stack x N
     double* W;           // 3 x N*N scratch for the stabilization
-    double* Wq;          // 2 x 16 x N scratch of the blocked QR (GEMM panel and T factors)
+    double* Wq;          // 3 x 16 x (N + 16) scratch of the blocked QR (GEMM panels and T factors)
     double* X;           // nspin x N*kdelay   delayed-update columns
     double* Y;           // nspin x N*kdelay   delayed-update rows
     const double* expK;  // N*N (dense only)

@@ -32,7 +32,7 @@ namespace dqmc {
 // LDS layout of one workgroup
 // ---------------------------------------------------------------------------
 #define NB 16                    // panel width of the blocked LU and QR
-#define PANEL_DOUBLES 12288      // 96 KB: an LU/QR panel (up to NMAX x NB) plus its NB x NB blocks
+#define PANEL_DOUBLES DQMC_PANEL_DOUBLES   // an LU/QR panel (up to NMAX x NB) plus its NB x NB blocks
 
 struct Lds {
     double As[16][64];         // GEMM A tile (k-major)
@@ -46,6 +46,20 @@ struct Lds {
     double red[16];
     double gam[2];
     int ivar[4];
+    unsigned long long prof[2];   // sub-phase ticks (walk, panels) of the running launch
+    int prof_on;
+};
+
+// one LDS object per workgroup, at namespace scope so that separately
+// compiled (noinline) phases address it with ds_* instructions
+__shared__ Lds g_lds;
+
+// sub-phase timer: thread 0 adds the 100 MHz wall-clock ticks between start()
+// and stop() to g_lds.prof[slot] when the launch profiles
+struct SubClock {
+    unsigned long long t0 = 0;
+    __device__ void start() { if (g_lds.prof_on && threadIdx.x == 0) t0 = wall_clock64(); }
+    __device__ void stop(int slot) { if (g_lds.prof_on && threadIdx.x == 0) g_lds.prof[slot] += wall_clock64() - t0; }
 };
 
 __device__ __forceinline__ int tid() { return threadIdx.x; }
@@ -68,8 +82,17 @@ __device__ __forceinline__ void wave_argmax(double& v, int& idx) {
         if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
     }
 }
+// value of lane l (uniform l) of a double, through the scalar unit
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffffll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // sum over the workgroup of n values at once (4 per LDS pass); all threads get the result
-__device__ void wg_sum8(double* v, int n, Lds& s) {
+__device__ void wg_sum8(double* v, int n, Lds&) {
+    Lds& s = g_lds;
     for (int q = 0; q < n; ++q) v[q] = wave_sum(v[q]);
     for (int base = 0; base < n; base += 4) {
         const int m = (n - base) < 4 ? (n - base) : 4;
@@ -80,7 +103,8 @@ __device__ void wg_sum8(double* v, int n, Lds& s) {
         __syncthreads();
     }
 }
-__device__ double wg_max(double v, Lds& s) {
+__device__ double wg_max(double v, Lds&) {
+    Lds& s = g_lds;
     v = wave_max(v);
     if (lane() == 0) s.red[wave()] = v;
     __syncthreads();
@@ -101,7 +125,7 @@ __device__ __forceinline__ uint64_t minstd_next(uint64_t x) {
     if (r >= 0x7fffffffull) r -= 0x7fffffffull;
     return r;
 }
-__device__ __noinline__ double canonical(uint64_t& x) {
+__device__ __forceinline__ double canonical(uint64_t& x) {
 #pragma clang fp contract(off)
     const double R = 2147483646.0;
     x = minstd_next(x);
@@ -124,8 +148,9 @@ __device__ __noinline__ double canonical(uint64_t& x) {
 // 16-deep k-slabs staged through LDS.  rs/ms/cs may be null (=1).
 // C must not alias A or B.
 // ---------------------------------------------------------------------------
-__device__ void wg_gemm(int M, int Nc, int K, const double* A, int lda, bool tA, const double* B, int ldb, bool tB,
-                        double* C, int ldc, const double* rs, const double* ms, const double* cs, double beta, Lds& s) {
+__device__ __noinline__ void wg_gemm(int M, int Nc, int K, const double* A, int lda, bool tA, const double* B, int ldb, bool tB,
+                        double* C, int ldc, const double* rs, const double* ms, const double* cs, double beta, Lds&) {
+    Lds& s = g_lds;
     const int tx = tid() & 15, ty = tid() >> 4;
     const int tm = (M + 63) >> 6, tn = (Nc + 63) >> 6;
     for (int tile = 0; tile < tm * tn; ++tile) {
@@ -191,26 +216,68 @@ __device__ void wg_gemm(int M, int Nc, int K, const double* A, int lda, bool tA,
 
 // ---------------------------------------------------------------------------
 // The same contract on the matrix cores.  Each of the 4 wavefronts owns a
-// 32x32 quarter of the 64x64 tile as 2x2 blocks of v_mfma_f64_16x16x4f64.
+// 32x32 quarter of the 64x64 tile as 2x2 blocks of v_mfma_f64_16x16x4f64;
+// 16x16 blocks outside the output are skipped.
 // Operand lanes: A[row = lane%16][k = lane/16], B[k = lane/16][col = lane%16];
 // result register i of lane l holds D[l/16 + 4i][l%16] (FP64 16x16x4 layout).
 // With lowp the tile is rounded to FP32 and multiplied by
 // v_mfma_f32_16x16x4f32 with FP32 accumulation (result D[4(l/16)+i][l%16]).
+// Pipelining: the 8 loads of the next 16-deep k-slab (or of the next tile's
+// first slab) are issued before the matrix-core work on the current slab, so
+// their latency hides behind it.  aL / bL say that A / B live in LDS; otherwise
+// they are read with global (not flat) loads, which keeps the LDS wait counter
+// free of them.  The middle scaling ms, when given, lives in LDS.
 // ---------------------------------------------------------------------------
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
+#define AS_G __attribute__((address_space(1)))
+#define AS_L __attribute__((address_space(3)))
+template <bool lds> struct MemSpace { typedef const AS_G double* cptr; };
+template <> struct MemSpace<true> { typedef const AS_L double* cptr; };
 
-template <bool lowp>
-__device__ void wg_gemm_mfma(int M, int Nc, int K, const double* A, int lda, bool tA, const double* B, int ldb, bool tB,
-                             double* C, int ldc, const double* rs, const double* ms, const double* cs, double beta,
-                             Lds& s, double alpha = 1.0) {
+template <bool lowp, bool aL, bool bL>
+__device__ __noinline__ void wg_gemm_mfma(int M, int Nc, int K, const double* A_, int lda, bool tA, const double* B_,
+                                          int ldb, bool tB, double* C, int ldc, const double* rs, const double* ms_,
+                                          const double* cs, double beta, Lds&, double alpha = 1.0) {
+    Lds& s = g_lds;
+    typedef typename MemSpace<aL>::cptr APtr;
+    typedef typename MemSpace<bL>::cptr BPtr;
+    const APtr A = (APtr)A_;
+    const BPtr B = (BPtr)B_;
+    const bool has_ms = ms_ != nullptr;
+    const AS_L double* ms = (const AS_L double*)ms_;
     const int l = lane(), w = wave();
     const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
-    const int tm = (M + 63) >> 6, tn = (Nc + 63) >> 6;
+    const int tm = (M + 63) >> 6, tn = (Nc + 63) >> 6, ntile = tm * tn;
     float* Af = reinterpret_cast<float*>(&s.As[0][0]);  // [16][64] floats in lowp mode
     float* Bf = reinterpret_cast<float*>(&s.Bs[0][0]);
-    for (int tile = 0; tile < tm * tn; ++tile) {
+    // element q of this thread's share of a slab: A at (ma[q], ka[q]), B at (kb[q], nb[q])
+    int ma[4], ka[4], nb[4], kbq[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int e = tid() + NT * q;
+        if (!tA) { ma[q] = e & 63; ka[q] = e >> 6; } else { ka[q] = e & 15; ma[q] = e >> 4; }
+        if (!tB) { kbq[q] = e & 15; nb[q] = e >> 4; } else { nb[q] = e & 63; kbq[q] = e >> 6; }
+    }
+    double va[4], vb[4];
+    auto load_slab = [&](int r0, int c0, int k0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int gm = r0 + ma[q], gk = k0 + ka[q];
+            va[q] = (gm < M && gk < K) ? (tA ? A[(size_t)gm * lda + gk] : A[(size_t)gk * lda + gm]) : 0.0;
+            const int gn = c0 + nb[q], gk2 = k0 + kbq[q];
+            vb[q] = (gn < Nc && gk2 < K) ? (tB ? B[(size_t)gk2 * ldb + gn] : B[(size_t)gn * ldb + gk2]) : 0.0;
+        }
+    };
+    if (ntile > 0) load_slab(0, 0, 0);
+#pragma unroll 1
+    for (int tile = 0; tile < ntile; ++tile) {
         const int r0 = (tile % tm) << 6, c0 = (tile / tm) << 6;
+        bool blk[2][2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) blk[a][b] = (r0 + wr + 16 * a < M) && (c0 + wc + 16 * b < Nc);
         dbl4 acc[2][2];
         flt4 accf[2][2];
 #pragma unroll
@@ -220,44 +287,35 @@ __device__ void wg_gemm_mfma(int M, int Nc, int K, const double* A, int lda, boo
                 acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
                 accf[a][b] = flt4{0.f, 0.f, 0.f, 0.f};
             }
+#pragma unroll 1
         for (int k0 = 0; k0 < K; k0 += 16) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int e = tid() + NT * q;
-                int m, k;
-                if (!tA) { m = e & 63; k = e >> 6; } else { k = e & 15; m = e >> 4; }
-                const int gm = r0 + m, gk = k0 + k;
-                double v = 0.0;
-                if (gm < M && gk < K) {
-                    v = tA ? A[(size_t)gm * lda + gk] : A[(size_t)gk * lda + gm];
-                    if (ms) v *= ms[gk];
-                }
-                if (lowp) Af[k * 64 + m] = (float)v; else s.As[k][m] = v;
-                int n;
-                if (!tB) { k = e & 15; n = e >> 4; } else { n = e & 63; k = e >> 6; }
-                const int gn = c0 + n, gk2 = k0 + k;
-                double x = 0.0;
-                if (gn < Nc && gk2 < K) x = tB ? B[(size_t)gk2 * ldb + gn] : B[(size_t)gn * ldb + gk2];
-                if (lowp) Bf[k * 64 + n] = (float)x; else s.Bs[k][n] = x;
+                double v = va[q];
+                if (has_ms && k0 + ka[q] < K) v *= ms[k0 + ka[q]];
+                if (lowp) { Af[ka[q] * 64 + ma[q]] = (float)v; Bf[kbq[q] * 64 + nb[q]] = (float)vb[q]; }
+                else { s.As[ka[q]][ma[q]] = v; s.Bs[kbq[q]][nb[q]] = vb[q]; }
             }
             __syncthreads();
+            if (k0 + 16 < K) load_slab(r0, c0, k0 + 16);
+            else if (tile + 1 < ntile) load_slab(((tile + 1) % tm) << 6, ((tile + 1) / tm) << 6, 0);
 #pragma unroll
             for (int kk = 0; kk < 4; ++kk) {
                 const int k = kk * 4 + (l >> 4);
                 if (lowp) {
                     const float a0 = Af[k * 64 + wr + (l & 15)], a1 = Af[k * 64 + wr + 16 + (l & 15)];
                     const float b0 = Bf[k * 64 + wc + (l & 15)], b1 = Bf[k * 64 + wc + 16 + (l & 15)];
-                    accf[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, accf[0][0], 0, 0, 0);
-                    accf[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, accf[0][1], 0, 0, 0);
-                    accf[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, accf[1][0], 0, 0, 0);
-                    accf[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, accf[1][1], 0, 0, 0);
+                    if (blk[0][0]) accf[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, accf[0][0], 0, 0, 0);
+                    if (blk[0][1]) accf[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, accf[0][1], 0, 0, 0);
+                    if (blk[1][0]) accf[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, accf[1][0], 0, 0, 0);
+                    if (blk[1][1]) accf[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, accf[1][1], 0, 0, 0);
                 } else {
                     const double a0 = s.As[k][wr + (l & 15)], a1 = s.As[k][wr + 16 + (l & 15)];
                     const double b0 = s.Bs[k][wc + (l & 15)], b1 = s.Bs[k][wc + 16 + (l & 15)];
-                    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-                    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-                    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-                    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+                    if (blk[0][0]) acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+                    if (blk[0][1]) acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+                    if (blk[1][0]) acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+                    if (blk[1][1]) acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
                 }
             }
             __syncthreads();
@@ -265,7 +323,8 @@ __device__ void wg_gemm_mfma(int M, int Nc, int K, const double* A, int lda, boo
 #pragma unroll
         for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-            for (int cb = 0; cb < 2; ++cb)
+            for (int cb = 0; cb < 2; ++cb) {
+                if (!blk[rb][cb]) continue;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const int row = r0 + wr + rb * 16 + (lowp ? 4 * (l >> 4) + i : (l >> 4) + 4 * i);
@@ -277,23 +336,40 @@ __device__ void wg_gemm_mfma(int M, int Nc, int K, const double* A, int lda, boo
                     double* cp = C + (size_t)col * ldc + row;
                     *cp = (beta != 0.0) ? v + beta * (*cp) : v;
                 }
+            }
     }
     __syncthreads();
 }
 
 // GEMMs of the engine: FP64 on the matrix cores for everything that stays
 // double precision; the slice products and wraps drop to FP32 MFMA when the
-// chain runs in mixed precision.
+// chain runs in mixed precision.  gemm64: A and B in global memory;
+// gemm64_lg: A in LDS; gemm64_ll: both in LDS.
 __device__ __forceinline__ void gemm64(int M, int Nc, int K, const double* A, int lda, bool tA, const double* B,
                                        int ldb, bool tB, double* C, int ldc, const double* rs, const double* ms,
-                                       const double* cs, double beta, Lds& s, double alpha = 1.0) {
-    wg_gemm_mfma<false>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, s, alpha);
+                                       const double* cs, double beta, Lds&, double alpha = 1.0) {
+    Lds& s = g_lds;
+    wg_gemm_mfma<false, false, false>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, s, alpha);
+}
+__device__ __forceinline__ void gemm64_lg(int M, int Nc, int K, const double* A, int lda, bool tA, const double* B,
+                                          int ldb, bool tB, double* C, int ldc, double beta, Lds&, double alpha = 1.0) {
+    Lds& s = g_lds;
+    wg_gemm_mfma<false, true, false>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, nullptr, nullptr, nullptr, beta, s,
+                                     alpha);
+}
+__device__ __forceinline__ void gemm64_ll(int M, int Nc, int K, const double* A, int lda, bool tA, const double* B,
+                                          int ldb, bool tB, double* C, int ldc, double beta, Lds&, double alpha = 1.0) {
+    Lds& s = g_lds;
+    wg_gemm_mfma<false, true, true>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, nullptr, nullptr, nullptr, beta, s,
+                                    alpha);
 }
 __device__ __forceinline__ void gemmB(const ChainDesc& d, int M, int Nc, int K, const double* A, int lda, bool tA,
                                       const double* B, int ldb, bool tB, double* C, int ldc, const double* rs,
-                                      const double* ms, const double* cs, Lds& s) {
-    if (d.precision == 1) wg_gemm_mfma<true>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, 0.0, s);
-    else wg_gemm_mfma<false>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, 0.0, s);
+                                      const double* ms, const double* cs, Lds&) {
+    Lds& s = g_lds;
+    if (d.precision == 1)
+        wg_gemm_mfma<true, false, false>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, 0.0, s);
+    else wg_gemm_mfma<false, false, false>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, 0.0, s);
 }
 
 __device__ void wg_copy(double* dst, const double* src, int n2) {
@@ -357,8 +433,115 @@ __device__ void wg_cb_right(double* M, int N, int side, const double P[3], int s
     __syncthreads();
 }
 
+// Fused checkerboard products.  A left product mixes rows within each
+// column, a right product mixes columns within each row, so a sequence
+// [pre-scale, sublattice pass, sublattice pass, post-scale] runs on blocks of
+// whole columns (left) or whole rows (right) staged in LDS: one read and one
+// write of the matrix instead of one per pass.  starts[] lists the sublattice
+// order (1 = B plaquettes at odd (x,y), 0 = A at even).
+__device__ void cb_pass_lds_left(double* T, int ld, int ncols, int side, const double P[3], int start) {
+    const int np = (side / 2) * (side / 2);
+    for (int e = tid(); e < np * ncols; e += NT) {
+        const int pq = e % np, col = e / np;
+        int idx[4]; plaq_sites(pq, side, start, idx);
+        double* c = T + (size_t)col * ld;
+        double v[4] = {c[idx[0]], c[idx[1]], c[idx[2]], c[idx[3]]}, o[4];
+        plaq_apply(P, v, o);
+        c[idx[0]] = o[0]; c[idx[1]] = o[1]; c[idx[2]] = o[2]; c[idx[3]] = o[3];
+    }
+    __syncthreads();
+}
+// T holds nr rows x N columns, element (r, c) at c * nr + r
+__device__ void cb_pass_lds_right(double* T, int nr, int side, const double P[3], int start) {
+    const int np = (side / 2) * (side / 2);
+    for (int e = tid(); e < np * nr; e += NT) {
+        const int r = e % nr, pq = e / nr;
+        int idx[4]; plaq_sites(pq, side, start, idx);
+        double v[4] = {T[idx[0] * nr + r], T[idx[1] * nr + r], T[idx[2] * nr + r], T[idx[3] * nr + r]}, o[4];
+        plaq_apply(P, v, o);
+        T[idx[0] * nr + r] = o[0]; T[idx[1] * nr + r] = o[1]; T[idx[2] * nr + r] = o[2]; T[idx[3] * nr + r] = o[3];
+    }
+    __syncthreads();
+}
+// M <- post .* E_s2 E_s1 (pre .* M)  (row scalings; pre/post may be null)
+__device__ __noinline__ void wg_cb_left_fused(double* M, int N, int side, const double P[3], int s1, int s2, const double* pre,
+                                 const double* post, Lds&) {
+    Lds& s = g_lds;
+    const int cb = max(1, min(N, PANEL_DOUBLES / N));
+    for (int c0 = 0; c0 < N; c0 += cb) {
+        const int nc = min(cb, N - c0);
+        for (int e = tid(); e < nc * N; e += NT) {
+            const int r = e % N;
+            const double v = M[(size_t)c0 * N + e];
+            s.panel[e] = pre ? v * pre[r] : v;
+        }
+        __syncthreads();
+        cb_pass_lds_left(s.panel, N, nc, side, P, s1);
+        cb_pass_lds_left(s.panel, N, nc, side, P, s2);
+        for (int e = tid(); e < nc * N; e += NT) {
+            const int r = e % N;
+            const double v = s.panel[e];
+            M[(size_t)c0 * N + e] = post ? v * post[r] : v;
+        }
+        __syncthreads();
+    }
+}
+// M <- (M .* pre) E_s1 E_s2 .* post  (column scalings)
+__device__ __noinline__ void wg_cb_right_fused(double* M, int N, int side, const double P[3], int s1, int s2, const double* pre,
+                                  const double* post, Lds&) {
+    Lds& s = g_lds;
+    const int rb = max(1, min(N, PANEL_DOUBLES / N));
+    for (int r0 = 0; r0 < N; r0 += rb) {
+        const int nr = min(rb, N - r0);
+        for (int e = tid(); e < nr * N; e += NT) {
+            const int r = e % nr, c = e / nr;
+            const double v = M[(size_t)c * N + r0 + r];
+            s.panel[c * nr + r] = pre ? v * pre[c] : v;
+        }
+        __syncthreads();
+        cb_pass_lds_right(s.panel, nr, side, P, s1);
+        cb_pass_lds_right(s.panel, nr, side, P, s2);
+        for (int e = tid(); e < nr * N; e += NT) {
+            const int r = e % nr, c = e / nr;
+            const double v = s.panel[c * nr + r];
+            M[(size_t)c * N + r0 + r] = post ? v * post[c] : v;
+        }
+        __syncthreads();
+    }
+}
+// whole wrap in LDS when the matrix fits: up  G <- B G B^-1, down  G <- B^-1 G B
+__device__ __noinline__ void wg_cb_wrap_lds(const ChainDesc& d, bool up, double* G, const double* ep, const double* en, Lds&) {
+    Lds& s = g_lds;
+    const int N = d.N, side = d.side;
+    for (int e = tid(); e < N * N; e += NT) s.panel[e] = G[e];
+    __syncthreads();
+    if (up) {
+        // B_left: E (B then A), rows * ep;  invB_right: E^-1 (B then A) on columns, columns * en
+        cb_pass_lds_left(s.panel, N, N, side, d.P, 1);
+        cb_pass_lds_left(s.panel, N, N, side, d.P, 0);
+        for (int e = tid(); e < N * N; e += NT) s.panel[e] *= ep[e % N];
+        __syncthreads();
+        cb_pass_lds_right(s.panel, N, side, d.Pi, 1);
+        cb_pass_lds_right(s.panel, N, side, d.Pi, 0);
+        for (int e = tid(); e < N * N; e += NT) G[e] = s.panel[e] * en[e / N];
+    } else {
+        // B_right: columns * ep, E (A then B);  invB_left: rows * en, E^-1 (A then B)
+        for (int e = tid(); e < N * N; e += NT) s.panel[e] *= ep[e / N];
+        __syncthreads();
+        cb_pass_lds_right(s.panel, N, side, d.P, 0);
+        cb_pass_lds_right(s.panel, N, side, d.P, 1);
+        for (int e = tid(); e < N * N; e += NT) s.panel[e] *= en[e % N];
+        __syncthreads();
+        cb_pass_lds_left(s.panel, N, N, side, d.Pi, 0);
+        cb_pass_lds_left(s.panel, N, N, side, d.Pi, 1);
+        for (int e = tid(); e < N * N; e += NT) G[e] = s.panel[e];
+    }
+    __syncthreads();
+}
+
 // Field-row scalings exp(+alpha s_i) / exp(-alpha s_i) into LDS.
-__device__ void load_field_row(const ChainDesc& d, int e, Lds& s) {
+__device__ void load_field_row(const ChainDesc& d, int e, Lds&) {
+    Lds& s = g_lds;
     for (int i = tid(); i < d.N; i += NT) {
         const int f = d.field[(size_t)e * d.N + i];
         s.frow[i] = f;
@@ -368,7 +551,8 @@ __device__ void load_field_row(const ChainDesc& d, int e, Lds& s) {
     __syncthreads();
 }
 // after Metropolis flipped entries of the LDS field row
-__device__ void refresh_scalings(const ChainDesc& d, Lds& s) {
+__device__ void refresh_scalings(const ChainDesc& d, Lds&) {
+    Lds& s = g_lds;
     for (int i = tid(); i < d.N; i += NT) {
         const int f = s.frow[i];
         s.scp[i] = f > 0 ? d.ev[0] : d.ev[1];
@@ -377,26 +561,29 @@ __device__ void refresh_scalings(const ChainDesc& d, Lds& s) {
     __syncthreads();
 }
 // exp(+sigma_eff alpha s) for spin index sp (0 up, 1 down); attractive ignores the spin
-__device__ __forceinline__ const double* vpos(const ChainDesc& d, int sp, Lds& s) {
+__device__ __forceinline__ const double* vpos(const ChainDesc& d, int sp, Lds&) {
+    Lds& s = g_lds;
     return (d.model == 1 || sp == 0) ? s.scp : s.scm;
 }
-__device__ __forceinline__ const double* vneg(const ChainDesc& d, int sp, Lds& s) {
+__device__ __forceinline__ const double* vneg(const ChainDesc& d, int sp, Lds&) {
+    Lds& s = g_lds;
     return (d.model == 1 || sp == 0) ? s.scm : s.scp;
 }
 
 // B(t) products on an N x N matrix M (in place; W is N*N scratch for the dense path).
 // op: 0 B_left, 1 B_right, 2 invB_left, 3 invB_right, 4 transB_left
-__device__ void wg_bmul(const ChainDesc& d, int op, int sp, double* M, double* W, Lds& s) {
+__device__ void wg_bmul(const ChainDesc& d, int op, int sp, double* M, double* W, Lds&) {
+    Lds& s = g_lds;
     const int N = d.N;
     const double* ep = vpos(d, sp, s);
     const double* en = vneg(d, sp, s);
     if (d.cb) {
         switch (op) {
-            case 0: wg_cb_left(M, N, d.side, d.P, 1); wg_cb_left(M, N, d.side, d.P, 0); wg_scale_rows(M, N, ep); break;
-            case 1: wg_scale_cols(M, N, ep); wg_cb_right(M, N, d.side, d.P, 0); wg_cb_right(M, N, d.side, d.P, 1); break;
-            case 2: wg_scale_rows(M, N, en); wg_cb_left(M, N, d.side, d.Pi, 0); wg_cb_left(M, N, d.side, d.Pi, 1); break;
-            case 3: wg_cb_right(M, N, d.side, d.Pi, 1); wg_cb_right(M, N, d.side, d.Pi, 0); wg_scale_cols(M, N, en); break;
-            case 4: wg_scale_rows(M, N, ep); wg_cb_left(M, N, d.side, d.P, 0); wg_cb_left(M, N, d.side, d.P, 1); break;
+            case 0: wg_cb_left_fused(M, N, d.side, d.P, 1, 0, nullptr, ep, s); break;
+            case 1: wg_cb_right_fused(M, N, d.side, d.P, 0, 1, ep, nullptr, s); break;
+            case 2: wg_cb_left_fused(M, N, d.side, d.Pi, 0, 1, en, nullptr, s); break;
+            case 3: wg_cb_right_fused(M, N, d.side, d.Pi, 1, 0, nullptr, en, s); break;
+            case 4: wg_cb_left_fused(M, N, d.side, d.P, 0, 1, ep, nullptr, s); break;
         }
         return;
     }
@@ -411,7 +598,8 @@ __device__ void wg_bmul(const ChainDesc& d, int op, int sp, double* M, double* W
 }
 
 // G <- B(t) G B(t)^-1 (up) or G <- B(t)^-1 G B(t) (down) for spin index sp
-__device__ void wg_wrap(const ChainDesc& d, int sp, bool up, double* G, double* W, Lds& s) {
+__device__ void wg_wrap(const ChainDesc& d, int sp, bool up, double* G, double* W, Lds&) {
+    Lds& s = g_lds;
     const int N = d.N;
     if (!d.cb) {
         const double* ep = vpos(d, sp, s);
@@ -425,6 +613,7 @@ __device__ void wg_wrap(const ChainDesc& d, int sp, bool up, double* G, double* 
         }
         return;
     }
+    if (N * N <= PANEL_DOUBLES) { wg_cb_wrap_lds(d, up, G, vpos(d, sp, s), vneg(d, sp, s), s); return; }
     if (up) { wg_bmul(d, 0, sp, G, W, s); wg_bmul(d, 3, sp, G, W, s); }
     else { wg_bmul(d, 1, sp, G, W, s); wg_bmul(d, 2, sp, G, W, s); }
 }
@@ -438,12 +627,15 @@ __device__ void wg_wrap(const ChainDesc& d, int sp, bool up, double* G, double* 
 // (right-looking, Golub & Van Loan Alg. 3.4.2).  Back substitution runs by
 // NB-row blocks with a GEMM for the rows above.  Returns the sign of det(A).
 // ---------------------------------------------------------------------------
-__device__ double wg_lu_solve(double* A, double* B, int N, int nrhs, Lds& s) {
+__device__ __noinline__ double wg_lu_solve(double* A, double* B, int N, int nrhs, Lds&) {
+    Lds& s = g_lds;
     double detsign = 1.0;
     double* P = s.panel;                      // m x jb, ld = m
     double* T11 = s.panel + NMAX * NB;        // NB x NB
     for (int k0 = 0; k0 < N; k0 += NB) {
         const int jb = min(NB, N - k0), m = N - k0;
+        SubClock pclk;
+        pclk.start();
         for (int e = tid(); e < m * jb; e += NT) {
             const int r = e % m, c = e / m;
             P[c * m + r] = A[(size_t)(k0 + c) * N + k0 + r];
@@ -479,6 +671,7 @@ __device__ double wg_lu_solve(double* A, double* B, int N, int nrhs, Lds& s) {
             for (int r = j + 1 + tid(); r < m; r += NT) P[j * m + r] /= piv;
             __syncthreads();
         }
+        pclk.stop(1);
         for (int e = tid(); e < m * jb; e += NT) {
             const int r = e % m, c = e / m;
             A[(size_t)(k0 + c) * N + k0 + r] = P[c * m + r];
@@ -506,11 +699,10 @@ __device__ double wg_lu_solve(double* A, double* B, int N, int nrhs, Lds& s) {
         __syncthreads();
         if (jb < m) {
             if (nright > 0)
-                gemm64(m - jb, nright, jb, P + jb, m, false, A + (size_t)(k0 + jb) * N + k0, N, false,
-                       A + (size_t)(k0 + jb) * N + k0 + jb, N, nullptr, nullptr, nullptr, 1.0, s, -1.0);
+                gemm64_lg(m - jb, nright, jb, P + jb, m, false, A + (size_t)(k0 + jb) * N + k0, N, false,
+                          A + (size_t)(k0 + jb) * N + k0 + jb, N, 1.0, s, -1.0);
             if (nrhs > 0)
-                gemm64(m - jb, nrhs, jb, P + jb, m, false, B + k0, N, false, B + k0 + jb, N, nullptr, nullptr, nullptr,
-                       1.0, s, -1.0);
+                gemm64_lg(m - jb, nrhs, jb, P + jb, m, false, B + k0, N, false, B + k0 + jb, N, 1.0, s, -1.0);
         }
         __syncthreads();
     }
@@ -556,15 +748,17 @@ __device__ double wg_lu_solve(double* A, double* B, int N, int nrhs, Lds& s) {
 // I - V T V^T (LAPACK dlarft, forward/columnwise) updates the trailing
 // columns and forms Q with FP64 MFMA GEMMs.
 // Buffers: M is overwritten by the reflectors V (unit lower, explicit),
-// Mp (N x N) receives M P and then R, Wq holds NB x N of GEMM scratch and
+// Mp (N x N) receives M P and then R, Wq holds 2 x NB x N of GEMM scratch and
 // the NB x NB T factor of every panel.
 // ---------------------------------------------------------------------------
-__device__ void wg_qr_udv(double* M, double* Mp, int N, double* U, double* D, double* Tl, double* Wq, Lds& s) {
+__device__ __noinline__ void wg_qr_udv(double* M, double* Mp, int N, double* U, double* D, double* Tl, double* Wq, Lds&) {
+    Lds& s = g_lds;
     double* P = s.panel;                  // m x jb panel, ld = m
     double* Tb = s.panel + NMAX * NB;     // NB x NB T of the current panel (ld NB)
     double* Gm = Tb + NB * NB;            // NB x NB Gram matrix V^T V
     double* Wsm = Wq;                     // NB x N GEMM scratch (ld NB)
-    double* Tg = Wq + (size_t)NB * N;     // T factors of all panels, NB x NB each
+    double* Wsm2 = Wq + (size_t)NB * N;   // NB x N GEMM scratch (ld NB)
+    double* Tg = Wq + (size_t)2 * NB * N; // T factors of all panels, NB x NB each
     double* tau = s.v1;
     // column norms and the sorting permutation
     for (int j = wave(); j < N; j += 4) {
@@ -591,6 +785,8 @@ __device__ void wg_qr_udv(double* M, double* Mp, int N, double* U, double* D, do
     __syncthreads();
     for (int k0 = 0; k0 < N; k0 += NB) {
         const int jb = min(NB, N - k0), m = N - k0;
+        SubClock pclk;
+        pclk.start();
         for (int e = tid(); e < m * jb; e += NT) {
             const int r = e % m, c = e / m;
             P[c * m + r] = Mp[(size_t)(k0 + c) * N + k0 + r];
@@ -663,29 +859,15 @@ __device__ void wg_qr_udv(double* M, double* Mp, int N, double* U, double* D, do
         }
         for (int e = tid(); e < NB * NB; e += NT) Tg[(size_t)(k0 / NB) * NB * NB + e] = Tb[e];
         __syncthreads();
+        pclk.stop(1);
         // trailing columns: A2 <- (I - V T^T V^T) A2
         const int n2 = N - k0 - jb;
         if (n2 > 0) {
             const double* V = M + (size_t)k0 * N + k0;
             double* A2 = Mp + (size_t)(k0 + jb) * N + k0;
             gemm64(jb, n2, m, V, N, true, A2, N, false, Wsm, NB, nullptr, nullptr, nullptr, 0.0, s);
-            for (int c = tid(); c < n2; c += NT) {
-                double x[NB], y[NB];
-#pragma unroll
-                for (int i = 0; i < NB; ++i) x[i] = (i < jb) ? Wsm[(size_t)c * NB + i] : 0.0;
-#pragma unroll
-                for (int i = 0; i < NB; ++i) {
-                    double acc = 0.0;
-#pragma unroll
-                    for (int l = 0; l <= i; ++l) acc += Tb[i * NB + l] * x[l];   // (T^T)(i,l) = T(l,i)
-                    y[i] = acc;
-                }
-#pragma unroll
-                for (int i = 0; i < NB; ++i)
-                    if (i < jb) Wsm[(size_t)c * NB + i] = y[i];
-            }
-            __syncthreads();
-            gemm64(m, n2, jb, V, N, false, Wsm, NB, false, A2, N, nullptr, nullptr, nullptr, 1.0, s, -1.0);
+            gemm64_lg(jb, n2, jb, Tb, NB, true, Wsm, NB, false, Wsm2, NB, 0.0, s);
+            gemm64(m, n2, jb, V, N, false, Wsm2, NB, false, A2, N, nullptr, nullptr, nullptr, 1.0, s, -1.0);
         }
     }
     // D and Tl = D^-1 R P^T
@@ -704,23 +886,8 @@ __device__ void wg_qr_udv(double* M, double* Mp, int N, double* U, double* D, do
         const double* T = Tg + (size_t)(k0 / NB) * NB * NB;
         double* U2 = U + (size_t)k0 * N + k0;
         gemm64(jb, m, m, V, N, true, U2, N, false, Wsm, NB, nullptr, nullptr, nullptr, 0.0, s);
-        for (int c = tid(); c < m; c += NT) {
-            double x[NB], y[NB];
-#pragma unroll
-            for (int i = 0; i < NB; ++i) x[i] = (i < jb) ? Wsm[(size_t)c * NB + i] : 0.0;
-#pragma unroll
-            for (int i = 0; i < NB; ++i) {
-                double acc = 0.0;
-#pragma unroll
-                for (int l = i; l < NB; ++l) acc += T[l * NB + i] * x[l];   // T(i,l), upper
-                y[i] = acc;
-            }
-#pragma unroll
-            for (int i = 0; i < NB; ++i)
-                if (i < jb) Wsm[(size_t)c * NB + i] = y[i];
-        }
-        __syncthreads();
-        gemm64(m, m, jb, V, N, false, Wsm, NB, false, U2, N, nullptr, nullptr, nullptr, 1.0, s, -1.0);
+        gemm64(jb, m, jb, T, NB, false, Wsm, NB, false, Wsm2, NB, nullptr, nullptr, nullptr, 0.0, s);
+        gemm64(m, m, jb, V, N, false, Wsm2, NB, false, U2, N, nullptr, nullptr, nullptr, 1.0, s, -1.0);
     }
 }
 
@@ -742,7 +909,8 @@ __device__ __forceinline__ double* slotD(const ChainDesc& d, int sp, int k) {
 }
 
 // push A onto the stack whose current top is slot `below` (-1 if empty); result at `slot`
-__device__ void wg_push(const ChainDesc& d, int sp, int slot, int below, const double* A, Lds& s) {
+__device__ __noinline__ void wg_push(const ChainDesc& d, int sp, int slot, int below, const double* A, Lds&) {
+    Lds& s = g_lds;
     const int N = d.N;
     double* W0 = d.W;
     double* W1 = d.W + (size_t)N * N;
@@ -755,7 +923,8 @@ __device__ void wg_push(const ChainDesc& d, int sp, int slot, int below, const d
 }
 
 // G = (1 + B_left B_right)^-1 from the stack tops (lslot/rslot = -1 when that stack is empty)
-__device__ void wg_fresh_green(const ChainDesc& d, int sp, int lslot, int rslot, double* Gout, Lds& s) {
+__device__ __noinline__ void wg_fresh_green(const ChainDesc& d, int sp, int lslot, int rslot, double* Gout, Lds&) {
+    Lds& s = g_lds;
     const int N = d.N;
     double* W0 = d.W;
     double* W1 = d.W + (size_t)N * N;
@@ -809,91 +978,175 @@ __device__ void wg_fresh_green(const ChainDesc& d, int sp, int lslot, int rslot,
 // G_cur = G0 + X Y^T; the diagonal of G_cur is kept in LDS; thread 0 draws
 // the decisions (it can walk over rejected sites without synchronization).
 // ---------------------------------------------------------------------------
-// delayed-update depth that fits the LDS panels (X and Y for every spin)
-__device__ __forceinline__ int delay_depth(const ChainDesc& d) {
-    const int cap = PANEL_DOUBLES / (2 * d.nspin * d.N);
-    return min(d.kdelay, cap);
-}
+// Submatrix updates (Nukala, Maier, Summers, Alvarez, Schulthess, PRB 80,
+// 195111 (2009)).  The sites of a slice are taken in blocks S of k
+// consecutive sites.  Within a block every accepted flip keeps
+//   G_cur = G0 + G0[:,S] C (G0[S,:] - E_S),   C (k x k), C <- 0 at block start,
+// so the decisions need only the k x k submatrix G0[S,S]:
+//   t = C (G0[S,S] - I) e_p,  G_cur(s_p,s_p) = G0(s_p,s_p) + G0[S,S](p,:) t,
+//   and an accepted flip with gamma does C <- C + gamma (e_p + t)(e_p + C^T G0[S,S](p,:)^T)^T,
+// which is the reference's rank-1 update G <- G - gamma G[:,i] (e_i^T - G[i,:])
+// (src/model/repulsive_hubbard.cpp:127-155) written in the block basis.
+// One wavefront walks the block (lane a owns row a and column a of C in LDS,
+// so the walk needs shuffles only); the block then lands in G with
+// W = C (G0[S,:] - E_S) and G += G0[:,S] W on the matrix cores.
+#define KB 32   // one half-wavefront per spin in the walk
 
-__device__ void wg_flush(const ChainDesc& d, int cnt, int k, Lds& s) {
-    const int N = d.N;
-    for (int sp = 0; sp < d.nspin; ++sp) {
-        const double* X = s.panel + (size_t)sp * 2 * N * k;
-        const double* Y = X + (size_t)N * k;
-        gemm64(N, N, cnt, X, N, false, Y, N, true, d.G + (size_t)sp * N * N, N, nullptr, nullptr, nullptr, 1.0, s);
-    }
-}
+__device__ __forceinline__ int block_depth(const ChainDesc& d) { return min(d.kdelay, KB); }
 
-__device__ void wg_metropolis(const ChainDesc& d, int e, uint64_t& rng, double& sign, bool record, int& naccept, Lds& s) {
-    const int N = d.N;
-    const int ns = d.nspin;
-    const int k = delay_depth(d);
-    double* dgs[2] = {s.du, ns == 2 ? s.dd : s.du};
-    for (int i = tid(); i < N; i += NT) {
-        s.du[i] = d.G[(size_t)i * N + i];
-        if (ns == 2) s.dd[i] = d.G[(size_t)N * N + (size_t)i * N + i];
-    }
-    __syncthreads();
-    int cnt = 0;
-    int i0 = 0;
-    while (true) {
-        if (tid() == 0) {
-            int j = i0;
-            for (; j < N; ++j) {
-                const int f = s.frow[j];
-                const int si = f > 0 ? 0 : 1;
-                const double du = dgs[0][j], dd = dgs[1][j];
-                const double r = d.pref[si] * (1.0 + (1.0 - du) * d.dm1[0][si]) * (1.0 + (1.0 - dd) * d.dm1[1][si]);
-                const double p = fmin(1.0, fabs(r));
-                const bool ok = canonical(rng) < p;
-                if (record) d.recAcc[(size_t)e * N + j] = ok ? 1 : 0;
-                if (ok) {
-                    sign = (r >= 0) ? sign : -sign;
-                    s.gam[0] = d.dm1[0][si] / (1.0 + (1.0 - du) * d.dm1[0][si]);
-                    s.gam[1] = d.dm1[1][si] / (1.0 + (1.0 - dd) * d.dm1[1][si]);
-                    s.frow[j] = -f;
-                    d.field[(size_t)e * N + j] = (int8_t)(-f);
-                    break;
-                }
-            }
-            s.ivar[0] = j;
-        }
-        __syncthreads();
-        const int j = s.ivar[0];
-        if (j >= N) break;
-        ++naccept;
+__device__ __noinline__ void wg_metropolis(const ChainDesc& d, int e, uint64_t& rng_io, double& sign_io, bool record,
+                                          int& naccept, Lds&) {
+    Lds& s = g_lds;
+    // uniform scalars of the descriptor in SGPRs, the chain's random state and
+    // sign in registers for the whole slice
+    const int N = __builtin_amdgcn_readfirstlane(d.N);
+    const int ns = __builtin_amdgcn_readfirstlane(d.nspin);
+    const double pref[2] = {d.pref[0], d.pref[1]};
+    const double dm1[2][2] = {{d.dm1[0][0], d.dm1[0][1]}, {d.dm1[1][0], d.dm1[1][1]}};
+    int8_t* const field_row = d.field + (size_t)e * N;
+    uint8_t* const rec_row = d.recAcc + (size_t)e * N;
+    uint64_t rng = rng_io;
+    double sign = sign_io;
+    constexpr int KP = KB + 1;
+    constexpr int SPIN_LDS = 3 * KB * KP;   // Gss, CR, CC of one spin
+    const int used = 2 * SPIN_LDS + 128;    // two spins and the u, v broadcast vectors
+    // block size: the delay depth, reduced when G0[:,S] and W must go to the
+    // Gw workspace and would not fit there (2 N k doubles per spin <= N^2)
+    int kb = block_depth(d);
+    if (used + 2 * ns * N * kb > PANEL_DOUBLES) kb = max(1, min(kb, N / 2));
+    kb = __builtin_amdgcn_readfirstlane(kb);
+    // LDS, per spin: Gss = G0[S,S] (KB x KP), CR (row a of C at lane a), CC (column a of C at lane a).
+    // Addresses are formed from g_lds directly so that every access is a ds_* instruction.
+#define GSS(sp) (s.panel + (sp) * SPIN_LDS)
+#define CRP(sp) (s.panel + (sp) * SPIN_LDS + KB * KP)
+#define CCP(sp) (s.panel + (sp) * SPIN_LDS + 2 * KB * KP)
+    // G0[:,S] copies (N x kb) and W (kb x N) per spin: LDS when they fit, else the Gw workspace
+    const bool in_lds = used + 2 * ns * N * kb <= PANEL_DOUBLES;
+    for (int b0 = 0; b0 < N; b0 += kb) {
+        const int k = min(kb, N - b0);
         for (int sp = 0; sp < ns; ++sp) {
-            const double g = s.gam[sp];
-            const double* G0 = d.G + (size_t)sp * N * N;
-            double* X = s.panel + (size_t)sp * 2 * N * k;
-            double* Y = X + (size_t)N * k;
-            double* dg = dgs[sp];
-            for (int a = tid(); a < N; a += NT) {
-                double x = G0[(size_t)j * N + a];
-                double y = G0[(size_t)a * N + j] - (a == j ? 1.0 : 0.0);
-                for (int m = 0; m < cnt; ++m) {
-                    x += X[m * N + a] * Y[m * N + j];
-                    y += X[m * N + j] * Y[m * N + a];
-                }
-                X[cnt * N + a] = g * x;
-                Y[cnt * N + a] = y;
-                dg[a] += g * x * y;
+            const double* G = d.G + (size_t)sp * N * N;
+            double* gss = GSS(sp);
+            double* cr = CRP(sp);
+            double* cc = CCP(sp);
+            for (int q = tid(); q < k * k; q += NT) {
+                const int a = q % k, c = q / k;
+                gss[a * KP + c] = G[(size_t)(b0 + c) * N + b0 + a];
+                cr[a * KP + c] = 0.0;
+                cc[a * KP + c] = 0.0;
             }
         }
-        ++cnt;
+        if (tid() == 0) s.ivar[2] = 0;
         __syncthreads();
-        if (cnt == k) { wg_flush(d, cnt, k, s); cnt = 0; }
-        i0 = j + 1;
-        if (i0 >= N) break;
+        SubClock walk_clk;
+        walk_clk.start();
+        if (wave() == 0) {
+            // lanes 0..31 carry spin up, lanes 32..63 spin down; lane h*32 + a
+            // owns site s_a of the block in spin h: its current diagonal dq,
+            // row a and column a of C.  Lanes with a >= k work on rows the
+            // block does not use, so no lane is masked off in the walk.
+            const int h = lane() >> 5, a = lane() & 31;
+            const int hb = h * SPIN_LDS;
+            double* const cr = s.panel + hb + KB * KP + a * KP;
+            double* const cc = s.panel + hb + 2 * KB * KP + a * KP;
+            const double* const gs = s.panel + hb;
+            double* const uv = s.panel + 2 * SPIN_LDS;   // u at [lane], v at [64 + lane]
+            double* const ush = uv + h * 32;
+            double* const vsh = uv + 64 + h * 32;
+            int nacc_blk = 0;
+            double dq = gs[a * KP + a];
+            for (int p = 0; p < k; ++p) {
+                const double dp0 = readlane_d(dq, p);
+                const double dp1 = ns == 2 ? readlane_d(dq, 32 + p) : dp0;
+                // the decision is computed by every lane on identical data
+                const int j = b0 + p;
+                const int f = __builtin_amdgcn_readfirstlane(s.frow[j]);
+                const int si = f > 0 ? 0 : 1;
+                const double x0 = 1.0 + (1.0 - dp0) * dm1[0][si];
+                const double x1 = 1.0 + (1.0 - dp1) * dm1[1][si];
+                const double r = pref[si] * x0 * x1;
+                const double pr = fmin(1.0, fabs(r));
+                const int ok = __builtin_amdgcn_readfirstlane(canonical(rng) < pr ? 1 : 0);
+                if (record && lane() == 0) rec_row[j] = (uint8_t)ok;
+                if (!ok) continue;
+                ++nacc_blk;
+                sign = (r >= 0) ? sign : -sign;
+                if (lane() == 0) {
+                    s.frow[j] = -f;
+                    field_row[j] = (int8_t)(-f);
+                }
+                const double g = h == 0 ? dm1[0][si] / x0 : dm1[1][si] / x1;
+                // u_a = d_ap + (C (Gss - I))(a,p),  v_a = d_ap + (Gss C)(p,a)
+                double t = 0.0, w = 0.0;
+#pragma unroll 8
+                for (int b = 0; b < k; ++b) {
+                    t += cr[b] * (gs[b * KP + p] - (b == p ? 1.0 : 0.0));
+                    w += gs[p * KP + b] * cc[b];
+                }
+                const double u = (a == p ? 1.0 : 0.0) + t;
+                const double v = (a == p ? 1.0 : 0.0) + w;
+                ush[a] = u;
+                vsh[a] = v;
+                __builtin_amdgcn_wave_barrier();
+                // C += g u v^T (row a and column a), and the block diagonal
+                // d_a += g (Gss u)_a (v^T (Gss - I))_a
+                double x = 0.0, y = 0.0;
+#pragma unroll 8
+                for (int b = 0; b < k; ++b) {
+                    const double ub = ush[b], vb = vsh[b];
+                    cr[b] += g * u * vb;
+                    cc[b] += g * ub * v;
+                    x += gs[a * KP + b] * ub;
+                    y += vb * (gs[b * KP + a] - (b == a ? 1.0 : 0.0));
+                }
+                __builtin_amdgcn_wave_barrier();
+                dq += g * x * y;
+            }
+            if (lane() == 0) s.ivar[2] = nacc_blk;
+        }
+        __syncthreads();
+        walk_clk.stop(0);
+        const int nblk = s.ivar[2];
+        naccept += nblk;
+        if (nblk == 0) continue;
+        // W = C (G0[S,:] - E_S) and the copy of G0[:,S], then G += G0[:,S] W
+        for (int sp = 0; sp < ns; ++sp) {
+            double* G = d.G + (size_t)sp * N * N;
+            double* acp = in_lds ? s.panel + used + (size_t)sp * 2 * N * kb : d.Gw + (size_t)sp * N * N;
+            double* wb = acp + (size_t)N * kb;
+            const double* cr = CRP(sp);
+            for (int q = tid(); q < N * k; q += NT) {
+                const int r = q % N, c = q / N;
+                acp[(size_t)c * N + r] = G[(size_t)(b0 + c) * N + r];
+            }
+            gemm64_lg(k, N, k, cr, KP, true, G + b0, N, false, wb, k, 0.0, s);
+            for (int q = tid(); q < k * k; q += NT) {
+                const int a = q % k, c = q / k;
+                wb[(size_t)(b0 + c) * k + a] -= cr[a * KP + c];
+            }
+        }
+        __syncthreads();
+        for (int sp = 0; sp < ns; ++sp) {
+            double* acp = in_lds ? s.panel + used + (size_t)sp * 2 * N * kb : d.Gw + (size_t)sp * N * N;
+            if (in_lds) gemm64_ll(N, N, k, acp, N, false, acp + (size_t)N * kb, k, false, d.G + (size_t)sp * N * N, N,
+                                  1.0, s);
+            else gemm64(N, N, k, acp, N, false, acp + (size_t)N * kb, k, false, d.G + (size_t)sp * N * N, N, nullptr,
+                        nullptr, nullptr, 1.0, s);
+        }
     }
-    if (cnt > 0) wg_flush(d, cnt, k, s);
+#undef GSS
+#undef CRP
+#undef CCP
+    rng_io = rng;
+    sign_io = sign;
     __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
 // Equal-time measurements of one slice (src/measure/measure_methods.cpp:22-255)
 // ---------------------------------------------------------------------------
-__device__ void wg_measure_from(const ChainDesc& d, const double* G, double cs, Lds& s) {
+__device__ __noinline__ void wg_measure_from(const ChainDesc& d, const double* G, double cs, Lds&) {
+    Lds& s = g_lds;
     const int N = d.N, side = d.side;
     const double* Gu = G;
     const double* Gd = d.nspin == 2 ? G + (size_t)N * N : G;
@@ -944,7 +1197,7 @@ __device__ void wg_measure_from(const ChainDesc& d, const double* G, double cs, 
     __syncthreads();
 }
 
-__device__ void wg_measure(const ChainDesc& d, double cs, Lds& s) { wg_measure_from(d, d.G, cs, s); }
+__device__ void wg_measure(const ChainDesc& d, double cs, Lds&) { Lds& s = g_lds; wg_measure_from(d, d.G, cs, s); }
 
 __device__ void record_slot(const ChainDesc& d, int slot, double sign, bool with_sign) {
     const int N = d.N;
@@ -955,7 +1208,8 @@ __device__ void record_slot(const ChainDesc& d, int slot, double sign, bool with
 }
 
 // fresh G for both spins into Gw, wrap error against G, then G <- Gw
-__device__ void stabilize_green(const ChainDesc& d, int lslot, int rslot, bool track_error, ChainState& st, Lds& s) {
+__device__ __noinline__ void stabilize_green(const ChainDesc& d, int lslot, int rslot, bool track_error, ChainState& st, Lds&) {
+    Lds& s = g_lds;
     const int N = d.N;
     const size_t n2 = (size_t)N * N;
     double err = 0.0;
@@ -986,7 +1240,8 @@ struct PhaseClock {
 // ---------------------------------------------------------------------------
 // sweeps (src/dqmc_walker.cpp:246-325 and :334-412)
 // ---------------------------------------------------------------------------
-__device__ void sweep_up(const ChainDesc& d, ChainState& st, uint64_t& rng, double& sign, int flags, int& nacc, Lds& s) {
+__device__ void sweep_up(const ChainDesc& d, ChainState& st, uint64_t& rng, double& sign, int flags, int& nacc, Lds&) {
+    Lds& s = g_lds;
     const int N = d.N, L = d.L;
     const size_t n2 = (size_t)N * N;
     const bool rec = flags & DQMC_FLAG_RECORD, meas = flags & DQMC_FLAG_MEASURE;
@@ -1019,7 +1274,8 @@ __device__ void sweep_up(const ChainDesc& d, ChainState& st, uint64_t& rng, doub
     st.pos = L;
 }
 
-__device__ void sweep_down(const ChainDesc& d, ChainState& st, uint64_t& rng, double& sign, int flags, int& nacc, Lds& s) {
+__device__ void sweep_down(const ChainDesc& d, ChainState& st, uint64_t& rng, double& sign, int flags, int& nacc, Lds&) {
+    Lds& s = g_lds;
     const int N = d.N, L = d.L;
     const size_t n2 = (size_t)N * N;
     const bool rec = flags & DQMC_FLAG_RECORD, meas = flags & DQMC_FLAG_MEASURE;
@@ -1063,11 +1319,13 @@ __device__ void sweep_down(const ChainDesc& d, ChainState& st, uint64_t& rng, do
 }
 
 __global__ void __launch_bounds__(NT) k_init(const ChainDesc* __restrict__ descs) {
-    __shared__ Lds s;
+    Lds& s = g_lds;
     const ChainDesc& d = descs[blockIdx.x];
     ChainState st = *d.st;
     const int N = d.N, L = d.L;
     const size_t n2 = (size_t)N * N;
+    if (tid() == 0) s.prof_on = 0;
+    __syncthreads();
     for (int sp = 0; sp < d.nspin; ++sp) wg_identity(d.Tmp + sp * n2, N);
     st.n_right = 0;
     for (int t = L; t >= 1; --t) {   // src/dqmc_walker.cpp:118-133
@@ -1101,7 +1359,7 @@ __global__ void __launch_bounds__(NT) k_init(const ChainDesc* __restrict__ descs
 }
 
 __global__ void __launch_bounds__(NT) k_sweep(const ChainDesc* __restrict__ descs, int n_sweeps, int flags) {
-    __shared__ Lds s;
+    Lds& s = g_lds;
     const ChainDesc& d = descs[blockIdx.x];
     ChainState st = *d.st;
     uint64_t rng = st.rng;
@@ -1109,6 +1367,11 @@ __global__ void __launch_bounds__(NT) k_sweep(const ChainDesc* __restrict__ desc
     int nacc = 0;
     if (d.launch_sweeps > 0) n_sweeps = d.launch_sweeps;
     if (d.launch_flags >= 0) flags = d.launch_flags;
+    if (tid() == 0) {
+        s.prof[0] = s.prof[1] = 0;
+        s.prof_on = (flags & DQMC_FLAG_PROFILE) ? 1 : 0;
+    }
+    __syncthreads();
     for (int k = 0; k < n_sweeps; ++k) {
         if (st.pos == 0) sweep_up(d, st, rng, sign, flags, nacc, s);
         else sweep_down(d, st, rng, sign, flags, nacc, s);
@@ -1117,6 +1380,8 @@ __global__ void __launch_bounds__(NT) k_sweep(const ChainDesc* __restrict__ desc
         st.rng = rng;
         st.sign = sign;
         st.accepted = nacc;
+        st.phase[DQMC_PH_WALK] += s.prof[0];
+        st.phase[DQMC_PH_PANEL] += s.prof[1];
         *d.st = st;
     }
 }
@@ -1124,7 +1389,7 @@ __global__ void __launch_bounds__(NT) k_sweep(const ChainDesc* __restrict__ desc
 // MeasureHandler::equaltime_measure over the slices kept by a RECORD sweep
 // (src/measure/measure_handler.cpp:136-150: every method loops over the L stored slices)
 __global__ void __launch_bounds__(NT) k_measure_recorded(const ChainDesc* __restrict__ descs) {
-    __shared__ Lds s;
+    Lds& s = g_lds;
     const ChainDesc& d = descs[blockIdx.x];
     const size_t slot_size = (size_t)d.nspin * d.N * d.N;
     for (int slot = 0; slot < d.L; ++slot) wg_measure_from(d, d.recG + slot * slot_size, d.recSign[slot], s);
@@ -1134,10 +1399,28 @@ __global__ void __launch_bounds__(NT) k_measure_recorded(const ChainDesc* __rest
 __global__ void __launch_bounds__(NT) k_gemm(int variant, int M, int Nc, int K, const double* A, int lda, int tA,
                                              const double* B, int ldb, int tB, double* C, int ldc, const double* rs,
                                              const double* ms, const double* cs, double beta) {
-    __shared__ Lds s;
-    if (variant == 0) wg_gemm(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, s);
-    else if (variant == 1) wg_gemm_mfma<false>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, s);
-    else wg_gemm_mfma<true>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, s);
+    Lds& s = g_lds;
+    if (variant == 0) { wg_gemm(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, s); return; }
+    // the matrix-core GEMM takes its middle scaling from LDS
+    if (ms) {
+        for (int i = tid(); i < K; i += NT) s.v0[i] = ms[i];
+        ms = s.v0;
+    }
+    // variants 3 and 4 stage A (and B) in LDS first, as the engine's in-LDS operands are
+    const size_t na = (size_t)M * K, nb = (size_t)K * Nc;
+    if (variant >= 3) {
+        for (size_t e = tid(); e < na; e += NT) s.panel[e] = A[e];
+        A = s.panel;
+    }
+    if (variant == 4) {
+        for (size_t e = tid(); e < nb; e += NT) s.panel[na + e] = B[e];
+        B = s.panel + na;
+    }
+    __syncthreads();
+    if (variant == 1) wg_gemm_mfma<false, false, false>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, s);
+    else if (variant == 2) wg_gemm_mfma<true, false, false>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, s);
+    else if (variant == 3) wg_gemm_mfma<false, true, false>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, s);
+    else wg_gemm_mfma<false, true, true>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, s);
 }
 
 }  // namespace dqmc

@@ -92,7 +92,7 @@ Layout chain_layout(int N, int L, int n_stack, int nspin, int kdelay, bool dense
     l.stkT = take((size_t)nspin * n_stack * n2);
     l.stkD = take((size_t)nspin * n_stack * N * sizeof(double));
     l.W = take(3 * n2);
-    l.Wq = take((size_t)2 * 16 * (N + 16) * sizeof(double));
+    l.Wq = take((size_t)3 * 16 * (N + 16) * sizeof(double));
     l.X = take((size_t)nspin * N * kdelay * sizeof(double));
     l.Y = take((size_t)nspin * N * kdelay * sizeof(double));
     l.expK = dense ? take(n2) : o;
@@ -559,7 +559,7 @@ int dqmc_get_info(dqmc_batch_t b, int c, dqmc_chain_info* out) {
     out->status = st.status;
     out->accepted = st.accepted;
     out->sweeps = b->chains[c].sweeps;
-    for (int q = 0; q < 6; ++q) out->phase_seconds[q] = (double)st.phase[q] * 1e-8;  // 100 MHz device wall clock
+    for (int q = 0; q < DQMC_PH_COUNT; ++q) out->phase_seconds[q] = (double)st.phase[q] * 1e-8;  // 100 MHz clock
     return DQMC_OK;
 }
 
@@ -597,7 +597,10 @@ int dqmc_clear_measure(dqmc_batch_t b) {
 
 int dqmc_k_gemm(dqmc_ctx_t ctx, int variant, int M, int Nc, int K, int tA, int tB, const double* A, const double* B,
                 double* C, double beta, const double* rs, const double* ms, const double* cs) {
-    if (!ctx || !A || !B || !C || M < 1 || Nc < 1 || K < 1 || variant < 0 || variant > 2) return DQMC_ERR_ARG;
+    if (!ctx || !A || !B || !C || M < 1 || Nc < 1 || K < 1 || variant < 0 || variant > 4) return DQMC_ERR_ARG;
+    if (variant >= 1 && ms && K > DQMC_NMAX) return DQMC_ERR_ARG;   // the middle scaling is staged in LDS
+    if (variant == 3 && (size_t)M * K > DQMC_PANEL_DOUBLES) return DQMC_ERR_ARG;
+    if (variant == 4 && (size_t)M * K + (size_t)K * Nc > DQMC_PANEL_DOUBLES) return DQMC_ERR_ARG;
     HIPCHK(ctx, hipSetDevice(ctx->device));
     const size_t na = (size_t)M * K, nb = (size_t)K * Nc, nc = (size_t)M * Nc;
     double *dA = nullptr, *dB = nullptr, *dC = nullptr, *dv = nullptr;

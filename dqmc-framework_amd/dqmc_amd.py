@@ -16,7 +16,7 @@ OK, ERR_ARG, ERR_HIP, ERR_MEMORY, ERR_STATE, ERR_NUMERIC = 0, -1, -2, -3, -4, -5
 REPULSIVE, ATTRACTIVE = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 SWEEP_MEASURE, SWEEP_RECORD, SWEEP_FROZEN, SWEEP_PROFILE = 1, 2, 4, 8
-PHASES = ["wrap", "metropolis", "product", "push", "fresh", "measure"]
+PHASES = ["wrap", "metropolis", "product", "push", "fresh", "measure", "walk", "panels"]
 OBS_NAMES = ["sign", "filling_number", "double_occupancy", "kinetic_energy", "momentum_distribution",
              "local_spin_corr", "spin_density_structure_factor", "charge_density_structure_factor",
              "s_wave_pairing_corr"]
@@ -41,7 +41,7 @@ class ChainParams(C.Structure):
 class ChainInfo(C.Structure):
     _fields_ = [("sign", C.c_double), ("wrap_error", C.c_double), ("rng", C.c_ulonglong), ("pos", C.c_int),
                 ("status", C.c_int), ("accepted", C.c_longlong), ("sweeps", C.c_longlong),
-                ("phase_seconds", C.c_double * 6)]
+                ("phase_seconds", C.c_double * 8)]
 
 
 class Job(C.Structure):

@@ -105,8 +105,10 @@ typedef struct {
     int status;              /* 0 ok, else numerical failure */
     long long accepted;      /* accepted flips in the last sweep launch */
     long long sweeps;        /* single-direction sweeps done since init */
-    double phase_seconds[6]; /* wrap, Metropolis, slice product, stack push, fresh G, measurement
-                                (accumulated over DQMC_SWEEP_PROFILE sweeps since init) */
+    double phase_seconds[8]; /* wrap, Metropolis, slice product, stack push, fresh G, measurement,
+                                and two parts of those: the Metropolis in-block walk, the QR/LU
+                                panel factorizations (accumulated over DQMC_SWEEP_PROFILE sweeps
+                                since init) */
 } dqmc_chain_info;
 
 /* context: device selection, error string */
@@ -219,7 +221,9 @@ int dqmc_server_destroy(dqmc_server_t s);
  * include/utils/numerical_stable.hpp:222-238):
  *   C = diag(rs) op(A) diag(ms) op(B) diag(cs) + beta C,  column-major,
  *   op(A) M x K, op(B) K x Nc; rs/ms/cs may be NULL.
- * variant 0: vector FP64, 1: MFMA FP64, 2: MFMA FP32 (inputs rounded to FP32). */
+ * variant 0: vector FP64, 1: MFMA FP64, 2: MFMA FP32 (inputs rounded to FP32),
+ * 3: MFMA FP64 with A staged in LDS (M*K <= 12288), 4: A and B staged in LDS
+ * (M*K + K*Nc <= 12288).  With ms, K <= 512 for variants 1-4. */
 int dqmc_k_gemm(dqmc_ctx_t ctx, int variant, int M, int Nc, int K, int transA, int transB, const double* A,
                 const double* B, double* C, double beta, const double* rs, const double* ms, const double* cs);
 

@@ -36,6 +36,32 @@ def test_gemm_variants(ctx, variant, shape, tA, tB):
     assert err < tol, err
 
 
+LDS_SHAPES = [(16, 144, 16), (32, 144, 32), (144, 36, 32), (7, 65, 3), (64, 100, 64), (16, 400, 16)]
+
+
+@pytest.mark.parametrize("variant", [3, 4])
+@pytest.mark.parametrize("shape", LDS_SHAPES, ids=[f"{m}x{n}x{k}" for m, n, k in LDS_SHAPES])
+@pytest.mark.parametrize("tA,tB", [(False, False), (True, False), (False, True)])
+def test_gemm_lds_operands(ctx, variant, shape, tA, tB):
+    """The MFMA GEMM with A (3) or A and B (4) in LDS, as the LU, QR and
+    block-update callers use it."""
+    M, Nc, K = shape
+    if variant == 4 and M * K + K * Nc > 12288:
+        pytest.skip("operands exceed the LDS panel")
+    print(f"[gpu-gemm] variant {variant} shape {shape} tA {tA} tB {tB}", flush=True)
+    rng = np.random.default_rng(M * 5 + Nc * 11 + K + variant)
+    A = rng.standard_normal((K, M) if tA else (M, K))
+    B = rng.standard_normal((Nc, K) if tB else (K, Nc))
+    C0 = rng.standard_normal((M, Nc))
+    ms = rng.uniform(0.5, 2, K)
+    opA = A.T if tA else A
+    opB = B.T if tB else B
+    ref = opA @ np.diag(ms) @ opB - 1.5 * C0
+    got = ctx.gemm(variant, A, B, C0, transA=tA, transB=tB, beta=-1.5, ms=ms)
+    scale = np.abs(opA) @ np.diag(ms) @ np.abs(opB) + 1.5 * np.abs(C0)
+    assert np.max(np.abs(got - ref) / scale) < 1e-14 * K
+
+
 def test_gemm_plain(ctx):
     rng = np.random.default_rng(0)
     A = rng.standard_normal((80, 80)); B = rng.standard_normal((80, 80))
