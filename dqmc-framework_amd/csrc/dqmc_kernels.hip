@@ -42,6 +42,7 @@ struct Lds {
     double v0[NMAX], v1[NMAX], v2[NMAX], v3[NMAX];  // stabilization / QR scratch vectors
     int perm[NMAX];
     int frow[NMAX];            // field row of the current slice
+    int swp_row[2 * NB], swp_src[2 * NB];   // the row interchanges of one LU panel, composed
     double panel[PANEL_DOUBLES];  // LU / QR panel, or the delayed-update panels X, Y
     double red[16];
     double gam[2];
@@ -82,6 +83,23 @@ __device__ __forceinline__ void wave_argmax(double& v, int& idx) {
         if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
     }
 }
+// uniform values in scalar registers: function arguments of device functions
+// arrive in vector registers and would otherwise count as divergent
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ bool uni(bool v) { return __builtin_amdgcn_readfirstlane((int)v) != 0; }
+__device__ __forceinline__ double uni(double v) {
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(bits & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(bits >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <class T> __device__ __forceinline__ T* uni(T* p) {
+    const unsigned long long u = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)(u & 0xffffffffull));
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+    return (T*)(((unsigned long long)hi << 32) | lo);
+}
+
 // value of lane l (uniform l) of a double, through the scalar unit
 __device__ __forceinline__ double readlane_d(double v, int l) {
     const long long bits = __double_as_longlong(v);
@@ -240,101 +258,139 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int Nc, int K, const double* A_
                                           int ldb, bool tB, double* C, int ldc, const double* rs, const double* ms_,
                                           const double* cs, double beta, Lds&, double alpha = 1.0) {
     Lds& s = g_lds;
+    M = uni(M); Nc = uni(Nc); K = uni(K); lda = uni(lda); ldb = uni(ldb); ldc = uni(ldc);
+    tA = uni(tA); tB = uni(tB); beta = uni(beta); alpha = uni(alpha);
+    A_ = uni(A_); B_ = uni(B_); C = uni(C); rs = uni(rs); ms_ = uni(ms_); cs = uni(cs);
     typedef typename MemSpace<aL>::cptr APtr;
     typedef typename MemSpace<bL>::cptr BPtr;
     const APtr A = (APtr)A_;
     const BPtr B = (BPtr)B_;
     const bool has_ms = ms_ != nullptr;
     const AS_L double* ms = (const AS_L double*)ms_;
-    const int l = lane(), w = wave();
+    const int l = lane();
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
-    const int tm = (M + 63) >> 6, tn = (Nc + 63) >> 6, ntile = tm * tn;
+    const int tm = (M + 63) >> 6, tn = (Nc + 63) >> 6;
+    const int nslab = (K + 15) >> 4;
+    const int total = tm * tn * nslab;     // (tile, slab) steps, slab fastest
     float* Af = reinterpret_cast<float*>(&s.As[0][0]);  // [16][64] floats in lowp mode
     float* Bf = reinterpret_cast<float*>(&s.Bs[0][0]);
-    // element q of this thread's share of a slab: A at (ma[q], ka[q]), B at (kb[q], nb[q])
-    int ma[4], ka[4], nb[4], kbq[4];
+    // this thread's 4 elements of a slab: A at (row ma + 16q', k ka + 4q) etc., as in-slab
+    // coordinates packed (m | k << 8) and the element offsets relative to the slab origin
+    int pa[4], pb[4], oa[4], ob[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int e = tid() + NT * q;
-        if (!tA) { ma[q] = e & 63; ka[q] = e >> 6; } else { ka[q] = e & 15; ma[q] = e >> 4; }
-        if (!tB) { kbq[q] = e & 15; nb[q] = e >> 4; } else { nb[q] = e & 63; kbq[q] = e >> 6; }
+        int m, k, n, kb;
+        if (!tA) { m = e & 63; k = e >> 6; } else { k = e & 15; m = e >> 4; }
+        if (!tB) { kb = e & 15; n = e >> 4; } else { n = e & 63; kb = e >> 6; }
+        pa[q] = m | (k << 8);
+        pb[q] = n | (kb << 8);
+        oa[q] = tA ? m * lda + k : k * lda + m;
+        ob[q] = tB ? kb * ldb + n : n * ldb + kb;
     }
+    // offset of the C element (row l/16 (+4e) or 4(l/16) (+e), column l%16) of a 16x16 block
+    const int oc = (l & 15) * ldc + (lowp ? 4 * (l >> 4) : (l >> 4));
     double va[4], vb[4];
-    auto load_slab = [&](int r0, int c0, int k0) {
+    auto load_step = [&](int i) {
+        const int t = i / nslab;
+        const int k0 = (i - t * nslab) << 4, r0 = (t % tm) << 6, c0 = (t / tm) << 6;
+        const int ba = tA ? r0 * lda + k0 : k0 * lda + r0;
+        const int bb = tB ? k0 * ldb + c0 : c0 * ldb + k0;
+        const int mlim = M - r0, nlim = Nc - c0, klim = K - k0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int gm = r0 + ma[q], gk = k0 + ka[q];
-            va[q] = (gm < M && gk < K) ? (tA ? A[(size_t)gm * lda + gk] : A[(size_t)gk * lda + gm]) : 0.0;
-            const int gn = c0 + nb[q], gk2 = k0 + kbq[q];
-            vb[q] = (gn < Nc && gk2 < K) ? (tB ? B[(size_t)gk2 * ldb + gn] : B[(size_t)gn * ldb + gk2]) : 0.0;
+            va[q] = ((pa[q] & 255) < mlim && (pa[q] >> 8) < klim) ? A[ba + oa[q]] : 0.0;
+            vb[q] = ((pb[q] & 255) < nlim && (pb[q] >> 8) < klim) ? B[bb + ob[q]] : 0.0;
         }
     };
-    if (ntile > 0) load_slab(0, 0, 0);
+    dbl4 acc[2][2];
+    flt4 accf[2][2];
+    double cpre[2][2][4];   // C of the tile when beta != 0, loaded before the last slab's products
+    if (total > 0) load_step(0);
 #pragma unroll 1
-    for (int tile = 0; tile < ntile; ++tile) {
-        const int r0 = (tile % tm) << 6, c0 = (tile / tm) << 6;
+    for (int i = 0; i < total; ++i) {
+        const int t = i / nslab;
+        const int ks = i - t * nslab;
+        const int k0 = ks << 4, r0 = (t % tm) << 6, c0 = (t / tm) << 6;
         bool blk[2][2];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
             for (int b = 0; b < 2; ++b) blk[a][b] = (r0 + wr + 16 * a < M) && (c0 + wc + 16 * b < Nc);
-        dbl4 acc[2][2];
-        flt4 accf[2][2];
+        if (ks == 0) {
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+            for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
-                accf[a][b] = flt4{0.f, 0.f, 0.f, 0.f};
-            }
-#pragma unroll 1
-        for (int k0 = 0; k0 < K; k0 += 16) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                double v = va[q];
-                if (has_ms && k0 + ka[q] < K) v *= ms[k0 + ka[q]];
-                if (lowp) { Af[ka[q] * 64 + ma[q]] = (float)v; Bf[kbq[q] * 64 + nb[q]] = (float)vb[q]; }
-                else { s.As[ka[q]][ma[q]] = v; s.Bs[kbq[q]][nb[q]] = vb[q]; }
-            }
-            __syncthreads();
-            if (k0 + 16 < K) load_slab(r0, c0, k0 + 16);
-            else if (tile + 1 < ntile) load_slab(((tile + 1) % tm) << 6, ((tile + 1) / tm) << 6, 0);
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-                const int k = kk * 4 + (l >> 4);
-                if (lowp) {
-                    const float a0 = Af[k * 64 + wr + (l & 15)], a1 = Af[k * 64 + wr + 16 + (l & 15)];
-                    const float b0 = Bf[k * 64 + wc + (l & 15)], b1 = Bf[k * 64 + wc + 16 + (l & 15)];
-                    if (blk[0][0]) accf[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, accf[0][0], 0, 0, 0);
-                    if (blk[0][1]) accf[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, accf[0][1], 0, 0, 0);
-                    if (blk[1][0]) accf[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, accf[1][0], 0, 0, 0);
-                    if (blk[1][1]) accf[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, accf[1][1], 0, 0, 0);
-                } else {
-                    const double a0 = s.As[k][wr + (l & 15)], a1 = s.As[k][wr + 16 + (l & 15)];
-                    const double b0 = s.Bs[k][wc + (l & 15)], b1 = s.Bs[k][wc + 16 + (l & 15)];
-                    if (blk[0][0]) acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-                    if (blk[0][1]) acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-                    if (blk[1][0]) acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-                    if (blk[1][1]) acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+                for (int b = 0; b < 2; ++b) {
+                    acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+                    accf[a][b] = flt4{0.f, 0.f, 0.f, 0.f};
                 }
-            }
-            __syncthreads();
         }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int m = pa[q] & 255, k = pa[q] >> 8, n = pb[q] & 255, kb = pb[q] >> 8;
+            double v = va[q];
+            if (has_ms && k0 + k < K) v *= ms[k0 + k];
+            if (lowp) { Af[k * 64 + m] = (float)v; Bf[kb * 64 + n] = (float)vb[q]; }
+            else { s.As[k][m] = v; s.Bs[kb][n] = vb[q]; }
+        }
+        __syncthreads();
+        // the next step's slab (possibly the next tile's first) is in flight during the products
+        if (i + 1 < total) load_step(i + 1);
+        const bool last = ks == nslab - 1;
+        const int rows = M - r0 - wr, cols = Nc - c0 - wc;   // extent of this wave's quarter
+        const int cbase = (c0 + wc) * ldc + r0 + wr;
+        if (last && beta != 0.0) {
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int row = rb * 16 + (lowp ? 4 * (l >> 4) + e : (l >> 4) + 4 * e);
+                        const int col = cb * 16 + (l & 15);
+                        cpre[rb][cb][e] = (row < rows && col < cols)
+                                              ? C[cbase + oc + cb * 16 * ldc + rb * 16 + (lowp ? e : 4 * e)]
+                                              : 0.0;
+                    }
+        }
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const int k = kk * 4 + (l >> 4);
+            if (lowp) {
+                const float a0 = Af[k * 64 + wr + (l & 15)], a1 = Af[k * 64 + wr + 16 + (l & 15)];
+                const float b0 = Bf[k * 64 + wc + (l & 15)], b1 = Bf[k * 64 + wc + 16 + (l & 15)];
+                if (blk[0][0]) accf[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, accf[0][0], 0, 0, 0);
+                if (blk[0][1]) accf[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, accf[0][1], 0, 0, 0);
+                if (blk[1][0]) accf[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, accf[1][0], 0, 0, 0);
+                if (blk[1][1]) accf[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, accf[1][1], 0, 0, 0);
+            } else {
+                const double a0 = s.As[k][wr + (l & 15)], a1 = s.As[k][wr + 16 + (l & 15)];
+                const double b0 = s.Bs[k][wc + (l & 15)], b1 = s.Bs[k][wc + 16 + (l & 15)];
+                if (blk[0][0]) acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+                if (blk[0][1]) acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+                if (blk[1][0]) acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+                if (blk[1][1]) acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+        if (!last) continue;
 #pragma unroll
         for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
             for (int cb = 0; cb < 2; ++cb) {
                 if (!blk[rb][cb]) continue;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int row = r0 + wr + rb * 16 + (lowp ? 4 * (l >> 4) + i : (l >> 4) + 4 * i);
-                    const int col = c0 + wc + cb * 16 + (l & 15);
-                    if (row >= M || col >= Nc) continue;
-                    double v = (lowp ? (double)accf[rb][cb][i] : acc[rb][cb][i]) * alpha;
-                    if (cs) v *= cs[col];
-                    if (rs) v *= rs[row];
-                    double* cp = C + (size_t)col * ldc + row;
-                    *cp = (beta != 0.0) ? v + beta * (*cp) : v;
+                for (int e = 0; e < 4; ++e) {
+                    const int row = rb * 16 + (lowp ? 4 * (l >> 4) + e : (l >> 4) + 4 * e);
+                    const int col = cb * 16 + (l & 15);
+                    if (row >= rows || col >= cols) continue;
+                    double v = (lowp ? (double)accf[rb][cb][e] : acc[rb][cb][e]) * alpha;
+                    if (cs) v *= cs[c0 + wc + col];
+                    if (rs) v *= rs[r0 + wr + row];
+                    double* cp = C + cbase + oc + cb * 16 * ldc + rb * 16 + (lowp ? e : 4 * e);
+                    *cp = (beta != 0.0) ? v + beta * cpre[rb][cb][e] : v;
                 }
             }
     }
@@ -372,8 +428,30 @@ __device__ __forceinline__ void gemmB(const ChainDesc& d, int M, int Nc, int K, 
     else wg_gemm_mfma<false, false, false>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, 0.0, s);
 }
 
-__device__ void wg_copy(double* dst, const double* src, int n2) {
-    for (int e = tid(); e < n2; e += NT) dst[e] = src[e];
+// Elementwise streams over e in [0, n): st(e, ld(e)), with U loads of each
+// thread issued before any of its stores, so a pass costs about one memory
+// latency per U elements instead of one per element (the compiler cannot
+// reorder a load above a store that may alias it).
+template <int U = 8, class Ld, class St>
+__device__ __forceinline__ void wg_stream(int n, Ld ld, St st) {
+    int e0 = tid();
+    for (; e0 + (U - 1) * NT < n; e0 += U * NT) {
+        double v[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) v[q] = ld(e0 + q * NT);
+#pragma unroll
+        for (int q = 0; q < U; ++q) st(e0 + q * NT, v[q]);
+    }
+    for (; e0 < n; e0 += NT) st(e0, ld(e0));
+}
+__device__ __forceinline__ const AS_G double* gptr(const double* p) { return (const AS_G double*)p; }
+__device__ __forceinline__ AS_G double* gptr(double* p) { return (AS_G double*)p; }
+
+// dst <- src, both in global memory
+__device__ __noinline__ void wg_copy(double* dst_, const double* src_, int n2) {
+    AS_G double* dst = gptr(uni(dst_));
+    const AS_G double* src = gptr(uni(src_));
+    wg_stream(uni(n2), [&](int e) { return src[e]; }, [&](int e, double v) { dst[e] = v; });
     __syncthreads();
 }
 __device__ void wg_identity(double* M, int N) {
@@ -467,21 +545,25 @@ __device__ void cb_pass_lds_right(double* T, int nr, int side, const double P[3]
 __device__ __noinline__ void wg_cb_left_fused(double* M, int N, int side, const double P[3], int s1, int s2, const double* pre,
                                  const double* post, Lds&) {
     Lds& s = g_lds;
+    N = uni(N);
+    AS_G double* Mg = gptr(uni(M));
+    // scalings live in LDS; test the generic pointers for null (a null generic
+    // pointer cast to the LDS address space is not the LDS null)
+    const bool has_pre = uni(pre) != nullptr, has_post = uni(post) != nullptr;
+    const AS_L double* prel = (const AS_L double*)uni(pre);
+    const AS_L double* postl = (const AS_L double*)uni(post);
     const int cb = max(1, min(N, PANEL_DOUBLES / N));
     for (int c0 = 0; c0 < N; c0 += cb) {
         const int nc = min(cb, N - c0);
-        for (int e = tid(); e < nc * N; e += NT) {
-            const int r = e % N;
-            const double v = M[(size_t)c0 * N + e];
-            s.panel[e] = pre ? v * pre[r] : v;
-        }
+        AS_G double* blk = Mg + (size_t)c0 * N;
+        wg_stream(nc * N, [&](int e) { return blk[e]; },
+                  [&](int e, double v) { s.panel[e] = has_pre ? v * prel[e % N] : v; });
         __syncthreads();
         cb_pass_lds_left(s.panel, N, nc, side, P, s1);
         cb_pass_lds_left(s.panel, N, nc, side, P, s2);
         for (int e = tid(); e < nc * N; e += NT) {
-            const int r = e % N;
             const double v = s.panel[e];
-            M[(size_t)c0 * N + e] = post ? v * post[r] : v;
+            blk[e] = has_post ? v * postl[e % N] : v;
         }
         __syncthreads();
     }
@@ -490,21 +572,25 @@ __device__ __noinline__ void wg_cb_left_fused(double* M, int N, int side, const 
 __device__ __noinline__ void wg_cb_right_fused(double* M, int N, int side, const double P[3], int s1, int s2, const double* pre,
                                   const double* post, Lds&) {
     Lds& s = g_lds;
+    N = uni(N);
+    AS_G double* Mg = gptr(uni(M));
+    // scalings live in LDS; test the generic pointers for null (a null generic
+    // pointer cast to the LDS address space is not the LDS null)
+    const bool has_pre = uni(pre) != nullptr, has_post = uni(post) != nullptr;
+    const AS_L double* prel = (const AS_L double*)uni(pre);
+    const AS_L double* postl = (const AS_L double*)uni(post);
     const int rb = max(1, min(N, PANEL_DOUBLES / N));
     for (int r0 = 0; r0 < N; r0 += rb) {
         const int nr = min(rb, N - r0);
-        for (int e = tid(); e < nr * N; e += NT) {
-            const int r = e % nr, c = e / nr;
-            const double v = M[(size_t)c * N + r0 + r];
-            s.panel[c * nr + r] = pre ? v * pre[c] : v;
-        }
+        wg_stream(nr * N, [&](int e) { const int r = e % nr, c = e / nr; return Mg[(size_t)c * N + r0 + r]; },
+                  [&](int e, double v) { s.panel[e] = has_pre ? v * prel[e / nr] : v; });
         __syncthreads();
         cb_pass_lds_right(s.panel, nr, side, P, s1);
         cb_pass_lds_right(s.panel, nr, side, P, s2);
         for (int e = tid(); e < nr * N; e += NT) {
             const int r = e % nr, c = e / nr;
-            const double v = s.panel[c * nr + r];
-            M[(size_t)c * N + r0 + r] = post ? v * post[c] : v;
+            const double v = s.panel[e];
+            Mg[(size_t)c * N + r0 + r] = has_post ? v * postl[c] : v;
         }
         __syncthreads();
     }
@@ -512,8 +598,11 @@ __device__ __noinline__ void wg_cb_right_fused(double* M, int N, int side, const
 // whole wrap in LDS when the matrix fits: up  G <- B G B^-1, down  G <- B^-1 G B
 __device__ __noinline__ void wg_cb_wrap_lds(const ChainDesc& d, bool up, double* G, const double* ep, const double* en, Lds&) {
     Lds& s = g_lds;
-    const int N = d.N, side = d.side;
-    for (int e = tid(); e < N * N; e += NT) s.panel[e] = G[e];
+    const int N = uni(d.N), side = uni(d.side);
+    {
+        const AS_G double* Gg = gptr(uni(G));
+        wg_stream(N * N, [&](int e) { return Gg[e]; }, [&](int e, double v) { s.panel[e] = v; });
+    }
     __syncthreads();
     if (up) {
         // B_left: E (B then A), rows * ep;  invB_right: E^-1 (B then A) on columns, columns * en
@@ -632,14 +721,17 @@ __device__ __noinline__ double wg_lu_solve(double* A, double* B, int N, int nrhs
     double detsign = 1.0;
     double* P = s.panel;                      // m x jb, ld = m
     double* T11 = s.panel + NMAX * NB;        // NB x NB
+    N = uni(N);
+    nrhs = uni(nrhs);
+    A = uni(A);
+    B = uni(B);
+    const AS_G double* Ag = gptr(A);
     for (int k0 = 0; k0 < N; k0 += NB) {
         const int jb = min(NB, N - k0), m = N - k0;
         SubClock pclk;
         pclk.start();
-        for (int e = tid(); e < m * jb; e += NT) {
-            const int r = e % m, c = e / m;
-            P[c * m + r] = A[(size_t)(k0 + c) * N + k0 + r];
-        }
+        wg_stream(m * jb, [&](int e) { const int r = e % m, c = e / m; return Ag[(size_t)(k0 + c) * N + k0 + r]; },
+                  [&](int e, double v) { P[e] = v; });
         __syncthreads();
         for (int j = 0; j < jb; ++j) {
             if (wave() == 0) {
@@ -676,17 +768,31 @@ __device__ __noinline__ double wg_lu_solve(double* A, double* B, int N, int nrhs
             const int r = e % m, c = e / m;
             A[(size_t)(k0 + c) * N + k0 + r] = P[c * m + r];
         }
-        // interchanges + L11^-1 on the columns right of the panel and on B
-        const int nright = N - k0 - jb;
-        for (int c = tid(); c < nright + nrhs; c += NT) {
-            double* col = c < nright ? A + (size_t)(k0 + jb + c) * N : B + (size_t)(c - nright) * N;
+        // the panel's interchanges (row k0+j <-> perm[k0+j], j = 0..jb-1 in order) composed into
+        // one gather: row swp_row[i] receives the old row swp_src[i]; rows k0..k0+jb-1 come first
+        if (tid() == 0) {
+            int nrow = jb;
+            for (int j = 0; j < jb; ++j) { s.swp_row[j] = k0 + j; s.swp_src[j] = k0 + j; }
             for (int j = 0; j < jb; ++j) {
                 const int pr = s.perm[k0 + j];
-                if (pr != k0 + j) { const double t = col[k0 + j]; col[k0 + j] = col[pr]; col[pr] = t; }
+                int ip = -1;
+                for (int i = 0; i < nrow; ++i) if (s.swp_row[i] == pr) ip = i;
+                if (ip < 0) { ip = nrow++; s.swp_row[ip] = pr; s.swp_src[ip] = pr; }
+                const int t = s.swp_src[j]; s.swp_src[j] = s.swp_src[ip]; s.swp_src[ip] = t;
             }
-            double x[NB];
+            for (int i = nrow; i < 2 * NB; ++i) s.swp_row[i] = -1;
+        }
+        __syncthreads();
+        // interchanges + L11^-1 on the columns right of the panel and on B, one column per thread
+        const int nright = N - k0 - jb;
+        for (int c = tid(); c < nright + nrhs; c += NT) {
+            AS_G double* col = gptr(c < nright ? A + (size_t)(k0 + jb + c) * N : B + (size_t)(c - nright) * N);
+            double x[2 * NB];
 #pragma unroll
-            for (int j = 0; j < NB; ++j) x[j] = (j < jb) ? col[k0 + j] : 0.0;
+            for (int i = 0; i < 2 * NB; ++i) x[i] = s.swp_row[i] >= 0 ? col[s.swp_src[i]] : 0.0;
+#pragma unroll
+            for (int i = 0; i < 2 * NB; ++i)
+                if (i >= jb && s.swp_row[i] >= 0) col[s.swp_row[i]] = x[i];
 #pragma unroll
             for (int j = 0; j < NB; ++j)
 #pragma unroll
@@ -760,37 +866,45 @@ __device__ __noinline__ void wg_qr_udv(double* M, double* Mp, int N, double* U, 
     double* Wsm2 = Wq + (size_t)NB * N;   // NB x N GEMM scratch (ld NB)
     double* Tg = Wq + (size_t)2 * NB * N; // T factors of all panels, NB x NB each
     double* tau = s.v1;
-    // column norms and the sorting permutation
+    N = uni(N);
+    const AS_G double* Mg = gptr(uni(M));
+    AS_G double* Mpg = gptr(uni(Mp));
+    // column norms (all of a lane's <= NMAX/64 elements in flight) and the sorting permutation
     for (int j = wave(); j < N; j += 4) {
+        double v[NMAX / 64];
+#pragma unroll
+        for (int q = 0; q < NMAX / 64; ++q) {
+            const int i = lane() + 64 * q;
+            v[q] = i < N ? Mg[(size_t)j * N + i] : 0.0;
+        }
         double acc = 0.0;
-        for (int i = lane(); i < N; i += 64) { const double v = M[(size_t)j * N + i]; acc += v * v; }
+#pragma unroll
+        for (int q = 0; q < NMAX / 64; ++q) acc += v[q] * v[q];
         acc = wave_sum(acc);
         if (lane() == 0) s.v0[j] = acc;
     }
     __syncthreads();
+    // rank by decreasing norm, ties by index; a NaN norm ranks as +inf so the
+    // ranks always form a permutation
     for (int j = tid(); j < N; j += NT) {
-        const double nj = s.v0[j];
+        const double nj = isnan(s.v0[j]) ? INFINITY : s.v0[j];
         int r = 0;
         for (int i = 0; i < N; ++i) {
-            const double ni = s.v0[i];
+            const double ni = isnan(s.v0[i]) ? INFINITY : s.v0[i];
             r += (ni > nj) || (ni == nj && i < j);
         }
         s.perm[r] = j;
     }
     __syncthreads();
-    for (int e = tid(); e < N * N; e += NT) {
-        const int r = e % N, c = e / N;
-        Mp[e] = M[(size_t)s.perm[c] * N + r];
-    }
+    wg_stream(N * N, [&](int e) { const int r = e % N, c = e / N; return Mg[(size_t)s.perm[c] * N + r]; },
+              [&](int e, double v) { Mpg[e] = v; });
     __syncthreads();
     for (int k0 = 0; k0 < N; k0 += NB) {
         const int jb = min(NB, N - k0), m = N - k0;
         SubClock pclk;
         pclk.start();
-        for (int e = tid(); e < m * jb; e += NT) {
-            const int r = e % m, c = e / m;
-            P[c * m + r] = Mp[(size_t)(k0 + c) * N + k0 + r];
-        }
+        wg_stream(m * jb, [&](int e) { const int r = e % m, c = e / m; return Mpg[(size_t)(k0 + c) * N + k0 + r]; },
+                  [&](int e, double v) { P[e] = v; });
         __syncthreads();
         for (int j = 0; j < jb; ++j) {
             double acc = 0.0;
@@ -870,12 +984,17 @@ __device__ __noinline__ void wg_qr_udv(double* M, double* Mp, int N, double* U, 
             gemm64(m, n2, jb, V, N, false, Wsm2, NB, false, A2, N, nullptr, nullptr, nullptr, 1.0, s, -1.0);
         }
     }
-    // D and Tl = D^-1 R P^T
-    for (int k = tid(); k < N; k += NT) D[k] = fabs(Mp[(size_t)k * N + k]);
+    // D and Tl = D^-1 R P^T (D also kept in LDS for the division)
+    for (int k = tid(); k < N; k += NT) {
+        const double dk = fabs(Mpg[(size_t)k * N + k]);
+        D[k] = dk;
+        s.v2[k] = dk;
+    }
     __syncthreads();
-    for (int e = tid(); e < N * N; e += NT) {
-        const int r = e % N, c = e / N;
-        Tl[(size_t)s.perm[c] * N + r] = (c >= r) ? Mp[(size_t)c * N + r] / D[r] : 0.0;
+    {
+        AS_G double* Tlg = gptr(uni(Tl));
+        wg_stream(N * N, [&](int e) { const int r = e % N, c = e / N; return c >= r ? Mpg[e] / s.v2[r] : 0.0; },
+                  [&](int e, double v) { const int r = e % N, c = e / N; Tlg[(size_t)s.perm[c] * N + r] = v; });
     }
     // Q = H_0 H_1 ... : U <- (I - V T V^T) U, panels from the last
     for (int e = tid(); e < N * N; e += NT) U[e] = ((e % N) == (e / N)) ? 1.0 : 0.0;
@@ -922,6 +1041,46 @@ __device__ __noinline__ void wg_push(const ChainDesc& d, int sp, int slot, int b
     else gemm64(N, N, N, W1, N, false, slotT(d, sp, below), N, false, slotT(d, sp, slot), N, nullptr, nullptr, nullptr, 0.0, s);
 }
 
+// dst1(r,c) = sc[r] src(c,r) (transpose, optional row scaling) and, with dst2,
+// dst2(r,c) = dst1(r,c) + sa[r] add(r,c).  Column blocks of src are staged in
+// LDS with coalesced reads, so both the reads and the writes run along columns.
+// sc, sa live in LDS; all matrices are N x N in global memory.
+__device__ __noinline__ void wg_transpose_scaled(double* dst1_, double* dst2_, const double* src_, const double* add_,
+                                                 int N, const double* sc_, const double* sa_, Lds&) {
+    Lds& s = g_lds;
+    N = uni(N);
+    AS_G double* dst1 = gptr(uni(dst1_));
+    AS_G double* dst2 = gptr(uni(dst2_));
+    const AS_G double* src = gptr(uni(src_));
+    const AS_G double* add = gptr(uni(add_));
+    // LDS views of the scalings; the null tests below use the generic pointers
+    const AS_L double* sc = (const AS_L double*)uni(sc_);
+    const AS_L double* sa = (const AS_L double*)uni(sa_);
+    const int nb = max(1, min(N, PANEL_DOUBLES / N));
+    for (int j0 = 0; j0 < N; j0 += nb) {
+        const int nc = min(nb, N - j0);
+        const AS_G double* blk = src + (size_t)j0 * N;
+        wg_stream(nc * N, [&](int e) { return blk[e]; }, [&](int e, double v) { s.panel[e] = v; });
+        __syncthreads();
+        // element (r, c) of dst with r = j0 + e % nc, c = e / nc is src(c, r) = panel[c + (r - j0) N]
+        if (dst2_) {
+            wg_stream(nc * N, [&](int e) { const int r = j0 + e % nc, c = e / nc; return add[(size_t)c * N + r]; },
+                      [&](int e, double a) {
+                          const int rr = e % nc, r = j0 + rr, c = e / nc;
+                          const double v = (sc_ ? sc[r] : 1.0) * s.panel[c + rr * N];
+                          dst1[(size_t)c * N + r] = v;
+                          dst2[(size_t)c * N + r] = v + sa[r] * a;
+                      });
+        } else {
+            for (int e = tid(); e < nc * N; e += NT) {
+                const int rr = e % nc, r = j0 + rr, c = e / nc;
+                dst1[(size_t)c * N + r] = (sc_ ? sc[r] : 1.0) * s.panel[c + rr * N];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // G = (1 + B_left B_right)^-1 from the stack tops (lslot/rslot = -1 when that stack is empty)
 __device__ __noinline__ void wg_fresh_green(const ChainDesc& d, int sp, int lslot, int rslot, double* Gout, Lds&) {
     Lds& s = g_lds;
@@ -942,8 +1101,7 @@ __device__ __noinline__ void wg_fresh_green(const ChainDesc& d, int sp, int lslo
         gemm64(N, N, N, UL, N, true, UR, N, false, W0, N, s.v0, nullptr, s.v2, 0.0, s);
         gemm64(N, N, N, TL, N, false, TR, N, true, W0, N, s.v1, nullptr, s.v3, 1.0, s);
         // W1 = dlmax^-1 UL^T
-        for (int e = tid(); e < N * N; e += NT) { const int r = e % N, c = e / N; W1[e] = s.v0[r] * UL[(size_t)r * N + c]; }
-        __syncthreads();
+        wg_transpose_scaled(W1, nullptr, UL, nullptr, N, s.v0, nullptr, s);
         wg_lu_solve(W0, W1, N, N, s);
         // G = UR drmax^-1 W1
         gemm64(N, N, N, UR, N, false, W1, N, false, Gout, N, nullptr, s.v2, nullptr, 0.0, s);
@@ -958,19 +1116,11 @@ __device__ __noinline__ void wg_fresh_green(const ChainDesc& d, int sp, int lslo
         s.v1[i] = x > 1.0 ? 1.0 : x;
     }
     __syncthreads();
-    for (int e = tid(); e < N * N; e += NT) {
-        const int r = e % N, c = e / N;
-        const double ut = s.v0[r] * U[(size_t)r * N + c];
-        W1[e] = ut;
-        W0[e] = ut + s.v1[r] * T[e];
-    }
-    __syncthreads();
+    // W1 = Db U^T, W0 = W1 + Ds T
+    wg_transpose_scaled(W1, W0, U, T, N, s.v0, s.v1, s);
     wg_lu_solve(W0, W1, N, N, s);
     if (lslot >= 0) wg_copy(Gout, W1, N * N);
-    else {  // the right stack holds the transposed product: G = X^T
-        for (int e = tid(); e < N * N; e += NT) { const int r = e % N, c = e / N; Gout[e] = W1[(size_t)r * N + c]; }
-        __syncthreads();
-    }
+    else wg_transpose_scaled(Gout, nullptr, W1, nullptr, N, nullptr, nullptr, s);  // right stack: G = X^T
 }
 
 // ---------------------------------------------------------------------------
@@ -1029,12 +1179,14 @@ __device__ __noinline__ void wg_metropolis(const ChainDesc& d, int e, uint64_t& 
             double* gss = GSS(sp);
             double* cr = CRP(sp);
             double* cc = CCP(sp);
-            for (int q = tid(); q < k * k; q += NT) {
-                const int a = q % k, c = q / k;
-                gss[a * KP + c] = G[(size_t)(b0 + c) * N + b0 + a];
-                cr[a * KP + c] = 0.0;
-                cc[a * KP + c] = 0.0;
-            }
+            const AS_G double* Gg = gptr(G);
+            wg_stream(k * k, [&](int q) { const int a = q % k, c = q / k; return Gg[(size_t)(b0 + c) * N + b0 + a]; },
+                      [&](int q, double v) {
+                          const int a = q % k, c = q / k;
+                          gss[a * KP + c] = v;
+                          cr[a * KP + c] = 0.0;
+                          cc[a * KP + c] = 0.0;
+                      });
         }
         if (tid() == 0) s.ivar[2] = 0;
         __syncthreads();
@@ -1115,15 +1267,11 @@ __device__ __noinline__ void wg_metropolis(const ChainDesc& d, int e, uint64_t& 
             double* acp = in_lds ? s.panel + used + (size_t)sp * 2 * N * kb : d.Gw + (size_t)sp * N * N;
             double* wb = acp + (size_t)N * kb;
             const double* cr = CRP(sp);
-            for (int q = tid(); q < N * k; q += NT) {
-                const int r = q % N, c = q / N;
-                acp[(size_t)c * N + r] = G[(size_t)(b0 + c) * N + r];
-            }
+            const AS_G double* Gcols = gptr(G) + (size_t)b0 * N;
+            wg_stream(N * k, [&](int q) { return Gcols[q]; }, [&](int q, double v) { acp[q] = v; });
             gemm64_lg(k, N, k, cr, KP, true, G + b0, N, false, wb, k, 0.0, s);
-            for (int q = tid(); q < k * k; q += NT) {
-                const int a = q % k, c = q / k;
-                wb[(size_t)(b0 + c) * k + a] -= cr[a * KP + c];
-            }
+            wg_stream(k * k, [&](int q) { const int a = q % k, c = q / k; return wb[(size_t)(b0 + c) * k + a] - cr[a * KP + c]; },
+                      [&](int q, double v) { const int a = q % k, c = q / k; wb[(size_t)(b0 + c) * k + a] = v; });
         }
         __syncthreads();
         for (int sp = 0; sp < ns; ++sp) {
@@ -1162,13 +1310,29 @@ __device__ __noinline__ void wg_measure_from(const ChainDesc& d, const double* G
         v[2] += Gu[(size_t)jx * N + a] + Gd[(size_t)jx * N + a] + Gu[(size_t)jy * N + a] + Gd[(size_t)jy * N + a];
     }
     __syncthreads();
-    for (int p = tid(); p < N * N; p += NT) {
+    const AS_G double* Gug = gptr(Gu);
+    const AS_G double* Gdg = gptr(Gd);
+    const AS_G double* cosq = gptr(d.cosq);
+    constexpr int MU = 4;   // elements per thread with their loads in flight together
+    for (int p0 = tid(); p0 < N * N; p0 += MU * NT) {
+      double q_gu_ij[MU], q_gd_ij[MU], q_gu_ji[MU], q_gd_ji[MU], q_ff[MU];
+#pragma unroll
+      for (int u = 0; u < MU; ++u) {
+        const int p = min(p0 + u * NT, N * N - 1);
         const int i = p % N, j = p / N;
         const int xi = i % side, yi = i / side, xj = j % side, yj = j / side;
         const int dsp = (xj - xi + side) % side + side * ((yj - yi + side) % side);
-        const double ff = d.cosq[dsp];
-        const double gu_ij = Gu[(size_t)j * N + i], gd_ij = Gd[(size_t)j * N + i];
-        const double gu_ji = Gu[(size_t)i * N + j], gd_ji = Gd[(size_t)i * N + j];
+        q_ff[u] = cosq[dsp];
+        q_gu_ij[u] = Gug[(size_t)j * N + i]; q_gd_ij[u] = Gdg[(size_t)j * N + i];
+        q_gu_ji[u] = Gug[(size_t)i * N + j]; q_gd_ji[u] = Gdg[(size_t)i * N + j];
+      }
+#pragma unroll
+      for (int u = 0; u < MU; ++u) {
+        const int p = p0 + u * NT;
+        if (p >= N * N) break;
+        const int i = p % N, j = p / N;
+        const double ff = q_ff[u];
+        const double gu_ij = q_gu_ij[u], gd_ij = q_gd_ij[u], gu_ji = q_gu_ji[u], gd_ji = q_gd_ji[u];
         const double dij = (i == j) ? 1.0 : 0.0;
         const double guc_ij = dij - gu_ji, gdc_ij = dij - gd_ji;
         const double guc_ii = 1.0 - s.du[i], gdc_ii = 1.0 - s.dd[i];
@@ -1179,6 +1343,7 @@ __device__ __noinline__ void wg_measure_from(const ChainDesc& d, const double* G
         v[5] += ff * (common - cross);
         v[6] += ff * (common + cross);
         v[7] += guc_ij * gdc_ij;
+      }
     }
     wg_sum8(v, 8, s);
     if (tid() == 0) {
@@ -1202,7 +1367,9 @@ __device__ void wg_measure(const ChainDesc& d, double cs, Lds&) { Lds& s = g_lds
 __device__ void record_slot(const ChainDesc& d, int slot, double sign, bool with_sign) {
     const int N = d.N;
     const size_t n2 = (size_t)N * N;
-    for (size_t e = tid(); e < n2 * d.nspin; e += NT) d.recG[(size_t)slot * d.nspin * n2 + e] = d.G[e];
+    AS_G double* dst = gptr(d.recG + (size_t)slot * d.nspin * n2);
+    const AS_G double* src = gptr(d.G);
+    wg_stream((int)(n2 * d.nspin), [&](int e) { return src[e]; }, [&](int e, double v) { dst[e] = v; });
     if (with_sign && tid() == 0) d.recSign[slot] = sign;
     __syncthreads();
 }
@@ -1216,8 +1383,9 @@ __device__ __noinline__ void stabilize_green(const ChainDesc& d, int lslot, int 
     for (int sp = 0; sp < d.nspin; ++sp) {
         double* Gf = d.Gw + sp * n2;
         wg_fresh_green(d, sp, lslot, rslot, Gf, s);
-        const double* G = d.G + sp * n2;
-        for (size_t e = tid(); e < n2; e += NT) err = fmax(err, fabs(Gf[e] - G[e]));
+        const AS_G double* Gfg = gptr(Gf);
+        const AS_G double* Gg = gptr(d.G + sp * n2);
+        wg_stream((int)n2, [&](int e) { return fabs(Gfg[e] - Gg[e]); }, [&](int, double v) { err = fmax(err, v); });
     }
     err = wg_max(err, s);
     if (track_error && tid() == 0 && err > st.wrap_error) st.wrap_error = err;
@@ -1318,7 +1486,7 @@ __device__ void sweep_down(const ChainDesc& d, ChainState& st, uint64_t& rng, do
     st.pos = 0;
 }
 
-__global__ void __launch_bounds__(NT) k_init(const ChainDesc* __restrict__ descs) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) k_init(const ChainDesc* __restrict__ descs) {
     Lds& s = g_lds;
     const ChainDesc& d = descs[blockIdx.x];
     ChainState st = *d.st;
@@ -1358,7 +1526,7 @@ __global__ void __launch_bounds__(NT) k_init(const ChainDesc* __restrict__ descs
     }
 }
 
-__global__ void __launch_bounds__(NT) k_sweep(const ChainDesc* __restrict__ descs, int n_sweeps, int flags) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) k_sweep(const ChainDesc* __restrict__ descs, int n_sweeps, int flags) {
     Lds& s = g_lds;
     const ChainDesc& d = descs[blockIdx.x];
     ChainState st = *d.st;
@@ -1388,7 +1556,7 @@ __global__ void __launch_bounds__(NT) k_sweep(const ChainDesc* __restrict__ desc
 
 // MeasureHandler::equaltime_measure over the slices kept by a RECORD sweep
 // (src/measure/measure_handler.cpp:136-150: every method loops over the L stored slices)
-__global__ void __launch_bounds__(NT) k_measure_recorded(const ChainDesc* __restrict__ descs) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) k_measure_recorded(const ChainDesc* __restrict__ descs) {
     Lds& s = g_lds;
     const ChainDesc& d = descs[blockIdx.x];
     const size_t slot_size = (size_t)d.nspin * d.N * d.N;
@@ -1396,7 +1564,7 @@ __global__ void __launch_bounds__(NT) k_measure_recorded(const ChainDesc* __rest
 }
 
 // one workgroup runs one GEMM variant (0 vector FP64, 1 MFMA FP64, 2 MFMA FP32)
-__global__ void __launch_bounds__(NT) k_gemm(int variant, int M, int Nc, int K, const double* A, int lda, int tA,
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) k_gemm(int variant, int M, int Nc, int K, const double* A, int lda, int tA,
                                              const double* B, int ldb, int tB, double* C, int ldc, const double* rs,
                                              const double* ms, const double* cs, double beta) {
     Lds& s = g_lds;
