@@ -162,7 +162,20 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=48, help="jobs of the serving sample (0 skips it)")
     ap.add_argument("--serve-scale", type=float, default=0.05, help="sweep-count scale of the serving sample")
     ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrivals per second")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="only fill and copy a device buffer of this size and exit (a known-traffic reference for "
+                         "profiler byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration > 0:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"fill_bytes_written": n * 8, "copy_bytes_read": n * 8, "copy_bytes_written": n * 8}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: chains are independent and would be sharded by chain across GPUs, "
               f"but only the single-GPU figure is measured here (asked for --gpus {args.gpus}).", file=sys.stderr)

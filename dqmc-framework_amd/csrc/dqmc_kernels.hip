@@ -33,6 +33,17 @@ namespace dqmc {
 // ---------------------------------------------------------------------------
 #define NB 16                    // panel width of the blocked LU and QR
 #define PANEL_DOUBLES DQMC_PANEL_DOUBLES   // an LU/QR panel (up to NMAX x NB) plus its NB x NB blocks
+// panel-region layout of the LU / QR: P (NMAX x NB), two NB x NB blocks, then the
+// reduction scratch of the register-resident panel factorizations
+#define PANEL_BLK (NMAX * NB)
+#define PANEL_RED (NMAX * NB + 2 * NB * NB)
+#define RED_SLOTS (2 * 4 * NB)      // [parity][wave][slot]
+#define RED_ROWJ (RED_SLOTS)        // [parity][NB]: row j of the panel
+#define RED_CAND (RED_ROWJ + 2 * NB)   // [parity][wave][NB]: pivot candidate rows (LU)
+#define RED_CVAL (RED_CAND + 2 * 4 * NB)   // [parity][wave]: candidate |value|, then row index
+#define RED_GRAM (RED_CVAL + 2 * 2 * 4)    // [wave][128]: Gram partials (QR)
+#define RED_END (RED_GRAM + 4 * 128 + 128)
+static_assert(PANEL_RED + RED_END <= DQMC_PANEL_DOUBLES, "panel scratch exceeds the LDS panel");
 
 struct Lds {
     double As[16][64];         // GEMM A tile (k-major)
@@ -83,6 +94,24 @@ __device__ __forceinline__ void wave_argmax(double& v, int& idx) {
         if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
     }
 }
+// Reduce-scatter of 16 values over a wavefront: afterwards lane L holds the
+// wavefront sum of slot (L >> 2) & 15 (17 shuffles instead of 16 x 6).
+__device__ __forceinline__ double wave_reduce16(const double (&v)[16]) {
+    const int L = lane();
+    double a8[8], a4[4], a2[2];
+    const bool h5 = L & 32, h4 = L & 16, h3 = L & 8, h2 = L & 4;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a8[i] = (h5 ? v[i + 8] : v[i]) + __shfl_xor(h5 ? v[i] : v[i + 8], 32);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a4[i] = (h4 ? a8[i + 4] : a8[i]) + __shfl_xor(h4 ? a8[i] : a8[i + 4], 16);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) a2[i] = (h3 ? a4[i + 2] : a4[i]) + __shfl_xor(h3 ? a4[i] : a4[i + 2], 8);
+    double a1 = (h2 ? a2[1] : a2[0]) + __shfl_xor(h2 ? a2[0] : a2[1], 4);
+    a1 += __shfl_xor(a1, 2);
+    a1 += __shfl_xor(a1, 1);
+    return a1;
+}
+
 // uniform values in scalar registers: function arguments of device functions
 // arrive in vector registers and would otherwise count as divergent
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -708,6 +737,116 @@ __device__ void wg_wrap(const ChainDesc& d, int sp, bool up, double* G, double* 
 }
 
 // ---------------------------------------------------------------------------
+// LU with partial pivoting of the m x jb panel staged in LDS (s.panel, ld m),
+// in place (unit-lower L below the diagonal, U on and above it).  Thread t keeps
+// rows t and t + NT in registers.  Per column one workgroup barrier: every
+// wavefront publishes its best pivot candidate (largest |value| among rows >= j,
+// smallest row on ties) together with that row, and the owner of row j
+// publishes row j, so after the barrier every thread knows the pivot row and
+// does the interchange and the elimination of its own rows.  The pivot of step
+// j is stored as s.perm[k0 + j] = k0 + p.  Returns the sign the interchanges
+// and pivots contribute to det.
+// ---------------------------------------------------------------------------
+__device__ __noinline__ double lu_panel(int m, int jb, int k0, Lds&) {
+    Lds& s = g_lds;
+    m = uni(m); jb = uni(jb); k0 = uni(k0);
+    const int t = tid(), w = wave();
+    double* const P = s.panel;
+    double* const red = s.panel + PANEL_RED;
+    double x[2][NB];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            const int r = t + i * NT;
+            x[i][c] = (r < m && c < jb) ? P[c * m + r] : 0.0;
+        }
+    double detsign = 1.0;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      if (j < jb) {   // (a guard, not a break, so that the loop unrolls and x stays in registers)
+        const int par = j & 1;
+        double bv = -1.0;
+        int bi = j;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int r = t + i * NT;
+            if (r >= j && r < m) {
+                const double v = fabs(x[i][j]);
+                if (v > bv || (v == bv && r < bi)) { bv = v; bi = r; }
+            }
+        }
+        wave_argmax(bv, bi);
+        if (bi % NT == t) {   // this thread holds the wavefront's candidate row
+            const int slot = bi / NT;
+#pragma unroll
+            for (int c = 0; c < NB; ++c) red[RED_CAND + (par * 4 + w) * NB + c] = slot ? x[1][c] : x[0][c];
+        }
+        if (lane() == 0) {
+            red[RED_CVAL + (par * 2 + 0) * 4 + w] = bv;
+            red[RED_CVAL + (par * 2 + 1) * 4 + w] = (double)bi;
+        }
+        if (t == j)
+#pragma unroll
+            for (int c = 0; c < NB; ++c) red[RED_ROWJ + par * NB + c] = x[0][c];
+        __syncthreads();
+        // the pivot: best of the 4 candidates (ties: smaller row, then lower wavefront)
+        int wb = 0;
+        double vb = red[RED_CVAL + (par * 2 + 0) * 4 + 0];
+        int pb = (int)red[RED_CVAL + (par * 2 + 1) * 4 + 0];
+#pragma unroll
+        for (int q = 1; q < 4; ++q) {
+            const double v = red[RED_CVAL + (par * 2 + 0) * 4 + q];
+            const int pr = (int)red[RED_CVAL + (par * 2 + 1) * 4 + q];
+            if (v > vb || (v == vb && pr < pb)) { vb = v; pb = pr; wb = q; }
+        }
+        const int p = pb;
+        if (t == 0) s.perm[k0 + j] = k0 + p;
+        double rowp[NB];
+#pragma unroll
+        for (int c = 0; c < NB; ++c) rowp[c] = red[RED_CAND + (par * 4 + wb) * NB + c];
+        if (p != j) {
+            detsign = -detsign;
+            // interchange rows j and p (both from the published copies)
+            if (t == p % NT) {
+#pragma unroll
+                for (int c = 0; c < NB; ++c) {
+                    const double v = red[RED_ROWJ + par * NB + c];
+                    if (p / NT) x[1][c] = v; else x[0][c] = v;
+                }
+            }
+            if (t == j)
+#pragma unroll
+                for (int c = 0; c < NB; ++c) x[0][c] = rowp[c];
+        }
+        const double piv = rowp[j];
+        if (piv < 0) detsign = -detsign;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int r = t + i * NT;
+            if (r > j && r < m) {
+                const double l = x[i][j] / piv;
+#pragma unroll
+                for (int c = 0; c < NB; ++c)
+                    if (c > j) x[i][c] -= l * rowp[c];
+                x[i][j] = l;
+            }
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int r = t + i * NT;
+        if (r >= m) continue;
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+            if (c < jb) P[c * m + r] = x[i][c];
+    }
+    __syncthreads();
+    return detsign;
+}
+
+// ---------------------------------------------------------------------------
 // Blocked LU with partial pivoting: A X = B, X overwrites B (N x nrhs); A is
 // destroyed.  Panels of NB columns are factored in LDS (pivot search, swaps
 // and rank-1 updates never leave the CU); the row interchanges and L11^-1 are
@@ -733,36 +872,7 @@ __device__ __noinline__ double wg_lu_solve(double* A, double* B, int N, int nrhs
         wg_stream(m * jb, [&](int e) { const int r = e % m, c = e / m; return Ag[(size_t)(k0 + c) * N + k0 + r]; },
                   [&](int e, double v) { P[e] = v; });
         __syncthreads();
-        for (int j = 0; j < jb; ++j) {
-            if (wave() == 0) {
-                double best = -1.0; int bi = j;
-                for (int r = j + lane(); r < m; r += 64) {
-                    const double v = fabs(P[j * m + r]);
-                    if (v > best || (v == best && r < bi)) { best = v; bi = r; }
-                }
-                wave_argmax(best, bi);
-                if (lane() == 0) { s.ivar[0] = bi; s.perm[k0 + j] = k0 + bi; }
-            }
-            __syncthreads();
-            const int p = s.ivar[0];
-            if (p != j) {
-                for (int c = tid(); c < jb; c += NT) {
-                    const double t = P[c * m + j]; P[c * m + j] = P[c * m + p]; P[c * m + p] = t;
-                }
-                detsign = -detsign;
-                __syncthreads();
-            }
-            const double piv = P[j * m + j];
-            if (piv < 0) detsign = -detsign;
-            const int nr = m - j - 1, nc = jb - j - 1;
-            for (int e = tid(); e < nr * nc; e += NT) {
-                const int r = j + 1 + e % nr, c = j + 1 + e / nr;
-                P[c * m + r] -= (P[j * m + r] / piv) * P[c * m + j];
-            }
-            __syncthreads();
-            for (int r = j + 1 + tid(); r < m; r += NT) P[j * m + r] /= piv;
-            __syncthreads();
-        }
+        detsign *= lu_panel(m, jb, k0, s);
         pclk.stop(1);
         for (int e = tid(); e < m * jb; e += NT) {
             const int r = e % m, c = e / m;
@@ -846,6 +956,167 @@ __device__ __noinline__ double wg_lu_solve(double* A, double* B, int N, int nrhs
 }
 
 // ---------------------------------------------------------------------------
+// Householder QR of the m x jb panel staged in LDS (s.panel, ld m).  Thread t
+// keeps rows t and t + NT of the panel in registers, so each column costs one
+// workgroup barrier: the reduction of its norm and of its dot products with the
+// columns to its right (LAPACK dgeqr2 with the reflector scaled to a unit
+// leading element, tau = (alpha - x0) / alpha written as -v0 / alpha).  Then the
+// Gram matrix V^T V of the reflectors and the T factor of the block reflector
+// I - V T V^T (dlarft, forward columnwise) are formed.  Outputs: R (upper part)
+// to Mp and the explicit V (unit lower) to M, both at (k0, k0) with ld N; tau to
+// s.v1[k0..]; T to Tb (LDS, ld NB) and Tg.
+// ---------------------------------------------------------------------------
+__device__ __noinline__ void qr_panel(int m, int jb, int k0, int N, double* M_, double* Mp_, double* Tg_, Lds&) {
+    Lds& s = g_lds;
+    m = uni(m); jb = uni(jb); k0 = uni(k0); N = uni(N);
+    AS_G double* M = gptr(uni(M_));
+    AS_G double* Mp = gptr(uni(Mp_));
+    AS_G double* Tg = gptr(uni(Tg_));
+    const int t = tid(), w = wave();
+    double* const P = s.panel;
+    double* const red = s.panel + PANEL_RED;
+    double x[2][NB];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            const int r = t + i * NT;
+            x[i][c] = (r < m && c < jb) ? P[c * m + r] : 0.0;
+        }
+    double* const tau = s.v1 + k0;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      if (j < jb) {   // (a guard, not a continue, so that the loop unrolls and x stays in registers)
+        const int par = j & 1;
+        // partial sums over this thread's rows r > j: slot 0 the squared norm of
+        // column j below the diagonal, slot q the dot product with column j + q
+        double part[NB];
+#pragma unroll
+        for (int q = 0; q < NB; ++q) part[q] = 0.0;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int r = t + i * NT;
+            if (r > j && r < m) {
+                const double xj = x[i][j];
+                part[0] += xj * xj;
+#pragma unroll
+                for (int q = 1; q < NB; ++q)
+                    if (j + q < NB) part[q] += xj * x[i][j + q];
+            }
+        }
+        if (t == j)
+#pragma unroll
+            for (int q = 0; q < NB; ++q) red[RED_ROWJ + par * NB + q] = (j + q < NB) ? x[0][j + q] : 0.0;
+        const double wsum = wave_reduce16(part);
+        if ((lane() & 3) == 0) red[(par * 4 + w) * NB + ((lane() >> 2) & 15)] = wsum;
+        __syncthreads();
+        // workgroup totals of slot q: the 4 wavefront sums
+#define QR_TOT(q) (red[(par * 4 + 0) * NB + (q)] + red[(par * 4 + 1) * NB + (q)] + \
+                   red[(par * 4 + 2) * NB + (q)] + red[(par * 4 + 3) * NB + (q)])
+        const double x0 = red[RED_ROWJ + par * NB];
+        const double nrm = sqrt(x0 * x0 + QR_TOT(0));
+        double tk = 0.0, inv_v0 = 1.0, alpha = x0;
+        if (nrm != 0.0) {
+            alpha = x0 > 0 ? -nrm : nrm;
+            const double v0 = x0 - alpha;
+            tk = -v0 / alpha;
+            inv_v0 = 1.0 / v0;
+        }
+        if (t == 0) tau[j] = tk;
+        // H_j on the columns right of j: w_q = tau (P(j, j+q) + v^T P(j+1:, j+q))
+        double wq[NB];
+#pragma unroll
+        for (int q = 1; q < NB; ++q) wq[q] = (j + q < NB) ? tk * (red[RED_ROWJ + par * NB + q] + inv_v0 * QR_TOT(q)) : 0.0;
+#undef QR_TOT
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int r = t + i * NT;
+            if (r > j && r < m && tk != 0.0) {
+                const double v = x[i][j] * inv_v0;
+                x[i][j] = v;
+#pragma unroll
+                for (int q = 1; q < NB; ++q)
+                    if (j + q < NB) x[i][j + q] -= wq[q] * v;
+            }
+        }
+        if (t == j) {
+            x[0][j] = alpha;
+#pragma unroll
+            for (int q = 1; q < NB; ++q)
+                if (j + q < NB) x[0][j + q] -= wq[q];
+        }
+      }
+    }
+    // R to Mp, explicit V to M
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int r = t + i * NT;
+        if (r >= m) continue;
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            if (c < jb) {
+                const size_t o = (size_t)(k0 + c) * N + k0 + r;
+                Mp[o] = (r <= c) ? x[i][c] : 0.0;
+                M[o] = (r == c) ? 1.0 : (r > c ? x[i][c] : 0.0);
+            }
+        }
+    }
+    // Gram matrix G(l, j) = V_l^T V_j (l < j) = V(j, l) + sum_{r > j} V(r, l) V(r, j):
+    // 120 pairs in 8 groups of 16, reduce-scattered per wavefront
+    double* const gram = red + RED_GRAM;
+#pragma unroll
+    for (int g = 0; g < 8; ++g) {
+        double part[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            part[q] = 0.0;
+            const int pidx = g * 16 + q;
+            // pair index p = j (j - 1) / 2 + l, 0 <= l < j < NB
+            int j = 1;
+            while ((j + 1) * j / 2 <= pidx) ++j;
+            const int l = pidx - j * (j - 1) / 2;
+            if (j < NB && j < jb) {
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    const int r = t + i * NT;
+                    if (r < m && r > j) part[q] += x[i][l] * x[i][j];
+                    else if (r < m && r == j) part[q] += x[i][l];
+                }
+            }
+        }
+        const double wsum = wave_reduce16(part);
+        if ((lane() & 3) == 0) gram[w * 128 + g * 16 + ((lane() >> 2) & 15)] = wsum;
+    }
+    __syncthreads();
+    if (t < 128) gram[4 * 128 + t] = gram[t] + gram[128 + t] + gram[256 + t] + gram[384 + t];
+    __syncthreads();
+    // T: lane i of wave 0 builds row i, T(i, j) = -tau_j sum_{l=i}^{j-1} T(i, l) G(l, j), T(j, j) = tau_j
+    double* const Tb = s.panel + PANEL_BLK;
+    if (w == 0 && lane() < NB) {
+        const int i = lane();
+        double trow[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            trow[j] = 0.0;
+            if (j < jb && i == j) trow[j] = tau[j];
+            else if (j < jb && i < j) {
+                double acc = 0.0;
+#pragma unroll
+                for (int l = 0; l < NB; ++l)
+                    if (l >= i && l < j) acc += trow[l] * gram[4 * 128 + j * (j - 1) / 2 + l];
+                trow[j] = -tau[j] * acc;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            Tb[j * NB + i] = trow[j];
+            Tg[(size_t)(k0 / NB) * NB * NB + j * NB + i] = trow[j];
+        }
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
 // UDV by pre-pivoted blocked Householder QR of M (N x N):
 //   columns sorted once by decreasing norm (P), then M P = Q R without
 //   further pivoting (Bauer, SciPost Phys. Core 2, 011 (2020));
@@ -861,7 +1132,6 @@ __device__ __noinline__ void wg_qr_udv(double* M, double* Mp, int N, double* U, 
     Lds& s = g_lds;
     double* P = s.panel;                  // m x jb panel, ld = m
     double* Tb = s.panel + NMAX * NB;     // NB x NB T of the current panel (ld NB)
-    double* Gm = Tb + NB * NB;            // NB x NB Gram matrix V^T V
     double* Wsm = Wq;                     // NB x N GEMM scratch (ld NB)
     double* Wsm2 = Wq + (size_t)NB * N;   // NB x N GEMM scratch (ld NB)
     double* Tg = Wq + (size_t)2 * NB * N; // T factors of all panels, NB x NB each
@@ -906,73 +1176,7 @@ __device__ __noinline__ void wg_qr_udv(double* M, double* Mp, int N, double* U, 
         wg_stream(m * jb, [&](int e) { const int r = e % m, c = e / m; return Mpg[(size_t)(k0 + c) * N + k0 + r]; },
                   [&](int e, double v) { P[e] = v; });
         __syncthreads();
-        for (int j = 0; j < jb; ++j) {
-            double acc = 0.0;
-            for (int r = j + 1 + tid(); r < m; r += NT) acc += P[j * m + r] * P[j * m + r];
-            wg_sum8(&acc, 1, s);
-            if (tid() == 0) {
-                const double x0 = P[j * m + j];
-                const double nrm = sqrt(x0 * x0 + acc);
-                if (nrm == 0.0) { tau[k0 + j] = 0.0; s.red[8] = 1.0; s.red[9] = x0; }
-                else {
-                    const double alpha = x0 > 0 ? -nrm : nrm;
-                    const double v0 = x0 - alpha;
-                    tau[k0 + j] = -v0 / alpha;
-                    s.red[8] = 1.0 / v0;
-                    s.red[9] = alpha;
-                }
-            }
-            __syncthreads();
-            const double tk = tau[k0 + j];
-            const double inv_v0 = s.red[8];
-            if (tk != 0.0)
-                for (int r = j + 1 + tid(); r < m; r += NT) P[j * m + r] *= inv_v0;
-            __syncthreads();
-            if (tid() == 0) P[j * m + j] = s.red[9];
-            // apply H_j to the remaining panel columns, one wave per column
-            for (int c = j + 1 + wave(); c < jb; c += 4) {
-                double w = 0.0;
-                for (int r = j + 1 + lane(); r < m; r += 64) w += P[j * m + r] * P[c * m + r];
-                w = wave_sum(w);
-                w = tk * (P[c * m + j] + w);
-                for (int r = j + 1 + lane(); r < m; r += 64) P[c * m + r] -= w * P[j * m + r];
-                if (lane() == 0) P[c * m + j] -= w;
-            }
-            __syncthreads();
-        }
-        // Gram matrix of the reflectors (V unit lower: V(j,j) = 1, V(r,l) = P(r,l) for r > l)
-        for (int q = wave(); q < jb * jb; q += 4) {
-            const int l = q % jb, j = q / jb;
-            if (l >= j) continue;
-            double g = 0.0;
-            for (int r = j + 1 + lane(); r < m; r += 64) g += P[l * m + r] * P[j * m + r];
-            g = wave_sum(g);
-            if (lane() == 0) Gm[j * NB + l] = P[l * m + j] + g;
-        }
-        __syncthreads();
-        if (tid() == 0) {
-            for (int e = 0; e < NB * NB; ++e) Tb[e] = 0.0;
-            for (int j = 0; j < jb; ++j) {
-                const double tj = tau[k0 + j];
-                for (int i = 0; i < j; ++i) {
-                    double acc = 0.0;
-                    for (int l = i; l < j; ++l) acc += Tb[l * NB + i] * Gm[j * NB + l];
-                    Tb[j * NB + i] = -tj * acc;
-                }
-                Tb[j * NB + j] = tj;
-                for (int i = j + 1; i < NB; ++i) Tb[j * NB + i] = 0.0;
-            }
-        }
-        __syncthreads();
-        // R to Mp, explicit V to M, T to Tg
-        for (int e = tid(); e < m * jb; e += NT) {
-            const int r = e % m, c = e / m;
-            const double v = P[c * m + r];
-            Mp[(size_t)(k0 + c) * N + k0 + r] = (r <= c) ? v : 0.0;
-            M[(size_t)(k0 + c) * N + k0 + r] = (r == c) ? 1.0 : (r > c ? v : 0.0);
-        }
-        for (int e = tid(); e < NB * NB; e += NT) Tg[(size_t)(k0 / NB) * NB * NB + e] = Tb[e];
-        __syncthreads();
+        qr_panel(m, jb, k0, N, M, Mp, Tg, s);
         pclk.stop(1);
         // trailing columns: A2 <- (I - V T^T V^T) A2
         const int n2 = N - k0 - jb;
