@@ -1346,149 +1346,246 @@ __device__ __noinline__ void wg_fresh_green(const ChainDesc& d, int sp, int lslo
 // W = C (G0[S,:] - E_S) and G += G0[:,S] W on the matrix cores.
 #define KB 32   // one half-wavefront per spin in the walk
 
-__device__ __forceinline__ int block_depth(const ChainDesc& d) { return min(d.kdelay, KB); }
+// LDS layout of the Metropolis phase (doubles from s.panel, KP = KB + 1):
+//   CR(sp)  = row a of the block's C at a*KP, both spins (kept until the block lands in G)
+//   GSS(sp) = G0[S,S] (row a at a*KP), CC(sp) = column a of C at a*KP, UV = the walk's u, v
+//   after the walk the space from OVER holds, per spin in turn, ACP = G0[:,S]
+//   (N x k4, ld N) and GSW = G0[S,:] (k4 x N, ld N), which becomes W in place.
+constexpr int KP = KB + 1;
+#define MP_CR(sp) (s.panel + (sp) * KB * KP)
+#define MP_GSS(sp) (s.panel + 2 * KB * KP + (sp) * 2 * KB * KP)
+#define MP_CC(sp) (MP_GSS(sp) + KB * KP)
+#define MP_UV (s.panel + 6 * KB * KP)
+#define MP_OVER (2 * KB * KP)
+#define MP_SLACK 16   // reads of partial 16x16 blocks run up to 16 doubles past GSW
+
+// block size: the delay depth, at most KB, and small enough that ACP and GSW fit
+__device__ __forceinline__ int block_depth(const ChainDesc& d) {
+    const int N = d.N;
+    int kb = min(d.kdelay, KB);
+    const int cap = ((PANEL_DOUBLES - MP_OVER - MP_SLACK) / (2 * N)) & ~3;
+    return max(1, min(kb, cap));
+}
+
+// The block's update of G for one spin, on the matrix cores with operands in LDS:
+//   W = C (G0[S,:] - E_S)   (k x N; C from CR, G0[S,:] staged in GSW, W written over it)
+//   G += G0[:,S] W           (G0[:,S] staged in ACP; G read-modify-written in 16x16 blocks,
+//                             8 blocks per wavefront at a time with all their loads in flight)
+__device__ __noinline__ void block_update(double* G_, int N, int b0, int k, int sp, Lds&) {
+    Lds& s = g_lds;
+    N = uni(N); b0 = uni(b0); k = uni(k); sp = uni(sp);
+    AS_G double* G = gptr(uni(G_));
+    const int k4 = (k + 3) & ~3;
+    const int l = lane(), w = wave();
+    double* const acp = s.panel + MP_OVER;
+    double* const gsw = acp + (size_t)N * k4;
+    const double* const cr = MP_CR(sp);
+    // stage G0[:,S] (columns b0..b0+k) and G0[S,:] (rows b0..b0+k), zero-padded to k4
+    wg_stream(N * k4, [&](int e) { const int c = e / N; return c < k ? G[(size_t)(b0 + c) * N + (e - c * N)] : 0.0; },
+              [&](int e, double v) { acp[e] = v; });
+    wg_stream(N * k4, [&](int e) { const int b = e / N, c = e - b * N; return b < k ? G[(size_t)c * N + b0 + b] : 0.0; },
+              [&](int e, double v) { gsw[e] = v; });
+    __syncthreads();
+    // W = C G0[S,:] - C E_S, column block by column block (each wavefront owns whole column blocks,
+    // so W may overwrite GSW in place)
+    const int nbc = (N + 15) >> 4;
+    for (int jb = w; jb < nbc; jb += 4) {
+        const int col = jb * 16 + (l & 15);
+        dbl4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+        for (int kk = 0; kk < k4; kk += 4) {
+            const int kr = kk + (l >> 4);
+            const double bv = col < N ? gsw[kr * N + col] : 0.0;
+            const double a0 = cr[(l & 15) * KP + kr];
+            acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv, acc0, 0, 0, 0);
+            if (k4 > 16) {
+                const double a1 = cr[(16 + (l & 15)) * KP + kr];
+                acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv, acc1, 0, 0, 0);
+            }
+        }
+        if (col < N) {
+            const bool in_s = col >= b0 && col < b0 + k;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int a = (l >> 4) + 4 * i;
+                if (a < k4) gsw[a * N + col] = acc0[i] - ((in_s && a < k) ? cr[a * KP + col - b0] : 0.0);
+                const int a1 = 16 + a;
+                if (k4 > 16 && a1 < k4) gsw[a1 * N + col] = acc1[i] - ((in_s && a1 < k) ? cr[a1 * KP + col - b0] : 0.0);
+            }
+        }
+    }
+    __syncthreads();
+    // G += ACP W
+    const int nbr = nbc, nblk = nbr * nbc;
+    constexpr int CH = 8;
+    for (int q0 = w; q0 < nblk; q0 += 4 * CH) {
+        double cv[CH][4];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            const int q = q0 + 4 * u;
+            const int rb = q % nbr, cb = q / nbr;
+            const int col = cb * 16 + (l & 15);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int row = rb * 16 + (l >> 4) + 4 * i;
+                cv[u][i] = (q < nblk && row < N && col < N) ? G[(size_t)col * N + row] : 0.0;
+            }
+        }
+        dbl4 acc[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            acc[u] = dbl4{0.0, 0.0, 0.0, 0.0};
+            const int q = q0 + 4 * u;
+            if (q < nblk) {
+                const int rb = q % nbr, cb = q / nbr;
+                for (int kk = 0; kk < k4; kk += 4) {
+                    const int kr = kk + (l >> 4);
+                    const double av = acp[kr * N + rb * 16 + (l & 15)];
+                    const double bv = gsw[kr * N + cb * 16 + (l & 15)];
+                    acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[u], 0, 0, 0);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            const int q = q0 + 4 * u;
+            const int rb = q % nbr, cb = q / nbr;
+            const int col = cb * 16 + (l & 15);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int row = rb * 16 + (l >> 4) + 4 * i;
+                if (q < nblk && row < N && col < N) G[(size_t)col * N + row] = cv[u][i] + acc[u][i];
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// The in-block walk, run by wavefront 0 only.  Lanes 0..31 carry spin up and
+// lanes 32..63 spin down; lane h*32 + a owns site s_a of the block in spin h
+// and keeps row a and column a of that spin's C in registers for the whole
+// block, so a flip costs LDS reads only (the loops run over all KB rows with
+// fixed trip counts and unroll; G0[S,S] and C are zero outside the k x k block,
+// so the padding terms vanish exactly).  The decisions are computed by every
+// lane on identical data, in the reference's site order.  On return, row a of
+// C is in CR(h) for the block update.
+struct WalkOut {
+    uint64_t rng;
+    double sign;
+    int accepted;
+};
+__device__ __forceinline__ WalkOut walk_block(const ChainDesc& d, int e, int b0, int k, uint64_t rng, double sign,
+                                           bool record, Lds&) {
+    Lds& s = g_lds;
+    const int N = uni(d.N);
+    const int ns = uni(d.nspin);
+    b0 = uni(b0); k = uni(k);
+    const double pref0 = uni(d.pref[0]), pref1 = uni(d.pref[1]);
+    const double dm00 = uni(d.dm1[0][0]), dm01 = uni(d.dm1[0][1]), dm10 = uni(d.dm1[1][0]), dm11 = uni(d.dm1[1][1]);
+    int8_t* const field_row = uni(d.field) + (size_t)e * N;
+    uint8_t* const rec_row = uni(d.recAcc) + (size_t)e * N;
+    const int h = lane() >> 5, a = lane() & 31;
+    const double* const gs = MP_GSS(h);
+    double* const ush = MP_UV + h * 32;        // u at [lane], v at [64 + lane]
+    double* const vsh = MP_UV + 64 + h * 32;
+    double crr[KB], ccr[KB];
+#pragma unroll
+    for (int b = 0; b < KB; ++b) { crr[b] = 0.0; ccr[b] = 0.0; }
+    double dq = gs[a * KP + a];
+    int nacc = 0;
+    for (int p = 0; p < k; ++p) {
+        const double dp0 = readlane_d(dq, p);
+        const double dp1 = ns == 2 ? readlane_d(dq, 32 + p) : dp0;
+        const int j = b0 + p;
+        const int f = __builtin_amdgcn_readfirstlane(s.frow[j]);
+        const bool up = f > 0;
+        const double m0 = up ? dm00 : dm01, m1 = up ? dm10 : dm11;
+        const double x0 = 1.0 + (1.0 - dp0) * m0;
+        const double x1 = 1.0 + (1.0 - dp1) * m1;
+        const double r = (up ? pref0 : pref1) * x0 * x1;
+        const double pr = fmin(1.0, fabs(r));
+        const int ok = __builtin_amdgcn_readfirstlane(canonical(rng) < pr ? 1 : 0);
+        if (record && lane() == 0) rec_row[j] = (uint8_t)ok;
+        if (!ok) continue;
+        ++nacc;
+        sign = (r >= 0) ? sign : -sign;
+        if (lane() == 0) {
+            s.frow[j] = -f;
+            field_row[j] = (int8_t)(-f);
+        }
+        const double g = h == 0 ? m0 / x0 : m1 / x1;
+        // u_a = d_ap + (C (Gss - I))(a,p),  v_a = d_ap + (Gss C)(p,a)
+        double t = 0.0, wv = 0.0;
+#pragma unroll
+        for (int b = 0; b < KB; ++b) {
+            t += crr[b] * (gs[b * KP + p] - (b == p ? 1.0 : 0.0));
+            wv += gs[p * KP + b] * ccr[b];
+        }
+        const double u = (a == p ? 1.0 : 0.0) + t;
+        const double v = (a == p ? 1.0 : 0.0) + wv;
+        ush[a] = u;
+        vsh[a] = v;
+        __builtin_amdgcn_wave_barrier();
+        // C += g u v^T (row a and column a), and the block diagonal
+        // d_a += g (Gss u)_a (v^T (Gss - I))_a
+        double x = 0.0, y = 0.0;
+        const double gu = g * u, gv = g * v;
+#pragma unroll
+        for (int b = 0; b < KB; ++b) {
+            const double ub = ush[b], vb = vsh[b];
+            crr[b] += gu * vb;
+            ccr[b] += ub * gv;
+            x += gs[a * KP + b] * ub;
+            y += vb * (gs[b * KP + a] - (b == a ? 1.0 : 0.0));
+        }
+        __builtin_amdgcn_wave_barrier();
+        dq += g * x * y;
+    }
+    double* const cro = MP_CR(h) + a * KP;
+#pragma unroll
+    for (int b = 0; b < KB; ++b) cro[b] = crr[b];
+    return WalkOut{rng, sign, nacc};
+}
 
 __device__ __noinline__ void wg_metropolis(const ChainDesc& d, int e, uint64_t& rng_io, double& sign_io, bool record,
                                           int& naccept, Lds&) {
     Lds& s = g_lds;
-    // uniform scalars of the descriptor in SGPRs, the chain's random state and
-    // sign in registers for the whole slice
     const int N = __builtin_amdgcn_readfirstlane(d.N);
     const int ns = __builtin_amdgcn_readfirstlane(d.nspin);
-    const double pref[2] = {d.pref[0], d.pref[1]};
-    const double dm1[2][2] = {{d.dm1[0][0], d.dm1[0][1]}, {d.dm1[1][0], d.dm1[1][1]}};
-    int8_t* const field_row = d.field + (size_t)e * N;
-    uint8_t* const rec_row = d.recAcc + (size_t)e * N;
     uint64_t rng = rng_io;
     double sign = sign_io;
-    constexpr int KP = KB + 1;
-    constexpr int SPIN_LDS = 3 * KB * KP;   // Gss, CR, CC of one spin
-    const int used = 2 * SPIN_LDS + 128;    // two spins and the u, v broadcast vectors
-    // block size: the delay depth, reduced when G0[:,S] and W must go to the
-    // Gw workspace and would not fit there (2 N k doubles per spin <= N^2)
-    int kb = block_depth(d);
-    if (used + 2 * ns * N * kb > PANEL_DOUBLES) kb = max(1, min(kb, N / 2));
-    kb = __builtin_amdgcn_readfirstlane(kb);
-    // LDS, per spin: Gss = G0[S,S] (KB x KP), CR (row a of C at lane a), CC (column a of C at lane a).
-    // Addresses are formed from g_lds directly so that every access is a ds_* instruction.
-#define GSS(sp) (s.panel + (sp) * SPIN_LDS)
-#define CRP(sp) (s.panel + (sp) * SPIN_LDS + KB * KP)
-#define CCP(sp) (s.panel + (sp) * SPIN_LDS + 2 * KB * KP)
-    // G0[:,S] copies (N x kb) and W (kb x N) per spin: LDS when they fit, else the Gw workspace
-    const bool in_lds = used + 2 * ns * N * kb <= PANEL_DOUBLES;
+    const int kb = __builtin_amdgcn_readfirstlane(block_depth(d));
     for (int b0 = 0; b0 < N; b0 += kb) {
         const int k = min(kb, N - b0);
+        // G0[S,S] of both spins, zero outside the k x k block (both halves of the walk
+        // read their spin's arrays, also in the one-spin model)
+        for (int sp = 0; sp < 2; ++sp) {
+            double* gss = MP_GSS(sp);
+            for (int q = tid(); q < KB * KP; q += NT) gss[q] = 0.0;
+        }
+        __syncthreads();
         for (int sp = 0; sp < ns; ++sp) {
-            const double* G = d.G + (size_t)sp * N * N;
-            double* gss = GSS(sp);
-            double* cr = CRP(sp);
-            double* cc = CCP(sp);
-            const AS_G double* Gg = gptr(G);
+            const AS_G double* Gg = gptr(d.G + (size_t)sp * N * N);
+            double* gss = MP_GSS(sp);
             wg_stream(k * k, [&](int q) { const int a = q % k, c = q / k; return Gg[(size_t)(b0 + c) * N + b0 + a]; },
-                      [&](int q, double v) {
-                          const int a = q % k, c = q / k;
-                          gss[a * KP + c] = v;
-                          cr[a * KP + c] = 0.0;
-                          cc[a * KP + c] = 0.0;
-                      });
+                      [&](int q, double v) { const int a = q % k, c = q / k; gss[a * KP + c] = v; });
         }
         if (tid() == 0) s.ivar[2] = 0;
         __syncthreads();
         SubClock walk_clk;
         walk_clk.start();
         if (wave() == 0) {
-            // lanes 0..31 carry spin up, lanes 32..63 spin down; lane h*32 + a
-            // owns site s_a of the block in spin h: its current diagonal dq,
-            // row a and column a of C.  Lanes with a >= k work on rows the
-            // block does not use, so no lane is masked off in the walk.
-            const int h = lane() >> 5, a = lane() & 31;
-            const int hb = h * SPIN_LDS;
-            double* const cr = s.panel + hb + KB * KP + a * KP;
-            double* const cc = s.panel + hb + 2 * KB * KP + a * KP;
-            const double* const gs = s.panel + hb;
-            double* const uv = s.panel + 2 * SPIN_LDS;   // u at [lane], v at [64 + lane]
-            double* const ush = uv + h * 32;
-            double* const vsh = uv + 64 + h * 32;
-            int nacc_blk = 0;
-            double dq = gs[a * KP + a];
-            for (int p = 0; p < k; ++p) {
-                const double dp0 = readlane_d(dq, p);
-                const double dp1 = ns == 2 ? readlane_d(dq, 32 + p) : dp0;
-                // the decision is computed by every lane on identical data
-                const int j = b0 + p;
-                const int f = __builtin_amdgcn_readfirstlane(s.frow[j]);
-                const int si = f > 0 ? 0 : 1;
-                const double x0 = 1.0 + (1.0 - dp0) * dm1[0][si];
-                const double x1 = 1.0 + (1.0 - dp1) * dm1[1][si];
-                const double r = pref[si] * x0 * x1;
-                const double pr = fmin(1.0, fabs(r));
-                const int ok = __builtin_amdgcn_readfirstlane(canonical(rng) < pr ? 1 : 0);
-                if (record && lane() == 0) rec_row[j] = (uint8_t)ok;
-                if (!ok) continue;
-                ++nacc_blk;
-                sign = (r >= 0) ? sign : -sign;
-                if (lane() == 0) {
-                    s.frow[j] = -f;
-                    field_row[j] = (int8_t)(-f);
-                }
-                const double g = h == 0 ? dm1[0][si] / x0 : dm1[1][si] / x1;
-                // u_a = d_ap + (C (Gss - I))(a,p),  v_a = d_ap + (Gss C)(p,a)
-                double t = 0.0, w = 0.0;
-#pragma unroll 8
-                for (int b = 0; b < k; ++b) {
-                    t += cr[b] * (gs[b * KP + p] - (b == p ? 1.0 : 0.0));
-                    w += gs[p * KP + b] * cc[b];
-                }
-                const double u = (a == p ? 1.0 : 0.0) + t;
-                const double v = (a == p ? 1.0 : 0.0) + w;
-                ush[a] = u;
-                vsh[a] = v;
-                __builtin_amdgcn_wave_barrier();
-                // C += g u v^T (row a and column a), and the block diagonal
-                // d_a += g (Gss u)_a (v^T (Gss - I))_a
-                double x = 0.0, y = 0.0;
-#pragma unroll 8
-                for (int b = 0; b < k; ++b) {
-                    const double ub = ush[b], vb = vsh[b];
-                    cr[b] += g * u * vb;
-                    cc[b] += g * ub * v;
-                    x += gs[a * KP + b] * ub;
-                    y += vb * (gs[b * KP + a] - (b == a ? 1.0 : 0.0));
-                }
-                __builtin_amdgcn_wave_barrier();
-                dq += g * x * y;
-            }
-            if (lane() == 0) s.ivar[2] = nacc_blk;
+            const WalkOut o = walk_block(d, e, b0, k, rng, sign, record, s);
+            rng = o.rng;
+            sign = o.sign;
+            if (lane() == 0) s.ivar[2] = o.accepted;
         }
         __syncthreads();
         walk_clk.stop(0);
         const int nblk = s.ivar[2];
         naccept += nblk;
         if (nblk == 0) continue;
-        // W = C (G0[S,:] - E_S) and the copy of G0[:,S], then G += G0[:,S] W
-        for (int sp = 0; sp < ns; ++sp) {
-            double* G = d.G + (size_t)sp * N * N;
-            double* acp = in_lds ? s.panel + used + (size_t)sp * 2 * N * kb : d.Gw + (size_t)sp * N * N;
-            double* wb = acp + (size_t)N * kb;
-            const double* cr = CRP(sp);
-            const AS_G double* Gcols = gptr(G) + (size_t)b0 * N;
-            wg_stream(N * k, [&](int q) { return Gcols[q]; }, [&](int q, double v) { acp[q] = v; });
-            gemm64_lg(k, N, k, cr, KP, true, G + b0, N, false, wb, k, 0.0, s);
-            wg_stream(k * k, [&](int q) { const int a = q % k, c = q / k; return wb[(size_t)(b0 + c) * k + a] - cr[a * KP + c]; },
-                      [&](int q, double v) { const int a = q % k, c = q / k; wb[(size_t)(b0 + c) * k + a] = v; });
-        }
-        __syncthreads();
-        for (int sp = 0; sp < ns; ++sp) {
-            double* acp = in_lds ? s.panel + used + (size_t)sp * 2 * N * kb : d.Gw + (size_t)sp * N * N;
-            if (in_lds) gemm64_ll(N, N, k, acp, N, false, acp + (size_t)N * kb, k, false, d.G + (size_t)sp * N * N, N,
-                                  1.0, s);
-            else gemm64(N, N, k, acp, N, false, acp + (size_t)N * kb, k, false, d.G + (size_t)sp * N * N, N, nullptr,
-                        nullptr, nullptr, 1.0, s);
-        }
+        for (int sp = 0; sp < ns; ++sp) block_update(d.G + (size_t)sp * N * N, N, b0, k, sp, s);
     }
-#undef GSS
-#undef CRP
-#undef CCP
     rng_io = rng;
     sign_io = sign;
     __syncthreads();
