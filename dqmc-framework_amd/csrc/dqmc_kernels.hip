@@ -273,7 +273,9 @@ __device__ __noinline__ void wg_gemm(int M, int Nc, int K, const double* A, int 
 // first slab) are issued before the matrix-core work on the current slab, so
 // their latency hides behind it.  aL / bL say that A / B live in LDS; otherwise
 // they are read with global (not flat) loads, which keeps the LDS wait counter
-// free of them.  The middle scaling ms, when given, lives in LDS.
+// free of them.  The scalings rs, ms, cs, when given, live in LDS, and C in
+// global memory: a flat access could only be waited for together with every
+// load in flight.
 // ---------------------------------------------------------------------------
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
@@ -282,6 +284,10 @@ typedef float flt4 __attribute__((ext_vector_type(4)));
 template <bool lds> struct MemSpace { typedef const AS_G double* cptr; };
 template <> struct MemSpace<true> { typedef const AS_L double* cptr; };
 
+// Register-staged slab pipeline: while a step multiplies its 16-deep slab out
+// of LDS, the loads of the next step are in flight.  The (tile, slab) cursor is
+// kept in scalar registers and advanced by additions; each thread's element
+// offsets within a slab are computed once; interior slabs skip the bounds tests.
 template <bool lowp, bool aL, bool bL>
 __device__ __noinline__ void wg_gemm_mfma(int M, int Nc, int K, const double* A_, int lda, bool tA, const double* B_,
                                           int ldb, bool tB, double* C, int ldc, const double* rs, const double* ms_,
@@ -294,60 +300,62 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int Nc, int K, const double* A_
     typedef typename MemSpace<bL>::cptr BPtr;
     const APtr A = (APtr)A_;
     const BPtr B = (BPtr)B_;
-    const bool has_ms = ms_ != nullptr;
+    const bool has_ms = ms_ != nullptr, has_rs = rs != nullptr, has_cs = cs != nullptr;
+    // the scalings live in LDS (null tests above on the generic pointers); C is in global memory
     const AS_L double* ms = (const AS_L double*)ms_;
+    const AS_L double* rsl = (const AS_L double*)rs;
+    const AS_L double* csl = (const AS_L double*)cs;
+    AS_G double* const Cg = (AS_G double*)C;
     const int l = lane();
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
-    const int tm = (M + 63) >> 6, tn = (Nc + 63) >> 6;
-    const int nslab = (K + 15) >> 4;
-    const int total = tm * tn * nslab;     // (tile, slab) steps, slab fastest
+    if (M <= 0 || Nc <= 0 || K <= 0) { __syncthreads(); return; }
     float* Af = reinterpret_cast<float*>(&s.As[0][0]);  // [16][64] floats in lowp mode
     float* Bf = reinterpret_cast<float*>(&s.Bs[0][0]);
-    // this thread's 4 elements of a slab: A at (row ma + 16q', k ka + 4q) etc., as in-slab
-    // coordinates packed (m | k << 8) and the element offsets relative to the slab origin
-    int pa[4], pb[4], oa[4], ob[4];
+    const int t = tid();
+    // this thread's 4 elements of a slab: A (row ma, depth ka), B (depth kb, column nb);
+    // offsets relative to the slab origin and the LDS slots
+    int ma[4], ka[4], kb[4], nb[4], oa[4], ob[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const int e = tid() + NT * q;
-        int m, k, n, kb;
-        if (!tA) { m = e & 63; k = e >> 6; } else { k = e & 15; m = e >> 4; }
-        if (!tB) { kb = e & 15; n = e >> 4; } else { n = e & 63; kb = e >> 6; }
-        pa[q] = m | (k << 8);
-        pb[q] = n | (kb << 8);
-        oa[q] = tA ? m * lda + k : k * lda + m;
-        ob[q] = tB ? kb * ldb + n : n * ldb + kb;
+        const int e = t + NT * q;
+        if (!tA) { ma[q] = e & 63; ka[q] = e >> 6; } else { ka[q] = e & 15; ma[q] = e >> 4; }
+        if (!tB) { kb[q] = e & 15; nb[q] = e >> 4; } else { nb[q] = e & 63; kb[q] = e >> 6; }
+        oa[q] = tA ? ma[q] * lda + ka[q] : ka[q] * lda + ma[q];
+        ob[q] = tB ? kb[q] * ldb + nb[q] : nb[q] * ldb + kb[q];
     }
-    // offset of the C element (row l/16 (+4e) or 4(l/16) (+e), column l%16) of a 16x16 block
-    const int oc = (l & 15) * ldc + (lowp ? 4 * (l >> 4) : (l >> 4));
+    // slab origin (r0, c0, k0) -> offsets of its first element in A and B
+    auto a_base = [&](int r0, int k0) { return tA ? r0 * lda + k0 : k0 * lda + r0; };
+    auto b_base = [&](int c0, int k0) { return tB ? k0 * ldb + c0 : c0 * ldb + k0; };
     double va[4], vb[4];
-    auto load_step = [&](int i) {
-        const int t = i / nslab;
-        const int k0 = (i - t * nslab) << 4, r0 = (t % tm) << 6, c0 = (t / tm) << 6;
-        const int ba = tA ? r0 * lda + k0 : k0 * lda + r0;
-        const int bb = tB ? k0 * ldb + c0 : c0 * ldb + k0;
-        const int mlim = M - r0, nlim = Nc - c0, klim = K - k0;
+    auto load = [&](int r0, int c0, int k0) {
+        const int ba = a_base(r0, k0), bb = b_base(c0, k0);
+        if (r0 + 64 <= M && c0 + 64 <= Nc && k0 + 16 <= K) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            va[q] = ((pa[q] & 255) < mlim && (pa[q] >> 8) < klim) ? A[ba + oa[q]] : 0.0;
-            vb[q] = ((pb[q] & 255) < nlim && (pb[q] >> 8) < klim) ? B[bb + ob[q]] : 0.0;
+            for (int q = 0; q < 4; ++q) { va[q] = A[ba + oa[q]]; vb[q] = B[bb + ob[q]]; }
+        } else {
+            const int mr = M - r0, nr = Nc - c0, kr = K - k0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                va[q] = A[(ma[q] < mr && ka[q] < kr) ? ba + oa[q] : 0];
+                vb[q] = B[(nb[q] < nr && kb[q] < kr) ? bb + ob[q] : 0];
+            }
         }
     };
+    // cursors: (r0, c0, k0) of the step being multiplied and of the next load
+    int r0 = 0, c0 = 0, k0 = 0;
+    int lr = 0, lc = 0, lk = 0;
+    auto advance = [&](int& r, int& c, int& k) {
+        k += 16;
+        if (k >= K) { k = 0; r += 64; if (r >= M) { r = 0; c += 64; } }
+    };
+    load(lr, lc, lk);
+    advance(lr, lc, lk);
     dbl4 acc[2][2];
     flt4 accf[2][2];
-    double cpre[2][2][4];   // C of the tile when beta != 0, loaded before the last slab's products
-    if (total > 0) load_step(0);
-#pragma unroll 1
-    for (int i = 0; i < total; ++i) {
-        const int t = i / nslab;
-        const int ks = i - t * nslab;
-        const int k0 = ks << 4, r0 = (t % tm) << 6, c0 = (t / tm) << 6;
-        bool blk[2][2];
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b) blk[a][b] = (r0 + wr + 16 * a < M) && (c0 + wc + 16 * b < Nc);
-        if (ks == 0) {
+    const int oc = (l & 15) * ldc + (lowp ? 4 * (l >> 4) : (l >> 4));
+    while (c0 < Nc) {
+        if (k0 == 0) {
 #pragma unroll
             for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -356,21 +364,50 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int Nc, int K, const double* A_
                     accf[a][b] = flt4{0.f, 0.f, 0.f, 0.f};
                 }
         }
+        // slab to LDS (out-of-range elements zeroed here, after their loads arrived)
+        const bool full = r0 + 64 <= M && c0 + 64 <= Nc && k0 + 16 <= K;
+        const int mr = M - r0, nr = Nc - c0, kr = K - k0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int m = pa[q] & 255, k = pa[q] >> 8, n = pb[q] & 255, kb = pb[q] >> 8;
-            double v = va[q];
-            if (has_ms && k0 + k < K) v *= ms[k0 + k];
-            if (lowp) { Af[k * 64 + m] = (float)v; Bf[kb * 64 + n] = (float)vb[q]; }
-            else { s.As[k][m] = v; s.Bs[kb][n] = vb[q]; }
+            double v = (full || (ma[q] < mr && ka[q] < kr)) ? va[q] : 0.0;
+            if (has_ms) v *= ms[min(k0 + ka[q], K - 1)];
+            const double x = (full || (nb[q] < nr && kb[q] < kr)) ? vb[q] : 0.0;
+            if (lowp) { Af[ka[q] * 64 + ma[q]] = (float)v; Bf[kb[q] * 64 + nb[q]] = (float)x; }
+            else { s.As[ka[q]][ma[q]] = v; s.Bs[kb[q]][nb[q]] = x; }
         }
         __syncthreads();
-        // the next step's slab (possibly the next tile's first) is in flight during the products
-        if (i + 1 < total) load_step(i + 1);
-        const bool last = ks == nslab - 1;
+        if (lc < Nc) {   // the next step's slab is in flight during the products
+            load(lr, lc, lk);
+            advance(lr, lc, lk);
+        }
         const int rows = M - r0 - wr, cols = Nc - c0 - wc;   // extent of this wave's quarter
-        const int cbase = (c0 + wc) * ldc + r0 + wr;
-        if (last && beta != 0.0) {
+        const bool b00 = rows > 0 && cols > 0, b01 = rows > 0 && cols > 16;
+        const bool b10 = rows > 16 && cols > 0, b11 = rows > 16 && cols > 16;
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const int k = kk * 4 + (l >> 4);
+            if (lowp) {
+                const float a0 = Af[k * 64 + wr + (l & 15)], a1 = Af[k * 64 + wr + 16 + (l & 15)];
+                const float bb0 = Bf[k * 64 + wc + (l & 15)], bb1 = Bf[k * 64 + wc + 16 + (l & 15)];
+                if (b00) accf[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, bb0, accf[0][0], 0, 0, 0);
+                if (b01) accf[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, bb1, accf[0][1], 0, 0, 0);
+                if (b10) accf[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, bb0, accf[1][0], 0, 0, 0);
+                if (b11) accf[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, bb1, accf[1][1], 0, 0, 0);
+            } else {
+                const double a0 = s.As[k][wr + (l & 15)], a1 = s.As[k][wr + 16 + (l & 15)];
+                const double bb0 = s.Bs[k][wc + (l & 15)], bb1 = s.Bs[k][wc + 16 + (l & 15)];
+                if (b00) acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bb0, acc[0][0], 0, 0, 0);
+                if (b01) acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bb1, acc[0][1], 0, 0, 0);
+                if (b10) acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bb0, acc[1][0], 0, 0, 0);
+                if (b11) acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bb1, acc[1][1], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+        if (k0 + 16 >= K) {
+            // tile done: C (when beta != 0) and the scalings, all loaded before any store
+            const int cbase = (c0 + wc) * ldc + r0 + wr;
+            double cv[2][2][4];
+            double csv[2] = {1.0, 1.0}, rsv[2][4] = {{1.0, 1.0, 1.0, 1.0}, {1.0, 1.0, 1.0, 1.0}};
 #pragma unroll
             for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
@@ -379,49 +416,44 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int Nc, int K, const double* A_
                     for (int e = 0; e < 4; ++e) {
                         const int row = rb * 16 + (lowp ? 4 * (l >> 4) + e : (l >> 4) + 4 * e);
                         const int col = cb * 16 + (l & 15);
-                        cpre[rb][cb][e] = (row < rows && col < cols)
-                                              ? C[cbase + oc + cb * 16 * ldc + rb * 16 + (lowp ? e : 4 * e)]
-                                              : 0.0;
+                        const bool ok = row < rows && col < cols;   // (others are never stored)
+                        cv[rb][cb][e] = beta != 0.0 ? Cg[ok ? cbase + oc + cb * 16 * ldc + rb * 16 + (lowp ? e : 4 * e) : 0]
+                                                    : 0.0;
                     }
-        }
+            if (has_cs) {
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk) {
-            const int k = kk * 4 + (l >> 4);
-            if (lowp) {
-                const float a0 = Af[k * 64 + wr + (l & 15)], a1 = Af[k * 64 + wr + 16 + (l & 15)];
-                const float b0 = Bf[k * 64 + wc + (l & 15)], b1 = Bf[k * 64 + wc + 16 + (l & 15)];
-                if (blk[0][0]) accf[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, accf[0][0], 0, 0, 0);
-                if (blk[0][1]) accf[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, accf[0][1], 0, 0, 0);
-                if (blk[1][0]) accf[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, accf[1][0], 0, 0, 0);
-                if (blk[1][1]) accf[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, accf[1][1], 0, 0, 0);
-            } else {
-                const double a0 = s.As[k][wr + (l & 15)], a1 = s.As[k][wr + 16 + (l & 15)];
-                const double b0 = s.Bs[k][wc + (l & 15)], b1 = s.Bs[k][wc + 16 + (l & 15)];
-                if (blk[0][0]) acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-                if (blk[0][1]) acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-                if (blk[1][0]) acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-                if (blk[1][1]) acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
-            }
-        }
-        __syncthreads();
-        if (!last) continue;
-#pragma unroll
-        for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-            for (int cb = 0; cb < 2; ++cb) {
-                if (!blk[rb][cb]) continue;
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int row = rb * 16 + (lowp ? 4 * (l >> 4) + e : (l >> 4) + 4 * e);
+                for (int cb = 0; cb < 2; ++cb) {
                     const int col = cb * 16 + (l & 15);
-                    if (row >= rows || col >= cols) continue;
-                    double v = (lowp ? (double)accf[rb][cb][e] : acc[rb][cb][e]) * alpha;
-                    if (cs) v *= cs[c0 + wc + col];
-                    if (rs) v *= rs[r0 + wr + row];
-                    double* cp = C + cbase + oc + cb * 16 * ldc + rb * 16 + (lowp ? e : 4 * e);
-                    *cp = (beta != 0.0) ? v + beta * cpre[rb][cb][e] : v;
+                    csv[cb] = csl[col < cols ? c0 + wc + col : 0];
                 }
             }
+            if (has_rs) {
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int row = rb * 16 + (lowp ? 4 * (l >> 4) + e : (l >> 4) + 4 * e);
+                        rsv[rb][e] = rsl[row < rows ? r0 + wr + row : 0];
+                    }
+            }
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                for (int cb = 0; cb < 2; ++cb) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int row = rb * 16 + (lowp ? 4 * (l >> 4) + e : (l >> 4) + 4 * e);
+                        const int col = cb * 16 + (l & 15);
+                        if (row >= rows || col >= cols) continue;
+                        double v = (lowp ? (double)accf[rb][cb][e] : acc[rb][cb][e]) * alpha;
+                        if (has_cs) v *= csv[cb];
+                        if (has_rs) v *= rsv[rb][e];
+                        Cg[cbase + oc + cb * 16 * ldc + rb * 16 + (lowp ? e : 4 * e)] =
+                            (beta != 0.0) ? v + beta * cv[rb][cb][e] : v;
+                    }
+                }
+        }
+        advance(r0, c0, k0);
     }
     __syncthreads();
 }
@@ -880,17 +912,28 @@ __device__ __noinline__ double wg_lu_solve(double* A, double* B, int N, int nrhs
         }
         // the panel's interchanges (row k0+j <-> perm[k0+j], j = 0..jb-1 in order) composed into
         // one gather: row swp_row[i] receives the old row swp_src[i]; rows k0..k0+jb-1 come first
-        if (tid() == 0) {
+        // (wavefront 0: lane i tracks entry i; a ballot finds the lane holding the pivot row)
+        if (wave() == 0) {
+            const int i = lane();
+            int row = i < jb ? k0 + i : -1, src = row;
             int nrow = jb;
-            for (int j = 0; j < jb; ++j) { s.swp_row[j] = k0 + j; s.swp_src[j] = k0 + j; }
             for (int j = 0; j < jb; ++j) {
-                const int pr = s.perm[k0 + j];
-                int ip = -1;
-                for (int i = 0; i < nrow; ++i) if (s.swp_row[i] == pr) ip = i;
-                if (ip < 0) { ip = nrow++; s.swp_row[ip] = pr; s.swp_src[ip] = pr; }
-                const int t = s.swp_src[j]; s.swp_src[j] = s.swp_src[ip]; s.swp_src[ip] = t;
+                const int pr = __builtin_amdgcn_readfirstlane(s.perm[k0 + j]);
+                const unsigned long long hit = __ballot(row == pr);
+                int ip;
+                if (hit) ip = __ffsll((long long)hit) - 1;
+                else {
+                    ip = nrow++;
+                    if (i == ip) { row = pr; src = pr; }
+                }
+                const int sj = __builtin_amdgcn_readlane(src, j), sp = __builtin_amdgcn_readlane(src, ip);
+                if (i == j) src = sp;
+                if (i == ip) src = sj;
             }
-            for (int i = nrow; i < 2 * NB; ++i) s.swp_row[i] = -1;
+            if (i < 2 * NB) {
+                s.swp_row[i] = i < nrow ? row : -1;
+                s.swp_src[i] = src;
+            }
         }
         __syncthreads();
         // interchanges + L11^-1 on the columns right of the panel and on B, one column per thread
@@ -1238,7 +1281,12 @@ __device__ __noinline__ void wg_push(const ChainDesc& d, int sp, int slot, int b
     double* W0 = d.W;
     double* W1 = d.W + (size_t)N * N;
     if (below < 0) wg_copy(W0, A, N * N);
-    else gemm64(N, N, N, A, N, false, slotU(d, sp, below), N, false, W0, N, nullptr, nullptr, slotD(d, sp, below), 0.0, s);
+    else {
+        const double* Db = slotD(d, sp, below);   // the GEMM's column scaling, staged in LDS
+        for (int i = tid(); i < N; i += NT) s.v3[i] = Db[i];
+        __syncthreads();
+        gemm64(N, N, N, A, N, false, slotU(d, sp, below), N, false, W0, N, nullptr, nullptr, s.v3, 0.0, s);
+    }
     wg_qr_udv(W0, d.W + 2 * (size_t)N * N, N, slotU(d, sp, slot), slotD(d, sp, slot), W1, d.Wq, s);
     __syncthreads();
     if (below < 0) wg_copy(slotT(d, sp, slot), W1, N * N);
@@ -1370,7 +1418,7 @@ __device__ __forceinline__ int block_depth(const ChainDesc& d) {
 // The block's update of G for one spin, on the matrix cores with operands in LDS:
 //   W = C (G0[S,:] - E_S)   (k x N; C from CR, G0[S,:] staged in GSW, W written over it)
 //   G += G0[:,S] W           (G0[:,S] staged in ACP; G read-modify-written in 16x16 blocks,
-//                             8 blocks per wavefront at a time with all their loads in flight)
+//                             4 blocks per wavefront at a time with all their loads in flight)
 __device__ __noinline__ void block_update(double* G_, int N, int b0, int k, int sp, Lds&) {
     Lds& s = g_lds;
     N = uni(N); b0 = uni(b0); k = uni(k); sp = uni(sp);
@@ -1416,7 +1464,7 @@ __device__ __noinline__ void block_update(double* G_, int N, int b0, int k, int 
     __syncthreads();
     // G += ACP W
     const int nbr = nbc, nblk = nbr * nbc;
-    constexpr int CH = 8;
+    constexpr int CH = 4;
     for (int q0 = w; q0 < nblk; q0 += 4 * CH) {
         double cv[CH][4];
 #pragma unroll
@@ -1430,20 +1478,24 @@ __device__ __noinline__ void block_update(double* G_, int N, int b0, int k, int 
                 cv[u][i] = (q < nblk && row < N && col < N) ? G[(size_t)col * N + row] : 0.0;
             }
         }
+        // the chunk's blocks advance together: per 4-deep step all operands are read
+        // first, then CH independent MFMAs issue back to back
         dbl4 acc[CH];
+        int oa[CH], ob[CH];
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
             acc[u] = dbl4{0.0, 0.0, 0.0, 0.0};
-            const int q = q0 + 4 * u;
-            if (q < nblk) {
-                const int rb = q % nbr, cb = q / nbr;
-                for (int kk = 0; kk < k4; kk += 4) {
-                    const int kr = kk + (l >> 4);
-                    const double av = acp[kr * N + rb * 16 + (l & 15)];
-                    const double bv = gsw[kr * N + cb * 16 + (l & 15)];
-                    acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[u], 0, 0, 0);
-                }
-            }
+            const int q = min(q0 + 4 * u, nblk - 1);   // (a block past the end repeats the last; not stored)
+            oa[u] = (q % nbr) * 16 + (l & 15);
+            ob[u] = (q / nbr) * 16 + (l & 15);
+        }
+        for (int kk = 0; kk < k4; kk += 4) {
+            const int kr = (kk + (l >> 4)) * N;
+            double av[CH], bv[CH];
+#pragma unroll
+            for (int u = 0; u < CH; ++u) { av[u] = acp[kr + oa[u]]; bv[u] = gsw[kr + ob[u]]; }
+#pragma unroll
+            for (int u = 0; u < CH; ++u) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc[u], 0, 0, 0);
         }
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
@@ -1870,10 +1922,18 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1)))
                                              const double* ms, const double* cs, double beta) {
     Lds& s = g_lds;
     if (variant == 0) { wg_gemm(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, s); return; }
-    // the matrix-core GEMM takes its middle scaling from LDS
+    // the matrix-core GEMM takes its scalings from LDS
     if (ms) {
         for (int i = tid(); i < K; i += NT) s.v0[i] = ms[i];
         ms = s.v0;
+    }
+    if (rs) {
+        for (int i = tid(); i < M; i += NT) s.v1[i] = rs[i];
+        rs = s.v1;
+    }
+    if (cs) {
+        for (int i = tid(); i < Nc; i += NT) s.v2[i] = cs[i];
+        cs = s.v2;
     }
     // variants 3 and 4 stage A (and B) in LDS first, as the engine's in-LDS operands are
     const size_t na = (size_t)M * K, nb = (size_t)K * Nc;
