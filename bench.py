@@ -72,6 +72,15 @@ def algorithmic_flops_per_chain_sweep(w, accepted_per_chain_sweep):
     return (wrap + prod + upd + stab) * spins / 2.0
 
 
+def autotune(w, candidates=(8, 16, 32, 64)):
+    """Delayed-update depth for the workload's chain shape, chosen by the engine's
+    autotuner (each candidate timed on a full batch of the workload's size)."""
+    ctx = D.Context(0)
+    best, ms = ctx.autotune_delay(chain_params(w, 1)[0], w["chains"], candidates=candidates, sweeps=2)
+    ctx.close()
+    return best, ms
+
+
 def run_gpu(w, steps, warmup, chains=None):
     ctx = D.Context(0)
     b = D.Batch(ctx, chain_params(w, chains))
@@ -159,6 +168,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the single-chain comparison run")
     ap.add_argument("--phases", action="store_true", help="also report device time per sweep phase")
+    ap.add_argument("--no-autotune", action="store_true", help="use the workload's fixed delay depth")
     ap.add_argument("--serve-jobs", type=int, default=48, help="jobs of the serving sample (0 skips it)")
     ap.add_argument("--serve-scale", type=float, default=0.05, help="sweep-count scale of the serving sample")
     ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrivals per second")
@@ -180,7 +190,12 @@ def main():
         print(f"bench.py runs on one GPU: chains are independent and would be sharded by chain across GPUs, "
               f"but only the single-GPU figure is measured here (asked for --gpus {args.gpus}).", file=sys.stderr)
         sys.exit(2)
-    w = WORKLOADS[args.workload]
+    w = dict(WORKLOADS[args.workload])
+    tune = None
+    if not args.no_autotune and w["chains"] > 1:
+        best, tune = autotune(w)
+        w["delay"] = best
+        print(f"[bench] delay depth autotuned: {best} (ms per 2 sweeps {tune})", file=sys.stderr, flush=True)
     print(f"[bench] {w['desc']}: warmup {args.warmup}, steps {args.steps}", file=sys.stderr, flush=True)
     r = run_gpu(w, args.steps, args.warmup)
     chain_sweeps = r["chains"] * args.steps
@@ -200,6 +215,8 @@ def main():
                      "kernel": "k_sweep", "kernel_ms": kms, "algorithmic_gflop_per_launch": flops_launch / 1e9},
         "accepted_per_chain_sweep": r["accepted"], "max_wrap_error": r["wrap_error"],
     }
+    if tune is not None:
+        out["config"]["delay_autotune_ms"] = tune
     if not args.no_plain and r["chains"] > 1:
         print("[bench] plain path: the same chain alone, one chain per launch", file=sys.stderr, flush=True)
         p = run_gpu(w, max(2, args.steps // 4), 1, chains=1)

@@ -353,7 +353,6 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int Nc, int K, const double* A_
     advance(lr, lc, lk);
     dbl4 acc[2][2];
     flt4 accf[2][2];
-    const int oc = (l & 15) * ldc + (lowp ? 4 * (l >> 4) : (l >> 4));
     while (c0 < Nc) {
         if (k0 == 0) {
 #pragma unroll
@@ -386,55 +385,60 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int Nc, int K, const double* A_
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
             const int k = kk * 4 + (l >> 4);
+            // the tile is formed transposed (B^T A^T): lane l then holds C rows l%16 of its
+            // block, so the C reads and writes of the epilogue run down columns
             if (lowp) {
                 const float a0 = Af[k * 64 + wr + (l & 15)], a1 = Af[k * 64 + wr + 16 + (l & 15)];
                 const float bb0 = Bf[k * 64 + wc + (l & 15)], bb1 = Bf[k * 64 + wc + 16 + (l & 15)];
-                if (b00) accf[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, bb0, accf[0][0], 0, 0, 0);
-                if (b01) accf[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, bb1, accf[0][1], 0, 0, 0);
-                if (b10) accf[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, bb0, accf[1][0], 0, 0, 0);
-                if (b11) accf[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, bb1, accf[1][1], 0, 0, 0);
+                if (b00) accf[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(bb0, a0, accf[0][0], 0, 0, 0);
+                if (b01) accf[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(bb1, a0, accf[0][1], 0, 0, 0);
+                if (b10) accf[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(bb0, a1, accf[1][0], 0, 0, 0);
+                if (b11) accf[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(bb1, a1, accf[1][1], 0, 0, 0);
             } else {
                 const double a0 = s.As[k][wr + (l & 15)], a1 = s.As[k][wr + 16 + (l & 15)];
                 const double bb0 = s.Bs[k][wc + (l & 15)], bb1 = s.Bs[k][wc + 16 + (l & 15)];
-                if (b00) acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bb0, acc[0][0], 0, 0, 0);
-                if (b01) acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bb1, acc[0][1], 0, 0, 0);
-                if (b10) acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bb0, acc[1][0], 0, 0, 0);
-                if (b11) acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bb1, acc[1][1], 0, 0, 0);
+                if (b00) acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(bb0, a0, acc[0][0], 0, 0, 0);
+                if (b01) acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(bb1, a0, acc[0][1], 0, 0, 0);
+                if (b10) acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(bb0, a1, acc[1][0], 0, 0, 0);
+                if (b11) acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(bb1, a1, acc[1][1], 0, 0, 0);
             }
         }
         __syncthreads();
         if (k0 + 16 >= K) {
             // tile done: C (when beta != 0) and the scalings, all loaded before any store
             const int cbase = (c0 + wc) * ldc + r0 + wr;
+            // element e of acc[rb][cb]: row rb*16 + l%16, column cb*16 + (lowp ? 4(l/16) + e : l/16 + 4e)
+            const int ocl = (l & 15) + (lowp ? 4 * (l >> 4) : (l >> 4)) * ldc;
             double cv[2][2][4];
-            double csv[2] = {1.0, 1.0}, rsv[2][4] = {{1.0, 1.0, 1.0, 1.0}, {1.0, 1.0, 1.0, 1.0}};
+            double rsv[2] = {1.0, 1.0}, csv[2][4] = {{1.0, 1.0, 1.0, 1.0}, {1.0, 1.0, 1.0, 1.0}};
 #pragma unroll
             for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
                 for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
-                        const int row = rb * 16 + (lowp ? 4 * (l >> 4) + e : (l >> 4) + 4 * e);
-                        const int col = cb * 16 + (l & 15);
+                        const int row = rb * 16 + (l & 15);
+                        const int col = cb * 16 + (lowp ? 4 * (l >> 4) + e : (l >> 4) + 4 * e);
                         const bool ok = row < rows && col < cols;   // (others are never stored)
-                        cv[rb][cb][e] = beta != 0.0 ? Cg[ok ? cbase + oc + cb * 16 * ldc + rb * 16 + (lowp ? e : 4 * e) : 0]
-                                                    : 0.0;
+                        cv[rb][cb][e] = beta != 0.0
+                                            ? Cg[ok ? cbase + ocl + (cb * 16 + (lowp ? e : 4 * e)) * ldc + rb * 16 : 0]
+                                            : 0.0;
                     }
             if (has_cs) {
 #pragma unroll
-                for (int cb = 0; cb < 2; ++cb) {
-                    const int col = cb * 16 + (l & 15);
-                    csv[cb] = csl[col < cols ? c0 + wc + col : 0];
-                }
+                for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int col = cb * 16 + (lowp ? 4 * (l >> 4) + e : (l >> 4) + 4 * e);
+                        csv[cb][e] = csl[col < cols ? c0 + wc + col : 0];
+                    }
             }
             if (has_rs) {
 #pragma unroll
-                for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const int row = rb * 16 + (lowp ? 4 * (l >> 4) + e : (l >> 4) + 4 * e);
-                        rsv[rb][e] = rsl[row < rows ? r0 + wr + row : 0];
-                    }
+                for (int rb = 0; rb < 2; ++rb) {
+                    const int row = rb * 16 + (l & 15);
+                    rsv[rb] = rsl[row < rows ? r0 + wr + row : 0];
+                }
             }
 #pragma unroll
             for (int rb = 0; rb < 2; ++rb)
@@ -442,13 +446,13 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int Nc, int K, const double* A_
                 for (int cb = 0; cb < 2; ++cb) {
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
-                        const int row = rb * 16 + (lowp ? 4 * (l >> 4) + e : (l >> 4) + 4 * e);
-                        const int col = cb * 16 + (l & 15);
+                        const int row = rb * 16 + (l & 15);
+                        const int col = cb * 16 + (lowp ? 4 * (l >> 4) + e : (l >> 4) + 4 * e);
                         if (row >= rows || col >= cols) continue;
                         double v = (lowp ? (double)accf[rb][cb][e] : acc[rb][cb][e]) * alpha;
-                        if (has_cs) v *= csv[cb];
-                        if (has_rs) v *= rsv[rb][e];
-                        Cg[cbase + oc + cb * 16 * ldc + rb * 16 + (lowp ? e : 4 * e)] =
+                        if (has_cs) v *= csv[cb][e];
+                        if (has_rs) v *= rsv[rb];
+                        Cg[cbase + ocl + (cb * 16 + (lowp ? e : 4 * e)) * ldc + rb * 16] =
                             (beta != 0.0) ? v + beta * cv[rb][cb][e] : v;
                     }
                 }
@@ -1429,35 +1433,57 @@ __device__ __noinline__ void block_update(double* G_, int N, int b0, int k, int 
     double* const gsw = acp + (size_t)N * k4;
     const double* const cr = MP_CR(sp);
     // stage G0[:,S] (columns b0..b0+k) and G0[S,:] (rows b0..b0+k), zero-padded to k4
-    wg_stream(N * k4, [&](int e) { const int c = e / N; return c < k ? G[(size_t)(b0 + c) * N + (e - c * N)] : 0.0; },
-              [&](int e, double v) { acp[e] = v; });
-    wg_stream(N * k4, [&](int e) { const int b = e / N, c = e - b * N; return b < k ? G[(size_t)c * N + b0 + b] : 0.0; },
-              [&](int e, double v) { gsw[e] = v; });
+    wg_stream(N * k4, [&](int e) { return G[(size_t)b0 * N + min(e, N * k - 1)]; },
+              [&](int e, double v) { acp[e] = e < N * k ? v : 0.0; });
+    // (element e = c k4 + b: consecutive threads read down a column of G)
+    wg_stream(N * k4, [&](int e) { const int c = e / k4, b = e - c * k4; return G[(size_t)c * N + b0 + min(b, k - 1)]; },
+              [&](int e, double v) { const int c = e / k4, b = e - c * k4; gsw[b * N + c] = b < k ? v : 0.0; });
     __syncthreads();
     // W = C G0[S,:] - C E_S, column block by column block (each wavefront owns whole column blocks,
-    // so W may overwrite GSW in place)
+    // so W may overwrite GSW in place); a wavefront advances up to 4 of its column blocks together.
+    // All loads are unconditional (clamped indices) so the compiler can keep them in flight.
     const int nbc = (N + 15) >> 4;
-    for (int jb = w; jb < nbc; jb += 4) {
-        const int col = jb * 16 + (l & 15);
-        dbl4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+    const int lr = l & 15, lk = l >> 4;
+    for (int j0 = w; j0 < nbc; j0 += 16) {
+        dbl4 acc0[4], acc1[4];
+        int colq[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            acc0[u] = dbl4{0.0, 0.0, 0.0, 0.0};
+            acc1[u] = dbl4{0.0, 0.0, 0.0, 0.0};
+            colq[u] = min((j0 + 4 * u) * 16 + lr, N - 1);   // a column past N repeats column N-1; not stored
+        }
         for (int kk = 0; kk < k4; kk += 4) {
-            const int kr = kk + (l >> 4);
-            const double bv = col < N ? gsw[kr * N + col] : 0.0;
-            const double a0 = cr[(l & 15) * KP + kr];
-            acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv, acc0, 0, 0, 0);
-            if (k4 > 16) {
-                const double a1 = cr[(16 + (l & 15)) * KP + kr];
-                acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv, acc1, 0, 0, 0);
+            const int kr = kk + lk;
+            const double a0 = cr[lr * KP + kr], a1 = cr[(16 + lr) * KP + kr];
+            double bv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) bv[u] = gsw[kr * N + colq[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                acc0[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv[u], acc0[u], 0, 0, 0);
+                acc1[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv[u], acc1[u], 0, 0, 0);
             }
         }
-        if (col < N) {
+        // rows 16..31 of C are zero when k <= 16, so acc1 is then zero and not stored
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int col = (j0 + 4 * u) * 16 + lr;
+            const bool colok = j0 + 4 * u < nbc && col < N;
             const bool in_s = col >= b0 && col < b0 + k;
+            const int cs = min(max(col - b0, 0), KB - 1);
+            double corr0[4], corr1[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int a = (l >> 4) + 4 * i;
-                if (a < k4) gsw[a * N + col] = acc0[i] - ((in_s && a < k) ? cr[a * KP + col - b0] : 0.0);
-                const int a1 = 16 + a;
-                if (k4 > 16 && a1 < k4) gsw[a1 * N + col] = acc1[i] - ((in_s && a1 < k) ? cr[a1 * KP + col - b0] : 0.0);
+                const int a = lk + 4 * i;
+                corr0[i] = cr[a * KP + cs];
+                corr1[i] = cr[(16 + a) * KP + cs];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int a = lk + 4 * i;
+                if (colok && a < k4) gsw[a * N + col] = acc0[u][i] - ((in_s && a < k) ? corr0[i] : 0.0);
+                if (colok && 16 + a < k4) gsw[(16 + a) * N + col] = acc1[u][i] - ((in_s && 16 + a < k) ? corr1[i] : 0.0);
             }
         }
     }
@@ -1466,45 +1492,40 @@ __device__ __noinline__ void block_update(double* G_, int N, int b0, int k, int 
     const int nbr = nbc, nblk = nbr * nbc;
     constexpr int CH = 4;
     for (int q0 = w; q0 < nblk; q0 += 4 * CH) {
-        double cv[CH][4];
-#pragma unroll
-        for (int u = 0; u < CH; ++u) {
-            const int q = q0 + 4 * u;
-            const int rb = q % nbr, cb = q / nbr;
-            const int col = cb * 16 + (l & 15);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int row = rb * 16 + (l >> 4) + 4 * i;
-                cv[u][i] = (q < nblk && row < N && col < N) ? G[(size_t)col * N + row] : 0.0;
-            }
-        }
         // the chunk's blocks advance together: per 4-deep step all operands are read
         // first, then CH independent MFMAs issue back to back
+        double cv[CH][4];
         dbl4 acc[CH];
-        int oa[CH], ob[CH];
+        int oa[CH], ob[CH], og[CH];
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
-            acc[u] = dbl4{0.0, 0.0, 0.0, 0.0};
             const int q = min(q0 + 4 * u, nblk - 1);   // (a block past the end repeats the last; not stored)
-            oa[u] = (q % nbr) * 16 + (l & 15);
-            ob[u] = (q / nbr) * 16 + (l & 15);
+            const int rb = q % nbr, cb = q / nbr;
+            oa[u] = rb * 16 + lr;
+            ob[u] = cb * 16 + lr;
+            // the product is formed transposed (W^T ACP^T), so lane l holds the G elements
+            // (row rb*16 + l%16, column cb*16 + l/16 + 4i): 16 lanes run down one column
+            og[u] = min(rb * 16 + lr, N - 1);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) cv[u][i] = G[(size_t)min(cb * 16 + lk + 4 * i, N - 1) * N + og[u]];
+            acc[u] = dbl4{0.0, 0.0, 0.0, 0.0};
         }
         for (int kk = 0; kk < k4; kk += 4) {
-            const int kr = (kk + (l >> 4)) * N;
+            const int kr = (kk + lk) * N;
             double av[CH], bv[CH];
 #pragma unroll
             for (int u = 0; u < CH; ++u) { av[u] = acp[kr + oa[u]]; bv[u] = gsw[kr + ob[u]]; }
 #pragma unroll
-            for (int u = 0; u < CH; ++u) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc[u], 0, 0, 0);
+            for (int u = 0; u < CH; ++u) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[u], av[u], acc[u], 0, 0, 0);
         }
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
             const int q = q0 + 4 * u;
             const int rb = q % nbr, cb = q / nbr;
-            const int col = cb * 16 + (l & 15);
+            const int row = rb * 16 + lr;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int row = rb * 16 + (l >> 4) + 4 * i;
+                const int col = cb * 16 + lk + 4 * i;
                 if (q < nblk && row < N && col < N) G[(size_t)col * N + row] = cv[u][i] + acc[u][i];
             }
         }
