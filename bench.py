@@ -44,6 +44,21 @@ WORKLOADS = {
 
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+# rocprofv3 summary holding the measured HBM bytes per k_sweep launch of a workload
+# (FETCH_SIZE and WRITE_SIZE passes scaled by a copy of known size)
+TRAFFIC_PROFILES = {"batched-12x12": "profiles/k_sweep_256x12x12_L60_checkerboard_fp64.md"}
+
+
+def profiled_traffic(name):
+    """HBM bytes per launch recorded by the workload's profile summary, or None."""
+    path = TRAFFIC_PROFILES.get(name)
+    if path is None or not os.path.exists(os.path.join(ROOT, path)):
+        return None, None
+    with open(os.path.join(ROOT, path)) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1]), path
+    return None, None
 
 
 def chain_params(w, chains=None, seed0=0):
@@ -204,6 +219,7 @@ def main():
     kms = sum(r["kernel_ms"]) / len(r["kernel_ms"])
     flops_launch = algorithmic_flops_per_chain_sweep(w, r["accepted"]) * r["chains"]
     achieved = flops_launch / (kms * 1e-3) / 1e12
+    traffic, traffic_src = profiled_traffic(args.workload)
     out = {
         "metric": METRIC, "value": value, "unit": "chain-sweeps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
@@ -211,12 +227,15 @@ def main():
         "config": {"workload": w["desc"], "chains": r["chains"], "side": w["side"], "time_size": w["time_size"],
                    "beta": w["beta"], "pace": w["pace"], "checkerboard": w["checkerboard"], "delay": w["delay"]},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "kernel": "k_sweep", "kernel_ms": kms, "algorithmic_gflop_per_launch": flops_launch / 1e9},
         "accepted_per_chain_sweep": r["accepted"], "max_wrap_error": r["wrap_error"],
     }
     if tune is not None:
         out["config"]["delay_autotune_ms"] = tune
+    if traffic is not None:
+        out["roofline"]["traffic_source"] = f"{traffic_src} (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated)"
+        out["roofline"]["traffic_tbs_at_kernel_ms"] = traffic / (kms * 1e-3) / 1e12
     if not args.no_plain and r["chains"] > 1:
         print("[bench] plain path: the same chain alone, one chain per launch", file=sys.stderr, flush=True)
         p = run_gpu(w, max(2, args.steps // 4), 1, chains=1)
