@@ -58,8 +58,6 @@ struct ChainDesc {
     double* stkD;        // nspin x n_stack x N
     double* W;           // 3 x N*N scratch for the stabilization
     double* Wq;          // 3 x 16 x (N + 16) scratch of the blocked QR (GEMM panels and T factors)
-    double* X;           // nspin x N*kdelay   delayed-update columns
-    double* Y;           // nspin x N*kdelay   delayed-update rows
     const double* expK;  // N*N (dense only)
     const double* invExpK;
     const double* cosq;  // N: Re e^{-i Q r_d} for displacement index d

@@ -189,6 +189,44 @@ __device__ __forceinline__ double canonical(uint64_t& x) {
     return ret;
 }
 
+// minstd_rand0 is a Lehmer generator, x_{n+j} = 16807^j x_n mod (2^31 - 1), so
+// the draws of a whole Metropolis block can be formed at once: the canonical
+// double of site p of a block uses draws 2p+1 and 2p+2 after the block's start.
+struct MinstdPowers {
+    uint64_t v[2 * 32 + 2];
+};
+constexpr MinstdPowers make_minstd_powers() {
+    MinstdPowers t{};
+    uint64_t x = 1;
+    for (int i = 0; i < 2 * 32 + 2; ++i) { t.v[i] = x; x = (x * 16807ull) % 0x7fffffffull; }
+    return t;
+}
+__constant__ MinstdPowers kMinstdPow = make_minstd_powers();
+
+__device__ __forceinline__ uint64_t mulmod_m31(uint64_t x, uint64_t y) {   // x, y < 2^31 - 1
+    const uint64_t p = x * y;                                          // < 2^62
+    uint64_t r = (p & 0x7fffffffull) + (p >> 31);                      // < 2^32
+    r = (r & 0x7fffffffull) + (r >> 31);                               // <= 2^31
+    if (r >= 0x7fffffffull) r -= 0x7fffffffull;
+    return r;
+}
+// the canonical double of site p of a block starting at state x (two draws per site),
+// with the same operations as canonical()
+__device__ __forceinline__ double block_canonical(uint64_t x, int p) {
+#pragma clang fp contract(off)
+    const double R = 2147483646.0;
+    const double a = (double)(mulmod_m31(x, kMinstdPow.v[2 * p + 1]) - 1);
+    const double b = (double)(mulmod_m31(x, kMinstdPow.v[2 * p + 2]) - 1);
+    double sum = 0.0, tmp = 1.0;
+    sum = sum + a * tmp;
+    tmp = tmp * R;
+    sum = sum + b * tmp;
+    tmp = tmp * R;
+    double ret = sum / tmp;
+    if (ret >= 1.0) ret = 0x1.fffffffffffffp-1;
+    return ret;
+}
+
 // ---------------------------------------------------------------------------
 // Workgroup GEMM (FP64):  C = rs .* (opA(A) diag(ms) opB(B)) .* cs + beta*C
 // column-major; 64x64 output tiles, 4x4 register micro-tiles per thread,
@@ -1565,6 +1603,8 @@ __device__ __forceinline__ WalkOut walk_block(const ChainDesc& d, int e, int b0,
     for (int b = 0; b < KB; ++b) { crr[b] = 0.0; ccr[b] = 0.0; }
     double dq = gs[a * KP + a];
     int nacc = 0;
+    // the block's k uniform draws, one per lane (site a), bit-identical to drawing them in order
+    const double draw = block_canonical(rng, a < k ? a : 0);
     for (int p = 0; p < k; ++p) {
         const double dp0 = readlane_d(dq, p);
         const double dp1 = ns == 2 ? readlane_d(dq, 32 + p) : dp0;
@@ -1576,7 +1616,7 @@ __device__ __forceinline__ WalkOut walk_block(const ChainDesc& d, int e, int b0,
         const double x1 = 1.0 + (1.0 - dp1) * m1;
         const double r = (up ? pref0 : pref1) * x0 * x1;
         const double pr = fmin(1.0, fabs(r));
-        const int ok = __builtin_amdgcn_readfirstlane(canonical(rng) < pr ? 1 : 0);
+        const int ok = __builtin_amdgcn_readfirstlane(readlane_d(draw, p) < pr ? 1 : 0);
         if (record && lane() == 0) rec_row[j] = (uint8_t)ok;
         if (!ok) continue;
         ++nacc;
@@ -1587,35 +1627,37 @@ __device__ __forceinline__ WalkOut walk_block(const ChainDesc& d, int e, int b0,
         }
         const double g = h == 0 ? m0 / x0 : m1 / x1;
         // u_a = d_ap + (C (Gss - I))(a,p),  v_a = d_ap + (Gss C)(p,a)
-        double t = 0.0, wv = 0.0;
+        // (four partial sums each, so the FMA dependency chains are 8 long)
+        double t4[4] = {0.0, 0.0, 0.0, 0.0}, w4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int b = 0; b < KB; ++b) {
-            t += crr[b] * (gs[b * KP + p] - (b == p ? 1.0 : 0.0));
-            wv += gs[p * KP + b] * ccr[b];
+            t4[b & 3] += crr[b] * (gs[b * KP + p] - (b == p ? 1.0 : 0.0));
+            w4[b & 3] += gs[p * KP + b] * ccr[b];
         }
-        const double u = (a == p ? 1.0 : 0.0) + t;
-        const double v = (a == p ? 1.0 : 0.0) + wv;
+        const double u = (a == p ? 1.0 : 0.0) + ((t4[0] + t4[1]) + (t4[2] + t4[3]));
+        const double v = (a == p ? 1.0 : 0.0) + ((w4[0] + w4[1]) + (w4[2] + w4[3]));
         ush[a] = u;
         vsh[a] = v;
         __builtin_amdgcn_wave_barrier();
         // C += g u v^T (row a and column a), and the block diagonal
         // d_a += g (Gss u)_a (v^T (Gss - I))_a
-        double x = 0.0, y = 0.0;
+        double x4[4] = {0.0, 0.0, 0.0, 0.0}, y4[4] = {0.0, 0.0, 0.0, 0.0};
         const double gu = g * u, gv = g * v;
 #pragma unroll
         for (int b = 0; b < KB; ++b) {
             const double ub = ush[b], vb = vsh[b];
             crr[b] += gu * vb;
             ccr[b] += ub * gv;
-            x += gs[a * KP + b] * ub;
-            y += vb * (gs[b * KP + a] - (b == a ? 1.0 : 0.0));
+            x4[b & 3] += gs[a * KP + b] * ub;
+            y4[b & 3] += vb * (gs[b * KP + a] - (b == a ? 1.0 : 0.0));
         }
         __builtin_amdgcn_wave_barrier();
-        dq += g * x * y;
+        dq += g * ((x4[0] + x4[1]) + (x4[2] + x4[3])) * ((y4[0] + y4[1]) + (y4[2] + y4[3]));
     }
     double* const cro = MP_CR(h) + a * KP;
 #pragma unroll
     for (int b = 0; b < KB; ++b) cro[b] = crr[b];
+    rng = mulmod_m31(rng, kMinstdPow.v[2 * k]);   // the block consumed 2k draws
     return WalkOut{rng, sign, nacc};
 }
 

@@ -93,8 +93,6 @@ Layout chain_layout(int N, int L, int n_stack, int nspin, int kdelay, bool dense
     l.stkD = take((size_t)nspin * n_stack * N * sizeof(double));
     l.W = take(3 * n2);
     l.Wq = take((size_t)3 * 16 * (N + 16) * sizeof(double));
-    l.X = take((size_t)nspin * N * kdelay * sizeof(double));
-    l.Y = take((size_t)nspin * N * kdelay * sizeof(double));
     l.expK = dense ? take(n2) : o;
     l.invExpK = dense ? take(n2) : o;
     l.cosq = take(N * sizeof(double));
@@ -269,7 +267,7 @@ int batch_bind(dqmc_batch* b) {
         }
         d.G = (double*)(base + l.G); d.Gw = (double*)(base + l.Gw); d.Tmp = (double*)(base + l.Tmp);
         d.stkU = (double*)(base + l.stkU); d.stkT = (double*)(base + l.stkT); d.stkD = (double*)(base + l.stkD);
-        d.W = (double*)(base + l.W); d.Wq = (double*)(base + l.Wq); d.X = (double*)(base + l.X); d.Y = (double*)(base + l.Y);
+        d.W = (double*)(base + l.W); d.Wq = (double*)(base + l.Wq);
         d.expK = p.checkerboard ? nullptr : (const double*)(base + l.expK);
         d.invExpK = p.checkerboard ? nullptr : (const double*)(base + l.invExpK);
         d.cosq = (const double*)(base + l.cosq);

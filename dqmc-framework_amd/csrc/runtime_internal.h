@@ -31,7 +31,7 @@ struct HostChain {
 };
 
 struct Layout {
-    size_t G, Gw, Tmp, stkU, stkT, stkD, W, Wq, X, Y, expK, invExpK, cosq, field, st, acc, recG, recSign, recAcc, total;
+    size_t G, Gw, Tmp, stkU, stkT, stkD, W, Wq, expK, invExpK, cosq, field, st, acc, recG, recSign, recAcc, total;
 };
 
 struct dqmc_batch {
