@@ -1436,41 +1436,55 @@ __device__ __noinline__ void wg_fresh_green(const ChainDesc& d, int sp, int lslo
 // W = C (G0[S,:] - E_S) and G += G0[:,S] W on the matrix cores.
 #define KB 32   // one half-wavefront per spin in the walk
 
-// LDS layout of the Metropolis phase (doubles from s.panel, KP = KB + 1):
-//   CR(sp)  = row a of the block's C at a*KP, both spins (kept until the block lands in G)
-//   GSS(sp) = G0[S,S] (row a at a*KP), CC(sp) = column a of C at a*KP, UV = the walk's u, v
-//   after the walk the space from OVER holds, per spin in turn, ACP = G0[:,S]
-//   (N x k4, ld N) and GSW = G0[S,:] (k4 x N, ld N), which becomes W in place.
+// LDS layout of the Metropolis phase (KP = KB + 1).  The walk's arrays live in
+// the front of the LDS object (the GEMM staging tiles and the scaling and
+// scratch vectors, none of which the Metropolis phase uses; the scalings are
+// recomputed after it):
+//   CR(sp)  = row a of the block's C at a*KP (written at the end of the walk)
+//   GSS(sp) = G0[S,S] (row a at a*KP), zero outside the k x k block
+//   UV      = the walk's u, v broadcast vectors
+// The panel holds, for one spin at a time, ACP = G0[:,S] (N x k4, ld N) and
+// GSW = G0[S,:] (k4 x N, ld N), which becomes W in place.  Because the walk
+// does not touch the panel, the last spin's G += ACP W of a block runs on
+// wavefronts 1..3 while wavefront 0 walks the next block.
 constexpr int KP = KB + 1;
-#define MP_CR(sp) (s.panel + (sp) * KB * KP)
-#define MP_GSS(sp) (s.panel + 2 * KB * KP + (sp) * 2 * KB * KP)
-#define MP_CC(sp) (MP_GSS(sp) + KB * KP)
-#define MP_UV (s.panel + 6 * KB * KP)
-#define MP_OVER (2 * KB * KP)
+#define MP_BASE (reinterpret_cast<double*>(&g_lds))
+#define MP_CR(sp) (MP_BASE + (sp) * KB * KP)
+#define MP_GSS(sp) (MP_BASE + 2 * KB * KP + (sp) * KB * KP)
+#define MP_UV (MP_BASE + 4 * KB * KP)
 #define MP_SLACK 16   // reads of partial 16x16 blocks run up to 16 doubles past GSW
+static_assert(offsetof(Lds, As) == 0 && offsetof(Lds, perm) >= (4 * KB * (KB + 1) + 128) * sizeof(double),
+              "the walk's arrays must fit in front of Lds::perm");
 
 // block size: the delay depth, at most KB, and small enough that ACP and GSW fit
 __device__ __forceinline__ int block_depth(const ChainDesc& d) {
     const int N = d.N;
     int kb = min(d.kdelay, KB);
-    const int cap = ((PANEL_DOUBLES - MP_OVER - MP_SLACK) / (2 * N)) & ~3;
+    const int cap = ((PANEL_DOUBLES - MP_SLACK) / (2 * N)) & ~3;
     return max(1, min(kb, cap));
 }
 
-// The block's update of G for one spin, on the matrix cores with operands in LDS:
-//   W = C (G0[S,:] - E_S)   (k x N; C from CR, G0[S,:] staged in GSW, W written over it)
-//   G += G0[:,S] W           (G0[:,S] staged in ACP; G read-modify-written in 16x16 blocks,
-//                             4 blocks per wavefront at a time with all their loads in flight)
-__device__ __noinline__ void block_update(double* G_, int N, int b0, int k, int sp, Lds&) {
+// G0[S,S] of block (b0, k) of one spin into GSS, zero outside the k x k block
+__device__ void stage_gss(const double* G_, int N, int b0, int k, int sp, Lds&) {
+    const AS_G double* Gg = gptr(G_);
+    double* gss = MP_GSS(sp);
+    for (int q = tid(); q < KB * KP; q += NT) gss[q] = 0.0;
+    __syncthreads();
+    wg_stream(k * k, [&](int q) { const int a = q % k, c = q / k; return Gg[(size_t)(b0 + c) * N + b0 + a]; },
+              [&](int q, double v) { const int a = q % k, c = q / k; gss[a * KP + c] = v; });
+}
+
+// The block's update of G for one spin, on the matrix cores with operands in LDS.
+// block_stage_w:  ACP = G0[:,S], GSW = G0[S,:], then W = C (G0[S,:] - E_S) over GSW
+__device__ __noinline__ void block_stage_w(const double* G_, int N, int b0, int k, int sp, Lds&) {
     Lds& s = g_lds;
     N = uni(N); b0 = uni(b0); k = uni(k); sp = uni(sp);
-    AS_G double* G = gptr(uni(G_));
+    const AS_G double* G = gptr(uni(G_));
     const int k4 = (k + 3) & ~3;
     const int l = lane(), w = wave();
-    double* const acp = s.panel + MP_OVER;
+    double* const acp = s.panel;
     double* const gsw = acp + (size_t)N * k4;
     const double* const cr = MP_CR(sp);
-    // stage G0[:,S] (columns b0..b0+k) and G0[S,:] (rows b0..b0+k), zero-padded to k4
     wg_stream(N * k4, [&](int e) { return G[(size_t)b0 * N + min(e, N * k - 1)]; },
               [&](int e, double v) { acp[e] = e < N * k ? v : 0.0; });
     // (element e = c k4 + b: consecutive threads read down a column of G)
@@ -1526,26 +1540,37 @@ __device__ __noinline__ void block_update(double* G_, int N, int b0, int k, int 
         }
     }
     __syncthreads();
-    // G += ACP W
-    const int nbr = nbc, nblk = nbr * nbc;
+}
+
+// block_apply:  G += ACP W on wavefronts wave_lo..3 (no workgroup barrier inside, so it can run
+// beside the walk of wavefront 0).  The product is formed transposed (W^T ACP^T), so 16 lanes
+// run down one column of G; 4 blocks per wavefront advance together with all their loads in flight.
+__device__ __noinline__ void block_apply(double* G_, int N, int k, int wave_lo, Lds&) {
+    Lds& s = g_lds;
+    N = uni(N); k = uni(k); wave_lo = uni(wave_lo);
+    AS_G double* G = gptr(uni(G_));
+    const int k4 = (k + 3) & ~3;
+    const int l = lane(), w = wave();
+    if (w < wave_lo) return;
+    const int nw = 4 - wave_lo, wi = w - wave_lo;
+    const double* const acp = s.panel;
+    const double* const gsw = acp + (size_t)N * k4;
+    const int lr = l & 15, lk = l >> 4;
+    const int nbr = (N + 15) >> 4, nblk = nbr * nbr;
     constexpr int CH = 4;
-    for (int q0 = w; q0 < nblk; q0 += 4 * CH) {
-        // the chunk's blocks advance together: per 4-deep step all operands are read
-        // first, then CH independent MFMAs issue back to back
+    for (int q0 = wi; q0 < nblk; q0 += nw * CH) {
         double cv[CH][4];
         dbl4 acc[CH];
-        int oa[CH], ob[CH], og[CH];
+        int oa[CH], ob[CH];
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
-            const int q = min(q0 + 4 * u, nblk - 1);   // (a block past the end repeats the last; not stored)
+            const int q = min(q0 + nw * u, nblk - 1);   // (a block past the end repeats the last; not stored)
             const int rb = q % nbr, cb = q / nbr;
             oa[u] = rb * 16 + lr;
             ob[u] = cb * 16 + lr;
-            // the product is formed transposed (W^T ACP^T), so lane l holds the G elements
-            // (row rb*16 + l%16, column cb*16 + l/16 + 4i): 16 lanes run down one column
-            og[u] = min(rb * 16 + lr, N - 1);
+            const int og = min(rb * 16 + lr, N - 1);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) cv[u][i] = G[(size_t)min(cb * 16 + lk + 4 * i, N - 1) * N + og[u]];
+            for (int i = 0; i < 4; ++i) cv[u][i] = G[(size_t)min(cb * 16 + lk + 4 * i, N - 1) * N + og];
             acc[u] = dbl4{0.0, 0.0, 0.0, 0.0};
         }
         for (int kk = 0; kk < k4; kk += 4) {
@@ -1558,7 +1583,7 @@ __device__ __noinline__ void block_update(double* G_, int N, int b0, int k, int 
         }
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
-            const int q = q0 + 4 * u;
+            const int q = q0 + nw * u;
             const int rb = q % nbr, cb = q / nbr;
             const int row = rb * 16 + lr;
 #pragma unroll
@@ -1568,7 +1593,27 @@ __device__ __noinline__ void block_update(double* G_, int N, int b0, int k, int 
             }
         }
     }
+}
+
+// G0[S',S'] of the next block (b1, k1) of one spin, formed from the current
+// block's factors before they are applied: G[S',S'] + ACP[S',:] W[:,S']
+// (the same update block_apply makes to those elements), into GSS
+__device__ __noinline__ void gss_from_factors(const double* G_, int N, int k, int b1, int k1, int sp, Lds&) {
+    Lds& s = g_lds;
+    N = uni(N); k = uni(k); b1 = uni(b1); k1 = uni(k1); sp = uni(sp);
+    const AS_G double* G = gptr(uni(G_));
+    const int k4 = (k + 3) & ~3;
+    const double* const acp = s.panel;
+    const double* const gsw = acp + (size_t)N * k4;
+    double* gss = MP_GSS(sp);
+    for (int q = tid(); q < KB * KP; q += NT) gss[q] = 0.0;
     __syncthreads();
+    for (int q = tid(); q < k1 * k1; q += NT) {
+        const int a = q % k1, c = q / k1;
+        double v = G[(size_t)(b1 + c) * N + b1 + a];
+        for (int j = 0; j < k4; ++j) v += acp[j * N + b1 + a] * gsw[j * N + b1 + c];
+        gss[a * KP + c] = v;
+    }
 }
 
 // The in-block walk, run by wavefront 0 only.  Lanes 0..31 carry spin up and
@@ -1669,21 +1714,13 @@ __device__ __noinline__ void wg_metropolis(const ChainDesc& d, int e, uint64_t& 
     uint64_t rng = rng_io;
     double sign = sign_io;
     const int kb = __builtin_amdgcn_readfirstlane(block_depth(d));
+    auto Gs = [&](int sp) { return d.G + (size_t)sp * N * N; };
+    // G0[S,S] of the first block, both halves of the walk (also in the one-spin model)
+    for (int sp = 0; sp < 2; ++sp) stage_gss(Gs(sp < ns ? sp : 0), N, 0, min(kb, N), sp, s);
+    bool pending = false;   // the last spin's update of the previous block is still to be applied
+    int pk = 0;
     for (int b0 = 0; b0 < N; b0 += kb) {
         const int k = min(kb, N - b0);
-        // G0[S,S] of both spins, zero outside the k x k block (both halves of the walk
-        // read their spin's arrays, also in the one-spin model)
-        for (int sp = 0; sp < 2; ++sp) {
-            double* gss = MP_GSS(sp);
-            for (int q = tid(); q < KB * KP; q += NT) gss[q] = 0.0;
-        }
-        __syncthreads();
-        for (int sp = 0; sp < ns; ++sp) {
-            const AS_G double* Gg = gptr(d.G + (size_t)sp * N * N);
-            double* gss = MP_GSS(sp);
-            wg_stream(k * k, [&](int q) { const int a = q % k, c = q / k; return Gg[(size_t)(b0 + c) * N + b0 + a]; },
-                      [&](int q, double v) { const int a = q % k, c = q / k; gss[a * KP + c] = v; });
-        }
         if (tid() == 0) s.ivar[2] = 0;
         __syncthreads();
         SubClock walk_clk;
@@ -1693,13 +1730,35 @@ __device__ __noinline__ void wg_metropolis(const ChainDesc& d, int e, uint64_t& 
             rng = o.rng;
             sign = o.sign;
             if (lane() == 0) s.ivar[2] = o.accepted;
+        } else if (pending) {
+            block_apply(Gs(ns - 1), N, pk, 1, s);
         }
         __syncthreads();
         walk_clk.stop(0);
+        pending = false;
         const int nblk = s.ivar[2];
         naccept += nblk;
-        if (nblk == 0) continue;
-        for (int sp = 0; sp < ns; ++sp) block_update(d.G + (size_t)sp * N * N, N, b0, k, sp, s);
+        const int b1 = b0 + kb, k1 = min(kb, N - b1);   // the next block (k1 <= 0: none)
+        if (nblk > 0) {
+            for (int sp = 0; sp < ns; ++sp) {
+                block_stage_w(Gs(sp), N, b0, k, sp, s);
+                if (sp == ns - 1 && k1 > 0) {
+                    // the next walk starts from the factors; the update itself runs beside it
+                    gss_from_factors(Gs(sp), N, k, b1, k1, sp, s);
+                    pending = true;
+                    pk = k;
+                } else {
+                    block_apply(Gs(sp), N, k, 0, s);
+                    __syncthreads();
+                }
+            }
+        }
+        if (k1 > 0) {
+            for (int sp = 0; sp < 2; ++sp) {
+                if (pending && sp == ns - 1) continue;
+                stage_gss(Gs(sp < ns ? sp : 0), N, b1, k1, sp, s);
+            }
+        }
     }
     rng_io = rng;
     sign_io = sign;
