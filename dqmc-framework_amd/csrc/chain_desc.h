@@ -5,9 +5,9 @@
 #include <stdint.h>
 
 #define DQMC_NT 256          // threads per workgroup (4 wavefronts of 64)
-#define DQMC_NMAX 512        // largest lattice (sites) a chain may have
+#define DQMC_NMAX 448        // largest lattice (sites) a chain may have (side <= 21)
 #define DQMC_KMAX 64         // largest delayed-update depth
-#define DQMC_PANEL_DOUBLES 12288   // LDS panel of a workgroup (96 KB)
+#define DQMC_PANEL_DOUBLES 11776   // LDS panel of a workgroup (92 KB)
 
 enum {
     DQMC_FLAG_MEASURE = 1,   // accumulate equal-time observables after each sweep

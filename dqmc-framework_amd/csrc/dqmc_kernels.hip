@@ -32,6 +32,7 @@ namespace dqmc {
 // LDS layout of one workgroup
 // ---------------------------------------------------------------------------
 #define NB 16                    // panel width of the blocked LU and QR
+#define GLD 80                   // row stride of the GEMM slabs in LDS (64 + 16)
 #define PANEL_DOUBLES DQMC_PANEL_DOUBLES   // an LU/QR panel (up to NMAX x NB) plus its NB x NB blocks
 // panel-region layout of the LU / QR: P (NMAX x NB), two NB x NB blocks, then the
 // reduction scratch of the register-resident panel factorizations
@@ -46,8 +47,8 @@ namespace dqmc {
 static_assert(PANEL_RED + RED_END <= DQMC_PANEL_DOUBLES, "panel scratch exceeds the LDS panel");
 
 struct Lds {
-    double As[16][64];         // GEMM A tile (k-major)
-    double Bs[16][64];         // GEMM B tile (k-major)
+    double As[16][GLD];        // GEMM A slab (k-major, 16 deep; rows padded so that the four
+    double Bs[16][GLD];        // lane groups of an MFMA operand read fall on disjoint LDS banks)
     double du[NMAX], dd[NMAX]; // diagonals of the current G (delayed updates, measurements)
     double scp[NMAX], scm[NMAX];   // exp(+alpha s_i), exp(-alpha s_i) of the current field row
     double v0[NMAX], v1[NMAX], v2[NMAX], v3[NMAX];  // stabilization / QR scratch vectors
@@ -307,7 +308,7 @@ __device__ __noinline__ void wg_gemm(int M, int Nc, int K, const double* A, int 
 // result register i of lane l holds D[l/16 + 4i][l%16] (FP64 16x16x4 layout).
 // With lowp the tile is rounded to FP32 and multiplied by
 // v_mfma_f32_16x16x4f32 with FP32 accumulation (result D[4(l/16)+i][l%16]).
-// Pipelining: the 8 loads of the next 16-deep k-slab (or of the next tile's
+// Pipelining: the loads of the next 16-deep k-slab (or of the next tile's
 // first slab) are issued before the matrix-core work on the current slab, so
 // their latency hides behind it.  aL / bL say that A / B live in LDS; otherwise
 // they are read with global (not flat) loads, which keeps the LDS wait counter
@@ -322,6 +323,8 @@ typedef float flt4 __attribute__((ext_vector_type(4)));
 template <bool lds> struct MemSpace { typedef const AS_G double* cptr; };
 template <> struct MemSpace<true> { typedef const AS_L double* cptr; };
 
+#define KS 16            // slab depth of the GEMM
+#define SQ (KS * 64 / NT)  // elements of A (and of B) per thread and slab
 // Register-staged slab pipeline: while a step multiplies its 16-deep slab out
 // of LDS, the loads of the next step are in flight.  The (tile, slab) cursor is
 // kept in scalar registers and advanced by additions; each thread's element
@@ -348,35 +351,37 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int Nc, int K, const double* A_
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
     if (M <= 0 || Nc <= 0 || K <= 0) { __syncthreads(); return; }
-    float* Af = reinterpret_cast<float*>(&s.As[0][0]);  // [16][64] floats in lowp mode
+    float* Af = reinterpret_cast<float*>(&s.As[0][0]);  // [16][2 GLD] floats in lowp mode
     float* Bf = reinterpret_cast<float*>(&s.Bs[0][0]);
     const int t = tid();
-    // this thread's 4 elements of a slab: A (row ma, depth ka), B (depth kb, column nb);
-    // offsets relative to the slab origin and the LDS slots
-    int ma[4], ka[4], kb[4], nb[4], oa[4], ob[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int e = t + NT * q;
-        if (!tA) { ma[q] = e & 63; ka[q] = e >> 6; } else { ka[q] = e & 15; ma[q] = e >> 4; }
-        if (!tB) { kb[q] = e & 15; nb[q] = e >> 4; } else { nb[q] = e & 63; kb[q] = e >> 6; }
-        oa[q] = tA ? ma[q] * lda + ka[q] : ka[q] * lda + ma[q];
-        ob[q] = tB ? kb[q] * ldb + nb[q] : nb[q] * ldb + kb[q];
-    }
+    // this thread's SQ elements of a slab, q = 0..SQ-1, are affine in q:
+    //   A element q at (row ma0 + q*dma, depth ka0 + q*dka), offset oa0 + q*doa from the slab origin,
+    //   B element q at (depth kb0 + q*dkb, column nb0 + q*dnb), offset ob0 + q*dob
+    constexpr int TPR = NT / KS;   // threads per 64-wide row of a transposed slab load
+    const int ma0 = tA ? t / KS : t & 63, ka0 = tA ? t % KS : t >> 6;
+    const int dma = tA ? NT / KS : 0, dka = tA ? 0 : NT / 64;
+    const int kb0 = tB ? t >> 6 : t % KS, nb0 = tB ? t & 63 : t / KS;
+    const int dkb = tB ? NT / 64 : 0, dnb = tB ? 0 : NT / KS;
+    const int oa0 = tA ? ma0 * lda + ka0 : ka0 * lda + ma0, doa = tA ? dma * lda : dka * lda;
+    const int ob0 = tB ? kb0 * ldb + nb0 : nb0 * ldb + kb0, dob = tB ? dkb * ldb : dnb * ldb;
+    (void)TPR;
     // slab origin (r0, c0, k0) -> offsets of its first element in A and B
     auto a_base = [&](int r0, int k0) { return tA ? r0 * lda + k0 : k0 * lda + r0; };
     auto b_base = [&](int c0, int k0) { return tB ? k0 * ldb + c0 : c0 * ldb + k0; };
-    double va[4], vb[4];
+    double va[SQ], vb[SQ];
     auto load = [&](int r0, int c0, int k0) {
         const int ba = a_base(r0, k0), bb = b_base(c0, k0);
-        if (r0 + 64 <= M && c0 + 64 <= Nc && k0 + 16 <= K) {
+        if (r0 + 64 <= M && c0 + 64 <= Nc && k0 + KS <= K) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) { va[q] = A[ba + oa[q]]; vb[q] = B[bb + ob[q]]; }
+            for (int q = 0; q < SQ; ++q) { va[q] = A[ba + oa0 + q * doa]; vb[q] = B[bb + ob0 + q * dob]; }
         } else {
             const int mr = M - r0, nr = Nc - c0, kr = K - k0;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                va[q] = A[(ma[q] < mr && ka[q] < kr) ? ba + oa[q] : 0];
-                vb[q] = B[(nb[q] < nr && kb[q] < kr) ? bb + ob[q] : 0];
+            for (int q = 0; q < SQ; ++q) {
+                const bool okA = ma0 + q * dma < mr && ka0 + q * dka < kr;
+                const bool okB = nb0 + q * dnb < nr && kb0 + q * dkb < kr;
+                va[q] = A[okA ? ba + oa0 + q * doa : 0];
+                vb[q] = B[okB ? bb + ob0 + q * dob : 0];
             }
         }
     };
@@ -384,7 +389,7 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int Nc, int K, const double* A_
     int r0 = 0, c0 = 0, k0 = 0;
     int lr = 0, lc = 0, lk = 0;
     auto advance = [&](int& r, int& c, int& k) {
-        k += 16;
+        k += KS;
         if (k >= K) { k = 0; r += 64; if (r >= M) { r = 0; c += 64; } }
     };
     load(lr, lc, lk);
@@ -402,15 +407,16 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int Nc, int K, const double* A_
                 }
         }
         // slab to LDS (out-of-range elements zeroed here, after their loads arrived)
-        const bool full = r0 + 64 <= M && c0 + 64 <= Nc && k0 + 16 <= K;
+        const bool full = r0 + 64 <= M && c0 + 64 <= Nc && k0 + KS <= K;
         const int mr = M - r0, nr = Nc - c0, kr = K - k0;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            double v = (full || (ma[q] < mr && ka[q] < kr)) ? va[q] : 0.0;
-            if (has_ms) v *= ms[min(k0 + ka[q], K - 1)];
-            const double x = (full || (nb[q] < nr && kb[q] < kr)) ? vb[q] : 0.0;
-            if (lowp) { Af[ka[q] * 64 + ma[q]] = (float)v; Bf[kb[q] * 64 + nb[q]] = (float)x; }
-            else { s.As[ka[q]][ma[q]] = v; s.Bs[kb[q]][nb[q]] = x; }
+        for (int q = 0; q < SQ; ++q) {
+            const int m = ma0 + q * dma, k = ka0 + q * dka, kbq = kb0 + q * dkb, n = nb0 + q * dnb;
+            double v = (full || (m < mr && k < kr)) ? va[q] : 0.0;
+            if (has_ms) v *= ms[min(k0 + k, K - 1)];
+            const double x = (full || (n < nr && kbq < kr)) ? vb[q] : 0.0;
+            if (lowp) { Af[k * 2 * GLD + m] = (float)v; Bf[kbq * 2 * GLD + n] = (float)x; }
+            else { s.As[k][m] = v; s.Bs[kbq][n] = x; }
         }
         __syncthreads();
         if (lc < Nc) {   // the next step's slab is in flight during the products
@@ -420,14 +426,16 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int Nc, int K, const double* A_
         const int rows = M - r0 - wr, cols = Nc - c0 - wc;   // extent of this wave's quarter
         const bool b00 = rows > 0 && cols > 0, b01 = rows > 0 && cols > 16;
         const bool b10 = rows > 16 && cols > 0, b11 = rows > 16 && cols > 16;
+        const int ksteps = min(KS, kr + 3) >> 2;   // 4-deep steps holding data (the rest of the slab is zero)
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk) {
+        for (int kk = 0; kk < KS / 4; ++kk) {
+            if (kk >= ksteps) break;
             const int k = kk * 4 + (l >> 4);
             // the tile is formed transposed (B^T A^T): lane l then holds C rows l%16 of its
             // block, so the C reads and writes of the epilogue run down columns
             if (lowp) {
-                const float a0 = Af[k * 64 + wr + (l & 15)], a1 = Af[k * 64 + wr + 16 + (l & 15)];
-                const float bb0 = Bf[k * 64 + wc + (l & 15)], bb1 = Bf[k * 64 + wc + 16 + (l & 15)];
+                const float a0 = Af[k * 2 * GLD + wr + (l & 15)], a1 = Af[k * 2 * GLD + wr + 16 + (l & 15)];
+                const float bb0 = Bf[k * 2 * GLD + wc + (l & 15)], bb1 = Bf[k * 2 * GLD + wc + 16 + (l & 15)];
                 if (b00) accf[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(bb0, a0, accf[0][0], 0, 0, 0);
                 if (b01) accf[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(bb1, a0, accf[0][1], 0, 0, 0);
                 if (b10) accf[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(bb0, a1, accf[1][0], 0, 0, 0);
@@ -442,7 +450,7 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int Nc, int K, const double* A_
             }
         }
         __syncthreads();
-        if (k0 + 16 >= K) {
+        if (k0 + KS >= K) {
             // tile done: C (when beta != 0) and the scalings, all loaded before any store
             const int cbase = (c0 + wc) * ldc + r0 + wr;
             // element e of acc[rb][cb]: row rb*16 + l%16, column cb*16 + (lowp ? 4(l/16) + e : l/16 + 4e)

@@ -108,7 +108,7 @@ Layout chain_layout(int N, int L, int n_stack, int nspin, int kdelay, bool dense
 
 int validate_params(const dqmc_chain_params& p, std::string& why) {
     const int N = p.side * p.side;
-    if (p.side < 2 || N > DQMC_NMAX) { why = "side must satisfy 2 <= side and side*side <= 512"; return DQMC_ERR_ARG; }
+    if (p.side < 2 || N > DQMC_NMAX) { why = "side must satisfy 2 <= side and side*side <= 448"; return DQMC_ERR_ARG; }
     if (p.time_size < 1) { why = "time_size must be >= 1"; return DQMC_ERR_ARG; }
     if (p.stabilization_pace < 1) { why = "stabilization_pace must be >= 1"; return DQMC_ERR_ARG; }
     if (p.model != DQMC_MODEL_REPULSIVE && p.model != DQMC_MODEL_ATTRACTIVE) { why = "unknown model"; return DQMC_ERR_ARG; }

@@ -83,7 +83,7 @@ typedef struct dqmc_batch* dqmc_batch_t;
  * MonteCarlo.beta/time_size/stabilization_pace, Lattice.momentum. */
 typedef struct {
     int model;               /* DQMC_MODEL_* ; attractive takes |U| (src/model/attractive_hubbard.cpp:37) */
-    int side;                /* square lattice side length, N = side*side, 2 <= side, N <= 512 */
+    int side;                /* square lattice side length, N = side*side, 2 <= side, N <= 448 */
     int time_size;           /* L (dtau = beta / L, src/dqmc_walker.cpp:30) */
     int stabilization_pace;  /* p */
     int checkerboard;        /* 0 dense exp(-dtau K), 1 checkerboard break-up (even side only) */
@@ -222,8 +222,9 @@ int dqmc_server_destroy(dqmc_server_t s);
  *   C = diag(rs) op(A) diag(ms) op(B) diag(cs) + beta C,  column-major,
  *   op(A) M x K, op(B) K x Nc; rs/ms/cs may be NULL.
  * variant 0: vector FP64, 1: MFMA FP64, 2: MFMA FP32 (inputs rounded to FP32),
- * 3: MFMA FP64 with A staged in LDS (M*K <= 12288), 4: A and B staged in LDS
- * (M*K + K*Nc <= 12288).  With ms, K <= 512 for variants 1-4. */
+ * 3: MFMA FP64 with A staged in LDS (M*K <= 11776), 4: A and B staged in LDS
+ * (M*K + K*Nc <= 11776).  The scalings rs, ms, cs are staged in LDS too, so
+ * M, K, Nc <= 448 where they are given (variants 1-4). */
 int dqmc_k_gemm(dqmc_ctx_t ctx, int variant, int M, int Nc, int K, int transA, int transB, const double* A,
                 const double* B, double* C, double beta, const double* rs, const double* ms, const double* cs);
 
