@@ -325,10 +325,17 @@ template <> struct MemSpace<true> { typedef const AS_L double* cptr; };
 
 #define KS 16            // slab depth of the GEMM
 #define SQ (KS * 64 / NT)  // elements of A (and of B) per thread and slab
-// Register-staged slab pipeline: while a step multiplies its 16-deep slab out
-// of LDS, the loads of the next step are in flight.  The (tile, slab) cursor is
-// kept in scalar registers and advanced by additions; each thread's element
-// offsets within a slab are computed once; interior slabs skip the bounds tests.
+template <int V> struct IntC { static constexpr int value = V; };
+// Two-ahead slab pipeline.  A step first reads every operand of its 16-deep
+// slab from LDS into registers and releases the slab at a barrier; its MFMAs
+// are then issued in one basic block together with the LDS store of the next
+// slab (whose global loads were issued one step earlier) and the global loads
+// of the slab after that, so the matrix cores run while the slab traffic is
+// issued.  Out-of-range elements are read from offset 0 and zeroed by a bit
+// mask at the store, without branches.  The (tile, slab) cursors are kept in
+// scalar registers and advanced by additions; each thread's element offsets
+// within a slab are computed once.  C is prefetched before the last step of a
+// tile when beta != 0.
 template <bool lowp, bool aL, bool bL>
 __device__ __noinline__ void wg_gemm_mfma(int M, int Nc, int K, const double* A_, int lda, bool tA, const double* B_,
                                           int ldb, bool tB, double* C, int ldc, const double* rs, const double* ms_,
@@ -343,7 +350,7 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int Nc, int K, const double* A_
     const BPtr B = (BPtr)B_;
     const bool has_ms = ms_ != nullptr, has_rs = rs != nullptr, has_cs = cs != nullptr;
     // the scalings live in LDS (null tests above on the generic pointers); C is in global memory
-    const AS_L double* ms = (const AS_L double*)ms_;
+    const AS_L double* ms = has_ms ? (const AS_L double*)ms_ : (const AS_L double*)&s.As[0][0];
     const AS_L double* rsl = (const AS_L double*)rs;
     const AS_L double* csl = (const AS_L double*)cs;
     AS_G double* const Cg = (AS_G double*)C;
@@ -357,35 +364,54 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int Nc, int K, const double* A_
     // this thread's SQ elements of a slab, q = 0..SQ-1, are affine in q:
     //   A element q at (row ma0 + q*dma, depth ka0 + q*dka), offset oa0 + q*doa from the slab origin,
     //   B element q at (depth kb0 + q*dkb, column nb0 + q*dnb), offset ob0 + q*dob
-    constexpr int TPR = NT / KS;   // threads per 64-wide row of a transposed slab load
     const int ma0 = tA ? t / KS : t & 63, ka0 = tA ? t % KS : t >> 6;
     const int dma = tA ? NT / KS : 0, dka = tA ? 0 : NT / 64;
     const int kb0 = tB ? t >> 6 : t % KS, nb0 = tB ? t & 63 : t / KS;
     const int dkb = tB ? NT / 64 : 0, dnb = tB ? 0 : NT / KS;
     const int oa0 = tA ? ma0 * lda + ka0 : ka0 * lda + ma0, doa = tA ? dma * lda : dka * lda;
     const int ob0 = tB ? kb0 * ldb + nb0 : nb0 * ldb + kb0, dob = tB ? dkb * ldb : dnb * ldb;
-    (void)TPR;
-    // slab origin (r0, c0, k0) -> offsets of its first element in A and B
-    auto a_base = [&](int r0, int k0) { return tA ? r0 * lda + k0 : k0 * lda + r0; };
-    auto b_base = [&](int c0, int k0) { return tB ? k0 * ldb + c0 : c0 * ldb + k0; };
-    double va[SQ], vb[SQ];
+    // slab origin (r0, c0, k0) -> offsets of its first element in A and B (strides
+    // chosen once, so that the per-step offsets need no select on the transposes)
+    const int sar = tA ? lda : 1, sak = tA ? 1 : lda, sbc = tB ? 1 : ldb, sbk = tB ? ldb : 1;
+    auto a_base = [&](int r0, int k0) { return r0 * sar + k0 * sak; };
+    auto b_base = [&](int c0, int k0) { return c0 * sbc + k0 * sbk; };
+    double va[SQ], vb[SQ], msv[SQ];
+    unsigned okm = 0;   // bit q: A element q of the loaded slab in range; bit SQ + q: B element q
     auto load = [&](int r0, int c0, int k0) {
         const int ba = a_base(r0, k0), bb = b_base(c0, k0);
-        if (r0 + 64 <= M && c0 + 64 <= Nc && k0 + KS <= K) {
+        const int mr = M - r0, nr = Nc - c0, kr = K - k0;
+        unsigned ok = 0;
 #pragma unroll
-            for (int q = 0; q < SQ; ++q) { va[q] = A[ba + oa0 + q * doa]; vb[q] = B[bb + ob0 + q * dob]; }
-        } else {
-            const int mr = M - r0, nr = Nc - c0, kr = K - k0;
+        for (int q = 0; q < SQ; ++q) {
+            const bool okA = (ma0 + q * dma < mr) & (ka0 + q * dka < kr);
+            const bool okB = (nb0 + q * dnb < nr) & (kb0 + q * dkb < kr);
+            va[q] = A[okA ? ba + oa0 + q * doa : 0];
+            vb[q] = B[okB ? bb + ob0 + q * dob : 0];
+            ok |= (unsigned)okA << q;
+            ok |= (unsigned)okB << (SQ + q);
+        }
+        okm = ok;
+    };
+    // the K scalings of the loaded slab (depth origin k0s), read ahead of its store
+    auto load_ms = [&](int k0s) {
 #pragma unroll
-            for (int q = 0; q < SQ; ++q) {
-                const bool okA = ma0 + q * dma < mr && ka0 + q * dka < kr;
-                const bool okB = nb0 + q * dnb < nr && kb0 + q * dkb < kr;
-                va[q] = A[okA ? ba + oa0 + q * doa : 0];
-                vb[q] = B[okB ? bb + ob0 + q * dob : 0];
-            }
+        for (int q = 0; q < SQ; ++q) msv[q] = ms[has_ms ? min(k0s + ka0 + q * dka, K - 1) : 0];
+    };
+    // loaded slab -> LDS
+    auto store = [&]() {
+#pragma unroll
+        for (int q = 0; q < SQ; ++q) {
+            const int m = ma0 + q * dma, k = ka0 + q * dka, kbq = kb0 + q * dkb, n = nb0 + q * dnb;
+            double v = ((okm >> q) & 1u) ? va[q] : 0.0;
+            const double x = ((okm >> (SQ + q)) & 1u) ? vb[q] : 0.0;
+            v = has_ms ? v * msv[q] : v;
+            if (lowp) { Af[k * 2 * GLD + m] = (float)v; Bf[kbq * 2 * GLD + n] = (float)x; }
+            else { s.As[k][m] = v; s.Bs[kbq][n] = x; }
         }
     };
-    // cursors: (r0, c0, k0) of the step being multiplied and of the next load
+    // cursors: (r0, c0, k0) the slab multiplied in this step, sk the depth origin of
+    // the slab held in registers, (lr, lc, lk) the next slab to load; loads past the
+    // last slab read in-range elements of A and are never used
     int r0 = 0, c0 = 0, k0 = 0;
     int lr = 0, lc = 0, lk = 0;
     auto advance = [&](int& r, int& c, int& k) {
@@ -394,8 +420,16 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int Nc, int K, const double* A_
     };
     load(lr, lc, lk);
     advance(lr, lc, lk);
+    load_ms(0);
+    store();
+    int sk = lk;
+    load(lr, lc, lk);
+    advance(lr, lc, lk);
     dbl4 acc[2][2];
     flt4 accf[2][2];
+    double cv[2][2][4];
+    // element e of acc[rb][cb]: row rb*16 + l%16, column cb*16 + (lowp ? 4(l/16) + e : l/16 + 4e)
+    const int ocl = (l & 15) + (lowp ? 4 * (l >> 4) : (l >> 4)) * ldc;
     while (c0 < Nc) {
         if (k0 == 0) {
 #pragma unroll
@@ -406,57 +440,10 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int Nc, int K, const double* A_
                     accf[a][b] = flt4{0.f, 0.f, 0.f, 0.f};
                 }
         }
-        // slab to LDS (out-of-range elements zeroed here, after their loads arrived)
-        const bool full = r0 + 64 <= M && c0 + 64 <= Nc && k0 + KS <= K;
-        const int mr = M - r0, nr = Nc - c0, kr = K - k0;
-#pragma unroll
-        for (int q = 0; q < SQ; ++q) {
-            const int m = ma0 + q * dma, k = ka0 + q * dka, kbq = kb0 + q * dkb, n = nb0 + q * dnb;
-            double v = (full || (m < mr && k < kr)) ? va[q] : 0.0;
-            if (has_ms) v *= ms[min(k0 + k, K - 1)];
-            const double x = (full || (n < nr && kbq < kr)) ? vb[q] : 0.0;
-            if (lowp) { Af[k * 2 * GLD + m] = (float)v; Bf[kbq * 2 * GLD + n] = (float)x; }
-            else { s.As[k][m] = v; s.Bs[kbq][n] = x; }
-        }
-        __syncthreads();
-        if (lc < Nc) {   // the next step's slab is in flight during the products
-            load(lr, lc, lk);
-            advance(lr, lc, lk);
-        }
         const int rows = M - r0 - wr, cols = Nc - c0 - wc;   // extent of this wave's quarter
-        const bool b00 = rows > 0 && cols > 0, b01 = rows > 0 && cols > 16;
-        const bool b10 = rows > 16 && cols > 0, b11 = rows > 16 && cols > 16;
-        const int ksteps = min(KS, kr + 3) >> 2;   // 4-deep steps holding data (the rest of the slab is zero)
-#pragma unroll
-        for (int kk = 0; kk < KS / 4; ++kk) {
-            if (kk >= ksteps) break;
-            const int k = kk * 4 + (l >> 4);
-            // the tile is formed transposed (B^T A^T): lane l then holds C rows l%16 of its
-            // block, so the C reads and writes of the epilogue run down columns
-            if (lowp) {
-                const float a0 = Af[k * 2 * GLD + wr + (l & 15)], a1 = Af[k * 2 * GLD + wr + 16 + (l & 15)];
-                const float bb0 = Bf[k * 2 * GLD + wc + (l & 15)], bb1 = Bf[k * 2 * GLD + wc + 16 + (l & 15)];
-                if (b00) accf[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(bb0, a0, accf[0][0], 0, 0, 0);
-                if (b01) accf[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(bb1, a0, accf[0][1], 0, 0, 0);
-                if (b10) accf[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(bb0, a1, accf[1][0], 0, 0, 0);
-                if (b11) accf[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(bb1, a1, accf[1][1], 0, 0, 0);
-            } else {
-                const double a0 = s.As[k][wr + (l & 15)], a1 = s.As[k][wr + 16 + (l & 15)];
-                const double bb0 = s.Bs[k][wc + (l & 15)], bb1 = s.Bs[k][wc + 16 + (l & 15)];
-                if (b00) acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(bb0, a0, acc[0][0], 0, 0, 0);
-                if (b01) acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(bb1, a0, acc[0][1], 0, 0, 0);
-                if (b10) acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(bb0, a1, acc[1][0], 0, 0, 0);
-                if (b11) acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(bb1, a1, acc[1][1], 0, 0, 0);
-            }
-        }
-        __syncthreads();
-        if (k0 + KS >= K) {
-            // tile done: C (when beta != 0) and the scalings, all loaded before any store
-            const int cbase = (c0 + wc) * ldc + r0 + wr;
-            // element e of acc[rb][cb]: row rb*16 + l%16, column cb*16 + (lowp ? 4(l/16) + e : l/16 + 4e)
-            const int ocl = (l & 15) + (lowp ? 4 * (l >> 4) : (l >> 4)) * ldc;
-            double cv[2][2][4];
-            double rsv[2] = {1.0, 1.0}, csv[2][4] = {{1.0, 1.0, 1.0, 1.0}, {1.0, 1.0, 1.0, 1.0}};
+        const bool last = k0 + KS >= K;
+        const int cbase = (c0 + wc) * ldc + r0 + wr;
+        if (last && beta != 0.0) {   // C of the tile, in flight during the last step
 #pragma unroll
             for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
@@ -465,11 +452,65 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int Nc, int K, const double* A_
                     for (int e = 0; e < 4; ++e) {
                         const int row = rb * 16 + (l & 15);
                         const int col = cb * 16 + (lowp ? 4 * (l >> 4) + e : (l >> 4) + 4 * e);
-                        const bool ok = row < rows && col < cols;   // (others are never stored)
-                        cv[rb][cb][e] = beta != 0.0
-                                            ? Cg[ok ? cbase + ocl + (cb * 16 + (lowp ? e : 4 * e)) * ldc + rb * 16 : 0]
-                                            : 0.0;
+                        const bool ok = (row < rows) & (col < cols);   // (others are never stored)
+                        cv[rb][cb][e] = Cg[ok ? cbase + ocl + (cb * 16 + (lowp ? e : 4 * e)) * ldc + rb * 16 : 0];
                     }
+        }
+        __syncthreads();   // the slab of this step is in LDS
+        double oa[4][2], ob[4][2];
+        float fa[4][2], fb[4][2];
+#pragma unroll
+        for (int kk = 0; kk < KS / 4; ++kk) {
+            const int k = kk * 4 + (l >> 4);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                if (lowp) {
+                    fa[kk][h] = Af[k * 2 * GLD + wr + 16 * h + (l & 15)];
+                    fb[kk][h] = Bf[k * 2 * GLD + wc + 16 * h + (l & 15)];
+                } else {
+                    oa[kk][h] = s.As[k][wr + 16 * h + (l & 15)];
+                    ob[kk][h] = s.Bs[k][wc + 16 * h + (l & 15)];
+                }
+            }
+        }
+        load_ms(sk);
+        __syncthreads();   // operands in registers: the slab buffer is free
+        // the tile is formed transposed (B^T A^T): lane l then holds C rows l%16 of its
+        // block, so the C reads and writes of the epilogue run down columns
+        auto step = [&](auto RBc, auto CBc) {
+            constexpr int RB = decltype(RBc)::value, CB = decltype(CBc)::value;
+#pragma unroll
+            for (int kk = 0; kk < KS / 4; ++kk)
+#pragma unroll
+                for (int a = 0; a < RB; ++a)
+#pragma unroll
+                    for (int b = 0; b < CB; ++b) {
+                        if (lowp) accf[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[kk][b], fa[kk][a], accf[a][b], 0, 0, 0);
+                        else acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ob[kk][b], oa[kk][a], acc[a][b], 0, 0, 0);
+                    }
+            store();
+            sk = lk;
+            load(lr, lc, lk);
+            // issue order: each MFMA followed by a share of the slab traffic, so that
+            // the stores, loads and their address arithmetic run under the matrix cores
+            constexpr int NM = RB * CB * (KS / 4);
+#pragma unroll
+            for (int i = 0; i < NM; ++i) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
+                __builtin_amdgcn_sched_group_barrier(0x002, 6, 0);   // VALU
+                __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);   // LDS write
+                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);   // global load
+            }
+        };
+        if (rows > 16 && cols > 16) step(IntC<2>(), IntC<2>());
+        else if (rows > 16 && cols > 0) step(IntC<2>(), IntC<1>());
+        else if (rows > 0 && cols > 16) step(IntC<1>(), IntC<2>());
+        else if (rows > 0 && cols > 0) step(IntC<1>(), IntC<1>());
+        else step(IntC<0>(), IntC<0>());
+        advance(lr, lc, lk);
+        if (last) {
+            // tile done: the scalings, then the stores
+            double rsv[2] = {1.0, 1.0}, csv[2][4] = {{1.0, 1.0, 1.0, 1.0}, {1.0, 1.0, 1.0, 1.0}};
             if (has_cs) {
 #pragma unroll
                 for (int cb = 0; cb < 2; ++cb)
