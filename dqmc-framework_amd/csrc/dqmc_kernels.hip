@@ -1491,6 +1491,10 @@ __device__ __noinline__ void wg_fresh_green(const ChainDesc& d, int sp, int lslo
 // recomputed after it):
 //   CR(sp)  = row a of the block's C at a*KP (written at the end of the walk)
 //   GSS(sp) = G0[S,S] (row a at a*KP), zero outside the k x k block
+//   GST(sp) = G0[S,S]^T (column a of G0[S,S] at a*KP) in the space of CR(sp):
+//             it is staged after block_stage_w has read CR and read by the walk
+//             before the walk writes CR, so that every read of the walk runs
+//             along a row
 //   UV      = the walk's u, v broadcast vectors
 // The panel holds, for one spin at a time, ACP = G0[:,S] (N x k4, ld N) and
 // GSW = G0[S,:] (k4 x N, ld N), which becomes W in place.  Because the walk
@@ -1500,6 +1504,7 @@ constexpr int KP = KB + 1;
 #define MP_BASE (reinterpret_cast<double*>(&g_lds))
 #define MP_CR(sp) (MP_BASE + (sp) * KB * KP)
 #define MP_GSS(sp) (MP_BASE + 2 * KB * KP + (sp) * KB * KP)
+#define MP_GST(sp) MP_CR(sp)
 #define MP_UV (MP_BASE + 4 * KB * KP)
 #define MP_SLACK 16   // reads of partial 16x16 blocks run up to 16 doubles past GSW
 static_assert(offsetof(Lds, As) == 0 && offsetof(Lds, perm) >= (4 * KB * (KB + 1) + 128) * sizeof(double),
@@ -1517,10 +1522,11 @@ __device__ __forceinline__ int block_depth(const ChainDesc& d) {
 __device__ void stage_gss(const double* G_, int N, int b0, int k, int sp, Lds&) {
     const AS_G double* Gg = gptr(G_);
     double* gss = MP_GSS(sp);
-    for (int q = tid(); q < KB * KP; q += NT) gss[q] = 0.0;
+    double* gst = MP_GST(sp);
+    for (int q = tid(); q < KB * KP; q += NT) { gss[q] = 0.0; gst[q] = 0.0; }
     __syncthreads();
     wg_stream(k * k, [&](int q) { const int a = q % k, c = q / k; return Gg[(size_t)(b0 + c) * N + b0 + a]; },
-              [&](int q, double v) { const int a = q % k, c = q / k; gss[a * KP + c] = v; });
+              [&](int q, double v) { const int a = q % k, c = q / k; gss[a * KP + c] = v; gst[c * KP + a] = v; });
 }
 
 // The block's update of G for one spin, on the matrix cores with operands in LDS.
@@ -1655,13 +1661,15 @@ __device__ __noinline__ void gss_from_factors(const double* G_, int N, int k, in
     const double* const acp = s.panel;
     const double* const gsw = acp + (size_t)N * k4;
     double* gss = MP_GSS(sp);
-    for (int q = tid(); q < KB * KP; q += NT) gss[q] = 0.0;
+    double* gst = MP_GST(sp);
+    for (int q = tid(); q < KB * KP; q += NT) { gss[q] = 0.0; gst[q] = 0.0; }
     __syncthreads();
     for (int q = tid(); q < k1 * k1; q += NT) {
         const int a = q % k1, c = q / k1;
         double v = G[(size_t)(b1 + c) * N + b1 + a];
         for (int j = 0; j < k4; ++j) v += acp[j * N + b1 + a] * gsw[j * N + b1 + c];
         gss[a * KP + c] = v;
+        gst[c * KP + a] = v;
     }
 }
 
@@ -1678,6 +1686,15 @@ struct WalkOut {
     double sign;
     int accepted;
 };
+// An LDS row address as an opaque register value: the element offsets of the
+// reads along the row then fold into the LDS instructions' offset fields
+// (otherwise the compiler reassociates the row base with the array base and
+// materializes one address register per read).
+__device__ __forceinline__ const AS_L double* lds_row(const double* p) {
+    unsigned v = (unsigned)(uintptr_t)(const AS_L double*)p;
+    asm volatile("" : "+v"(v));
+    return (const AS_L double*)(uintptr_t)v;
+}
 __device__ __forceinline__ WalkOut walk_block(const ChainDesc& d, int e, int b0, int k, uint64_t rng, double sign,
                                            bool record, Lds&) {
     Lds& s = g_lds;
@@ -1690,6 +1707,7 @@ __device__ __forceinline__ WalkOut walk_block(const ChainDesc& d, int e, int b0,
     uint8_t* const rec_row = uni(d.recAcc) + (size_t)e * N;
     const int h = lane() >> 5, a = lane() & 31;
     const double* const gs = MP_GSS(h);
+    const double* const gt = MP_GST(h);
     double* const ush = MP_UV + h * 32;        // u at [lane], v at [64 + lane]
     double* const vsh = MP_UV + 64 + h * 32;
     double crr[KB], ccr[KB];
@@ -1722,11 +1740,31 @@ __device__ __forceinline__ WalkOut walk_block(const ChainDesc& d, int e, int b0,
         const double g = h == 0 ? m0 / x0 : m1 / x1;
         // u_a = d_ap + (C (Gss - I))(a,p),  v_a = d_ap + (Gss C)(p,a)
         // (four partial sums each, so the FMA dependency chains are 8 long)
+        // (column p of Gss is row p of its transpose; the reads of chunk c + 1 of
+        // four are issued before the products of chunk c)
         double t4[4] = {0.0, 0.0, 0.0, 0.0}, w4[4] = {0.0, 0.0, 0.0, 0.0};
+        {
+            const AS_L double* const colp = lds_row(gt + p * KP);
+            const AS_L double* const rowp = lds_row(gs + p * KP);
+            double xc[2][4], yr[2][4];
 #pragma unroll
-        for (int b = 0; b < KB; ++b) {
-            t4[b & 3] += crr[b] * (gs[b * KP + p] - (b == p ? 1.0 : 0.0));
-            w4[b & 3] += gs[p * KP + b] * ccr[b];
+            for (int j = 0; j < 4; ++j) { xc[0][j] = colp[j]; yr[0][j] = rowp[j]; }
+#pragma unroll
+            for (int c = 0; c < KB / 4; ++c) {
+                const int cur = c & 1, nxt = cur ^ 1;
+                if (c + 1 < KB / 4) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) { xc[nxt][j] = colp[4 * c + 4 + j]; yr[nxt][j] = rowp[4 * c + 4 + j]; }
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int b = 4 * c + j;
+                    t4[j] += crr[b] * (xc[cur][j] - (b == p ? 1.0 : 0.0));
+                    w4[j] += yr[cur][j] * ccr[b];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
         const double u = (a == p ? 1.0 : 0.0) + ((t4[0] + t4[1]) + (t4[2] + t4[3]));
         const double v = (a == p ? 1.0 : 0.0) + ((w4[0] + w4[1]) + (w4[2] + w4[3]));
@@ -1734,19 +1772,41 @@ __device__ __forceinline__ WalkOut walk_block(const ChainDesc& d, int e, int b0,
         vsh[a] = v;
         __builtin_amdgcn_wave_barrier();
         // C += g u v^T (row a and column a), and the block diagonal
-        // d_a += g (Gss u)_a (v^T (Gss - I))_a
+        // d_a += g (Gss u)_a (v^T (Gss - I))_a, the identity's share v_a taken out of the sum
         double x4[4] = {0.0, 0.0, 0.0, 0.0}, y4[4] = {0.0, 0.0, 0.0, 0.0};
         const double gu = g * u, gv = g * v;
+        {
+            const AS_L double* const rowa = lds_row(gs + a * KP);
+            const AS_L double* const cola = lds_row(gt + a * KP);
+            const AS_L double* const uo = lds_row(ush);
+            const AS_L double* const vo = lds_row(vsh);
+            double ub[2][4], vb[2][4], gr[2][4], gc[2][4];
 #pragma unroll
-        for (int b = 0; b < KB; ++b) {
-            const double ub = ush[b], vb = vsh[b];
-            crr[b] += gu * vb;
-            ccr[b] += ub * gv;
-            x4[b & 3] += gs[a * KP + b] * ub;
-            y4[b & 3] += vb * (gs[b * KP + a] - (b == a ? 1.0 : 0.0));
+            for (int j = 0; j < 4; ++j) { ub[0][j] = uo[j]; vb[0][j] = vo[j]; gr[0][j] = rowa[j]; gc[0][j] = cola[j]; }
+#pragma unroll
+            for (int c = 0; c < KB / 4; ++c) {
+                const int cur = c & 1, nxt = cur ^ 1;
+                if (c + 1 < KB / 4) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int b = 4 * c + 4 + j;
+                        ub[nxt][j] = uo[b]; vb[nxt][j] = vo[b]; gr[nxt][j] = rowa[b]; gc[nxt][j] = cola[b];
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int b = 4 * c + j;
+                    crr[b] += gu * vb[cur][j];
+                    ccr[b] += ub[cur][j] * gv;
+                    x4[j] += gr[cur][j] * ub[cur][j];
+                    y4[j] += vb[cur][j] * gc[cur][j];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
         __builtin_amdgcn_wave_barrier();
-        dq += g * ((x4[0] + x4[1]) + (x4[2] + x4[3])) * ((y4[0] + y4[1]) + (y4[2] + y4[3]));
+        dq += g * ((x4[0] + x4[1]) + (x4[2] + x4[3])) * (((y4[0] + y4[1]) + (y4[2] + y4[3])) - v);
     }
     double* const cro = MP_CR(h) + a * KP;
 #pragma unroll
