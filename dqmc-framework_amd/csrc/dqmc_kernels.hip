@@ -320,6 +320,11 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
 #define AS_G __attribute__((address_space(1)))
 #define AS_L __attribute__((address_space(3)))
+#define AS_C __attribute__((address_space(4)))
+// A chain's descriptor, read through the constant address space: its fields are
+// uniform, so they load through the scalar cache (a generic reference would make
+// every field read a flat load that also holds up the next LDS wait)
+typedef const AS_C ChainDesc& Desc;
 template <bool lds> struct MemSpace { typedef const AS_G double* cptr; };
 template <> struct MemSpace<true> { typedef const AS_L double* cptr; };
 
@@ -571,7 +576,7 @@ __device__ __forceinline__ void gemm64_ll(int M, int Nc, int K, const double* A,
     wg_gemm_mfma<false, true, true>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, nullptr, nullptr, nullptr, beta, s,
                                     alpha);
 }
-__device__ __forceinline__ void gemmB(const ChainDesc& d, int M, int Nc, int K, const double* A, int lda, bool tA,
+__device__ __forceinline__ void gemmB(Desc d, int M, int Nc, int K, const double* A, int lda, bool tA,
                                       const double* B, int ldb, bool tB, double* C, int ldc, const double* rs,
                                       const double* ms, const double* cs, Lds&) {
     Lds& s = g_lds;
@@ -606,7 +611,8 @@ __device__ __noinline__ void wg_copy(double* dst_, const double* src_, int n2) {
     wg_stream(uni(n2), [&](int e) { return src[e]; }, [&](int e, double v) { dst[e] = v; });
     __syncthreads();
 }
-__device__ void wg_identity(double* M, int N) {
+__device__ void wg_identity(double* M_, int N) {
+    AS_G double* const M = gptr(M_);
     for (int e = tid(); e < N * N; e += NT) M[e] = ((e % N) == (e / N)) ? 1.0 : 0.0;
     __syncthreads();
 }
@@ -630,14 +636,14 @@ __device__ __forceinline__ void plaq_sites(int p, int side, int start, int idx[4
     const int x1 = (x + 1) % side, y1 = (y + 1) % side;
     idx[0] = x + side * y; idx[1] = x1 + side * y; idx[2] = x + side * y1; idx[3] = x1 + side * y1;
 }
-__device__ __forceinline__ void plaq_apply(const double P[3], const double v[4], double o[4]) {
+__device__ __forceinline__ void plaq_apply(const AS_C double* P, const double v[4], double o[4]) {
     const double a = P[0], b = P[1], c = P[2];
     o[0] = a * v[0] + b * v[1] + b * v[2] + c * v[3];
     o[1] = b * v[0] + a * v[1] + c * v[2] + b * v[3];
     o[2] = b * v[0] + c * v[1] + a * v[2] + b * v[3];
     o[3] = c * v[0] + b * v[1] + b * v[2] + a * v[3];
 }
-__device__ void wg_cb_left(double* M, int N, int side, const double P[3], int start) {
+__device__ void wg_cb_left(double* M, int N, int side, const AS_C double* P, int start) {
     const int np = N >> 2;
     for (int e = tid(); e < np * N; e += NT) {
         const int p = e % np, col = e / np;
@@ -649,7 +655,7 @@ __device__ void wg_cb_left(double* M, int N, int side, const double P[3], int st
     }
     __syncthreads();
 }
-__device__ void wg_cb_right(double* M, int N, int side, const double P[3], int start) {
+__device__ void wg_cb_right(double* M, int N, int side, const AS_C double* P, int start) {
     const int np = N >> 2;
     for (int e = tid(); e < np * N; e += NT) {
         const int row = e % N, p = e / N;
@@ -669,7 +675,7 @@ __device__ void wg_cb_right(double* M, int N, int side, const double P[3], int s
 // whole columns (left) or whole rows (right) staged in LDS: one read and one
 // write of the matrix instead of one per pass.  starts[] lists the sublattice
 // order (1 = B plaquettes at odd (x,y), 0 = A at even).
-__device__ void cb_pass_lds_left(double* T, int ld, int ncols, int side, const double P[3], int start) {
+__device__ void cb_pass_lds_left(double* T, int ld, int ncols, int side, const AS_C double* P, int start) {
     const int np = (side / 2) * (side / 2);
     for (int e = tid(); e < np * ncols; e += NT) {
         const int pq = e % np, col = e / np;
@@ -682,7 +688,7 @@ __device__ void cb_pass_lds_left(double* T, int ld, int ncols, int side, const d
     __syncthreads();
 }
 // T holds nr rows x N columns, element (r, c) at c * nr + r
-__device__ void cb_pass_lds_right(double* T, int nr, int side, const double P[3], int start) {
+__device__ void cb_pass_lds_right(double* T, int nr, int side, const AS_C double* P, int start) {
     const int np = (side / 2) * (side / 2);
     for (int e = tid(); e < np * nr; e += NT) {
         const int r = e % nr, pq = e / nr;
@@ -694,7 +700,7 @@ __device__ void cb_pass_lds_right(double* T, int nr, int side, const double P[3]
     __syncthreads();
 }
 // M <- post .* E_s2 E_s1 (pre .* M)  (row scalings; pre/post may be null)
-__device__ __noinline__ void wg_cb_left_fused(double* M, int N, int side, const double P[3], int s1, int s2, const double* pre,
+__device__ __noinline__ void wg_cb_left_fused(double* M, int N, int side, const AS_C double* P, int s1, int s2, const double* pre,
                                  const double* post, Lds&) {
     Lds& s = g_lds;
     N = uni(N);
@@ -721,7 +727,7 @@ __device__ __noinline__ void wg_cb_left_fused(double* M, int N, int side, const 
     }
 }
 // M <- (M .* pre) E_s1 E_s2 .* post  (column scalings)
-__device__ __noinline__ void wg_cb_right_fused(double* M, int N, int side, const double P[3], int s1, int s2, const double* pre,
+__device__ __noinline__ void wg_cb_right_fused(double* M, int N, int side, const AS_C double* P, int s1, int s2, const double* pre,
                                   const double* post, Lds&) {
     Lds& s = g_lds;
     N = uni(N);
@@ -748,13 +754,15 @@ __device__ __noinline__ void wg_cb_right_fused(double* M, int N, int side, const
     }
 }
 // whole wrap in LDS when the matrix fits: up  G <- B G B^-1, down  G <- B^-1 G B
-__device__ __noinline__ void wg_cb_wrap_lds(const ChainDesc& d, bool up, double* G, const double* ep, const double* en, Lds&) {
+__device__ __noinline__ void wg_cb_wrap_lds(Desc d, bool up, double* G_, const double* ep_, const double* en_,
+                                            Lds&) {
     Lds& s = g_lds;
     const int N = uni(d.N), side = uni(d.side);
-    {
-        const AS_G double* Gg = gptr(uni(G));
-        wg_stream(N * N, [&](int e) { return Gg[e]; }, [&](int e, double v) { s.panel[e] = v; });
-    }
+    // the scalings live in LDS and G in global memory (no generic accesses)
+    const AS_L double* const ep = (const AS_L double*)uni(ep_);
+    const AS_L double* const en = (const AS_L double*)uni(en_);
+    AS_G double* const G = gptr(uni(G_));
+    wg_stream(N * N, [&](int e) { return G[e]; }, [&](int e, double v) { s.panel[e] = v; });
     __syncthreads();
     if (up) {
         // B_left: E (B then A), rows * ep;  invB_right: E^-1 (B then A) on columns, columns * en
@@ -781,10 +789,10 @@ __device__ __noinline__ void wg_cb_wrap_lds(const ChainDesc& d, bool up, double*
 }
 
 // Field-row scalings exp(+alpha s_i) / exp(-alpha s_i) into LDS.
-__device__ void load_field_row(const ChainDesc& d, int e, Lds&) {
+__device__ void load_field_row(Desc d, int e, Lds&) {
     Lds& s = g_lds;
     for (int i = tid(); i < d.N; i += NT) {
-        const int f = d.field[(size_t)e * d.N + i];
+        const int f = ((const AS_G int8_t*)d.field)[(size_t)e * d.N + i];
         s.frow[i] = f;
         s.scp[i] = f > 0 ? d.ev[0] : d.ev[1];
         s.scm[i] = f > 0 ? d.ev[1] : d.ev[0];
@@ -792,7 +800,7 @@ __device__ void load_field_row(const ChainDesc& d, int e, Lds&) {
     __syncthreads();
 }
 // after Metropolis flipped entries of the LDS field row
-__device__ void refresh_scalings(const ChainDesc& d, Lds&) {
+__device__ void refresh_scalings(Desc d, Lds&) {
     Lds& s = g_lds;
     for (int i = tid(); i < d.N; i += NT) {
         const int f = s.frow[i];
@@ -802,18 +810,18 @@ __device__ void refresh_scalings(const ChainDesc& d, Lds&) {
     __syncthreads();
 }
 // exp(+sigma_eff alpha s) for spin index sp (0 up, 1 down); attractive ignores the spin
-__device__ __forceinline__ const double* vpos(const ChainDesc& d, int sp, Lds&) {
+__device__ __forceinline__ const double* vpos(Desc d, int sp, Lds&) {
     Lds& s = g_lds;
     return (d.model == 1 || sp == 0) ? s.scp : s.scm;
 }
-__device__ __forceinline__ const double* vneg(const ChainDesc& d, int sp, Lds&) {
+__device__ __forceinline__ const double* vneg(Desc d, int sp, Lds&) {
     Lds& s = g_lds;
     return (d.model == 1 || sp == 0) ? s.scm : s.scp;
 }
 
 // B(t) products on an N x N matrix M (in place; W is N*N scratch for the dense path).
 // op: 0 B_left, 1 B_right, 2 invB_left, 3 invB_right, 4 transB_left
-__device__ void wg_bmul(const ChainDesc& d, int op, int sp, double* M, double* W, Lds&) {
+__device__ void wg_bmul(Desc d, int op, int sp, double* M, double* W, Lds&) {
     Lds& s = g_lds;
     const int N = d.N;
     const double* ep = vpos(d, sp, s);
@@ -839,7 +847,7 @@ __device__ void wg_bmul(const ChainDesc& d, int op, int sp, double* M, double* W
 }
 
 // G <- B(t) G B(t)^-1 (up) or G <- B(t)^-1 G B(t) (down) for spin index sp
-__device__ void wg_wrap(const ChainDesc& d, int sp, bool up, double* G, double* W, Lds&) {
+__device__ void wg_wrap(Desc d, int sp, bool up, double* G, double* W, Lds&) {
     Lds& s = g_lds;
     const int N = d.N;
     if (!d.cb) {
@@ -999,7 +1007,7 @@ __device__ __noinline__ double wg_lu_solve(double* A, double* B, int N, int nrhs
         pclk.stop(1);
         for (int e = tid(); e < m * jb; e += NT) {
             const int r = e % m, c = e / m;
-            A[(size_t)(k0 + c) * N + k0 + r] = P[c * m + r];
+            gptr(A)[(size_t)(k0 + c) * N + k0 + r] = P[c * m + r];
         }
         // the panel's interchanges (row k0+j <-> perm[k0+j], j = 0..jb-1 in order) composed into
         // one gather: row swp_row[i] receives the old row swp_src[i]; rows k0..k0+jb-1 come first
@@ -1061,11 +1069,11 @@ __device__ __noinline__ double wg_lu_solve(double* A, double* B, int N, int nrhs
         const int jb = min(NB, N - k0);
         for (int e = tid(); e < jb * jb; e += NT) {
             const int r = e % jb, c = e / jb;
-            T11[c * NB + r] = A[(size_t)(k0 + c) * N + k0 + r];
+            T11[c * NB + r] = Ag[(size_t)(k0 + c) * N + k0 + r];
         }
         __syncthreads();
         for (int c = tid(); c < nrhs; c += NT) {
-            double* col = B + (size_t)c * N;
+            AS_G double* col = gptr(B + (size_t)c * N);
             double x[NB];
 #pragma unroll
             for (int j = 0; j < NB; ++j) x[j] = (j < jb) ? col[k0 + j] : 0.0;
@@ -1325,7 +1333,7 @@ __device__ __noinline__ void wg_qr_udv(double* M, double* Mp, int N, double* U, 
     // D and Tl = D^-1 R P^T (D also kept in LDS for the division)
     for (int k = tid(); k < N; k += NT) {
         const double dk = fabs(Mpg[(size_t)k * N + k]);
-        D[k] = dk;
+        gptr(D)[k] = dk;
         s.v2[k] = dk;
     }
     __syncthreads();
@@ -1335,7 +1343,7 @@ __device__ __noinline__ void wg_qr_udv(double* M, double* Mp, int N, double* U, 
                   [&](int e, double v) { const int r = e % N, c = e / N; Tlg[(size_t)s.perm[c] * N + r] = v; });
     }
     // Q = H_0 H_1 ... : U <- (I - V T V^T) U, panels from the last
-    for (int e = tid(); e < N * N; e += NT) U[e] = ((e % N) == (e / N)) ? 1.0 : 0.0;
+    wg_identity(U, N);
     __syncthreads();
     for (int k0 = ((N - 1) / NB) * NB; k0 >= 0; k0 -= NB) {
         const int jb = min(NB, N - k0), m = N - k0;
@@ -1355,18 +1363,18 @@ __device__ __noinline__ void wg_qr_udv(double* M, double* Mp, int N, double* U, 
 // the other always meet at the same slot.  Each slot keeps U, D and the
 // cumulative T (= V^T) of the stack product up to that level.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double* slotU(const ChainDesc& d, int sp, int k) {
+__device__ __forceinline__ double* slotU(Desc d, int sp, int k) {
     return d.stkU + ((size_t)sp * d.n_stack + k) * d.N * d.N;
 }
-__device__ __forceinline__ double* slotT(const ChainDesc& d, int sp, int k) {
+__device__ __forceinline__ double* slotT(Desc d, int sp, int k) {
     return d.stkT + ((size_t)sp * d.n_stack + k) * d.N * d.N;
 }
-__device__ __forceinline__ double* slotD(const ChainDesc& d, int sp, int k) {
+__device__ __forceinline__ double* slotD(Desc d, int sp, int k) {
     return d.stkD + ((size_t)sp * d.n_stack + k) * d.N;
 }
 
 // push A onto the stack whose current top is slot `below` (-1 if empty); result at `slot`
-__device__ __noinline__ void wg_push(const ChainDesc& d, int sp, int slot, int below, const double* A, Lds&) {
+__device__ __noinline__ void wg_push(Desc d, int sp, int slot, int below, const double* A, Lds&) {
     Lds& s = g_lds;
     const int N = d.N;
     double* W0 = d.W;
@@ -1374,7 +1382,7 @@ __device__ __noinline__ void wg_push(const ChainDesc& d, int sp, int slot, int b
     if (below < 0) wg_copy(W0, A, N * N);
     else {
         const double* Db = slotD(d, sp, below);   // the GEMM's column scaling, staged in LDS
-        for (int i = tid(); i < N; i += NT) s.v3[i] = Db[i];
+        for (int i = tid(); i < N; i += NT) s.v3[i] = gptr(Db)[i];
         __syncthreads();
         gemm64(N, N, N, A, N, false, slotU(d, sp, below), N, false, W0, N, nullptr, nullptr, s.v3, 0.0, s);
     }
@@ -1425,7 +1433,7 @@ __device__ __noinline__ void wg_transpose_scaled(double* dst1_, double* dst2_, c
 }
 
 // G = (1 + B_left B_right)^-1 from the stack tops (lslot/rslot = -1 when that stack is empty)
-__device__ __noinline__ void wg_fresh_green(const ChainDesc& d, int sp, int lslot, int rslot, double* Gout, Lds&) {
+__device__ __noinline__ void wg_fresh_green(Desc d, int sp, int lslot, int rslot, double* Gout, Lds&) {
     Lds& s = g_lds;
     const int N = d.N;
     double* W0 = d.W;
@@ -1435,7 +1443,7 @@ __device__ __noinline__ void wg_fresh_green(const ChainDesc& d, int sp, int lslo
         const double* UR = slotU(d, sp, rslot); const double* TR = slotT(d, sp, rslot); const double* DR = slotD(d, sp, rslot);
         // v0 = 1/dlmax, v1 = dlmin, v2 = 1/drmax, v3 = drmin  (include/utils/numerical_stable.hpp:57-71)
         for (int i = tid(); i < N; i += NT) {
-            const double dl = DL[i], dr = DR[i];
+            const double dl = gptr(DL)[i], dr = gptr(DR)[i];
             s.v0[i] = dl >= 1.0 ? 1.0 / dl : 1.0; s.v1[i] = dl >= 1.0 ? 1.0 : dl;
             s.v2[i] = dr >= 1.0 ? 1.0 / dr : 1.0; s.v3[i] = dr >= 1.0 ? 1.0 : dr;
         }
@@ -1454,7 +1462,7 @@ __device__ __noinline__ void wg_fresh_green(const ChainDesc& d, int sp, int lslo
     const int k = lslot >= 0 ? lslot : rslot;
     const double* U = slotU(d, sp, k); const double* T = slotT(d, sp, k); const double* D = slotD(d, sp, k);
     for (int i = tid(); i < N; i += NT) {
-        const double x = D[i];
+        const double x = gptr(D)[i];
         s.v0[i] = x > 1.0 ? 1.0 / x : 1.0;
         s.v1[i] = x > 1.0 ? 1.0 : x;
     }
@@ -1490,11 +1498,14 @@ __device__ __noinline__ void wg_fresh_green(const ChainDesc& d, int sp, int lslo
 // scratch vectors, none of which the Metropolis phase uses; the scalings are
 // recomputed after it):
 //   CR(sp)  = row a of the block's C at a*KP (written at the end of the walk)
-//   GSS(sp) = G0[S,S] (row a at a*KP), zero outside the k x k block
-//   GST(sp) = G0[S,S]^T (column a of G0[S,S] at a*KP) in the space of CR(sp):
-//             it is staged after block_stage_w has read CR and read by the walk
-//             before the walk writes CR, so that every read of the walk runs
-//             along a row
+//   GR      = G0[S,S] of both spins interleaved by walk lane: element (a, b) of
+//             spin sp at b*64 + 32 sp + a, zero outside the k x k block, so that
+//             lane 32 sp + a reads its row a at stride 64 (one conflict-free
+//             512-byte access per element) and row p of a spin is a broadcast
+//   GT      = the same for G0[S,S]^T: element (b, a) at b*64 + 32 sp + a (the
+//             columns of G0[S,S]), in the space of CR(0) and CR(1): it is staged
+//             after block_stage_w has read CR and read by the walk before the
+//             walk writes CR
 //   UV      = the walk's u, v broadcast vectors
 // The panel holds, for one spin at a time, ACP = G0[:,S] (N x k4, ld N) and
 // GSW = G0[S,:] (k4 x N, ld N), which becomes W in place.  Because the walk
@@ -1503,15 +1514,16 @@ __device__ __noinline__ void wg_fresh_green(const ChainDesc& d, int sp, int lslo
 constexpr int KP = KB + 1;
 #define MP_BASE (reinterpret_cast<double*>(&g_lds))
 #define MP_CR(sp) (MP_BASE + (sp) * KB * KP)
-#define MP_GSS(sp) (MP_BASE + 2 * KB * KP + (sp) * KB * KP)
-#define MP_GST(sp) MP_CR(sp)
+#define MP_GR (MP_BASE + 2 * KB * KP)
+#define MP_GT MP_BASE
+static_assert(64 * KB <= 2 * KB * KP, "the lane-interleaved G0[S,S] must fit in two C blocks");
 #define MP_UV (MP_BASE + 4 * KB * KP)
 #define MP_SLACK 16   // reads of partial 16x16 blocks run up to 16 doubles past GSW
 static_assert(offsetof(Lds, As) == 0 && offsetof(Lds, perm) >= (4 * KB * (KB + 1) + 128) * sizeof(double),
               "the walk's arrays must fit in front of Lds::perm");
 
 // block size: the delay depth, at most KB, and small enough that ACP and GSW fit
-__device__ __forceinline__ int block_depth(const ChainDesc& d) {
+__device__ __forceinline__ int block_depth(Desc d) {
     const int N = d.N;
     int kb = min(d.kdelay, KB);
     const int cap = ((PANEL_DOUBLES - MP_SLACK) / (2 * N)) & ~3;
@@ -1521,12 +1533,12 @@ __device__ __forceinline__ int block_depth(const ChainDesc& d) {
 // G0[S,S] of block (b0, k) of one spin into GSS, zero outside the k x k block
 __device__ void stage_gss(const double* G_, int N, int b0, int k, int sp, Lds&) {
     const AS_G double* Gg = gptr(G_);
-    double* gss = MP_GSS(sp);
-    double* gst = MP_GST(sp);
-    for (int q = tid(); q < KB * KP; q += NT) { gss[q] = 0.0; gst[q] = 0.0; }
+    double* gr = MP_GR + 32 * sp;
+    double* gt = MP_GT + 32 * sp;
+    for (int q = tid(); q < KB * KB; q += NT) { gr[(q >> 5) * 64 + (q & 31)] = 0.0; gt[(q >> 5) * 64 + (q & 31)] = 0.0; }
     __syncthreads();
     wg_stream(k * k, [&](int q) { const int a = q % k, c = q / k; return Gg[(size_t)(b0 + c) * N + b0 + a]; },
-              [&](int q, double v) { const int a = q % k, c = q / k; gss[a * KP + c] = v; gst[c * KP + a] = v; });
+              [&](int q, double v) { const int a = q % k, c = q / k; gr[c * 64 + a] = v; gt[a * 64 + c] = v; });
 }
 
 // The block's update of G for one spin, on the matrix cores with operands in LDS.
@@ -1660,16 +1672,16 @@ __device__ __noinline__ void gss_from_factors(const double* G_, int N, int k, in
     const int k4 = (k + 3) & ~3;
     const double* const acp = s.panel;
     const double* const gsw = acp + (size_t)N * k4;
-    double* gss = MP_GSS(sp);
-    double* gst = MP_GST(sp);
-    for (int q = tid(); q < KB * KP; q += NT) { gss[q] = 0.0; gst[q] = 0.0; }
+    double* gr = MP_GR + 32 * sp;
+    double* gt = MP_GT + 32 * sp;
+    for (int q = tid(); q < KB * KB; q += NT) { gr[(q >> 5) * 64 + (q & 31)] = 0.0; gt[(q >> 5) * 64 + (q & 31)] = 0.0; }
     __syncthreads();
     for (int q = tid(); q < k1 * k1; q += NT) {
         const int a = q % k1, c = q / k1;
         double v = G[(size_t)(b1 + c) * N + b1 + a];
         for (int j = 0; j < k4; ++j) v += acp[j * N + b1 + a] * gsw[j * N + b1 + c];
-        gss[a * KP + c] = v;
-        gst[c * KP + a] = v;
+        gr[c * 64 + a] = v;
+        gt[a * 64 + c] = v;
     }
 }
 
@@ -1695,7 +1707,7 @@ __device__ __forceinline__ const AS_L double* lds_row(const double* p) {
     asm volatile("" : "+v"(v));
     return (const AS_L double*)(uintptr_t)v;
 }
-__device__ __forceinline__ WalkOut walk_block(const ChainDesc& d, int e, int b0, int k, uint64_t rng, double sign,
+__device__ __forceinline__ WalkOut walk_block(Desc d, int e, int b0, int k, uint64_t rng, double sign,
                                            bool record, Lds&) {
     Lds& s = g_lds;
     const int N = uni(d.N);
@@ -1703,17 +1715,19 @@ __device__ __forceinline__ WalkOut walk_block(const ChainDesc& d, int e, int b0,
     b0 = uni(b0); k = uni(k);
     const double pref0 = uni(d.pref[0]), pref1 = uni(d.pref[1]);
     const double dm00 = uni(d.dm1[0][0]), dm01 = uni(d.dm1[0][1]), dm10 = uni(d.dm1[1][0]), dm11 = uni(d.dm1[1][1]);
-    int8_t* const field_row = uni(d.field) + (size_t)e * N;
-    uint8_t* const rec_row = uni(d.recAcc) + (size_t)e * N;
+    // (global, not generic, stores: a generic store would also count against the
+    // LDS counter and make the walk's next LDS wait a global-memory round trip)
+    AS_G int8_t* const field_row = (AS_G int8_t*)(uni(d.field) + (size_t)e * N);
+    AS_G uint8_t* const rec_row = (AS_G uint8_t*)(uni(d.recAcc) + (size_t)e * N);
     const int h = lane() >> 5, a = lane() & 31;
-    const double* const gs = MP_GSS(h);
-    const double* const gt = MP_GST(h);
+    const double* const gr = MP_GR;
+    const double* const gt = MP_GT;
     double* const ush = MP_UV + h * 32;        // u at [lane], v at [64 + lane]
     double* const vsh = MP_UV + 64 + h * 32;
     double crr[KB], ccr[KB];
 #pragma unroll
     for (int b = 0; b < KB; ++b) { crr[b] = 0.0; ccr[b] = 0.0; }
-    double dq = gs[a * KP + a];
+    double dq = gr[a * 64 + lane()];
     int nacc = 0;
     // the block's k uniform draws, one per lane (site a), bit-identical to drawing them in order
     const double draw = block_canonical(rng, a < k ? a : 0);
@@ -1740,21 +1754,21 @@ __device__ __forceinline__ WalkOut walk_block(const ChainDesc& d, int e, int b0,
         const double g = h == 0 ? m0 / x0 : m1 / x1;
         // u_a = d_ap + (C (Gss - I))(a,p),  v_a = d_ap + (Gss C)(p,a)
         // (four partial sums each, so the FMA dependency chains are 8 long)
-        // (column p of Gss is row p of its transpose; the reads of chunk c + 1 of
-        // four are issued before the products of chunk c)
+        // (row p and column p of this spin's Gss are broadcasts at stride 64; the
+        // reads of chunk c + 1 of four are issued before the products of chunk c)
         double t4[4] = {0.0, 0.0, 0.0, 0.0}, w4[4] = {0.0, 0.0, 0.0, 0.0};
         {
-            const AS_L double* const colp = lds_row(gt + p * KP);
-            const AS_L double* const rowp = lds_row(gs + p * KP);
+            const AS_L double* const colp = lds_row(gt + h * 32 + p);
+            const AS_L double* const rowp = lds_row(gr + h * 32 + p);
             double xc[2][4], yr[2][4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) { xc[0][j] = colp[j]; yr[0][j] = rowp[j]; }
+            for (int j = 0; j < 4; ++j) { xc[0][j] = colp[64 * j]; yr[0][j] = rowp[64 * j]; }
 #pragma unroll
             for (int c = 0; c < KB / 4; ++c) {
                 const int cur = c & 1, nxt = cur ^ 1;
                 if (c + 1 < KB / 4) {
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) { xc[nxt][j] = colp[4 * c + 4 + j]; yr[nxt][j] = rowp[4 * c + 4 + j]; }
+                    for (int j = 0; j < 4; ++j) { xc[nxt][j] = colp[64 * (4 * c + 4 + j)]; yr[nxt][j] = rowp[64 * (4 * c + 4 + j)]; }
                 }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -1776,13 +1790,13 @@ __device__ __forceinline__ WalkOut walk_block(const ChainDesc& d, int e, int b0,
         double x4[4] = {0.0, 0.0, 0.0, 0.0}, y4[4] = {0.0, 0.0, 0.0, 0.0};
         const double gu = g * u, gv = g * v;
         {
-            const AS_L double* const rowa = lds_row(gs + a * KP);
-            const AS_L double* const cola = lds_row(gt + a * KP);
+            const AS_L double* const rowa = lds_row(gr + lane());
+            const AS_L double* const cola = lds_row(gt + lane());
             const AS_L double* const uo = lds_row(ush);
             const AS_L double* const vo = lds_row(vsh);
-            double ub[2][4], vb[2][4], gr[2][4], gc[2][4];
+            double ub[2][4], vb[2][4], ga[2][4], gc[2][4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) { ub[0][j] = uo[j]; vb[0][j] = vo[j]; gr[0][j] = rowa[j]; gc[0][j] = cola[j]; }
+            for (int j = 0; j < 4; ++j) { ub[0][j] = uo[j]; vb[0][j] = vo[j]; ga[0][j] = rowa[64 * j]; gc[0][j] = cola[64 * j]; }
 #pragma unroll
             for (int c = 0; c < KB / 4; ++c) {
                 const int cur = c & 1, nxt = cur ^ 1;
@@ -1790,7 +1804,7 @@ __device__ __forceinline__ WalkOut walk_block(const ChainDesc& d, int e, int b0,
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const int b = 4 * c + 4 + j;
-                        ub[nxt][j] = uo[b]; vb[nxt][j] = vo[b]; gr[nxt][j] = rowa[b]; gc[nxt][j] = cola[b];
+                        ub[nxt][j] = uo[b]; vb[nxt][j] = vo[b]; ga[nxt][j] = rowa[64 * b]; gc[nxt][j] = cola[64 * b];
                     }
                 }
                 __builtin_amdgcn_sched_barrier(0);
@@ -1799,7 +1813,7 @@ __device__ __forceinline__ WalkOut walk_block(const ChainDesc& d, int e, int b0,
                     const int b = 4 * c + j;
                     crr[b] += gu * vb[cur][j];
                     ccr[b] += ub[cur][j] * gv;
-                    x4[j] += gr[cur][j] * ub[cur][j];
+                    x4[j] += ga[cur][j] * ub[cur][j];
                     y4[j] += vb[cur][j] * gc[cur][j];
                 }
                 __builtin_amdgcn_sched_barrier(0);
@@ -1815,13 +1829,13 @@ __device__ __forceinline__ WalkOut walk_block(const ChainDesc& d, int e, int b0,
     return WalkOut{rng, sign, nacc};
 }
 
-__device__ __noinline__ void wg_metropolis(const ChainDesc& d, int e, uint64_t& rng_io, double& sign_io, bool record,
-                                          int& naccept, Lds&) {
+// (the results come back by value: references to the caller's variables would be
+// generic pointers into private memory, read and written with flat instructions)
+__device__ __noinline__ WalkOut wg_metropolis(Desc d, int e, uint64_t rng, double sign, bool record, Lds&) {
     Lds& s = g_lds;
     const int N = __builtin_amdgcn_readfirstlane(d.N);
     const int ns = __builtin_amdgcn_readfirstlane(d.nspin);
-    uint64_t rng = rng_io;
-    double sign = sign_io;
+    int naccept = 0;
     const int kb = __builtin_amdgcn_readfirstlane(block_depth(d));
     auto Gs = [&](int sp) { return d.G + (size_t)sp * N * N; };
     // G0[S,S] of the first block, both halves of the walk (also in the one-spin model)
@@ -1869,19 +1883,18 @@ __device__ __noinline__ void wg_metropolis(const ChainDesc& d, int e, uint64_t& 
             }
         }
     }
-    rng_io = rng;
-    sign_io = sign;
     __syncthreads();
+    return WalkOut{rng, sign, naccept};
 }
 
 // ---------------------------------------------------------------------------
 // Equal-time measurements of one slice (src/measure/measure_methods.cpp:22-255)
 // ---------------------------------------------------------------------------
-__device__ __noinline__ void wg_measure_from(const ChainDesc& d, const double* G, double cs, Lds&) {
+__device__ __noinline__ void wg_measure_from(Desc d, const double* G, double cs, Lds&) {
     Lds& s = g_lds;
     const int N = d.N, side = d.side;
-    const double* Gu = G;
-    const double* Gd = d.nspin == 2 ? G + (size_t)N * N : G;
+    const AS_G double* Gu = gptr(G);
+    const AS_G double* Gd = d.nspin == 2 ? Gu + (size_t)N * N : Gu;
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // tr, docc, kin, loc, md, sdw, cdw, sw
     for (int a = tid(); a < N; a += NT) {
         const double du = Gu[(size_t)a * N + a], dd = Gd[(size_t)a * N + a];
@@ -1894,8 +1907,8 @@ __device__ __noinline__ void wg_measure_from(const ChainDesc& d, const double* G
         v[2] += Gu[(size_t)jx * N + a] + Gd[(size_t)jx * N + a] + Gu[(size_t)jy * N + a] + Gd[(size_t)jy * N + a];
     }
     __syncthreads();
-    const AS_G double* Gug = gptr(Gu);
-    const AS_G double* Gdg = gptr(Gd);
+    const AS_G double* Gug = Gu;
+    const AS_G double* Gdg = Gd;
     const AS_G double* cosq = gptr(d.cosq);
     constexpr int MU = 4;   // elements per thread with their loads in flight together
     for (int p0 = tid(); p0 < N * N; p0 += MU * NT) {
@@ -1932,7 +1945,7 @@ __device__ __noinline__ void wg_measure_from(const ChainDesc& d, const double* G
     wg_sum8(v, 8, s);
     if (tid() == 0) {
         const double Nd = (double)N;
-        double* acc = d.acc;
+        AS_G double* acc = gptr(d.acc);
         acc[0] += cs;
         acc[1] += cs * (2 - v[0] / Nd);
         acc[2] += cs * v[1] / Nd;
@@ -1946,20 +1959,20 @@ __device__ __noinline__ void wg_measure_from(const ChainDesc& d, const double* G
     __syncthreads();
 }
 
-__device__ void wg_measure(const ChainDesc& d, double cs, Lds&) { Lds& s = g_lds; wg_measure_from(d, d.G, cs, s); }
+__device__ void wg_measure(Desc d, double cs, Lds&) { Lds& s = g_lds; wg_measure_from(d, d.G, cs, s); }
 
-__device__ void record_slot(const ChainDesc& d, int slot, double sign, bool with_sign) {
+__device__ void record_slot(Desc d, int slot, double sign, bool with_sign) {
     const int N = d.N;
     const size_t n2 = (size_t)N * N;
     AS_G double* dst = gptr(d.recG + (size_t)slot * d.nspin * n2);
     const AS_G double* src = gptr(d.G);
     wg_stream((int)(n2 * d.nspin), [&](int e) { return src[e]; }, [&](int e, double v) { dst[e] = v; });
-    if (with_sign && tid() == 0) d.recSign[slot] = sign;
+    if (with_sign && tid() == 0) gptr(d.recSign)[slot] = sign;
     __syncthreads();
 }
 
 // fresh G for both spins into Gw, wrap error against G, then G <- Gw
-__device__ __noinline__ void stabilize_green(const ChainDesc& d, int lslot, int rslot, bool track_error, ChainState& st, Lds&) {
+__device__ __noinline__ void stabilize_green(Desc d, int lslot, int rslot, bool track_error, ChainState& st, Lds&) {
     Lds& s = g_lds;
     const int N = d.N;
     const size_t n2 = (size_t)N * N;
@@ -1992,7 +2005,7 @@ struct PhaseClock {
 // ---------------------------------------------------------------------------
 // sweeps (src/dqmc_walker.cpp:246-325 and :334-412)
 // ---------------------------------------------------------------------------
-__device__ void sweep_up(const ChainDesc& d, ChainState& st, uint64_t& rng, double& sign, int flags, int& nacc, Lds&) {
+__device__ void sweep_up(Desc d, ChainState& st, uint64_t& rng, double& sign, int flags, int& nacc, Lds&) {
     Lds& s = g_lds;
     const int N = d.N, L = d.L;
     const size_t n2 = (size_t)N * N;
@@ -2004,7 +2017,12 @@ __device__ void sweep_up(const ChainDesc& d, ChainState& st, uint64_t& rng, doub
         load_field_row(d, e, s);
         for (int sp = 0; sp < d.nspin; ++sp) wg_wrap(d, sp, true, d.G + sp * n2, d.Gw + sp * n2, s);
         clk.lap(st, DQMC_PH_WRAP);
-        if (!(flags & DQMC_FLAG_FROZEN)) wg_metropolis(d, e, rng, sign, rec, nacc, s);
+        if (!(flags & DQMC_FLAG_FROZEN)) {
+            const WalkOut o = wg_metropolis(d, e, rng, sign, rec, s);
+            rng = o.rng;
+            sign = o.sign;
+            nacc += o.accepted;
+        }
         clk.lap(st, DQMC_PH_METROPOLIS);
         refresh_scalings(d, s);
         for (int sp = 0; sp < d.nspin; ++sp) wg_bmul(d, 0, sp, d.Tmp + sp * n2, d.Gw + sp * n2, s);
@@ -2026,7 +2044,7 @@ __device__ void sweep_up(const ChainDesc& d, ChainState& st, uint64_t& rng, doub
     st.pos = L;
 }
 
-__device__ void sweep_down(const ChainDesc& d, ChainState& st, uint64_t& rng, double& sign, int flags, int& nacc, Lds&) {
+__device__ void sweep_down(Desc d, ChainState& st, uint64_t& rng, double& sign, int flags, int& nacc, Lds&) {
     Lds& s = g_lds;
     const int N = d.N, L = d.L;
     const size_t n2 = (size_t)N * N;
@@ -2047,7 +2065,12 @@ __device__ void sweep_down(const ChainDesc& d, ChainState& st, uint64_t& rng, do
             clk.lap(st, DQMC_PH_FRESH);
         }
         load_field_row(d, e, s);
-        if (!(flags & DQMC_FLAG_FROZEN)) wg_metropolis(d, e, rng, sign, rec, nacc, s);
+        if (!(flags & DQMC_FLAG_FROZEN)) {
+            const WalkOut o = wg_metropolis(d, e, rng, sign, rec, s);
+            rng = o.rng;
+            sign = o.sign;
+            nacc += o.accepted;
+        }
         clk.lap(st, DQMC_PH_METROPOLIS);
         if (rec) record_slot(d, e, sign, true);
         if (t == L) sign_top = sign;
@@ -2072,7 +2095,7 @@ __device__ void sweep_down(const ChainDesc& d, ChainState& st, uint64_t& rng, do
 
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) k_init(const ChainDesc* __restrict__ descs) {
     Lds& s = g_lds;
-    const ChainDesc& d = descs[blockIdx.x];
+    Desc d = ((const AS_C ChainDesc*)descs)[blockIdx.x];
     ChainState st = *d.st;
     const int N = d.N, L = d.L;
     const size_t n2 = (size_t)N * N;
@@ -2112,7 +2135,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1)))
 
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) k_sweep(const ChainDesc* __restrict__ descs, int n_sweeps, int flags) {
     Lds& s = g_lds;
-    const ChainDesc& d = descs[blockIdx.x];
+    Desc d = ((const AS_C ChainDesc*)descs)[blockIdx.x];
     ChainState st = *d.st;
     uint64_t rng = st.rng;
     double sign = st.sign;
@@ -2142,7 +2165,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1)))
 // (src/measure/measure_handler.cpp:136-150: every method loops over the L stored slices)
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) k_measure_recorded(const ChainDesc* __restrict__ descs) {
     Lds& s = g_lds;
-    const ChainDesc& d = descs[blockIdx.x];
+    Desc d = ((const AS_C ChainDesc*)descs)[blockIdx.x];
     const size_t slot_size = (size_t)d.nspin * d.N * d.N;
     for (int slot = 0; slot < d.L; ++slot) wg_measure_from(d, d.recG + slot * slot_size, d.recSign[slot], s);
 }
