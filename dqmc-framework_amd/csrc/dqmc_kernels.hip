@@ -79,37 +79,112 @@ __device__ __forceinline__ int tid() { return threadIdx.x; }
 __device__ __forceinline__ int lane() { return threadIdx.x & 63; }
 __device__ __forceinline__ int wave() { return threadIdx.x >> 6; }
 
+// Cross-lane moves without the LDS crossbar: DPP within 16-lane rows and the
+// gfx950 row/half swaps across them.  dpp_d<CTRL>: the value of a partner lane
+// (0xB1: lane ^ 1, 0x4E: lane ^ 2, 0x141: lane 7 - i within 8, 0x140: lane
+// 15 - i within 16, 0x128: lane ^ 8).  swap16 / swap32 return, for a pair of
+// values (x, y), x with its odd rows (upper half) exchanged for the even rows
+// (lower half) of y, and y with the exchanged rows: with x = y every lane sees
+// both rows of its row pair (both halves of the wavefront).
+__device__ __forceinline__ double pack_d(unsigned lo, unsigned hi) {
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <int CTRL> __device__ __forceinline__ double dpp_d(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xf, 0xf, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
+    return pack_d(lo, hi);
+}
+template <int CTRL> __device__ __forceinline__ int dpp_i(int v) { return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, false); }
+__device__ __forceinline__ void swap16_d(double& x, double& y) {
+    const unsigned long long bx = (unsigned long long)__double_as_longlong(x), by = (unsigned long long)__double_as_longlong(y);
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)bx, (unsigned)by, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(bx >> 32), (unsigned)(by >> 32), false, false);
+    x = pack_d(lo[0], hi[0]);
+    y = pack_d(lo[1], hi[1]);
+}
+__device__ __forceinline__ void swap32_d(double& x, double& y) {
+    const unsigned long long bx = (unsigned long long)__double_as_longlong(x), by = (unsigned long long)__double_as_longlong(y);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)bx, (unsigned)by, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(bx >> 32), (unsigned)(by >> 32), false, false);
+    x = pack_d(lo[0], hi[0]);
+    y = pack_d(lo[1], hi[1]);
+}
+// the value of lane ^ 16 / lane ^ 32
+__device__ __forceinline__ double xor16_d(double v) { double a = v, b = v; swap16_d(a, b); return (threadIdx.x & 16) ? a : b; }
+__device__ __forceinline__ double xor32_d(double v) { double a = v, b = v; swap32_d(a, b); return (threadIdx.x & 32) ? a : b; }
+__device__ __forceinline__ int xor16_i(int v) {
+    const auto r = __builtin_amdgcn_permlane16_swap((unsigned)v, (unsigned)v, false, false);
+    return (int)((threadIdx.x & 16) ? r[0] : r[1]);
+}
+__device__ __forceinline__ int xor32_i(int v) {
+    const auto r = __builtin_amdgcn_permlane32_swap((unsigned)v, (unsigned)v, false, false);
+    return (int)((threadIdx.x & 32) ? r[0] : r[1]);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
+    v += dpp_d<0xB1>(v);
+    v += dpp_d<0x4E>(v);
+    v += dpp_d<0x141>(v);
+    v += dpp_d<0x140>(v);
+    double a = v, b = v;
+    swap16_d(a, b);
+    v = a + b;
+    a = v; b = v;
+    swap32_d(a, b);
+    return a + b;
 }
 __device__ __forceinline__ double wave_max(double v) {
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-    return v;
+    v = fmax(v, dpp_d<0xB1>(v));
+    v = fmax(v, dpp_d<0x4E>(v));
+    v = fmax(v, dpp_d<0x141>(v));
+    v = fmax(v, dpp_d<0x140>(v));
+    double a = v, b = v;
+    swap16_d(a, b);
+    v = fmax(a, b);
+    a = v; b = v;
+    swap32_d(a, b);
+    return fmax(a, b);
 }
-// argmax with smallest index on ties
+// argmax with smallest index on ties (the result does not depend on the pairing order)
+__device__ __forceinline__ void argmax_step(double& v, int& idx, double ov, int oi) {
+    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+}
 __device__ __forceinline__ void wave_argmax(double& v, int& idx) {
-    for (int o = 32; o > 0; o >>= 1) {
-        double ov = __shfl_xor(v, o);
-        int oi = __shfl_xor(idx, o);
-        if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
-    }
+    argmax_step(v, idx, dpp_d<0xB1>(v), dpp_i<0xB1>(idx));
+    argmax_step(v, idx, dpp_d<0x4E>(v), dpp_i<0x4E>(idx));
+    argmax_step(v, idx, dpp_d<0x141>(v), dpp_i<0x141>(idx));
+    argmax_step(v, idx, dpp_d<0x140>(v), dpp_i<0x140>(idx));
+    argmax_step(v, idx, xor16_d(v), xor16_i(idx));
+    argmax_step(v, idx, xor32_d(v), xor32_i(idx));
 }
 // Reduce-scatter of 16 values over a wavefront: afterwards lane L holds the
-// wavefront sum of slot (L >> 2) & 15 (17 shuffles instead of 16 x 6).
+// wavefront sum of slot (L >> 2) & 15.  Across the halves and rows one swap
+// exchanges a pair of slots at once (the lower lanes keep the first of the
+// pair, the upper lanes the second); within a row the partner across bit 3 is
+// lane ^ 8 and across bit 2 lane 7 - i of the same 8 (any partner across the
+// bit that shares the higher bits gives the same sums).
 __device__ __forceinline__ double wave_reduce16(const double (&v)[16]) {
     const int L = lane();
     double a8[8], a4[4], a2[2];
-    const bool h5 = L & 32, h4 = L & 16, h3 = L & 8, h2 = L & 4;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) a8[i] = (h5 ? v[i + 8] : v[i]) + __shfl_xor(h5 ? v[i] : v[i + 8], 32);
+    for (int i = 0; i < 8; ++i) {
+        double x = v[i], y = v[i + 8];
+        swap32_d(x, y);
+        a8[i] = x + y;
+    }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) a4[i] = (h4 ? a8[i + 4] : a8[i]) + __shfl_xor(h4 ? a8[i] : a8[i + 4], 16);
+    for (int i = 0; i < 4; ++i) {
+        double x = a8[i], y = a8[i + 4];
+        swap16_d(x, y);
+        a4[i] = x + y;
+    }
+    const bool h3 = L & 8, h2 = L & 4;
 #pragma unroll
-    for (int i = 0; i < 2; ++i) a2[i] = (h3 ? a4[i + 2] : a4[i]) + __shfl_xor(h3 ? a4[i] : a4[i + 2], 8);
-    double a1 = (h2 ? a2[1] : a2[0]) + __shfl_xor(h2 ? a2[0] : a2[1], 4);
-    a1 += __shfl_xor(a1, 2);
-    a1 += __shfl_xor(a1, 1);
+    for (int i = 0; i < 2; ++i) a2[i] = (h3 ? a4[i + 2] : a4[i]) + dpp_d<0x128>(h3 ? a4[i] : a4[i + 2]);
+    double a1 = (h2 ? a2[1] : a2[0]) + dpp_d<0x141>(h2 ? a2[0] : a2[1]);
+    a1 += dpp_d<0x4E>(a1);
+    a1 += dpp_d<0xB1>(a1);
     return a1;
 }
 

@@ -87,8 +87,8 @@ def test_sweeps_match_oracle(ctx, case):
     side, L, beta, pace, model, cb, U, mu = case
     print(f"[gpu-parity] case {case}", flush=True)
     orc, b = pair(ctx, side, L, beta, pace, model, cb, U, mu)
-    for spin in (1, -1):
-        assert np.max(np.abs(b.green(0, spin) - orc.green(spin))) < 1e-12
+    for spin in (1, -1):   # the initial G is recomputed from the UDV stack: the fresh-G bar
+        assert np.max(np.abs(b.green(0, spin) - orc.green(spin))) < TOL_G
     assert b.info(0).sign == orc.sign
     for it in range(3):
         orc.sweep_forward(); b.sweep(1, record=True)
