@@ -417,9 +417,9 @@ template <int V> struct IntC { static constexpr int value = V; };
 // within a slab are computed once.  C is prefetched before the last step of a
 // tile when beta != 0.
 template <bool lowp, bool aL, bool bL>
-__device__ __noinline__ void wg_gemm_mfma(int M, int Nc, int K, const double* A_, int lda, bool tA, const double* B_,
+__device__ __forceinline__ void gemm_body(int M, int Nc, int K, const double* A_, int lda, bool tA, const double* B_,
                                           int ldb, bool tB, double* C, int ldc, const double* rs, const double* ms_,
-                                          const double* cs, double beta, Lds&, double alpha = 1.0) {
+                                          const double* cs, double beta, double alpha) {
     Lds& s = g_lds;
     M = uni(M); Nc = uni(Nc); K = uni(K); lda = uni(lda); ldb = uni(ldb); ldc = uni(ldc);
     tA = uni(tA); tB = uni(tB); beta = uni(beta); alpha = uni(alpha);
@@ -627,6 +627,31 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int Nc, int K, const double* A_
         advance(r0, c0, k0);
     }
     __syncthreads();
+}
+
+template <bool lowp, bool aL, bool bL>
+__device__ __noinline__ void wg_gemm_mfma(int M, int Nc, int K, const double* A, int lda, bool tA, const double* B,
+                                          int ldb, bool tB, double* C, int ldc, const double* rs, const double* ms,
+                                          const double* cs, double beta, Lds&, double alpha = 1.0) {
+    gemm_body<lowp, aL, bL>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, alpha);
+}
+// C <- C - V op(T) (V^T C) for a block reflector (V: m x jb at ld ldv, T: jb x jb at ld NB,
+// op(T) = T^T when tT; C: m x n at ld ldc; W1, W2: jb x n scratch at ld NB), the three
+// products in one call so that the register save and restore of a call is paid once
+__device__ __noinline__ void wg_reflect(const double* V, int ldv, const double* T, bool tT, double* C, int ldc, int m,
+                                        int n, int jb, double* W1, double* W2, Lds&) {
+    for (int i = 0; i < 3; ++i) {
+        const int M = i < 2 ? jb : m, K = i == 0 ? m : jb;
+        const double* A = i == 1 ? T : V;
+        const int lda = i == 1 ? NB : ldv;
+        const bool tA = i == 0 ? true : (i == 1 ? tT : false);
+        const double* B = i == 0 ? C : (i == 1 ? W1 : W2);
+        const int ldb = i == 0 ? ldc : NB;
+        double* const Co = i == 0 ? W1 : (i == 1 ? W2 : C);
+        const int ldo = i == 2 ? ldc : NB;
+        gemm_body<false, false, false>(M, n, K, A, lda, tA, B, ldb, false, Co, ldo, nullptr, nullptr, nullptr,
+                                       i == 2 ? 1.0 : 0.0, i == 2 ? -1.0 : 1.0);
+    }
 }
 
 // GEMMs of the engine: FP64 on the matrix cores for everything that stays
@@ -1348,7 +1373,6 @@ __device__ __noinline__ void qr_panel(int m, int jb, int k0, int N, double* M_, 
 __device__ __noinline__ void wg_qr_udv(double* M, double* Mp, int N, double* U, double* D, double* Tl, double* Wq, Lds&) {
     Lds& s = g_lds;
     double* P = s.panel;                  // m x jb panel, ld = m
-    double* Tb = s.panel + NMAX * NB;     // NB x NB T of the current panel (ld NB)
     double* Wsm = Wq;                     // NB x N GEMM scratch (ld NB)
     double* Wsm2 = Wq + (size_t)NB * N;   // NB x N GEMM scratch (ld NB)
     double* Tg = Wq + (size_t)2 * NB * N; // T factors of all panels, NB x NB each
@@ -1400,9 +1424,7 @@ __device__ __noinline__ void wg_qr_udv(double* M, double* Mp, int N, double* U, 
         if (n2 > 0) {
             const double* V = M + (size_t)k0 * N + k0;
             double* A2 = Mp + (size_t)(k0 + jb) * N + k0;
-            gemm64(jb, n2, m, V, N, true, A2, N, false, Wsm, NB, nullptr, nullptr, nullptr, 0.0, s);
-            gemm64_lg(jb, n2, jb, Tb, NB, true, Wsm, NB, false, Wsm2, NB, 0.0, s);
-            gemm64(m, n2, jb, V, N, false, Wsm2, NB, false, A2, N, nullptr, nullptr, nullptr, 1.0, s, -1.0);
+            wg_reflect(V, N, Tg + (size_t)(k0 / NB) * NB * NB, true, A2, N, m, n2, jb, Wsm, Wsm2, s);
         }
     }
     // D and Tl = D^-1 R P^T (D also kept in LDS for the division)
@@ -1425,9 +1447,7 @@ __device__ __noinline__ void wg_qr_udv(double* M, double* Mp, int N, double* U, 
         const double* V = M + (size_t)k0 * N + k0;
         const double* T = Tg + (size_t)(k0 / NB) * NB * NB;
         double* U2 = U + (size_t)k0 * N + k0;
-        gemm64(jb, m, m, V, N, true, U2, N, false, Wsm, NB, nullptr, nullptr, nullptr, 0.0, s);
-        gemm64(jb, m, jb, T, NB, false, Wsm, NB, false, Wsm2, NB, nullptr, nullptr, nullptr, 0.0, s);
-        gemm64(m, m, jb, V, N, false, Wsm2, NB, false, U2, N, nullptr, nullptr, nullptr, 1.0, s, -1.0);
+        wg_reflect(V, N, T, false, U2, N, m, m, jb, Wsm, Wsm2, s);
     }
 }
 
