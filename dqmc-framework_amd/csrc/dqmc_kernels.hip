@@ -1156,11 +1156,18 @@ __device__ __noinline__ double wg_lu_solve(double* A, double* B, int N, int nrhs
         }
         __syncthreads();
         if (jb < m) {
-            if (nright > 0)
-                gemm64_lg(m - jb, nright, jb, P + jb, m, false, A + (size_t)(k0 + jb) * N + k0, N, false,
-                          A + (size_t)(k0 + jb) * N + k0 + jb, N, 1.0, s, -1.0);
-            if (nrhs > 0)
-                gemm64_lg(m - jb, nrhs, jb, P + jb, m, false, B + k0, N, false, B + k0 + jb, N, 1.0, s, -1.0);
+            if (B == A + (size_t)N * N) {
+                // B continues A in memory: one product over the columns right of the panel and B
+                if (nright + nrhs > 0)
+                    gemm64_lg(m - jb, nright + nrhs, jb, P + jb, m, false, A + (size_t)(k0 + jb) * N + k0, N, false,
+                              A + (size_t)(k0 + jb) * N + k0 + jb, N, 1.0, s, -1.0);
+            } else {
+                if (nright > 0)
+                    gemm64_lg(m - jb, nright, jb, P + jb, m, false, A + (size_t)(k0 + jb) * N + k0, N, false,
+                              A + (size_t)(k0 + jb) * N + k0 + jb, N, 1.0, s, -1.0);
+                if (nrhs > 0)
+                    gemm64_lg(m - jb, nrhs, jb, P + jb, m, false, B + k0, N, false, B + k0 + jb, N, 1.0, s, -1.0);
+            }
         }
         __syncthreads();
     }
@@ -1376,7 +1383,6 @@ __device__ __noinline__ void wg_qr_udv(double* M, double* Mp, int N, double* U, 
     double* Wsm = Wq;                     // NB x N GEMM scratch (ld NB)
     double* Wsm2 = Wq + (size_t)NB * N;   // NB x N GEMM scratch (ld NB)
     double* Tg = Wq + (size_t)2 * NB * N; // T factors of all panels, NB x NB each
-    double* tau = s.v1;
     N = uni(N);
     const AS_G double* Mg = gptr(uni(M));
     AS_G double* Mpg = gptr(uni(Mp));
