@@ -699,7 +699,14 @@ __device__ __forceinline__ void wg_stream(int n, Ld ld, St st) {
 #pragma unroll
         for (int q = 0; q < U; ++q) st(e0 + q * NT, v[q]);
     }
-    for (; e0 < n; e0 += NT) st(e0, ld(e0));
+    if (e0 < n) {   // the rest in one more round: indices clamped for the loads, stores predicated
+        double v[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) v[q] = ld(min(e0 + q * NT, n - 1));
+#pragma unroll
+        for (int q = 0; q < U; ++q)
+            if (e0 + q * NT < n) st(e0 + q * NT, v[q]);
+    }
 }
 __device__ __forceinline__ const AS_G double* gptr(const double* p) { return (const AS_G double*)p; }
 __device__ __forceinline__ AS_G double* gptr(double* p) { return (AS_G double*)p; }
@@ -708,7 +715,7 @@ __device__ __forceinline__ AS_G double* gptr(double* p) { return (AS_G double*)p
 __device__ __noinline__ void wg_copy(double* dst_, const double* src_, int n2) {
     AS_G double* dst = gptr(uni(dst_));
     const AS_G double* src = gptr(uni(src_));
-    wg_stream(uni(n2), [&](int e) { return src[e]; }, [&](int e, double v) { dst[e] = v; });
+    wg_stream<16>(uni(n2), [&](int e) { return src[e]; }, [&](int e, double v) { dst[e] = v; });
     __syncthreads();
 }
 __device__ void wg_identity(double* M_, int N) {
@@ -814,7 +821,7 @@ __device__ __noinline__ void wg_cb_left_fused(double* M, int N, int side, const 
     for (int c0 = 0; c0 < N; c0 += cb) {
         const int nc = min(cb, N - c0);
         AS_G double* blk = Mg + (size_t)c0 * N;
-        wg_stream(nc * N, [&](int e) { return blk[e]; },
+        wg_stream<16>(nc * N, [&](int e) { return blk[e]; },
                   [&](int e, double v) { s.panel[e] = has_pre ? v * prel[e % N] : v; });
         __syncthreads();
         cb_pass_lds_left(s.panel, N, nc, side, P, s1);
@@ -840,7 +847,7 @@ __device__ __noinline__ void wg_cb_right_fused(double* M, int N, int side, const
     const int rb = max(1, min(N, PANEL_DOUBLES / N));
     for (int r0 = 0; r0 < N; r0 += rb) {
         const int nr = min(rb, N - r0);
-        wg_stream(nr * N, [&](int e) { const int r = e % nr, c = e / nr; return Mg[(size_t)c * N + r0 + r]; },
+        wg_stream<16>(nr * N, [&](int e) { const int r = e % nr, c = e / nr; return Mg[(size_t)c * N + r0 + r]; },
                   [&](int e, double v) { s.panel[e] = has_pre ? v * prel[e / nr] : v; });
         __syncthreads();
         cb_pass_lds_right(s.panel, nr, side, P, s1);
