@@ -708,6 +708,22 @@ __device__ __forceinline__ void wg_stream(int n, Ld ld, St st) {
             if (e0 + q * NT < n) st(e0 + q * NT, v[q]);
     }
 }
+// two streams of the same length with the loads of both in flight in every round
+template <int U = 8, class Ld1, class St1, class Ld2, class St2>
+__device__ __forceinline__ void wg_stream2(int n, Ld1 ld1, St1 st1, Ld2 ld2, St2 st2) {
+    for (int e0 = tid(); e0 < n; e0 += U * NT) {
+        double v[U], w[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int e = min(e0 + q * NT, n - 1);   // (clamped in the last round; not stored)
+            v[q] = ld1(e);
+            w[q] = ld2(e);
+        }
+#pragma unroll
+        for (int q = 0; q < U; ++q)
+            if (e0 + q * NT < n) { st1(e0 + q * NT, v[q]); st2(e0 + q * NT, w[q]); }
+    }
+}
 __device__ __forceinline__ const AS_G double* gptr(const double* p) { return (const AS_G double*)p; }
 __device__ __forceinline__ AS_G double* gptr(double* p) { return (AS_G double*)p; }
 
@@ -1660,11 +1676,11 @@ __device__ __noinline__ void block_stage_w(const double* G_, int N, int b0, int 
     double* const acp = s.panel;
     double* const gsw = acp + (size_t)N * k4;
     const double* const cr = MP_CR(sp);
-    wg_stream(N * k4, [&](int e) { return G[(size_t)b0 * N + min(e, N * k - 1)]; },
-              [&](int e, double v) { acp[e] = e < N * k ? v : 0.0; });
-    // (element e = c k4 + b: consecutive threads read down a column of G)
-    wg_stream(N * k4, [&](int e) { const int c = e / k4, b = e - c * k4; return G[(size_t)c * N + b0 + min(b, k - 1)]; },
-              [&](int e, double v) { const int c = e / k4, b = e - c * k4; gsw[b * N + c] = b < k ? v : 0.0; });
+    // ACP and GSW together (GSW element e = c k4 + b: consecutive threads read down a column of G)
+    wg_stream2(N * k4, [&](int e) { return G[(size_t)b0 * N + min(e, N * k - 1)]; },
+               [&](int e, double v) { acp[e] = e < N * k ? v : 0.0; },
+               [&](int e) { const int c = e / k4, b = e - c * k4; return G[(size_t)c * N + b0 + min(b, k - 1)]; },
+               [&](int e, double v) { const int c = e / k4, b = e - c * k4; gsw[b * N + c] = b < k ? v : 0.0; });
     __syncthreads();
     // W = C G0[S,:] - C E_S, column block by column block (each wavefront owns whole column blocks,
     // so W may overwrite GSW in place); a wavefront advances up to 4 of its column blocks together.
