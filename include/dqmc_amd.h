@@ -52,7 +52,10 @@ extern "C" {
 /* arithmetic of the slice products and wraps: FP64 throughout, or FP32 matrix
  * cores with FP32 accumulation for the B-matrix products and wraps while the
  * UDV stacks, the (1 + UDV)^-1 recompute, the delayed updates and the
- * Metropolis ratio stay FP64 */
+ * Metropolis ratio stay FP64.  The FP32 mode is for small beta*U: its fresh G
+ * carries ~1e-4 error, which the updates and wraps between stabilizations
+ * amplify (at beta = 10, U = 6, pace 8 the wrap error of live chains reaches
+ * O(0.1)); ChainInfo.wrap_error reports it per chain. */
 #define DQMC_PREC_FP64 0
 #define DQMC_PREC_MIXED 1
 

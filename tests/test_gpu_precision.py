@@ -6,7 +6,10 @@ runs visit identical fields, so the difference is the arithmetic alone.
 FP32 accumulation rounds every wrap at ~2^-24 relative, and the error grows
 over the wraps between stabilizations, so G carries 1e-5..1e-3 absolute
 error (not the 1e-6 of an FP64-accurate product); the equal-time observables
-agree to ~1e-5 relative.  The bars below state that measured envelope."""
+agree to ~1e-6 relative.  The bars below state that measured envelope.  On a
+live chain the Metropolis updates amplify the error further between
+stabilizations (at beta 10, U 6, pace 8 the wrap error reaches O(0.1)), which
+is why this comparison freezes the fields."""
 import numpy as np
 import pytest
 
