@@ -404,7 +404,6 @@ template <bool lds> struct MemSpace { typedef const AS_G double* cptr; };
 template <> struct MemSpace<true> { typedef const AS_L double* cptr; };
 
 #define KS 16            // slab depth of the GEMM
-#define SQ (KS * 64 / NT)  // elements of A (and of B) per thread and slab
 template <int V> struct IntC { static constexpr int value = V; };
 // Two-ahead slab pipeline.  A step first reads every operand of its 16-deep
 // slab from LDS into registers and releases the slab at a barrier; its MFMAs
@@ -416,7 +415,9 @@ template <int V> struct IntC { static constexpr int value = V; };
 // scalar registers and advanced by additions; each thread's element offsets
 // within a slab are computed once.  C is prefetched before the last step of a
 // tile when beta != 0.
-template <bool lowp, bool aL, bool bL>
+// TM = 64: 64x64 tiles, wavefront (w & 1, w >> 1) owns the 32x32 quarter; TM = 16 (outputs of at
+// most 16 rows): 16x128 tiles, wavefront w owns columns 32w..32w+31, so all four wavefronts work.
+template <bool lowp, bool aL, bool bL, int TM>
 __device__ __forceinline__ void gemm_body(int M, int Nc, int K, const double* A_, int lda, bool tA, const double* B_,
                                           int ldb, bool tB, double* C, int ldc, const double* rs, const double* ms_,
                                           const double* cs, double beta, double alpha) {
@@ -436,18 +437,28 @@ __device__ __forceinline__ void gemm_body(int M, int Nc, int K, const double* A_
     AS_G double* const Cg = (AS_G double*)C;
     const int l = lane();
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
+    constexpr int TN = TM == 64 ? 64 : 128;              // tile columns
+    constexpr int SQA = TM * KS / NT, SQB = TN * KS / NT;  // slab elements per thread (A, B)
+    constexpr int RBM = TM == 64 ? 2 : 1;                // 16-row blocks per wavefront
+    // slab layout in the As/Bs space (2 x 16 x GLD doubles): TM = 64: A and B at row stride GLD;
+    // TM = 16: B (16 x 128) at row stride 144, then A (16 x 16) at row stride 16 (both strides
+    // are 32 mod 64 dwords, so the lane groups of an operand read fall on disjoint banks)
+    constexpr int GA = TM == 64 ? GLD : 16, GB = TM == 64 ? GLD : 144;
+    static_assert(TM == 64 || KS * GB + KS * GA <= 2 * 16 * GLD, "skinny slabs exceed the GEMM staging space");
+    double* const Asl = TM == 64 ? &s.As[0][0] : &s.As[0][0] + KS * GB;
+    double* const Bsl = TM == 64 ? &s.Bs[0][0] : &s.As[0][0];
+    const int wr = TM == 64 ? (w & 1) * 32 : 0, wc = TM == 64 ? (w >> 1) * 32 : w * 32;
     if (M <= 0 || Nc <= 0 || K <= 0) { __syncthreads(); return; }
-    float* Af = reinterpret_cast<float*>(&s.As[0][0]);  // [16][2 GLD] floats in lowp mode
-    float* Bf = reinterpret_cast<float*>(&s.Bs[0][0]);
+    float* Af = reinterpret_cast<float*>(Asl);  // [16][2 GA] floats in lowp mode
+    float* Bf = reinterpret_cast<float*>(Bsl);
     const int t = tid();
-    // this thread's SQ elements of a slab, q = 0..SQ-1, are affine in q:
+    // this thread's elements of a slab (SQA of A, SQB of B) are affine in their index q:
     //   A element q at (row ma0 + q*dma, depth ka0 + q*dka), offset oa0 + q*doa from the slab origin,
     //   B element q at (depth kb0 + q*dkb, column nb0 + q*dnb), offset ob0 + q*dob
-    const int ma0 = tA ? t / KS : t & 63, ka0 = tA ? t % KS : t >> 6;
-    const int dma = tA ? NT / KS : 0, dka = tA ? 0 : NT / 64;
-    const int kb0 = tB ? t >> 6 : t % KS, nb0 = tB ? t & 63 : t / KS;
-    const int dkb = tB ? NT / 64 : 0, dnb = tB ? 0 : NT / KS;
+    const int ma0 = tA ? t / KS : t % TM, ka0 = tA ? t % KS : t / TM;
+    const int dma = tA ? NT / KS : 0, dka = tA ? 0 : NT / TM;
+    const int kb0 = tB ? t / TN : t % KS, nb0 = tB ? t % TN : t / KS;
+    const int dkb = tB ? NT / TN : 0, dnb = tB ? 0 : NT / KS;
     const int oa0 = tA ? ma0 * lda + ka0 : ka0 * lda + ma0, doa = tA ? dma * lda : dka * lda;
     const int ob0 = tB ? kb0 * ldb + nb0 : nb0 * ldb + kb0, dob = tB ? dkb * ldb : dnb * ldb;
     // slab origin (r0, c0, k0) -> offsets of its first element in A and B (strides
@@ -455,38 +466,47 @@ __device__ __forceinline__ void gemm_body(int M, int Nc, int K, const double* A_
     const int sar = tA ? lda : 1, sak = tA ? 1 : lda, sbc = tB ? 1 : ldb, sbk = tB ? ldb : 1;
     auto a_base = [&](int r0, int k0) { return r0 * sar + k0 * sak; };
     auto b_base = [&](int c0, int k0) { return c0 * sbc + k0 * sbk; };
-    double va[SQ], vb[SQ], msv[SQ];
-    unsigned okm = 0;   // bit q: A element q of the loaded slab in range; bit SQ + q: B element q
+    double va[SQA], vb[SQB], msv[SQA];
+    unsigned okm = 0;   // bit q: A element q of the loaded slab in range; bit SQA + q: B element q
     auto load = [&](int r0, int c0, int k0) {
         const int ba = a_base(r0, k0), bb = b_base(c0, k0);
         const int mr = M - r0, nr = Nc - c0, kr = K - k0;
         unsigned ok = 0;
 #pragma unroll
-        for (int q = 0; q < SQ; ++q) {
+        for (int q = 0; q < SQA; ++q) {
             const bool okA = (ma0 + q * dma < mr) & (ka0 + q * dka < kr);
-            const bool okB = (nb0 + q * dnb < nr) & (kb0 + q * dkb < kr);
             va[q] = A[okA ? ba + oa0 + q * doa : 0];
-            vb[q] = B[okB ? bb + ob0 + q * dob : 0];
             ok |= (unsigned)okA << q;
-            ok |= (unsigned)okB << (SQ + q);
+        }
+#pragma unroll
+        for (int q = 0; q < SQB; ++q) {
+            const bool okB = (nb0 + q * dnb < nr) & (kb0 + q * dkb < kr);
+            vb[q] = B[okB ? bb + ob0 + q * dob : 0];
+            ok |= (unsigned)okB << (SQA + q);
         }
         okm = ok;
     };
     // the K scalings of the loaded slab (depth origin k0s), read ahead of its store
     auto load_ms = [&](int k0s) {
 #pragma unroll
-        for (int q = 0; q < SQ; ++q) msv[q] = ms[has_ms ? min(k0s + ka0 + q * dka, K - 1) : 0];
+        for (int q = 0; q < SQA; ++q) msv[q] = ms[has_ms ? min(k0s + ka0 + q * dka, K - 1) : 0];
     };
     // loaded slab -> LDS
     auto store = [&]() {
 #pragma unroll
-        for (int q = 0; q < SQ; ++q) {
-            const int m = ma0 + q * dma, k = ka0 + q * dka, kbq = kb0 + q * dkb, n = nb0 + q * dnb;
+        for (int q = 0; q < SQA; ++q) {
+            const int m = ma0 + q * dma, k = ka0 + q * dka;
             double v = ((okm >> q) & 1u) ? va[q] : 0.0;
-            const double x = ((okm >> (SQ + q)) & 1u) ? vb[q] : 0.0;
             v = has_ms ? v * msv[q] : v;
-            if (lowp) { Af[k * 2 * GLD + m] = (float)v; Bf[kbq * 2 * GLD + n] = (float)x; }
-            else { s.As[k][m] = v; s.Bs[kbq][n] = x; }
+            if (lowp) Af[k * 2 * GA + m] = (float)v;
+            else Asl[k * GA + m] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < SQB; ++q) {
+            const int kbq = kb0 + q * dkb, n = nb0 + q * dnb;
+            const double x = ((okm >> (SQA + q)) & 1u) ? vb[q] : 0.0;
+            if (lowp) Bf[kbq * 2 * GB + n] = (float)x;
+            else Bsl[kbq * GB + n] = x;
         }
     };
     // cursors: (r0, c0, k0) the slab multiplied in this step, sk the depth origin of
@@ -496,7 +516,7 @@ __device__ __forceinline__ void gemm_body(int M, int Nc, int K, const double* A_
     int lr = 0, lc = 0, lk = 0;
     auto advance = [&](int& r, int& c, int& k) {
         k += KS;
-        if (k >= K) { k = 0; r += 64; if (r >= M) { r = 0; c += 64; } }
+        if (k >= K) { k = 0; r += TM; if (r >= M) { r = 0; c += TN; } }
     };
     load(lr, lc, lk);
     advance(lr, lc, lk);
@@ -525,7 +545,7 @@ __device__ __forceinline__ void gemm_body(int M, int Nc, int K, const double* A_
         const int cbase = (c0 + wc) * ldc + r0 + wr;
         if (last && beta != 0.0) {   // C of the tile, in flight during the last step
 #pragma unroll
-            for (int rb = 0; rb < 2; ++rb)
+            for (int rb = 0; rb < RBM; ++rb)
 #pragma unroll
                 for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
@@ -545,11 +565,11 @@ __device__ __forceinline__ void gemm_body(int M, int Nc, int K, const double* A_
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 if (lowp) {
-                    fa[kk][h] = Af[k * 2 * GLD + wr + 16 * h + (l & 15)];
-                    fb[kk][h] = Bf[k * 2 * GLD + wc + 16 * h + (l & 15)];
+                    if (h < RBM) fa[kk][h] = Af[k * 2 * GA + wr + 16 * h + (l & 15)];
+                    fb[kk][h] = Bf[k * 2 * GB + wc + 16 * h + (l & 15)];
                 } else {
-                    oa[kk][h] = s.As[k][wr + 16 * h + (l & 15)];
-                    ob[kk][h] = s.Bs[k][wc + 16 * h + (l & 15)];
+                    if (h < RBM) oa[kk][h] = Asl[k * GA + wr + 16 * h + (l & 15)];
+                    ob[kk][h] = Bsl[k * GB + wc + 16 * h + (l & 15)];
                 }
             }
         }
@@ -582,8 +602,8 @@ __device__ __forceinline__ void gemm_body(int M, int Nc, int K, const double* A_
                 __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);   // global load
             }
         };
-        if (rows > 16 && cols > 16) step(IntC<2>(), IntC<2>());
-        else if (rows > 16 && cols > 0) step(IntC<2>(), IntC<1>());
+        if (RBM == 2 && rows > 16 && cols > 16) step(IntC<RBM>(), IntC<2>());
+        else if (RBM == 2 && rows > 16 && cols > 0) step(IntC<RBM>(), IntC<1>());
         else if (rows > 0 && cols > 16) step(IntC<1>(), IntC<2>());
         else if (rows > 0 && cols > 0) step(IntC<1>(), IntC<1>());
         else step(IntC<0>(), IntC<0>());
@@ -602,13 +622,13 @@ __device__ __forceinline__ void gemm_body(int M, int Nc, int K, const double* A_
             }
             if (has_rs) {
 #pragma unroll
-                for (int rb = 0; rb < 2; ++rb) {
+                for (int rb = 0; rb < RBM; ++rb) {
                     const int row = rb * 16 + (l & 15);
                     rsv[rb] = rsl[row < rows ? r0 + wr + row : 0];
                 }
             }
 #pragma unroll
-            for (int rb = 0; rb < 2; ++rb)
+            for (int rb = 0; rb < RBM; ++rb)
 #pragma unroll
                 for (int cb = 0; cb < 2; ++cb) {
 #pragma unroll
@@ -633,25 +653,28 @@ template <bool lowp, bool aL, bool bL>
 __device__ __noinline__ void wg_gemm_mfma(int M, int Nc, int K, const double* A, int lda, bool tA, const double* B,
                                           int ldb, bool tB, double* C, int ldc, const double* rs, const double* ms,
                                           const double* cs, double beta, Lds&, double alpha = 1.0) {
-    gemm_body<lowp, aL, bL>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, alpha);
+    if (uni(M) <= 16) gemm_body<lowp, aL, bL, 16>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, alpha);
+    else gemm_body<lowp, aL, bL, 64>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, alpha);
 }
 // C <- C - V op(T) (V^T C) for a block reflector (V: m x jb at ld ldv, T: jb x jb at ld NB,
 // op(T) = T^T when tT; C: m x n at ld ldc; W1, W2: jb x n scratch at ld NB), the three
 // products in one call so that the register save and restore of a call is paid once
 __device__ __noinline__ void wg_reflect(const double* V, int ldv, const double* T, bool tT, double* C, int ldc, int m,
                                         int n, int jb, double* W1, double* W2, Lds&) {
-    for (int i = 0; i < 3; ++i) {
-        const int M = i < 2 ? jb : m, K = i == 0 ? m : jb;
+    // W1 = V^T C and W2 = op(T) W1 have jb <= 16 rows (16-row tiles), C -= V W2 has m rows
+    for (int i = 0; i < 2; ++i) {
+        const int K = i == 0 ? m : jb;
         const double* A = i == 1 ? T : V;
         const int lda = i == 1 ? NB : ldv;
-        const bool tA = i == 0 ? true : (i == 1 ? tT : false);
-        const double* B = i == 0 ? C : (i == 1 ? W1 : W2);
+        const bool tA = i == 0 ? true : tT;
+        const double* B = i == 0 ? C : W1;
         const int ldb = i == 0 ? ldc : NB;
-        double* const Co = i == 0 ? W1 : (i == 1 ? W2 : C);
-        const int ldo = i == 2 ? ldc : NB;
-        gemm_body<false, false, false>(M, n, K, A, lda, tA, B, ldb, false, Co, ldo, nullptr, nullptr, nullptr,
-                                       i == 2 ? 1.0 : 0.0, i == 2 ? -1.0 : 1.0);
+        double* const Co = i == 0 ? W1 : W2;
+        gemm_body<false, false, false, 16>(jb, n, K, A, lda, tA, B, ldb, false, Co, NB, nullptr, nullptr, nullptr, 0.0,
+                                           1.0);
     }
+    gemm_body<false, false, false, 64>(m, n, jb, V, ldv, false, W2, NB, false, C, ldc, nullptr, nullptr, nullptr, 1.0,
+                                       -1.0);
 }
 
 // GEMMs of the engine: FP64 on the matrix cores for everything that stays

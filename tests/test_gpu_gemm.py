@@ -7,7 +7,9 @@ import dqmc_amd as D
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(16, 16, 16), (64, 64, 64), (100, 36, 144), (144, 144, 144), (7, 65, 3), (256, 256, 256), (400, 400, 400)]
+SHAPES = [(16, 16, 16), (64, 64, 64), (100, 36, 144), (144, 144, 144), (7, 65, 3), (256, 256, 256), (400, 400, 400),
+          # outputs of at most 16 rows take 16 x 128 tiles: several column tiles, partial slabs
+          (16, 144, 144), (16, 300, 37), (9, 130, 20), (1, 257, 5)]
 
 
 @pytest.fixture(scope="module")
@@ -36,7 +38,7 @@ def test_gemm_variants(ctx, variant, shape, tA, tB):
     assert err < tol, err
 
 
-LDS_SHAPES = [(16, 144, 16), (32, 144, 32), (144, 36, 32), (7, 65, 3), (64, 100, 64), (16, 400, 16)]
+LDS_SHAPES = [(16, 144, 16), (32, 144, 32), (144, 36, 32), (7, 65, 3), (64, 100, 64), (16, 400, 16), (9, 300, 20)]
 
 
 @pytest.mark.parametrize("variant", [3, 4])
