@@ -1024,9 +1024,114 @@ __device__ void wg_wrap(Desc d, int sp, bool up, double* G, double* W, Lds&) {
 // j is stored as s.perm[k0 + j] = k0 + p.  Returns the sign the interchanges
 // and pivots contribute to det.
 // ---------------------------------------------------------------------------
+// One wavefront factors a panel of m <= 64 RPL rows: lane L holds rows L + 64 i (i < RPL) in
+// registers, the pivot is a wavefront argmax (DPP and row swaps) and rows j and p are broadcast
+// through LDS inside the wavefront, so a column step needs no workgroup barrier.  Same
+// operations in the same order as the four-wavefront form below (bit-identical factors).
+template <int RPL>
+__device__ __forceinline__ double lu_panel_wave(int m, int jb, int k0) {
+    Lds& s = g_lds;
+    const int L = lane();
+    double* const P = s.panel;
+    double* const rowbuf = s.panel + PANEL_RED + RED_CAND;   // row j at [0, NB), row p at [NB, 2 NB)
+    double x[RPL][NB];
+#pragma unroll
+    for (int i = 0; i < RPL; ++i)
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            const int r = L + 64 * i;
+            x[i][c] = (r < m && c < jb) ? P[c * m + r] : 0.0;
+        }
+    double detsign = 1.0;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      if (j < jb) {
+        double bv = -1.0;
+        int bi = j;
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) {
+            const int r = L + 64 * i;
+            if (r >= j && r < m) {
+                const double v = fabs(x[i][j]);
+                if (v > bv || (v == bv && r < bi)) { bv = v; bi = r; }
+            }
+        }
+        wave_argmax(bv, bi);
+        const int p = __builtin_amdgcn_readfirstlane(bi);
+        const int pl = p & 63, ps = p >> 6;
+        if (L == 0) s.perm[k0 + j] = k0 + p;
+        if (L == j)
+#pragma unroll
+            for (int c = 0; c < NB; ++c) rowbuf[c] = x[0][c];
+        if (L == pl)
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                double v = x[0][c];
+#pragma unroll
+                for (int i = 1; i < RPL; ++i) v = (ps == i) ? x[i][c] : v;
+                rowbuf[NB + c] = v;
+            }
+        __builtin_amdgcn_wave_barrier();
+        double rowj[NB], rowp[NB];
+#pragma unroll
+        for (int c = 0; c < NB; ++c) { rowj[c] = rowbuf[c]; rowp[c] = rowbuf[NB + c]; }
+        __builtin_amdgcn_wave_barrier();
+        if (p != j) {
+            detsign = -detsign;
+            if (L == pl)
+#pragma unroll
+                for (int i = 0; i < RPL; ++i)
+                    if (ps == i)
+#pragma unroll
+                        for (int c = 0; c < NB; ++c) x[i][c] = rowj[c];
+            if (L == j)
+#pragma unroll
+                for (int c = 0; c < NB; ++c) x[0][c] = rowp[c];
+        }
+        const double piv = rowp[j];
+        if (piv < 0) detsign = -detsign;
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) {
+            const int r = L + 64 * i;
+            if (r > j && r < m) {
+                const double l = x[i][j] / piv;
+#pragma unroll
+                for (int c = 0; c < NB; ++c)
+                    if (c > j) x[i][c] -= l * rowp[c];
+                x[i][j] = l;
+            }
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < RPL; ++i) {
+        const int r = L + 64 * i;
+        if (r >= m) continue;
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+            if (c < jb) P[c * m + r] = x[i][c];
+    }
+    return detsign;
+}
+
 __device__ __noinline__ double lu_panel(int m, int jb, int k0, Lds&) {
     Lds& s = g_lds;
     m = uni(m); jb = uni(jb); k0 = uni(k0);
+    if (m <= 256) {
+        double* const dsl = s.panel + PANEL_RED + RED_CVAL;
+        if (wave() == 0) {
+            double ds;
+            if (m <= 64) ds = lu_panel_wave<1>(m, jb, k0);
+            else if (m <= 128) ds = lu_panel_wave<2>(m, jb, k0);
+            else if (m <= 192) ds = lu_panel_wave<3>(m, jb, k0);
+            else ds = lu_panel_wave<4>(m, jb, k0);
+            if (lane() == 0) dsl[0] = ds;
+        }
+        __syncthreads();
+        const double ds = dsl[0];
+        __syncthreads();
+        return ds;
+    }
     const int t = tid(), w = wave();
     double* const P = s.panel;
     double* const red = s.panel + PANEL_RED;
