@@ -1366,12 +1366,163 @@ __device__ __noinline__ double wg_lu_solve(double* A, double* B, int N, int nrhs
 // to Mp and the explicit V (unit lower) to M, both at (k0, k0) with ld N; tau to
 // s.v1[k0..]; T to Tb (LDS, ld NB) and Tg.
 // ---------------------------------------------------------------------------
+// One wavefront factors a panel of m <= 64 RPL rows (lane L holds rows L + 64 i): the column's
+// norm and dot products are reduce-scattered within the wavefront and broadcast through LDS,
+// so a column step needs no workgroup barrier.  Same Householder steps, Gram matrix and T
+// factor as the four-wavefront form below (sums in a different order).
+template <int RPL>
+__device__ __forceinline__ void qr_panel_wave(int m, int jb, int k0, int N, AS_G double* M, AS_G double* Mp,
+                                              AS_G double* Tg) {
+    Lds& s = g_lds;
+    const int L = lane();
+    double* const P = s.panel;
+    double* const red = s.panel + PANEL_RED;
+    double x[RPL][NB];
+#pragma unroll
+    for (int i = 0; i < RPL; ++i)
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            const int r = L + 64 * i;
+            x[i][c] = (r < m && c < jb) ? P[c * m + r] : 0.0;
+        }
+    double* const tau = s.v1 + k0;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      if (j < jb) {
+        double part[NB];
+#pragma unroll
+        for (int q = 0; q < NB; ++q) part[q] = 0.0;
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) {
+            const int r = L + 64 * i;
+            if (r > j && r < m) {
+                const double xj = x[i][j];
+                part[0] += xj * xj;
+#pragma unroll
+                for (int q = 1; q < NB; ++q)
+                    if (j + q < NB) part[q] += xj * x[i][j + q];
+            }
+        }
+        if (L == j)
+#pragma unroll
+            for (int q = 0; q < NB; ++q) red[RED_ROWJ + q] = (j + q < NB) ? x[0][j + q] : 0.0;
+        const double wsum = wave_reduce16(part);
+        if ((L & 3) == 0) red[(L >> 2) & 15] = wsum;   // slot totals at the front of the scratch
+        __builtin_amdgcn_wave_barrier();
+        const double x0 = red[RED_ROWJ];
+        const double nrm = sqrt(x0 * x0 + red[0]);
+        double tk = 0.0, inv_v0 = 1.0, alpha = x0;
+        if (nrm != 0.0) {
+            alpha = x0 > 0 ? -nrm : nrm;
+            const double v0 = x0 - alpha;
+            tk = -v0 / alpha;
+            inv_v0 = 1.0 / v0;
+        }
+        if (L == 0) tau[j] = tk;
+        double wq[NB];
+#pragma unroll
+        for (int q = 1; q < NB; ++q)
+            wq[q] = (j + q < NB) ? tk * (red[RED_ROWJ + q] + inv_v0 * red[q]) : 0.0;
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) {
+            const int r = L + 64 * i;
+            if (r > j && r < m && tk != 0.0) {
+                const double v = x[i][j] * inv_v0;
+                x[i][j] = v;
+#pragma unroll
+                for (int q = 1; q < NB; ++q)
+                    if (j + q < NB) x[i][j + q] -= wq[q] * v;
+            }
+        }
+        if (L == j) {
+            x[0][j] = alpha;
+#pragma unroll
+            for (int q = 1; q < NB; ++q)
+                if (j + q < NB) x[0][j + q] -= wq[q];
+        }
+      }
+    }
+    // R to Mp, explicit V to M
+#pragma unroll
+    for (int i = 0; i < RPL; ++i) {
+        const int r = L + 64 * i;
+        if (r >= m) continue;
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            if (c < jb) {
+                const size_t o = (size_t)(k0 + c) * N + k0 + r;
+                Mp[o] = (r <= c) ? x[i][c] : 0.0;
+                M[o] = (r == c) ? 1.0 : (r > c ? x[i][c] : 0.0);
+            }
+        }
+    }
+    // Gram matrix G(l, j) = V(j, l) + sum_{r > j} V(r, l) V(r, j) (l < j), 8 groups of 16 pairs
+    double* const gram = red + RED_GRAM;
+#pragma unroll
+    for (int g = 0; g < 8; ++g) {
+        double part[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            part[q] = 0.0;
+            const int pidx = g * 16 + q;
+            int j = 1;
+            while ((j + 1) * j / 2 <= pidx) ++j;
+            const int l = pidx - j * (j - 1) / 2;
+            if (j < NB && j < jb) {
+#pragma unroll
+                for (int i = 0; i < RPL; ++i) {
+                    const int r = L + 64 * i;
+                    if (r < m && r > j) part[q] += x[i][l] * x[i][j];
+                    else if (r < m && r == j) part[q] += x[i][l];
+                }
+            }
+        }
+        const double wsum = wave_reduce16(part);
+        if ((L & 3) == 0) gram[4 * 128 + g * 16 + ((L >> 2) & 15)] = wsum;
+    }
+    __builtin_amdgcn_wave_barrier();
+    // T: lane i builds row i, T(i, j) = -tau_j sum_{l=i}^{j-1} T(i, l) G(l, j), T(j, j) = tau_j
+    double* const Tb = s.panel + PANEL_BLK;
+    if (L < NB) {
+        const int i = L;
+        double trow[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            trow[j] = 0.0;
+            if (j < jb && i == j) trow[j] = tau[j];
+            else if (j < jb && i < j) {
+                double acc = 0.0;
+#pragma unroll
+                for (int l = 0; l < NB; ++l)
+                    if (l >= i && l < j) acc += trow[l] * gram[4 * 128 + j * (j - 1) / 2 + l];
+                trow[j] = -tau[j] * acc;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            Tb[j * NB + i] = trow[j];
+            Tg[(size_t)(k0 / NB) * NB * NB + j * NB + i] = trow[j];
+        }
+    }
+}
+
 __device__ __noinline__ void qr_panel(int m, int jb, int k0, int N, double* M_, double* Mp_, double* Tg_, Lds&) {
     Lds& s = g_lds;
     m = uni(m); jb = uni(jb); k0 = uni(k0); N = uni(N);
     AS_G double* M = gptr(uni(M_));
     AS_G double* Mp = gptr(uni(Mp_));
     AS_G double* Tg = gptr(uni(Tg_));
+    if (m <= 256) {
+        if (wave() == 0) {
+            if (m <= 64) qr_panel_wave<1>(m, jb, k0, N, M, Mp, Tg);
+            else if (m <= 128) qr_panel_wave<2>(m, jb, k0, N, M, Mp, Tg);
+            else if (m <= 192) qr_panel_wave<3>(m, jb, k0, N, M, Mp, Tg);
+            else qr_panel_wave<4>(m, jb, k0, N, M, Mp, Tg);
+        }
+        __syncthreads();
+        return;
+    }
     const int t = tid(), w = wave();
     double* const P = s.panel;
     double* const red = s.panel + PANEL_RED;
