@@ -577,6 +577,133 @@ static Mat compute_equaltime_greens(const SvdStack& left, const SvdStack& right)
     mult_v_invd_u(Z1, dlmax, transpose(ul), G);
     return G;
 }
+// v d u (:100-113)
+static void mult_v_d_u(const Mat& v, const vector<double>& d, const Mat& u, Mat& z) {
+    const int n = v.n;
+    z = Mat(n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double zt = 0.0;
+            for (int k = 0; k < n; ++k) zt += v(j, k) * u(k, i) * d[k];
+            z(j, i) = zt;
+        }
+}
+// (1 + U S V^T)^-1 U S V^T with H = Sbi U^T + Ss V^T, G = H^-1 Ss V^T  (:159-179)
+static Mat compute_greens_b0(const Mat& U, const vector<double>& S, const Mat& V) {
+    const int n = U.n;
+    vector<double> Sbi(n), Ss(n);
+    for (int i = 0; i < n; ++i) {
+        if (S[i] > 1) { Sbi[i] = 1.0 / S[i]; Ss[i] = 1.0; }
+        else { Sbi[i] = 1.0; Ss[i] = S[i]; }
+    }
+    Mat H(n), R(n);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            H(i, j) = Sbi[i] * U(j, i) + Ss[i] * V(j, i);
+            R(i, j) = Ss[i] * V(j, i);
+        }
+    return fullpiv_qr_solve(H, R);
+}
+// The reference's time-displaced G(t,0), G(0,t) from the two stacks (:246-325).  Its
+// middle-case g0t inverts V_L and V_R by transposition, so it holds for SVD stacks.
+static void compute_dynamic_greens(const SvdStack& left, const SvdStack& right, Mat& gt0, Mat& g0t) {
+    const int n = left.dim;
+    if (left.empty()) {
+        gt0 = compute_greens_00_bb(right.MatrixV(), right.SingularValues(), right.MatrixU());
+        g0t = gt0;
+        for (int i = 0; i < n; ++i) g0t(i, i) -= 1.0;
+        return;
+    }
+    if (right.empty()) {
+        gt0 = compute_greens_b0(left.MatrixU(), left.SingularValues(), left.MatrixV());
+        g0t = compute_greens_00_bb(left.MatrixU(), left.SingularValues(), left.MatrixV());
+        for (double& x : g0t.a) x = -x;
+        return;
+    }
+    const Mat ul = left.MatrixU(); const vector<double> dl = left.SingularValues(); const Mat vl = left.MatrixV();
+    const Mat ur = right.MatrixU(); const vector<double> dr = right.SingularValues(); const Mat vr = right.MatrixV();
+    vector<double> dlmax, dlmin, drmax, drmin;
+    div_dvec_max_min(dl, dlmax, dlmin);
+    div_dvec_max_min(dr, drmax, drmin);
+    Mat A = mul(transpose(ul), ur), B = mul(transpose(vl), vr);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            A(i, j) = A(i, j) / (dlmax[i] * drmax[j]);
+            B(i, j) = B(i, j) * dlmin[i] * drmin[j];
+        }
+    Mat tmp(n);
+    for (size_t e = 0; e < tmp.a.size(); ++e) tmp.a[e] = A.a[e] + B.a[e];
+    Mat Z;
+    mult_v_invd_u(ur, drmax, inverse(tmp), Z);
+    mult_v_d_u(Z, dlmin, transpose(vl), gt0);
+    Mat X = mul(transpose(vr), vl), Y = mul(transpose(ur), ul);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            X(i, j) = X(i, j) / (drmax[i] * dlmax[j]);
+            Y(i, j) = Y(i, j) * drmin[i] * dlmin[j];
+        }
+    for (size_t e = 0; e < tmp.a.size(); ++e) tmp.a[e] = X.a[e] + Y.a[e];
+    Mat mvl = vl;
+    for (double& x : mvl.a) x = -x;
+    mult_v_invd_u(mvl, dlmax, inverse(tmp), Z);
+    mult_v_d_u(Z, drmin, transpose(ur), g0t);
+}
+// The same three Green's functions for stacks whose V is not orthogonal (the QR-based
+// UDV of the engine), from orthogonality of U alone.  With T = V^T:
+//   1 + U_L D_L T_L T_R^T D_R U_R^T = U_L D_Lmax (A + B) D_Rmax U_R^T,
+//   A = D_Lmax^-1 U_L^T U_R D_Rmax^-1,  B = D_Lmin T_L T_R^T D_Rmin,
+// and by G(0,t) = -B(beta,t) G(t,t) = -[(1 + B(t,0)^T B(beta,t)^T)^-1 B(beta,t)^T]^T the
+// matrix of G(0,t) is (A + B)^T, so one solve (A + B) [X1 X2] = [D_Lmax^-1 U_L^T, D_Lmin T_L]
+// gives G(t,t) = U_R D_Rmax^-1 X1, G(t,0) = U_R D_Rmax^-1 X2, G(0,t) = -T_R^T D_Rmin X1.
+// At t = beta (right stack empty): H = Sbi U^T + Ss V^T, H [X1 X2] = [Sbi U^T, Ss V^T],
+// G(t,t) = X1, G(t,0) = X2, G(0,t) = -X1.
+static void compute_dynamic_greens_udt(const SvdStack& left, const SvdStack& right, Mat& gtt, Mat& gt0, Mat& g0t) {
+    const int n = left.dim;
+    Mat H(n), R1(n), R2(n);
+    if (right.empty()) {
+        const Mat U = left.MatrixU(), V = left.MatrixV();
+        const vector<double> S = left.SingularValues();
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                const double sbi = S[i] > 1 ? 1.0 / S[i] : 1.0, ss = S[i] > 1 ? 1.0 : S[i];
+                H(i, j) = sbi * U(j, i) + ss * V(j, i);
+                R1(i, j) = sbi * U(j, i);
+                R2(i, j) = ss * V(j, i);
+            }
+        gtt = fullpiv_qr_solve(H, R1);
+        gt0 = fullpiv_qr_solve(H, R2);
+        g0t = gtt;
+        for (double& x : g0t.a) x = -x;
+        return;
+    }
+    const Mat ul = left.MatrixU(); const vector<double> dl = left.SingularValues(); const Mat vl = left.MatrixV();
+    const Mat ur = right.MatrixU(); const vector<double> dr = right.SingularValues(); const Mat vr = right.MatrixV();
+    vector<double> dlmax, dlmin, drmax, drmin;
+    div_dvec_max_min(dl, dlmax, dlmin);
+    div_dvec_max_min(dr, drmax, drmin);
+    const Mat A = mul(transpose(ul), ur), B = mul(transpose(vl), vr);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            H(i, j) = A(i, j) / (dlmax[i] * drmax[j]) + B(i, j) * dlmin[i] * drmin[j];
+            R1(i, j) = ul(j, i) / dlmax[i];
+            R2(i, j) = vl(j, i) * dlmin[i];
+        }
+    Mat LU = H; vector<int> piv; int ns;
+    lu_factor(LU, piv, ns);
+    lu_solve(LU, piv, R1);
+    lu_solve(LU, piv, R2);
+    gtt = Mat(n); gt0 = Mat(n); g0t = Mat(n);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            double a = 0.0, b = 0.0, c = 0.0;
+            for (int k = 0; k < n; ++k) {
+                a += ur(i, k) / drmax[k] * R1(k, j);
+                b += ur(i, k) / drmax[k] * R2(k, j);
+                c += vr(i, k) * drmin[k] * R1(k, j);
+            }
+            gtt(i, j) = a; gt0(i, j) = b; g0t(i, j) = -c;
+        }
+}
 
 // ----------------------------------------------------------------------------
 // Model (src/model/{model_base,repulsive_hubbard,attractive_hubbard}.cpp) and
@@ -759,6 +886,14 @@ struct Engine {
     SvdStack lu_, ld_, ru_, rd_;
     vector<unsigned char> accepts;  // last sweep, (eff_t, i) at eff_t*N+i
     double acc[kNumObs] = {0}, cnt[kNumObs] = {0};
+    // time-displaced path: G(t,0), G(0,t) and their per-slice records (vec_green_t0/0t)
+    Mat gt0u, gt0d, g0tu, g0td;
+    vector<Mat> vgt0u, vgt0d, vg0tu, vg0td;
+    bool dyn_general = false;   // use the orthogonal-U form of the recompute even with SVD stacks
+    // dynamic accumulators: G(k,t) over every k-star (nk x L, (k,t) at k + nk t), D(t), DSS(t)
+    // per slice; rho_s; sign
+    vector<double> dgk, ddos, ddss;
+    double drho = 0.0, dsign = 0.0, dcount = 0.0;
 
     void setup(int side, int time_size, double beta_, int pace_, int kind, bool cb, double t, double U, double mu,
                int momentum_index, int factorization) {
@@ -769,6 +904,8 @@ struct Engine {
         momentum = momentum_index < 0 ? lat.mpoint : momentum_index;
         vgu.assign(L, Mat(N)); vgd.assign(L, Mat(N)); vsign.assign(L, 0.0);
         accepts.assign((size_t)L * N, 0);
+        vgt0u.assign(L, Mat(N)); vgt0d.assign(L, Mat(N)); vg0tu.assign(L, Mat(N)); vg0td.assign(L, Mat(N));
+        dgk.assign((size_t)lat.nk * L, 0.0); ddos.assign(L, 0.0); ddss.assign(L, 0.0);
     }
 
     void initial_svd_stacks() {                                    // :109-134
@@ -863,6 +1000,107 @@ struct Engine {
         gu = compute_equaltime_greens(lu_, ru_);
         gd = compute_equaltime_greens(ld_, rd_);
         vgu[L - 1] = gu; vgd[L - 1] = gd;
+    }
+
+    // DqmcWalker::sweep_for_dynamic_greens (src/dqmc_walker.cpp:424-524): from t = 0 (left stack
+    // empty, right stack full, as a backward sweep leaves it) to beta without field updates,
+    // wrapping G(t,t), G(t,0) = B(t) G(t-1,0) and G(0,t) = G(0,t-1) B(t)^-1, recomputed from the
+    // stacks at every stabilization (the wrap errors of G(t,0) and G(0,t) are collected).
+    void fresh_dynamic(const SvdStack& l, const SvdStack& r, Mat& gtt, Mat& gt0, Mat& g0t) {
+        if (fact == kSVD && !dyn_general) {
+            gtt = compute_equaltime_greens(l, r);
+            compute_dynamic_greens(l, r, gt0, g0t);
+        } else {
+            compute_dynamic_greens_udt(l, r, gtt, gt0, g0t);
+        }
+    }
+    void sweep_dynamic() {
+        gt0u = gu; gt0d = gd;
+        g0tu = gu; g0td = gd;
+        for (int i = 0; i < N; ++i) { g0tu(i, i) -= 1.0; g0td(i, i) -= 1.0; }
+        Mat tu = Mat::Identity(N), td = Mat::Identity(N);
+        for (int t = 1; t <= L; ++t) {
+            wrap_0_to_beta(t - 1);
+            vgu[t - 1] = gu; vgd[t - 1] = gd; vsign[t - 1] = sign;
+            model.mult_B_from_left(gt0u, t, +1); model.mult_B_from_left(gt0d, t, -1);
+            vgt0u[t - 1] = gt0u; vgt0d[t - 1] = gt0d;
+            model.mult_invB_from_right(g0tu, t, +1); model.mult_invB_from_right(g0td, t, -1);
+            vg0tu[t - 1] = g0tu; vg0td[t - 1] = g0td;
+            model.mult_B_from_left(tu, t, +1);
+            model.mult_B_from_left(td, t, -1);
+            if (t % pace == 0 || t == L) {
+                stabilize_swap(ru_, rd_, lu_, ld_, tu, td);
+                Mat fu, fd, au, ad, bu, bd;
+                fresh_dynamic(lu_, ru_, fu, au, bu);
+                fresh_dynamic(ld_, rd_, fd, ad, bd);
+                wrap_error = std::max(wrap_error, std::max(matrix_compare_error(au, gt0u), matrix_compare_error(ad, gt0d)));
+                wrap_error = std::max(wrap_error, std::max(matrix_compare_error(bu, g0tu), matrix_compare_error(bd, g0td)));
+                gu = fu; gd = fd; gt0u = au; gt0d = ad; g0tu = bu; g0td = bd;
+                vgu[t - 1] = gu; vgd[t - 1] = gd;
+                vgt0u[t - 1] = gt0u; vgt0d[t - 1] = gt0d; vg0tu[t - 1] = g0tu; vg0td[t - 1] = g0td;
+                tu = Mat::Identity(N); td = Mat::Identity(N);
+            }
+        }
+    }
+    // MeasureHandler::dynamic_measure over the stored slices (src/measure/measure_methods.cpp:264-461):
+    // G(k,t) for the momentum list of every k-star (the reference's "KstarsAll"), density of
+    // states D(t), superfluid stiffness rho_s, dynamic spin susceptibility, sign.  Slot t = 0
+    // takes the slice-(L-1) records, as there.
+    void dynamic_measure() {
+        const double Nd = (double)N, cs = sign;
+        const Mat& g00u = vgu[L - 1]; const Mat& g00d = vgd[L - 1];
+        for (int t = 0; t < L; ++t) {
+            const int e = (t == 0) ? L - 1 : t - 1;
+            // greens_functions and density_of_states (:278-325): the spin average of G(t,0), G(0,0) at t = 0
+            const Mat& au = (t == 0) ? vgu[L - 1] : vgt0u[t - 1];
+            const Mat& ad = (t == 0) ? vgd[L - 1] : vgt0d[t - 1];
+            double tr = 0.0;
+            for (int i = 0; i < N; ++i) {
+                tr += 0.5 * (au(i, i) + ad(i, i));
+                for (int j = 0; j < N; ++j)
+                    for (int k = 0; k < lat.nk; ++k)
+                        dgk[(size_t)t * lat.nk + k] += cs * 0.5 * (au(j, i) + ad(j, i)) / Nd *
+                                                       lat.FourierFactor(lat.disp[(size_t)i * N + j], k);
+            }
+            ddos[t] += cs * tr / Nd;
+            // superfluid_stiffness (:335-405) and dynamic_spin_susceptibility (:413-461)
+            const Mat& gttu = vgu[e]; const Mat& gttd = vgd[e];
+            const Mat& gt0u_ = vgt0u[e]; const Mat& gt0d_ = vgt0d[e];
+            const Mat& g0tu_ = vg0tu[e]; const Mat& g0td_ = vg0td[e];
+            double rho = 0.0;
+            for (int i = 0; i < N; ++i) {
+                const int ipx = lat.nn[i * 4 + 0];
+                for (int j = 0; j < N; ++j) {
+                    const int jpx = lat.nn[j * 4 + 0];
+                    const int dsp = lat.disp[(size_t)i * N + j];
+                    const int rx = dsp % lat.L, ry = dsp / lat.L;
+                    const double ff = lat.FourierFactor(rx, 1) - lat.FourierFactor(ry, 1);
+                    rho += model.t * model.t * cs * ff * (
+                        -(gttu(j, jpx) - gttu(jpx, j) + gttd(j, jpx) - gttd(jpx, j)) *
+                         (g00u(i, ipx) - g00u(ipx, i) + g00d(i, ipx) - g00d(ipx, i))
+                        - g0tu_(ipx, jpx) * gt0u_(j, i) - g0td_(ipx, jpx) * gt0d_(j, i)
+                        + g0tu_(i, jpx) * gt0u_(j, ipx) + g0td_(i, jpx) * gt0d_(j, ipx)
+                        + g0tu_(ipx, j) * gt0u_(jpx, i) + g0td_(ipx, j) * gt0d_(jpx, i)
+                        - g0tu_(i, j) * gt0u_(jpx, ipx) - g0td_(i, j) * gt0d_(jpx, ipx));
+                }
+            }
+            drho += 0.25 * rho / (Nd * Nd);
+            double dss = 0.0;
+            for (int i = 0; i < N; ++i) {
+                const double gcttu = 1.0 - gttu(i, i), gcttd = 1.0 - gttd(i, i);
+                const double gc00u = 1.0 - g00u(i, i), gc00d = 1.0 - g00d(i, i);
+                dss += 0.25 * cs / Nd * (gcttu * gc00u - g0tu_(i, i) * gt0u_(i, i) + gcttd * gc00d - g0td_(i, i) * gt0d_(i, i)
+                                         - gcttu * gc00d - gc00u * gcttd);
+            }
+            ddss[t] += dss;
+        }
+        dsign += cs;
+        dcount += 1.0;
+    }
+    void clear_dynamic() {
+        std::fill(dgk.begin(), dgk.end(), 0.0); std::fill(ddos.begin(), ddos.end(), 0.0);
+        std::fill(ddss.begin(), ddss.end(), 0.0);
+        drho = dsign = dcount = 0.0;
     }
 
     // MeasureHandler::equaltime_measure over the stored slices
@@ -970,6 +1208,26 @@ void rc_get_measure(void* h, double* sums, double* counts) {
     for (int o = 0; o < kNumObs; ++o) { sums[o] = e->acc[o]; counts[o] = e->cnt[o]; }
 }
 void rc_clear_measure(void* h) { ((Engine*)h)->clear_measure(); }
+// time-displaced path
+void rc_sweep_dynamic(void* h) { ((Engine*)h)->sweep_dynamic(); }
+void rc_set_dynamic_formula(void* h, int general) { ((Engine*)h)->dyn_general = general != 0; }
+// which: 0 G(t,0), 1 G(0,t)
+void rc_get_dynamic_green(void* h, int slot, int spin, int which, double* out) {
+    Engine* e = (Engine*)h;
+    const Mat& g = which == 0 ? (spin > 0 ? e->vgt0u[slot] : e->vgt0d[slot]) : (spin > 0 ? e->vg0tu[slot] : e->vg0td[slot]);
+    std::memcpy(out, g.a.data(), sizeof(double) * g.a.size());
+}
+void rc_dynamic_measure(void* h) { ((Engine*)h)->dynamic_measure(); }
+// out: G(k,t)[nk x L, (k,t) at k + nk t], D(t)[L], DSS(t)[L], rho_s, sign sum, count
+void rc_get_dynamic_measure(void* h, double* out) {
+    Engine* e = (Engine*)h;
+    const int L = e->L, nk = e->lat.nk;
+    for (size_t q = 0; q < e->dgk.size(); ++q) out[q] = e->dgk[q];
+    out += (size_t)nk * L;
+    for (int t = 0; t < L; ++t) { out[t] = e->ddos[t]; out[L + t] = e->ddss[t]; }
+    out[2 * L] = e->drho; out[2 * L + 1] = e->dsign; out[2 * L + 2] = e->dcount;
+}
+void rc_clear_dynamic(void* h) { ((Engine*)h)->clear_dynamic(); }
 double rc_alpha(void* h) { return ((Engine*)h)->model.alpha; }
 // which: 0 expK, 1 invExpK, 2 transExpK
 void rc_get_expK(void* h, int which, double* out) {

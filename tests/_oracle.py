@@ -61,6 +61,11 @@ def lib():
         _lib.rc_greens_00_bb.argtypes = [C.c_int] + [C.c_void_p] * 4
         _lib.rc_inverse.argtypes = [C.c_int, C.c_void_p, C.c_void_p]
         _lib.rc_fullpiv_solve.argtypes = [C.c_int] + [C.c_void_p] * 3
+        for name in ["rc_sweep_dynamic", "rc_dynamic_measure", "rc_clear_dynamic"]:
+            getattr(_lib, name).argtypes = vp
+        _lib.rc_set_dynamic_formula.argtypes = [C.c_void_p, C.c_int]
+        _lib.rc_get_dynamic_green.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
+        _lib.rc_get_dynamic_measure.argtypes = [C.c_void_p, C.c_void_p]
     return _lib
 
 
@@ -245,6 +250,40 @@ class Chain:
 
     def flip(self, t, i):
         lib().rc_flip(self.h, t, i)
+
+    # time-displaced path (DqmcWalker::sweep_for_dynamic_greens, MeasureHandler::dynamic_measure)
+    @property
+    def nk(self):
+        side = int(round(self.N ** 0.5))
+        return (side // 2 + 1) * (side // 2 + 2) // 2
+
+    def sweep_dynamic(self):
+        lib().rc_sweep_dynamic(self.h)
+
+    def set_dynamic_formula(self, general):
+        """False: the reference's SVD-stack formulas; True: the orthogonal-U (general UDT) form."""
+        lib().rc_set_dynamic_formula(self.h, int(bool(general)))
+
+    def dynamic_green(self, slot, spin, which):
+        """which 0: G(t,0), 1: G(0,t) at slot t-1 of the last dynamic sweep."""
+        g = np.zeros((self.N, self.N), order="F")
+        lib().rc_get_dynamic_green(self.h, slot, spin, which, _p(g))
+        return g
+
+    def dynamic_measure(self):
+        lib().rc_dynamic_measure(self.h)
+
+    def dynamic_accum(self):
+        """Sums of the dynamic observables: gk (nk x L), dos (L), dss (L), rho, sign, count."""
+        nk, L = self.nk, self.L
+        out = np.zeros(nk * L + 2 * L + 3)
+        lib().rc_get_dynamic_measure(self.h, _p(out))
+        o = nk * L
+        return dict(gk=out[:o].reshape(L, nk).T.copy(), dos=out[o:o + L].copy(), dss=out[o + L:o + 2 * L].copy(),
+                    rho=out[o + 2 * L], sign=out[o + 2 * L + 1], count=out[o + 2 * L + 2])
+
+    def clear_dynamic(self):
+        lib().rc_clear_dynamic(self.h)
 
 
 def bin_stats(bins):
