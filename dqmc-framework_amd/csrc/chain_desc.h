@@ -14,6 +14,8 @@ enum {
     DQMC_FLAG_RECORD = 2,    // keep G, sign per slice and accept bits of the last sweep
     DQMC_FLAG_FROZEN = 4,    // propagate and measure without field updates (fixed configuration)
     DQMC_FLAG_PROFILE = 8,   // accumulate per-phase device time into ChainState::phase
+    DQMC_FLAG_DYNAMIC = 16,  // an upward sweep is the time-displaced one: no field updates, G(t,0) and
+                             // G(0,t) propagated and recomputed with G(t,t), dynamic observables
 };
 
 // phases timed with DQMC_FLAG_PROFILE (100 MHz device wall clock ticks)
@@ -26,6 +28,12 @@ enum { DQMC_PH_WRAP = 0, DQMC_PH_METROPOLIS, DQMC_PH_PRODUCT, DQMC_PH_PUSH, DQMC
 // sign, filling, double occupancy, kinetic energy, momentum distribution,
 // local spin correlation, SDW structure factor, CDW structure factor, s-wave pairing
 #define DQMC_NUM_OBS 9
+
+// dynamic accumulators of a chain (L slices, N sites), in this order:
+// gr[L][N]  sign x sum over site pairs (i, j = i + r) of the spin-averaged G(t,0)(j,i), per
+//           displacement r (slice entry 0 holds G(0,0)); G(k,t) and D(t) are linear in it
+// dss[L]    dynamic spin susceptibility sums, rho (superfluid stiffness), sign, count
+#define DQMC_DYN_ACC_SIZE(L, N) ((size_t)(L) * (N) + (L) + 3)
 
 struct ChainState {
     uint64_t rng;        // minstd_rand0 state
@@ -64,7 +72,14 @@ struct ChainDesc {
     int8_t* field;       // L x N, (t,i) at t*N+i
     ChainState* st;
     double* acc;         // DQMC_NUM_OBS sums
+    double* Gt0;         // nspin x N*N   G(t,0) of the time-displaced sweep
+    double* G0t;         // nspin x N*N   G(0,t)
+    double* dacc;        // DQMC_DYN_ACC_SIZE(L, N) dynamic sums
+    double* dscr;        // L x N + N: per-slice spin-density differences and the summed current vector
+    const double* rhoq;  // N: cos(kx rx) - cos(kx ry) of displacement d = rx + side ry (stiffness factor)
     double* recG;        // L x nspin x N*N (RECORD)
     double* recSign;     // L (RECORD)
     uint8_t* recAcc;     // L x N (RECORD)
+    double* recGt0;      // L x nspin x N*N (RECORD): G(t,0) per slot of the last time-displaced sweep
+    double* recG0t;      // L x nspin x N*N (RECORD): G(0,t)
 };

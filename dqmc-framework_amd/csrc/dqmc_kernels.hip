@@ -2392,6 +2392,229 @@ __device__ __noinline__ void stabilize_green(Desc d, int lslot, int rslot, bool 
     wg_copy(d.G, d.Gw, (int)(n2 * d.nspin));
 }
 
+// ---------------------------------------------------------------------------
+// Time-displaced Green's functions (src/dqmc_walker.cpp:424-524).  With the left
+// stack B(t,0) = U_L D_L T_L and the right stack B(beta,t)^T = U_R D_R T_R (U
+// orthogonal, T = V^T of any conditioning) and D = Dmax Dmin split at 1:
+//   1 + B(t,0) B(beta,t) = U_L D_Lmax H D_Rmax U_R^T,
+//   H = D_Lmax^-1 U_L^T U_R D_Rmax^-1 + D_Lmin T_L T_R^T D_Rmin,
+// one LU of H with 2N right-hand sides H [X1 X2] = [D_Lmax^-1 U_L^T, D_Lmin T_L] gives
+//   G(t,t) = U_R D_Rmax^-1 X1,  G(t,0) = G(t,t) B(t,0) = U_R D_Rmax^-1 X2,
+//   G(0,t) = -B(beta,t) G(t,t) = -T_R^T D_Rmin X1.
+// At t = beta (right stack empty), H = Db U^T + Ds T with Db = 1/max(D,1), Ds =
+// min(D,1): H [X1 X2] = [Db U^T, Ds T] gives G(beta,beta) = X1, G(beta,0) = X2
+// and G(0,beta) = -X1 (include/utils/numerical_stable.hpp:159-179, :246-325 with
+// V^T written as T).  The wrap errors of G(t,0) and G(0,t) are collected
+// (src/dqmc_walker.cpp:486-492); the fresh G(t,t) replaces the wrapped one.
+// ---------------------------------------------------------------------------
+__device__ __noinline__ void dynamic_fresh(Desc d, int lslot, int rslot, ChainState& st, Lds&) {
+    Lds& s = g_lds;
+    const int N = d.N;
+    const size_t n2 = (size_t)N * N;
+    double* W0 = d.W;
+    double* W1 = d.W + n2;
+    double* W2 = d.W + 2 * n2;
+    double err = 0.0;
+    for (int sp = 0; sp < d.nspin; ++sp) {
+        const double* UL = slotU(d, sp, lslot); const double* TL = slotT(d, sp, lslot); const double* DL = slotD(d, sp, lslot);
+        double* G = d.G + sp * n2;
+        double* Gw = d.Gw + sp * n2;
+        if (rslot >= 0) {
+            const double* UR = slotU(d, sp, rslot); const double* TR = slotT(d, sp, rslot); const double* DR = slotD(d, sp, rslot);
+            for (int i = tid(); i < N; i += NT) {
+                const double dl = gptr(DL)[i], dr = gptr(DR)[i];
+                s.v0[i] = dl >= 1.0 ? 1.0 / dl : 1.0; s.v1[i] = dl >= 1.0 ? 1.0 : dl;
+                s.v2[i] = dr >= 1.0 ? 1.0 / dr : 1.0; s.v3[i] = dr >= 1.0 ? 1.0 : dr;
+            }
+            __syncthreads();
+            gemm64(N, N, N, UL, N, true, UR, N, false, W0, N, s.v0, nullptr, s.v2, 0.0, s);
+            gemm64(N, N, N, TL, N, false, TR, N, true, W0, N, s.v1, nullptr, s.v3, 1.0, s);
+            wg_transpose_scaled(W1, nullptr, UL, nullptr, N, s.v0, nullptr, s);
+        } else {
+            for (int i = tid(); i < N; i += NT) {
+                const double x = gptr(DL)[i];
+                s.v0[i] = x > 1.0 ? 1.0 / x : 1.0;
+                s.v1[i] = x > 1.0 ? 1.0 : x;
+            }
+            __syncthreads();
+            wg_transpose_scaled(W1, W0, UL, TL, N, s.v0, s.v1, s);
+        }
+        {   // W2 = D_Lmin T_L (Ds T at t = beta): the second block of right-hand sides
+            const AS_G double* src = gptr(TL);
+            AS_G double* dst = gptr(W2);
+            const AS_L double* v1 = (const AS_L double*)s.v1;
+            wg_stream((int)n2, [&](int e) { return src[e]; }, [&](int e, double v) { dst[e] = v1[e % N] * v; });
+            __syncthreads();
+        }
+        wg_lu_solve(W0, W1, N, 2 * N, s);
+        const AS_G double* x1 = gptr(W1);
+        const AS_G double* x2 = gptr(W2);
+        const AS_G double* f0t = x1;   // fresh G(0,t) = -f0t
+        const AS_G double* ft0 = x2;   // fresh G(t,0)
+        if (rslot >= 0) {
+            const double* UR = slotU(d, sp, rslot); const double* TR = slotT(d, sp, rslot);
+            gemm64(N, N, N, UR, N, false, W2, N, false, Gw, N, nullptr, s.v2, nullptr, 0.0, s);
+            gemm64(N, N, N, TR, N, true, W1, N, false, W0, N, nullptr, s.v3, nullptr, 0.0, s);
+            gemm64(N, N, N, UR, N, false, W1, N, false, G, N, nullptr, s.v2, nullptr, 0.0, s);
+            ft0 = gptr(Gw);
+            f0t = gptr(W0);
+        } else {
+            wg_copy(G, W1, (int)n2);
+        }
+        AS_G double* gt0 = gptr(d.Gt0 + sp * n2);
+        AS_G double* g0t = gptr(d.G0t + sp * n2);
+        wg_stream2((int)n2, [&](int e) { return ft0[e]; },
+                   [&](int e, double v) { err = fmax(err, fabs(v - gt0[e])); gt0[e] = v; },
+                   [&](int e) { return f0t[e]; },
+                   [&](int e, double v) { err = fmax(err, fabs(-v - g0t[e])); g0t[e] = -v; });
+        __syncthreads();
+    }
+    err = wg_max(err, s);
+    if (tid() == 0 && err > st.wrap_error) st.wrap_error = err;
+}
+
+// ---------------------------------------------------------------------------
+// Dynamic measurements (src/measure/measure_methods.cpp:264-461), accumulated
+// while the time-displaced sweep runs.  Entry t >= 1 of the reference's loop
+// reads slot t-1, i.e. the matrices current at slice t; entry 0 reads slot L-1
+// (slice L, measured after the last slice).  Terms that need G(0,0) = G(L-1,L-1)
+// (the uncorrelated parts of the spin susceptibility and of rho_s) are kept as
+// per-slice vectors and closed by dyn_finalize once slot L-1 exists:
+//   DSS(t) = cs/(4N) sum_i [d_t(i) d_0(i) - sum_spin G(0,t)_ii G(t,0)_ii],
+//     d_t(i) = G(t,t)_dn,ii - G(t,t)_up,ii,
+//   rho_s  = t^2 cs/(4N^2) [-sum_ij f(i,j) A(j) A_0(i) + sum_t C_t],
+//     A(j) = sum_t sum_spin (G(t,t)(j,j+x) - G(t,t)(j+x,j)), A_0 its slice-L term,
+//     C_t the correlated four-term sum of the reference, f = cos(kx rx) - cos(kx ry).
+// G(k,t) and D(t) are linear in the displacement sums gr[t][r] (the host applies
+// the Fourier factors of the momenta it is asked for).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int site_shift(int i, int r, int side) {
+    const int x = (i % side + r % side) % side, y = (i / side + r / side) % side;
+    return x + side * y;
+}
+
+__device__ __noinline__ void dyn_measure_entry(Desc d, int t, double cs, Lds&) {
+    Lds& s = g_lds;
+    const int N = d.N, side = d.side, L = d.L;
+    const size_t n2 = (size_t)N * N;
+    const int ns = d.nspin;
+    const double spin_w = ns == 2 ? 1.0 : 2.0;   // attractive chains carry one spin: G_dn = G_up
+    AS_G double* acc = gptr(d.dacc);
+    AS_G double* delta = gptr(d.dscr + (size_t)t * N);
+    AS_G double* abar = gptr(d.dscr + (size_t)L * N);
+    const AS_G double* gtt = gptr(d.G);
+    const AS_G double* gt0 = gptr(d.Gt0);
+    double v[2] = {0.0, 0.0};   // sum_i sum_spin G(0,t)_ii G(t,0)_ii, correlated stiffness sum
+    for (int i = tid(); i < N; i += NT) {
+        const int x = i % side, y = i / side;
+        const int ipx = (x + 1) % side + side * y;
+        const double du = gtt[(size_t)i * N + i];
+        const double dd = ns == 2 ? gtt[n2 + (size_t)i * N + i] : du;
+        delta[i] = dd - du;
+        double a = 0.0, c = 0.0;
+        for (int sp = 0; sp < ns; ++sp) {
+            const size_t o = sp * n2;
+            a += gtt[o + (size_t)ipx * N + i] - gtt[o + (size_t)i * N + ipx];
+            c += gptr(d.G0t)[o + (size_t)i * N + i] * gt0[o + (size_t)i * N + i];
+        }
+        abar[i] += spin_w * a;
+        v[0] += spin_w * c;
+    }
+    // displacement sums of the spin-averaged G(t,0) (G(0,0) at entry 0): thread r owns gr[t][r]
+    for (int r = tid(); r < N; r += NT) {
+        double g = 0.0;
+        const AS_G double* src = t == 0 ? gtt : gt0;
+        for (int i = 0; i < N; ++i) {
+            const int j = site_shift(i, r, side);
+            g += src[(size_t)i * N + j];
+            if (ns == 2) g += src[n2 + (size_t)i * N + j];
+        }
+        acc[(size_t)t * N + r] += cs * (ns == 2 ? 0.5 * g : g);
+    }
+    // G(0,t)^T of each spin into W, so that the stiffness pairs read both matrices along columns
+    for (int sp = 0; sp < ns; ++sp)
+        wg_transpose_scaled(d.W + sp * n2, nullptr, d.G0t + sp * n2, nullptr, N, nullptr, nullptr, s);
+    const AS_G double* g0tT = gptr(d.W);
+    const AS_G double* rq = gptr(d.rhoq);
+    for (int p = tid(); p < N * N; p += NT) {
+        const int j = p % N, i = p / N;
+        const int xi = i % side, yi = i / side, xj = j % side, yj = j / side;
+        const int ipx = (xi + 1) % side + side * yi, jpx = (xj + 1) % side + side * yj;
+        const int dsp = (xj - xi + side) % side + side * ((yj - yi + side) % side);
+        double w = 0.0;
+        for (int sp = 0; sp < ns; ++sp) {
+            const size_t o = sp * n2;
+            // g0t(a,b) = g0tT[a*N + b], gt0(a,b) = gt0[b*N + a]
+            w += -g0tT[o + (size_t)ipx * N + jpx] * gt0[o + (size_t)i * N + j]
+                 + g0tT[o + (size_t)i * N + jpx] * gt0[o + (size_t)ipx * N + j]
+                 + g0tT[o + (size_t)ipx * N + j] * gt0[o + (size_t)i * N + jpx]
+                 - g0tT[o + (size_t)i * N + j] * gt0[o + (size_t)ipx * N + jpx];
+        }
+        v[1] += spin_w * rq[dsp] * w;
+    }
+    wg_sum8(v, 2, s);
+    if (tid() == 0) {
+        const double Nd = (double)N;
+        acc[(size_t)L * N + t] -= 0.25 * cs / Nd * v[0];
+        acc[(size_t)L * N + L] += 0.25 * d.hop_t * d.hop_t * cs * v[1] / (Nd * Nd);
+    }
+    __syncthreads();
+}
+
+// the G(0,0)-dependent parts, once entry 0 (slot L-1) is measured; then sign and count
+__device__ __noinline__ void dyn_finalize(Desc d, double cs, Lds&) {
+    Lds& s = g_lds;
+    const int N = d.N, side = d.side, L = d.L;
+    const size_t n2 = (size_t)N * N;
+    const int ns = d.nspin;
+    const double spin_w = ns == 2 ? 1.0 : 2.0;
+    AS_G double* acc = gptr(d.dacc);
+    const AS_G double* delta = gptr(d.dscr);
+    const AS_G double* gtt = gptr(d.G);
+    // A_0(i) into v0, the summed current vector into v1, d_0 into v2
+    for (int i = tid(); i < N; i += NT) {
+        const int x = i % side, y = i / side;
+        const int ipx = (x + 1) % side + side * y;
+        double a = 0.0;
+        for (int sp = 0; sp < ns; ++sp) a += gtt[sp * n2 + (size_t)ipx * N + i] - gtt[sp * n2 + (size_t)i * N + ipx];
+        s.v0[i] = spin_w * a;
+        s.v1[i] = gptr(d.dscr)[(size_t)L * N + i];
+        s.v2[i] = delta[i];
+    }
+    __syncthreads();
+    double v[1] = {0.0};
+    for (int i = tid(); i < N; i += NT) {
+        const int xi = i % side, yi = i / side;
+        double f = 0.0;
+        for (int j = 0; j < N; ++j) {
+            const int xj = j % side, yj = j / side;
+            const int dsp = (xj - xi + side) % side + side * ((yj - yi + side) % side);
+            f += gptr(d.rhoq)[dsp] * s.v1[j];
+        }
+        v[0] -= s.v0[i] * f;
+    }
+    // uncorrelated spin susceptibility, one slice entry per thread
+    for (int t = tid(); t < L; t += NT) {
+        double u = 0.0;
+        for (int i = 0; i < N; ++i) u += delta[(size_t)t * N + i] * s.v2[i];
+        acc[(size_t)L * N + t] += 0.25 * cs / (double)N * u;
+    }
+    wg_sum8(v, 1, s);
+    if (tid() == 0) {
+        const double Nd = (double)N;
+        acc[(size_t)L * N + L] += 0.25 * d.hop_t * d.hop_t * cs * v[0] / (Nd * Nd);
+        acc[(size_t)L * N + L + 1] += cs;
+        acc[(size_t)L * N + L + 2] += 1.0;
+    }
+    __syncthreads();
+}
+
+__device__ void record_dynamic(Desc d, int slot) {
+    const size_t n = (size_t)d.nspin * d.N * d.N;
+    wg_copy(d.recGt0 + slot * n, d.Gt0, (int)n);
+    wg_copy(d.recG0t + slot * n, d.G0t, (int)n);
+}
+
 // per-phase device time (thread 0 reads the 100 MHz wall clock at barriers)
 struct PhaseClock {
     bool on;
@@ -2496,6 +2719,63 @@ __device__ void sweep_down(Desc d, ChainState& st, uint64_t& rng, double& sign, 
     st.pos = 0;
 }
 
+// DqmcWalker::sweep_for_dynamic_greens (src/dqmc_walker.cpp:424-524): from slice 0 (left stack
+// empty) to beta with the fields fixed.  G(t,t) is wrapped as in the upward sweep, G(t,0) <- B(t)
+// G(t-1,0) and G(0,t) <- G(0,t-1) B(t)^-1 from G(0,0) and G(0,0) - 1, the slice product
+// accumulates for the left stack, and at every stabilization the three are recomputed from the
+// stacks (dynamic_fresh).  No equal-time measurement runs in this direction (src/dqmc.cpp:54-57).
+__device__ void sweep_dynamic(Desc d, ChainState& st, double sign, int flags, Lds&) {
+    Lds& s = g_lds;
+    const int N = d.N, L = d.L;
+    const size_t n2 = (size_t)N * N;
+    const bool rec = flags & DQMC_FLAG_RECORD, meas = flags & DQMC_FLAG_MEASURE;
+    PhaseClock clk(flags & DQMC_FLAG_PROFILE);
+    for (int sp = 0; sp < d.nspin; ++sp) {
+        const AS_G double* g = gptr(d.G + sp * n2);
+        AS_G double* a = gptr(d.Gt0 + sp * n2);
+        AS_G double* b = gptr(d.G0t + sp * n2);
+        wg_stream((int)n2, [&](int e) { return g[e]; }, [&](int e, double v) {
+            a[e] = v;
+            b[e] = (e % N == e / N) ? v - 1.0 : v;
+        });
+        wg_identity(d.Tmp + sp * n2, N);
+    }
+    if (meas)
+        for (int i = tid(); i < N; i += NT) gptr(d.dscr)[(size_t)L * N + i] = 0.0;
+    __syncthreads();
+    for (int t = 1; t <= L; ++t) {
+        const int e = t - 1;
+        load_field_row(d, e, s);
+        for (int sp = 0; sp < d.nspin; ++sp) wg_wrap(d, sp, true, d.G + sp * n2, d.Gw + sp * n2, s);
+        clk.lap(st, DQMC_PH_WRAP);
+        for (int sp = 0; sp < d.nspin; ++sp) {
+            wg_bmul(d, 0, sp, d.Gt0 + sp * n2, d.Gw + sp * n2, s);
+            wg_bmul(d, 3, sp, d.G0t + sp * n2, d.Gw + sp * n2, s);
+            wg_bmul(d, 0, sp, d.Tmp + sp * n2, d.Gw + sp * n2, s);
+        }
+        clk.lap(st, DQMC_PH_PRODUCT);
+        if (t % d.pace == 0 || t == L) {
+            st.n_right -= 1;
+            const int slot = st.n_right;
+            for (int sp = 0; sp < d.nspin; ++sp)
+                wg_push(d, sp, slot, slot + 1 < d.n_stack ? slot + 1 : -1, d.Tmp + sp * n2, s);
+            clk.lap(st, DQMC_PH_PUSH);
+            dynamic_fresh(d, slot, slot > 0 ? slot - 1 : -1, st, s);
+            for (int sp = 0; sp < d.nspin; ++sp) wg_identity(d.Tmp + sp * n2, N);
+            clk.lap(st, DQMC_PH_FRESH);
+        }
+        if (rec) {
+            record_slot(d, e, sign, true);
+            record_dynamic(d, e);
+        }
+        if (meas) dyn_measure_entry(d, t == L ? 0 : t, sign, s);
+        clk.lap(st, DQMC_PH_MEASURE);
+    }
+    if (meas) dyn_finalize(d, sign, s);
+    clk.lap(st, DQMC_PH_MEASURE);
+    st.pos = L;
+}
+
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) k_init(const ChainDesc* __restrict__ descs) {
     Lds& s = g_lds;
     Desc d = ((const AS_C ChainDesc*)descs)[blockIdx.x];
@@ -2551,8 +2831,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1)))
     }
     __syncthreads();
     for (int k = 0; k < n_sweeps; ++k) {
-        if (st.pos == 0) sweep_up(d, st, rng, sign, flags, nacc, s);
-        else sweep_down(d, st, rng, sign, flags, nacc, s);
+        if (st.pos == 0) {
+            if (flags & DQMC_FLAG_DYNAMIC) sweep_dynamic(d, st, sign, flags, s);
+            else sweep_up(d, st, rng, sign, flags, nacc, s);
+        } else sweep_down(d, st, rng, sign, flags, nacc, s);
     }
     if (tid() == 0) {
         st.rng = rng;

@@ -19,6 +19,11 @@ extern "C" hipError_t dqmc_launch_gemm(int variant, int M, int Nc, int K, const 
                                        const double* B, int ldb, int tB, double* C, int ldc, const double* rs,
                                        const double* ms, const double* cs, double beta, hipStream_t stream);
 
+static_assert(DQMC_SWEEP_MEASURE == DQMC_FLAG_MEASURE && DQMC_SWEEP_RECORD == DQMC_FLAG_RECORD &&
+                  DQMC_SWEEP_FROZEN == DQMC_FLAG_FROZEN && DQMC_SWEEP_PROFILE == DQMC_FLAG_PROFILE &&
+                  DQMC_SWEEP_DYNAMIC == DQMC_FLAG_DYNAMIC,
+              "the sweep flags of the C-ABI are the device flags");
+
 namespace {
 
 // minstd_rand0 and the bernoulli draw of libstdc++ (see the device twin in dqmc_kernels.hip)
@@ -99,9 +104,16 @@ Layout chain_layout(int N, int L, int n_stack, int nspin, int kdelay, bool dense
     l.field = take((size_t)L * N);
     l.st = take(sizeof(ChainState));
     l.acc = take(16 * sizeof(double));
+    l.Gt0 = take(nspin * n2);
+    l.G0t = take(nspin * n2);
+    l.dacc = take(DQMC_DYN_ACC_SIZE(L, N) * sizeof(double));
+    l.dscr = take(((size_t)L * N + N) * sizeof(double));
+    l.rhoq = take(N * sizeof(double));
     l.recG = record ? take((size_t)L * nspin * n2) : o;
     l.recSign = record ? take(L * sizeof(double)) : o;
     l.recAcc = record ? take((size_t)L * N) : o;
+    l.recGt0 = record ? take((size_t)L * nspin * n2) : o;
+    l.recG0t = record ? take((size_t)L * nspin * n2) : o;
     l.total = o;
     return l;
 }
@@ -214,6 +226,15 @@ int batch_prepare(dqmc_context* ctx, int n_chains, const dqmc_chain_params* para
         const int q = p.momentum_index < 0 ? (int)kx.size() - 1 : p.momentum_index;
         h.cosq.resize(h.N);
         for (int d = 0; d < h.N; ++d) h.cosq[d] = std::cos((-(d % side)) * kx[q] - (d / side) * ky[q]);
+        // FourierFactor(rx, 1) - FourierFactor(ry, 1) of the stiffness (src/measure/measure_methods.cpp:371-389):
+        // sites (rx, 0) and (ry, 0) against the k-star of index 1
+        h.rhoq.resize(h.N);
+        for (int d = 0; d < h.N; ++d) {
+            const int rx = d % side, ry = d / side;
+            h.rhoq[d] = std::cos((-rx) * kx[1] - 0 * ky[1]) - std::cos((-ry) * kx[1] - 0 * ky[1]);
+        }
+        h.kx = kx;
+        h.ky = ky;
         b->layouts[c] = chain_layout(h.N, p.time_size, h.n_stack, h.nspin, h.kdelay, !p.checkerboard, b->record);
         b->base[c] = off;
         off += b->layouts[c].total;
@@ -277,11 +298,19 @@ int batch_bind(dqmc_batch* b) {
         d.recG = b->record ? (double*)(base + l.recG) : nullptr;
         d.recSign = b->record ? (double*)(base + l.recSign) : nullptr;
         d.recAcc = b->record ? (uint8_t*)(base + l.recAcc) : nullptr;
+        d.Gt0 = (double*)(base + l.Gt0);
+        d.G0t = (double*)(base + l.G0t);
+        d.dacc = (double*)(base + l.dacc);
+        d.dscr = (double*)(base + l.dscr);
+        d.rhoq = (const double*)(base + l.rhoq);
+        d.recGt0 = b->record ? (double*)(base + l.recGt0) : nullptr;
+        d.recG0t = b->record ? (double*)(base + l.recG0t) : nullptr;
         if (!p.checkerboard) {
             HIPCHK(ctx, hipMemcpyAsync((void*)d.expK, h.expK.data(), h.expK.size() * 8, hipMemcpyHostToDevice, b->stream));
             HIPCHK(ctx, hipMemcpyAsync((void*)d.invExpK, h.invExpK.data(), h.invExpK.size() * 8, hipMemcpyHostToDevice, b->stream));
         }
         HIPCHK(ctx, hipMemcpyAsync((void*)d.cosq, h.cosq.data(), h.cosq.size() * 8, hipMemcpyHostToDevice, b->stream));
+        HIPCHK(ctx, hipMemcpyAsync((void*)d.rhoq, h.rhoq.data(), h.rhoq.size() * 8, hipMemcpyHostToDevice, b->stream));
     }
     HIPCHK(ctx, hipMemcpyAsync(b->d_descs, b->descs.data(), sizeof(ChainDesc) * b->n, hipMemcpyHostToDevice, b->stream));
     return DQMC_OK;
@@ -443,8 +472,11 @@ int dqmc_init(dqmc_batch_t b) {
     HIPCHK(b->ctx, hipSetDevice(b->ctx->device));
     for (int c = 0; c < b->n; ++c) {
         HIPCHK(b->ctx, hipMemsetAsync(b->descs[c].acc, 0, 16 * sizeof(double), b->stream));
+        HIPCHK(b->ctx, hipMemsetAsync(b->descs[c].dacc, 0, DQMC_DYN_ACC_SIZE(b->chains[c].p.time_size, b->chains[c].N) * 8,
+                                      b->stream));
         b->chains[c].counts = 0;
         b->chains[c].sweeps = 0;
+        b->chains[c].pos = 0;
     }
     HIPCHK(b->ctx, dqmc_launch_init(b->d_descs, b->n, b->stream));
     HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
@@ -480,7 +512,12 @@ int dqmc_sweep(dqmc_batch_t b, int n_sweeps, int flags) {
     b->timed = true;
     for (auto& h : b->chains) {
         h.sweeps += n_sweeps;
-        if (flags & DQMC_SWEEP_MEASURE) h.counts += (double)n_sweeps * h.p.time_size;
+        // with DQMC_SWEEP_DYNAMIC the upward sweeps are the time-displaced ones, which take no
+        // equal-time measurement
+        const int n_up = h.pos == 0 ? (n_sweeps + 1) / 2 : n_sweeps / 2;
+        const int n_eq = (flags & DQMC_SWEEP_DYNAMIC) ? n_sweeps - n_up : n_sweeps;
+        if (flags & DQMC_SWEEP_MEASURE) h.counts += (double)n_eq * h.p.time_size;
+        h.pos = (h.pos + n_sweeps) & 1;
     }
     return DQMC_OK;
 }
@@ -587,10 +624,70 @@ int dqmc_clear_measure(dqmc_batch_t b) {
     if (!b) return DQMC_ERR_ARG;
     for (int c = 0; c < b->n; ++c) {
         HIPCHK(b->ctx, hipMemsetAsync(b->descs[c].acc, 0, 16 * sizeof(double), b->stream));
+        HIPCHK(b->ctx, hipMemsetAsync(b->descs[c].dacc, 0, DQMC_DYN_ACC_SIZE(b->chains[c].p.time_size, b->chains[c].N) * 8,
+                                      b->stream));
         b->chains[c].counts = 0;
     }
     HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
     return DQMC_OK;
+}
+
+int dqmc_get_dynamic_green(dqmc_batch_t b, int c, int slot, int spin, int which, double* out) {
+    int rc = check_chain(b, c);
+    if (rc) return rc;
+    const HostChain& h = b->chains[c];
+    if (!b->record) { b->ctx->err = "batch created without record buffers"; return DQMC_ERR_STATE; }
+    if (!out || slot < 0 || slot >= h.p.time_size || (spin != 1 && spin != -1) || (which != 0 && which != 1))
+        return DQMC_ERR_ARG;
+    const int sp = (spin == 1 || h.nspin == 1) ? 0 : 1;
+    const size_t n2 = (size_t)h.N * h.N;
+    const double* src = which == 0 ? b->descs[c].recGt0 : b->descs[c].recG0t;
+    HIPCHK(b->ctx, hipMemcpyAsync(out, src + ((size_t)slot * h.nspin + sp) * n2, n2 * 8, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
+    return DQMC_OK;
+}
+
+int dqmc_get_dynamic_measure(dqmc_batch_t b, int c, int n_k, const int* k_index, double* greens_k, double* dos,
+                             double* dss, double* scalars) {
+    int rc = check_chain(b, c);
+    if (rc) return rc;
+    const HostChain& h = b->chains[c];
+    const int N = h.N, L = h.p.time_size, nk = (int)h.kx.size();
+    if (n_k < 0 || (n_k > 0 && (!k_index || !greens_k))) return DQMC_ERR_ARG;
+    for (int q = 0; q < n_k; ++q)
+        if (k_index[q] < 0 || k_index[q] >= nk) { b->ctx->err = "momentum index out of range"; return DQMC_ERR_ARG; }
+    std::vector<double> acc(DQMC_DYN_ACC_SIZE(L, N));
+    HIPCHK(b->ctx, hipMemcpyAsync(acc.data(), b->descs[c].dacc, acc.size() * 8, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
+    const double* gr = acc.data();
+    const int side = h.p.side;
+    // G(k,t) = 1/N sum_r FourierFactor(r, k) gr[t][r]  (src/measure/measure_methods.cpp:288-306),
+    // element (k, t) at k + n_k t; D(t) = gr[t][0] / N (the trace, :313-325)
+    for (int t = 0; t < L; ++t) {
+        for (int q = 0; q < n_k; ++q) {
+            const int k = k_index[q];
+            double g = 0.0;
+            for (int r = 0; r < N; ++r) g += std::cos((-(r % side)) * h.kx[k] - (r / side) * h.ky[k]) * gr[(size_t)t * N + r];
+            greens_k[(size_t)t * n_k + q] = g / N;
+        }
+        if (dos) dos[t] = gr[(size_t)t * N] / N;
+        if (dss) dss[t] = acc[(size_t)L * N + t];
+    }
+    if (scalars) {
+        scalars[0] = acc[(size_t)L * N + L];       // rho_s sum
+        scalars[1] = acc[(size_t)L * N + L + 1];   // sign sum
+        scalars[2] = acc[(size_t)L * N + L + 2];   // measurement count
+    }
+    return DQMC_OK;
+}
+
+int dqmc_num_kstars(dqmc_batch_t b, int c, double* kxy) {
+    int rc = check_chain(b, c);
+    if (rc) return rc;
+    const HostChain& h = b->chains[c];
+    if (kxy)
+        for (size_t k = 0; k < h.kx.size(); ++k) { kxy[2 * k] = h.kx[k]; kxy[2 * k + 1] = h.ky[k]; }
+    return (int)h.kx.size();
 }
 
 int dqmc_k_gemm(dqmc_ctx_t ctx, int variant, int M, int Nc, int K, int tA, int tB, const double* A, const double* B,

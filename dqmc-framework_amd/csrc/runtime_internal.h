@@ -24,14 +24,17 @@ struct HostChain {
     int N = 0, nspin = 2, n_stack = 0, kdelay = 1;
     double dtau = 0, alpha = 0;
     std::vector<int8_t> fields;
-    std::vector<double> expK, invExpK, cosq;
+    std::vector<double> expK, invExpK, cosq, rhoq;
+    std::vector<double> kx, ky;   // k-stars of the lattice (src/lattice/square.cpp:42-66)
     long long sweeps = 0;
     double counts = 0;  // measured slices (same count for every observable)
+    int pos = 0;        // 0: the next sweep goes up (from slice 0), 1: down
     size_t off_desc_bytes = 0;
 };
 
 struct Layout {
-    size_t G, Gw, Tmp, stkU, stkT, stkD, W, Wq, expK, invExpK, cosq, field, st, acc, recG, recSign, recAcc, total;
+    size_t G, Gw, Tmp, stkU, stkT, stkD, W, Wq, expK, invExpK, cosq, field, st, acc, Gt0, G0t, dacc, dscr, rhoq, recG,
+        recSign, recAcc, recGt0, recG0t, total;
 };
 
 struct dqmc_batch {

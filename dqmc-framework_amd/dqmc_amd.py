@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libdqmc_amd.so")
 OK, ERR_ARG, ERR_HIP, ERR_MEMORY, ERR_STATE, ERR_NUMERIC = 0, -1, -2, -3, -4, -5
 REPULSIVE, ATTRACTIVE = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
-SWEEP_MEASURE, SWEEP_RECORD, SWEEP_FROZEN, SWEEP_PROFILE = 1, 2, 4, 8
+SWEEP_MEASURE, SWEEP_RECORD, SWEEP_FROZEN, SWEEP_PROFILE, SWEEP_DYNAMIC = 1, 2, 4, 8, 16
 PHASES = ["wrap", "metropolis", "product", "push", "fresh", "measure", "walk", "panels"]
 OBS_NAMES = ["sign", "filling_number", "double_occupancy", "kinetic_energy", "momentum_distribution",
              "local_spin_corr", "spin_density_structure_factor", "charge_density_structure_factor",
@@ -26,7 +26,7 @@ EXPORTS = ["dqmc_context_create", "dqmc_context_destroy", "dqmc_context_error", 
            "dqmc_get_fields", "dqmc_set_rng_state", "dqmc_set_expk", "dqmc_get_expk", "dqmc_init", "dqmc_sweep",
            "dqmc_synchronize", "dqmc_last_sweep_ms", "dqmc_get_green", "dqmc_get_slice_green", "dqmc_get_slice_signs",
            "dqmc_get_accepts", "dqmc_get_info", "dqmc_get_measure", "dqmc_measure_recorded", "dqmc_clear_measure",
-           "dqmc_k_gemm", "dqmc_autotune_delay",
+           "dqmc_get_dynamic_green", "dqmc_get_dynamic_measure", "dqmc_num_kstars", "dqmc_k_gemm", "dqmc_autotune_delay",
            "dqmc_server_create", "dqmc_server_submit", "dqmc_server_cancel", "dqmc_server_poll",
            "dqmc_server_wait_idle", "dqmc_server_get_stats", "dqmc_server_error", "dqmc_server_destroy"]
 
@@ -100,6 +100,9 @@ def lib():
         L.dqmc_get_accepts.argtypes = [vp, ip, vp]
         L.dqmc_get_info.argtypes = [vp, ip, C.POINTER(ChainInfo)]
         L.dqmc_get_measure.argtypes = [vp, ip, vp, vp]
+        L.dqmc_get_dynamic_green.argtypes = [vp, ip, ip, ip, ip, vp]
+        L.dqmc_get_dynamic_measure.argtypes = [vp, ip, ip, vp, vp, vp, vp, vp]
+        L.dqmc_num_kstars.argtypes = [vp, ip, vp]
         L.dqmc_k_gemm.argtypes = [vp, ip, ip, ip, ip, ip, ip, vp, vp, vp, C.c_double, vp, vp, vp]
         L.dqmc_autotune_delay.argtypes = [vp, C.POINTER(ChainParams), ip, vp, ip, ip, C.POINTER(ip), vp]
         L.dqmc_server_create.argtypes = [vp, C.c_size_t, ip, ip, C.POINTER(vp)]
@@ -232,9 +235,11 @@ class Batch:
     def init(self):
         self.ctx.check(lib().dqmc_init(self.h), "dqmc_init")
 
-    def sweep(self, n, measure=False, record=False, frozen=False, profile=False):
+    def sweep(self, n, measure=False, record=False, frozen=False, profile=False, dynamic=False):
+        """n single-direction sweeps; with dynamic=True every 0->beta sweep is the time-displaced
+        one (DqmcWalker::sweep_for_dynamic_greens) and measures the dynamic observables."""
         flags = ((SWEEP_MEASURE if measure else 0) | (SWEEP_RECORD if record else 0) | (SWEEP_FROZEN if frozen else 0)
-                 | (SWEEP_PROFILE if profile else 0))
+                 | (SWEEP_PROFILE if profile else 0) | (SWEEP_DYNAMIC if dynamic else 0))
         self.ctx.check(lib().dqmc_sweep(self.h, int(n), flags), "dqmc_sweep")
 
     def last_sweep_ms(self):
@@ -282,6 +287,34 @@ class Batch:
 
     def clear_measure(self):
         self.ctx.check(lib().dqmc_clear_measure(self.h), "dqmc_clear_measure")
+
+    def kstars(self, c):
+        nk = lib().dqmc_num_kstars(self.h, c, None)
+        if nk < 0:
+            self.ctx.check(nk, "dqmc_num_kstars")
+        k = np.zeros((nk, 2))
+        lib().dqmc_num_kstars(self.h, c, _p(k))
+        return k
+
+    def dynamic_green(self, c, slot, spin, which):
+        """which 0: G(t,0), 1: G(0,t) of slot t-1 kept by the last dynamic RECORD sweep."""
+        n = self.N[c]
+        g = np.zeros((n, n), order="F")
+        self.ctx.check(lib().dqmc_get_dynamic_green(self.h, c, slot, spin, which, _p(g)), "dqmc_get_dynamic_green")
+        return g
+
+    def dynamic_accum(self, c, k_index=None):
+        """Dynamic sums: gk (n_k x L, all k-stars unless k_index is given), dos (L), dss (L), rho,
+        sign, count (MeasureHandler::dynamic_measure, src/measure/measure_methods.cpp:264-461)."""
+        L = self.L[c]
+        if k_index is None:
+            k_index = np.arange(lib().dqmc_num_kstars(self.h, c, None))
+        k_index = np.ascontiguousarray(k_index, dtype=np.int32)
+        nk = len(k_index)
+        gk = np.zeros(nk * L); dos = np.zeros(L); dss = np.zeros(L); sc = np.zeros(3)
+        self.ctx.check(lib().dqmc_get_dynamic_measure(self.h, c, nk, _p(k_index), _p(gk), _p(dos), _p(dss), _p(sc)),
+                       "dqmc_get_dynamic_measure")
+        return dict(gk=gk.reshape(L, nk).T.copy(), dos=dos, dss=dss, rho=sc[0], sign=sc[1], count=sc[2])
 
 
 def normalize_bin(sums, counts):

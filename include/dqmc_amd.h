@@ -65,6 +65,13 @@ extern "C" {
 #define DQMC_SWEEP_FROZEN 4  /* no field updates: propagate, stabilize and measure a fixed
                                 configuration (as the reference's sweep_for_dynamic_greens does) */
 #define DQMC_SWEEP_PROFILE 8 /* accumulate device time per sweep phase (dqmc_chain_info.phase_seconds) */
+#define DQMC_SWEEP_DYNAMIC 16 /* every 0->beta sweep is the time-displaced one of
+                                 DqmcWalker::sweep_for_dynamic_greens (src/dqmc_walker.cpp:424-524):
+                                 fields fixed, G(t,0) and G(0,t) propagated and recomputed with G(t,t);
+                                 with DQMC_SWEEP_MEASURE it accumulates the dynamic observables
+                                 (MeasureHandler::dynamic_measure) instead of the equal-time ones, as
+                                 Dqmc::sweep_forth_and_back does (src/dqmc.cpp:48-70); the beta->0
+                                 sweeps stay updating sweeps with equal-time measurements */
 
 /* equal-time observables, index order of dqmc_get_measure */
 #define DQMC_OBS_SIGN 0
@@ -163,7 +170,27 @@ int dqmc_get_measure(dqmc_batch_t b, int chain, double* sums, double* counts);
 /* MeasureHandler::equaltime_measure (src/measure/measure_handler.cpp:136-150) on the
  * slices kept by the last DQMC_SWEEP_RECORD sweep (batch created with record = 1) */
 int dqmc_measure_recorded(dqmc_batch_t b);
+/* clears the equal-time and the dynamic accumulators */
 int dqmc_clear_measure(dqmc_batch_t b);
+
+/* Time-displaced path.  G(t,0) (which = 0) or G(0,t) (which = 1) of slot t-1 kept by the
+ * last DQMC_SWEEP_DYNAMIC | DQMC_SWEEP_RECORD sweep, column-major N x N
+ * (DqmcWalker::Greent0Up/Dn(t), Green0tUp/Dn(t), include/dqmc_walker.h; the vectors filled at
+ * src/dqmc_walker.cpp:455-463, :500-505). */
+int dqmc_get_dynamic_green(dqmc_batch_t b, int chain, int slot, int spin, int which, double* out);
+/* Dynamic accumulators (sums over DQMC_SWEEP_DYNAMIC | DQMC_SWEEP_MEASURE sweeps, each term
+ * weighted by the configuration sign; normalize per bin as the equal-time ones):
+ *   greens_k[k + n_k t]  G(k,t) for the k-stars k_index[0..n_k) (measure_greens_functions,
+ *                        src/measure/measure_methods.cpp:280-306; the reference's MomentumList)
+ *   dos[t]               density of states D(t) (:313-325)
+ *   dss[t]               dynamic spin susceptibility (:413-461)
+ *   scalars[0..2]        superfluid stiffness rho_s (:335-405), sign sum (:264-272), count
+ * for t = 0..L-1.  Any output pointer may be NULL (greens_k only when n_k = 0). */
+int dqmc_get_dynamic_measure(dqmc_batch_t b, int chain, int n_k, const int* k_index, double* greens_k, double* dos,
+                             double* dss, double* scalars);
+/* number of k-stars of the chain's lattice; kxy (may be NULL) receives (kx, ky) per k-star
+ * (Lattice::Square::kStars, src/lattice/square.cpp:42-66) */
+int dqmc_num_kstars(dqmc_batch_t b, int chain, double* kxy);
 
 /* Delayed-update depth autotuner (no counterpart in the reference, which
  * applies every flip at once): times `sweeps` sweeps of `n_chains` chains of
