@@ -146,6 +146,40 @@ def phase_breakdown(w, chains, sweeps=2):
     return out
 
 
+def run_dynamic(w, steps, chains=None, profile=False):
+    """The measuring schedule with time-displaced observables (src/dqmc.cpp:48-70): each
+    forth-and-back is a dynamic sweep (fields fixed, G(t,0) and G(0,t) with G(t,t), dynamic
+    observables accumulated) and an updating backward sweep.  Times the dynamic sweeps."""
+    ctx = D.Context(0)
+    b = D.Batch(ctx, chain_params(w, chains))
+    b.fields_random()
+    b.init()
+    b.sweep(2)
+    dyn_ms = []
+    for _ in range(steps):
+        b.sweep(1, measure=True, dynamic=True, profile=profile)
+        dyn_ms.append(b.last_sweep_ms())
+        b.sweep(1, measure=True, dynamic=True)
+    b.synchronize()
+    n = b.n
+    out = {"chains": n, "ms_per_dynamic_sweep": sum(dyn_ms) / len(dyn_ms),
+           "chain_dynamic_sweeps_per_s": n / (sum(dyn_ms) / len(dyn_ms) * 1e-3),
+           "max_wrap_error": max(b.info(c).wrap_error for c in range(n)),
+           "measurements_per_chain": b.dynamic_accum(0)["count"]}
+    if profile:
+        m = min(n, 8)
+        nph = len(D.PHASES)
+        tot = [0.0] * nph
+        for c in range(m):
+            ph = b.info(c).phase_seconds
+            for q in range(nph):
+                tot[q] += ph[q]
+        out["phases_ms_per_chain_sweep"] = {name: 1000.0 * tot[q] / (m * steps) for q, name in enumerate(D.PHASES)}
+    b.close()
+    ctx.close()
+    return out
+
+
 def cpu_baseline(w, seconds=12.0, max_procs=16):
     exe = os.path.join(ROOT, "oracle", "_build", "refcpu")
     if not os.path.exists(exe):
@@ -183,6 +217,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the single-chain comparison run")
     ap.add_argument("--phases", action="store_true", help="also report device time per sweep phase")
+    ap.add_argument("--dynamic-steps", type=int, default=3,
+                    help="time-displaced sweeps timed on the same batch (0 skips them)")
     ap.add_argument("--no-autotune", action="store_true", help="use the workload's fixed delay depth")
     ap.add_argument("--serve-jobs", type=int, default=48, help="jobs of the serving sample (0 skips it)")
     ap.add_argument("--serve-scale", type=float, default=0.05, help="sweep-count scale of the serving sample")
@@ -246,6 +282,9 @@ def main():
         print("[bench] phase breakdown", file=sys.stderr, flush=True)
         out["phases_ms_per_chain_sweep"] = {"batched": phase_breakdown(w, r["chains"]),
                                             "single": phase_breakdown(w, 1)}
+    if args.dynamic_steps > 0:
+        print(f"[bench] time-displaced sweeps: {args.dynamic_steps}", file=sys.stderr, flush=True)
+        out["dynamic"] = run_dynamic(w, args.dynamic_steps, profile=args.phases)
     if args.serve_jobs > 0:
         import serve
         print(f"[bench] serving sample: {args.serve_jobs} jobs at {args.serve_rate}/s", file=sys.stderr, flush=True)

@@ -2488,26 +2488,41 @@ __device__ __noinline__ void dynamic_fresh(Desc d, int lslot, int rslot, ChainSt
 // G(k,t) and D(t) are linear in the displacement sums gr[t][r] (the host applies
 // the Fourier factors of the momenta it is asked for).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int site_shift(int i, int r, int side) {
-    const int x = (i % side + r % side) % side, y = (i / side + r / side) % side;
-    return x + side * y;
-}
+// site coordinates without integer division (side <= 21, i < 448: the multiply-shift
+// quotient is exact since i (ceil(2^16/side) - 2^16/side) / 2^16 < 1/side)
+struct SiteMap {
+    int side, mg;
+    __device__ explicit SiteMap(int s) : side(s), mg((65536 + s - 1) / s) {}
+    __device__ __forceinline__ int y(int i) const { return (i * mg) >> 16; }
+    __device__ __forceinline__ int x(int i) const { return i - y(i) * side; }
+    __device__ __forceinline__ int wrap(int v) const { return v >= side ? v - side : (v < 0 ? v + side : v); }
+    __device__ __forceinline__ int plus_x(int i) const { const int yy = y(i), xx = i - yy * side; return wrap(xx + 1) + side * yy; }
+    // displacement index of j seen from i (Lattice::Square::Displacement, src/lattice/square.cpp:88-105)
+    __device__ __forceinline__ int disp(int i, int j) const {
+        const int yi = y(i), yj = y(j);
+        return wrap(j - yj * side - (i - yi * side)) + side * wrap(yj - yi);
+    }
+    __device__ __forceinline__ int shift(int i, int r) const {
+        const int yi = y(i), yr = y(r);
+        return wrap(i - yi * side + r - yr * side) + side * wrap(yi + yr);
+    }
+};
 
 __device__ __noinline__ void dyn_measure_entry(Desc d, int t, double cs, Lds&) {
     Lds& s = g_lds;
-    const int N = d.N, side = d.side, L = d.L;
+    const int N = uni(d.N), L = d.L, ns = uni(d.nspin);
+    const SiteMap sm(uni(d.side));
     const size_t n2 = (size_t)N * N;
-    const int ns = d.nspin;
     const double spin_w = ns == 2 ? 1.0 : 2.0;   // attractive chains carry one spin: G_dn = G_up
     AS_G double* acc = gptr(d.dacc);
     AS_G double* delta = gptr(d.dscr + (size_t)t * N);
     AS_G double* abar = gptr(d.dscr + (size_t)L * N);
     const AS_G double* gtt = gptr(d.G);
     const AS_G double* gt0 = gptr(d.Gt0);
+    const AS_G double* g0t = gptr(d.G0t);
     double v[2] = {0.0, 0.0};   // sum_i sum_spin G(0,t)_ii G(t,0)_ii, correlated stiffness sum
     for (int i = tid(); i < N; i += NT) {
-        const int x = i % side, y = i / side;
-        const int ipx = (x + 1) % side + side * y;
+        const int ipx = sm.plus_x(i);
         const double du = gtt[(size_t)i * N + i];
         const double dd = ns == 2 ? gtt[n2 + (size_t)i * N + i] : du;
         delta[i] = dd - du;
@@ -2515,42 +2530,65 @@ __device__ __noinline__ void dyn_measure_entry(Desc d, int t, double cs, Lds&) {
         for (int sp = 0; sp < ns; ++sp) {
             const size_t o = sp * n2;
             a += gtt[o + (size_t)ipx * N + i] - gtt[o + (size_t)i * N + ipx];
-            c += gptr(d.G0t)[o + (size_t)i * N + i] * gt0[o + (size_t)i * N + i];
+            c += g0t[o + (size_t)i * N + i] * gt0[o + (size_t)i * N + i];
         }
         abar[i] += spin_w * a;
         v[0] += spin_w * c;
     }
     // displacement sums of the spin-averaged G(t,0) (G(0,0) at entry 0): thread r owns gr[t][r]
-    for (int r = tid(); r < N; r += NT) {
-        double g = 0.0;
+    // and walks the columns, eight of them per round with their loads in flight together
+    {
         const AS_G double* src = t == 0 ? gtt : gt0;
-        for (int i = 0; i < N; ++i) {
-            const int j = site_shift(i, r, side);
-            g += src[(size_t)i * N + j];
-            if (ns == 2) g += src[n2 + (size_t)i * N + j];
+        for (int r = tid(); r < N; r += NT) {
+            double g = 0.0;
+            constexpr int U = 8;
+            for (int i0 = 0; i0 < N; i0 += U) {
+                double q[2 * U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int i = min(i0 + u, N - 1);
+                    const size_t e = (size_t)i * N + sm.shift(i, r);
+                    q[2 * u] = src[e];
+                    q[2 * u + 1] = ns == 2 ? src[n2 + e] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (i0 + u < N) g += q[2 * u] + q[2 * u + 1];
+            }
+            acc[(size_t)t * N + r] += cs * (ns == 2 ? 0.5 * g : g);
         }
-        acc[(size_t)t * N + r] += cs * (ns == 2 ? 0.5 * g : g);
     }
     // G(0,t)^T of each spin into W, so that the stiffness pairs read both matrices along columns
     for (int sp = 0; sp < ns; ++sp)
         wg_transpose_scaled(d.W + sp * n2, nullptr, d.G0t + sp * n2, nullptr, N, nullptr, nullptr, s);
-    const AS_G double* g0tT = gptr(d.W);
-    const AS_G double* rq = gptr(d.rhoq);
-    for (int p = tid(); p < N * N; p += NT) {
-        const int j = p % N, i = p / N;
-        const int xi = i % side, yi = i / side, xj = j % side, yj = j / side;
-        const int ipx = (xi + 1) % side + side * yi, jpx = (xj + 1) % side + side * yj;
-        const int dsp = (xj - xi + side) % side + side * ((yj - yi + side) % side);
-        double w = 0.0;
-        for (int sp = 0; sp < ns; ++sp) {
-            const size_t o = sp * n2;
-            // g0t(a,b) = g0tT[a*N + b], gt0(a,b) = gt0[b*N + a]
-            w += -g0tT[o + (size_t)ipx * N + jpx] * gt0[o + (size_t)i * N + j]
-                 + g0tT[o + (size_t)i * N + jpx] * gt0[o + (size_t)ipx * N + j]
-                 + g0tT[o + (size_t)ipx * N + j] * gt0[o + (size_t)i * N + jpx]
-                 - g0tT[o + (size_t)i * N + j] * gt0[o + (size_t)ipx * N + jpx];
+    for (int i = tid(); i < N; i += NT) s.v0[i] = gptr(d.rhoq)[i];
+    __syncthreads();
+    const AS_L double* rq = (const AS_L double*)s.v0;
+    // p / N by multiply-high: exact for p < N^2 <= 448^2 since p (ceil(2^32/N) - 2^32/N) / 2^32 < 1/N
+    const unsigned mgN = (unsigned)((0x100000000ULL + N - 1) / N);
+    for (int sp = 0; sp < ns; ++sp) {
+        const AS_G double* g0tT = gptr(d.W + sp * n2);   // g0t(a,b) at a*N + b
+        const AS_G double* gt = gt0 + sp * n2;            // gt0(a,b) at b*N + a
+        constexpr int MU = 4;
+        for (int p0 = tid(); p0 < N * N; p0 += MU * NT) {
+            double q[MU][8];
+            double f[MU];
+#pragma unroll
+            for (int u = 0; u < MU; ++u) {
+                const int p = min(p0 + u * NT, N * N - 1);
+                const int i = (int)__umulhi((unsigned)p, mgN), j = p - i * N;
+                const int ipx = sm.plus_x(i), jpx = sm.plus_x(j);
+                f[u] = rq[sm.disp(i, j)];
+                q[u][0] = g0tT[(size_t)ipx * N + jpx]; q[u][1] = gt[(size_t)i * N + j];
+                q[u][2] = g0tT[(size_t)i * N + jpx];   q[u][3] = gt[(size_t)ipx * N + j];
+                q[u][4] = g0tT[(size_t)ipx * N + j];   q[u][5] = gt[(size_t)i * N + jpx];
+                q[u][6] = g0tT[(size_t)i * N + j];     q[u][7] = gt[(size_t)ipx * N + jpx];
+            }
+#pragma unroll
+            for (int u = 0; u < MU; ++u)
+                if (p0 + u * NT < N * N)
+                    v[1] += spin_w * f[u] * (-q[u][0] * q[u][1] + q[u][2] * q[u][3] + q[u][4] * q[u][5] - q[u][6] * q[u][7]);
         }
-        v[1] += spin_w * rq[dsp] * w;
     }
     wg_sum8(v, 2, s);
     if (tid() == 0) {
@@ -2564,39 +2602,42 @@ __device__ __noinline__ void dyn_measure_entry(Desc d, int t, double cs, Lds&) {
 // the G(0,0)-dependent parts, once entry 0 (slot L-1) is measured; then sign and count
 __device__ __noinline__ void dyn_finalize(Desc d, double cs, Lds&) {
     Lds& s = g_lds;
-    const int N = d.N, side = d.side, L = d.L;
+    const int N = uni(d.N), L = d.L, ns = uni(d.nspin);
+    const SiteMap sm(uni(d.side));
     const size_t n2 = (size_t)N * N;
-    const int ns = d.nspin;
     const double spin_w = ns == 2 ? 1.0 : 2.0;
     AS_G double* acc = gptr(d.dacc);
     const AS_G double* delta = gptr(d.dscr);
     const AS_G double* gtt = gptr(d.G);
-    // A_0(i) into v0, the summed current vector into v1, d_0 into v2
+    // A_0(i) into v0, the summed current vector into v1, d_0 into v2, the stiffness factors into v3
     for (int i = tid(); i < N; i += NT) {
-        const int x = i % side, y = i / side;
-        const int ipx = (x + 1) % side + side * y;
+        const int ipx = sm.plus_x(i);
         double a = 0.0;
         for (int sp = 0; sp < ns; ++sp) a += gtt[sp * n2 + (size_t)ipx * N + i] - gtt[sp * n2 + (size_t)i * N + ipx];
         s.v0[i] = spin_w * a;
         s.v1[i] = gptr(d.dscr)[(size_t)L * N + i];
         s.v2[i] = delta[i];
+        s.v3[i] = gptr(d.rhoq)[i];
     }
     __syncthreads();
     double v[1] = {0.0};
     for (int i = tid(); i < N; i += NT) {
-        const int xi = i % side, yi = i / side;
         double f = 0.0;
-        for (int j = 0; j < N; ++j) {
-            const int xj = j % side, yj = j / side;
-            const int dsp = (xj - xi + side) % side + side * ((yj - yi + side) % side);
-            f += gptr(d.rhoq)[dsp] * s.v1[j];
-        }
+        for (int j = 0; j < N; ++j) f += s.v3[sm.disp(i, j)] * s.v1[j];
         v[0] -= s.v0[i] * f;
     }
     // uncorrelated spin susceptibility, one slice entry per thread
     for (int t = tid(); t < L; t += NT) {
         double u = 0.0;
-        for (int i = 0; i < N; ++i) u += delta[(size_t)t * N + i] * s.v2[i];
+        constexpr int U = 8;
+        for (int i0 = 0; i0 < N; i0 += U) {
+            double q[U];
+#pragma unroll
+            for (int k = 0; k < U; ++k) q[k] = delta[(size_t)t * N + min(i0 + k, N - 1)];
+#pragma unroll
+            for (int k = 0; k < U; ++k)
+                if (i0 + k < N) u += q[k] * s.v2[i0 + k];
+        }
         acc[(size_t)L * N + t] += 0.25 * cs / (double)N * u;
     }
     wg_sum8(v, 1, s);

@@ -200,10 +200,17 @@ namespace QuantumMonteCarlo {
 class DqmcWalker {
     int m_space_size{}, m_time_size{}, m_stabilization_pace{10}, m_delay{16};
     double m_beta{}, m_time_interval{};
-    bool m_is_equaltime{false};
+    bool m_is_equaltime{false}, m_is_dynamic{false};
     dqmc_batch_t m_batch{nullptr};
     friend class DqmcInitializer;
     friend class Measure::MeasureHandler;
+
+    std::vector<double> fetch_dynamic(int slot, int spin, int which) const {
+        std::vector<double> g((size_t)m_space_size * m_space_size);
+        dqmc_detail::check(dqmc_get_dynamic_green(m_batch, 0, slot, spin, which, g.data()), dqmc_detail::context(),
+                           which == 0 ? "DqmcWalker::Greent0" : "DqmcWalker::Green0t");
+        return g;
+    }
 
     std::vector<double> fetch(int slot, int spin) const {
         std::vector<double> g((size_t)m_space_size * m_space_size);
@@ -250,11 +257,28 @@ class DqmcWalker {
         if (i.pos != m_time_size) throw std::logic_error("sweep_from_beta_to_0: walker is not at slice beta");
         sweep_once();
     }
+    // src/dqmc_walker.cpp:424-524.  The engine accumulates the dynamic observables while this
+    // sweep runs (Measure::MeasureHandler::dynamic_measure is then bookkeeping only), so a walker
+    // whose handler measures dynamic observables is expected to pair it with dynamic_measure, as
+    // Dqmc::sweep_forth_and_back does.
+    void sweep_for_dynamic_greens(Model::ModelBase&) {
+        if (!m_is_dynamic) return;   // as the reference: nothing to do without dynamic observables
+        dqmc_chain_info i{};
+        dqmc_get_info(m_batch, 0, &i);
+        if (i.pos != 0) throw std::logic_error("sweep_for_dynamic_greens: walker is not at slice 0");
+        dqmc_detail::check(dqmc_sweep(m_batch, 1, DQMC_SWEEP_DYNAMIC | DQMC_SWEEP_MEASURE | DQMC_SWEEP_RECORD),
+                           dqmc_detail::context(), "DqmcWalker::sweep_for_dynamic_greens");
+    }
     // accessors (include/dqmc_walker.h:136-164); matrices are copied from the device
     std::vector<double> GreenttUp() const { return fetch(-1, +1); }
     std::vector<double> GreenttDn() const { return fetch(-1, -1); }
     std::vector<double> GreenttUp(int t) const { return fetch(t, +1); }
     std::vector<double> GreenttDn(int t) const { return fetch(t, -1); }
+    // time-displaced G(t,0), G(0,t) of slot t (t+1 slices from 0) of the last dynamic sweep
+    std::vector<double> Greent0Up(int t) const { return fetch_dynamic(t, +1, 0); }
+    std::vector<double> Greent0Dn(int t) const { return fetch_dynamic(t, -1, 0); }
+    std::vector<double> Green0tUp(int t) const { return fetch_dynamic(t, +1, 1); }
+    std::vector<double> Green0tDn(int t) const { return fetch_dynamic(t, -1, 1); }
     double ConfigSign() const {
         dqmc_chain_info i{};
         dqmc_get_info(m_batch, 0, &i);
@@ -284,15 +308,26 @@ inline const std::vector<std::string>& EqualTimeNames() {
                                                "s_wave_pairing_corr"};
     return n;
 }
+// names of the dynamic observables (src/measure/observable_handler.cpp:23-29)
+inline const std::vector<std::string>& DynamicNames() {
+    static const std::vector<std::string> n = {"greens_functions", "density_of_states", "superfluid_stiffness",
+                                               "dynamic_spin_susceptibility"};
+    return n;
+}
 
 class MeasureHandler {
     int m_sweeps_warmup{}, m_bin_num{}, m_bin_size{}, m_sweeps_between_bins{};
     ObsList m_obs_list{};
     int m_momentum{-1};
-    bool m_is_equaltime{false};
+    std::vector<int> m_momentum_list{};
+    bool m_is_equaltime{false}, m_is_dynamic{false};
     dqmc_batch_t m_batch{nullptr};
+    int m_time_size{};
     std::vector<std::vector<double>> m_bins;
     std::vector<double> m_mean, m_err;
+    // dynamic bins: |sign|, rho_s, G(k,t) (k fastest), D(t), DSS(t)
+    std::vector<std::vector<double>> m_dyn_bins;
+    std::vector<double> m_dyn_mean, m_dyn_err;
     friend class QuantumMonteCarlo::DqmcInitializer;
 
    public:
@@ -304,18 +339,24 @@ class MeasureHandler {
         m_sweeps_between_bins = sweeps_between_bins;
     }
     void set_observables(ObsList obs) {
+        m_is_equaltime = m_is_dynamic = false;
         for (const auto& o : obs) {
-            bool known = false;
-            for (const auto& n : EqualTimeNames()) known = known || (o == n);
-            if (!known) throw std::invalid_argument("unsupported observable '" + o + "' (equal-time only)");
+            bool eq = false, dyn = false;
+            for (const auto& n : EqualTimeNames()) eq = eq || (o == n);
+            for (const auto& n : DynamicNames()) dyn = dyn || (o == n);
+            if (!eq && !dyn) throw std::invalid_argument("unsupported observable '" + o + "'");
+            m_is_equaltime = m_is_equaltime || eq;
+            m_is_dynamic = m_is_dynamic || dyn;
         }
         m_obs_list = std::move(obs);
-        m_is_equaltime = !m_obs_list.empty();
     }
     void set_measured_momentum(int momentum_index) { m_momentum = momentum_index; }
+    // momentum indices (k-stars) of G(k,t) (MeasureHandler::set_measured_momentum_list)
+    void set_measured_momentum_list(std::vector<int> k) { m_momentum_list = std::move(k); }
+    const std::vector<int>& MomentumList() const { return m_momentum_list; }
     bool isWarmUp() const { return m_sweeps_warmup != 0; }
     bool isEqualTime() const { return m_is_equaltime; }
-    bool isDynamic() const { return false; }
+    bool isDynamic() const { return m_is_dynamic; }
     int WarmUpSweeps() const { return m_sweeps_warmup; }
     int SweepsBetweenBins() const { return m_sweeps_between_bins; }
     int BinsNum() const { return m_bin_num; }
@@ -326,33 +367,71 @@ class MeasureHandler {
     void equaltime_measure(const QuantumMonteCarlo::DqmcWalker&, const Model::ModelBase&, const Lattice::LatticeBase&) {
         dqmc_detail::check(dqmc_measure_recorded(m_batch), dqmc_detail::context(), "equaltime_measure");
     }
+    // src/measure/measure_handler.cpp:153-167: the engine has accumulated these during the
+    // time-displaced sweep (DqmcWalker::sweep_for_dynamic_greens)
+    void dynamic_measure(const QuantumMonteCarlo::DqmcWalker&, const Model::ModelBase&, const Lattice::LatticeBase&) {}
     // :170-240 — normalize by counts and <sign>, keep |<sign>|, store the bin
     void normalize_stats() {
-        double sums[DQMC_NUM_OBSERVABLES], counts[DQMC_NUM_OBSERVABLES];
-        dqmc_detail::check(dqmc_get_measure(m_batch, 0, sums, counts), dqmc_detail::context(), "normalize_stats");
-        std::vector<double> v(DQMC_NUM_OBSERVABLES);
-        const double sg = sums[0] / counts[0];
-        for (int o = 1; o < DQMC_NUM_OBSERVABLES; ++o) v[o] = sums[o] / (counts[o] * sg);
-        v[0] = std::fabs(sg);
-        m_bins.push_back(v);
+        if (m_is_equaltime) {
+            double sums[DQMC_NUM_OBSERVABLES], counts[DQMC_NUM_OBSERVABLES];
+            dqmc_detail::check(dqmc_get_measure(m_batch, 0, sums, counts), dqmc_detail::context(), "normalize_stats");
+            std::vector<double> v(DQMC_NUM_OBSERVABLES);
+            const double sg = sums[0] / counts[0];
+            for (int o = 1; o < DQMC_NUM_OBSERVABLES; ++o) v[o] = sums[o] / (counts[o] * sg);
+            v[0] = std::fabs(sg);
+            m_bins.push_back(v);
+        }
+        if (m_is_dynamic) {
+            const int L = m_time_size, nk = (int)m_momentum_list.size();
+            std::vector<double> gk((size_t)nk * L), dos(L), dss(L), sc(3);
+            dqmc_detail::check(dqmc_get_dynamic_measure(m_batch, 0, nk, m_momentum_list.data(), gk.data(), dos.data(),
+                                                        dss.data(), sc.data()),
+                               dqmc_detail::context(), "normalize_stats");
+            const double sg = sc[1] / sc[2], den = sc[2] * sg;
+            std::vector<double> w = {std::fabs(sg), sc[0] / den};
+            for (double x : gk) w.push_back(x / den);
+            for (double x : dos) w.push_back(x / den);
+            for (double x : dss) w.push_back(x / den);
+            m_dyn_bins.push_back(w);
+        }
     }
     void write_stats_to_bins(int) {}
     void clear_temporary() { dqmc_detail::check(dqmc_clear_measure(m_batch), dqmc_detail::context(), "clear_temporary"); }
     // include/measure/observable.h:169-193
     void analyse_stats() {
-        const size_t nb = m_bins.size();
-        m_mean.assign(DQMC_NUM_OBSERVABLES, 0.0);
-        m_err.assign(DQMC_NUM_OBSERVABLES, 0.0);
-        for (int o = 0; o < DQMC_NUM_OBSERVABLES; ++o) {
-            double m = 0.0, m2 = 0.0;
-            for (auto& b : m_bins) { m += b[o]; m2 += b[o] * b[o]; }
-            m /= nb; m2 /= nb;
-            m_mean[o] = m;
-            m_err[o] = nb > 1 ? std::sqrt(std::max(0.0, m2 - m * m)) / std::sqrt((double)nb - 1) : 0.0;
-        }
+        auto stats = [](const std::vector<std::vector<double>>& bins, std::vector<double>& mean, std::vector<double>& err) {
+            const size_t nb = bins.size(), nv = nb ? bins[0].size() : 0;
+            mean.assign(nv, 0.0);
+            err.assign(nv, 0.0);
+            for (size_t o = 0; o < nv; ++o) {
+                double m = 0.0, m2 = 0.0;
+                for (auto& b : bins) { m += b[o]; m2 += b[o] * b[o]; }
+                m /= nb; m2 /= nb;
+                mean[o] = m;
+                err[o] = nb > 1 ? std::sqrt(std::max(0.0, m2 - m * m)) / std::sqrt((double)nb - 1) : 0.0;
+            }
+        };
+        stats(m_bins, m_mean, m_err);
+        stats(m_dyn_bins, m_dyn_mean, m_dyn_err);
     }
     double mean(const std::string& name) const { return m_mean.at(index(name)); }
     double error(const std::string& name) const { return m_err.at(index(name)); }
+    // dynamic results: "dynamic_sign" and "superfluid_stiffness" one value, "greens_functions"
+    // n_k x L (k fastest), "density_of_states" and "dynamic_spin_susceptibility" L values
+    std::vector<double> dynamic_mean(const std::string& name) const { return slice(m_dyn_mean, name); }
+    std::vector<double> dynamic_error(const std::string& name) const { return slice(m_dyn_err, name); }
+    std::vector<double> slice(const std::vector<double>& v, const std::string& name) const {
+        const size_t L = m_time_size, nk = m_momentum_list.size();
+        size_t o = 0, n = 1;
+        if (name == "dynamic_sign") o = 0;
+        else if (name == "superfluid_stiffness") o = 1;
+        else if (name == "greens_functions") { o = 2; n = nk * L; }
+        else if (name == "density_of_states") { o = 2 + nk * L; n = L; }
+        else if (name == "dynamic_spin_susceptibility") { o = 2 + nk * L + L; n = L; }
+        else throw std::invalid_argument("unknown dynamic observable " + name);
+        if (v.size() < o + n) throw std::logic_error("no dynamic statistics");
+        return std::vector<double>(v.begin() + o, v.begin() + o + n);
+    }
     static int index(const std::string& name) {
         const auto& n = EqualTimeNames();
         for (size_t k = 0; k < n.size(); ++k)
@@ -372,6 +451,8 @@ class DqmcInitializer {
         if (!lattice.InitialStatus()) lattice.initial();
         walker.m_space_size = lattice.SpaceSize();
         walker.m_is_equaltime = meas.isEqualTime();
+        walker.m_is_dynamic = meas.isDynamic();
+        meas.m_time_size = walker.TimeSize();
         model.m_space_size = lattice.SpaceSize();
         model.m_time_size = walker.TimeSize();
         model.m_bosonic_field.assign((size_t)walker.TimeSize() * lattice.SpaceSize(), 1);
@@ -397,8 +478,9 @@ class DqmcInitializer {
         p.seed = 1;
         if (walker.m_batch) dqmc_batch_destroy(walker.m_batch);
         walker.m_batch = nullptr;
-        dqmc_detail::check(dqmc_batch_create(ctx, 1, &p, walker.m_is_equaltime ? 1 : 0, &walker.m_batch), ctx,
-                           "dqmc_batch_create");
+        dqmc_detail::check(dqmc_batch_create(ctx, 1, &p, (walker.m_is_equaltime || walker.m_is_dynamic) ? 1 : 0,
+                                             &walker.m_batch),
+                           ctx, "dqmc_batch_create");
         dqmc_detail::check(dqmc_set_fields(walker.m_batch, 0, model.m_bosonic_field.data()), ctx, "dqmc_set_fields");
         // the device continues the global engine's stream (Metropolis draws follow the field draws)
         dqmc_detail::check(dqmc_set_rng_state(walker.m_batch, 0, Utils::Random::state()), ctx, "dqmc_set_rng_state");
@@ -412,8 +494,13 @@ class Dqmc {
     // src/dqmc.cpp:48-70
     static void sweep_forth_and_back(DqmcWalker& walker, Model::ModelBase& model, Lattice::LatticeBase& lattice,
                                      Measure::MeasureHandler& meas) {
-        walker.sweep_from_0_to_beta(model);
-        if (meas.isEqualTime()) meas.equaltime_measure(walker, model, lattice);
+        if (meas.isDynamic()) {
+            walker.sweep_for_dynamic_greens(model);
+            meas.dynamic_measure(walker, model, lattice);
+        } else {
+            walker.sweep_from_0_to_beta(model);
+            if (meas.isEqualTime()) meas.equaltime_measure(walker, model, lattice);
+        }
         walker.sweep_from_beta_to_0(model);
         if (meas.isEqualTime()) meas.equaltime_measure(walker, model, lattice);
     }
@@ -428,7 +515,7 @@ class Dqmc {
     // src/dqmc.cpp:108-151
     static void measure(DqmcWalker& walker, Model::ModelBase& model, Lattice::LatticeBase& lattice,
                         Measure::MeasureHandler& meas) {
-        if (!meas.isEqualTime()) return;
+        if (!meas.isEqualTime() && !meas.isDynamic()) return;
         for (int bin = 0; bin < meas.BinsNum(); ++bin) {
             for (int s = 1; s <= meas.BinsSize() / 2; ++s) sweep_forth_and_back(walker, model, lattice, meas);
             meas.normalize_stats();

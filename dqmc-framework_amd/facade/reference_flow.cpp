@@ -3,7 +3,11 @@
 // initial_dqmc, thermalize, measure, analyse), on the MI355X engine through
 // the C++ facade.  Prints one JSON line with the equal-time observables.
 //
-//   reference_flow side time_size beta pace model(0|1) checkerboard(0|1) U mu seed warmup bins bin_size between
+//   reference_flow side time_size beta pace model(0|1) checkerboard(0|1) U mu seed warmup bins bin_size between [dynamic]
+//
+// With dynamic = 1 the handler also measures the four dynamic observables over every k-star
+// (the reference's momentum_list = "KstarsAll"), so each measuring forth-and-back starts with
+// the time-displaced sweep (src/dqmc.cpp:48-70).
 #include <cstdio>
 #include <cstdlib>
 #include <memory>
@@ -11,10 +15,12 @@
 #include "dqmc_framework.hpp"
 
 int main(int argc, char** argv) {
-    if (argc != 14) {
-        std::fprintf(stderr, "usage: %s side time_size beta pace model cb U mu seed warmup bins bin_size between\n", argv[0]);
+    if (argc != 14 && argc != 15) {
+        std::fprintf(stderr, "usage: %s side time_size beta pace model cb U mu seed warmup bins bin_size between [dynamic]\n",
+                     argv[0]);
         return 2;
     }
+    const bool dynamic = argc == 15 && std::atoi(argv[14]) != 0;
     const int side = std::atoi(argv[1]), time_size = std::atoi(argv[2]);
     const double beta = std::atof(argv[3]);
     const int pace = std::atoi(argv[4]), model_type = std::atoi(argv[5]), cb = std::atoi(argv[6]);
@@ -39,9 +45,16 @@ int main(int argc, char** argv) {
 
     Measure::MeasureHandler meas;
     meas.set_measure_params(warmup, bins, bin_size, between);
-    meas.set_observables({"filling_number", "double_occupancy", "kinetic_energy", "momentum_distribution",
-                          "local_spin_corr", "spin_density_structure_factor", "charge_density_structure_factor",
-                          "s_wave_pairing_corr"});
+    Measure::ObsList obs = {"filling_number", "double_occupancy", "kinetic_energy", "momentum_distribution",
+                            "local_spin_corr", "spin_density_structure_factor", "charge_density_structure_factor",
+                            "s_wave_pairing_corr"};
+    if (dynamic) {
+        for (const auto& n : Measure::DynamicNames()) obs.push_back(n);
+        std::vector<int> ks(lattice.kStarsNum());
+        for (int k = 0; k < lattice.kStarsNum(); ++k) ks[k] = k;
+        meas.set_measured_momentum_list(ks);
+    }
+    meas.set_observables(obs);
     meas.set_measured_momentum(lattice.MPointIndex());
 
     CheckerBoard::Square checkerboard;
@@ -60,6 +73,17 @@ int main(int argc, char** argv) {
     std::printf("{\"wrap_error\": %.6e, \"sign\": %.1f, \"trace_g_up\": %.15f", walker.WrapError(), walker.ConfigSign(), tr);
     for (const auto& name : Measure::EqualTimeNames())
         std::printf(", \"%s\": [%.15f, %.15f]", name.c_str(), meas.mean(name), meas.error(name));
+    if (dynamic) {
+        for (const char* name : {"dynamic_sign", "superfluid_stiffness", "greens_functions", "density_of_states",
+                                 "dynamic_spin_susceptibility"}) {
+            const auto m = meas.dynamic_mean(name), e = meas.dynamic_error(name);
+            std::printf(", \"%s\": [[", name);
+            for (size_t q = 0; q < m.size(); ++q) std::printf("%s%.15e", q ? ", " : "", m[q]);
+            std::printf("], [");
+            for (size_t q = 0; q < e.size(); ++q) std::printf("%s%.15e", q ? ", " : "", e[q]);
+            std::printf("]]");
+        }
+    }
     std::printf("}\n");
     return 0;
 }

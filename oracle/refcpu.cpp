@@ -1322,7 +1322,7 @@ int main(int argc, char** argv) {
     int side = 4, slices = 80, pace = 10, kind = 0, cb = 0, warm = 100, bins = 10, bin_size = 20, between = 0;
     double beta = 4.0, t = 1.0, U = 4.0, mu = 0.0, max_seconds = -1.0;
     long long seed = 42;
-    int fact = 0;
+    int fact = 0, dynamic = 0;
     for (int a = 1; a + 1 < argc; a += 2) {
         std::string k = argv[a]; const char* v = argv[a + 1];
         if (k == "--side") side = atoi(v); else if (k == "--slices") slices = atoi(v);
@@ -1333,6 +1333,7 @@ int main(int argc, char** argv) {
         else if (k == "--bin-size") bin_size = atoi(v); else if (k == "--between") between = atoi(v);
         else if (k == "--seed") seed = atoll(v); else if (k == "--fact") fact = atoi(v);
         else if (k == "--max-seconds") max_seconds = atof(v);
+        else if (k == "--dynamic") dynamic = atoi(v);
     }
     Engine e;
     e.setup(side, slices, beta, pace, kind, cb != 0, t, U, mu, -1, fact);
@@ -1347,10 +1348,11 @@ int main(int argc, char** argv) {
         e.sweep_0_to_beta(); e.sweep_beta_to_0(); sweeps += 2;
         if (max_seconds > 0 && elapsed() > max_seconds) stop = true;
     }
-    vector<vector<double>> bin_vals;
+    vector<vector<double>> bin_vals, dyn_bins;   // dyn: |sign|, rho_s, gk (nk x L), dos (L), dss (L)
     for (int b = 0; b < bins && !stop; ++b) {
-        for (int s = 0; s < bin_size / 2; ++s) {
-            e.sweep_0_to_beta(); e.equaltime_measure();
+        for (int s = 0; s < bin_size / 2; ++s) {   // Dqmc::sweep_forth_and_back (src/dqmc.cpp:48-70)
+            if (dynamic) { e.sweep_dynamic(); e.dynamic_measure(); }
+            else { e.sweep_0_to_beta(); e.equaltime_measure(); }
             e.sweep_beta_to_0(); e.equaltime_measure();
             sweeps += 2;
         }
@@ -1360,6 +1362,15 @@ int main(int argc, char** argv) {
         v[0] = std::fabs(sg);
         bin_vals.push_back(v);
         e.clear_measure();
+        if (dynamic) {   // MeasureHandler::normalize_stats, dynamic branch (src/measure/measure_handler.cpp:191-206)
+            const double dsg = e.dsign / e.dcount, den = e.dcount * dsg;
+            vector<double> w = {std::fabs(dsg), e.drho / den};
+            for (double x : e.dgk) w.push_back(x / den);
+            for (double x : e.ddos) w.push_back(x / den);
+            for (double x : e.ddss) w.push_back(x / den);
+            dyn_bins.push_back(w);
+            e.clear_dynamic();
+        }
         for (int s = 0; s < between / 2; ++s) { e.sweep_0_to_beta(); e.sweep_beta_to_0(); sweeps += 2; }
         if (max_seconds > 0 && elapsed() > max_seconds) stop = true;
     }
@@ -1375,6 +1386,30 @@ int main(int argc, char** argv) {
         m /= nb; m2 /= nb;
         const double err = std::sqrt(std::max(0.0, m2 - m * m)) / std::sqrt((double)nb - 1);
         printf(", \"%s\": [%.12f, %.12f]", names[o], m, err);
+    }
+    if (dynamic && dyn_bins.size() > 1) {
+        const size_t nd = dyn_bins.size(), nv = dyn_bins[0].size();
+        vector<double> m(nv, 0.0), err(nv, 0.0);
+        for (size_t q = 0; q < nv; ++q) {
+            double a = 0.0, a2 = 0.0;
+            for (auto& w : dyn_bins) { a += w[q]; a2 += w[q] * w[q]; }
+            a /= nd; a2 /= nd;
+            m[q] = a;
+            err[q] = std::sqrt(std::max(0.0, a2 - a * a)) / std::sqrt((double)nd - 1);
+        }
+        auto arr = [&](const char* name, size_t o, size_t n) {
+            printf(", \"%s\": [[", name);
+            for (size_t q = 0; q < n; ++q) printf("%s%.12e", q ? ", " : "", m[o + q]);
+            printf("], [");
+            for (size_t q = 0; q < n; ++q) printf("%s%.12e", q ? ", " : "", err[o + q]);
+            printf("]]");
+        };
+        const size_t nk = e.dgk.size() / slices;
+        arr("dynamic_sign", 0, 1);
+        arr("superfluid_stiffness", 1, 1);
+        arr("greens_functions", 2, nk * slices);
+        arr("density_of_states", 2 + nk * slices, slices);
+        arr("dynamic_spin_susceptibility", 2 + nk * slices + slices, slices);
     }
     printf("}\n");
     return 0;

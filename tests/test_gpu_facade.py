@@ -7,6 +7,7 @@ import json
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
 import _oracle as O
@@ -45,3 +46,39 @@ def test_facade_driver_matches_oracle_driver(cfg):
         rm, re_ = ref[theirs]
         assert abs(m - rm) < 1e-9 * max(1.0, abs(rm)), (ours, m, rm)
         assert abs(e - re_) < 1e-8 * max(1.0, abs(re_)) + 1e-12, (ours, e, re_)
+
+
+DYN_NAMES = ["dynamic_sign", "superfluid_stiffness", "greens_functions", "density_of_states",
+             "dynamic_spin_susceptibility"]
+
+
+@pytest.mark.parametrize("cfg", [
+    # side L beta pace model cb U mu seed warm bins bin_size between
+    (4, 40, 3.0, 10, 0, 0, 4.0, -0.2, 42, 10, 3, 6, 2),
+    (4, 24, 2.0, 6, 1, 1, 4.0, -0.5, 7, 6, 3, 4, 0),
+])
+def test_facade_dynamic_driver_matches_oracle_driver(cfg):
+    # the measuring forth-and-back starts with the time-displaced sweep when the handler
+    # measures dynamic observables (src/dqmc.cpp:48-70); equal-time ones come from the
+    # backward sweeps only
+    print(f"[gpu-facade] dynamic {cfg}", flush=True)
+    side, L, beta, pace, model, cb, U, mu, seed, warm, bins, bin_size, between = cfg
+    args = [str(x) for x in cfg] + ["1"]
+    out = subprocess.run([DRIVER] + args, check=True, capture_output=True, text=True, timeout=600).stdout
+    got = json.loads(out.strip().splitlines()[-1])
+    orc = subprocess.run([ORACLE, "--side", str(side), "--slices", str(L), "--beta", str(beta), "--pace", str(pace),
+                          "--model", str(model), "--cb", str(cb), "--U", str(U), "--mu", str(mu),
+                          "--seed", str(seed), "--warm", str(warm), "--bins", str(bins),
+                          "--bin-size", str(bin_size), "--between", str(between), "--dynamic", "1"],
+                         check=True, capture_output=True, text=True, timeout=600).stdout
+    ref = json.loads(orc.strip().splitlines()[-1])
+    for ours, theirs in NAMES:
+        m, e = got[ours]
+        rm, re_ = ref[theirs]
+        assert abs(m - rm) < 1e-9 * max(1.0, abs(rm)), (ours, m, rm)
+    for name in DYN_NAMES:
+        m, e = (np.array(x) for x in got[name])
+        rm, re_ = (np.array(x) for x in ref[name])
+        assert m.shape == rm.shape, name
+        assert np.max(np.abs(m - rm)) < 1e-8, (name, np.max(np.abs(m - rm)))
+        assert np.max(np.abs(e - re_)) < 1e-7, (name, np.max(np.abs(e - re_)))
