@@ -26,7 +26,9 @@ EXPORTS = ["dqmc_context_create", "dqmc_context_destroy", "dqmc_context_error", 
            "dqmc_get_fields", "dqmc_set_rng_state", "dqmc_set_expk", "dqmc_get_expk", "dqmc_init", "dqmc_sweep",
            "dqmc_synchronize", "dqmc_last_sweep_ms", "dqmc_get_green", "dqmc_get_slice_green", "dqmc_get_slice_signs",
            "dqmc_get_accepts", "dqmc_get_info", "dqmc_get_measure", "dqmc_measure_recorded", "dqmc_clear_measure",
-           "dqmc_get_dynamic_green", "dqmc_get_dynamic_measure", "dqmc_num_kstars", "dqmc_k_gemm", "dqmc_autotune_delay",
+           "dqmc_get_dynamic_green", "dqmc_get_dynamic_measure", "dqmc_num_kstars", "dqmc_write_fields_file",
+           "dqmc_read_fields_file", "dqmc_write_kstars_file", "dqmc_write_tgrids_file", "dqmc_write_observable_file",
+           "dqmc_write_observable_bins_file", "dqmc_k_gemm", "dqmc_autotune_delay",
            "dqmc_server_create", "dqmc_server_submit", "dqmc_server_cancel", "dqmc_server_poll",
            "dqmc_server_wait_idle", "dqmc_server_get_stats", "dqmc_server_error", "dqmc_server_destroy"]
 
@@ -103,6 +105,13 @@ def lib():
         L.dqmc_get_dynamic_green.argtypes = [vp, ip, ip, ip, ip, vp]
         L.dqmc_get_dynamic_measure.argtypes = [vp, ip, ip, vp, vp, vp, vp, vp]
         L.dqmc_num_kstars.argtypes = [vp, ip, vp]
+        cp = C.c_char_p
+        L.dqmc_write_fields_file.argtypes = [cp, ip, ip, vp]
+        L.dqmc_read_fields_file.argtypes = [cp, ip, ip, vp]
+        L.dqmc_write_kstars_file.argtypes = [cp, ip]
+        L.dqmc_write_tgrids_file.argtypes = [cp, ip, C.c_double]
+        L.dqmc_write_observable_file.argtypes = [cp, ip, ip, ip, vp, vp]
+        L.dqmc_write_observable_bins_file.argtypes = [cp, ip, ip, ip, ip, vp]
         L.dqmc_k_gemm.argtypes = [vp, ip, ip, ip, ip, ip, ip, vp, vp, vp, C.c_double, vp, vp, vp]
         L.dqmc_autotune_delay.argtypes = [vp, C.POINTER(ChainParams), ip, vp, ip, ip, C.POINTER(ip), vp]
         L.dqmc_server_create.argtypes = [vp, C.c_size_t, ip, ip, C.POINTER(vp)]
@@ -315,6 +324,58 @@ class Batch:
         self.ctx.check(lib().dqmc_get_dynamic_measure(self.h, c, nk, _p(k_index), _p(gk), _p(dos), _p(dss), _p(sc)),
                        "dqmc_get_dynamic_measure")
         return dict(gk=gk.reshape(L, nk).T.copy(), dos=dos, dss=dss, rho=sc[0], sign=sc[1], count=sc[2])
+
+
+# ---------------------------------------------------------------------------
+# On-disk formats (QuantumMonteCarlo::DqmcIO, include/dqmc_io.h), written and read by the
+# library's host code
+# ---------------------------------------------------------------------------
+def _io(rc, what, path):
+    if rc != 0:
+        raise DqmcError(f"{what}({path!r}) failed with status {rc}")
+
+
+def write_fields(path, fields):
+    """DqmcIO::output_bosonic_fields: fields (L x N of +-1) to the reference's text format."""
+    f = np.ascontiguousarray(fields, dtype=np.int8)
+    _io(lib().dqmc_write_fields_file(os.fsencode(path), f.shape[0], f.shape[1], _p(f)), "dqmc_write_fields_file", path)
+
+
+def read_fields(path, time_size, space_size):
+    """DqmcIO::read_bosonic_fields_from_file for a chain of time_size x space_size fields."""
+    f = np.ones((time_size, space_size), np.int8)
+    _io(lib().dqmc_read_fields_file(os.fsencode(path), time_size, space_size, _p(f)), "dqmc_read_fields_file", path)
+    return f
+
+
+def write_kstars(path, side):
+    _io(lib().dqmc_write_kstars_file(os.fsencode(path), side), "dqmc_write_kstars_file", path)
+
+
+def write_tgrids(path, time_size, beta):
+    _io(lib().dqmc_write_tgrids_file(os.fsencode(path), time_size, float(beta)), "dqmc_write_tgrids_file", path)
+
+
+def write_observable(path, mean, err):
+    """DqmcIO::output_observable (file): a scalar, a vector or a matrix (n_k x L for G(k,t))."""
+    m = np.asfortranarray(np.atleast_1d(np.asarray(mean, dtype=np.float64)))
+    e = np.asfortranarray(np.atleast_1d(np.asarray(err, dtype=np.float64)))
+    rank = np.ndim(mean)
+    rows = m.shape[0] if rank >= 1 else 1
+    cols = m.shape[1] if rank == 2 else 1
+    _io(lib().dqmc_write_observable_file(os.fsencode(path), rank, rows, cols, _p(m), _p(e)),
+        "dqmc_write_observable_file", path)
+
+
+def write_bins(path, bins):
+    """DqmcIO::output_observable_in_bins: bins has the bin index first (n_bins, [rows, [cols]])."""
+    b = np.asarray(bins, dtype=np.float64)
+    rank = b.ndim - 1
+    rows = b.shape[1] if rank >= 1 else 1
+    cols = b.shape[2] if rank == 2 else 1
+    flat = np.ascontiguousarray(np.stack([np.asfortranarray(x).ravel(order="F") for x in b]) if rank == 2 else b)
+    _io(lib().dqmc_write_observable_bins_file(os.fsencode(path), rank, b.shape[0], rows, cols, _p(flat)),
+        "dqmc_write_observable_bins_file", path)
 
 
 def normalize_bin(sums, counts):

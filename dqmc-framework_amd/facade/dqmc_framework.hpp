@@ -13,6 +13,7 @@
 //   QuantumMonteCarlo::DqmcInitializer::initial_modules / initial_dqmc
 //                                   include/dqmc_initializer.h:47-78
 //   QuantumMonteCarlo::Dqmc::thermalize / measure / analyse   include/dqmc.h:61-73
+//   QuantumMonteCarlo::DqmcIO       include/dqmc_io.h (file outputs and the field file)
 //   Utils::Random::set_seed         include/random.h:12-17
 // Matrices are column-major std::vector<double> (Eigen::MatrixXd layout).
 // Errors: the reference asserts or prints and exits; here a failing engine
@@ -130,6 +131,7 @@ class Square : public LatticeBase {
 namespace QuantumMonteCarlo {
 class DqmcWalker;
 class DqmcInitializer;
+class DqmcIO;
 }  // namespace QuantumMonteCarlo
 
 namespace CheckerBoard {
@@ -157,8 +159,10 @@ class ModelBase {
     bool m_checkerboard{false};
     int m_space_size{}, m_time_size{};
     std::vector<int8_t> m_bosonic_field;  // (t, i) at t * N + i
+    dqmc_batch_t m_batch{nullptr};        // the chain on the device once initialized (fields live there)
     friend class QuantumMonteCarlo::DqmcWalker;
     friend class QuantumMonteCarlo::DqmcInitializer;
+    friend class QuantumMonteCarlo::DqmcIO;
 
    public:
     virtual ~ModelBase() = default;
@@ -432,6 +436,28 @@ class MeasureHandler {
         if (v.size() < o + n) throw std::logic_error("no dynamic statistics");
         return std::vector<double>(v.begin() + o, v.begin() + o + n);
     }
+    // bins of one observable, bin-major (the element order of dynamic_mean for dynamic ones)
+    std::vector<double> bins(const std::string& name) const {
+        std::vector<double> out;
+        bool dyn = false;
+        for (const auto& n : DynamicNames()) dyn = dyn || n == name;
+        if (dyn) {
+            for (const auto& b : m_dyn_bins) {
+                const auto v = slice(b, name);
+                out.insert(out.end(), v.begin(), v.end());
+            }
+        } else {
+            const int o = index(name);
+            for (const auto& b : m_bins) out.push_back(b[o]);
+        }
+        return out;
+    }
+    int BinsCount(const std::string& name) const {
+        for (const auto& n : DynamicNames())
+            if (n == name) return (int)m_dyn_bins.size();
+        return (int)m_bins.size();
+    }
+    int TimeSize() const { return m_time_size; }
     static int index(const std::string& name) {
         const auto& n = EqualTimeNames();
         for (size_t k = 0; k < n.size(); ++k)
@@ -486,6 +512,7 @@ class DqmcInitializer {
         dqmc_detail::check(dqmc_set_rng_state(walker.m_batch, 0, Utils::Random::state()), ctx, "dqmc_set_rng_state");
         dqmc_detail::check(dqmc_init(walker.m_batch), ctx, "dqmc_init");
         meas.m_batch = walker.m_batch;
+        model.m_batch = walker.m_batch;
     }
 };
 
@@ -528,6 +555,60 @@ class Dqmc {
         }
     }
     static void analyse(Measure::MeasureHandler& meas) { meas.analyse_stats(); }
+};
+
+// QuantumMonteCarlo::DqmcIO (include/dqmc_io.h:267-629): the file outputs of the reference's
+// main (src/dqmc_main.cpp:270-360) and its field file, written by the engine library's host code
+// in the reference's formats.  Path-based: the reference takes an open std::ofstream.
+class DqmcIO {
+    static void io(int rc, const std::string& what, const std::string& path) {
+        if (rc != DQMC_OK) throw std::runtime_error(what + "('" + path + "') failed with status " + std::to_string(rc));
+    }
+
+   public:
+    // the current fields: from the device once the chain is initialized
+    static void output_bosonic_fields(const std::string& path, const Model::ModelBase& model) {
+        std::vector<int8_t> f = model.m_bosonic_field;
+        if (model.m_batch)
+            dqmc_detail::check(dqmc_get_fields(model.m_batch, 0, f.data()), dqmc_detail::context(), "dqmc_get_fields");
+        io(dqmc_write_fields_file(path.c_str(), model.m_time_size, model.m_space_size, f.data()), "output_bosonic_fields",
+           path);
+    }
+    // before DqmcInitializer::initial_dqmc, as src/dqmc_main.cpp:148-160 does
+    static void read_bosonic_fields_from_file(const std::string& path, Model::ModelBase& model) {
+        io(dqmc_read_fields_file(path.c_str(), model.m_time_size, model.m_space_size, model.m_bosonic_field.data()),
+           "read_bosonic_fields_from_file", path);
+    }
+    static void output_k_stars(const std::string& path, const Lattice::LatticeBase& lattice) {
+        io(dqmc_write_kstars_file(path.c_str(), lattice.SideLength()), "output_k_stars", path);
+    }
+    static void output_imaginary_time_grids(const std::string& path, const DqmcWalker& walker) {
+        io(dqmc_write_tgrids_file(path.c_str(), walker.TimeSize(), walker.Beta()), "output_imaginary_time_grids", path);
+    }
+    // rank of each observable as the reference stores it (scalar, vector over t, matrix k x t)
+    static void shape(const Measure::MeasureHandler& meas, const std::string& name, int& rank, int& rows, int& cols) {
+        rank = 0; rows = 1; cols = 1;
+        if (name == "density_of_states" || name == "dynamic_spin_susceptibility") { rank = 1; rows = meas.TimeSize(); }
+        else if (name == "greens_functions") {
+            rank = 2; rows = (int)meas.MomentumList().size(); cols = meas.TimeSize();
+        }
+    }
+    static void output_observable(const std::string& path, const Measure::MeasureHandler& meas, const std::string& name) {
+        int rank, rows, cols;
+        shape(meas, name, rank, rows, cols);
+        std::vector<double> m, e;
+        if (rank == 0 && name != "superfluid_stiffness") { m = {meas.mean(name)}; e = {meas.error(name)}; }
+        else { m = meas.dynamic_mean(name); e = meas.dynamic_error(name); }
+        io(dqmc_write_observable_file(path.c_str(), rank, rows, cols, m.data(), e.data()), "output_observable", path);
+    }
+    static void output_observable_in_bins(const std::string& path, const Measure::MeasureHandler& meas,
+                                          const std::string& name) {
+        int rank, rows, cols;
+        shape(meas, name, rank, rows, cols);
+        const std::vector<double> b = meas.bins(name);
+        io(dqmc_write_observable_bins_file(path.c_str(), rank, meas.BinsCount(name), rows, cols, b.data()),
+           "output_observable_in_bins", path);
+    }
 };
 
 }  // namespace QuantumMonteCarlo

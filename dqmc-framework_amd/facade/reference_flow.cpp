@@ -3,24 +3,32 @@
 // initial_dqmc, thermalize, measure, analyse), on the MI355X engine through
 // the C++ facade.  Prints one JSON line with the equal-time observables.
 //
-//   reference_flow side time_size beta pace model(0|1) checkerboard(0|1) U mu seed warmup bins bin_size between [dynamic]
+//   reference_flow side time_size beta pace model(0|1) checkerboard(0|1) U mu seed warmup bins bin_size between
+//                  [dynamic [out_dir]]
 //
+// With out_dir the driver also writes the reference's output files there (src/dqmc_main.cpp:270-360):
+// fields.out, kstars.out, tgrids.out, swave.out / swave.bins.out and, with dynamic = 1,
+// dos, greens and dss (.out / .bins.out).
 // With dynamic = 1 the handler also measures the four dynamic observables over every k-star
 // (the reference's momentum_list = "KstarsAll"), so each measuring forth-and-back starts with
 // the time-displaced sweep (src/dqmc.cpp:48-70).
 #include <cstdio>
 #include <cstdlib>
 #include <memory>
+#include <string>
+#include <utility>
 
 #include "dqmc_framework.hpp"
 
 int main(int argc, char** argv) {
-    if (argc != 14 && argc != 15) {
-        std::fprintf(stderr, "usage: %s side time_size beta pace model cb U mu seed warmup bins bin_size between [dynamic]\n",
+    if (argc < 14 || argc > 16) {
+        std::fprintf(stderr,
+                     "usage: %s side time_size beta pace model cb U mu seed warmup bins bin_size between [dynamic [out_dir]]\n",
                      argv[0]);
         return 2;
     }
-    const bool dynamic = argc == 15 && std::atoi(argv[14]) != 0;
+    const bool dynamic = argc >= 15 && std::atoi(argv[14]) != 0;
+    const std::string out_dir = argc == 16 ? argv[15] : "";
     const int side = std::atoi(argv[1]), time_size = std::atoi(argv[2]);
     const double beta = std::atof(argv[3]);
     const int pace = std::atoi(argv[4]), model_type = std::atoi(argv[5]), cb = std::atoi(argv[6]);
@@ -66,6 +74,22 @@ int main(int argc, char** argv) {
     QuantumMonteCarlo::Dqmc::thermalize(walker, *model, lattice, meas);
     QuantumMonteCarlo::Dqmc::measure(walker, *model, lattice, meas);
     QuantumMonteCarlo::Dqmc::analyse(meas);
+
+    if (!out_dir.empty()) {
+        using QuantumMonteCarlo::DqmcIO;
+        DqmcIO::output_bosonic_fields(out_dir + "/fields.out", *model);
+        DqmcIO::output_k_stars(out_dir + "/kstars.out", lattice);
+        DqmcIO::output_imaginary_time_grids(out_dir + "/tgrids.out", walker);
+        DqmcIO::output_observable(out_dir + "/swave.out", meas, "s_wave_pairing_corr");
+        DqmcIO::output_observable_in_bins(out_dir + "/swave.bins.out", meas, "s_wave_pairing_corr");
+        if (dynamic) {
+            for (const auto& [file, name] : {std::pair<std::string, std::string>{"dos", "density_of_states"},
+                                             {"greens", "greens_functions"}, {"dss", "dynamic_spin_susceptibility"}}) {
+                DqmcIO::output_observable(out_dir + "/" + file + ".out", meas, name);
+                DqmcIO::output_observable_in_bins(out_dir + "/" + file + ".bins.out", meas, name);
+            }
+        }
+    }
 
     const auto g = walker.GreenttUp();
     double tr = 0.0;

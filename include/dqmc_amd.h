@@ -44,6 +44,7 @@ extern "C" {
 #define DQMC_ERR_MEMORY -3   /* device allocation failed */
 #define DQMC_ERR_STATE -4    /* call out of order (e.g. sweep before init) */
 #define DQMC_ERR_NUMERIC -5  /* non-finite input or a numerical failure on the device */
+#define DQMC_ERR_IO -6       /* a file could not be opened, written or parsed */
 
 /* models (Model::RepulsiveHubbard / Model::AttractiveHubbard) */
 #define DQMC_MODEL_REPULSIVE 0
@@ -191,6 +192,25 @@ int dqmc_get_dynamic_measure(dqmc_batch_t b, int chain, int n_k, const int* k_in
 /* number of k-stars of the chain's lattice; kxy (may be NULL) receives (kx, ky) per k-star
  * (Lattice::Square::kStars, src/lattice/square.cpp:42-66) */
 int dqmc_num_kstars(dqmc_batch_t b, int chain, double* kxy);
+
+/* ------------------------------------------------------------------------
+ * On-disk formats of the reference (QuantumMonteCarlo::DqmcIO, include/dqmc_io.h).
+ * Host-only (no device work); byte-compatible with the reference's files.
+ * ------------------------------------------------------------------------ */
+/* HS-field file (DqmcIO::output_bosonic_fields, include/dqmc_io.h:478-534): "L N" then one
+ * "t i s" line per field; fields[t * space_size + i] as in dqmc_set_fields */
+int dqmc_write_fields_file(const char* path, int time_size, int space_size, const int8_t* fields);
+/* DqmcIO::read_bosonic_fields_from_file (include/dqmc_io.h:537-623): DQMC_ERR_ARG when the
+ * header's sizes differ from the chain's, DQMC_ERR_IO for unreadable files or entries */
+int dqmc_read_fields_file(const char* path, int time_size, int space_size, int8_t* fields);
+/* DqmcIO::output_k_stars (include/dqmc_io.h:432-455) and output_imaginary_time_grids (:458-475) */
+int dqmc_write_kstars_file(const char* path, int side);
+int dqmc_write_tgrids_file(const char* path, int time_size, double beta);
+/* DqmcIO::output_observable, file branch (include/dqmc_io.h:305-355): rank 0 scalar, 1 vector
+ * (rows), 2 matrix (rows x cols, element (i, j) at i + rows j); mean, error, relative error */
+int dqmc_write_observable_file(const char* path, int rank, int rows, int cols, const double* mean, const double* err);
+/* DqmcIO::output_observable_in_bins (include/dqmc_io.h:365-429): bins[b * size + element] */
+int dqmc_write_observable_bins_file(const char* path, int rank, int n_bins, int rows, int cols, const double* bins);
 
 /* Delayed-update depth autotuner (no counterpart in the reference, which
  * applies every flip at once): times `sweeps` sweeps of `n_chains` chains of

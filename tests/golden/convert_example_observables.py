@@ -5,6 +5,7 @@ forth-and-back sweeps on 10 ranks) into example_observables_L160_N64.npz:
   dos, dss        (2, 160)      mean and error of D(tau), DSS(tau)      (output/dos.out, dss.out)
   greens          (2, 15, 160)  mean and error of G(k,tau) per k-star   (output/greens.out)
   scalar_names, scalars (n, 2)  equal-time observables and rho_s       (log.out)
+  dos_bins        (20, 160)     the bins of D(tau)                      (output/dos.bins.out)
 
 The text formats are those written by include/dqmc_io.h:267-429 (a header with the sizes, then
 one line per entry: indices, mean, error, relative error).  Run from the repository root with the
@@ -49,6 +50,17 @@ def read_matrix(path):
     return out
 
 
+def read_vector_bins(path):
+    with open(path) as f:
+        nb, n = (int(x) for x in f.readline().split()[:2])
+        out = np.zeros((nb, n))
+        for line in f:
+            p = line.split()
+            if len(p) == 3:
+                out[int(p[0]), int(p[1])] = float(p[2])
+    return out
+
+
 def read_log(path):
     names, vals = [], []
     pat = re.compile(r"^\s*(.+?)\s+->\s+(\S+)\s+pm\s+(\S+)")
@@ -68,5 +80,6 @@ if __name__ == "__main__":
                         dos=read_vector(os.path.join(ex, "output", "dos.out")),
                         dss=read_vector(os.path.join(ex, "output", "dss.out")),
                         greens=read_matrix(os.path.join(ex, "output", "greens.out")),
-                        scalar_names=names, scalars=vals)
+                        scalar_names=names, scalars=vals,
+                        dos_bins=read_vector_bins(os.path.join(ex, "output", "dos.bins.out")))
     print(list(names))

@@ -57,13 +57,13 @@ DYN_NAMES = ["dynamic_sign", "superfluid_stiffness", "greens_functions", "densit
     (4, 40, 3.0, 10, 0, 0, 4.0, -0.2, 42, 10, 3, 6, 2),
     (4, 24, 2.0, 6, 1, 1, 4.0, -0.5, 7, 6, 3, 4, 0),
 ])
-def test_facade_dynamic_driver_matches_oracle_driver(cfg):
+def test_facade_dynamic_driver_matches_oracle_driver(cfg, tmp_path):
     # the measuring forth-and-back starts with the time-displaced sweep when the handler
     # measures dynamic observables (src/dqmc.cpp:48-70); equal-time ones come from the
     # backward sweeps only
     print(f"[gpu-facade] dynamic {cfg}", flush=True)
     side, L, beta, pace, model, cb, U, mu, seed, warm, bins, bin_size, between = cfg
-    args = [str(x) for x in cfg] + ["1"]
+    args = [str(x) for x in cfg] + ["1", str(tmp_path)]
     out = subprocess.run([DRIVER] + args, check=True, capture_output=True, text=True, timeout=600).stdout
     got = json.loads(out.strip().splitlines()[-1])
     orc = subprocess.run([ORACLE, "--side", str(side), "--slices", str(L), "--beta", str(beta), "--pace", str(pace),
@@ -82,3 +82,19 @@ def test_facade_dynamic_driver_matches_oracle_driver(cfg):
         assert m.shape == rm.shape, name
         assert np.max(np.abs(m - rm)) < 1e-8, (name, np.max(np.abs(m - rm)))
         assert np.max(np.abs(e - re_)) < 1e-7, (name, np.max(np.abs(e - re_)))
+    # the reference's output files (src/dqmc_main.cpp:270-360), written by the facade's DqmcIO
+    import dqmc_amd as D
+    f = D.read_fields(str(tmp_path / "fields.out"), L, side * side)
+    assert set(np.unique(f)) <= {-1, 1}
+    sw = [float(x) for x in open(tmp_path / "swave.out").read().split()]
+    assert abs(sw[0] - got["s_wave_pairing_corr"][0]) < 1e-9 * max(1.0, abs(sw[0]))
+    lines = open(tmp_path / "dos.out").read().splitlines()
+    assert int(lines[0]) == L and len(lines) == L + 1
+    dos = np.array([float(x.split()[1]) for x in lines[1:]])
+    assert np.max(np.abs(dos - np.array(got["density_of_states"][0]))) < 1e-9
+    head = open(tmp_path / "greens.out").readline().split()
+    nk = (side // 2 + 1) * (side // 2 + 2) // 2
+    assert [int(x) for x in head] == [nk, L]
+    head = open(tmp_path / "dss.bins.out").readline().split()
+    assert [int(x) for x in head] == [bins, L]
+    assert open(tmp_path / "tgrids.out").readline().split()[0] == str(L)
