@@ -87,6 +87,22 @@ def algorithmic_flops_per_chain_sweep(w, accepted_per_chain_sweep):
     return (wrap + prod + upd + stab) * spins / 2.0
 
 
+def algorithmic_flops_per_dynamic_sweep(w):
+    """Per chain FLOPs of a time-displaced sweep (both spins; attractive counts one spin):
+    the wrap of G(t,t), three slice products per slice (G(t,0), G(0,t) and the stack's
+    product), and per stabilization the push plus a recompute with 2N right-hand sides and two
+    more N^3 products than the equal-time one (41.3 n_s N^3 for both spins)."""
+    N = w["side"] ** 2
+    L = w["time_size"]
+    ns = -(-L // w["pace"])
+    spins = 1 if w["model"] == D.ATTRACTIVE else 2
+    if w["checkerboard"]:
+        wrap, prod = 64.0 * L * N * N, 32.0 * L * N * N
+    else:
+        wrap, prod = 8.0 * L * N ** 3, 4.0 * L * N ** 3
+    return (wrap + 3.0 * prod + 41.3 * ns * N ** 3) * spins / 2.0
+
+
 def autotune(w, candidates=(8, 16, 32, 64)):
     """Delayed-update depth for the workload's chain shape, chosen by the engine's
     autotuner (each candidate timed on a full batch of the workload's size)."""
@@ -162,8 +178,10 @@ def run_dynamic(w, steps, chains=None, profile=False):
         b.sweep(1, measure=True, dynamic=True)
     b.synchronize()
     n = b.n
-    out = {"chains": n, "ms_per_dynamic_sweep": sum(dyn_ms) / len(dyn_ms),
-           "chain_dynamic_sweeps_per_s": n / (sum(dyn_ms) / len(dyn_ms) * 1e-3),
+    kms = sum(dyn_ms) / len(dyn_ms)
+    achieved = algorithmic_flops_per_dynamic_sweep(w) * n / (kms * 1e-3) / 1e12
+    out = {"chains": n, "ms_per_dynamic_sweep": kms, "chain_dynamic_sweeps_per_s": n / (kms * 1e-3),
+           "achieved_tflops": achieved, "frac_fp64_peak": achieved / PEAK_FP64_TFLOPS,
            "max_wrap_error": max(b.info(c).wrap_error for c in range(n)),
            "measurements_per_chain": b.dynamic_accum(0)["count"]}
     if profile:
