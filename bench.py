@@ -162,6 +162,48 @@ def phase_breakdown(w, chains, sweeps=2):
     return out
 
 
+def phase_rooflines(w, phases, chains, accepted):
+    """Chip-level rate of each sweep phase: its algorithmic work per chain-sweep x chains
+    over the phase's device time per chain-sweep (the chains run concurrently, one per CU).
+    HBM-bound phases in GB/s of algorithmic bytes (G read and written per pass), MFMA-bound ones
+    in TFLOP/s.  The Metropolis phase is priced at the reference's unblocked rank-1 updates
+    (2 N^2 FLOP and 16 N^2 B per spin and accepted flip), which the delayed block updates
+    replace; the same phase's executed MFMA work (2 N^3 + 2 k N^2 per slice and spin) is given
+    beside it."""
+    N = w["side"] ** 2
+    L = w["time_size"]
+    ns = -(-L // w["pace"])
+    spins = 1 if w["model"] == D.ATTRACTIVE else 2
+    k = min(w["delay"], 32)
+    cb = w["checkerboard"]
+    out = {}
+
+    def rate(name, amount, unit, peak):
+        ms = phases.get(name, 0.0)
+        if ms <= 0:
+            return
+        a = amount * chains / (ms * 1e-3) / (1e9 if unit == "GB/s" else 1e12)
+        out[name] = {"achieved": a, "unit": unit, "peak": peak, "frac": a / peak, "per_chain_sweep": amount}
+
+    if cb:
+        rate("wrap", 4 * 8.0 * N * N * L * spins, "GB/s", PEAK_HBM_GBS)
+        rate("product", 2 * 8.0 * N * N * L * spins, "GB/s", PEAK_HBM_GBS)
+    else:
+        rate("wrap", 8.0 * L * N ** 3 * spins / 2, "TFLOP/s", PEAK_FP64_TFLOPS)
+        rate("product", 4.0 * L * N ** 3 * spins / 2, "TFLOP/s", PEAK_FP64_TFLOPS)
+    rate("metropolis", 16.0 * accepted * N * N * spins, "GB/s", PEAK_HBM_GBS)
+    if "metropolis" in out:
+        blocks = L * spins * (2.0 * N ** 3 + 2.0 * k * N * N)
+        out["metropolis"]["executed_mfma_tflops"] = blocks * chains / (phases["metropolis"] * 1e-3) / 1e12
+    stab_ms = phases.get("push", 0.0) + phases.get("fresh", 0.0)
+    if stab_ms > 0:
+        amount = 29.3 * ns * N ** 3 * spins / 2
+        a = amount * chains / (stab_ms * 1e-3) / 1e12
+        out["push+fresh"] = {"achieved": a, "unit": "TFLOP/s", "peak": PEAK_FP64_TFLOPS, "frac": a / PEAK_FP64_TFLOPS,
+                             "per_chain_sweep": amount}
+    return out
+
+
 def run_dynamic(w, steps, chains=None, profile=False):
     """The measuring schedule with time-displaced observables (src/dqmc.cpp:48-70): each
     forth-and-back is a dynamic sweep (fields fixed, G(t,0) and G(0,t) with G(t,t), dynamic
@@ -300,6 +342,8 @@ def main():
         print("[bench] phase breakdown", file=sys.stderr, flush=True)
         out["phases_ms_per_chain_sweep"] = {"batched": phase_breakdown(w, r["chains"]),
                                             "single": phase_breakdown(w, 1)}
+        out["phase_rooflines"] = phase_rooflines(w, out["phases_ms_per_chain_sweep"]["batched"], r["chains"],
+                                                 r["accepted"])
     if args.dynamic_steps > 0:
         print(f"[bench] time-displaced sweeps: {args.dynamic_steps}", file=sys.stderr, flush=True)
         out["dynamic"] = run_dynamic(w, args.dynamic_steps, profile=args.phases)
