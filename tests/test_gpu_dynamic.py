@@ -89,6 +89,8 @@ CASES = [
     (4, 24, 2.0, 5, 1, True, 4.0, -0.5),      # attractive checkerboard
     (2, 16, 1.0, 4, 0, False, 6.0, 0.2),      # side 2
     (4, 10, 1.0, 10, 0, False, 4.0, 0.1),     # one stack level: the recompute at t = beta only
+    (3, 1, 0.2, 1, 0, False, 2.0, 0.1),       # a single slice
+    (5, 12, 1.2, 5, 0, False, 4.0, 0.0),      # odd side (site maps, the stiffness k-star of an odd lattice)
 ]
 
 
@@ -142,12 +144,13 @@ def test_batched_dynamic_sweeps(ctx):
         assert np.array_equal(alone.dynamic_accum(0)["gk"], a["gk"])
 
 
-def test_free_fermions_at_the_production_size(ctx):
-    # U = 0 at 12 x 12, beta 6, 60 slices (dense): G(t,0) and G(0,t) have closed forms in the
-    # plane-wave basis at every slice, whatever the fields
-    side, L, beta, mu = 12, 60, 6.0, -0.3
+@pytest.mark.parametrize("side,L,beta", [(12, 60, 6.0), (21, 16, 2.0)])
+def test_free_fermions_closed_form_on_the_gpu(ctx, side, L, beta):
+    # U = 0 at the production size (12 x 12, beta 6, 60 slices) and the largest lattice (21 x 21,
+    # N = 441): G(t,0) and G(0,t) have closed forms in the plane-wave basis at every slice
+    mu = -0.3
     n = side * side
-    b = D.Batch(ctx, [D.make_params(side, L, beta, pace=10, U=0.0, mu=mu, seed=1)], record=True)
+    b = D.Batch(ctx, [D.make_params(side, L, beta, pace=10 if side == 12 else 4, U=0.0, mu=mu, seed=1)], record=True)
     b.fields_random(); b.init()
     b.sweep(1, record=True, measure=True, dynamic=True)
     lat = O.lattice(side)
@@ -160,7 +163,7 @@ def test_free_fermions_at_the_production_size(ctx):
         g0t = -(V * (np.exp(-(beta - tau * dtau) * eps) / (1 + np.exp(-beta * eps)))) @ V.T
         worst = max(worst, np.max(np.abs(b.dynamic_green(0, tau - 1, 1, 0) - gt0)),
                     np.max(np.abs(b.dynamic_green(0, tau - 1, -1, 1) - g0t)))
-    print(f"[gpu-dynamic] free fermions 12x12: max error {worst:.2e}", flush=True)
+    print(f"[gpu-dynamic] free fermions {side}x{side}: max error {worst:.2e}", flush=True)
     assert worst < 1e-10
     acc = b.dynamic_accum(0)
     ks = b.kstars(0)
