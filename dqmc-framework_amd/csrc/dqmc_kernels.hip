@@ -2535,9 +2535,17 @@ __device__ __noinline__ void dyn_measure_entry(Desc d, int t, double cs, Lds&) {
         abar[i] += spin_w * a;
         v[0] += spin_w * c;
     }
+    // G(t,0) columns and G(0,t) rows of one lattice row of sites fit in LDS up to side 18
+    const int side = sm.side;
+    const bool staged = 2 * side * N <= PANEL_DOUBLES;
+    for (int i = tid(); i < N; i += NT) s.v0[i] = gptr(d.rhoq)[i];
+    __syncthreads();
+    const AS_L double* rq = (const AS_L double*)s.v0;
+    // p / N by multiply-high: exact for p < N^2 <= 448^2 since p (ceil(2^32/N) - 2^32/N) / 2^32 < 1/N
+    const unsigned mgN = (unsigned)((0x100000000ULL + N - 1) / N);
     // displacement sums of the spin-averaged G(t,0) (G(0,0) at entry 0): thread r owns gr[t][r]
     // and walks the columns, eight of them per round with their loads in flight together
-    {
+    if (!staged || t == 0) {
         const AS_G double* src = t == 0 ? gtt : gt0;
         for (int r = tid(); r < N; r += NT) {
             double g = 0.0;
@@ -2558,36 +2566,88 @@ __device__ __noinline__ void dyn_measure_entry(Desc d, int t, double cs, Lds&) {
             acc[(size_t)t * N + r] += cs * (ns == 2 ? 0.5 * g : g);
         }
     }
-    // G(0,t)^T of each spin into W, so that the stiffness pairs read both matrices along columns
-    for (int sp = 0; sp < ns; ++sp)
-        wg_transpose_scaled(d.W + sp * n2, nullptr, d.G0t + sp * n2, nullptr, N, nullptr, nullptr, s);
-    for (int i = tid(); i < N; i += NT) s.v0[i] = gptr(d.rhoq)[i];
-    __syncthreads();
-    const AS_L double* rq = (const AS_L double*)s.v0;
-    // p / N by multiply-high: exact for p < N^2 <= 448^2 since p (ceil(2^32/N) - 2^32/N) / 2^32 < 1/N
-    const unsigned mgN = (unsigned)((0x100000000ULL + N - 1) / N);
-    for (int sp = 0; sp < ns; ++sp) {
-        const AS_G double* g0tT = gptr(d.W + sp * n2);   // g0t(a,b) at a*N + b
-        const AS_G double* gt = gt0 + sp * n2;            // gt0(a,b) at b*N + a
-        constexpr int MU = 4;
-        for (int p0 = tid(); p0 < N * N; p0 += MU * NT) {
-            double q[MU][8];
-            double f[MU];
+    if (staged) {
+        // per block of R lattice rows (sites i0 .. i0 + R side - 1, closed under i -> i + x):
+        // A[c N + j] = G(t,0)(j, i0 + c) (contiguous columns), B[c N + b] = G(0,t)(i0 + c, b)
+        // (pieces of every column), so each site pair reads both matrices from LDS; for entries
+        // t >= 1 the same columns give the displacement sums
+        const int rows = max(1, min(side, PANEL_DOUBLES / (2 * side * N)));
+        const unsigned mgS = (unsigned)((0x100000000ULL + side - 1) / side);   // e / side for e < 2^16
+        double gb[2] = {0.0, 0.0};   // bins r = tid and tid + NT of entry t (N <= 448 < 2 NT)
+        for (int sp = 0; sp < ns; ++sp) {
+            const AS_G double* gt = gt0 + sp * n2;
+            const AS_G double* gz = g0t + sp * n2;
+            for (int y0 = 0; y0 < side; y0 += rows) {
+                const int i0 = y0 * side, nc = min(rows, side - y0) * side, sN = nc * N;
+                AS_L double* A = (AS_L double*)s.panel;
+                AS_L double* B = A + sN;
+                const unsigned mgC = (unsigned)((0x100000000ULL + nc - 1) / nc);   // e / nc for e < 2^16
+                wg_stream2(sN, [&](int e) { return gt[(size_t)i0 * N + e]; }, [&](int e, double x) { A[e] = x; },
+                           [&](int e) {
+                               const int bcol = (int)__umulhi((unsigned)e, mgC), c = e - bcol * nc;
+                               return gz[(size_t)bcol * N + i0 + c];
+                           },
+                           [&](int e, double x) {
+                               const int bcol = (int)__umulhi((unsigned)e, mgC), c = e - bcol * nc;
+                               B[c * N + bcol] = x;
+                           });
+                __syncthreads();
+                for (int e = tid(); e < sN; e += NT) {
+                    const int c = (int)__umulhi((unsigned)e, mgN), j = e - c * N;
+                    const int cx = c - (int)__umulhi((unsigned)c, mgS) * side;
+                    const int cp = cx + 1 == side ? c + 1 - side : c + 1;
+                    const int jpx = sm.plus_x(j);
+                    const double f = rq[sm.disp(i0 + c, j)];
+                    // -g0t(ipx,jpx) gt0(j,i) + g0t(i,jpx) gt0(j,ipx) + g0t(ipx,j) gt0(jpx,i) - g0t(i,j) gt0(jpx,ipx)
+                    const double w = -B[cp * N + jpx] * A[c * N + j] + B[c * N + jpx] * A[cp * N + j]
+                                     + B[cp * N + j] * A[c * N + jpx] - B[c * N + j] * A[cp * N + jpx];
+                    v[1] += spin_w * f * w;
+                }
+                if (t > 0) {
 #pragma unroll
-            for (int u = 0; u < MU; ++u) {
-                const int p = min(p0 + u * NT, N * N - 1);
-                const int i = (int)__umulhi((unsigned)p, mgN), j = p - i * N;
-                const int ipx = sm.plus_x(i), jpx = sm.plus_x(j);
-                f[u] = rq[sm.disp(i, j)];
-                q[u][0] = g0tT[(size_t)ipx * N + jpx]; q[u][1] = gt[(size_t)i * N + j];
-                q[u][2] = g0tT[(size_t)i * N + jpx];   q[u][3] = gt[(size_t)ipx * N + j];
-                q[u][4] = g0tT[(size_t)ipx * N + j];   q[u][5] = gt[(size_t)i * N + jpx];
-                q[u][6] = g0tT[(size_t)i * N + j];     q[u][7] = gt[(size_t)ipx * N + jpx];
+                    for (int q = 0; q < 2; ++q) {
+                        const int r = tid() + q * NT;
+                        if (r < N)
+                            for (int c = 0; c < nc; ++c) gb[q] += A[c * N + sm.shift(i0 + c, r)];
+                    }
+                }
+                __syncthreads();
             }
+        }
+        if (t > 0) {
 #pragma unroll
-            for (int u = 0; u < MU; ++u)
-                if (p0 + u * NT < N * N)
-                    v[1] += spin_w * f[u] * (-q[u][0] * q[u][1] + q[u][2] * q[u][3] + q[u][4] * q[u][5] - q[u][6] * q[u][7]);
+            for (int q = 0; q < 2; ++q) {
+                const int r = tid() + q * NT;
+                if (r < N) acc[(size_t)t * N + r] += cs * (ns == 2 ? 0.5 * gb[q] : gb[q]);
+            }
+        }
+    } else {
+        // G(0,t)^T of each spin into W, so that the stiffness pairs read both matrices along columns
+        for (int sp = 0; sp < ns; ++sp)
+            wg_transpose_scaled(d.W + sp * n2, nullptr, d.G0t + sp * n2, nullptr, N, nullptr, nullptr, s);
+        for (int sp = 0; sp < ns; ++sp) {
+            const AS_G double* g0tT = gptr(d.W + sp * n2);   // g0t(a,b) at a*N + b
+            const AS_G double* gt = gt0 + sp * n2;            // gt0(a,b) at b*N + a
+            constexpr int MU = 4;
+            for (int p0 = tid(); p0 < N * N; p0 += MU * NT) {
+                double q[MU][8];
+                double f[MU];
+#pragma unroll
+                for (int u = 0; u < MU; ++u) {
+                    const int p = min(p0 + u * NT, N * N - 1);
+                    const int i = (int)__umulhi((unsigned)p, mgN), j = p - i * N;
+                    const int ipx = sm.plus_x(i), jpx = sm.plus_x(j);
+                    f[u] = rq[sm.disp(i, j)];
+                    q[u][0] = g0tT[(size_t)ipx * N + jpx]; q[u][1] = gt[(size_t)i * N + j];
+                    q[u][2] = g0tT[(size_t)i * N + jpx];   q[u][3] = gt[(size_t)ipx * N + j];
+                    q[u][4] = g0tT[(size_t)ipx * N + j];   q[u][5] = gt[(size_t)i * N + jpx];
+                    q[u][6] = g0tT[(size_t)i * N + j];     q[u][7] = gt[(size_t)ipx * N + jpx];
+                }
+#pragma unroll
+                for (int u = 0; u < MU; ++u)
+                    if (p0 + u * NT < N * N)
+                        v[1] += spin_w * f[u] * (-q[u][0] * q[u][1] + q[u][2] * q[u][3] + q[u][4] * q[u][5] - q[u][6] * q[u][7]);
+            }
         }
     }
     wg_sum8(v, 2, s);
