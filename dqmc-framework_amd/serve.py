@@ -10,6 +10,7 @@ measured in bins of 10 sweeps.  `scale` shrinks the sweep counts for short
 runs; the shape distribution is unchanged.
 
 Usage: python serve.py --jobs 200 --rate 20 --scale 0.1
+       python serve.py --configs a.toml b.toml ...   (jobs from the reference's config files)
 """
 import argparse
 import json
@@ -44,6 +45,20 @@ def generate_jobs(n, seed=2024, scale=1.0, precision=D.PREC_MIXED, checkerboard=
                           U=abs(U), mu=mu, seed=jid, precision=precision)
         jobs.append(dict(id=jid, params=p, warmup=warm, measure=meas, bin_size=bin_size, side=side, L=L,
                          beta=beta, U=U, mu=mu))
+    return jobs
+
+
+def jobs_from_configs(paths, precision=D.PREC_FP64):
+    """Serving jobs from TOML configuration files in the reference's format
+    (src/dqmc_initializer.cpp:27-338); job k uses seed k."""
+    import jobspec
+    jobs = []
+    for k, path in enumerate(paths):
+        spec = jobspec.load(path)
+        j = jobspec.job(spec, seed=k, precision=precision)
+        j.update(id=k, side=spec.side, L=spec.time_size, beta=spec.beta, U=spec.onsite_u, mu=spec.chemical_potential,
+                 config=path)
+        jobs.append(j)
     return jobs
 
 
@@ -99,8 +114,12 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=64.0)
     ap.add_argument("--lanes", type=int, default=3)
     ap.add_argument("--fp64", action="store_true", help="FP64 products instead of the mixed path")
+    ap.add_argument("--configs", nargs="+", default=None, help="TOML configuration files to serve as jobs")
     args = ap.parse_args()
-    jobs = generate_jobs(args.jobs, scale=args.scale, precision=D.PREC_FP64 if args.fp64 else D.PREC_MIXED)
+    if args.configs:
+        jobs = jobs_from_configs(args.configs, precision=D.PREC_FP64 if args.fp64 else D.PREC_MIXED)
+    else:
+        jobs = generate_jobs(args.jobs, scale=args.scale, precision=D.PREC_FP64 if args.fp64 else D.PREC_MIXED)
     ctx = D.Context(0)
     summary, _ = run(ctx, jobs, args.rate, int(args.budget_gb * 2 ** 30), lanes=args.lanes, log=sys.stderr)
     print(json.dumps(summary))

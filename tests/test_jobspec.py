@@ -84,3 +84,18 @@ def test_serving_job_from_spec():
     j = J.job(s, seed=5)
     assert j["warmup"] == 512 and j["bin_size"] == 100 and j["measure"] == 2000
     assert j["params"].seed == 5 and j["params"].side == 8
+
+
+def test_serving_jobs_from_config_files(tmp_path):
+    import serve
+    p = tmp_path / "small.toml"
+    p.write_text('[Model]\ntype = "AttractiveHubbard"\n[Model.Params]\nonsite_u = 3.0\n'
+                 '[Lattice]\ncell = [6, 6]\nmomentum = "MPoint"\nmomentum_list = "KstarsAll"\n'
+                 '[MonteCarlo]\nbeta = 2.0\ntime_size = 20\n'
+                 '[Measure]\nsweeps_warmup = 11\nbin_num = 3\nbin_size = 7\nobservables = ["double_occupancy"]\n')
+    jobs = serve.jobs_from_configs([os.path.join(HERE, "golden", "example_config.toml"), str(p)])
+    assert [j["id"] for j in jobs] == [0, 1]
+    j = jobs[1]
+    assert (j["warmup"], j["measure"], j["bin_size"]) == (10, 18, 6)     # whole forth-and-back iterations
+    assert j["params"].side == 6 and j["params"].time_size == 20 and j["params"].model == D.ATTRACTIVE
+    assert j["params"].onsite_u == 3.0 and j["params"].seed == 1
