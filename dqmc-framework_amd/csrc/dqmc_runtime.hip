@@ -663,11 +663,14 @@ int dqmc_get_dynamic_measure(dqmc_batch_t b, int c, int n_k, const int* k_index,
     const int side = h.p.side;
     // G(k,t) = 1/N sum_r FourierFactor(r, k) gr[t][r]  (src/measure/measure_methods.cpp:288-306),
     // element (k, t) at k + n_k t; D(t) = gr[t][0] / N (the trace, :313-325)
+    std::vector<double> ff((size_t)n_k * N);
+    for (int q = 0; q < n_k; ++q)
+        for (int r = 0; r < N; ++r)
+            ff[(size_t)q * N + r] = std::cos((-(r % side)) * h.kx[k_index[q]] - (r / side) * h.ky[k_index[q]]);
     for (int t = 0; t < L; ++t) {
         for (int q = 0; q < n_k; ++q) {
-            const int k = k_index[q];
             double g = 0.0;
-            for (int r = 0; r < N; ++r) g += std::cos((-(r % side)) * h.kx[k] - (r / side) * h.ky[k]) * gr[(size_t)t * N + r];
+            for (int r = 0; r < N; ++r) g += ff[(size_t)q * N + r] * gr[(size_t)t * N + r];
             greens_k[(size_t)t * n_k + q] = g / N;
         }
         if (dos) dos[t] = gr[(size_t)t * N] / N;
