@@ -1,5 +1,5 @@
 // refcpu — the oracle: a CPU FP64 restatement of the dqmc-framework equal-time
-// DQMC sweep, used ONLY as test infrastructure (parity checker in tests/,
+// DQMC sweep and its time-displaced sweep, used ONLY as test infrastructure (parity checker in tests/,
 // __graft_entry__.smoke() and the cpu_baseline leg of bench.py).  Nothing in
 // the shipped library links, loads or calls this file.
 //
@@ -25,7 +25,11 @@
 //     std::generate_canonical / std::bernoulli_distribution: from the C++
 //     standard's definitions [rand.eng.lcong], [rand.util.canonical].
 //
-// Parity pins for every function are listed in the tests (tests/test_oracle_*.py).
+// Parity pins for every function are listed in the tests (tests/test_oracle_*.py); the
+// time-displaced functions (compute_dynamic_greens, compute_greens_b0, the orthogonal-U form
+// compute_dynamic_greens_udt, Engine::sweep_dynamic, Engine::dynamic_measure) are pinned by
+// U = 0 closed forms and by time-ordered traces in the many-body Fock space of a 2x2 lattice
+// (tests/test_oracle_dynamic.py).
 // Built with -ffp-contract=off so the canonical-double computation of the
 // random stream is not fused into FMAs.
 
