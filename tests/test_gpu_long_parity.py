@@ -5,6 +5,7 @@ equal-time measurements agree to rounding.
 
 * 4x4 repulsive, U=4, mu=0, beta=4, 80 slices, dense, pace 10: 100 warm-up + 200 measured sweeps
 * 8x8 attractive, |U|=4, mu=-0.5, beta=8, 160 slices, checkerboard, pace 10, delay 16: 200 sweeps
+* the 256-chain 12x12 batch (checkerboard, FP64, delay 32): three of its chains over 20 sweeps
 """
 import numpy as np
 import pytest
@@ -63,4 +64,28 @@ def test_8x8_checkerboard_200_sweeps():
     info = b.info(0)
     assert info.rng == orc.rng_state and info.sign == orc.sign
     assert np.max(np.abs(b.green(0, 1) - orc.green(1))) < 1e-9
+    ctx.close()
+
+
+def test_batched_12x12_chains_match_the_oracle():
+    # the headline workload itself: 256 chains of 12x12 (checkerboard, FP64, delay 32) in one
+    # launch; three of them against the oracle over 20 sweeps.  Seeds 0 and 1 give the same
+    # minstd_rand0 stream, so chains 0 and 1 are the same chain.
+    print("[gpu-long] 256 x 12x12 batched: chains 0, 1, 255 vs the oracle over 20 sweeps", flush=True)
+    ctx = D.Context(0)
+    ps = [D.make_params(12, 60, 6.0, pace=10, checkerboard=True, delay=32, U=4.0, mu=0.0, seed=c) for c in range(256)]
+    b = D.Batch(ctx, ps)
+    b.fields_random(); b.init()
+    b.sweep(20)
+    for c in (0, 1, 255):
+        orc = O.Chain(12, 60, 6.0, pace=10, model=0, checkerboard=True, U=4.0, mu=0.0)
+        orc.seed(c); orc.fields_random(); orc.init()
+        for _ in range(10):
+            orc.sweep_forward(); orc.sweep_backward()
+        assert np.array_equal(b.fields(c).astype(np.float64), orc.fields()), c
+        info = b.info(c)
+        assert info.rng == orc.rng_state and info.sign == orc.sign, c
+        assert np.max(np.abs(b.green(c, -1) - orc.green(-1))) < 1e-10, c
+        print(f"[gpu-long] chain {c}: identical, wrap error {info.wrap_error:.2e}", flush=True)
+    assert np.array_equal(b.fields(0), b.fields(1))
     ctx.close()
