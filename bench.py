@@ -277,6 +277,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the single-chain comparison run")
     ap.add_argument("--phases", action="store_true", help="also report device time per sweep phase")
+    ap.add_argument("--precision", choices=["workload", "fp64", "mixed"], default="workload",
+                    help="override the workload's precision mode")
     ap.add_argument("--dynamic-steps", type=int, default=3,
                     help="time-displaced sweeps timed on the same batch (0 skips them)")
     ap.add_argument("--no-autotune", action="store_true", help="use the workload's fixed delay depth")
@@ -302,6 +304,9 @@ def main():
               f"but only the single-GPU figure is measured here (asked for --gpus {args.gpus}).", file=sys.stderr)
         sys.exit(2)
     w = dict(WORKLOADS[args.workload])
+    if args.precision != "workload":
+        w["precision"] = D.PREC_FP64 if args.precision == "fp64" else D.PREC_MIXED
+        w["desc"] += f" (precision override: {args.precision})"
     tune = None
     if not args.no_autotune and w["chains"] > 1:
         best, tune = autotune(w)
