@@ -104,6 +104,15 @@ struct Job {
     double cost = 1.0;  // FLOP per chain-sweep (model)
     double t_submit = 0, t_start = 0;
     std::vector<std::vector<double>> bins;
+    // DQMC_JOB_DYNAMIC: equal-time measured sweeps of the running bin (the backward ones) and
+    // the dynamic bins [|sign|, rho_s, G(k,t) (n_k x L), D(t), DSS(t)]
+    int eq_in_bin = 0;
+    std::vector<std::vector<double>> dyn_bins;
+};
+
+struct DynResult {
+    int n_k = 0, time_size = 0;
+    std::vector<double> mean, err;
 };
 
 struct Lane {
@@ -130,6 +139,7 @@ struct dqmc_server {
     std::condition_variable cv;
     std::vector<Job*> active;
     std::vector<dqmc_job_result> results;
+    std::map<long long, DynResult> dyn_results;
     std::thread worker;
     std::atomic<bool> stop{false};
     long long next_id = 1, submitted = 0, completed = 0, chain_sweeps = 0, launches = 0;
@@ -180,12 +190,29 @@ void finish_job(dqmc_server* s, Job* j, int status, hipStream_t stream) {
         r.mean[o] = m;
         r.error[o] = nb > 1 ? std::sqrt(std::max(0.0, m2 - m * m)) / std::sqrt((double)nb - 1) : 0.0;
     }
+    DynResult dr;
+    const bool dyn = (j->spec.options & DQMC_JOB_DYNAMIC) && !j->dyn_bins.empty();
+    if (dyn) {
+        const size_t nd = j->dyn_bins.size(), nv = j->dyn_bins[0].size();
+        dr.time_size = j->spec.chain.time_size;
+        dr.n_k = (int)((nv - 2 - 2 * (size_t)dr.time_size) / dr.time_size);
+        dr.mean.assign(nv, 0.0);
+        dr.err.assign(nv, 0.0);
+        for (size_t q = 0; q < nv; ++q) {   // Observable::analyse (include/measure/observable.h:169-193)
+            double m = 0.0, m2 = 0.0;
+            for (auto& v : j->dyn_bins) { m += v[q]; m2 += v[q] * v[q]; }
+            m /= nd; m2 /= nd;
+            dr.mean[q] = m;
+            dr.err[q] = nd > 1 ? std::sqrt(std::max(0.0, m2 - m * m)) / std::sqrt((double)nd - 1) : 0.0;
+        }
+    }
     if (j->b) {
         s->pool.release(j->off, j->bytes);
         s->used -= j->bytes;
         j->b.reset();
     }
     std::lock_guard<std::mutex> lk(s->mu);
+    if (dyn) s->dyn_results[j->id] = std::move(dr);
     s->results.push_back(r);
     s->completed += 1;
     s->active.erase(std::remove(s->active.begin(), s->active.end(), j), s->active.end());
@@ -238,6 +265,14 @@ void complete_lane(dqmc_server* s, Lane& ln) {
         s->rate = 0.8 * s->rate + 0.2 * r;
     }
     for (Job* j : ln.in_flight) {
+        const bool dyn = j->spec.options & DQMC_JOB_DYNAMIC;
+        if (j->launched_flags & DQMC_FLAG_MEASURE) {
+            // with the dynamic option the sweeps that start at slice 0 (even sweep count so far)
+            // are time-displaced and take no equal-time measurement
+            const int n = j->launched;
+            const int n_up = (j->done % 2 == 0) ? (n + 1) / 2 : n / 2;
+            j->eq_in_bin += dyn ? n - n_up : n;
+        }
         j->done += j->launched;
         s->chain_sweeps += j->launched;
         if (j->launched_flags & DQMC_FLAG_MEASURE) {
@@ -247,13 +282,42 @@ void complete_lane(dqmc_server* s, Lane& ln) {
                 hipMemcpyAsync(acc, j->b->descs[0].acc, sizeof(acc), hipMemcpyDeviceToHost, ln.stream);
                 hipStreamSynchronize(ln.stream);
                 hipMemsetAsync(j->b->descs[0].acc, 0, sizeof(acc), ln.stream);
-                const double cnt = (double)j->in_bin * j->spec.chain.time_size;
+                if (dyn) {
+                    const HostChain& h = j->b->chains[0];
+                    const int N = h.N, L = h.p.time_size, side = h.p.side, nk = (int)h.kx.size();
+                    std::vector<double> da(DQMC_DYN_ACC_SIZE(L, N));
+                    hipMemcpyAsync(da.data(), j->b->descs[0].dacc, da.size() * 8, hipMemcpyDeviceToHost, ln.stream);
+                    hipStreamSynchronize(ln.stream);
+                    hipMemsetAsync(j->b->descs[0].dacc, 0, da.size() * 8, ln.stream);
+                    // MeasureHandler::normalize_stats, dynamic branch (src/measure/measure_handler.cpp:191-206):
+                    // sums / (count <sign>) = sums / sign sum; G(k,t) and D(t) from the displacement sums
+                    const double ssum = da[(size_t)L * N + L + 1], cnt = da[(size_t)L * N + L + 2];
+                    std::vector<double> v;
+                    v.reserve(2 + (size_t)nk * L + 2 * (size_t)L);
+                    v.push_back(std::fabs(ssum / cnt));
+                    v.push_back(da[(size_t)L * N + L] / ssum);
+                    std::vector<double> ff((size_t)nk * N);
+                    for (int k = 0; k < nk; ++k)
+                        for (int r = 0; r < N; ++r)
+                            ff[(size_t)k * N + r] = std::cos((-(r % side)) * h.kx[k] - (r / side) * h.ky[k]);
+                    for (int t = 0; t < L; ++t)
+                        for (int k = 0; k < nk; ++k) {
+                            double g = 0.0;
+                            for (int r = 0; r < N; ++r) g += ff[(size_t)k * N + r] * da[(size_t)t * N + r];
+                            v.push_back(g / N / ssum);
+                        }
+                    for (int t = 0; t < L; ++t) v.push_back(da[(size_t)t * N] / N / ssum);
+                    for (int t = 0; t < L; ++t) v.push_back(da[(size_t)L * N + t] / ssum);
+                    j->dyn_bins.push_back(std::move(v));
+                }
+                const double cnt = (double)(dyn ? j->eq_in_bin : j->in_bin) * j->spec.chain.time_size;
                 std::vector<double> v(DQMC_NUM_OBSERVABLES);
                 const double sg = acc[0] / cnt;  // MeasureHandler::normalize_stats
                 for (int o = 1; o < DQMC_NUM_OBSERVABLES; ++o) v[o] = acc[o] / (cnt * sg);
                 v[0] = std::fabs(sg);
                 j->bins.push_back(v);
                 j->in_bin = 0;
+                j->eq_in_bin = 0;
             }
         }
         j->launched = 0;
@@ -294,7 +358,10 @@ bool launch_lane(dqmc_server* s, Lane& ln, int lane_index) {
         const dqmc_job& sp = j->spec;
         int phase_left, flags;
         if (j->done < sp.warmup_sweeps) { phase_left = (int)(sp.warmup_sweeps - j->done); flags = 0; }
-        else { phase_left = std::min(sp.bin_size - j->in_bin, (int)(j->total - j->done)); flags = DQMC_FLAG_MEASURE; }
+        else {
+            phase_left = std::min(sp.bin_size - j->in_bin, (int)(j->total - j->done));
+            flags = DQMC_FLAG_MEASURE | ((sp.options & DQMC_JOB_DYNAMIC) ? DQMC_FLAG_DYNAMIC : 0);
+        }
         const double est = j->cost / s->rate;  // seconds per sweep
         int n = (int)std::floor(s->target_s / est);
         n = std::max(1, std::min(n, phase_left));
@@ -394,6 +461,11 @@ int dqmc_server_submit(dqmc_server_t s, const dqmc_job* job, long long* id) {
         why = "sweep counts: warmup >= 0, measure >= 0 and a multiple of bin_size >= 1";
         rc = DQMC_ERR_ARG;
     }
+    if (rc == DQMC_OK && (job->options & ~DQMC_JOB_DYNAMIC)) { why = "unknown job option"; rc = DQMC_ERR_ARG; }
+    if (rc == DQMC_OK && (job->options & DQMC_JOB_DYNAMIC) && (job->bin_size % 2 != 0 || job->warmup_sweeps % 2 != 0)) {
+        why = "dynamic jobs measure whole forth-and-back iterations: even warmup and bin_size";
+        rc = DQMC_ERR_ARG;
+    }
     size_t bytes = 0;
     if (rc == DQMC_OK) rc = dqmc_batch_footprint(1, &job->chain, 0, &bytes);
     if (rc == DQMC_OK && bytes > s->budget) { why = "job footprint exceeds the server's device budget"; rc = DQMC_ERR_MEMORY; }
@@ -467,6 +539,18 @@ int dqmc_server_get_stats(dqmc_server_t s, dqmc_server_stats* out) {
     out->bytes_peak = s->peak;
     out->budget = s->budget;
     out->elapsed_s = now_s(s->t0);
+    return DQMC_OK;
+}
+
+int dqmc_server_get_dynamic(dqmc_server_t s, long long id, int* n_k, int* time_size, double* mean, double* err) {
+    if (!s) return DQMC_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->dyn_results.find(id);
+    if (it == s->dyn_results.end()) { s->err = "no dynamic results for this job"; return DQMC_ERR_ARG; }
+    if (n_k) *n_k = it->second.n_k;
+    if (time_size) *time_size = it->second.time_size;
+    if (mean) std::memcpy(mean, it->second.mean.data(), it->second.mean.size() * sizeof(double));
+    if (err) std::memcpy(err, it->second.err.data(), it->second.err.size() * sizeof(double));
     return DQMC_OK;
 }
 

@@ -30,7 +30,8 @@ EXPORTS = ["dqmc_context_create", "dqmc_context_destroy", "dqmc_context_error", 
            "dqmc_read_fields_file", "dqmc_write_kstars_file", "dqmc_write_tgrids_file", "dqmc_write_observable_file",
            "dqmc_write_observable_bins_file", "dqmc_k_gemm", "dqmc_autotune_delay",
            "dqmc_server_create", "dqmc_server_submit", "dqmc_server_cancel", "dqmc_server_poll",
-           "dqmc_server_wait_idle", "dqmc_server_get_stats", "dqmc_server_error", "dqmc_server_destroy"]
+           "dqmc_server_wait_idle", "dqmc_server_get_stats", "dqmc_server_error", "dqmc_server_destroy",
+           "dqmc_server_get_dynamic"]
 
 
 class ChainParams(C.Structure):
@@ -46,9 +47,12 @@ class ChainInfo(C.Structure):
                 ("phase_seconds", C.c_double * 8)]
 
 
+JOB_DYNAMIC = 1
+
+
 class Job(C.Structure):
     _fields_ = [("chain", ChainParams), ("warmup_sweeps", C.c_int), ("measure_sweeps", C.c_int),
-                ("bin_size", C.c_int)]
+                ("bin_size", C.c_int), ("options", C.c_int)]
 
 
 class JobResult(C.Structure):
@@ -123,6 +127,7 @@ def lib():
         L.dqmc_server_error.argtypes = [vp]
         L.dqmc_server_error.restype = C.c_char_p
         L.dqmc_server_destroy.argtypes = [vp]
+        L.dqmc_server_get_dynamic.argtypes = [vp, C.c_longlong, C.POINTER(ip), C.POINTER(ip), vp, vp]
         _lib = L
     return _lib
 
@@ -411,8 +416,8 @@ class Server:
         ctx.check(lib().dqmc_server_create(ctx.h, int(hbm_budget_bytes), int(max_active), int(lanes),
                                            C.byref(self.h)), "dqmc_server_create")
 
-    def submit(self, params, warmup, measure, bin_size):
-        j = Job(params, int(warmup), int(measure), int(bin_size))
+    def submit(self, params, warmup, measure, bin_size, dynamic=False):
+        j = Job(params, int(warmup), int(measure), int(bin_size), JOB_DYNAMIC if dynamic else 0)
         jid = C.c_longlong(0)
         rc = lib().dqmc_server_submit(self.h, C.byref(j), C.byref(jid))
         if rc != OK:
@@ -421,6 +426,23 @@ class Server:
 
     def cancel(self, jid):
         return lib().dqmc_server_cancel(self.h, int(jid)) == OK
+
+    def dynamic_result(self, jid):
+        """Mean and error of the dynamic observables of a finished dynamic job: dicts with
+        sign, rho, gk (n_k x L), dos (L), dss (L)."""
+        nk, L = C.c_int(0), C.c_int(0)
+        rc = lib().dqmc_server_get_dynamic(self.h, int(jid), C.byref(nk), C.byref(L), None, None)
+        if rc != OK:
+            raise DqmcError(f"dqmc_server_get_dynamic failed ({rc}): {lib().dqmc_server_error(self.h).decode()}")
+        n = 2 + nk.value * L.value + 2 * L.value
+        m = np.zeros(n); e = np.zeros(n)
+        lib().dqmc_server_get_dynamic(self.h, int(jid), C.byref(nk), C.byref(L), _p(m), _p(e))
+
+        def split(v):
+            k, t = nk.value, L.value
+            return dict(sign=v[0], rho=v[1], gk=v[2:2 + k * t].reshape(t, k).T.copy(),
+                        dos=v[2 + k * t:2 + k * t + t].copy(), dss=v[2 + k * t + t:].copy())
+        return split(m), split(e)
 
     def poll(self, max_results=256):
         buf = (JobResult * max_results)()

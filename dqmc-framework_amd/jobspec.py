@@ -155,10 +155,11 @@ def load(path=None, text=None, world_size=1):
 
 def job(spec, seed=0, precision=D.PREC_FP64):
     """A serving job (Server.submit arguments) for the specification: its warm-up, then
-    bin_num x bin_size measured sweeps in bins of bin_size (the server runs no sweeps between
-    bins and measures equal-time observables)."""
+    bin_num x bin_size measured sweeps in bins of bin_size, with the dynamic option when the
+    specification asks for dynamic observables (the server runs no sweeps between bins)."""
     return dict(params=spec.params(seed=seed, precision=precision), warmup=2 * (spec.sweeps_warmup // 2),
-                measure=spec.bin_num * 2 * (spec.bin_size // 2), bin_size=2 * (spec.bin_size // 2))
+                measure=spec.bin_num * 2 * (spec.bin_size // 2), bin_size=2 * (spec.bin_size // 2),
+                dynamic=spec.is_dynamic)
 
 
 def run(spec, ctx, chains=1, seed=0, fields=None, out_dir=None, log=None):

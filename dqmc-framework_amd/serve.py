@@ -77,7 +77,7 @@ def run(ctx, jobs, rate, budget_bytes, lanes=3, max_active=4096, arrival_seed=7,
         now = time.perf_counter() - t0
         while k < len(jobs) and now >= t_next:
             j = jobs[k]
-            sid = srv.submit(j["params"], j["warmup"], j["measure"], j["bin_size"])
+            sid = srv.submit(j["params"], j["warmup"], j["measure"], j["bin_size"], dynamic=j.get("dynamic", False))
             idmap[sid] = j["id"]
             k += 1
             if k < len(jobs):

@@ -227,11 +227,15 @@ int dqmc_autotune_delay(dqmc_ctx_t ctx, const dqmc_chain_params* shape, int n_ch
  * ------------------------------------------------------------------------ */
 typedef struct dqmc_server* dqmc_server_t;
 
+#define DQMC_JOB_DYNAMIC 1   /* measure the dynamic observables too: every measured 0->beta sweep is the
+                                time-displaced one (Dqmc::sweep_forth_and_back, src/dqmc.cpp:48-70) */
+
 typedef struct {
     dqmc_chain_params chain;
     int warmup_sweeps;    /* unmeasured single-direction sweeps */
     int measure_sweeps;   /* measured sweeps, a multiple of bin_size */
-    int bin_size;         /* measured sweeps per bin */
+    int bin_size;         /* measured sweeps per bin (even with DQMC_JOB_DYNAMIC) */
+    int options;          /* DQMC_JOB_* */
 } dqmc_job;
 
 typedef struct {
@@ -258,6 +262,11 @@ int dqmc_server_submit(dqmc_server_t s, const dqmc_job* job, long long* id);
 int dqmc_server_cancel(dqmc_server_t s, long long id);
 int dqmc_server_poll(dqmc_server_t s, dqmc_job_result* out, int max_results, int* n_out);
 int dqmc_server_wait_idle(dqmc_server_t s, double timeout_s);
+/* Dynamic results of a finished DQMC_JOB_DYNAMIC job (kept until the server is destroyed):
+ * mean and error over the bins of [|dynamic sign|, rho_s, G(k,t) over every k-star
+ * (n_k x L, k fastest), D(t) (L), DSS(t) (L)], 2 + n_k L + 2 L values each.  With mean = NULL
+ * only *n_k and *time_size are returned.  DQMC_ERR_ARG if the job has no dynamic results. */
+int dqmc_server_get_dynamic(dqmc_server_t s, long long id, int* n_k, int* time_size, double* mean, double* err);
 int dqmc_server_get_stats(dqmc_server_t s, dqmc_server_stats* out);
 const char* dqmc_server_error(dqmc_server_t s);
 int dqmc_server_destroy(dqmc_server_t s);

@@ -102,3 +102,46 @@ def test_cancel_frees_slot_and_leaves_others_unchanged(ctx):
     st = srv.stats()
     assert st["bytes_in_use"] == 0
     srv.close()
+
+
+def test_dynamic_job_in_the_mix_matches_batch_api(ctx):
+    # a job with the dynamic option (time-displaced sweeps opening each measuring iteration)
+    # served among equal-time jobs gives the batch API's dynamic and equal-time results
+    print("[gpu-serving] dynamic job in a mix", flush=True)
+    p = D.make_params(6, 24, 2.0, pace=6, checkerboard=True, U=3.0, mu=-0.1, seed=77)
+    warm, meas, bs = 6, 24, 8
+    srv = D.Server(ctx, 1 << 30)
+    others = small_jobs(60, seed=5)[:6]
+    assert len(others) >= 4
+    ids = [srv.submit(j["params"], j["warmup"], j["measure"], j["bin_size"]) for j in others[:3]]
+    jid = srv.submit(p, warm, meas, bs, dynamic=True)
+    ids += [srv.submit(j["params"], j["warmup"], j["measure"], j["bin_size"]) for j in others[3:]]
+    results = {}
+    t0 = time.time()
+    while len(results) < len(ids) + 1 and time.time() - t0 < 120:
+        for r in srv.poll():
+            results[r["id"]] = r
+        time.sleep(0.01)
+    assert len(results) == len(ids) + 1 and results[jid]["status"] == 0
+    with pytest.raises(D.DqmcError):
+        srv.submit(p, 5, 24, 8, dynamic=True)   # dynamic jobs measure whole forth-and-back iterations
+    dm, de = srv.dynamic_result(jid)
+    with pytest.raises(D.DqmcError):
+        srv.dynamic_result(ids[0])            # an equal-time job has no dynamic results
+    srv.close()
+    b = D.Batch(ctx, [p])
+    b.fields_random(); b.init(); b.sweep(warm)
+    eq_bins, dyn_bins = [], []
+    for _ in range(meas // bs):
+        b.clear_measure()
+        b.sweep(bs, measure=True, dynamic=True)
+        s, c = b.measure_accum(0)
+        eq_bins.append(D.normalize_bin(s, c))
+        a = b.dynamic_accum(0)
+        dyn_bins.append(np.concatenate([[abs(a["sign"] / a["count"]), a["rho"] / a["sign"]],
+                                        (a["gk"] / a["sign"]).T.ravel(), a["dos"] / a["sign"], a["dss"] / a["sign"]]))
+    em, _ = D.bin_stats(eq_bins)
+    assert np.allclose(results[jid]["mean"], em, rtol=1e-12, atol=1e-12)
+    m, e = D.bin_stats(dyn_bins)
+    ours = np.concatenate([[dm["sign"], dm["rho"]], dm["gk"].T.ravel(), dm["dos"], dm["dss"]])
+    assert np.allclose(ours, m, rtol=1e-12, atol=1e-12)
