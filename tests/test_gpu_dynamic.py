@@ -172,3 +172,20 @@ def test_free_fermions_closed_form_on_the_gpu(ctx, side, L, beta):
     ek = -2 * (np.cos(ks[:, 0]) + np.cos(ks[:, 1])) + mu
     for t in range(1, L):
         assert np.max(np.abs(acc["gk"][:, t] - np.exp(-t * dtau * ek) / (1 + np.exp(-beta * ek)))) < 1e-10
+
+
+def test_dynamic_accessor_errors(ctx):
+    b = D.Batch(ctx, [D.make_params(4, 20, 2.0, pace=5, seed=3)])
+    b.fields_random(); b.init(); b.sweep(2, measure=True, dynamic=True)
+    with pytest.raises(D.DqmcError):
+        b.dynamic_green(0, 0, 1, 0)                 # batch created without record buffers
+    with pytest.raises(D.DqmcError):
+        b.dynamic_accum(0, k_index=[0, 6])          # 4x4 has 6 k-stars: index 6 is out of range
+    a = b.dynamic_accum(0, k_index=[5, 0])          # any subset, any order
+    full = b.dynamic_accum(0)
+    assert np.array_equal(a["gk"][0], full["gk"][5]) and np.array_equal(a["gk"][1], full["gk"][0])
+    r = D.Batch(ctx, [D.make_params(4, 20, 2.0, pace=5, seed=3)], record=True)
+    r.fields_random(); r.init(); r.sweep(1, record=True, dynamic=True)
+    for bad in [(20, 1, 0), (0, 2, 0), (0, 1, 2)]:
+        with pytest.raises(D.DqmcError):
+            r.dynamic_green(0, *bad)
