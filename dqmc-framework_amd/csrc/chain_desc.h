@@ -18,6 +18,8 @@ enum {
                              // G(0,t) propagated and recomputed with G(t,t), dynamic observables
 };
 
+// building blocks run alone by the single-kernel entry points (one workgroup, chain 0)
+enum { DQMC_UNIT_BMUL = 0, DQMC_UNIT_RANK1 = 1, DQMC_UNIT_PUSH = 2, DQMC_UNIT_FRESH = 3 };
 // phases timed with DQMC_FLAG_PROFILE (100 MHz device wall clock ticks)
 enum { DQMC_PH_WRAP = 0, DQMC_PH_METROPOLIS, DQMC_PH_PRODUCT, DQMC_PH_PUSH, DQMC_PH_FRESH, DQMC_PH_MEASURE,
        DQMC_PH_WALK,     // part of METROPOLIS: the in-block decisions and C updates (one wavefront)

@@ -2992,6 +2992,67 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1)))
     else wg_gemm_mfma<false, true, true>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, s);
 }
 
+// The single-flip update of the equal-time Green's functions after the field at
+// (t, i) flips (src/model/repulsive_hubbard.cpp:127-155,
+// src/model/attractive_hubbard.cpp:128-156):
+//   G <- G - gamma G[:,i] (e_i^T - G[i,:]),  gamma = m / (1 + (1 - G(i,i)) m),
+// m = exp(-2 sigma alpha s) - 1 (d.dm1); the attractive model applies the spin-up
+// gamma to both matrices.  The sweep applies the same update in blocks (the
+// submatrix walk); this is its one-flip form.  The field row of t is in LDS.
+__device__ void wg_rank1(Desc d, int i, double* Gu_, double* Gd_, Lds&) {
+    Lds& s = g_lds;
+    const int N = d.N;
+    const int neg = s.frow[i] < 0 ? 1 : 0;
+    const double m0 = d.dm1[0][neg];
+    const double gam0 = m0 / (1.0 + (1.0 - gptr(Gu_)[(size_t)i * N + i]) * m0);
+    for (int sp = 0; sp < 2; ++sp) {
+        AS_G double* G = gptr(sp ? Gd_ : Gu_);
+        if (!(sp ? Gd_ : Gu_)) continue;
+        double gam = gam0;
+        if (sp == 1 && d.model != 1) {
+            const double m1 = d.dm1[1][neg];
+            gam = m1 / (1.0 + (1.0 - G[(size_t)i * N + i]) * m1);
+        }
+        for (int a = tid(); a < N; a += NT) {
+            s.v0[a] = G[(size_t)i * N + a];                        // column i
+            s.v1[a] = (a == i ? 1.0 : 0.0) - G[(size_t)a * N + i];  // e_i - row i
+        }
+        __syncthreads();
+        const int n2 = N * N;
+        for (int e = tid(); e < n2; e += NT) G[e] -= gam * s.v0[e % N] * s.v1[e / N];
+        __syncthreads();
+    }
+}
+
+// One building block of the sweep on chain 0 of a descriptor array (one workgroup):
+//   DQMC_UNIT_BMUL   X <- B(a) X or its variants (op c: 0 B X, 1 X B, 2 B^-1 X, 3 X B^-1,
+//                    4 B^T X; 5 B X B^-1, 6 B^-1 X B) for spin index b
+//   DQMC_UNIT_RANK1  the single-flip update of X (spin up) and Y (spin down, may be null)
+//                    for the field at (a, b)
+//   DQMC_UNIT_PUSH   push X onto spin-0 stack slot b (-1: empty stack), result in slot a
+//   DQMC_UNIT_FRESH  X = (1 + B_left B_right^T)^-1 from spin-0 stack slots a (left) and b
+//                    (right); -1 marks an empty stack
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) k_unit(const ChainDesc* __restrict__ descs, int unit, int a, int b, int c,
+                                             double* X, double* Y) {
+    Lds& s = g_lds;
+    Desc d = ((const AS_C ChainDesc*)descs)[0];
+    if (tid() == 0) s.prof_on = 0;
+    __syncthreads();
+    switch (unit) {
+        case DQMC_UNIT_BMUL:
+            load_field_row(d, a, s);
+            if (c <= 4) wg_bmul(d, c, b, X, d.W, s);
+            else wg_wrap(d, b, c == 5, X, d.W, s);
+            break;
+        case DQMC_UNIT_RANK1:
+            load_field_row(d, a, s);
+            wg_rank1(d, b, X, Y, s);
+            break;
+        case DQMC_UNIT_PUSH: wg_push(d, 0, a, b, X, s); break;
+        case DQMC_UNIT_FRESH: wg_fresh_green(d, 0, a, b, X, s); break;
+    }
+}
+
 }  // namespace dqmc
 
 extern "C" {
@@ -3004,6 +3065,11 @@ hipError_t dqmc_launch_gemm(int variant, int M, int Nc, int K, const double* A, 
                             double beta, hipStream_t stream) {
     hipLaunchKernelGGL(dqmc::k_gemm, dim3(1), dim3(NT), 0, stream, variant, M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc,
                        rs, ms, cs, beta);
+    return hipGetLastError();
+}
+hipError_t dqmc_launch_unit(const ChainDesc* d_descs, int unit, int a, int b, int c, double* X, double* Y,
+                            hipStream_t stream) {
+    hipLaunchKernelGGL(dqmc::k_unit, dim3(1), dim3(NT), 0, stream, d_descs, unit, a, b, c, X, Y);
     return hipGetLastError();
 }
 hipError_t dqmc_launch_init(const ChainDesc* d_descs, int n_chains, hipStream_t stream) {

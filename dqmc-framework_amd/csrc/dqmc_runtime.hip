@@ -19,6 +19,9 @@ extern "C" hipError_t dqmc_launch_gemm(int variant, int M, int Nc, int K, const 
                                        const double* B, int ldb, int tB, double* C, int ldc, const double* rs,
                                        const double* ms, const double* cs, double beta, hipStream_t stream);
 
+extern "C" hipError_t dqmc_launch_unit(const ChainDesc* d_descs, int unit, int a, int b, int c, double* X, double* Y,
+                                       hipStream_t stream);
+
 static_assert(DQMC_SWEEP_MEASURE == DQMC_FLAG_MEASURE && DQMC_SWEEP_RECORD == DQMC_FLAG_RECORD &&
                   DQMC_SWEEP_FROZEN == DQMC_FLAG_FROZEN && DQMC_SWEEP_PROFILE == DQMC_FLAG_PROFILE &&
                   DQMC_SWEEP_DYNAMIC == DQMC_FLAG_DYNAMIC,
@@ -725,6 +728,181 @@ int dqmc_k_gemm(dqmc_ctx_t ctx, int variant, int M, int Nc, int K, int tA, int t
     if (dv) hipFree(dv);
     if (e != hipSuccess) { ctx->err = std::string("dqmc_k_gemm: ") + hipGetErrorString(e); return DQMC_ERR_HIP; }
     return DQMC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Single building blocks of the sweep, each run alone on a one-chain batch
+// (host buffers in and out, synchronous).
+// ---------------------------------------------------------------------------
+namespace {
+
+// a one-chain batch whose descriptor, stacks and scratch carry one building block
+int unit_batch(dqmc_ctx_t ctx, const dqmc_chain_params& p, const int8_t* fields, const double* expK,
+               const double* invExpK, dqmc_batch_t* out) {
+    dqmc_batch_t b = nullptr;
+    int rc = dqmc_batch_create(ctx, 1, &p, 0, &b);
+    if (rc != DQMC_OK) return rc;
+    if (fields) rc = dqmc_set_fields(b, 0, fields);
+    if (rc == DQMC_OK && expK) {
+        if (!invExpK) { ctx->err = "exp(K) needs its inverse"; rc = DQMC_ERR_ARG; }
+        else rc = dqmc_set_expk(b, 0, expK, invExpK);
+    }
+    if (rc != DQMC_OK) { dqmc_batch_destroy(b); return rc; }
+    *out = b;
+    return DQMC_OK;
+}
+
+// stack-only building blocks take matrices of the engine's sizes: N = side^2, 2 <= side, N <= 448
+int unit_stack_params(dqmc_ctx_t ctx, int N, dqmc_chain_params& p) {
+    int side = 2;
+    while (side * side < N) ++side;
+    if (N < 4 || side * side != N || N > DQMC_NMAX) {
+        ctx->err = "N must be side*side with 2 <= side and N <= 448";
+        return DQMC_ERR_ARG;
+    }
+    std::memset(&p, 0, sizeof(p));
+    p.side = side;
+    p.time_size = 2;   // two stack levels: slot 0 and slot 1 of the spin-0 stack
+    p.stabilization_pace = 1;
+    p.beta = 1.0;
+    p.hopping_t = 1.0;
+    p.model = DQMC_MODEL_REPULSIVE;
+    p.delay = 1;
+    p.momentum_index = -1;
+    p.precision = DQMC_PREC_FP64;
+    return DQMC_OK;
+}
+
+hipError_t h2d(dqmc_batch_t b, void* dst, const void* src, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, b->stream);
+}
+hipError_t d2h(dqmc_batch_t b, void* dst, const void* src, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, b->stream);
+}
+
+void transpose(int N, const double* src, double* dst) {
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < N; ++i) dst[(size_t)j * N + i] = src[(size_t)i * N + j];
+}
+
+// stack slot k of spin 0 <- (U, D, V^T)
+hipError_t put_slot(dqmc_batch_t b, int k, const double* U, const double* D, const double* V) {
+    const ChainDesc& d = b->descs[0];
+    const size_t n2 = (size_t)d.N * d.N;
+    std::vector<double> T(n2);
+    transpose(d.N, V, T.data());
+    hipError_t e = h2d(b, d.stkU + k * n2, U, n2 * 8);
+    if (e == hipSuccess) e = h2d(b, d.stkD + (size_t)k * d.N, D, (size_t)d.N * 8);
+    if (e == hipSuccess) e = h2d(b, d.stkT + k * n2, T.data(), n2 * 8);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    return e;
+}
+
+int unit_finish(dqmc_batch_t b, const char* who, hipError_t e) {
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    dqmc_ctx_t ctx = b->ctx;
+    dqmc_batch_destroy(b);
+    if (e != hipSuccess) { ctx->err = std::string(who) + ": " + hipGetErrorString(e); return DQMC_ERR_HIP; }
+    return DQMC_OK;
+}
+
+}  // namespace
+
+// One slice's B-matrix product on an N x N matrix M, in place
+// (RepulsiveHubbard::mult_B_from_left .. mult_transB_from_left, src/model/repulsive_hubbard.cpp:158-245,
+// and the wraps of DqmcWalker::wrap_from_0_to_beta / wrap_from_beta_to_0, src/dqmc_walker.cpp:208-245)
+int dqmc_k_wrap(dqmc_ctx_t ctx, const dqmc_chain_params* p, const int8_t* fields, const double* expK,
+                const double* invExpK, int t, int spin, int op, double* M) {
+    if (!ctx || !p || !fields || !M || op < 0 || op > 6 || (spin != 1 && spin != -1)) return DQMC_ERR_ARG;
+    if (t < 0 || t > p->time_size) { ctx->err = "time index out of range"; return DQMC_ERR_ARG; }
+    // the reference's time index t acts with field row t - 1; t = 0 is the slice at beta
+    const int row = t == 0 ? p->time_size - 1 : t - 1;
+    dqmc_batch_t b = nullptr;
+    int rc = unit_batch(ctx, *p, fields, expK, invExpK, &b);
+    if (rc != DQMC_OK) return rc;
+    const ChainDesc& d = b->descs[0];
+    const size_t n2 = (size_t)d.N * d.N;
+    const int sp = (spin == 1 || d.nspin == 1) ? 0 : 1;
+    hipError_t e = h2d(b, d.G, M, n2 * 8);
+    if (e == hipSuccess) e = dqmc_launch_unit(b->d_descs, DQMC_UNIT_BMUL, row, sp, op, d.G, nullptr, b->stream);
+    if (e == hipSuccess) e = d2h(b, M, d.G, n2 * 8);
+    return unit_finish(b, "dqmc_k_wrap", e);
+}
+
+// The single-flip update of the equal-time Green's functions for the field at (t, i)
+// (RepulsiveHubbard::update_greens_function, src/model/repulsive_hubbard.cpp:127-155;
+// AttractiveHubbard::update_greens_function, src/model/attractive_hubbard.cpp:128-156)
+int dqmc_k_rank1(dqmc_ctx_t ctx, const dqmc_chain_params* p, const int8_t* fields, int t, int i, double* G_up,
+                 double* G_dn) {
+    if (!ctx || !p || !fields || !G_up) return DQMC_ERR_ARG;
+    if (t < 0 || t >= p->time_size || i < 0 || i >= p->side * p->side) {
+        ctx->err = "field index out of range";
+        return DQMC_ERR_ARG;
+    }
+    dqmc_batch_t b = nullptr;
+    int rc = unit_batch(ctx, *p, fields, nullptr, nullptr, &b);
+    if (rc != DQMC_OK) return rc;
+    const ChainDesc& d = b->descs[0];
+    const size_t n2 = (size_t)d.N * d.N;
+    // spin up in G, spin down in the product workspace (both at least N x N)
+    hipError_t e = h2d(b, d.G, G_up, n2 * 8);
+    if (e == hipSuccess && G_dn) e = h2d(b, d.Gw, G_dn, n2 * 8);
+    if (e == hipSuccess)
+        e = dqmc_launch_unit(b->d_descs, DQMC_UNIT_RANK1, t, i, 0, d.G, G_dn ? d.Gw : nullptr, b->stream);
+    if (e == hipSuccess) e = d2h(b, G_up, d.G, n2 * 8);
+    if (e == hipSuccess && G_dn) e = d2h(b, G_dn, d.Gw, n2 * 8);
+    return unit_finish(b, "dqmc_k_rank1", e);
+}
+
+// SvdStack::push (src/svd_stack.cpp:34-61) in the engine's QR form: U diag(D) V^T =
+// A (U_in diag(D_in) V_in^T), or of A alone when U_in is NULL (empty stack); U orthogonal,
+// D > 0, V not orthogonal (it carries the triangular factors)
+int dqmc_k_udv_push(dqmc_ctx_t ctx, int N, const double* A, const double* U_in, const double* D_in,
+                    const double* V_in, double* U, double* D, double* V) {
+    if (!ctx || !A || !U || !D || !V) return DQMC_ERR_ARG;
+    if (U_in && (!D_in || !V_in)) { ctx->err = "a stack top needs U, D and V"; return DQMC_ERR_ARG; }
+    dqmc_chain_params p;
+    int rc = unit_stack_params(ctx, N, p);
+    if (rc != DQMC_OK) return rc;
+    dqmc_batch_t b = nullptr;
+    rc = unit_batch(ctx, p, nullptr, nullptr, nullptr, &b);
+    if (rc != DQMC_OK) return rc;
+    const ChainDesc& d = b->descs[0];
+    const size_t n2 = (size_t)N * N;
+    const int slot = U_in ? 1 : 0;
+    hipError_t e = U_in ? put_slot(b, 0, U_in, D_in, V_in) : hipSuccess;
+    if (e == hipSuccess) e = h2d(b, d.G, A, n2 * 8);
+    if (e == hipSuccess) e = dqmc_launch_unit(b->d_descs, DQMC_UNIT_PUSH, slot, slot - 1, 0, d.G, nullptr, b->stream);
+    std::vector<double> T(n2);
+    if (e == hipSuccess) e = d2h(b, U, d.stkU + slot * n2, n2 * 8);
+    if (e == hipSuccess) e = d2h(b, D, d.stkD + (size_t)slot * N, (size_t)N * 8);
+    if (e == hipSuccess) e = d2h(b, T.data(), d.stkT + slot * n2, n2 * 8);
+    rc = unit_finish(b, "dqmc_k_udv_push", e);
+    if (rc == DQMC_OK) transpose(N, T.data(), V);
+    return rc;
+}
+
+// Equal-time Green's function from the two stack tops (NumericalStable::compute_equaltime_greens,
+// include/utils/numerical_stable.hpp:186-238): G = (1 + B_L B_R^T)^-1 with B_L = U_L diag(D_L) V_L^T
+// and B_R = U_R diag(D_R) V_R^T; a side given as NULL is an empty stack (B = 1)
+int dqmc_k_greens_recompute(dqmc_ctx_t ctx, int N, const double* UL, const double* DL, const double* VL,
+                            const double* UR, const double* DR, const double* VR, double* G) {
+    if (!ctx || !G || (!UL && !UR)) return DQMC_ERR_ARG;
+    if ((UL && (!DL || !VL)) || (UR && (!DR || !VR))) { ctx->err = "a stack top needs U, D and V"; return DQMC_ERR_ARG; }
+    dqmc_chain_params p;
+    int rc = unit_stack_params(ctx, N, p);
+    if (rc != DQMC_OK) return rc;
+    dqmc_batch_t b = nullptr;
+    rc = unit_batch(ctx, p, nullptr, nullptr, nullptr, &b);
+    if (rc != DQMC_OK) return rc;
+    const ChainDesc& d = b->descs[0];
+    const size_t n2 = (size_t)N * N;
+    hipError_t e = UL ? put_slot(b, 0, UL, DL, VL) : hipSuccess;
+    if (e == hipSuccess && UR) e = put_slot(b, 1, UR, DR, VR);
+    if (e == hipSuccess)
+        e = dqmc_launch_unit(b->d_descs, DQMC_UNIT_FRESH, UL ? 0 : -1, UR ? 1 : -1, 0, d.G, nullptr, b->stream);
+    if (e == hipSuccess) e = d2h(b, G, d.G, n2 * 8);
+    return unit_finish(b, "dqmc_k_greens_recompute", e);
 }
 
 // Delayed-update depth autotuner: times `sweeps` single-direction sweeps of a

@@ -28,7 +28,8 @@ EXPORTS = ["dqmc_context_create", "dqmc_context_destroy", "dqmc_context_error", 
            "dqmc_get_accepts", "dqmc_get_info", "dqmc_get_measure", "dqmc_measure_recorded", "dqmc_clear_measure",
            "dqmc_get_dynamic_green", "dqmc_get_dynamic_measure", "dqmc_num_kstars", "dqmc_write_fields_file",
            "dqmc_read_fields_file", "dqmc_write_kstars_file", "dqmc_write_tgrids_file", "dqmc_write_observable_file",
-           "dqmc_write_observable_bins_file", "dqmc_k_gemm", "dqmc_autotune_delay",
+           "dqmc_write_observable_bins_file", "dqmc_k_gemm", "dqmc_k_wrap", "dqmc_k_rank1",
+           "dqmc_k_udv_push", "dqmc_k_greens_recompute", "dqmc_autotune_delay",
            "dqmc_server_create", "dqmc_server_submit", "dqmc_server_cancel", "dqmc_server_poll",
            "dqmc_server_wait_idle", "dqmc_server_get_stats", "dqmc_server_error", "dqmc_server_destroy",
            "dqmc_server_get_dynamic"]
@@ -117,6 +118,10 @@ def lib():
         L.dqmc_write_observable_file.argtypes = [cp, ip, ip, ip, vp, vp]
         L.dqmc_write_observable_bins_file.argtypes = [cp, ip, ip, ip, ip, vp]
         L.dqmc_k_gemm.argtypes = [vp, ip, ip, ip, ip, ip, ip, vp, vp, vp, C.c_double, vp, vp, vp]
+        L.dqmc_k_wrap.argtypes = [vp, C.POINTER(ChainParams), vp, vp, vp, ip, ip, ip, vp]
+        L.dqmc_k_rank1.argtypes = [vp, C.POINTER(ChainParams), vp, ip, ip, vp, vp]
+        L.dqmc_k_udv_push.argtypes = [vp, ip, vp, vp, vp, vp, vp, vp, vp]
+        L.dqmc_k_greens_recompute.argtypes = [vp, ip, vp, vp, vp, vp, vp, vp, vp]
         L.dqmc_autotune_delay.argtypes = [vp, C.POINTER(ChainParams), ip, vp, ip, ip, C.POINTER(ip), vp]
         L.dqmc_server_create.argtypes = [vp, C.c_size_t, ip, ip, C.POINTER(vp)]
         L.dqmc_server_submit.argtypes = [vp, C.POINTER(Job), C.POINTER(C.c_longlong)]
@@ -171,6 +176,61 @@ class Context:
         self.check(lib().dqmc_k_gemm(self.h, variant, M, Nc, K, int(transA), int(transB), _p(A), _p(B), _p(C),
                                      float(beta), ptr[0], ptr[1], ptr[2]), "dqmc_k_gemm")
         return C
+
+    # B-matrix products by name: ModelBase mult_* (src/model/repulsive_hubbard.cpp:158-245) and the wraps
+    WRAP_OPS = {"B_left": 0, "B_right": 1, "invB_left": 2, "invB_right": 3, "transB_left": 4,
+                "wrap_up": 5, "wrap_down": 6}
+
+    def k_wrap(self, params, fields, t, spin, op, M, expK=None, invExpK=None):
+        """One slice's B-matrix product (or wrap) applied to M on the device; returns the result.
+        t is the reference's time index (field row t - 1, t = 0 the slice at beta)."""
+        op = self.WRAP_OPS.get(op, op)
+        f = np.ascontiguousarray(fields, dtype=np.int8)
+        M = np.array(M, dtype=np.float64, order="F", copy=True)
+        ek = None if expK is None else np.asfortranarray(expK, dtype=np.float64)
+        ik = None if invExpK is None else np.asfortranarray(invExpK, dtype=np.float64)
+        self.check(lib().dqmc_k_wrap(self.h, C.byref(params), _p(f), None if ek is None else _p(ek),
+                                     None if ik is None else _p(ik), int(t), int(spin), int(op), _p(M)),
+                   "dqmc_k_wrap")
+        return M
+
+    def k_rank1(self, params, fields, t, i, G_up, G_dn=None):
+        """Single-flip update of the Green's functions for the field at (t, i); returns (G_up, G_dn)."""
+        f = np.ascontiguousarray(fields, dtype=np.int8)
+        gu = np.array(G_up, dtype=np.float64, order="F", copy=True)
+        gd = None if G_dn is None else np.array(G_dn, dtype=np.float64, order="F", copy=True)
+        self.check(lib().dqmc_k_rank1(self.h, C.byref(params), _p(f), int(t), int(i), _p(gu),
+                                      None if gd is None else _p(gd)), "dqmc_k_rank1")
+        return gu, gd
+
+    def k_udv_push(self, A, top=None):
+        """Push A onto a stack whose top is (U, D, V) (None: empty); returns the new (U, D, V)
+        with U diag(D) V^T = A U_top diag(D_top) V_top^T."""
+        A = np.asfortranarray(A, dtype=np.float64)
+        n = A.shape[0]
+        U = np.zeros((n, n), order="F"); D = np.zeros(n); V = np.zeros((n, n), order="F")
+        t = [None] * 3 if top is None else [np.asfortranarray(top[0], dtype=np.float64),
+                                             np.ascontiguousarray(top[1], dtype=np.float64),
+                                             np.asfortranarray(top[2], dtype=np.float64)]
+        self.check(lib().dqmc_k_udv_push(self.h, n, _p(A), *[None if x is None else _p(x) for x in t],
+                                         _p(U), _p(D), _p(V)), "dqmc_k_udv_push")
+        return U, D, V
+
+    def k_greens_recompute(self, left=None, right=None):
+        """(1 + B_L B_R^T)^-1 from the stack tops left = (U, D, V), right = (U, D, V); None: empty."""
+        side = left if left is not None else right
+        n = np.asarray(side[0]).shape[0]
+        args = []
+        for top in (left, right):
+            if top is None:
+                args += [None] * 3
+            else:
+                args += [np.asfortranarray(top[0], dtype=np.float64), np.ascontiguousarray(top[1], dtype=np.float64),
+                         np.asfortranarray(top[2], dtype=np.float64)]
+        G = np.zeros((n, n), order="F")
+        self.check(lib().dqmc_k_greens_recompute(self.h, n, *[None if x is None else _p(x) for x in args], _p(G)),
+                   "dqmc_k_greens_recompute")
+        return G
 
     def autotune_delay(self, params, n_chains, candidates=(8, 16, 32, 64), sweeps=2):
         """Fastest delayed-update depth for chains of this shape (times every candidate)."""

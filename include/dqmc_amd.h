@@ -287,6 +287,44 @@ int dqmc_server_destroy(dqmc_server_t s);
 int dqmc_k_gemm(dqmc_ctx_t ctx, int variant, int M, int Nc, int K, int transA, int transB, const double* A,
                 const double* B, double* C, double beta, const double* rs, const double* ms, const double* cs);
 
+/* One slice's B-matrix product on the N x N matrix M, in place, for the chain `p`
+ * with fields (time_size x N, +-1) and, for the dense kinetic term, exp(-dtau K)
+ * and its inverse (NULL: the closed form the batch would use).  Replaces
+ * RepulsiveHubbard / AttractiveHubbard::mult_B_from_left, mult_B_from_right,
+ * mult_invB_from_left, mult_invB_from_right, mult_transB_from_left
+ * (src/model/repulsive_hubbard.cpp:158-245, src/model/attractive_hubbard.cpp:159-245)
+ * and the wraps of DqmcWalker::wrap_from_0_to_beta / wrap_from_beta_to_0
+ * (src/dqmc_walker.cpp:208-245).  t is the reference's time_index, 0 <= t <= time_size:
+ * B(t) takes field row t - 1, and t = 0 the last row (the slice at beta); spin +1 / -1;
+ * op 0 B M, 1 M B, 2 B^-1 M, 3 M B^-1, 4 B^T M, 5 B M B^-1, 6 B^-1 M B. */
+int dqmc_k_wrap(dqmc_ctx_t ctx, const dqmc_chain_params* p, const int8_t* fields, const double* expK,
+                const double* invExpK, int t, int spin, int op, double* M);
+
+/* The single-flip update of the equal-time Green's functions (column-major N x N,
+ * in place) for the field at (t, i) of `fields`, before that field is flipped
+ * (RepulsiveHubbard::update_greens_function, src/model/repulsive_hubbard.cpp:127-155;
+ * AttractiveHubbard::update_greens_function, src/model/attractive_hubbard.cpp:128-156:
+ * the attractive model updates both matrices with the spin-up factor).  G_dn may be
+ * NULL. */
+int dqmc_k_rank1(dqmc_ctx_t ctx, const dqmc_chain_params* p, const int8_t* fields, int t, int i, double* G_up,
+                 double* G_dn);
+
+/* Utils::SvdStack::push (src/svd_stack.cpp:34-61) in the engine's QR form:
+ * U diag(D) V^T = A (U_in diag(D_in) V_in^T), or of A alone when U_in is NULL (an
+ * empty stack).  U is orthogonal and D > 0 graded as singular values are; V is not
+ * orthogonal (it carries the triangular factors), so U, D, V equal the reference's
+ * SVD factors only through their product and the scales of D.  N = side * side with
+ * 2 <= side and N <= 448 (the engine's sizes). */
+int dqmc_k_udv_push(dqmc_ctx_t ctx, int N, const double* A, const double* U_in, const double* D_in,
+                    const double* V_in, double* U, double* D, double* V);
+
+/* NumericalStable::compute_equaltime_greens (include/utils/numerical_stable.hpp:186-238):
+ * G = (1 + B_L B_R^T)^-1 with B_L = U_L diag(D_L) V_L^T (left stack top) and
+ * B_R = U_R diag(D_R) V_R^T (right stack top); pass NULL for U, D and V of an empty
+ * stack (not both).  U_L, U_R orthogonal, D > 0.  Same sizes as dqmc_k_udv_push. */
+int dqmc_k_greens_recompute(dqmc_ctx_t ctx, int N, const double* UL, const double* DL, const double* VL,
+                            const double* UR, const double* DR, const double* VR, double* G);
+
 #ifdef __cplusplus
 }
 #endif
