@@ -3,7 +3,8 @@ entry points (dqmc_k_wrap, dqmc_k_rank1, dqmc_k_udv_push, dqmc_k_greens_recomput
 the oracle and against identities the mathematics fixes.
 
 Bars (FP64; the device GEMMs accumulate in another order than the oracle's loops):
-  B-matrix products and wraps      max |dM| <= 1e-12 x max |M_oracle|   (ModelBase mult_*)
+  B-matrix products and wraps      max |dM| <= 1e-12 x max |M_oracle|   (ModelBase mult_*);
+                                   in mixed precision (FP32 matrix cores) <= 1e-5 x max |M_oracle|
   single-flip update               max |dG| <= 1e-13 against the oracle, 1e-11 against the
                                    Sherman-Morrison closed form G' = G [1 + Delta (1 - G)]^-1
   UDV push (pre-pivoted QR)        U, V within 1e-10, D within 1e-12 relative, of the oracle's
@@ -74,6 +75,23 @@ def test_wrap_and_products_match_oracle(ctx, case):
                 worst = max(worst, err)
                 assert err < 1e-12, (t, spin, op, err)
     print(f"[gpu-units] wrap {case}: max relative error {worst:.2e}", flush=True)
+
+
+@pytest.mark.parametrize("side", [4, 12])
+def test_mixed_precision_products(ctx, side):
+    # mixed precision: the dense B products and wraps on FP32 matrix cores with FP32
+    # accumulation; bar 1e-5 of max |M| (FP32 rounding 6e-8 over N-term sums)
+    orc, _, f = chain(side, 10, 2.0, U=6.0)
+    p = D.make_params(side, 10, 2.0, pace=5, U=6.0, seed=7, precision=D.PREC_MIXED)
+    n = side * side
+    m = np.asfortranarray(np.random.default_rng(5).standard_normal((n, n)))
+    worst = 0.0
+    for op in range(7):
+        want = oracle_op(orc, op, 4, -1, m.copy(order="F"))
+        got = ctx.k_wrap(p, f, 4, -1, op, m, orc.expK(0), orc.expK(1))
+        worst = max(worst, np.max(np.abs(got - want)) / np.max(np.abs(want)))
+    print(f"[gpu-units] mixed precision side {side}: max relative error {worst:.2e}", flush=True)
+    assert 1e-12 < worst < 1e-5      # FP32 arithmetic shows, within its bound
 
 
 def test_wrap_closed_form_exp_k(ctx):
