@@ -15,6 +15,11 @@
 //   QuantumMonteCarlo::Dqmc::thermalize / measure / analyse   include/dqmc.h:61-73
 //   QuantumMonteCarlo::DqmcIO       include/dqmc_io.h (file outputs and the field file)
 //   Utils::Random::set_seed         include/random.h:12-17
+//   Utils::SvdStack,
+//   Utils::NumericalStable::compute_equaltime_greens   include/svd_stack.h:70-91,
+//                                   include/utils/numerical_stable.hpp:186-238
+//   ModelBase::mult_{B,invB}_from_{left,right}, mult_transB_from_left
+//                                   include/model/model_base.h:111-163
 // Matrices are column-major std::vector<double> (Eigen::MatrixXd layout).
 // Errors: the reference asserts or prints and exits; here a failing engine
 // call throws std::runtime_error with the engine's message.
@@ -68,6 +73,77 @@ inline dqmc_ctx_t context() {
     return ctx;
 }
 }  // namespace dqmc_detail
+
+namespace Utils {
+// Utils::SvdStack (include/svd_stack.h:70-91, src/svd_stack.cpp:15-87) on the engine's UDV push:
+// each level keeps U, D and the cumulative V of the product pushed so far, so MatrixV() is the
+// reference's product V_0 V_1 ... of the levels.  The engine factors by pre-pivoted QR: U is
+// orthogonal and D graded like singular values, V carries the triangular factors (not orthogonal).
+class SvdStack {
+    int m_dim{0}, m_len{0};
+    std::vector<std::vector<double>> m_u, m_d, m_v;
+
+   public:
+    SvdStack() = default;
+    SvdStack(int mat_dim, int stack_length)
+        : m_dim(mat_dim), m_u(stack_length), m_d(stack_length), m_v(stack_length) {}
+    bool empty() const { return m_len == 0; }
+    int MatDim() const { return m_dim; }
+    int StackLength() const { return m_len; }
+    std::vector<double> SingularValues() const { return top(m_d); }
+    std::vector<double> MatrixU() const { return top(m_u); }
+    std::vector<double> MatrixV() const { return top(m_v); }
+    void clear() { m_len = 0; }
+    // adds `matrix` from the left: the stack's product becomes matrix * (previous product)
+    void push(const std::vector<double>& matrix) {
+        const size_t n2 = (size_t)m_dim * m_dim;
+        if (matrix.size() != n2) throw std::invalid_argument("SvdStack::push: matrix size");
+        if (m_len >= (int)m_u.size()) throw std::length_error("SvdStack::push: stack is full");
+        std::vector<double> U(n2), D(m_dim), V(n2);
+        const bool e = empty();
+        dqmc_ctx_t ctx = dqmc_detail::context();
+        dqmc_detail::check(dqmc_k_udv_push(ctx, m_dim, matrix.data(), e ? nullptr : m_u[m_len - 1].data(),
+                                           e ? nullptr : m_d[m_len - 1].data(), e ? nullptr : m_v[m_len - 1].data(),
+                                           U.data(), D.data(), V.data()),
+                           ctx, "SvdStack::push");
+        m_u[m_len] = std::move(U);
+        m_d[m_len] = std::move(D);
+        m_v[m_len] = std::move(V);
+        ++m_len;
+    }
+    void pop() {
+        if (m_len == 0) throw std::logic_error("SvdStack::pop: stack is empty");
+        --m_len;
+    }
+
+   private:
+    std::vector<double> top(const std::vector<std::vector<double>>& level) const {
+        if (m_len == 0) throw std::logic_error("SvdStack: stack is empty");
+        return level[m_len - 1];
+    }
+};
+
+// Utils::NumericalStable::compute_equaltime_greens (include/utils/numerical_stable.hpp:186-238):
+// gtt = (1 + B_left B_right^T)^-1 from the two stacks' products, on the device
+class NumericalStable {
+   public:
+    static void compute_equaltime_greens(SvdStack& left, SvdStack& right, std::vector<double>& gtt) {
+        if (left.empty() && right.empty()) throw std::logic_error("compute_equaltime_greens: both stacks are empty");
+        if (!left.empty() && !right.empty() && left.MatDim() != right.MatDim())
+            throw std::invalid_argument("compute_equaltime_greens: stack dimensions differ");
+        const int n = left.empty() ? right.MatDim() : left.MatDim();
+        gtt.assign((size_t)n * n, 0.0);
+        std::vector<double> ul, dl, vl, ur, dr, vr;
+        if (!left.empty()) { ul = left.MatrixU(); dl = left.SingularValues(); vl = left.MatrixV(); }
+        if (!right.empty()) { ur = right.MatrixU(); dr = right.SingularValues(); vr = right.MatrixV(); }
+        auto ptr = [](const std::vector<double>& v) { return v.empty() ? nullptr : v.data(); };
+        dqmc_ctx_t ctx = dqmc_detail::context();
+        dqmc_detail::check(dqmc_k_greens_recompute(ctx, n, ptr(ul), ptr(dl), ptr(vl), ptr(ur), ptr(dr), ptr(vr),
+                                                   gtt.data()),
+                           ctx, "NumericalStable::compute_equaltime_greens");
+    }
+};
+}  // namespace Utils
 
 namespace Lattice {
 using LatticeIntVec = std::vector<int>;
@@ -183,6 +259,51 @@ class ModelBase {
         for (auto& f : m_bosonic_field) f = b(Utils::Random::Engine()) ? +1 : -1;
     }
     const std::vector<int8_t>& BosonicFields() const { return m_bosonic_field; }
+
+    // B(t) = exp(-dtau V_sigma(t)) exp(-dtau K) products on a caller's N x N matrix, in place
+    // (src/model/repulsive_hubbard.cpp:158-245, src/model/attractive_hubbard.cpp:159-245): t is
+    // the reference's time index in [0, L] (field row t - 1, t = 0 the slice at beta), spin +-1.
+    // They run on the device with the chain's current fields (after initial_modules).
+    void mult_B_from_left(std::vector<double>& green, int time_index, int spin) const {
+        mult(green, time_index, spin, 0, "mult_B_from_left");
+    }
+    void mult_B_from_right(std::vector<double>& green, int time_index, int spin) const {
+        mult(green, time_index, spin, 1, "mult_B_from_right");
+    }
+    void mult_invB_from_left(std::vector<double>& green, int time_index, int spin) const {
+        mult(green, time_index, spin, 2, "mult_invB_from_left");
+    }
+    void mult_invB_from_right(std::vector<double>& green, int time_index, int spin) const {
+        mult(green, time_index, spin, 3, "mult_invB_from_right");
+    }
+    void mult_transB_from_left(std::vector<double>& green, int time_index, int spin) const {
+        mult(green, time_index, spin, 4, "mult_transB_from_left");
+    }
+
+   private:
+    int m_side{};
+    double m_beta{};
+    void mult(std::vector<double>& g, int t, int spin, int op, const char* who) const {
+        if (m_side < 2 || g.size() != (size_t)m_space_size * m_space_size)
+            throw std::invalid_argument(std::string(who) + ": model not initialized or matrix size");
+        dqmc_chain_params p{};
+        p.model = m_model_type;
+        p.side = m_side;
+        p.time_size = m_time_size;
+        p.stabilization_pace = m_time_size;   // one stack level: the product needs no stack
+        p.checkerboard = m_checkerboard ? 1 : 0;
+        p.delay = 1;
+        p.momentum_index = -1;
+        p.precision = DQMC_PREC_FP64;
+        p.beta = m_beta;
+        p.hopping_t = m_hopping_t;
+        p.onsite_u = m_onsite_u;
+        p.chemical_potential = m_chemical_potential;
+        dqmc_ctx_t ctx = dqmc_detail::context();
+        std::vector<int8_t> f = m_bosonic_field;   // the device holds the fields once the chain runs there
+        if (m_batch) dqmc_detail::check(dqmc_get_fields(m_batch, 0, f.data()), ctx, who);
+        dqmc_detail::check(dqmc_k_wrap(ctx, &p, f.data(), nullptr, nullptr, t, spin, op, g.data()), ctx, who);
+    }
 };
 class RepulsiveHubbard : public ModelBase {
    public:
@@ -481,6 +602,8 @@ class DqmcInitializer {
         meas.m_time_size = walker.TimeSize();
         model.m_space_size = lattice.SpaceSize();
         model.m_time_size = walker.TimeSize();
+        model.m_side = lattice.SideLength();
+        model.m_beta = walker.Beta();
         model.m_bosonic_field.assign((size_t)walker.TimeSize() * lattice.SpaceSize(), 1);
         if (checkerboard) model.link(*checkerboard); else model.link();
     }

@@ -98,3 +98,38 @@ def test_facade_dynamic_driver_matches_oracle_driver(cfg, tmp_path):
     head = open(tmp_path / "dss.bins.out").readline().split()
     assert [int(x) for x in head] == [bins, L]
     assert open(tmp_path / "tgrids.out").readline().split()[0] == str(L)
+
+
+BLOCKS = os.path.join(ROOT, "dqmc-framework_amd", "lib", "building_blocks")
+
+
+@pytest.mark.parametrize("cfg", [
+    # side L beta pace model cb U mu seed
+    (4, 40, 4.0, 10, 0, 0, 4.0, -0.2, 42),
+    (6, 30, 3.0, 7, 1, 1, 4.0, -0.5, 7),
+])
+def test_facade_building_blocks_match_oracle(cfg):
+    # ModelBase::mult_B_from_left, SvdStack::push and NumericalStable::compute_equaltime_greens
+    # of the facade, used as DqmcWalker::initial_svd_stacks / initial_greens_functions use them
+    # (src/dqmc_walker.cpp:109-147), against the oracle's B(t) and its SVD-stack G(0)
+    print(f"[gpu-facade] building blocks {cfg}", flush=True)
+    side, L, beta, pace, model, cb, U, mu, seed = cfg
+    out = subprocess.run([BLOCKS] + [str(x) for x in cfg], check=True, capture_output=True, text=True,
+                         timeout=300).stdout
+    got = json.loads(out.strip().splitlines()[-1])
+    n = side * side
+    orc = O.Chain(side, L, beta, pace=pace, model=model, checkerboard=bool(cb), U=U, mu=mu)
+    orc.seed(seed)
+    orc.fields_random()
+    assert np.array_equal(np.array(got["fields"]).reshape(L, n), orc.fields())
+    eye = np.asfortranarray(np.eye(n))
+    for key, t in (("B1", 1), ("B0", 0)):
+        want = orc.mult(0, t, 1, eye)
+        assert np.max(np.abs(np.array(got[key]).reshape(n, n, order="F") - want)) < 1e-13 * np.max(np.abs(want))
+    assert got["stack_length"] == (L + pace - 1) // pace
+    orc.init()
+    g0 = orc.green(1)
+    for key in ("g0_stack", "g0_engine"):
+        err = np.max(np.abs(np.array(got[key]).reshape(n, n, order="F") - g0))
+        print(f"[gpu-facade] {key}: max |dG| {err:.2e}", flush=True)
+        assert err < 1e-10, key
