@@ -103,7 +103,7 @@ def algorithmic_flops_per_dynamic_sweep(w):
     return (wrap + 3.0 * prod + 41.3 * ns * N ** 3) * spins / 2.0
 
 
-def autotune(w, candidates=(8, 16, 32, 64)):
+def autotune(w, candidates=(8, 16, 32)):
     """Delayed-update depth for the workload's chain shape, chosen by the engine's
     autotuner (each candidate timed on a full batch of the workload's size)."""
     ctx = D.Context(0)

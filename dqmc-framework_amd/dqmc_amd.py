@@ -232,7 +232,7 @@ class Context:
                    "dqmc_k_greens_recompute")
         return G
 
-    def autotune_delay(self, params, n_chains, candidates=(8, 16, 32, 64), sweeps=2):
+    def autotune_delay(self, params, n_chains, candidates=(8, 16, 32), sweeps=2):
         """Fastest delayed-update depth for chains of this shape (times every candidate)."""
         cand = np.ascontiguousarray(candidates, dtype=np.int32)
         ms = np.zeros(len(cand), dtype=np.float32)

@@ -98,7 +98,8 @@ typedef struct {
     int time_size;           /* L (dtau = beta / L, src/dqmc_walker.cpp:30) */
     int stabilization_pace;  /* p */
     int checkerboard;        /* 0 dense exp(-dtau K), 1 checkerboard break-up (even side only) */
-    int delay;               /* delayed-update depth k (1..64); 1 applies every flip at once */
+    int delay;               /* delayed-update depth k (1..64; blocks take at most 32 sites, so k > 32 runs as 32);
+                                1 applies every flip at once */
     int momentum_index;      /* k-star index for momentum_distribution/SDW/CDW; -1 = M point (pi,pi) */
     int precision;           /* DQMC_PREC_FP64 or DQMC_PREC_MIXED */
     double beta;
