@@ -747,6 +747,17 @@ __device__ __forceinline__ void wg_stream2(int n, Ld1 ld1, St1 st1, Ld2 ld2, St2
             if (e0 + q * NT < n) { st1(e0 + q * NT, v[q]); st2(e0 + q * NT, w[q]); }
     }
 }
+// Division by a wave-uniform divisor 1 <= d <= 448 without the integer divide
+// sequence: q = floor(e m / 2^32) with m = ceil(2^32 / d).  With m d = 2^32 + r,
+// 0 <= r < d, the product overshoots e / d by e r / (d 2^32), below 1 / d for
+// e < 2^32 / 448, so q = floor(e / d) for every 0 <= e < 9.5e6 (the indices
+// divided here stay below N^2 <= 448^2).
+struct FastDiv {
+    uint64_t m;
+    int d;
+    __device__ explicit FastDiv(int d_) : m(((1ull << 32) + (uint64_t)d_ - 1) / (uint64_t)d_), d(d_) {}
+    __device__ __forceinline__ int div(int e) const { return (int)(((uint64_t)(uint32_t)e * m) >> 32); }
+};
 __device__ __forceinline__ const AS_G double* gptr(const double* p) { return (const AS_G double*)p; }
 __device__ __forceinline__ AS_G double* gptr(double* p) { return (AS_G double*)p; }
 
@@ -759,15 +770,18 @@ __device__ __noinline__ void wg_copy(double* dst_, const double* src_, int n2) {
 }
 __device__ void wg_identity(double* M_, int N) {
     AS_G double* const M = gptr(M_);
-    for (int e = tid(); e < N * N; e += NT) M[e] = ((e % N) == (e / N)) ? 1.0 : 0.0;
+    const FastDiv dN(N);
+    for (int e = tid(); e < N * N; e += NT) M[e] = (e == dN.div(e) * (N + 1)) ? 1.0 : 0.0;
     __syncthreads();
 }
 __device__ void wg_scale_rows(double* M, int N, const double* sc) {
-    for (int e = tid(); e < N * N; e += NT) M[e] *= sc[e % N];
+    const FastDiv dN(N);
+    for (int e = tid(); e < N * N; e += NT) M[e] *= sc[e - dN.div(e) * N];
     __syncthreads();
 }
 __device__ void wg_scale_cols(double* M, int N, const double* sc) {
-    for (int e = tid(); e < N * N; e += NT) M[e] *= sc[e / N];
+    const FastDiv dN(N);
+    for (int e = tid(); e < N * N; e += NT) M[e] *= sc[dN.div(e)];
     __syncthreads();
 }
 
@@ -776,6 +790,13 @@ __device__ void wg_scale_cols(double* M, int N, const double* sc) {
 // Plaquette at (x,y) with x,y = start mod 2 acts on the sites
 // {(x,y), (x+1,y), (x,y+1), (x+1,y+1)}; P = [[a,b,b,c],[b,a,c,b],[b,c,a,b],[c,b,b,a]].
 // ---------------------------------------------------------------------------
+// sites of plaquette p = px + h py of one sublattice (start 0: even corners, 1: odd)
+__device__ __forceinline__ void plaq_sites_h(int p, int side, int start, const FastDiv& dh, int idx[4]) {
+    const int py = dh.div(p), px = p - py * dh.d;
+    const int x = start + 2 * px, y = start + 2 * py;   // <= side - 1
+    const int x1 = x + 1 == side ? 0 : x + 1, y1 = y + 1 == side ? 0 : y + 1;
+    idx[0] = x + side * y; idx[1] = x1 + side * y; idx[2] = x + side * y1; idx[3] = x1 + side * y1;
+}
 __device__ __forceinline__ void plaq_sites(int p, int side, int start, int idx[4]) {
     const int h = side >> 1;
     const int x = start + 2 * (p % h), y = start + 2 * (p / h);
@@ -822,10 +843,11 @@ __device__ void wg_cb_right(double* M, int N, int side, const AS_C double* P, in
 // write of the matrix instead of one per pass.  starts[] lists the sublattice
 // order (1 = B plaquettes at odd (x,y), 0 = A at even).
 __device__ void cb_pass_lds_left(double* T, int ld, int ncols, int side, const AS_C double* P, int start) {
-    const int np = (side / 2) * (side / 2);
+    const int h = side / 2, np = h * h;
+    const FastDiv dnp(np), dh(h);
     for (int e = tid(); e < np * ncols; e += NT) {
-        const int pq = e % np, col = e / np;
-        int idx[4]; plaq_sites(pq, side, start, idx);
+        const int col = dnp.div(e), pq = e - col * np;
+        int idx[4]; plaq_sites_h(pq, side, start, dh, idx);
         double* c = T + (size_t)col * ld;
         double v[4] = {c[idx[0]], c[idx[1]], c[idx[2]], c[idx[3]]}, o[4];
         plaq_apply(P, v, o);
@@ -835,10 +857,11 @@ __device__ void cb_pass_lds_left(double* T, int ld, int ncols, int side, const A
 }
 // T holds nr rows x N columns, element (r, c) at c * nr + r
 __device__ void cb_pass_lds_right(double* T, int nr, int side, const AS_C double* P, int start) {
-    const int np = (side / 2) * (side / 2);
+    const int h = side / 2, np = h * h;
+    const FastDiv dnr(nr), dh(h);
     for (int e = tid(); e < np * nr; e += NT) {
-        const int r = e % nr, pq = e / nr;
-        int idx[4]; plaq_sites(pq, side, start, idx);
+        const int pq = dnr.div(e), r = e - pq * nr;
+        int idx[4]; plaq_sites_h(pq, side, start, dh, idx);
         double v[4] = {T[idx[0] * nr + r], T[idx[1] * nr + r], T[idx[2] * nr + r], T[idx[3] * nr + r]}, o[4];
         plaq_apply(P, v, o);
         T[idx[0] * nr + r] = o[0]; T[idx[1] * nr + r] = o[1]; T[idx[2] * nr + r] = o[2]; T[idx[3] * nr + r] = o[3];
@@ -856,18 +879,19 @@ __device__ __noinline__ void wg_cb_left_fused(double* M, int N, int side, const 
     const bool has_pre = uni(pre) != nullptr, has_post = uni(post) != nullptr;
     const AS_L double* prel = (const AS_L double*)uni(pre);
     const AS_L double* postl = (const AS_L double*)uni(post);
+    const FastDiv dN(N);
     const int cb = max(1, min(N, PANEL_DOUBLES / N));
     for (int c0 = 0; c0 < N; c0 += cb) {
         const int nc = min(cb, N - c0);
         AS_G double* blk = Mg + (size_t)c0 * N;
         wg_stream<16>(nc * N, [&](int e) { return blk[e]; },
-                  [&](int e, double v) { s.panel[e] = has_pre ? v * prel[e % N] : v; });
+                  [&](int e, double v) { s.panel[e] = has_pre ? v * prel[e - dN.div(e) * N] : v; });
         __syncthreads();
         cb_pass_lds_left(s.panel, N, nc, side, P, s1);
         cb_pass_lds_left(s.panel, N, nc, side, P, s2);
         for (int e = tid(); e < nc * N; e += NT) {
             const double v = s.panel[e];
-            blk[e] = has_post ? v * postl[e % N] : v;
+            blk[e] = has_post ? v * postl[e - dN.div(e) * N] : v;
         }
         __syncthreads();
     }
@@ -886,13 +910,14 @@ __device__ __noinline__ void wg_cb_right_fused(double* M, int N, int side, const
     const int rb = max(1, min(N, PANEL_DOUBLES / N));
     for (int r0 = 0; r0 < N; r0 += rb) {
         const int nr = min(rb, N - r0);
-        wg_stream<16>(nr * N, [&](int e) { const int r = e % nr, c = e / nr; return Mg[(size_t)c * N + r0 + r]; },
-                  [&](int e, double v) { s.panel[e] = has_pre ? v * prel[e / nr] : v; });
+        const FastDiv dnr(nr);
+        wg_stream<16>(nr * N, [&](int e) { const int c = dnr.div(e), r = e - c * nr; return Mg[(size_t)c * N + r0 + r]; },
+                  [&](int e, double v) { s.panel[e] = has_pre ? v * prel[dnr.div(e)] : v; });
         __syncthreads();
         cb_pass_lds_right(s.panel, nr, side, P, s1);
         cb_pass_lds_right(s.panel, nr, side, P, s2);
         for (int e = tid(); e < nr * N; e += NT) {
-            const int r = e % nr, c = e / nr;
+            const int c = dnr.div(e), r = e - c * nr;
             const double v = s.panel[e];
             Mg[(size_t)c * N + r0 + r] = has_post ? v * postl[c] : v;
         }
@@ -910,22 +935,23 @@ __device__ __noinline__ void wg_cb_wrap_lds(Desc d, bool up, double* G_, const d
     AS_G double* const G = gptr(uni(G_));
     wg_stream(N * N, [&](int e) { return G[e]; }, [&](int e, double v) { s.panel[e] = v; });
     __syncthreads();
+    const FastDiv dN(N);
     if (up) {
         // B_left: E (B then A), rows * ep;  invB_right: E^-1 (B then A) on columns, columns * en
         cb_pass_lds_left(s.panel, N, N, side, d.P, 1);
         cb_pass_lds_left(s.panel, N, N, side, d.P, 0);
-        for (int e = tid(); e < N * N; e += NT) s.panel[e] *= ep[e % N];
+        for (int e = tid(); e < N * N; e += NT) s.panel[e] *= ep[e - dN.div(e) * N];
         __syncthreads();
         cb_pass_lds_right(s.panel, N, side, d.Pi, 1);
         cb_pass_lds_right(s.panel, N, side, d.Pi, 0);
-        for (int e = tid(); e < N * N; e += NT) G[e] = s.panel[e] * en[e / N];
+        for (int e = tid(); e < N * N; e += NT) G[e] = s.panel[e] * en[dN.div(e)];
     } else {
         // B_right: columns * ep, E (A then B);  invB_left: rows * en, E^-1 (A then B)
-        for (int e = tid(); e < N * N; e += NT) s.panel[e] *= ep[e / N];
+        for (int e = tid(); e < N * N; e += NT) s.panel[e] *= ep[dN.div(e)];
         __syncthreads();
         cb_pass_lds_right(s.panel, N, side, d.P, 0);
         cb_pass_lds_right(s.panel, N, side, d.P, 1);
-        for (int e = tid(); e < N * N; e += NT) s.panel[e] *= en[e % N];
+        for (int e = tid(); e < N * N; e += NT) s.panel[e] *= en[e - dN.div(e) * N];
         __syncthreads();
         cb_pass_lds_left(s.panel, N, N, side, d.Pi, 0);
         cb_pass_lds_left(s.panel, N, N, side, d.Pi, 1);
@@ -1249,15 +1275,16 @@ __device__ __noinline__ double wg_lu_solve(double* A, double* B, int N, int nrhs
     const AS_G double* Ag = gptr(A);
     for (int k0 = 0; k0 < N; k0 += NB) {
         const int jb = min(NB, N - k0), m = N - k0;
+        const FastDiv dm(m);
         SubClock pclk;
         pclk.start();
-        wg_stream(m * jb, [&](int e) { const int r = e % m, c = e / m; return Ag[(size_t)(k0 + c) * N + k0 + r]; },
+        wg_stream(m * jb, [&](int e) { const int c = dm.div(e), r = e - c * m; return Ag[(size_t)(k0 + c) * N + k0 + r]; },
                   [&](int e, double v) { P[e] = v; });
         __syncthreads();
         detsign *= lu_panel(m, jb, k0, s);
         pclk.stop(1);
         for (int e = tid(); e < m * jb; e += NT) {
-            const int r = e % m, c = e / m;
+            const int c = dm.div(e), r = e - c * m;
             gptr(A)[(size_t)(k0 + c) * N + k0 + r] = P[c * m + r];
         }
         // the panel's interchanges (row k0+j <-> perm[k0+j], j = 0..jb-1 in order) composed into
@@ -1715,14 +1742,16 @@ __device__ __noinline__ void wg_qr_udv(double* M, double* Mp, int N, double* U, 
         s.perm[r] = j;
     }
     __syncthreads();
-    wg_stream(N * N, [&](int e) { const int r = e % N, c = e / N; return Mg[(size_t)s.perm[c] * N + r]; },
+    const FastDiv dN(N);
+    wg_stream(N * N, [&](int e) { const int c = dN.div(e), r = e - c * N; return Mg[(size_t)s.perm[c] * N + r]; },
               [&](int e, double v) { Mpg[e] = v; });
     __syncthreads();
     for (int k0 = 0; k0 < N; k0 += NB) {
         const int jb = min(NB, N - k0), m = N - k0;
+        const FastDiv dm(m);
         SubClock pclk;
         pclk.start();
-        wg_stream(m * jb, [&](int e) { const int r = e % m, c = e / m; return Mpg[(size_t)(k0 + c) * N + k0 + r]; },
+        wg_stream(m * jb, [&](int e) { const int c = dm.div(e), r = e - c * m; return Mpg[(size_t)(k0 + c) * N + k0 + r]; },
                   [&](int e, double v) { P[e] = v; });
         __syncthreads();
         qr_panel(m, jb, k0, N, M, Mp, Tg, s);
@@ -1744,8 +1773,8 @@ __device__ __noinline__ void wg_qr_udv(double* M, double* Mp, int N, double* U, 
     __syncthreads();
     {
         AS_G double* Tlg = gptr(uni(Tl));
-        wg_stream(N * N, [&](int e) { const int r = e % N, c = e / N; return c >= r ? Mpg[e] / s.v2[r] : 0.0; },
-                  [&](int e, double v) { const int r = e % N, c = e / N; Tlg[(size_t)s.perm[c] * N + r] = v; });
+        wg_stream(N * N, [&](int e) { const int c = dN.div(e), r = e - c * N; return c >= r ? Mpg[e] / s.v2[r] : 0.0; },
+                  [&](int e, double v) { const int c = dN.div(e), r = e - c * N; Tlg[(size_t)s.perm[c] * N + r] = v; });
     }
     // Q = H_0 H_1 ... : U <- (I - V T V^T) U, panels from the last
     wg_identity(U, N);
@@ -1813,21 +1842,22 @@ __device__ __noinline__ void wg_transpose_scaled(double* dst1_, double* dst2_, c
     const int nb = max(1, min(N, PANEL_DOUBLES / N));
     for (int j0 = 0; j0 < N; j0 += nb) {
         const int nc = min(nb, N - j0);
+        const FastDiv dnc(nc);
         const AS_G double* blk = src + (size_t)j0 * N;
         wg_stream(nc * N, [&](int e) { return blk[e]; }, [&](int e, double v) { s.panel[e] = v; });
         __syncthreads();
         // element (r, c) of dst with r = j0 + e % nc, c = e / nc is src(c, r) = panel[c + (r - j0) N]
         if (dst2_) {
-            wg_stream(nc * N, [&](int e) { const int r = j0 + e % nc, c = e / nc; return add[(size_t)c * N + r]; },
+            wg_stream(nc * N, [&](int e) { const int c = dnc.div(e), r = j0 + e - c * nc; return add[(size_t)c * N + r]; },
                       [&](int e, double a) {
-                          const int rr = e % nc, r = j0 + rr, c = e / nc;
+                          const int c = dnc.div(e), rr = e - c * nc, r = j0 + rr;
                           const double v = (sc_ ? sc[r] : 1.0) * s.panel[c + rr * N];
                           dst1[(size_t)c * N + r] = v;
                           dst2[(size_t)c * N + r] = v + sa[r] * a;
                       });
         } else {
             for (int e = tid(); e < nc * N; e += NT) {
-                const int rr = e % nc, r = j0 + rr, c = e / nc;
+                const int c = dnc.div(e), rr = e - c * nc, r = j0 + rr;
                 dst1[(size_t)c * N + r] = (sc_ ? sc[r] : 1.0) * s.panel[c + rr * N];
             }
         }
@@ -1951,6 +1981,7 @@ __device__ __noinline__ void block_stage_w(const double* G_, int N, int b0, int 
     N = uni(N); b0 = uni(b0); k = uni(k); sp = uni(sp);
     const AS_G double* G = gptr(uni(G_));
     const int k4 = (k + 3) & ~3;
+    const FastDiv dk4(k4);
     const int l = lane(), w = wave();
     double* const acp = s.panel;
     double* const gsw = acp + (size_t)N * k4;
@@ -1958,8 +1989,8 @@ __device__ __noinline__ void block_stage_w(const double* G_, int N, int b0, int 
     // ACP and GSW together (GSW element e = c k4 + b: consecutive threads read down a column of G)
     wg_stream2(N * k4, [&](int e) { return G[(size_t)b0 * N + min(e, N * k - 1)]; },
                [&](int e, double v) { acp[e] = e < N * k ? v : 0.0; },
-               [&](int e) { const int c = e / k4, b = e - c * k4; return G[(size_t)c * N + b0 + min(b, k - 1)]; },
-               [&](int e, double v) { const int c = e / k4, b = e - c * k4; gsw[b * N + c] = b < k ? v : 0.0; });
+               [&](int e) { const int c = dk4.div(e), b = e - c * k4; return G[(size_t)c * N + b0 + min(b, k - 1)]; },
+               [&](int e, double v) { const int c = dk4.div(e), b = e - c * k4; gsw[b * N + c] = b < k ? v : 0.0; });
     __syncthreads();
     // W = C G0[S,:] - C E_S, column block by column block (each wavefront owns whole column blocks,
     // so W may overwrite GSW in place); a wavefront advances up to 4 of its column blocks together.
@@ -2443,7 +2474,8 @@ __device__ __noinline__ void dynamic_fresh(Desc d, int lslot, int rslot, ChainSt
             const AS_G double* src = gptr(TL);
             AS_G double* dst = gptr(W2);
             const AS_L double* v1 = (const AS_L double*)s.v1;
-            wg_stream((int)n2, [&](int e) { return src[e]; }, [&](int e, double v) { dst[e] = v1[e % N] * v; });
+            const FastDiv dN(N);
+            wg_stream((int)n2, [&](int e) { return src[e]; }, [&](int e, double v) { dst[e] = v1[e - dN.div(e) * N] * v; });
             __syncthreads();
         }
         wg_lu_solve(W0, W1, N, 2 * N, s);
@@ -2831,13 +2863,14 @@ __device__ void sweep_dynamic(Desc d, ChainState& st, double sign, int flags, Ld
     const size_t n2 = (size_t)N * N;
     const bool rec = flags & DQMC_FLAG_RECORD, meas = flags & DQMC_FLAG_MEASURE;
     PhaseClock clk(flags & DQMC_FLAG_PROFILE);
+    const FastDiv dN(N);
     for (int sp = 0; sp < d.nspin; ++sp) {
         const AS_G double* g = gptr(d.G + sp * n2);
         AS_G double* a = gptr(d.Gt0 + sp * n2);
         AS_G double* b = gptr(d.G0t + sp * n2);
         wg_stream((int)n2, [&](int e) { return g[e]; }, [&](int e, double v) {
             a[e] = v;
-            b[e] = (e % N == e / N) ? v - 1.0 : v;
+            b[e] = (e == dN.div(e) * (N + 1)) ? v - 1.0 : v;
         });
         wg_identity(d.Tmp + sp * n2, N);
     }
