@@ -1970,8 +1970,9 @@ __device__ void stage_gss(const double* G_, int N, int b0, int k, int sp, Lds&) 
     double* gt = MP_GT + 32 * sp;
     for (int q = tid(); q < KB * KB; q += NT) { gr[(q >> 5) * 64 + (q & 31)] = 0.0; gt[(q >> 5) * 64 + (q & 31)] = 0.0; }
     __syncthreads();
-    wg_stream(k * k, [&](int q) { const int a = q % k, c = q / k; return Gg[(size_t)(b0 + c) * N + b0 + a]; },
-              [&](int q, double v) { const int a = q % k, c = q / k; gr[c * 64 + a] = v; gt[a * 64 + c] = v; });
+    const FastDiv dk(k);
+    wg_stream(k * k, [&](int q) { const int c = dk.div(q), a = q - c * k; return Gg[(size_t)(b0 + c) * N + b0 + a]; },
+              [&](int q, double v) { const int c = dk.div(q), a = q - c * k; gr[c * 64 + a] = v; gt[a * 64 + c] = v; });
 }
 
 // The block's update of G for one spin, on the matrix cores with operands in LDS.
@@ -2110,8 +2111,9 @@ __device__ __noinline__ void gss_from_factors(const double* G_, int N, int k, in
     double* gt = MP_GT + 32 * sp;
     for (int q = tid(); q < KB * KB; q += NT) { gr[(q >> 5) * 64 + (q & 31)] = 0.0; gt[(q >> 5) * 64 + (q & 31)] = 0.0; }
     __syncthreads();
+    const FastDiv dk1(k1);
     for (int q = tid(); q < k1 * k1; q += NT) {
-        const int a = q % k1, c = q / k1;
+        const int c = dk1.div(q), a = q - c * k1;
         double v = G[(size_t)(b1 + c) * N + b1 + a];
         for (int j = 0; j < k4; ++j) v += acp[j * N + b1 + a] * gsw[j * N + b1 + c];
         gr[c * 64 + a] = v;
@@ -2327,6 +2329,7 @@ __device__ __noinline__ WalkOut wg_metropolis(Desc d, int e, uint64_t rng, doubl
 __device__ __noinline__ void wg_measure_from(Desc d, const double* G, double cs, Lds&) {
     Lds& s = g_lds;
     const int N = d.N, side = d.side;
+    const FastDiv dN(N), dS(side);
     const AS_G double* Gu = gptr(G);
     const AS_G double* Gd = d.nspin == 2 ? Gu + (size_t)N * N : Gu;
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // tr, docc, kin, loc, md, sdw, cdw, sw
@@ -2350,9 +2353,10 @@ __device__ __noinline__ void wg_measure_from(Desc d, const double* G, double cs,
 #pragma unroll
       for (int u = 0; u < MU; ++u) {
         const int p = min(p0 + u * NT, N * N - 1);
-        const int i = p % N, j = p / N;
-        const int xi = i % side, yi = i / side, xj = j % side, yj = j / side;
-        const int dsp = (xj - xi + side) % side + side * ((yj - yi + side) % side);
+        const int j = dN.div(p), i = p - j * N;
+        const int yi = dS.div(i), xi = i - yi * side, yj = dS.div(j), xj = j - yj * side;
+        const int dx = xj - xi + side, dy = yj - yi + side;   // in [1, 2 side - 1]
+        const int dsp = (dx >= side ? dx - side : dx) + side * (dy >= side ? dy - side : dy);
         q_ff[u] = cosq[dsp];
         q_gu_ij[u] = Gug[(size_t)j * N + i]; q_gd_ij[u] = Gdg[(size_t)j * N + i];
         q_gu_ji[u] = Gug[(size_t)i * N + j]; q_gd_ji[u] = Gdg[(size_t)i * N + j];
@@ -2361,7 +2365,7 @@ __device__ __noinline__ void wg_measure_from(Desc d, const double* G, double cs,
       for (int u = 0; u < MU; ++u) {
         const int p = p0 + u * NT;
         if (p >= N * N) break;
-        const int i = p % N, j = p / N;
+        const int j = dN.div(p), i = p - j * N;
         const double ff = q_ff[u];
         const double gu_ij = q_gu_ij[u], gd_ij = q_gd_ij[u], gu_ji = q_gu_ji[u], gd_ji = q_gd_ji[u];
         const double dij = (i == j) ? 1.0 : 0.0;
