@@ -360,7 +360,7 @@ def main():
         summary, _ = serve.run(ctx, jobs, args.serve_rate, 64 << 30, log=sys.stderr)
         ctx.close()
         summary["job_mix"] = (f"side in {{4..20}}, beta U[2,12], dtau in {{0.05,0.1}}, U U[-8,8], mu U[-1,1], "
-                              f"sweeps U[50,500] x {args.serve_scale}, seed = job id, mixed precision, checkerboard")
+                              f"sweeps U[50,500] x {args.serve_scale}, seed = job id, FP64, checkerboard")
         out["serving"] = summary
     if not args.no_cpu:
         print("[bench] CPU baseline (oracle, one chain per process)", file=sys.stderr, flush=True)

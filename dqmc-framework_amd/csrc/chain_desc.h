@@ -4,9 +4,12 @@
 #pragma once
 #include <stdint.h>
 
+#include "../../include/dqmc_amd.h"   // DQMC_STATUS_* bits of ChainState::status
+
 #define DQMC_NT 256          // threads per workgroup (4 wavefronts of 64)
 #define DQMC_NMAX 448        // largest lattice (sites) a chain may have (side <= 21)
-#define DQMC_KMAX 64         // largest delayed-update depth
+#define DQMC_KMAX 32         // largest delayed-update depth (a Metropolis block holds its sites in the 32
+                             // lanes of a half-wavefront per spin)
 #define DQMC_PANEL_DOUBLES 11776   // LDS panel of a workgroup (92 KB)
 
 enum {
@@ -19,7 +22,16 @@ enum {
 };
 
 // building blocks run alone by the single-kernel entry points (one workgroup, chain 0)
-enum { DQMC_UNIT_BMUL = 0, DQMC_UNIT_RANK1 = 1, DQMC_UNIT_PUSH = 2, DQMC_UNIT_FRESH = 3 };
+enum { DQMC_UNIT_BMUL = 0, DQMC_UNIT_RANK1 = 1, DQMC_UNIT_PUSH = 2, DQMC_UNIT_FRESH = 3, DQMC_UNIT_RATIO = 4,
+       DQMC_UNIT_FLIP = 5, DQMC_UNIT_PUSH_EXT = 6, DQMC_UNIT_FRESH_EXT = 7 };
+// caller buffers of the building blocks that run on device memory the caller owns
+struct DqmcUnitPtrs {
+    const double *A, *U_in, *D_in, *V_in;          // PUSH_EXT: pushed matrix and the stack top
+    double *U, *D, *V;                             // PUSH_EXT: result
+    const double *UL, *DL, *VL, *UR, *DR, *VR;     // FRESH_EXT: the two stack tops
+    double* out;                                   // FRESH_EXT: G; RATIO: the ratio
+    int* status;                                   // failure bits of the call (may be null)
+};
 // phases timed with DQMC_FLAG_PROFILE (100 MHz device wall clock ticks)
 enum { DQMC_PH_WRAP = 0, DQMC_PH_METROPOLIS, DQMC_PH_PRODUCT, DQMC_PH_PUSH, DQMC_PH_FRESH, DQMC_PH_MEASURE,
        DQMC_PH_WALK,     // part of METROPOLIS: the in-block decisions and C updates (one wavefront)
@@ -30,6 +42,7 @@ enum { DQMC_PH_WRAP = 0, DQMC_PH_METROPOLIS, DQMC_PH_PRODUCT, DQMC_PH_PUSH, DQMC
 // sign, filling, double occupancy, kinetic energy, momentum distribution,
 // local spin correlation, SDW structure factor, CDW structure factor, s-wave pairing
 #define DQMC_NUM_OBS 9
+#define DQMC_ACC_SCRATCH 12   // accumulator slot (of 16) that carries a per-site update ratio to the host
 
 // dynamic accumulators of a chain (L slices, N sites), in this order:
 // gr[L][N]  sign x sum over site pairs (i, j = i + r) of the spin-averaged G(t,0)(j,i), per
@@ -44,7 +57,7 @@ struct ChainState {
     double sign_top;     // sign recorded at slice L of the running backward sweep
     int pos;             // 0: at slice 0 (next sweep goes up), L: at slice L (next goes down)
     int n_right;         // length of the right UDV stack (left length = n_stack - n_right)
-    int status;          // 0 ok; nonzero: numerical failure code
+    int status;          // DQMC_STATUS_* bits (0 ok)
     int accepted;        // accepted flips in the last launch (for reporting)
     unsigned long long phase[8];  // device ticks per phase (DQMC_FLAG_PROFILE)
 };
@@ -54,6 +67,7 @@ struct ChainDesc {
     int launch_sweeps;   // > 0: sweeps of this chain in the current launch (overrides the launch argument)
     int launch_flags;    // >= 0: flags of this chain in the current launch (overrides the launch argument)
     double hop_t;
+    double wrap_limit;   // wrap error above which the chain is flagged DQMC_STATUS_WRAP_LIMIT
     double ev[2];        // exp(+alpha), exp(-alpha): exp(alpha*s) for s=+1, s=-1
     double dm1[2][2];    // [spin][s<0]: exp(-2 sigma alpha s) - 1 entering ratio and gamma
     double pref[2];      // attractive ratio prefactor exp(2 alpha s) for s=+1, s=-1 (1 for repulsive)
@@ -73,7 +87,7 @@ struct ChainDesc {
     const double* cosq;  // N: Re e^{-i Q r_d} for displacement index d
     int8_t* field;       // L x N, (t,i) at t*N+i
     ChainState* st;
-    double* acc;         // DQMC_NUM_OBS sums
+    double* acc;         // 16 doubles: DQMC_NUM_OBS sums, then scratch (DQMC_ACC_SCRATCH)
     double* Gt0;         // nspin x N*N   G(t,0) of the time-displaced sweep
     double* G0t;         // nspin x N*N   G(0,t)
     double* dacc;        // DQMC_DYN_ACC_SIZE(L, N) dynamic sums

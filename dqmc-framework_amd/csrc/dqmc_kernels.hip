@@ -59,6 +59,7 @@ struct Lds {
     double red[16];
     double gam[2];
     int ivar[4];
+    int status;                   // DQMC_STATUS_* bits met by the running launch (thread 0 / panel lanes set them)
     unsigned long long prof[2];   // sub-phase ticks (walk, panels) of the running launch
     int prof_on;
 };
@@ -146,6 +147,26 @@ __device__ __forceinline__ double wave_max(double v) {
     swap32_d(a, b);
     return fmax(a, b);
 }
+// maximum that propagates NaN from either operand (fmax drops it), so that a
+// non-finite Green's function cannot report a small error
+__device__ __forceinline__ double nan_max(double a, double b) { return (a != a || a > b) ? a : b; }
+__device__ __forceinline__ double wave_max_nan(double v) {
+    v = nan_max(v, dpp_d<0xB1>(v));
+    v = nan_max(v, dpp_d<0x4E>(v));
+    v = nan_max(v, dpp_d<0x141>(v));
+    v = nan_max(v, dpp_d<0x140>(v));
+    double a = v, b = v;
+    swap16_d(a, b);
+    v = nan_max(a, b);
+    a = v; b = v;
+    swap32_d(a, b);
+    return nan_max(a, b);
+}
+// a pivot the elimination cannot divide by safely: zero, subnormal, inf or NaN
+__device__ __forceinline__ bool bad_pivot(double piv) {
+    const double a = fabs(piv);
+    return !(a >= 2.2250738585072014e-308 && a <= 1.7976931348623157e308);
+}
 // argmax with smallest index on ties (the result does not depend on the pairing order)
 __device__ __forceinline__ void argmax_step(double& v, int& idx, double ov, int oi) {
     if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
@@ -228,10 +249,10 @@ __device__ void wg_sum8(double* v, int n, Lds&) {
 }
 __device__ double wg_max(double v, Lds&) {
     Lds& s = g_lds;
-    v = wave_max(v);
+    v = wave_max_nan(v);
     if (lane() == 0) s.red[wave()] = v;
     __syncthreads();
-    double r = fmax(fmax(s.red[0], s.red[1]), fmax(s.red[2], s.red[3]));
+    double r = nan_max(nan_max(s.red[0], s.red[1]), nan_max(s.red[2], s.red[3]));
     __syncthreads();
     return r;
 }
@@ -699,11 +720,19 @@ __device__ __forceinline__ void gemm64_ll(int M, int Nc, int K, const double* A,
     wg_gemm_mfma<false, true, true>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, nullptr, nullptr, nullptr, beta, s,
                                     alpha);
 }
-__device__ __forceinline__ void gemmB(Desc d, int M, int Nc, int K, const double* A, int lda, bool tA,
+// FP32 matrix cores for this product: every B product in DQMC_PREC_MIXED, only the
+// propagations of G (wraps, and G(t,0), G(0,t) of the time-displaced sweep) in
+// DQMC_PREC_FP32_WRAP, only the slice products accumulated into the stacks in
+// DQMC_PREC_FP32_PRODUCT
+__device__ __forceinline__ bool lowp_product(Desc d, bool wrap) {
+    const int p = d.precision;
+    return p == DQMC_PREC_MIXED || (p == DQMC_PREC_FP32_WRAP && wrap) || (p == DQMC_PREC_FP32_PRODUCT && !wrap);
+}
+__device__ __forceinline__ void gemmB(Desc d, bool wrap, int M, int Nc, int K, const double* A, int lda, bool tA,
                                       const double* B, int ldb, bool tB, double* C, int ldc, const double* rs,
                                       const double* ms, const double* cs, Lds&) {
     Lds& s = g_lds;
-    if (d.precision == 1)
+    if (lowp_product(d, wrap))
         wg_gemm_mfma<true, false, false>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, 0.0, s);
     else wg_gemm_mfma<false, false, false>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, 0.0, s);
 }
@@ -993,7 +1022,8 @@ __device__ __forceinline__ const double* vneg(Desc d, int sp, Lds&) {
 
 // B(t) products on an N x N matrix M (in place; W is N*N scratch for the dense path).
 // op: 0 B_left, 1 B_right, 2 invB_left, 3 invB_right, 4 transB_left
-__device__ void wg_bmul(Desc d, int op, int sp, double* M, double* W, Lds&) {
+// (wrap: the product propagates G, which selects its arithmetic in the FP32 modes)
+__device__ void wg_bmul(Desc d, int op, int sp, double* M, double* W, Lds&, bool wrap = false) {
     Lds& s = g_lds;
     const int N = d.N;
     const double* ep = vpos(d, sp, s);
@@ -1009,11 +1039,11 @@ __device__ void wg_bmul(Desc d, int op, int sp, double* M, double* W, Lds&) {
         return;
     }
     switch (op) {
-        case 0: gemmB(d, N, N, N, d.expK, N, false, M, N, false, W, N, ep, nullptr, nullptr, s); break;
-        case 1: gemmB(d, N, N, N, M, N, false, d.expK, N, false, W, N, nullptr, ep, nullptr, s); break;
-        case 2: gemmB(d, N, N, N, d.invExpK, N, false, M, N, false, W, N, nullptr, en, nullptr, s); break;
-        case 3: gemmB(d, N, N, N, M, N, false, d.invExpK, N, false, W, N, nullptr, nullptr, en, s); break;
-        case 4: gemmB(d, N, N, N, d.expK, N, true, M, N, false, W, N, nullptr, ep, nullptr, s); break;
+        case 0: gemmB(d, wrap, N, N, N, d.expK, N, false, M, N, false, W, N, ep, nullptr, nullptr, s); break;
+        case 1: gemmB(d, wrap, N, N, N, M, N, false, d.expK, N, false, W, N, nullptr, ep, nullptr, s); break;
+        case 2: gemmB(d, wrap, N, N, N, d.invExpK, N, false, M, N, false, W, N, nullptr, en, nullptr, s); break;
+        case 3: gemmB(d, wrap, N, N, N, M, N, false, d.invExpK, N, false, W, N, nullptr, nullptr, en, s); break;
+        case 4: gemmB(d, wrap, N, N, N, d.expK, N, true, M, N, false, W, N, nullptr, ep, nullptr, s); break;
     }
     wg_copy(M, W, N * N);
 }
@@ -1026,17 +1056,17 @@ __device__ void wg_wrap(Desc d, int sp, bool up, double* G, double* W, Lds&) {
         const double* ep = vpos(d, sp, s);
         const double* en = vneg(d, sp, s);
         if (up) {
-            gemmB(d, N, N, N, d.expK, N, false, G, N, false, W, N, ep, nullptr, nullptr, s);
-            gemmB(d, N, N, N, W, N, false, d.invExpK, N, false, G, N, nullptr, nullptr, en, s);
+            gemmB(d, true, N, N, N, d.expK, N, false, G, N, false, W, N, ep, nullptr, nullptr, s);
+            gemmB(d, true, N, N, N, W, N, false, d.invExpK, N, false, G, N, nullptr, nullptr, en, s);
         } else {
-            gemmB(d, N, N, N, G, N, false, d.expK, N, false, W, N, nullptr, ep, nullptr, s);
-            gemmB(d, N, N, N, d.invExpK, N, false, W, N, false, G, N, nullptr, en, nullptr, s);
+            gemmB(d, true, N, N, N, G, N, false, d.expK, N, false, W, N, nullptr, ep, nullptr, s);
+            gemmB(d, true, N, N, N, d.invExpK, N, false, W, N, false, G, N, nullptr, en, nullptr, s);
         }
         return;
     }
     if (N * N <= PANEL_DOUBLES) { wg_cb_wrap_lds(d, up, G, vpos(d, sp, s), vneg(d, sp, s), s); return; }
-    if (up) { wg_bmul(d, 0, sp, G, W, s); wg_bmul(d, 3, sp, G, W, s); }
-    else { wg_bmul(d, 1, sp, G, W, s); wg_bmul(d, 2, sp, G, W, s); }
+    if (up) { wg_bmul(d, 0, sp, G, W, s, true); wg_bmul(d, 3, sp, G, W, s, true); }
+    else { wg_bmul(d, 1, sp, G, W, s, true); wg_bmul(d, 2, sp, G, W, s, true); }
 }
 
 // ---------------------------------------------------------------------------
@@ -1116,6 +1146,7 @@ __device__ __forceinline__ double lu_panel_wave(int m, int jb, int k0) {
         }
         const double piv = rowp[j];
         if (piv < 0) detsign = -detsign;
+        if (bad_pivot(piv) && L == 0) s.status |= DQMC_STATUS_SINGULAR;
 #pragma unroll
         for (int i = 0; i < RPL; ++i) {
             const int r = L + 64 * i;
@@ -1229,6 +1260,7 @@ __device__ __noinline__ double lu_panel(int m, int jb, int k0, Lds&) {
         }
         const double piv = rowp[j];
         if (piv < 0) detsign = -detsign;
+        if (bad_pivot(piv) && t == 0) s.status |= DQMC_STATUS_SINGULAR;
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
             const int r = t + i * NT;
@@ -2409,6 +2441,16 @@ __device__ void record_slot(Desc d, int slot, double sign, bool with_sign) {
     __syncthreads();
 }
 
+// the status bits of a recompute's error |G_fresh - G_wrapped| (NaN or inf when either held
+// non-finite values) and the chain's running maximum (thread 0)
+__device__ __forceinline__ void note_error(Desc d, ChainState& st, double err, bool track_error) {
+    Lds& s = g_lds;
+    if (!(err <= 1.7976931348623157e308)) s.status |= DQMC_STATUS_NONFINITE;
+    if (!track_error) return;
+    if (err > d.wrap_limit) s.status |= DQMC_STATUS_WRAP_LIMIT;
+    if (!(err <= st.wrap_error)) st.wrap_error = err;
+}
+
 // fresh G for both spins into Gw, wrap error against G, then G <- Gw
 __device__ __noinline__ void stabilize_green(Desc d, int lslot, int rslot, bool track_error, ChainState& st, Lds&) {
     Lds& s = g_lds;
@@ -2420,10 +2462,10 @@ __device__ __noinline__ void stabilize_green(Desc d, int lslot, int rslot, bool 
         wg_fresh_green(d, sp, lslot, rslot, Gf, s);
         const AS_G double* Gfg = gptr(Gf);
         const AS_G double* Gg = gptr(d.G + sp * n2);
-        wg_stream((int)n2, [&](int e) { return fabs(Gfg[e] - Gg[e]); }, [&](int, double v) { err = fmax(err, v); });
+        wg_stream((int)n2, [&](int e) { return fabs(Gfg[e] - Gg[e]); }, [&](int, double v) { err = nan_max(err, v); });
     }
     err = wg_max(err, s);
-    if (track_error && tid() == 0 && err > st.wrap_error) st.wrap_error = err;
+    if (tid() == 0) note_error(d, st, err, track_error);
     wg_copy(d.G, d.Gw, (int)(n2 * d.nspin));
 }
 
@@ -2500,13 +2542,14 @@ __device__ __noinline__ void dynamic_fresh(Desc d, int lslot, int rslot, ChainSt
         AS_G double* gt0 = gptr(d.Gt0 + sp * n2);
         AS_G double* g0t = gptr(d.G0t + sp * n2);
         wg_stream2((int)n2, [&](int e) { return ft0[e]; },
-                   [&](int e, double v) { err = fmax(err, fabs(v - gt0[e])); gt0[e] = v; },
+                   [&](int e, double v) { err = nan_max(err, fabs(v - gt0[e])); gt0[e] = v; },
                    [&](int e) { return f0t[e]; },
-                   [&](int e, double v) { err = fmax(err, fabs(-v - g0t[e])); g0t[e] = -v; });
+                   [&](int e, double v) { err = nan_max(err, fabs(-v - g0t[e])); g0t[e] = -v; });
         __syncthreads();
     }
     err = wg_max(err, s);
-    if (tid() == 0 && err > st.wrap_error) st.wrap_error = err;
+    if (tid() == 0) note_error(d, st, err, true);
+    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -2887,8 +2930,8 @@ __device__ void sweep_dynamic(Desc d, ChainState& st, double sign, int flags, Ld
         for (int sp = 0; sp < d.nspin; ++sp) wg_wrap(d, sp, true, d.G + sp * n2, d.Gw + sp * n2, s);
         clk.lap(st, DQMC_PH_WRAP);
         for (int sp = 0; sp < d.nspin; ++sp) {
-            wg_bmul(d, 0, sp, d.Gt0 + sp * n2, d.Gw + sp * n2, s);
-            wg_bmul(d, 3, sp, d.G0t + sp * n2, d.Gw + sp * n2, s);
+            wg_bmul(d, 0, sp, d.Gt0 + sp * n2, d.Gw + sp * n2, s, true);
+            wg_bmul(d, 3, sp, d.G0t + sp * n2, d.Gw + sp * n2, s, true);
             wg_bmul(d, 0, sp, d.Tmp + sp * n2, d.Gw + sp * n2, s);
         }
         clk.lap(st, DQMC_PH_PRODUCT);
@@ -2920,7 +2963,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1)))
     ChainState st = *d.st;
     const int N = d.N, L = d.L;
     const size_t n2 = (size_t)N * N;
-    if (tid() == 0) s.prof_on = 0;
+    if (tid() == 0) { s.prof_on = 0; s.status = 0; }
     __syncthreads();
     for (int sp = 0; sp < d.nspin; ++sp) wg_identity(d.Tmp + sp * n2, N);
     st.n_right = 0;
@@ -2935,6 +2978,13 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1)))
         }
     }
     for (int sp = 0; sp < d.nspin; ++sp) wg_fresh_green(d, sp, -1, st.n_right - 1, d.G + sp * n2, s);
+    {   // a non-finite G(0) (e.g. from overflowing products) is a failed initialization
+        double m = 0.0;
+        const AS_G double* g = gptr(d.G);
+        wg_stream((int)(n2 * d.nspin), [&](int e) { return fabs(g[e]); }, [&](int, double v) { m = nan_max(m, v); });
+        m = wg_max(m, s);
+        if (tid() == 0 && !(m <= 1.7976931348623157e308)) s.status |= DQMC_STATUS_NONFINITE;
+    }
     // sign of det(G_up) det(G_dn)  (src/dqmc_walker.cpp:162)
     double sg = 1.0;
     for (int sp = 0; sp < d.nspin; ++sp) {
@@ -2947,7 +2997,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1)))
         st.sign = sg >= 0 ? 1.0 : -1.0;
         st.wrap_error = 0.0;
         st.pos = 0;
-        st.status = 0;
+        st.status = s.status;
         st.accepted = 0;
         for (int q = 0; q < 8; ++q) st.phase[q] = 0;
         *d.st = st;
@@ -2966,15 +3016,20 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1)))
     if (tid() == 0) {
         s.prof[0] = s.prof[1] = 0;
         s.prof_on = (flags & DQMC_FLAG_PROFILE) ? 1 : 0;
+        s.status = st.status;
     }
     __syncthreads();
-    for (int k = 0; k < n_sweeps; ++k) {
+    // a chain that met a fatal numerical failure sweeps no further (every thread reads the
+    // status after a barrier, so the whole workgroup leaves the loop together)
+    for (int k = 0; k < n_sweeps && !(s.status & DQMC_STATUS_FATAL); ++k) {
         if (st.pos == 0) {
             if (flags & DQMC_FLAG_DYNAMIC) sweep_dynamic(d, st, sign, flags, s);
             else sweep_up(d, st, rng, sign, flags, nacc, s);
         } else sweep_down(d, st, rng, sign, flags, nacc, s);
+        __syncthreads();
     }
     if (tid() == 0) {
+        st.status = s.status;
         st.rng = rng;
         st.sign = sign;
         st.accepted = nacc;
@@ -3061,6 +3116,32 @@ __device__ void wg_rank1(Desc d, int i, double* Gu_, double* Gd_, Lds&) {
     }
 }
 
+// Update ratio of flipping the field at (t, i) from the current G of both spins
+// (RepulsiveHubbard::get_update_ratio, src/model/repulsive_hubbard.cpp:112-124;
+// AttractiveHubbard::get_update_ratio, src/model/attractive_hubbard.cpp:112-125):
+//   r = pref(s) [1 + (1 - G_up(i,i)) m_up(s)] [1 + (1 - G_dn(i,i)) m_dn(s)],
+// the same expression the submatrix walk evaluates for its decisions.
+__device__ double unit_ratio(Desc d, int t, int i, const double* Gu, const double* Gd) {
+    const int N = d.N;
+    const int f = ((const AS_G int8_t*)d.field)[(size_t)t * N + i];
+    const bool up = f > 0;
+    const double m0 = up ? d.dm1[0][0] : d.dm1[0][1], m1 = up ? d.dm1[1][0] : d.dm1[1][1];
+    const double g0 = gptr(Gu)[(size_t)i * N + i];
+    const double g1 = (d.nspin == 2 && Gd) ? gptr(Gd)[(size_t)i * N + i] : g0;
+    const double x0 = 1.0 + (1.0 - g0) * m0, x1 = 1.0 + (1.0 - g1) * m1;
+    return (up ? d.pref[0] : d.pref[1]) * x0 * x1;
+}
+
+// stack slot k of spin 0 <- (U, D, V^T) from caller buffers (U, V column-major N x N)
+__device__ void unit_put_slot(Desc d, int k, const double* U, const double* D, const double* V, Lds&) {
+    Lds& s = g_lds;
+    const int N = uni(d.N);
+    wg_copy(slotU(d, 0, k), U, N * N);
+    for (int i = tid(); i < N; i += NT) gptr(slotD(d, 0, k))[i] = gptr(D)[i];
+    wg_transpose_scaled(slotT(d, 0, k), nullptr, V, nullptr, N, nullptr, nullptr, s);
+    __syncthreads();
+}
+
 // One building block of the sweep on chain 0 of a descriptor array (one workgroup):
 //   DQMC_UNIT_BMUL   X <- B(a) X or its variants (op c: 0 B X, 1 X B, 2 B^-1 X, 3 X B^-1,
 //                    4 B^T X; 5 B X B^-1, 6 B^-1 X B) for spin index b
@@ -3069,12 +3150,36 @@ __device__ void wg_rank1(Desc d, int i, double* Gu_, double* Gd_, Lds&) {
 //   DQMC_UNIT_PUSH   push X onto spin-0 stack slot b (-1: empty stack), result in slot a
 //   DQMC_UNIT_FRESH  X = (1 + B_left B_right^T)^-1 from spin-0 stack slots a (left) and b
 //                    (right); -1 marks an empty stack
+//   DQMC_UNIT_RATIO  *ext.out = the update ratio of the field at (a, b) from X (up), Y (down)
+//   DQMC_UNIT_FLIP   the field at (a, b) changes sign
+//   DQMC_UNIT_PUSH_EXT   (ext.U, ext.D, ext.V) = push of ext.A onto the top (ext.U_in, D_in,
+//                    V_in; U_in null: empty stack), through stack slots 0 and 1
+//   DQMC_UNIT_FRESH_EXT  ext.G = (1 + B_L B_R^T)^-1 from the tops (ext.UL, DL, VL) and (ext.UR,
+//                    DR, VR) (UL or UR null: that stack is empty), through slots 0 and 1
+// The failure bits of PUSH, FRESH and their external forms go to the chain state and, for the
+// external forms, to *ext.status when it is given.
+struct UnitPtrs {
+    const double *A, *U_in, *D_in, *V_in;
+    double *U, *D, *V;
+    const double *UL, *DL, *VL, *UR, *DR, *VR;
+    double* out;     // the recompute's G, or the update ratio (one double)
+    int* status;     // failure bits of the call (may be null)
+};
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) k_unit(const ChainDesc* __restrict__ descs, int unit, int a, int b, int c,
-                                             double* X, double* Y) {
+                                             double* X, double* Y, UnitPtrs ext) {
     Lds& s = g_lds;
     Desc d = ((const AS_C ChainDesc*)descs)[0];
-    if (tid() == 0) s.prof_on = 0;
+    if (tid() == 0) { s.prof_on = 0; s.status = 0; }
     __syncthreads();
+    // a non-finite entry among the first n of M sets DQMC_STATUS_NONFINITE
+    auto check_finite = [&](const double* M, int n) {
+        double m = 0.0;
+        const AS_G double* g = gptr(M);
+        wg_stream(n, [&](int e) { return fabs(g[e]); }, [&](int, double v) { m = nan_max(m, v); });
+        m = wg_max(m, s);
+        if (tid() == 0 && !(m <= 1.7976931348623157e308)) s.status |= DQMC_STATUS_NONFINITE;
+    };
+    const int N = uni(d.N);
     switch (unit) {
         case DQMC_UNIT_BMUL:
             load_field_row(d, a, s);
@@ -3085,9 +3190,57 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1)))
             load_field_row(d, a, s);
             wg_rank1(d, b, X, Y, s);
             break;
-        case DQMC_UNIT_PUSH: wg_push(d, 0, a, b, X, s); break;
-        case DQMC_UNIT_FRESH: wg_fresh_green(d, 0, a, b, X, s); break;
+        case DQMC_UNIT_PUSH:
+            wg_push(d, 0, a, b, X, s);
+            check_finite(slotU(d, 0, a), N * N);
+            check_finite(slotT(d, 0, a), N * N);
+            break;
+        case DQMC_UNIT_FRESH:
+            wg_fresh_green(d, 0, a, b, X, s);
+            check_finite(X, N * N);
+            break;
+        case DQMC_UNIT_RATIO:
+            if (tid() == 0) gptr(ext.out)[0] = unit_ratio(d, a, b, X, Y);
+            break;
+        case DQMC_UNIT_FLIP:
+            if (tid() == 0) {
+                AS_G int8_t* f = (AS_G int8_t*)d.field + (size_t)a * N + b;
+                *f = (int8_t)(-*f);
+            }
+            break;
+        case DQMC_UNIT_PUSH_EXT: {
+            const bool top = ext.U_in != nullptr;
+            if (top) unit_put_slot(d, 0, ext.U_in, ext.D_in, ext.V_in, s);
+            wg_push(d, 0, 1, top ? 0 : -1, ext.A, s);
+            __syncthreads();
+            wg_copy(ext.U, slotU(d, 0, 1), N * N);
+            for (int i = tid(); i < N; i += NT) gptr(ext.D)[i] = gptr(slotD(d, 0, 1))[i];
+            wg_transpose_scaled(ext.V, nullptr, slotT(d, 0, 1), nullptr, N, nullptr, nullptr, s);
+            __syncthreads();
+            check_finite(ext.U, N * N);
+            check_finite(ext.V, N * N);
+            break;
+        }
+        case DQMC_UNIT_FRESH_EXT: {
+            const bool left = ext.UL != nullptr, right = ext.UR != nullptr;
+            if (left) unit_put_slot(d, 0, ext.UL, ext.DL, ext.VL, s);
+            if (right) unit_put_slot(d, 1, ext.UR, ext.DR, ext.VR, s);
+            wg_fresh_green(d, 0, left ? 0 : -1, right ? 1 : -1, ext.out, s);
+            __syncthreads();
+            check_finite(ext.out, N * N);
+            break;
+        }
     }
+    __syncthreads();
+    if (tid() == 0) {
+        d.st->status = s.status;   // the building block's failure bits, for the host
+        if (ext.status) ((AS_G int*)ext.status)[0] = s.status;
+    }
+}
+
+// a descriptor passed by value into device memory (graph-capturable, no host copy)
+__global__ void k_put_desc(ChainDesc desc, ChainDesc* dst) {
+    if (threadIdx.x == 0) *dst = desc;
 }
 
 }  // namespace dqmc
@@ -3106,7 +3259,28 @@ hipError_t dqmc_launch_gemm(int variant, int M, int Nc, int K, const double* A, 
 }
 hipError_t dqmc_launch_unit(const ChainDesc* d_descs, int unit, int a, int b, int c, double* X, double* Y,
                             hipStream_t stream) {
-    hipLaunchKernelGGL(dqmc::k_unit, dim3(1), dim3(NT), 0, stream, d_descs, unit, a, b, c, X, Y);
+    dqmc::UnitPtrs ext{};
+    hipLaunchKernelGGL(dqmc::k_unit, dim3(1), dim3(NT), 0, stream, d_descs, unit, a, b, c, X, Y, ext);
+    return hipGetLastError();
+}
+// a building block on caller-provided device buffers: the descriptor goes to d_desc by value
+// (unless desc is null), then the block runs on it (both on `stream`)
+hipError_t dqmc_launch_unit_ext(const ChainDesc* desc, ChainDesc* d_desc, int unit, int a, int b, int c, double* X,
+                                double* Y, const DqmcUnitPtrs* ptrs, hipStream_t stream) {
+    dqmc::UnitPtrs ext{};
+    if (ptrs) {
+        ext.A = ptrs->A; ext.U_in = ptrs->U_in; ext.D_in = ptrs->D_in; ext.V_in = ptrs->V_in;
+        ext.U = ptrs->U; ext.D = ptrs->D; ext.V = ptrs->V;
+        ext.UL = ptrs->UL; ext.DL = ptrs->DL; ext.VL = ptrs->VL;
+        ext.UR = ptrs->UR; ext.DR = ptrs->DR; ext.VR = ptrs->VR;
+        ext.out = ptrs->out; ext.status = ptrs->status;
+    }
+    if (desc) {   // (null: the descriptor is already in place)
+        hipLaunchKernelGGL(dqmc::k_put_desc, dim3(1), dim3(64), 0, stream, *desc, d_desc);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(dqmc::k_unit, dim3(1), dim3(NT), 0, stream, (const ChainDesc*)d_desc, unit, a, b, c, X, Y, ext);
     return hipGetLastError();
 }
 hipError_t dqmc_launch_init(const ChainDesc* d_descs, int n_chains, hipStream_t stream) {

@@ -29,27 +29,6 @@ static_assert(DQMC_SWEEP_MEASURE == DQMC_FLAG_MEASURE && DQMC_SWEEP_RECORD == DQ
 
 namespace {
 
-// minstd_rand0 and the bernoulli draw of libstdc++ (see the device twin in dqmc_kernels.hip)
-struct Minstd {
-    uint64_t x = 1;
-    void seed(long long s) {
-        x = ((uint64_t)(int64_t)s) % 2147483647ULL;
-        if (x == 0) x = 1;
-    }
-    uint64_t next() { x = (16807ULL * x) % 2147483647ULL; return x; }
-};
-double canonical(Minstd& g) {
-#pragma clang fp contract(off)
-    const double R = 2147483646.0;
-    double sum = 0.0, tmp = 1.0;
-    for (int k = 0; k < 2; ++k) {
-        sum = sum + (double)(g.next() - 1) * tmp;
-        tmp = tmp * R;
-    }
-    double r = sum / tmp;
-    if (r >= 1.0) r = std::nextafter(1.0, 0.0);
-    return r;
-}
 
 
 bool is_finite(double x) { return std::isfinite(x); }
@@ -128,8 +107,8 @@ int validate_params(const dqmc_chain_params& p, std::string& why) {
     if (p.stabilization_pace < 1) { why = "stabilization_pace must be >= 1"; return DQMC_ERR_ARG; }
     if (p.model != DQMC_MODEL_REPULSIVE && p.model != DQMC_MODEL_ATTRACTIVE) { why = "unknown model"; return DQMC_ERR_ARG; }
     if (p.checkerboard && (p.side % 2 != 0)) { why = "checkerboard break-up needs an even side length"; return DQMC_ERR_ARG; }
-    if (p.delay < 1 || p.delay > DQMC_KMAX) { why = "delay must be in [1, 64]"; return DQMC_ERR_ARG; }
-    if (p.precision != DQMC_PREC_FP64 && p.precision != DQMC_PREC_MIXED) { why = "unsupported precision mode"; return DQMC_ERR_ARG; }
+    if (p.delay < 1 || p.delay > DQMC_KMAX) { why = "delay must be in [1, 32] (a block of the submatrix walk takes at most 32 sites)"; return DQMC_ERR_ARG; }
+    if (p.precision < DQMC_PREC_FP64 || p.precision > DQMC_PREC_FP32_PRODUCT) { why = "unsupported precision mode"; return DQMC_ERR_ARG; }
     if (!is_finite(p.beta) || !is_finite(p.hopping_t) || !is_finite(p.onsite_u) || !is_finite(p.chemical_potential)) {
         why = "non-finite model parameter"; return DQMC_ERR_NUMERIC;
     }
@@ -264,6 +243,7 @@ int batch_bind(dqmc_batch* b) {
         d.N = h.N; d.L = p.time_size; d.side = p.side; d.pace = p.stabilization_pace; d.n_stack = h.n_stack;
         d.model = p.model; d.cb = p.checkerboard; d.kdelay = h.kdelay; d.nspin = h.nspin; d.precision = p.precision;
         d.hop_t = p.hopping_t;
+        d.wrap_limit = b->wrap_limit;
         // exp(+spin alpha s) of src/model/repulsive_hubbard.cpp:173 for spin*s = +1 / -1
         d.ev[0] = std::exp(+1.0 * h.alpha);
         d.ev[1] = std::exp(-1.0 * h.alpha);
@@ -389,12 +369,9 @@ int dqmc_fields_random(dqmc_batch_t b) {
     HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
     for (int c = 0; c < b->n; ++c) {
         HostChain& h = b->chains[c];
-        Minstd g;
-        g.seed(h.p.seed);
-        // t-outer, i-inner, one bernoulli(0.5) each (src/model/repulsive_hubbard.cpp:92-97)
-        for (size_t e = 0; e < h.fields.size(); ++e) h.fields[e] = canonical(g) < 0.5 ? 1 : -1;
+        const uint64_t x = draw_random_fields(h.p.seed, h.fields);
         HIPCHK(b->ctx, hipMemcpyAsync(b->descs[c].field, h.fields.data(), h.fields.size(), hipMemcpyHostToDevice, b->stream));
-        const int rc = upload_state(b, c, g.x);
+        const int rc = upload_state(b, c, x);
         if (rc) return rc;
     }
     HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
@@ -522,6 +499,16 @@ int dqmc_sweep(dqmc_batch_t b, int n_sweeps, int flags) {
         if (flags & DQMC_SWEEP_MEASURE) h.counts += (double)n_eq * h.p.time_size;
         h.pos = (h.pos + n_sweeps) & 1;
     }
+    return DQMC_OK;
+}
+
+int dqmc_set_wrap_error_limit(dqmc_batch_t b, double limit) {
+    if (!b || !(limit > 0.0)) return DQMC_ERR_ARG;
+    b->wrap_limit = limit;
+    for (auto& d : b->descs) d.wrap_limit = limit;
+    HIPCHK(b->ctx, hipSetDevice(b->ctx->device));
+    HIPCHK(b->ctx, hipMemcpyAsync(b->d_descs, b->descs.data(), sizeof(ChainDesc) * b->n, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
     return DQMC_OK;
 }
 
@@ -798,11 +785,19 @@ hipError_t put_slot(dqmc_batch_t b, int k, const double* U, const double* D, con
     return e;
 }
 
+// synchronizes, then reports a HIP error or the numerical failure bits the building block left
+// in the chain state (DQMC_ERR_NUMERIC)
 int unit_finish(dqmc_batch_t b, const char* who, hipError_t e) {
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    ChainState st{};
+    if (e == hipSuccess) e = hipMemcpy(&st, b->descs[0].st, sizeof(st), hipMemcpyDeviceToHost);
     dqmc_ctx_t ctx = b->ctx;
     dqmc_batch_destroy(b);
     if (e != hipSuccess) { ctx->err = std::string(who) + ": " + hipGetErrorString(e); return DQMC_ERR_HIP; }
+    if (st.status != 0) {
+        ctx->err = std::string(who) + ": numerical failure (status bits " + std::to_string(st.status) + ")";
+        return DQMC_ERR_NUMERIC;
+    }
     return DQMC_OK;
 }
 
@@ -903,6 +898,208 @@ int dqmc_k_greens_recompute(dqmc_ctx_t ctx, int N, const double* UL, const doubl
         e = dqmc_launch_unit(b->d_descs, DQMC_UNIT_FRESH, UL ? 0 : -1, UR ? 1 : -1, 0, d.G, nullptr, b->stream);
     if (e == hipSuccess) e = d2h(b, G, d.G, n2 * 8);
     return unit_finish(b, "dqmc_k_greens_recompute", e);
+}
+
+// ---------------------------------------------------------------------------
+// Building blocks on caller-provided device buffers.  A workspace is a one-chain batch
+// whose arena carries the descriptor slot, two stack slots and the scratch; each call
+// patches a host copy of its descriptor with the caller's pointers, and the device
+// receives it by value (k_put_desc) on the caller's stream before the block runs.
+// ---------------------------------------------------------------------------
+namespace {
+int ws_check(dqmc_workspace_t w) { return (w && w->b) ? DQMC_OK : DQMC_ERR_ARG; }
+int ws_launch(dqmc_workspace_t w, const ChainDesc& d, int unit, int a, int bb, int c, double* X, double* Y,
+              const DqmcUnitPtrs* ptrs, void* stream, const char* who) {
+    dqmc_context* ctx = w->b->ctx;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = dqmc_launch_unit_ext(&d, w->b->d_descs, unit, a, bb, c, X, Y, ptrs, (hipStream_t)stream);
+    if (e != hipSuccess) { ctx->err = std::string(who) + ": " + hipGetErrorString(e); return DQMC_ERR_HIP; }
+    return DQMC_OK;
+}
+}  // namespace
+
+int dqmc_workspace_create(dqmc_ctx_t ctx, const dqmc_chain_params* p, dqmc_workspace_t* out) {
+    if (!ctx || !p || !out) return DQMC_ERR_ARG;
+    // two stack slots serve the push and the recompute: L' = L (2 for a one-slice chain, with
+    // beta doubled so that dtau and every constant stay the same) at pace ceil(L'/2)
+    dqmc_chain_params q = *p;
+    if (q.time_size == 1) { q.time_size = 2; q.beta *= 2.0; }
+    if (q.time_size >= 2) q.stabilization_pace = (q.time_size + 1) / 2;
+    dqmc_batch_t b = nullptr;
+    const int rc = dqmc_batch_create(ctx, 1, &q, 0, &b);
+    if (rc != DQMC_OK) return rc;
+    dqmc_workspace* w = new dqmc_workspace();
+    w->b = b;
+    w->time_size = p->time_size;
+    *out = w;
+    return DQMC_OK;
+}
+
+int dqmc_workspace_destroy(dqmc_workspace_t w) {
+    if (!w) return DQMC_ERR_ARG;
+    if (w->b) dqmc_batch_destroy(w->b);
+    delete w;
+    return DQMC_OK;
+}
+
+int dqmc_workspace_expk(dqmc_workspace_t w, double* expK, double* invExpK) {
+    if (ws_check(w) || !expK || !invExpK) return DQMC_ERR_ARG;
+    return dqmc_get_expk(w->b, 0, expK, invExpK);
+}
+
+int dqmc_kd_wrap(dqmc_workspace_t w, const int8_t* fields, const double* expK, const double* invExpK, int t,
+                 int spin, int op, double* M, void* stream) {
+    if (ws_check(w) || !fields || !M || op < 0 || op > 6 || (spin != 1 && spin != -1)) return DQMC_ERR_ARG;
+    if (t < 0 || t > w->time_size) { w->b->ctx->err = "time index out of range"; return DQMC_ERR_ARG; }
+    ChainDesc d = w->b->descs[0];
+    if ((expK != nullptr) != (invExpK != nullptr)) { w->b->ctx->err = "exp(K) needs its inverse"; return DQMC_ERR_ARG; }
+    if (expK && !d.cb) { d.expK = expK; d.invExpK = invExpK; }
+    d.field = const_cast<int8_t*>(fields);
+    const int row = t == 0 ? w->time_size - 1 : t - 1;   // time index t acts with field row t - 1
+    const int sp = (spin == 1 || d.nspin == 1) ? 0 : 1;
+    return ws_launch(w, d, DQMC_UNIT_BMUL, row, sp, op, M, nullptr, nullptr, stream, "dqmc_kd_wrap");
+}
+
+int dqmc_kd_update_ratio(dqmc_workspace_t w, const int8_t* fields, int t, int i, const double* G_up,
+                         const double* G_dn, double* ratio, void* stream) {
+    if (ws_check(w) || !fields || !G_up || !ratio) return DQMC_ERR_ARG;
+    ChainDesc d = w->b->descs[0];
+    if (d.nspin == 2 && !G_dn) return DQMC_ERR_ARG;
+    if (t < 0 || t >= w->time_size || i < 0 || i >= d.N) { w->b->ctx->err = "field index out of range"; return DQMC_ERR_ARG; }
+    d.field = const_cast<int8_t*>(fields);
+    DqmcUnitPtrs ptr{};
+    ptr.out = ratio;
+    return ws_launch(w, d, DQMC_UNIT_RATIO, t, i, 0, const_cast<double*>(G_up), const_cast<double*>(G_dn), &ptr,
+                     stream, "dqmc_kd_update_ratio");
+}
+
+int dqmc_kd_rank1(dqmc_workspace_t w, const int8_t* fields, int t, int i, double* G_up, double* G_dn, void* stream) {
+    if (ws_check(w) || !fields || !G_up) return DQMC_ERR_ARG;
+    ChainDesc d = w->b->descs[0];
+    if (t < 0 || t >= w->time_size || i < 0 || i >= d.N) { w->b->ctx->err = "field index out of range"; return DQMC_ERR_ARG; }
+    d.field = const_cast<int8_t*>(fields);
+    return ws_launch(w, d, DQMC_UNIT_RANK1, t, i, 0, G_up, G_dn, nullptr, stream, "dqmc_kd_rank1");
+}
+
+int dqmc_kd_flip_field(dqmc_workspace_t w, int8_t* fields, int t, int i, void* stream) {
+    if (ws_check(w) || !fields) return DQMC_ERR_ARG;
+    ChainDesc d = w->b->descs[0];
+    if (t < 0 || t >= w->time_size || i < 0 || i >= d.N) { w->b->ctx->err = "field index out of range"; return DQMC_ERR_ARG; }
+    d.field = fields;
+    return ws_launch(w, d, DQMC_UNIT_FLIP, t, i, 0, nullptr, nullptr, nullptr, stream, "dqmc_kd_flip_field");
+}
+
+int dqmc_kd_udv_push(dqmc_workspace_t w, const double* A, const double* U_in, const double* D_in, const double* V_in,
+                     double* U, double* D, double* V, int* status, void* stream) {
+    if (ws_check(w) || !A || !U || !D || !V) return DQMC_ERR_ARG;
+    if (U_in && (!D_in || !V_in)) { w->b->ctx->err = "a stack top needs U, D and V"; return DQMC_ERR_ARG; }
+    DqmcUnitPtrs ptr{};
+    ptr.A = A; ptr.U_in = U_in; ptr.D_in = D_in; ptr.V_in = V_in;
+    ptr.U = U; ptr.D = D; ptr.V = V; ptr.status = status;
+    return ws_launch(w, w->b->descs[0], DQMC_UNIT_PUSH_EXT, 0, 0, 0, nullptr, nullptr, &ptr, stream, "dqmc_kd_udv_push");
+}
+
+int dqmc_kd_greens_recompute(dqmc_workspace_t w, const double* UL, const double* DL, const double* VL,
+                             const double* UR, const double* DR, const double* VR, double* G, int* status,
+                             void* stream) {
+    if (ws_check(w) || !G || (!UL && !UR)) return DQMC_ERR_ARG;
+    if ((UL && (!DL || !VL)) || (UR && (!DR || !VR))) { w->b->ctx->err = "a stack top needs U, D and V"; return DQMC_ERR_ARG; }
+    DqmcUnitPtrs ptr{};
+    ptr.UL = UL; ptr.DL = DL; ptr.VL = VL; ptr.UR = UR; ptr.DR = DR; ptr.VR = VR;
+    ptr.out = G; ptr.status = status;
+    return ws_launch(w, w->b->descs[0], DQMC_UNIT_FRESH_EXT, 0, 0, 0, nullptr, nullptr, &ptr, stream,
+                     "dqmc_kd_greens_recompute");
+}
+
+// Per-site plugin calls on a batch chain's resident G and fields (synchronous on the batch
+// stream); the chain's own descriptor is already on the device
+namespace {
+int chain_unit(dqmc_batch_t b, int c, int unit, int a, int bb, int op, double* X, double* Y, const DqmcUnitPtrs* ptrs,
+               const char* who) {
+    HIPCHK(b->ctx, hipSetDevice(b->ctx->device));
+    hipError_t e = dqmc_launch_unit_ext(nullptr, b->d_descs + c, unit, a, bb, op, X, Y, ptrs, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) { b->ctx->err = std::string(who) + ": " + hipGetErrorString(e); return DQMC_ERR_HIP; }
+    return DQMC_OK;
+}
+int site_check(dqmc_batch_t b, int c, int t, int i) {
+    const int rc = check_chain(b, c);
+    if (rc) return rc;
+    if (!b->initialized) { b->ctx->err = "chain not initialized (dqmc_init)"; return DQMC_ERR_STATE; }
+    if (t < 0 || t >= b->chains[c].p.time_size || i < 0 || i >= b->chains[c].N) {
+        b->ctx->err = "field index out of range";
+        return DQMC_ERR_ARG;
+    }
+    return DQMC_OK;
+}
+}  // namespace
+
+int dqmc_chain_update_ratio(dqmc_batch_t b, int c, int t, int i, double* ratio) {
+    int rc = site_check(b, c, t, i);
+    if (rc) return rc;
+    if (!ratio) return DQMC_ERR_ARG;
+    const ChainDesc& d = b->descs[c];
+    DqmcUnitPtrs ptr{};
+    ptr.out = d.acc + DQMC_ACC_SCRATCH;   // a spare accumulator slot carries the ratio to the host
+    rc = chain_unit(b, c, DQMC_UNIT_RATIO, t, i, 0, d.G, d.nspin == 2 ? d.G + (size_t)d.N * d.N : d.G, &ptr,
+                    "dqmc_chain_update_ratio");
+    if (rc) return rc;
+    HIPCHK(b->ctx, hipMemcpy(ratio, ptr.out, sizeof(double), hipMemcpyDeviceToHost));
+    return DQMC_OK;
+}
+
+int dqmc_chain_update_green(dqmc_batch_t b, int c, int t, int i) {
+    const int rc = site_check(b, c, t, i);
+    if (rc) return rc;
+    const ChainDesc& d = b->descs[c];
+    return chain_unit(b, c, DQMC_UNIT_RANK1, t, i, 0, d.G, d.nspin == 2 ? d.G + (size_t)d.N * d.N : nullptr, nullptr,
+                      "dqmc_chain_update_green");
+}
+
+int dqmc_chain_flip_field(dqmc_batch_t b, int c, int t, int i) {
+    const int rc = site_check(b, c, t, i);
+    if (rc) return rc;
+    return chain_unit(b, c, DQMC_UNIT_FLIP, t, i, 0, nullptr, nullptr, nullptr, "dqmc_chain_flip_field");
+}
+
+int dqmc_chain_mult_green(dqmc_batch_t b, int c, int t, int spin, int op) {
+    const int rc = check_chain(b, c);
+    if (rc) return rc;
+    if (!b->initialized) { b->ctx->err = "chain not initialized (dqmc_init)"; return DQMC_ERR_STATE; }
+    const HostChain& h = b->chains[c];
+    if (op < 0 || op > 6 || (spin != 1 && spin != -1)) return DQMC_ERR_ARG;
+    if (t < 0 || t > h.p.time_size) { b->ctx->err = "time index out of range"; return DQMC_ERR_ARG; }
+    const ChainDesc& d = b->descs[c];
+    const int sp = (spin == 1 || d.nspin == 1) ? 0 : 1;
+    const int row = t == 0 ? h.p.time_size - 1 : t - 1;
+    return chain_unit(b, c, DQMC_UNIT_BMUL, row, sp, op, d.G + (size_t)sp * d.N * d.N, nullptr, nullptr,
+                      "dqmc_chain_mult_green");
+}
+
+int dqmc_set_green(dqmc_batch_t b, int c, int spin, const double* G) {
+    const int rc = check_chain(b, c);
+    if (rc) return rc;
+    if ((spin != 1 && spin != -1) || !G) return DQMC_ERR_ARG;
+    const HostChain& h = b->chains[c];
+    const size_t n2 = (size_t)h.N * h.N;
+    for (size_t e = 0; e < n2; ++e)
+        if (!std::isfinite(G[e])) { b->ctx->err = "non-finite G"; return DQMC_ERR_NUMERIC; }
+    const int sp = (spin == 1 || h.nspin == 1) ? 0 : 1;
+    HIPCHK(b->ctx, hipMemcpyAsync(b->descs[c].G + sp * n2, G, n2 * 8, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
+    return DQMC_OK;
+}
+
+int dqmc_chain_device_buffers(dqmc_batch_t b, int c, double** G_up, double** G_dn, int8_t** fields, void** stream) {
+    const int rc = check_chain(b, c);
+    if (rc) return rc;
+    const ChainDesc& d = b->descs[c];
+    const size_t n2 = (size_t)d.N * d.N;
+    if (G_up) *G_up = d.G;
+    if (G_dn) *G_dn = d.nspin == 2 ? d.G + n2 : d.G;
+    if (fields) *fields = d.field;
+    if (stream) *stream = (void*)b->stream;
+    return DQMC_OK;
 }
 
 // Delayed-update depth autotuner: times `sweeps` single-direction sweeps of a

@@ -34,26 +34,6 @@
 
 namespace {
 
-struct Minstd {
-    uint64_t x = 1;
-    void seed(long long s) {
-        x = ((uint64_t)(int64_t)s) % 2147483647ULL;
-        if (x == 0) x = 1;
-    }
-    uint64_t next() { x = (16807ULL * x) % 2147483647ULL; return x; }
-};
-double canonical(Minstd& g) {
-#pragma clang fp contract(off)
-    const double R = 2147483646.0;
-    double sum = 0.0, tmp = 1.0;
-    for (int k = 0; k < 2; ++k) {
-        sum = sum + (double)(g.next() - 1) * tmp;
-        tmp = tmp * R;
-    }
-    double r = sum / tmp;
-    if (r >= 1.0) r = std::nextafter(1.0, 0.0);
-    return r;
-}
 
 // first-fit allocator over one device pool (offsets, 256-byte granules)
 struct Pool {
@@ -98,7 +78,8 @@ struct Job {
     int lane = 0;
     std::unique_ptr<dqmc_batch> b;
     ChainState st0{};
-    bool needs_init = true, cancel = false;
+    bool needs_init = true;
+    std::atomic<bool> cancel{false};   // set by dqmc_server_cancel (under s->mu), read by the scheduler
     long long done = 0, total = 0;
     int in_bin = 0, launched = 0, launched_flags = 0;
     double cost = 1.0;  // FLOP per chain-sweep (model)
@@ -179,8 +160,12 @@ void finish_job(dqmc_server* s, Job* j, int status, hipStream_t stream) {
     if (j->b) {
         ChainState st{};
         if (hipMemcpyAsync(&st, j->b->descs[0].st, sizeof(st), hipMemcpyDeviceToHost, stream) == hipSuccess &&
-            hipStreamSynchronize(stream) == hipSuccess)
+            hipStreamSynchronize(stream) == hipSuccess) {
             r.wrap_error = st.wrap_error;
+            r.chain_status = st.status;
+            // a chain that failed numerically stopped sweeping: its observables are not results
+            if ((st.status & DQMC_STATUS_FATAL) && status == DQMC_OK) r.status = DQMC_ERR_NUMERIC;
+        }
     }
     const size_t nb = j->bins.size();
     for (int o = 0; o < DQMC_NUM_OBSERVABLES; ++o) {
@@ -236,11 +221,9 @@ bool admit(dqmc_server* s, Lane& ln, Job* j) {
     b->stream = ln.stream;
     if (batch_bind(b) != DQMC_OK) { s->pool.release(off, j->bytes); j->b.reset(); return false; }
     HostChain& h = b->chains[0];
-    Minstd g;
-    g.seed(h.p.seed);
-    for (size_t e = 0; e < h.fields.size(); ++e) h.fields[e] = canonical(g) < 0.5 ? 1 : -1;
+    const uint64_t x = draw_random_fields(h.p.seed, h.fields);
     j->st0 = ChainState{};
-    j->st0.rng = g.x;
+    j->st0.rng = x;
     j->st0.sign = 1.0;
     hipMemcpyAsync(b->descs[0].field, h.fields.data(), h.fields.size(), hipMemcpyHostToDevice, ln.stream);
     hipMemcpyAsync(b->descs[0].st, &j->st0, sizeof(ChainState), hipMemcpyHostToDevice, ln.stream);

@@ -3,6 +3,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+#include <cstdint>
 #include <string>
 #include <vector>
 
@@ -13,6 +15,8 @@ extern "C" hipError_t dqmc_launch_init(const ChainDesc* d_descs, int n_chains, h
 extern "C" hipError_t dqmc_launch_measure_recorded(const ChainDesc* d_descs, int n_chains, hipStream_t stream);
 extern "C" hipError_t dqmc_launch_sweep(const ChainDesc* d_descs, int n_chains, int n_sweeps, int flags,
                                         hipStream_t stream);
+extern "C" hipError_t dqmc_launch_unit_ext(const ChainDesc* desc, ChainDesc* d_desc, int unit, int a, int b, int c,
+                                           double* X, double* Y, const DqmcUnitPtrs* ptrs, hipStream_t stream);
 
 struct dqmc_context {
     int device = 0;
@@ -52,11 +56,19 @@ struct dqmc_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;  // bracket the last sweep launch on the batch stream
     bool timed = false;
     bool owns_arena = false, owns_stream = false;
+    double wrap_limit = DQMC_DEFAULT_WRAP_LIMIT;   // copied into every chain's descriptor
     // sweep launch replayed from a captured HIP graph (per n_sweeps/flags)
     bool use_graph = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_n = -1, graph_flags = -1;
+};
+
+// scratch of the building blocks on caller-provided device buffers: a one-chain batch (host
+// constants and tables, descriptor slot, stack slots and GEMM/factorization scratch)
+struct dqmc_workspace {
+    dqmc_batch* b = nullptr;
+    int time_size = 0;   // the caller's L (the batch may carry L = 2 for a one-slice chain)
 };
 
 #define HIPCHK(ctx, call)                                                           \
@@ -69,6 +81,39 @@ struct dqmc_batch {
     } while (0)
 
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// minstd_rand0 (std::default_random_engine of libstdc++) and the canonical double of
+// std::bernoulli_distribution (C++ standard [rand.util.canonical], two engine draws per
+// decision), as the reference's Utils::Random draws them (src/random.cpp:6-10); the device
+// twin is in dqmc_kernels.hip
+struct Minstd {
+    uint64_t x = 1;
+    void seed(long long s) {
+        x = ((uint64_t)(int64_t)s) % 2147483647ULL;
+        if (x == 0) x = 1;
+    }
+    uint64_t next() { x = (16807ULL * x) % 2147483647ULL; return x; }
+};
+inline double canonical(Minstd& g) {
+#pragma clang fp contract(off)
+    const double R = 2147483646.0;
+    double sum = 0.0, tmp = 1.0;
+    for (int k = 0; k < 2; ++k) {
+        sum = sum + (double)(g.next() - 1) * tmp;
+        tmp = tmp * R;
+    }
+    double r = sum / tmp;
+    if (r >= 1.0) r = std::nextafter(1.0, 0.0);
+    return r;
+}
+// the random HS field of a chain (src/model/repulsive_hubbard.cpp:86-99: t-outer, i-inner, one
+// bernoulli(0.5) each); returns the engine state after the draws
+inline uint64_t draw_random_fields(long long seed, std::vector<int8_t>& fields) {
+    Minstd g;
+    g.seed(seed);
+    for (size_t e = 0; e < fields.size(); ++e) fields[e] = canonical(g) < 0.5 ? 1 : -1;
+    return g.x;
+}
 int batch_prepare(dqmc_context* ctx, int n_chains, const dqmc_chain_params* params, bool record, dqmc_batch* b);
 int batch_bind(dqmc_batch* b);
 

@@ -14,7 +14,10 @@ LIB_PATH = os.path.join(HERE, "lib", "libdqmc_amd.so")
 
 OK, ERR_ARG, ERR_HIP, ERR_MEMORY, ERR_STATE, ERR_NUMERIC = 0, -1, -2, -3, -4, -5
 REPULSIVE, ATTRACTIVE = 0, 1
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_FP32_WRAP, PREC_FP32_PRODUCT = 0, 1, 2, 3
+STATUS_NONFINITE, STATUS_SINGULAR, STATUS_WRAP_LIMIT = 1, 2, 4
+STATUS_FATAL = STATUS_NONFINITE | STATUS_SINGULAR
+DEFAULT_WRAP_LIMIT = 1e-3
 SWEEP_MEASURE, SWEEP_RECORD, SWEEP_FROZEN, SWEEP_PROFILE, SWEEP_DYNAMIC = 1, 2, 4, 8, 16
 PHASES = ["wrap", "metropolis", "product", "push", "fresh", "measure", "walk", "panels"]
 OBS_NAMES = ["sign", "filling_number", "double_occupancy", "kinetic_energy", "momentum_distribution",
@@ -24,6 +27,7 @@ OBS_NAMES = ["sign", "filling_number", "double_occupancy", "kinetic_energy", "mo
 EXPORTS = ["dqmc_context_create", "dqmc_context_destroy", "dqmc_context_error", "dqmc_batch_footprint",
            "dqmc_batch_create", "dqmc_batch_destroy", "dqmc_batch_size", "dqmc_batch_use_graph", "dqmc_fields_random", "dqmc_set_fields",
            "dqmc_get_fields", "dqmc_set_rng_state", "dqmc_set_expk", "dqmc_get_expk", "dqmc_init", "dqmc_sweep",
+           "dqmc_set_wrap_error_limit",
            "dqmc_synchronize", "dqmc_last_sweep_ms", "dqmc_get_green", "dqmc_get_slice_green", "dqmc_get_slice_signs",
            "dqmc_get_accepts", "dqmc_get_info", "dqmc_get_measure", "dqmc_measure_recorded", "dqmc_clear_measure",
            "dqmc_get_dynamic_green", "dqmc_get_dynamic_measure", "dqmc_num_kstars", "dqmc_write_fields_file",
@@ -32,7 +36,10 @@ EXPORTS = ["dqmc_context_create", "dqmc_context_destroy", "dqmc_context_error", 
            "dqmc_k_udv_push", "dqmc_k_greens_recompute", "dqmc_autotune_delay",
            "dqmc_server_create", "dqmc_server_submit", "dqmc_server_cancel", "dqmc_server_poll",
            "dqmc_server_wait_idle", "dqmc_server_get_stats", "dqmc_server_error", "dqmc_server_destroy",
-           "dqmc_server_get_dynamic"]
+           "dqmc_server_get_dynamic", "dqmc_workspace_create", "dqmc_workspace_destroy", "dqmc_workspace_expk",
+           "dqmc_kd_wrap", "dqmc_kd_update_ratio", "dqmc_kd_rank1", "dqmc_kd_flip_field", "dqmc_kd_udv_push",
+           "dqmc_kd_greens_recompute", "dqmc_chain_update_ratio", "dqmc_chain_update_green", "dqmc_chain_flip_field",
+           "dqmc_chain_mult_green", "dqmc_set_green", "dqmc_chain_device_buffers"]
 
 
 class ChainParams(C.Structure):
@@ -57,7 +64,8 @@ class Job(C.Structure):
 
 
 class JobResult(C.Structure):
-    _fields_ = [("id", C.c_longlong), ("status", C.c_int), ("n_bins", C.c_int), ("t_submit", C.c_double),
+    _fields_ = [("id", C.c_longlong), ("status", C.c_int), ("chain_status", C.c_int), ("n_bins", C.c_int),
+                ("t_submit", C.c_double),
                 ("t_start", C.c_double), ("t_finish", C.c_double), ("sweeps", C.c_longlong),
                 ("wrap_error", C.c_double), ("mean", C.c_double * 9), ("error", C.c_double * 9)]
 
@@ -100,6 +108,7 @@ def lib():
         L.dqmc_set_expk.argtypes = [vp, ip, vp, vp]
         L.dqmc_get_expk.argtypes = [vp, ip, vp, vp]
         L.dqmc_sweep.argtypes = [vp, ip, ip]
+        L.dqmc_set_wrap_error_limit.argtypes = [vp, C.c_double]
         L.dqmc_last_sweep_ms.argtypes = [vp, C.POINTER(C.c_float)]
         L.dqmc_get_green.argtypes = [vp, ip, ip, vp]
         L.dqmc_get_slice_green.argtypes = [vp, ip, ip, ip, vp]
@@ -133,6 +142,21 @@ def lib():
         L.dqmc_server_error.restype = C.c_char_p
         L.dqmc_server_destroy.argtypes = [vp]
         L.dqmc_server_get_dynamic.argtypes = [vp, C.c_longlong, C.POINTER(ip), C.POINTER(ip), vp, vp]
+        L.dqmc_workspace_create.argtypes = [vp, C.POINTER(ChainParams), C.POINTER(vp)]
+        L.dqmc_workspace_destroy.argtypes = [vp]
+        L.dqmc_workspace_expk.argtypes = [vp, vp, vp]
+        L.dqmc_kd_wrap.argtypes = [vp, vp, vp, vp, ip, ip, ip, vp, vp]
+        L.dqmc_kd_update_ratio.argtypes = [vp, vp, ip, ip, vp, vp, vp, vp]
+        L.dqmc_kd_rank1.argtypes = [vp, vp, ip, ip, vp, vp, vp]
+        L.dqmc_kd_flip_field.argtypes = [vp, vp, ip, ip, vp]
+        L.dqmc_kd_udv_push.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.dqmc_kd_greens_recompute.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.dqmc_chain_update_ratio.argtypes = [vp, ip, ip, ip, C.POINTER(C.c_double)]
+        L.dqmc_chain_update_green.argtypes = [vp, ip, ip, ip]
+        L.dqmc_chain_flip_field.argtypes = [vp, ip, ip, ip]
+        L.dqmc_chain_mult_green.argtypes = [vp, ip, ip, ip, ip]
+        L.dqmc_set_green.argtypes = [vp, ip, ip, vp]
+        L.dqmc_chain_device_buffers.argtypes = [vp, ip, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]
         _lib = L
     return _lib
 
@@ -351,6 +375,35 @@ class Batch:
         self.ctx.check(lib().dqmc_get_info(self.h, c, C.byref(i)), "dqmc_get_info")
         return i
 
+    def set_wrap_error_limit(self, limit):
+        self.ctx.check(lib().dqmc_set_wrap_error_limit(self.h, float(limit)), "dqmc_set_wrap_error_limit")
+
+    # per-site plugin calls of the reference's Metropolis loop on the resident chain
+    def update_ratio(self, c, t, i):
+        r = C.c_double(0.0)
+        self.ctx.check(lib().dqmc_chain_update_ratio(self.h, c, t, i, C.byref(r)), "dqmc_chain_update_ratio")
+        return r.value
+
+    def update_green(self, c, t, i):
+        self.ctx.check(lib().dqmc_chain_update_green(self.h, c, t, i), "dqmc_chain_update_green")
+
+    def flip_field(self, c, t, i):
+        self.ctx.check(lib().dqmc_chain_flip_field(self.h, c, t, i), "dqmc_chain_flip_field")
+
+    def mult_green(self, c, t, spin, op):
+        self.ctx.check(lib().dqmc_chain_mult_green(self.h, c, t, spin, op), "dqmc_chain_mult_green")
+
+    def set_green(self, c, spin, g):
+        g = np.asfortranarray(g, dtype=np.float64)
+        self.ctx.check(lib().dqmc_set_green(self.h, c, spin, _p(g)), "dqmc_set_green")
+
+    def device_buffers(self, c):
+        """(G_up, G_dn, fields, stream) device addresses of chain c's resident state."""
+        gu, gd, f, st = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+        self.ctx.check(lib().dqmc_chain_device_buffers(self.h, c, C.byref(gu), C.byref(gd), C.byref(f), C.byref(st)),
+                       "dqmc_chain_device_buffers")
+        return gu.value, gd.value, f.value, st.value
+
     def measure_accum(self, c):
         s = np.zeros(9); n = np.zeros(9)
         self.ctx.check(lib().dqmc_get_measure(self.h, c, _p(s), _p(n)), "dqmc_get_measure")
@@ -464,7 +517,56 @@ def bin_stats(bins):
 def result_dict(r):
     return {"id": r.id, "status": r.status, "n_bins": r.n_bins, "t_submit": r.t_submit, "t_start": r.t_start,
             "t_finish": r.t_finish, "latency": r.t_finish - r.t_submit, "sweeps": r.sweeps,
-            "wrap_error": r.wrap_error, "mean": list(r.mean), "error": list(r.error)}
+            "wrap_error": r.wrap_error, "chain_status": r.chain_status, "mean": list(r.mean),
+            "error": list(r.error)}
+
+
+class Workspace:
+    """Scratch of one chain shape for the building blocks on caller-provided device buffers
+    (dqmc_workspace_* and dqmc_kd_* of the C-ABI).  Buffer arguments are device addresses
+    (ints, e.g. torch tensors' data_ptr()); stream is a hipStream_t address or None."""
+
+    def __init__(self, ctx, params):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        self.N = params.side * params.side
+        ctx.check(lib().dqmc_workspace_create(ctx.h, C.byref(params), C.byref(self.h)), "dqmc_workspace_create")
+
+    def close(self):
+        if self.h:
+            lib().dqmc_workspace_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def expk(self):
+        a = np.zeros((self.N, self.N), order="F"); b = np.zeros((self.N, self.N), order="F")
+        self.ctx.check(lib().dqmc_workspace_expk(self.h, _p(a), _p(b)), "dqmc_workspace_expk")
+        return a, b
+
+    def wrap(self, fields, t, spin, op, M, stream=None, expK=None, invExpK=None):
+        self.ctx.check(lib().dqmc_kd_wrap(self.h, fields, expK, invExpK, t, spin, op, M, stream), "dqmc_kd_wrap")
+
+    def update_ratio(self, fields, t, i, G_up, G_dn, ratio, stream=None):
+        self.ctx.check(lib().dqmc_kd_update_ratio(self.h, fields, t, i, G_up, G_dn, ratio, stream),
+                       "dqmc_kd_update_ratio")
+
+    def rank1(self, fields, t, i, G_up, G_dn, stream=None):
+        self.ctx.check(lib().dqmc_kd_rank1(self.h, fields, t, i, G_up, G_dn, stream), "dqmc_kd_rank1")
+
+    def flip_field(self, fields, t, i, stream=None):
+        self.ctx.check(lib().dqmc_kd_flip_field(self.h, fields, t, i, stream), "dqmc_kd_flip_field")
+
+    def udv_push(self, A, top, U, D, V, status=None, stream=None):
+        ui, di, vi = top if top is not None else (None, None, None)
+        self.ctx.check(lib().dqmc_kd_udv_push(self.h, A, ui, di, vi, U, D, V, status, stream), "dqmc_kd_udv_push")
+
+    def greens_recompute(self, left, right, G, status=None, stream=None):
+        ul, dl, vl = left if left is not None else (None, None, None)
+        ur, dr, vr = right if right is not None else (None, None, None)
+        self.ctx.check(lib().dqmc_kd_greens_recompute(self.h, ul, dl, vl, ur, dr, vr, G, status, stream),
+                       "dqmc_kd_greens_recompute")
 
 
 class Server:

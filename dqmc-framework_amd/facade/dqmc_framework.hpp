@@ -208,6 +208,45 @@ namespace QuantumMonteCarlo {
 class DqmcWalker;
 class DqmcInitializer;
 class DqmcIO;
+
+// The walker's equal-time Green's function of one spin, resident on the device: the
+// reference's GreensFunc& that the model changes in place (include/dqmc_walker.h:64-72,
+// src/model/repulsive_hubbard.cpp:127-155).  Converting it to a host matrix copies it from the
+// device; assigning a host matrix copies it to the device.  The attractive model keeps one
+// matrix on the device (G_dn equals G_up bit for bit in the reference,
+// src/model/attractive_hubbard.cpp:149,175): its spin-down handle mirrors the spin-up one, and
+// products and assignments on the mirror are left out, since the reference applies the same
+// ones to both matrices.
+class GreensFunc {
+    dqmc_batch_t m_batch{nullptr};
+    int m_spin{+1}, m_dim{0};
+    bool m_mirror{false};
+    friend class DqmcWalker;
+    friend class DqmcInitializer;
+
+   public:
+    int rows() const { return m_dim; }
+    int cols() const { return m_dim; }
+    int spin() const { return m_spin; }
+    bool mirror() const { return m_mirror; }
+    dqmc_batch_t batch() const { return m_batch; }
+    std::vector<double> host() const {
+        if (!m_batch) throw std::logic_error("GreensFunc: walker not initialized");
+        std::vector<double> g((size_t)m_dim * m_dim);
+        dqmc_detail::check(dqmc_get_green(m_batch, 0, m_spin, g.data()), dqmc_detail::context(), "GreensFunc::host");
+        return g;
+    }
+    operator std::vector<double>() const { return host(); }
+    GreensFunc& operator=(const std::vector<double>& g) {
+        if (!m_batch) throw std::logic_error("GreensFunc: walker not initialized");
+        if (g.size() != (size_t)m_dim * m_dim) throw std::invalid_argument("GreensFunc: matrix size");
+        if (!m_mirror)
+            dqmc_detail::check(dqmc_set_green(m_batch, 0, m_spin, g.data()), dqmc_detail::context(), "GreensFunc::assign");
+        return *this;
+    }
+    // element (i, j) (one copy of the matrix per call)
+    double operator()(int i, int j) const { return host()[(size_t)j * m_dim + i]; }
+};
 }  // namespace QuantumMonteCarlo
 
 namespace CheckerBoard {
@@ -260,6 +299,42 @@ class ModelBase {
     }
     const std::vector<int8_t>& BosonicFields() const { return m_bosonic_field; }
 
+    // Model::ModelBase::initial / initial_params / initial_KV_matrices (include/model/model_base.h:120-122,
+    // src/model/repulsive_hubbard.cpp:44-72): the HS coupling alpha = acosh(exp(dtau |U| / 2)) and
+    // exp(-dtau K), exp(+dtau K) of the dense kinetic term as the engine computes them (the
+    // plane-wave closed form of K = -t H + mu I)
+    void initial(const Lattice::LatticeBase& lattice, const QuantumMonteCarlo::DqmcWalker& walker);
+    void initial_params(const Lattice::LatticeBase& lattice, const QuantumMonteCarlo::DqmcWalker& walker);
+    void initial_KV_matrices(const Lattice::LatticeBase& lattice, const QuantumMonteCarlo::DqmcWalker& walker);
+    double Alpha() const { return m_alpha; }
+    const std::vector<double>& ExpK() const { return m_expK; }
+    const std::vector<double>& InvExpK() const { return m_inv_expK; }
+
+    // The per-site plugin calls of DqmcWalker::metropolis_update (src/dqmc_walker.cpp:173-197;
+    // include/model/model_base.h:142-150) on the walker's resident G and the device fields; t is
+    // the field row (the walker's eff_t), 0 <= t < L.
+    double get_update_ratio(QuantumMonteCarlo::DqmcWalker& walker, int time_index, int space_index) const;
+    void update_greens_function(QuantumMonteCarlo::DqmcWalker& walker, int time_index, int space_index);
+    void update_bosonic_field(int time_index, int space_index);
+
+    // the same products on the walker's resident G, in place (DqmcWalker::wrap_from_0_to_beta,
+    // wrap_from_beta_to_0: src/dqmc_walker.cpp:208-237); the handle's spin must be the call's
+    void mult_B_from_left(QuantumMonteCarlo::GreensFunc& g, int time_index, int spin) const {
+        mult_dev(g, time_index, spin, 0, "mult_B_from_left");
+    }
+    void mult_B_from_right(QuantumMonteCarlo::GreensFunc& g, int time_index, int spin) const {
+        mult_dev(g, time_index, spin, 1, "mult_B_from_right");
+    }
+    void mult_invB_from_left(QuantumMonteCarlo::GreensFunc& g, int time_index, int spin) const {
+        mult_dev(g, time_index, spin, 2, "mult_invB_from_left");
+    }
+    void mult_invB_from_right(QuantumMonteCarlo::GreensFunc& g, int time_index, int spin) const {
+        mult_dev(g, time_index, spin, 3, "mult_invB_from_right");
+    }
+    void mult_transB_from_left(QuantumMonteCarlo::GreensFunc& g, int time_index, int spin) const {
+        mult_dev(g, time_index, spin, 4, "mult_transB_from_left");
+    }
+
     // B(t) = exp(-dtau V_sigma(t)) exp(-dtau K) products on a caller's N x N matrix, in place
     // (src/model/repulsive_hubbard.cpp:158-245, src/model/attractive_hubbard.cpp:159-245): t is
     // the reference's time index in [0, L] (field row t - 1, t = 0 the slice at beta), spin +-1.
@@ -283,6 +358,29 @@ class ModelBase {
    private:
     int m_side{};
     double m_beta{};
+    double m_alpha{};
+    std::vector<double> m_expK, m_inv_expK;
+    void mult_dev(QuantumMonteCarlo::GreensFunc& g, int t, int spin, int op, const char* who) const {
+        if (g.spin() != spin) throw std::invalid_argument(std::string(who) + ": the matrix belongs to the other spin");
+        if (g.mirror()) return;
+        dqmc_detail::check(dqmc_chain_mult_green(g.batch(), 0, t, spin, op), dqmc_detail::context(), who);
+    }
+    dqmc_chain_params chain_params(double beta, int time_size) const {
+        dqmc_chain_params p{};
+        p.model = m_model_type;
+        p.side = m_side;
+        p.time_size = time_size;
+        p.stabilization_pace = time_size;
+        p.checkerboard = m_checkerboard ? 1 : 0;
+        p.delay = 1;
+        p.momentum_index = -1;
+        p.precision = DQMC_PREC_FP64;
+        p.beta = beta;
+        p.hopping_t = m_hopping_t;
+        p.onsite_u = m_onsite_u;
+        p.chemical_potential = m_chemical_potential;
+        return p;
+    }
     void mult(std::vector<double>& g, int t, int spin, int op, const char* who) const {
         if (m_side < 2 || g.size() != (size_t)m_space_size * m_space_size)
             throw std::invalid_argument(std::string(who) + ": model not initialized or matrix size");
@@ -327,8 +425,10 @@ class DqmcWalker {
     double m_beta{}, m_time_interval{};
     bool m_is_equaltime{false}, m_is_dynamic{false};
     dqmc_batch_t m_batch{nullptr};
+    GreensFunc m_green_tt_up, m_green_tt_dn;
     friend class DqmcInitializer;
     friend class Measure::MeasureHandler;
+    friend class Model::ModelBase;
 
     std::vector<double> fetch_dynamic(int slot, int spin, int which) const {
         std::vector<double> g((size_t)m_space_size * m_space_size);
@@ -343,6 +443,11 @@ class DqmcWalker {
                                 : dqmc_get_slice_green(m_batch, 0, slot, spin, g.data());
         dqmc_detail::check(rc, dqmc_detail::context(), "DqmcWalker::Greentt");
         return g;
+    }
+    std::vector<std::vector<double>> fetch_all(int spin) const {
+        std::vector<std::vector<double>> v;
+        for (int t = 0; t < m_time_size; ++t) v.push_back(fetch(t, spin));
+        return v;
     }
     void sweep_once() {
         dqmc_detail::check(dqmc_sweep(m_batch, 1, m_is_equaltime ? DQMC_SWEEP_RECORD : 0), dqmc_detail::context(),
@@ -394,11 +499,18 @@ class DqmcWalker {
         dqmc_detail::check(dqmc_sweep(m_batch, 1, DQMC_SWEEP_DYNAMIC | DQMC_SWEEP_MEASURE | DQMC_SWEEP_RECORD),
                            dqmc_detail::context(), "DqmcWalker::sweep_for_dynamic_greens");
     }
-    // accessors (include/dqmc_walker.h:136-164); matrices are copied from the device
+    // accessors (include/dqmc_walker.h:136-164).  GreenttUp()/GreenttDn() give the resident
+    // matrices by reference (a device handle; the const forms copy them to the host); the
+    // per-slot ones copy the slots kept by the last sweep.
+    GreensFunc& GreenttUp() { return m_green_tt_up; }
+    GreensFunc& GreenttDn() { return m_green_tt_dn; }
     std::vector<double> GreenttUp() const { return fetch(-1, +1); }
     std::vector<double> GreenttDn() const { return fetch(-1, -1); }
     std::vector<double> GreenttUp(int t) const { return fetch(t, +1); }
     std::vector<double> GreenttDn(int t) const { return fetch(t, -1); }
+    // DqmcWalker::vecGreenttUp/Dn (include/dqmc_walker.h:141-142): G of every slot of the last sweep
+    std::vector<std::vector<double>> vecGreenttUp() const { return fetch_all(+1); }
+    std::vector<std::vector<double>> vecGreenttDn() const { return fetch_all(-1); }
     // time-displaced G(t,0), G(0,t) of slot t (t+1 slices from 0) of the last dynamic sweep
     std::vector<double> Greent0Up(int t) const { return fetch_dynamic(t, +1, 0); }
     std::vector<double> Greent0Dn(int t) const { return fetch_dynamic(t, -1, 0); }
@@ -421,6 +533,48 @@ class DqmcWalker {
     }
 };
 }  // namespace QuantumMonteCarlo
+
+namespace Model {
+inline void ModelBase::initial(const Lattice::LatticeBase& lattice, const QuantumMonteCarlo::DqmcWalker& walker) {
+    initial_params(lattice, walker);
+    initial_KV_matrices(lattice, walker);
+}
+inline void ModelBase::initial_params(const Lattice::LatticeBase& lattice, const QuantumMonteCarlo::DqmcWalker& walker) {
+    m_side = lattice.SideLength();
+    m_space_size = lattice.SpaceSize();
+    m_time_size = walker.TimeSize();
+    m_beta = walker.Beta();
+    m_alpha = std::acosh(std::exp(0.5 * walker.TimeInterval() * std::fabs(m_onsite_u)));
+}
+inline void ModelBase::initial_KV_matrices(const Lattice::LatticeBase&, const QuantumMonteCarlo::DqmcWalker& walker) {
+    m_expK.clear();
+    m_inv_expK.clear();
+    if (m_checkerboard) return;   // the break-up has no matrices (CheckerBoard::Square::initial)
+    dqmc_ctx_t ctx = dqmc_detail::context();
+    const dqmc_chain_params p = chain_params(walker.Beta(), walker.TimeSize());
+    dqmc_workspace_t w = nullptr;
+    dqmc_detail::check(dqmc_workspace_create(ctx, &p, &w), ctx, "initial_KV_matrices");
+    m_expK.resize((size_t)m_space_size * m_space_size);
+    m_inv_expK.resize(m_expK.size());
+    const int rc = dqmc_workspace_expk(w, m_expK.data(), m_inv_expK.data());
+    dqmc_workspace_destroy(w);
+    dqmc_detail::check(rc, ctx, "initial_KV_matrices");
+}
+inline double ModelBase::get_update_ratio(QuantumMonteCarlo::DqmcWalker& walker, int t, int i) const {
+    double r = 0.0;
+    dqmc_detail::check(dqmc_chain_update_ratio(walker.m_batch, 0, t, i, &r), dqmc_detail::context(), "get_update_ratio");
+    return r;
+}
+inline void ModelBase::update_greens_function(QuantumMonteCarlo::DqmcWalker& walker, int t, int i) {
+    dqmc_detail::check(dqmc_chain_update_green(walker.m_batch, 0, t, i), dqmc_detail::context(), "update_greens_function");
+}
+inline void ModelBase::update_bosonic_field(int t, int i) {
+    if (!m_batch) throw std::logic_error("update_bosonic_field: chain not initialized");
+    dqmc_detail::check(dqmc_chain_flip_field(m_batch, 0, t, i), dqmc_detail::context(), "update_bosonic_field");
+    int8_t& f = m_bosonic_field[(size_t)t * m_space_size + i];
+    f = (int8_t)(-f);
+}
+}  // namespace Model
 
 namespace Measure {
 using ObsList = std::vector<std::string>;
@@ -606,6 +760,7 @@ class DqmcInitializer {
         model.m_beta = walker.Beta();
         model.m_bosonic_field.assign((size_t)walker.TimeSize() * lattice.SpaceSize(), 1);
         if (checkerboard) model.link(*checkerboard); else model.link();
+        model.initial(lattice, walker);   // src/dqmc_initializer.cpp:361
     }
     // src/dqmc_initializer.cpp:395-408: stacks, G(0), sign — the chain moves to the device here
     static void initial_dqmc(Lattice::LatticeBase& lattice, Model::ModelBase& model, DqmcWalker& walker,
@@ -636,6 +791,13 @@ class DqmcInitializer {
         dqmc_detail::check(dqmc_init(walker.m_batch), ctx, "dqmc_init");
         meas.m_batch = walker.m_batch;
         model.m_batch = walker.m_batch;
+        for (int s = 0; s < 2; ++s) {
+            GreensFunc& g = s == 0 ? walker.m_green_tt_up : walker.m_green_tt_dn;
+            g.m_batch = walker.m_batch;
+            g.m_spin = s == 0 ? +1 : -1;
+            g.m_dim = lattice.SpaceSize();
+            g.m_mirror = s == 1 && p.model == DQMC_MODEL_ATTRACTIVE;
+        }
     }
 };
 

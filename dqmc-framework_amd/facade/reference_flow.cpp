@@ -91,7 +91,7 @@ int main(int argc, char** argv) {
         }
     }
 
-    const auto g = walker.GreenttUp();
+    const std::vector<double> g = walker.GreenttUp();
     double tr = 0.0;
     for (int i = 0; i < side * side; ++i) tr += g[(size_t)i * side * side + i];
     std::printf("{\"wrap_error\": %.6e, \"sign\": %.1f, \"trace_g_up\": %.15f", walker.WrapError(), walker.ConfigSign(), tr);

@@ -7,7 +7,10 @@ dtau = beta/time_size; U uniform in [-8,8] (U < 0 runs the attractive model
 with |U|); mu uniform in [-1,1]; sweeps uniform in [50,500]; seed = job id;
 stabilization pace 10.  A fifth of the sweeps (rounded) warm up, the rest are
 measured in bins of 10 sweeps.  `scale` shrinks the sweep counts for short
-runs; the shape distribution is unchanged.
+runs; the shape distribution is unchanged.  Jobs run in FP64; the FP32
+product modes are an explicit opt-in (--mixed), since on live chains they do
+not hold G to the FP64 chain's (the job results carry the wrap error and the
+chain's status bits).
 
 Usage: python serve.py --jobs 200 --rate 20 --scale 0.1
        python serve.py --configs a.toml b.toml ...   (jobs from the reference's config files)
@@ -26,7 +29,7 @@ import dqmc_amd as D  # noqa: E402
 SIDES = [4, 6, 8, 10, 12, 16, 20]
 
 
-def generate_jobs(n, seed=2024, scale=1.0, precision=D.PREC_MIXED, checkerboard=True, bin_size=10):
+def generate_jobs(n, seed=2024, scale=1.0, precision=D.PREC_FP64, checkerboard=True, bin_size=10):
     rng = np.random.default_rng(seed)
     jobs = []
     for jid in range(n):
@@ -113,13 +116,14 @@ def main():
     ap.add_argument("--scale", type=float, default=0.1, help="sweep-count scale of the job mix")
     ap.add_argument("--budget-gb", type=float, default=64.0)
     ap.add_argument("--lanes", type=int, default=3)
-    ap.add_argument("--fp64", action="store_true", help="FP64 products instead of the mixed path")
+    ap.add_argument("--mixed", action="store_true",
+                    help="FP32 matrix-core products and wraps (opt-in; FP64 by default)")
     ap.add_argument("--configs", nargs="+", default=None, help="TOML configuration files to serve as jobs")
     args = ap.parse_args()
     if args.configs:
-        jobs = jobs_from_configs(args.configs, precision=D.PREC_FP64 if args.fp64 else D.PREC_MIXED)
+        jobs = jobs_from_configs(args.configs, precision=D.PREC_MIXED if args.mixed else D.PREC_FP64)
     else:
-        jobs = generate_jobs(args.jobs, scale=args.scale, precision=D.PREC_FP64 if args.fp64 else D.PREC_MIXED)
+        jobs = generate_jobs(args.jobs, scale=args.scale, precision=D.PREC_MIXED if args.mixed else D.PREC_FP64)
     ctx = D.Context(0)
     summary, _ = run(ctx, jobs, args.rate, int(args.budget_gb * 2 ** 30), lanes=args.lanes, log=sys.stderr)
     print(json.dumps(summary))

@@ -46,19 +46,37 @@ extern "C" {
 #define DQMC_ERR_NUMERIC -5  /* non-finite input or a numerical failure on the device */
 #define DQMC_ERR_IO -6       /* a file could not be opened, written or parsed */
 
+/* dqmc_chain_info.status / dqmc_job_result.chain_status: numerical failures a chain met on
+ * the device (bit flags; the reference stops the run on a failed decomposition,
+ * include/utils/linear_algebra.hpp:70-74).  A chain with a fatal bit set does no further
+ * sweeps (its state is kept as it was when the failure was seen); the wrap-limit bit is a
+ * warning and the chain goes on. */
+#define DQMC_STATUS_NONFINITE 1   /* a recomputed or wrapped G held inf or NaN (fatal) */
+#define DQMC_STATUS_SINGULAR 2    /* an LU pivot of the recompute was zero, subnormal or non-finite (fatal) */
+#define DQMC_STATUS_WRAP_LIMIT 4  /* the wrap error exceeded the batch's limit (dqmc_set_wrap_error_limit) */
+#define DQMC_STATUS_FATAL (DQMC_STATUS_NONFINITE | DQMC_STATUS_SINGULAR)
+#define DQMC_DEFAULT_WRAP_LIMIT 1e-3
+
 /* models (Model::RepulsiveHubbard / Model::AttractiveHubbard) */
 #define DQMC_MODEL_REPULSIVE 0
 #define DQMC_MODEL_ATTRACTIVE 1
 
-/* arithmetic of the slice products and wraps: FP64 throughout, or FP32 matrix
- * cores with FP32 accumulation for the B-matrix products and wraps while the
- * UDV stacks, the (1 + UDV)^-1 recompute, the delayed updates and the
- * Metropolis ratio stay FP64.  The FP32 mode is for small beta*U: its fresh G
- * carries ~1e-4 error, which the updates and wraps between stabilizations
- * amplify (at beta = 10, U = 6, pace 8 the wrap error of live chains reaches
- * O(0.1)); ChainInfo.wrap_error reports it per chain. */
+/* arithmetic of the dense B-matrix products (exp(-dtau K) as a matrix; the checkerboard
+ * plaquette passes are FP64 in every mode): FP64 throughout, or FP32 matrix cores with FP32
+ * accumulation for some of them while the UDV stacks, the (1 + UDV)^-1 recompute, the delayed
+ * updates and the Metropolis ratio stay FP64:
+ *   DQMC_PREC_MIXED          the slice products accumulated into the stacks and the wraps
+ *   DQMC_PREC_FP32_WRAP      the wraps (and the G(t,0), G(0,t) propagation) only
+ *   DQMC_PREC_FP32_PRODUCT   the slice products only
+ * The FP32 modes do not hold G to 1e-6 of the FP64 chain on live chains: FP32 rounding in
+ * the products carries into the fresh G, in the wraps into the wrapped G, and the updates
+ * between stabilizations amplify either (the wrap error of live 16x16 chains at beta = 10,
+ * U = 6, pace 8 reaches O(0.1)).  dqmc_chain_info.wrap_error reports it per chain and the
+ * DQMC_STATUS_WRAP_LIMIT bit flags it. */
 #define DQMC_PREC_FP64 0
 #define DQMC_PREC_MIXED 1
+#define DQMC_PREC_FP32_WRAP 2
+#define DQMC_PREC_FP32_PRODUCT 3
 
 /* sweep flags */
 #define DQMC_SWEEP_MEASURE 1 /* equal-time measurement after every sweep (Dqmc::measure) */
@@ -98,10 +116,11 @@ typedef struct {
     int time_size;           /* L (dtau = beta / L, src/dqmc_walker.cpp:30) */
     int stabilization_pace;  /* p */
     int checkerboard;        /* 0 dense exp(-dtau K), 1 checkerboard break-up (even side only) */
-    int delay;               /* delayed-update depth k (1..64; blocks take at most 32 sites, so k > 32 runs as 32);
+    int delay;               /* delayed-update depth k, 1..32 (a block of the submatrix walk takes at most 32
+                                sites: one per lane of a half-wavefront and spin); larger k is DQMC_ERR_ARG;
                                 1 applies every flip at once */
     int momentum_index;      /* k-star index for momentum_distribution/SDW/CDW; -1 = M point (pi,pi) */
-    int precision;           /* DQMC_PREC_FP64 or DQMC_PREC_MIXED */
+    int precision;           /* DQMC_PREC_* */
     double beta;
     double hopping_t;
     double onsite_u;
@@ -114,7 +133,7 @@ typedef struct {
     double wrap_error;       /* DqmcWalker::WrapError() */
     unsigned long long rng;  /* engine state (minstd_rand0) */
     int pos;                 /* 0: at slice 0, time_size: at slice beta */
-    int status;              /* 0 ok, else numerical failure */
+    int status;              /* 0 ok, else DQMC_STATUS_* bits */
     long long accepted;      /* accepted flips in the last sweep launch */
     long long sweeps;        /* single-direction sweeps done since init */
     double phase_seconds[8]; /* wrap, Metropolis, slice product, stack push, fresh G, measurement,
@@ -152,6 +171,10 @@ int dqmc_get_expk(dqmc_batch_t b, int chain, double* expK, double* invExpK);
 
 /* stacks, G(0) and sign for the current fields */
 int dqmc_init(dqmc_batch_t b);
+
+/* wrap error above which a chain's status gets DQMC_STATUS_WRAP_LIMIT (default
+ * DQMC_DEFAULT_WRAP_LIMIT; the reference only reports it, DqmcWalker::WrapError) */
+int dqmc_set_wrap_error_limit(dqmc_batch_t b, double limit);
 
 /* n single-direction sweeps per chain, alternating 0->beta and beta->0 */
 int dqmc_sweep(dqmc_batch_t b, int n_sweeps, int flags);
@@ -241,7 +264,8 @@ typedef struct {
 
 typedef struct {
     long long id;
-    int status;           /* 0 done, 1 cancelled, < 0 error code */
+    int status;           /* 0 done, 1 cancelled, < 0 error code (DQMC_ERR_NUMERIC: a fatal chain_status) */
+    int chain_status;     /* DQMC_STATUS_* bits of the job's chain */
     int n_bins;
     double t_submit, t_start, t_finish;  /* seconds since the server started */
     long long sweeps;
@@ -325,6 +349,73 @@ int dqmc_k_udv_push(dqmc_ctx_t ctx, int N, const double* A, const double* U_in, 
  * stack (not both).  U_L, U_R orthogonal, D > 0.  Same sizes as dqmc_k_udv_push. */
 int dqmc_k_greens_recompute(dqmc_ctx_t ctx, int N, const double* UL, const double* DL, const double* VL,
                             const double* UR, const double* DR, const double* VR, double* G);
+
+/* ------------------------------------------------------------------------
+ * Single-kernel entry points on caller-provided device buffers (the form the
+ * reference's per-site plugin calls take once the walker's G lives on the device).
+ * A workspace holds the scratch of one chain shape (descriptor, two stack slots, GEMM and
+ * factorization scratch) and is allocated once; the calls themselves allocate nothing and
+ * copy nothing from the host: each is enqueued on `stream` (a hipStream_t; NULL is the
+ * default stream) and returns at once.  Calls on one workspace must not overlap (one stream
+ * at a time).  Pointers are device pointers; matrices column-major N x N, fields int8
+ * (t, i) at t*N+i as in dqmc_set_fields.
+ * ------------------------------------------------------------------------ */
+typedef struct dqmc_workspace* dqmc_workspace_t;
+
+/* scratch and constants for chains of this shape (Model::ModelBase::initial,
+ * initial_params, initial_KV_matrices: include/model/model_base.h:120-122) */
+int dqmc_workspace_create(dqmc_ctx_t ctx, const dqmc_chain_params* p, dqmc_workspace_t* out);
+int dqmc_workspace_destroy(dqmc_workspace_t w);
+/* host copies of exp(-dtau K), exp(+dtau K) the workspace's dense products use
+ * (ModelBase::initial_KV_matrices, src/model/repulsive_hubbard.cpp:53-72) */
+int dqmc_workspace_expk(dqmc_workspace_t w, double* expK, double* invExpK);
+
+/* dqmc_k_wrap on device buffers: op 0..6 on M in place with the field row of time index t
+ * (0 <= t <= time_size, the reference's time_index); expK/invExpK may be NULL (the workspace's) */
+int dqmc_kd_wrap(dqmc_workspace_t w, const int8_t* fields, const double* expK, const double* invExpK, int t,
+                 int spin, int op, double* M, void* stream);
+/* Model::ModelBase::get_update_ratio (include/model/model_base.h:146; src/model/repulsive_hubbard.cpp:112-124,
+ * src/model/attractive_hubbard.cpp:112-125): *ratio (one device double) = the Metropolis ratio of
+ * flipping the field at (t, i), 0 <= t < time_size, from G_up(i,i) and G_dn(i,i) (G_dn ignored,
+ * may be NULL, for the attractive model) */
+int dqmc_kd_update_ratio(dqmc_workspace_t w, const int8_t* fields, int t, int i, const double* G_up,
+                         const double* G_dn, double* ratio, void* stream);
+/* Model::ModelBase::update_greens_function (include/model/model_base.h:148) = dqmc_k_rank1 on
+ * device buffers, before the field at (t, i) is flipped */
+int dqmc_kd_rank1(dqmc_workspace_t w, const int8_t* fields, int t, int i, double* G_up, double* G_dn, void* stream);
+/* Model::ModelBase::update_bosonic_field (include/model/model_base.h:150; src/model/repulsive_hubbard.cpp:102-109) */
+int dqmc_kd_flip_field(dqmc_workspace_t w, int8_t* fields, int t, int i, void* stream);
+/* dqmc_k_udv_push and dqmc_k_greens_recompute on device buffers (U_in / UL / UR NULL: empty
+ * stack); *status (a device int, may be NULL) receives the DQMC_STATUS_* bits of the call */
+int dqmc_kd_udv_push(dqmc_workspace_t w, const double* A, const double* U_in, const double* D_in, const double* V_in,
+                     double* U, double* D, double* V, int* status, void* stream);
+int dqmc_kd_greens_recompute(dqmc_workspace_t w, const double* UL, const double* DL, const double* VL,
+                             const double* UR, const double* DR, const double* VR, double* G, int* status,
+                             void* stream);
+
+/* The per-site plugin calls of the reference's Metropolis loop (DqmcWalker::metropolis_update,
+ * src/dqmc_walker.cpp:173-197) on an initialized batch chain's resident G and fields,
+ * synchronous on the batch stream: Model::ModelBase::get_update_ratio (*ratio on the host),
+ * update_greens_function, update_bosonic_field (include/model/model_base.h:142-150) for the field
+ * at (t, i), 0 <= t < time_size; and the B-matrix products of the wraps on the chain's own G
+ * (op 0..6 as dqmc_k_wrap, time index 0 <= t <= time_size; DqmcWalker::wrap_from_0_to_beta /
+ * wrap_from_beta_to_0, src/dqmc_walker.cpp:208-237).  A caller that changes the chain this way
+ * keeps its own stacks (as the reference's walker does); the engine's sweeps need dqmc_init
+ * again afterwards. */
+int dqmc_chain_update_ratio(dqmc_batch_t b, int chain, int t, int i, double* ratio);
+int dqmc_chain_update_green(dqmc_batch_t b, int chain, int t, int i);
+int dqmc_chain_flip_field(dqmc_batch_t b, int chain, int t, int i);
+int dqmc_chain_mult_green(dqmc_batch_t b, int chain, int t, int spin, int op);
+/* *DqmcWalker::GreenttUp/Dn() = G (the reference assigns the fresh G this way,
+ * src/dqmc_walker.cpp:302-303): host matrix into the chain's resident G */
+int dqmc_set_green(dqmc_batch_t b, int chain, int spin, const double* G);
+
+/* Device buffers of a batch chain's state (DqmcWalker::GreenttUp/Dn() by reference,
+ * include/dqmc_walker.h:136-140): G of each spin (the same matrix twice for the attractive
+ * model) and the fields, plus the batch's stream; valid until the batch is destroyed.  Work
+ * enqueued on them must be ordered with the batch's own calls through that stream. */
+int dqmc_chain_device_buffers(dqmc_batch_t b, int chain, double** G_up, double** G_dn, int8_t** fields,
+                              void** stream);
 
 #ifdef __cplusplus
 }

@@ -43,4 +43,4 @@ def test_footprint_without_gpu():
     with pytest.raises(dqmc_amd.DqmcError):
         dqmc_amd.footprint([dqmc_amd.make_params(4, 10, 1.0, U=-1.0)])       # repulsive needs U >= 0
     with pytest.raises(dqmc_amd.DqmcError):
-        dqmc_amd.footprint([dqmc_amd.make_params(4, 10, 1.0, delay=65)])
+        dqmc_amd.footprint([dqmc_amd.make_params(4, 10, 1.0, delay=33)])   # blocks take at most 32 sites
