@@ -6,6 +6,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -65,6 +67,45 @@ void expk_closed_form(int side, double t, double mu, double s, std::vector<doubl
 
 
 
+// ---------------------------------------------------------------------------
+// Scratch of dqmc_k_greens_recompute_noalloc: one file-scope device array (each device holds its
+// own copy until the process ends).  Host code never passes `scratch` itself to a kernel: its
+// address on each device comes from hipGetSymbolAddress, taken once per device with that device
+// current when a context is created (noalloc_scratch_of).  Layout in doubles, sized for
+// DQMC_NOALLOC_MAX_N sites: the descriptor slot, the chain state, two stack slots (U, T, D) and
+// the recompute's scratch W (3 N^2).  Nothing in it needs to start at zero: every call writes
+// the descriptor, the stack slots and W before reading them, and the status word is stored,
+// not accumulated.
+// ---------------------------------------------------------------------------
+constexpr size_t na_round(size_t doubles) { return (doubles + 31) / 32 * 32; }   // 256-byte granules
+constexpr size_t NA_N2 = (size_t)DQMC_NOALLOC_MAX_N * DQMC_NOALLOC_MAX_N;
+constexpr size_t NA_DESC = 0;
+constexpr size_t NA_ST = NA_DESC + na_round((sizeof(ChainDesc) + 7) / 8);
+constexpr size_t NA_U = NA_ST + na_round((sizeof(ChainState) + 7) / 8);
+constexpr size_t NA_T = NA_U + 2 * NA_N2;
+constexpr size_t NA_D = NA_T + 2 * NA_N2;
+constexpr size_t NA_W = NA_D + na_round(2 * DQMC_NOALLOC_MAX_N);
+constexpr size_t NA_TOTAL = NA_W + 3 * NA_N2;
+}  // namespace
+
+static __device__ __align__(256) double scratch[NA_TOTAL];
+
+namespace {
+std::mutex g_noalloc_mu;
+std::map<int, double*> g_noalloc_ptr;   // device -> address of that device's copy of scratch
+
+// the device address of `scratch` on `device` (the current device), resolved once per device
+hipError_t noalloc_scratch_of(int device, double** out) {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    auto it = g_noalloc_ptr.find(device);
+    if (it != g_noalloc_ptr.end()) { *out = it->second; return hipSuccess; }
+    double* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_noalloc_ptr[device] = ptr;
+    *out = ptr;
+    return hipSuccess;
+}
 }  // namespace
 
 Layout chain_layout(int N, int L, int n_stack, int nspin, int kdelay, bool dense, bool record) {
@@ -134,6 +175,8 @@ int dqmc_context_create(int device, dqmc_ctx_t* out) {
     if (hipSetDevice(device) != hipSuccess) return DQMC_ERR_HIP;
     dqmc_context* c = new dqmc_context();
     c->device = device;
+    // set-up of the no-allocation entry point: this device's address of its static scratch
+    if (noalloc_scratch_of(device, &c->noalloc_scratch) != hipSuccess) { delete c; return DQMC_ERR_HIP; }
     *out = c;
     return DQMC_OK;
 }
@@ -1099,6 +1142,42 @@ int dqmc_chain_device_buffers(dqmc_batch_t b, int c, double** G_up, double** G_d
     if (G_dn) *G_dn = d.nspin == 2 ? d.G + n2 : d.G;
     if (fields) *fields = d.field;
     if (stream) *stream = (void*)b->stream;
+    return DQMC_OK;
+}
+
+// dqmc_k_greens_recompute on caller device buffers with the scratch of the static array: no
+// hipMalloc, no hipFree, no host-to-device copy (the descriptor reaches the device as a kernel
+// argument).  Calls that overlap on one device share the array and race.
+int dqmc_k_greens_recompute_noalloc(dqmc_ctx_t ctx, int N, const double* UL, const double* DL, const double* VL,
+                                    const double* UR, const double* DR, const double* VR, double* G, int* status,
+                                    void* stream) {
+    if (!ctx || !G || (!UL && !UR)) return DQMC_ERR_ARG;
+    if ((UL && (!DL || !VL)) || (UR && (!DR || !VR))) { ctx->err = "a stack top needs U, D and V"; return DQMC_ERR_ARG; }
+    int side = 2;
+    while (side * side < N) ++side;
+    if (N < 4 || side * side != N) { ctx->err = "N must be side*side with 2 <= side"; return DQMC_ERR_ARG; }
+    if (N > DQMC_NOALLOC_MAX_N) {
+        ctx->err = "dqmc_k_greens_recompute_noalloc: N = " + std::to_string(N) + " exceeds its static scratch (N <= " +
+                   std::to_string(DQMC_NOALLOC_MAX_N) + ")";
+        return DQMC_ERR_ARG;
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    double* base = ctx->noalloc_scratch;   // this device's address of `scratch`, never the symbol itself
+    if (!base) { ctx->err = "no-allocation scratch not set up"; return DQMC_ERR_STATE; }
+    ChainDesc d{};
+    d.N = N; d.side = side; d.L = 2; d.pace = 1; d.n_stack = 2; d.model = DQMC_MODEL_REPULSIVE; d.cb = 0;
+    d.kdelay = 1; d.nspin = 1; d.precision = DQMC_PREC_FP64; d.launch_flags = -1;
+    d.wrap_limit = DQMC_DEFAULT_WRAP_LIMIT;
+    d.st = (ChainState*)(base + NA_ST);
+    d.stkU = base + NA_U;
+    d.stkT = base + NA_T;
+    d.stkD = base + NA_D;
+    d.W = base + NA_W;
+    DqmcUnitPtrs ptr{};
+    ptr.UL = UL; ptr.DL = DL; ptr.VL = VL; ptr.UR = UR; ptr.DR = DR; ptr.VR = VR;
+    ptr.out = G; ptr.status = status;
+    HIPCHK(ctx, dqmc_launch_unit_ext(&d, (ChainDesc*)(base + NA_DESC), DQMC_UNIT_FRESH_EXT, 0, 0, 0, nullptr, nullptr,
+                                     &ptr, (hipStream_t)stream));
     return DQMC_OK;
 }
 

@@ -21,6 +21,7 @@ extern "C" hipError_t dqmc_launch_unit_ext(const ChainDesc* desc, ChainDesc* d_d
 struct dqmc_context {
     int device = 0;
     std::string err;
+    double* noalloc_scratch = nullptr;   // the device's address of the no-allocation entry point's scratch
 };
 
 struct HostChain {

@@ -39,7 +39,8 @@ EXPORTS = ["dqmc_context_create", "dqmc_context_destroy", "dqmc_context_error", 
            "dqmc_server_get_dynamic", "dqmc_workspace_create", "dqmc_workspace_destroy", "dqmc_workspace_expk",
            "dqmc_kd_wrap", "dqmc_kd_update_ratio", "dqmc_kd_rank1", "dqmc_kd_flip_field", "dqmc_kd_udv_push",
            "dqmc_kd_greens_recompute", "dqmc_chain_update_ratio", "dqmc_chain_update_green", "dqmc_chain_flip_field",
-           "dqmc_chain_mult_green", "dqmc_set_green", "dqmc_chain_device_buffers"]
+           "dqmc_chain_mult_green", "dqmc_set_green", "dqmc_chain_device_buffers",
+           "dqmc_k_greens_recompute_noalloc"]
 
 
 class ChainParams(C.Structure):
@@ -157,6 +158,7 @@ def lib():
         L.dqmc_chain_mult_green.argtypes = [vp, ip, ip, ip, ip]
         L.dqmc_set_green.argtypes = [vp, ip, ip, vp]
         L.dqmc_chain_device_buffers.argtypes = [vp, ip, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]
+        L.dqmc_k_greens_recompute_noalloc.argtypes = [vp, ip, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -255,6 +257,13 @@ class Context:
         self.check(lib().dqmc_k_greens_recompute(self.h, n, *[None if x is None else _p(x) for x in args], _p(G)),
                    "dqmc_k_greens_recompute")
         return G
+
+    def k_greens_recompute_noalloc(self, N, left, right, G, status=None, stream=None):
+        """The recompute on device buffers (addresses) with the library's static scratch."""
+        ul, dl, vl = left if left is not None else (None, None, None)
+        ur, dr, vr = right if right is not None else (None, None, None)
+        self.check(lib().dqmc_k_greens_recompute_noalloc(self.h, N, ul, dl, vl, ur, dr, vr, G, status, stream),
+                   "dqmc_k_greens_recompute_noalloc")
 
     def autotune_delay(self, params, n_chains, candidates=(8, 16, 32), sweeps=2):
         """Fastest delayed-update depth for chains of this shape (times every candidate)."""

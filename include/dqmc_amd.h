@@ -417,6 +417,20 @@ int dqmc_set_green(dqmc_batch_t b, int chain, int spin, const double* G);
 int dqmc_chain_device_buffers(dqmc_batch_t b, int chain, double** G_up, double** G_dn, int8_t** fields,
                               void** stream);
 
+/* dqmc_k_greens_recompute on caller device buffers with no allocation at all.  No counterpart
+ * in the reference: for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its scratch is a static
+ * device array sized for DQMC_NOALLOC_MAX_N sites (about 3.7 MB, held on each device the
+ * library runs on until the process ends), so N > DQMC_NOALLOC_MAX_N is DQMC_ERR_ARG.  Every call
+ * on one device shares that array: calls that overlap on one device (two streams, two threads)
+ * race.  Enqueued on `stream` (a hipStream_t, NULL the default stream); *status (a device int,
+ * may be NULL) receives the call's DQMC_STATUS_* bits.  Same arguments and result as
+ * dqmc_kd_greens_recompute. */
+#define DQMC_NOALLOC_MAX_N 256
+int dqmc_k_greens_recompute_noalloc(dqmc_ctx_t ctx, int N, const double* UL, const double* DL, const double* VL,
+                                    const double* UR, const double* DR, const double* VR, double* G, int* status,
+                                    void* stream);
+
 #ifdef __cplusplus
 }
 #endif
