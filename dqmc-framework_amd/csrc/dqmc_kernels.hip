@@ -1897,8 +1897,9 @@ __device__ __noinline__ void wg_transpose_scaled(double* dst1_, double* dst2_, c
     }
 }
 
-// G = (1 + B_left B_right)^-1 from the stack tops (lslot/rslot = -1 when that stack is empty)
-__device__ __noinline__ void wg_fresh_green(Desc d, int sp, int lslot, int rslot, double* Gout, Lds&) {
+// G = (1 + B_left B_right)^-1 from the stack tops (lslot/rslot = -1 when that stack is empty);
+// returns the sign of det of the matrix its LU factored (for one stack, H = Db U^T + Ds T)
+__device__ __noinline__ double wg_fresh_green(Desc d, int sp, int lslot, int rslot, double* Gout, Lds&) {
     Lds& s = g_lds;
     const int N = d.N;
     double* W0 = d.W;
@@ -1918,10 +1919,10 @@ __device__ __noinline__ void wg_fresh_green(Desc d, int sp, int lslot, int rslot
         gemm64(N, N, N, TL, N, false, TR, N, true, W0, N, s.v1, nullptr, s.v3, 1.0, s);
         // W1 = dlmax^-1 UL^T
         wg_transpose_scaled(W1, nullptr, UL, nullptr, N, s.v0, nullptr, s);
-        wg_lu_solve(W0, W1, N, N, s);
+        const double ds = wg_lu_solve(W0, W1, N, N, s);
         // G = UR drmax^-1 W1
         gemm64(N, N, N, UR, N, false, W1, N, false, Gout, N, nullptr, s.v2, nullptr, 0.0, s);
-        return;
+        return ds;
     }
     // one stack only: X = (Db U^T + Ds T)^-1 Db U^T   (include/utils/numerical_stable.hpp:132-152)
     const int k = lslot >= 0 ? lslot : rslot;
@@ -1934,9 +1935,10 @@ __device__ __noinline__ void wg_fresh_green(Desc d, int sp, int lslot, int rslot
     __syncthreads();
     // W1 = Db U^T, W0 = W1 + Ds T
     wg_transpose_scaled(W1, W0, U, T, N, s.v0, s.v1, s);
-    wg_lu_solve(W0, W1, N, N, s);
+    const double ds = wg_lu_solve(W0, W1, N, N, s);
     if (lslot >= 0) wg_copy(Gout, W1, N * N);
     else wg_transpose_scaled(Gout, nullptr, W1, nullptr, N, nullptr, nullptr, s);  // right stack: G = X^T
+    return ds;
 }
 
 // ---------------------------------------------------------------------------
@@ -2977,21 +2979,25 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1)))
             for (int sp = 0; sp < d.nspin; ++sp) wg_identity(d.Tmp + sp * n2, N);
         }
     }
-    for (int sp = 0; sp < d.nspin; ++sp) wg_fresh_green(d, sp, -1, st.n_right - 1, d.G + sp * n2, s);
+    // G(0) from the right stack, and the configuration sign from the same factorization: with the
+    // stack product M = U D T, det(1 + M) = det(U) det(Db)^-1 det(Db U^T + Ds T), so sgn det G(0) =
+    // sgn det U x sgn det H, H the matrix the recompute factors.  (The reference takes sgn det G,
+    // src/dqmc_walker.cpp:162, which is rounding noise once G has eigenvalues below FP64
+    // resolution; both are well-conditioned here.)
+    double sg = 1.0;
+    for (int sp = 0; sp < d.nspin; ++sp) {
+        const double dh = wg_fresh_green(d, sp, -1, st.n_right - 1, d.G + sp * n2, s);
+        wg_copy(d.Gw, slotU(d, sp, st.n_right - 1), (int)n2);
+        const double du = wg_lu_solve(d.Gw, nullptr, N, 0, s);
+        sg *= dh * du;
+        if (d.nspin == 1) sg *= dh * du;   // the attractive model's G_dn is G_up
+    }
     {   // a non-finite G(0) (e.g. from overflowing products) is a failed initialization
         double m = 0.0;
         const AS_G double* g = gptr(d.G);
         wg_stream((int)(n2 * d.nspin), [&](int e) { return fabs(g[e]); }, [&](int, double v) { m = nan_max(m, v); });
         m = wg_max(m, s);
         if (tid() == 0 && !(m <= 1.7976931348623157e308)) s.status |= DQMC_STATUS_NONFINITE;
-    }
-    // sign of det(G_up) det(G_dn)  (src/dqmc_walker.cpp:162)
-    double sg = 1.0;
-    for (int sp = 0; sp < d.nspin; ++sp) {
-        wg_copy(d.Gw, d.G + sp * n2, (int)n2);
-        const double ds = wg_lu_solve(d.Gw, nullptr, N, 0, s);
-        sg *= ds;
-        if (d.nspin == 1) sg *= ds;
     }
     if (tid() == 0) {
         st.sign = sg >= 0 ? 1.0 : -1.0;

@@ -82,6 +82,8 @@ struct Job {
     std::atomic<bool> cancel{false};   // set by dqmc_server_cancel (under s->mu), read by the scheduler
     long long done = 0, total = 0;
     int in_bin = 0, launched = 0, launched_flags = 0;
+    int stage = -1;                  // slot of the lane's pinned staging the running launch copies into
+    bool stage_acc = false, stage_st = false;   // which of acc / state that copy holds
     double cost = 1.0;  // FLOP per chain-sweep (model)
     double t_submit = 0, t_start = 0;
     std::vector<std::vector<double>> bins;
@@ -96,9 +98,14 @@ struct DynResult {
     std::vector<double> mean, err;
 };
 
+// per job of a launch: its 16 accumulators and its chain state, copied to pinned host memory
+// behind the launch on the lane stream (the scheduler never waits on a copy)
+constexpr int STAGE_DOUBLES = 16 + (int)((sizeof(ChainState) + 7) / 8);
+
 struct Lane {
     hipStream_t stream = nullptr;
     hipEvent_t ev = nullptr;
+    double* h_stage = nullptr;   // pinned, max_active x STAGE_DOUBLES
     ChainDesc* d_descs = nullptr;
     ChainDesc* d_init = nullptr;
     std::vector<ChainDesc> h_descs, h_init;
@@ -147,7 +154,7 @@ double chain_sweep_flop(const dqmc_chain_params& p) {
     return spins / 2.0 * (prop + upd + 29.3 * ns * N * N * N);
 }
 
-void finish_job(dqmc_server* s, Job* j, int status, hipStream_t stream) {
+void finish_job(dqmc_server* s, Job* j, int status, hipStream_t stream, const ChainState* staged = nullptr) {
     dqmc_job_result r{};
     r.id = j->id;
     r.status = status;
@@ -159,8 +166,11 @@ void finish_job(dqmc_server* s, Job* j, int status, hipStream_t stream) {
     r.wrap_error = 0.0;
     if (j->b) {
         ChainState st{};
-        if (hipMemcpyAsync(&st, j->b->descs[0].st, sizeof(st), hipMemcpyDeviceToHost, stream) == hipSuccess &&
-            hipStreamSynchronize(stream) == hipSuccess) {
+        bool ok = false;
+        if (staged) { st = *staged; ok = true; }
+        else ok = hipMemcpyAsync(&st, j->b->descs[0].st, sizeof(st), hipMemcpyDeviceToHost, stream) == hipSuccess &&
+                  hipStreamSynchronize(stream) == hipSuccess;
+        if (ok) {
             r.wrap_error = st.wrap_error;
             r.chain_status = st.status;
             // a chain that failed numerically stopped sweeping: its observables are not results
@@ -261,10 +271,9 @@ void complete_lane(dqmc_server* s, Lane& ln) {
         if (j->launched_flags & DQMC_FLAG_MEASURE) {
             j->in_bin += j->launched;
             if (j->in_bin >= j->spec.bin_size) {
-                double acc[16];
-                hipMemcpyAsync(acc, j->b->descs[0].acc, sizeof(acc), hipMemcpyDeviceToHost, ln.stream);
-                hipStreamSynchronize(ln.stream);
-                hipMemsetAsync(j->b->descs[0].acc, 0, sizeof(acc), ln.stream);
+                // the launch copied the bin's accumulators to the lane's pinned staging and
+                // cleared them on the device, stream-ordered behind the sweeps
+                const double* acc = ln.h_stage + (size_t)j->stage * STAGE_DOUBLES;
                 if (dyn) {
                     const HostChain& h = j->b->chains[0];
                     const int N = h.N, L = h.p.time_size, side = h.p.side, nk = (int)h.kx.size();
@@ -309,7 +318,12 @@ void complete_lane(dqmc_server* s, Lane& ln) {
     for (Job* j : ln.in_flight)
         if (j->done >= j->total || j->cancel) fin.push_back(j);
     ln.in_flight.clear();
-    for (Job* j : fin) finish_job(s, j, j->cancel && j->done < j->total ? 1 : DQMC_OK, ln.stream);
+    for (Job* j : fin) {
+        ChainState st{};
+        const bool staged = j->stage_st;
+        if (staged) std::memcpy(&st, ln.h_stage + (size_t)j->stage * STAGE_DOUBLES + 16, sizeof(ChainState));
+        finish_job(s, j, j->cancel && j->done < j->total ? 1 : DQMC_OK, ln.stream, staged ? &st : nullptr);
+    }
 }
 
 bool launch_lane(dqmc_server* s, Lane& ln, int lane_index) {
@@ -359,6 +373,20 @@ bool launch_lane(dqmc_server* s, Lane& ln, int lane_index) {
     }
     hipMemcpyAsync(ln.d_descs, ln.h_descs.data(), sizeof(ChainDesc) * ln.h_descs.size(), hipMemcpyHostToDevice, ln.stream);
     if (dqmc_launch_sweep(ln.d_descs, (int)ln.h_descs.size(), 1, 0, ln.stream) != hipSuccess) s->status = DQMC_ERR_HIP;
+    // the readbacks this launch ends in, behind it on the stream: a bin's accumulators (then
+    // cleared) and the state of a job that finishes
+    for (size_t q = 0; q < ln.in_flight.size(); ++q) {
+        Job* j = ln.in_flight[q];
+        j->stage = (int)q;
+        double* slot = ln.h_stage + q * STAGE_DOUBLES;
+        j->stage_acc = (j->launched_flags & DQMC_FLAG_MEASURE) && j->in_bin + j->launched >= j->spec.bin_size;
+        j->stage_st = j->done + j->launched >= j->total || j->cancel;
+        if (j->stage_acc) {
+            hipMemcpyAsync(slot, j->b->descs[0].acc, 16 * sizeof(double), hipMemcpyDeviceToHost, ln.stream);
+            hipMemsetAsync(j->b->descs[0].acc, 0, 16 * sizeof(double), ln.stream);
+        }
+        if (j->stage_st) hipMemcpyAsync(slot + 16, j->b->descs[0].st, sizeof(ChainState), hipMemcpyDeviceToHost, ln.stream);
+    }
     hipEventRecord(ln.ev, ln.stream);
     ln.t_launch = std::chrono::steady_clock::now();
     ln.busy = true;
@@ -427,6 +455,7 @@ int dqmc_server_create(dqmc_ctx_t ctx, size_t hbm_budget, int max_active, int n_
         if (e == hipSuccess) e = hipEventCreateWithFlags(&ln.ev, hipEventDisableTiming);
         if (e == hipSuccess) e = hipMalloc((void**)&ln.d_descs, sizeof(ChainDesc) * max_active);
         if (e == hipSuccess) e = hipMalloc((void**)&ln.d_init, sizeof(ChainDesc) * max_active);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&ln.h_stage, sizeof(double) * STAGE_DOUBLES * max_active, 0);
         if (e != hipSuccess) { ctx->err = hipGetErrorString(e); dqmc_server_destroy(s); return DQMC_ERR_HIP; }
     }
     s->t0 = std::chrono::steady_clock::now();
@@ -556,6 +585,7 @@ int dqmc_server_destroy(dqmc_server_t s) {
         if (ln.stream) hipStreamSynchronize(ln.stream);
         if (ln.d_descs) hipFree(ln.d_descs);
         if (ln.d_init) hipFree(ln.d_init);
+        if (ln.h_stage) hipHostFree(ln.h_stage);
         if (ln.ev) hipEventDestroy(ln.ev);
         if (ln.stream) hipStreamDestroy(ln.stream);
     }

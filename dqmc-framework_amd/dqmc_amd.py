@@ -581,7 +581,7 @@ class Workspace:
 class Server:
     """Continuous-batching job server (dqmc_server_* of the C-ABI)."""
 
-    def __init__(self, ctx, hbm_budget_bytes, max_active=4096, lanes=3):
+    def __init__(self, ctx, hbm_budget_bytes, max_active=4096, lanes=4):
         self.ctx = ctx
         self.h = C.c_void_p()
         ctx.check(lib().dqmc_server_create(ctx.h, int(hbm_budget_bytes), int(max_active), int(lanes),

@@ -65,7 +65,7 @@ def jobs_from_configs(paths, precision=D.PREC_FP64):
     return jobs
 
 
-def run(ctx, jobs, rate, budget_bytes, lanes=3, max_active=4096, arrival_seed=7, timeout_s=3000.0, log=None):
+def run(ctx, jobs, rate, budget_bytes, lanes=4, max_active=4096, arrival_seed=7, timeout_s=3000.0, log=None):
     """Submit jobs at Poisson arrival times (rate per second) and wait for all."""
     rng = np.random.default_rng(arrival_seed)
     gaps = rng.exponential(1.0 / rate, size=len(jobs)) if rate > 0 else np.zeros(len(jobs))
@@ -115,7 +115,7 @@ def main():
     ap.add_argument("--rate", type=float, default=20.0, help="Poisson arrivals per second")
     ap.add_argument("--scale", type=float, default=0.1, help="sweep-count scale of the job mix")
     ap.add_argument("--budget-gb", type=float, default=64.0)
-    ap.add_argument("--lanes", type=int, default=3)
+    ap.add_argument("--lanes", type=int, default=4, help="size-class lanes (N <= 64, <= 144, <= 256, > 256)")
     ap.add_argument("--mixed", action="store_true",
                     help="FP32 matrix-core products and wraps (opt-in; FP64 by default)")
     ap.add_argument("--configs", nargs="+", default=None, help="TOML configuration files to serve as jobs")

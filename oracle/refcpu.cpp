@@ -30,6 +30,12 @@
 // compute_dynamic_greens_udt, Engine::sweep_dynamic, Engine::dynamic_measure) are pinned by
 // U = 0 closed forms and by time-ordered traces in the many-body Fock space of a 2x2 lattice
 // (tests/test_oracle_dynamic.py).
+// One deliberate departure, on request (Engine::use_stable_sign; the CLI and the tests' binding
+// turn it on): the initial configuration sign from the stabilized factorization of the stack
+// (stable_green_sign) instead of sgn det G, which is rounding noise once G has eigenvalues below
+// FP64 resolution; pinned against exact 50-digit determinants, against the reference's
+// definition where that one is resolved, and against particle-hole symmetry
+// (tests/test_oracle_sign.py).
 // Built with -ffp-contract=off so the canonical-double computation of the
 // random stream is not fused into FMAs.
 
@@ -219,6 +225,15 @@ static Mat inverse(const Mat& A) {
     Mat X = Mat::Identity(A.n);
     lu_solve(LU, piv, X);
     return X;
+}
+// the sign of det(A) from its LU: interchange parity times the pivot signs (no product of
+// pivots, so it neither underflows nor overflows)
+static double det_sign(const Mat& A) {
+    Mat LU = A; vector<int> piv; int ns;
+    lu_factor(LU, piv, ns);
+    double d = (ns % 2) ? -1.0 : 1.0;
+    for (int i = 0; i < A.n; ++i) if (LU(i, i) < 0) d = -d;
+    return d;
 }
 static double det_sign_times(const Mat& A) {  // returns the determinant (for the sign)
     Mat LU = A; vector<int> piv; int ns;
@@ -511,6 +526,29 @@ struct SvdStack {
         return r;
     }
 };
+
+// Sign of det G(0) = 1 / det(1 + M) from the stabilized factorization of one stack holding the
+// whole product M = U diag(S) V^T (the right stack at initialization, the left one empty):
+//   det(1 + M) = det(U) det(Db)^-1 det(Db U^T + Ds V^T),  Db = 1 / max(S, 1), Ds = min(S, 1),
+// so sign det G = sign det U x sign det(Db U^T + Ds V^T), signs of well-conditioned matrices.
+// The reference takes the sign of det G itself (initial_config_sign, src/dqmc_walker.cpp:162),
+// which is rounding noise once G has eigenvalues below FP64 resolution (low temperature, large
+// U: at beta 10, U 6, 16x16 half filling it comes out negative for some configurations, where
+// particle-hole symmetry fixes it to +1).  This is the engine's definition; the oracle computes
+// it on request (stable_sign) and is pinned to the reference's where that one is reliable.
+static double stable_green_sign(const SvdStack& st) {
+    const Mat U = st.MatrixU();
+    const vector<double> S = st.SingularValues();
+    const Mat V = st.MatrixV();
+    const int n = U.n;
+    Mat H(n);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            const double db = S[i] > 1.0 ? 1.0 / S[i] : 1.0, ds = S[i] > 1.0 ? 1.0 : S[i];
+            H(i, j) = db * U(j, i) + ds * V(j, i);
+        }
+    return det_sign(U) * det_sign(H);
+}
 
 // ----------------------------------------------------------------------------
 // Utils::NumericalStable (include/utils/numerical_stable.hpp)
@@ -887,6 +925,8 @@ struct Engine {
     vector<Mat> vgu, vgd;      // vec_green_tt
     vector<double> vsign;      // vec_config_sign
     double sign = 1.0, wrap_error = 0.0;
+    bool use_stable_sign = false;   // initial sign from the stabilized factorization (stable_green_sign)
+    double sign_reference = 1.0, sign_stable = 1.0;   // both definitions at the last init
     SvdStack lu_, ld_, ru_, rd_;
     vector<unsigned char> accepts;  // last sweep, (eff_t, i) at eff_t*N+i
     double acc[kNumObs] = {0}, cnt[kNumObs] = {0};
@@ -930,7 +970,9 @@ struct Engine {
         gd = compute_equaltime_greens(ld_, rd_);
     }
     void initial_sign() {                                          // :150-163
-        sign = (det_sign_times(gu) * det_sign_times(gd) >= 0) ? +1.0 : -1.0;
+        sign_reference = (det_sign_times(gu) * det_sign_times(gd) >= 0) ? +1.0 : -1.0;
+        sign_stable = stable_green_sign(ru_) * stable_green_sign(rd_);
+        sign = use_stable_sign ? sign_stable : sign_reference;
     }
     void init() { initial_svd_stacks(); initial_greens(); initial_sign(); wrap_error = 0.0; }
 
@@ -1204,6 +1246,8 @@ void rc_get_slice_green(void* h, int slot, int spin, double* out) {
 }
 double rc_get_slice_sign(void* h, int slot) { return ((Engine*)h)->vsign[slot]; }
 double rc_sign(void* h) { return ((Engine*)h)->sign; }
+void rc_set_stable_sign(void* h, int on) { ((Engine*)h)->use_stable_sign = on != 0; }
+double rc_init_sign(void* h, int stable) { Engine* e = (Engine*)h; return stable ? e->sign_stable : e->sign_reference; }
 double rc_wrap_error(void* h) { return ((Engine*)h)->wrap_error; }
 void rc_get_accepts(void* h, unsigned char* out) { Engine* e = (Engine*)h; std::memcpy(out, e->accepts.data(), e->accepts.size()); }
 void rc_measure(void* h) { ((Engine*)h)->equaltime_measure(); }
@@ -1326,7 +1370,7 @@ int main(int argc, char** argv) {
     int side = 4, slices = 80, pace = 10, kind = 0, cb = 0, warm = 100, bins = 10, bin_size = 20, between = 0;
     double beta = 4.0, t = 1.0, U = 4.0, mu = 0.0, max_seconds = -1.0;
     long long seed = 42;
-    int fact = 0, dynamic = 0;
+    int fact = 0, dynamic = 0, stable_sign = 1;   // the engine's sign definition (--stable-sign 0: the reference's)
     for (int a = 1; a + 1 < argc; a += 2) {
         std::string k = argv[a]; const char* v = argv[a + 1];
         if (k == "--side") side = atoi(v); else if (k == "--slices") slices = atoi(v);
@@ -1338,9 +1382,11 @@ int main(int argc, char** argv) {
         else if (k == "--seed") seed = atoll(v); else if (k == "--fact") fact = atoi(v);
         else if (k == "--max-seconds") max_seconds = atof(v);
         else if (k == "--dynamic") dynamic = atoi(v);
+        else if (k == "--stable-sign") stable_sign = atoi(v);
     }
     Engine e;
     e.setup(side, slices, beta, pace, kind, cb != 0, t, U, mu, -1, fact);
+    e.use_stable_sign = stable_sign != 0;
     e.rng.seed(seed);
     e.model.set_fields_random(e.rng);
     e.init();

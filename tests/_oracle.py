@@ -66,6 +66,9 @@ def lib():
         _lib.rc_set_dynamic_formula.argtypes = [C.c_void_p, C.c_int]
         _lib.rc_get_dynamic_green.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
         _lib.rc_get_dynamic_measure.argtypes = [C.c_void_p, C.c_void_p]
+        _lib.rc_set_stable_sign.argtypes = [C.c_void_p, C.c_int]
+        _lib.rc_init_sign.argtypes = [C.c_void_p, C.c_int]
+        _lib.rc_init_sign.restype = C.c_double
     return _lib
 
 
@@ -140,9 +143,13 @@ class Chain:
     """One oracle Markov chain (Walker + Model + Lattice + SvdStacks)."""
 
     def __init__(self, side, time_size, beta, pace=10, model=0, checkerboard=False, t=1.0, U=4.0, mu=0.0,
-                 momentum=-1, factorization=0):
+                 momentum=-1, factorization=0, stable_sign=True):
+        """stable_sign: the initial configuration sign from the stabilized factorization (the
+        engine's definition) instead of the sign of det G (the reference's, rounding noise at low
+        temperature); the two agree wherever det G is resolved (pinned in test_oracle_model)."""
         self.h = lib().rc_create(side, time_size, beta, pace, model, int(checkerboard), t, U, mu, momentum,
                                  factorization)
+        lib().rc_set_stable_sign(self.h, int(bool(stable_sign)))
         self.N = side * side
         self.L = time_size
 
@@ -203,6 +210,10 @@ class Chain:
     @property
     def sign(self):
         return lib().rc_sign(self.h)
+
+    def init_sign(self, stable):
+        """The initial sign of the last init by either definition."""
+        return lib().rc_init_sign(self.h, int(bool(stable)))
 
     @property
     def wrap_error(self):

@@ -4,9 +4,14 @@ of every slice over hundreds of sweeps (identical fields and random state at the
 equal-time measurements agree to rounding.
 
 * 4x4 repulsive, U=4, mu=0, beta=4, 80 slices, dense, pace 10: 100 warm-up + 200 measured sweeps
-* 8x8 attractive, |U|=4, mu=-0.5, beta=8, 160 slices, checkerboard, pace 10, delay 16: 200 sweeps
-* the 256-chain 12x12 batch (checkerboard, FP64, delay 32): three of its chains over 20 sweeps
+* 8x8 attractive, |U|=4, mu=-0.5, beta=8, 160 slices, checkerboard, pace 10, delay 16: 200 warm-up
+  + 500 measured sweeps, measurement sums to 1e-10 relative
+* the 256-chain 12x12 batch (checkerboard, FP64, delay 32): two of its chains slot by slot over
+  the last two of 20 sweeps, three at the end
+The oracle chains run in threads beside the GPU launches (ctypes releases the GIL).
 """
+import threading
+
 import numpy as np
 import pytest
 
@@ -46,46 +51,136 @@ def test_4x4_dense_full_schedule():
     ctx.close()
 
 
-def test_8x8_checkerboard_200_sweeps():
-    print("[gpu-long] 8x8 attractive checkerboard: 200 sweeps", flush=True)
+def test_8x8_checkerboard_full_schedule():
+    """The single-chain checkerboard workload over its whole schedule: 200 warm-up and 500
+    measured sweeps with delay depth 16; the measurement sums (sign, filling, double occupancy,
+    s-wave pairing and the rest) agree with the oracle's to 1e-10 relative."""
+    print("[gpu-long] 8x8 attractive checkerboard: 200 + 500 sweeps", flush=True)
     orc = O.Chain(8, 160, 8.0, pace=10, model=1, checkerboard=True, U=4.0, mu=-0.5)
     orc.seed(42); orc.fields_random(); orc.init()
     ctx = D.Context(0)
     b = D.Batch(ctx, [D.make_params(8, 160, 8.0, pace=10, model=D.ATTRACTIVE, checkerboard=True, delay=16,
                                     U=4.0, mu=-0.5, seed=42)])
     b.fields_random(); b.init()
-    for k in range(4):
-        for _ in range(25):
-            orc.sweep_forward(); orc.sweep_backward()
-        b.sweep(50)
+
+    def oracle_part(n_pairs, measure):
+        for _ in range(n_pairs):
+            orc.sweep_forward()
+            if measure:
+                orc.measure()
+            orc.sweep_backward()
+            if measure:
+                orc.measure()
+
+    for k, (n, meas) in enumerate([(100, False), (100, False), (100, True), (100, True), (100, True), (100, True),
+                                   (100, True)]):
+        th = threading.Thread(target=oracle_part, args=(n // 2, meas))
+        th.start()
+        b.sweep(n, measure=meas)
+        b.synchronize()
+        th.join()
         same = np.array_equal(b.fields(0).astype(np.float64), orc.fields())
-        print(f"[gpu-long] 8x8: after {50 * (k + 1)} sweeps fields identical: {same}", flush=True)
+        print(f"[gpu-long] 8x8: after {100 * (k + 1)} sweeps fields identical: {same}", flush=True)
         assert same
     info = b.info(0)
     assert info.rng == orc.rng_state and info.sign == orc.sign
     assert np.max(np.abs(b.green(0, 1) - orc.green(1))) < 1e-9
+    s_o, c_o = orc.measure_accum()
+    s_g, c_g = b.measure_accum(0)
+    assert np.array_equal(c_o, c_g)
+    rel = np.abs(s_g - s_o) / np.maximum(np.abs(s_o), 1e-300)
+    print(f"[gpu-long] 8x8: 500 measured sweeps, max relative difference of the sums {rel.max():.2e}", flush=True)
+    assert np.allclose(s_g, s_o, rtol=1e-10, atol=0.0), (s_g, s_o)
     ctx.close()
 
 
+def fresh_slots(L, pace, up):
+    if up:
+        return {t - 1 for t in range(1, L + 1) if t % pace == 0 or t == L}
+    return {t - 1 for t in range(1, L) if t % pace == 0} | {L - 1}
+
+
 def test_batched_12x12_chains_match_the_oracle():
-    # the headline workload itself: 256 chains of 12x12 (checkerboard, FP64, delay 32) in one
-    # launch; three of them against the oracle over 20 sweeps.  Seeds 0 and 1 give the same
-    # minstd_rand0 stream, so chains 0 and 1 are the same chain.
+    """The headline workload itself: 256 chains of 12x12 (checkerboard, FP64, delay 32) in one
+    launch over 20 sweeps.  Chains 0 and 255 are compared slot by slot with their oracle chains
+    over the last forward and backward sweep, both spins, and chains 0, 1 and 255 at the end
+    (fields, accept bits, random state, sign, G).  After 19 sweeps the chains are thermalized and
+    G(0) = (1 + B...B)^-1 is conditioned so that two exact-arithmetic-equivalent FP64 recomputes
+    differ by more than 1e-10: the oracle's own SVD and pre-pivoted QR stacks differ by up to
+    ~1e-9 on the same fields.  So each oracle chain runs in both factorizations, and the bar on
+    recomputed slots is max(1e-10, 10x that spread); wrapped slots within 10x the oracle's own
+    wrap error.  Seeds 0 and 1 give the same minstd_rand0 stream, so chains 0 and 1 are the same
+    chain."""
     print("[gpu-long] 256 x 12x12 batched: chains 0, 1, 255 vs the oracle over 20 sweeps", flush=True)
     ctx = D.Context(0)
-    ps = [D.make_params(12, 60, 6.0, pace=10, checkerboard=True, delay=32, U=4.0, mu=0.0, seed=c) for c in range(256)]
-    b = D.Batch(ctx, ps)
+    L, pace = 60, 10
+    ps = [D.make_params(12, L, 6.0, pace=pace, checkerboard=True, delay=32, U=4.0, mu=0.0, seed=c) for c in range(256)]
+    b = D.Batch(ctx, ps, record=True)
     b.fields_random(); b.init()
-    b.sweep(20)
+    orcs, qrs = {}, {}
     for c in (0, 1, 255):
-        orc = O.Chain(12, 60, 6.0, pace=10, model=0, checkerboard=True, U=4.0, mu=0.0)
-        orc.seed(c); orc.fields_random(); orc.init()
-        for _ in range(10):
-            orc.sweep_forward(); orc.sweep_backward()
+        for store, fact in ((orcs, 0), (qrs, 2)):
+            if fact == 2 and c == 1:
+                continue
+            o = O.Chain(12, L, 6.0, pace=pace, model=0, checkerboard=True, U=4.0, mu=0.0, factorization=fact)
+            o.seed(c); o.fields_random(); o.init()
+            store[c] = o
+    everyone = list(orcs.values()) + list(qrs.values())
+
+    def oracle_sweeps(o, pairs):
+        for _ in range(pairs):
+            o.sweep_forward(); o.sweep_backward()
+
+    th = [threading.Thread(target=oracle_sweeps, args=(o, 9)) for o in everyone]
+    for t in th:
+        t.start()
+    b.sweep(18)
+    b.synchronize()
+    for t in th:
+        t.join()
+    worst = {"fresh": 0.0, "wrapped": 0.0}
+    for up in (True, False):
+        b.sweep(1, record=True)
+        th = [threading.Thread(target=(o.sweep_forward if up else o.sweep_backward)) for o in everyone]
+        for t in th:
+            t.start()
+        b.synchronize()
+        for t in th:
+            t.join()
+        fresh = fresh_slots(L, pace, up)
+        for c in (0, 255):
+            o, q = orcs[c], qrs[c]
+            assert np.array_equal(b.accepts(c), o.accepts()), (c, up)
+            assert np.array_equal(q.accepts(), o.accepts()), (c, up)
+            sg = b.slice_signs(c)
+            wf = ww = spread = 0.0
+            for slot in range(L):
+                assert sg[slot] == o.slice_sign(slot), (c, up, slot)
+                for spin in (1, -1):
+                    e = np.max(np.abs(b.slice_green(c, slot, spin) - o.slice_green(slot, spin)))
+                    if slot in fresh:
+                        wf = max(wf, e)
+                        spread = max(spread, np.max(np.abs(q.slice_green(slot, spin) - o.slice_green(slot, spin))))
+                    else:
+                        ww = max(ww, e)
+            print(f"[gpu-long] chain {c} {'up' if up else 'down'}: max|dG| recomputed slots {wf:.2e} (oracle SVD vs QR "
+                  f"stacks {spread:.2e}), wrapped slots {ww:.2e} (oracle wrap error {o.wrap_error:.2e})", flush=True)
+            assert wf < max(1e-10, 10.0 * spread), (c, up, wf, spread)
+            assert ww < max(1e-9, 10.0 * o.wrap_error), (c, up, ww)
+            worst["fresh"] = max(worst["fresh"], wf)
+            worst["wrapped"] = max(worst["wrapped"], ww)
+    for c in (0, 1, 255):
+        orc = orcs[c]
         assert np.array_equal(b.fields(c).astype(np.float64), orc.fields()), c
         info = b.info(c)
         assert info.rng == orc.rng_state and info.sign == orc.sign, c
-        assert np.max(np.abs(b.green(c, -1) - orc.green(-1))) < 1e-10, c
+        q = qrs.get(c, qrs[0])
+        spread = max(np.max(np.abs(q.green(spin) - orc.green(spin))) for spin in (1, -1))
+        for spin in (1, -1):
+            assert np.max(np.abs(b.green(c, spin) - orc.green(spin))) < max(1e-10, 10.0 * spread), c
         print(f"[gpu-long] chain {c}: identical, wrap error {info.wrap_error:.2e}", flush=True)
     assert np.array_equal(b.fields(0), b.fields(1))
+    mx = max(b.info(c).wrap_error for c in range(256))
+    print(f"[gpu-long] 256 chains: max wrap error {mx:.2e}; vs oracle max|dG| recomputed {worst['fresh']:.2e}, "
+          f"wrapped {worst['wrapped']:.2e}", flush=True)
     ctx.close()

@@ -145,3 +145,29 @@ def test_dynamic_job_in_the_mix_matches_batch_api(ctx):
     m, e = D.bin_stats(dyn_bins)
     ours = np.concatenate([[dm["sign"], dm["rho"]], dm["gk"].T.ravel(), dm["dos"], dm["dss"]])
     assert np.allclose(ours, m, rtol=1e-12, atol=1e-12)
+
+
+def test_large_lattices_in_the_mix_match_the_batch_api(ctx):
+    """A 16x16 and a 20x20 job (the two largest lattices of the serving mix, in lanes of their
+    own) served among small jobs give the batch API's results exactly, in FP64."""
+    print("[gpu-serving] 16x16 and 20x20 jobs in a mix", flush=True)
+    big = [dict(id=1000, params=D.make_params(16, 40, 2.0, pace=10, checkerboard=True, delay=16, U=4.0, mu=-0.3,
+                                              seed=1000), warmup=2, measure=4, bin_size=2),
+           dict(id=1001, params=D.make_params(20, 24, 2.4, pace=10, model=D.ATTRACTIVE, checkerboard=True, delay=16,
+                                              U=5.0, mu=0.2, seed=1001), warmup=2, measure=4, bin_size=2),
+           dict(id=1002, params=D.make_params(20, 30, 3.0, pace=10, checkerboard=True, delay=16, U=6.0, mu=-0.5,
+                                              seed=1002), warmup=2, measure=4, bin_size=2)]
+    jobs = small_jobs(60, seed=9)[:8] + big
+    summary, res = serve.run(ctx, jobs, rate=30.0, budget_bytes=8 << 30)
+    assert summary["failed"] == 0
+    for j in big:
+        assert j["params"].precision == D.PREC_FP64
+        mean, err = batch_reference(ctx, j)
+        assert np.array_equal(np.array(res[j["id"]]["mean"]), mean), j["id"]
+        assert np.array_equal(np.array(res[j["id"]]["error"]), err), j["id"]
+        assert res[j["id"]]["chain_status"] == 0 and res[j["id"]]["sweeps"] == 6
+    print(f"[gpu-serving] p50 {summary['p50_latency_s']:.2f} s p99 {summary['p99_latency_s']:.2f} s", flush=True)
+
+
+def test_generated_jobs_are_fp64_by_default():
+    assert all(j["params"].precision == D.PREC_FP64 for j in serve.generate_jobs(20))
