@@ -22,6 +22,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "dqmc-framework_amd"))
 
 import dqmc_amd as D  # noqa: E402
+import numpy as np  # noqa: E402
 
 METRIC = "chain-sweeps/s batched vs plain FP64 GPU at max|dG|<=1e-6; p50/p99 job latency"
 
@@ -112,9 +113,10 @@ def autotune(w, candidates=(8, 16, 32)):
     return best, ms
 
 
-def run_gpu(w, steps, warmup, chains=None):
+def run_gpu(w, steps, warmup, chains=None, graph=True):
     ctx = D.Context(0)
     b = D.Batch(ctx, chain_params(w, chains))
+    b.use_graph(graph)   # the sweep launch replayed from a captured HIP graph
     b.fields_random()
     b.init()
     for _ in range(warmup):
@@ -240,32 +242,55 @@ def run_dynamic(w, steps, chains=None, profile=False):
     return out
 
 
-def cpu_baseline(w, seconds=12.0, max_procs=16):
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return "host CPU"
+
+
+def oracle_run(args, cpus, seconds):
+    """Oracle chains, one process per chain, each pinned to its own CPU; chain-sweeps/s of each."""
     exe = os.path.join(ROOT, "oracle", "_build", "refcpu")
     if not os.path.exists(exe):
         subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
-    procs = max(1, min(max_procs, os.cpu_count() or 1))
-    args = [exe, "--side", str(w["side"]), "--slices", str(w["time_size"]), "--beta", str(w["beta"]),
-            "--pace", str(w["pace"]), "--model", str(w["model"]), "--cb", str(int(w["checkerboard"])),
-            "--U", str(w["U"]), "--mu", str(w["mu"]), "--warm", "100000", "--bins", "0",
-            "--max-seconds", str(seconds)]
-    ps = [subprocess.Popen(args + ["--seed", str(c)], stdout=subprocess.PIPE) for c in range(procs)]
-    total = 0.0
+    base = [exe] + args + ["--warm", "100000", "--bins", "0", "--max-seconds", str(seconds)]
+    ps = [subprocess.Popen(base + ["--seed", str(c)], stdout=subprocess.PIPE,
+                           preexec_fn=(lambda cpu=cpu: os.sched_setaffinity(0, {cpu})))
+          for c, cpu in enumerate(cpus)]
+    rates = []
     for p in ps:
         out, _ = p.communicate()
-        total += json.loads(out.decode().strip().splitlines()[-1])["sweeps_per_s"]
-    model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
-    return {"value": total, "unit": "chain-sweeps/s", "cores": procs, "kind": "port",
-            "sample": f"{procs} oracle processes (one chain each, {w['side']}x{w['side']}, L={w['time_size']}), "
-                      f"~{seconds:.0f} s of sweeps each, on {model or 'host CPU'}"}
+        rates.append(json.loads(out.decode().strip().splitlines()[-1])["sweeps_per_s"])
+    return rates
+
+
+def cpu_baseline(w, seconds=12.0, max_procs=16):
+    """The reference's CPU path (one chain per MPI rank, src/dqmc_main.cpp:117) timed as the
+    oracle, one chain per process pinned one per core, on every CPU this process may run on up to
+    the 16 a one-GPU box gives it.  Calibrated on the reference's example configuration (8x8
+    attractive, U 4, beta 8, 160 slices, dense, pace 10: its warm-up ran 512 sweeps in 53.95 s on
+    one rank, 9.49 sweeps/s, example/log.out), timed here on one core."""
+    cpus = sorted(os.sched_getaffinity(0))[:max_procs]
+    args = ["--side", str(w["side"]), "--slices", str(w["time_size"]), "--beta", str(w["beta"]),
+            "--pace", str(w["pace"]), "--model", str(w["model"]), "--cb", str(int(w["checkerboard"])),
+            "--U", str(w["U"]), "--mu", str(w["mu"])]
+    rates = oracle_run(args, cpus, seconds)
+    example = ["--side", "8", "--slices", "160", "--beta", "8.0", "--pace", "10", "--model", "1", "--cb", "0",
+               "--U", "4.0", "--mu", "0.0"]
+    cal = oracle_run(example, cpus[:1], 8.0)[0]
+    model = cpu_model()
+    return {"value": sum(rates), "unit": "chain-sweeps/s", "cores": len(cpus), "kind": "port",
+            "per_core": sum(rates) / len(cpus), "cpu": model,
+            "calibration": {"config": "example/config.toml: 8x8 attractive U=4 beta=8 L=160 dense p=10",
+                            "oracle_sweeps_per_s_per_core": cal, "reference_sweeps_per_s_per_core": 9.49,
+                            "oracle_over_reference": cal / 9.49},
+            "sample": f"{len(cpus)} oracle processes pinned one per core (one chain each, {w['side']}x{w['side']}, "
+                      f"L={w['time_size']}), ~{seconds:.0f} s of sweeps each, on {model}"}
 
 
 def main():
@@ -277,14 +302,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the single-chain comparison run")
     ap.add_argument("--phases", action="store_true", help="also report device time per sweep phase")
-    ap.add_argument("--precision", choices=["workload", "fp64", "mixed"], default="workload",
-                    help="override the workload's precision mode")
+    ap.add_argument("--precision", choices=["workload", "fp64", "mixed", "fp32_wrap", "fp32_product"],
+                    default="workload", help="override the workload's precision mode")
     ap.add_argument("--dynamic-steps", type=int, default=3,
                     help="time-displaced sweeps timed on the same batch (0 skips them)")
     ap.add_argument("--no-autotune", action="store_true", help="use the workload's fixed delay depth")
-    ap.add_argument("--serve-jobs", type=int, default=48, help="jobs of the serving sample (0 skips it)")
-    ap.add_argument("--serve-scale", type=float, default=0.05, help="sweep-count scale of the serving sample")
-    ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-jobs", type=int, default=2000, help="jobs of the serving stream (0 skips it)")
+    ap.add_argument("--serve-scale", type=float, default=0.1, help="sweep-count scale of the serving stream")
+    ap.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrivals per second")
+    ap.add_argument("--no-graph", action="store_true", help="launch the sweeps directly instead of replaying a graph")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only fill and copy a device buffer of this size and exit (a known-traffic reference for "
                          "profiler byte counters)")
@@ -305,7 +331,8 @@ def main():
         sys.exit(2)
     w = dict(WORKLOADS[args.workload])
     if args.precision != "workload":
-        w["precision"] = D.PREC_FP64 if args.precision == "fp64" else D.PREC_MIXED
+        w["precision"] = {"fp64": D.PREC_FP64, "mixed": D.PREC_MIXED, "fp32_wrap": D.PREC_FP32_WRAP,
+                          "fp32_product": D.PREC_FP32_PRODUCT}[args.precision]
         w["desc"] += f" (precision override: {args.precision})"
     tune = None
     if not args.no_autotune and w["chains"] > 1:
@@ -313,7 +340,7 @@ def main():
         w["delay"] = best
         print(f"[bench] delay depth autotuned: {best} (ms per 2 sweeps {tune})", file=sys.stderr, flush=True)
     print(f"[bench] {w['desc']}: warmup {args.warmup}, steps {args.steps}", file=sys.stderr, flush=True)
-    r = run_gpu(w, args.steps, args.warmup)
+    r = run_gpu(w, args.steps, args.warmup, graph=not args.no_graph)
     chain_sweeps = r["chains"] * args.steps
     value = chain_sweeps / r["wall"]
     ms_per_step = 1000.0 * r["wall"] / args.steps
@@ -324,13 +351,15 @@ def main():
     out = {
         "metric": METRIC, "value": value, "unit": "chain-sweeps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32+f64" if w.get("precision") == D.PREC_MIXED else "f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f64" if w.get("precision", D.PREC_FP64) == D.PREC_FP64 else "f32+f64",
+        "data": "synthetic",
         "config": {"workload": w["desc"], "chains": r["chains"], "side": w["side"], "time_size": w["time_size"],
                    "beta": w["beta"], "pace": w["pace"], "checkerboard": w["checkerboard"], "delay": w["delay"]},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "kernel": "k_sweep", "kernel_ms": kms, "algorithmic_gflop_per_launch": flops_launch / 1e9},
         "accepted_per_chain_sweep": r["accepted"], "max_wrap_error": r["wrap_error"],
+        "launch": "hipGraph replay" if not args.no_graph else "direct",
     }
     if tune is not None:
         out["config"]["delay_autotune_ms"] = tune
@@ -349,6 +378,17 @@ def main():
                                             "single": phase_breakdown(w, 1)}
         out["phase_rooflines"] = phase_rooflines(w, out["phases_ms_per_chain_sweep"]["batched"], r["chains"],
                                                  r["accepted"])
+    if w.get("precision", D.PREC_FP64) != D.PREC_FP64:
+        import precision
+        print("[bench] precision check: recomputed G at every stabilization vs FP64", file=sys.stderr, flush=True)
+        ctx = D.Context(0)
+        picks = [0, r["chains"] // 4] if r["chains"] > 1 else [0]
+        dg = precision.stabilization_errors(ctx, [chain_params(w)[c] for c in picks])
+        ctx.close()
+        out["precision_check"] = {
+            "chains": picks, "stabilizations_x_spins": int(dg.size), "dG_recomputed_median": float(np.median(dg)),
+            "dG_recomputed_max": float(dg.max()), "wrap_error_max_over_batch": r["wrap_error"],
+            "target_1e-6_met": bool(dg.max() <= 1e-6 and r["wrap_error"] <= 1e-6)}
     if args.dynamic_steps > 0:
         print(f"[bench] time-displaced sweeps: {args.dynamic_steps}", file=sys.stderr, flush=True)
         out["dynamic"] = run_dynamic(w, args.dynamic_steps, profile=args.phases)
@@ -357,7 +397,7 @@ def main():
         print(f"[bench] serving sample: {args.serve_jobs} jobs at {args.serve_rate}/s", file=sys.stderr, flush=True)
         jobs = serve.generate_jobs(args.serve_jobs, scale=args.serve_scale)
         ctx = D.Context(0)
-        summary, _ = serve.run(ctx, jobs, args.serve_rate, 64 << 30, log=sys.stderr)
+        summary, _ = serve.run(ctx, jobs, args.serve_rate, 160 << 30, log=sys.stderr)
         ctx.close()
         summary["job_mix"] = (f"side in {{4..20}}, beta U[2,12], dtau in {{0.05,0.1}}, U U[-8,8], mu U[-1,1], "
                               f"sweeps U[50,500] x {args.serve_scale}, seed = job id, FP64, checkerboard")

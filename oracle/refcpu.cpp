@@ -1245,6 +1245,13 @@ void rc_get_slice_green(void* h, int slot, int spin, double* out) {
     std::memcpy(out, g.a.data(), sizeof(double) * g.a.size());
 }
 double rc_get_slice_sign(void* h, int slot) { return ((Engine*)h)->vsign[slot]; }
+// a stored slot's G of both spins and its sign (the input of equaltime_measure)
+void rc_set_slice(void* h, int slot, const double* gu, const double* gd, double sign) {
+    Engine* e = (Engine*)h;
+    std::memcpy(e->vgu[slot].a.data(), gu, sizeof(double) * e->vgu[slot].a.size());
+    std::memcpy(e->vgd[slot].a.data(), gd, sizeof(double) * e->vgd[slot].a.size());
+    e->vsign[slot] = sign;
+}
 double rc_sign(void* h) { return ((Engine*)h)->sign; }
 void rc_set_stable_sign(void* h, int on) { ((Engine*)h)->use_stable_sign = on != 0; }
 double rc_init_sign(void* h, int stable) { Engine* e = (Engine*)h; return stable ? e->sign_stable : e->sign_reference; }

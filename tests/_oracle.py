@@ -67,6 +67,7 @@ def lib():
         _lib.rc_get_dynamic_green.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
         _lib.rc_get_dynamic_measure.argtypes = [C.c_void_p, C.c_void_p]
         _lib.rc_set_stable_sign.argtypes = [C.c_void_p, C.c_int]
+        _lib.rc_set_slice.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_double]
         _lib.rc_init_sign.argtypes = [C.c_void_p, C.c_int]
         _lib.rc_init_sign.restype = C.c_double
     return _lib
@@ -203,6 +204,10 @@ class Chain:
         g = np.zeros((self.N, self.N), order="F")
         lib().rc_get_slice_green(self.h, slot, spin, _p(g))
         return g
+
+    def set_slice(self, slot, gu, gd, sign=1.0):
+        a = np.asfortranarray(gu, dtype=np.float64); b = np.asfortranarray(gd, dtype=np.float64)
+        lib().rc_set_slice(self.h, slot, _p(a), _p(b), float(sign))
 
     def slice_sign(self, slot):
         return lib().rc_get_slice_sign(self.h, slot)

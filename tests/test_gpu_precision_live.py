@@ -25,6 +25,7 @@ import pytest
 
 import _oracle as O
 import dqmc_amd as D
+import precision as P
 
 pytestmark = pytest.mark.gpu
 
@@ -91,35 +92,8 @@ def test_fp64_batch_matches_oracle(ctx):
 
 def stabilization_errors(ctx, precision, chains=(0, 16), sweeps=4):
     """max|dG| of the recomputed G at every stabilization of the upward sweeps against an FP64
-    recompute of the configuration the chain had at that moment."""
-    r = D.Batch(ctx, [params(c, precision) for c in chains], record=True)
-    r.fields_random()
-    r.init()
-    slices = [t for t in range(1, L + 1) if t % PACE == 0 or t == L]
-    out = []
-    for s in range(sweeps):
-        before = [r.fields(k).copy() for k in range(len(chains))]
-        r.sweep(1, record=True)
-        after = [r.fields(k).copy() for k in range(len(chains))]
-        if s % 2 == 1:
-            continue   # the backward sweep keeps post-update G, not the recomputed one
-        cfgs = []
-        for k in range(len(chains)):
-            for t in slices:
-                h = before[k].copy()
-                h[:t] = after[k][:t]
-                cfgs.append((k, t, h))
-        ref = D.Batch(ctx, [params(chains[k], D.PREC_FP64) for k, _, _ in cfgs], record=True)
-        for q, (k, t, h) in enumerate(cfgs):
-            ref.set_fields(q, h)
-        ref.init()
-        ref.sweep(1, record=True, frozen=True)
-        for q, (k, t, h) in enumerate(cfgs):
-            for spin in (1, -1):
-                out.append(np.max(np.abs(r.slice_green(k, t - 1, spin) - ref.slice_green(q, t - 1, spin))))
-        ref.close()
-    r.close()
-    return np.array(out)
+    recompute of the configuration the chain had at that moment (precision.py)."""
+    return P.stabilization_errors(ctx, [params(c, precision) for c in chains], sweeps=sweeps)
 
 
 def test_fp64_stabilization_check_is_exact(ctx):
