@@ -11,6 +11,10 @@
 #define DQMC_KMAX 32         // largest delayed-update depth (a Metropolis block holds its sites in the 32
                              // lanes of a half-wavefront per spin)
 #define DQMC_PANEL_DOUBLES 11776   // LDS panel of a workgroup (92 KB)
+// spin-pair exchange area of a repulsive chain: the k x k walk blocks [spin][parity][KMAX^2], then
+// the per-sweep state slots [spin][parity][2] (status bits, wrap error)
+#define DQMC_PAIR_XBUF_DOUBLES (4 * DQMC_KMAX * DQMC_KMAX + 8)
+#define DQMC_PAIR_FLAG_WORDS 64    // per chain: the two spins' sequence flags, 128 bytes apart
 
 enum {
     DQMC_FLAG_MEASURE = 1,   // accumulate equal-time observables after each sweep
@@ -80,8 +84,8 @@ struct ChainDesc {
     double* stkU;        // nspin x n_stack x N*N
     double* stkT;        // nspin x n_stack x N*N   (cumulative V^T of each level)
     double* stkD;        // nspin x n_stack x N
-    double* W;           // 3 x N*N scratch for the stabilization
-    double* Wq;          // 3 x 16 x (N + 16) scratch of the blocked QR (GEMM panels and T factors)
+    double* W;           // nspin x 3 x N*N scratch for the stabilization (one set per spin)
+    double* Wq;          // nspin x 3 x 16 x (N + 16) scratch of the blocked QR (GEMM panels and T factors)
     const double* expK;  // N*N (dense only)
     const double* invExpK;
     const double* cosq;  // N: Re e^{-i Q r_d} for displacement index d
@@ -98,4 +102,6 @@ struct ChainDesc {
     uint8_t* recAcc;     // L x N (RECORD)
     double* recGt0;      // L x nspin x N*N (RECORD): G(t,0) per slot of the last time-displaced sweep
     double* recG0t;      // L x nspin x N*N (RECORD): G(0,t)
+    double* xbuf;        // DQMC_PAIR_XBUF_DOUBLES (repulsive chains): spin-pair exchange blocks
+    unsigned* xflag;     // DQMC_PAIR_FLAG_WORDS (repulsive chains): spin-pair flags, zeroed before a pair launch
 };

@@ -62,6 +62,11 @@ struct Lds {
     int status;                   // DQMC_STATUS_* bits met by the running launch (thread 0 / panel lanes set them)
     unsigned long long prof[2];   // sub-phase ticks (walk, panels) of the running launch
     int prof_on;
+    // spin range of this workgroup: [sp_lo, sp_hi) = [0, nspin), or one spin of a pair launch
+    int sp_lo, sp_hi;
+    int pair, me;                 // pair launch; this workgroup's spin in it
+    unsigned xseq;                // exchanges of this launch so far (pair launch)
+    int xbroken;                  // the partner stopped answering: no further waits
 };
 
 // one LDS object per workgroup, at namespace scope so that separately
@@ -1838,11 +1843,16 @@ __device__ __forceinline__ double* slotD(Desc d, int sp, int k) {
 }
 
 // push A onto the stack whose current top is slot `below` (-1 if empty); result at `slot`
+// stabilization scratch of one spin (each spin has its own, so the two spins of a pair launch
+// stabilize side by side)
+__device__ __forceinline__ double* spinW(Desc d, int sp) { return d.W + (size_t)sp * 3 * d.N * d.N; }
+__device__ __forceinline__ double* spinWq(Desc d, int sp) { return d.Wq + (size_t)sp * 3 * 16 * (d.N + 16); }
+
 __device__ __noinline__ void wg_push(Desc d, int sp, int slot, int below, const double* A, Lds&) {
     Lds& s = g_lds;
     const int N = d.N;
-    double* W0 = d.W;
-    double* W1 = d.W + (size_t)N * N;
+    double* W0 = spinW(d, sp);
+    double* W1 = W0 + (size_t)N * N;
     if (below < 0) wg_copy(W0, A, N * N);
     else {
         const double* Db = slotD(d, sp, below);   // the GEMM's column scaling, staged in LDS
@@ -1850,7 +1860,7 @@ __device__ __noinline__ void wg_push(Desc d, int sp, int slot, int below, const 
         __syncthreads();
         gemm64(N, N, N, A, N, false, slotU(d, sp, below), N, false, W0, N, nullptr, nullptr, s.v3, 0.0, s);
     }
-    wg_qr_udv(W0, d.W + 2 * (size_t)N * N, N, slotU(d, sp, slot), slotD(d, sp, slot), W1, d.Wq, s);
+    wg_qr_udv(W0, W0 + 2 * (size_t)N * N, N, slotU(d, sp, slot), slotD(d, sp, slot), W1, spinWq(d, sp), s);
     __syncthreads();
     if (below < 0) wg_copy(slotT(d, sp, slot), W1, N * N);
     else gemm64(N, N, N, W1, N, false, slotT(d, sp, below), N, false, slotT(d, sp, slot), N, nullptr, nullptr, nullptr, 0.0, s);
@@ -1902,8 +1912,8 @@ __device__ __noinline__ void wg_transpose_scaled(double* dst1_, double* dst2_, c
 __device__ __noinline__ double wg_fresh_green(Desc d, int sp, int lslot, int rslot, double* Gout, Lds&) {
     Lds& s = g_lds;
     const int N = d.N;
-    double* W0 = d.W;
-    double* W1 = d.W + (size_t)N * N;
+    double* W0 = spinW(d, sp);
+    double* W1 = W0 + (size_t)N * N;
     if (lslot >= 0 && rslot >= 0) {
         const double* UL = slotU(d, sp, lslot); const double* TL = slotT(d, sp, lslot); const double* DL = slotD(d, sp, lslot);
         const double* UR = slotU(d, sp, rslot); const double* TR = slotT(d, sp, rslot); const double* DR = slotD(d, sp, rslot);
@@ -2133,7 +2143,9 @@ __device__ __noinline__ void block_apply(double* G_, int N, int k, int wave_lo, 
 
 // G0[S',S'] of the next block (b1, k1) of one spin, formed from the current
 // block's factors before they are applied: G[S',S'] + ACP[S',:] W[:,S']
-// (the same update block_apply makes to those elements), into GSS
+// (the same update block_apply makes to those elements), into the spin's half
+// of GR only: GT shares its space with the C rows that block_stage_w of the
+// other spin may still read, and is filled by gss_transpose afterwards
 __device__ __noinline__ void gss_from_factors(const double* G_, int N, int k, int b1, int k1, int sp, Lds&) {
     Lds& s = g_lds;
     N = uni(N); k = uni(k); b1 = uni(b1); k1 = uni(k1); sp = uni(sp);
@@ -2142,8 +2154,7 @@ __device__ __noinline__ void gss_from_factors(const double* G_, int N, int k, in
     const double* const acp = s.panel;
     const double* const gsw = acp + (size_t)N * k4;
     double* gr = MP_GR + 32 * sp;
-    double* gt = MP_GT + 32 * sp;
-    for (int q = tid(); q < KB * KB; q += NT) { gr[(q >> 5) * 64 + (q & 31)] = 0.0; gt[(q >> 5) * 64 + (q & 31)] = 0.0; }
+    for (int q = tid(); q < KB * KB; q += NT) gr[(q >> 5) * 64 + (q & 31)] = 0.0;
     __syncthreads();
     const FastDiv dk1(k1);
     for (int q = tid(); q < k1 * k1; q += NT) {
@@ -2151,7 +2162,17 @@ __device__ __noinline__ void gss_from_factors(const double* G_, int N, int k, in
         double v = G[(size_t)(b1 + c) * N + b1 + a];
         for (int j = 0; j < k4; ++j) v += acp[j * N + b1 + a] * gsw[j * N + b1 + c];
         gr[c * 64 + a] = v;
-        gt[a * 64 + c] = v;
+    }
+}
+
+// GT half of one spin from its GR half (the whole 32 x 32, zeros included)
+__device__ void gss_transpose(int sp, Lds&) {
+    __syncthreads();
+    const double* gr = MP_GR + 32 * sp;
+    double* gt = MP_GT + 32 * sp;
+    for (int q = tid(); q < KB * KB; q += NT) {
+        const int a = q & 31, c = q >> 5;
+        gt[a * 64 + c] = gr[c * 64 + a];
     }
 }
 
@@ -2299,17 +2320,119 @@ __device__ __forceinline__ WalkOut walk_block(Desc d, int e, int b0, int k, uint
     return WalkOut{rng, sign, nacc};
 }
 
+// ---------------------------------------------------------------------------
+// Spin pairs.  In a pair launch a repulsive chain runs on two workgroups, one
+// per spin, launched cooperatively so that both are resident.  The spins are
+// independent except in the walk (the acceptance ratio is the product of the
+// two spins' ratios), so per delayed-update block each workgroup publishes its
+// spin's G0[S,S] and takes the partner's; both then walk the block with
+// identical inputs and reach identical decisions, fields, sign and random state,
+// and each applies the block to its own spin.  The partners also meet around a
+// measurement (which reads both spins' G) and at the end of every sweep (status
+// bits and wrap error merged, so both leave the sweep loop together).
+// An exchange point: every wavefront's stores reach the L2 (vmcnt(0)), thread
+// 0 releases the running exchange number on this spin's flag (agent scope, so
+// the partner may sit on another XCD) and acquires the partner's flag.  The wait
+// is bounded by the 100 MHz wall clock: a partner silent for 2 s sets
+// DQMC_STATUS_PAIR_SYNC (fatal) and later waits are skipped, so the kernel always
+// ends.  The flags are zeroed by the host before each pair launch and compared
+// by signed difference.  Data published at exchange q lives in buffer slot
+// q & 1: a workgroup writes slot q & 1 again only at exchange q + 2, after the
+// partner has arrived at q + 1, which it does only after reading slot q & 1.
+// ---------------------------------------------------------------------------
+#define PAIR_TIMEOUT_TICKS 200000000ull   // 2 s of the 100 MHz wall clock
+
+__device__ __forceinline__ int pair_slot() { return (int)((g_lds.xseq + 1u) & 1u); }   // slot of the next exchange
+
+__device__ __noinline__ void pair_exchange(Desc d, Lds&) {
+    Lds& s = g_lds;
+    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this wavefront's stores are in the L2
+    __syncthreads();
+    if (tid() == 0) {
+        const unsigned q = s.xseq + 1u;
+        s.xseq = q;
+        unsigned* mine = d.xflag + 32 * s.me;
+        unsigned* theirs = d.xflag + 32 * (1 - s.me);
+        __hip_atomic_store(mine, q, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (!s.xbroken) {
+            const unsigned long long t0 = wall_clock64();
+            while ((int)(__hip_atomic_load(theirs, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) - q) < 0) {
+                if (wall_clock64() - t0 > PAIR_TIMEOUT_TICKS) {
+                    s.xbroken = 1;
+                    s.status |= DQMC_STATUS_PAIR_SYNC;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// publish this workgroup's half of the walk's G0[S,S] (k x k, from LDS) and take the partner's
+// into the other half (zero outside the k x k block, as stage_gss leaves it)
+__device__ __noinline__ void pair_swap_gss(Desc d, int k, Lds&) {
+    Lds& s = g_lds;
+    k = uni(k);
+    const int me = uni(s.me), ot = 1 - me;
+    const int slot = uni(pair_slot());
+    const FastDiv dk(k);
+    __syncthreads();   // this spin's half is complete
+    AS_G double* mine = gptr(d.xbuf + (size_t)(me * 2 + slot) * KB * KB);
+    const double* gr = MP_GR + 32 * me;
+    for (int q = tid(); q < k * k; q += NT) {
+        const int c = dk.div(q), a = q - c * k;
+        mine[q] = gr[c * 64 + a];
+    }
+    pair_exchange(d, s);
+    const AS_G double* theirs = gptr(d.xbuf + (size_t)(ot * 2 + slot) * KB * KB);
+    double* gro = MP_GR + 32 * ot;
+    double* gto = MP_GT + 32 * ot;
+    for (int q = tid(); q < KB * KB; q += NT) { gro[(q >> 5) * 64 + (q & 31)] = 0.0; gto[(q >> 5) * 64 + (q & 31)] = 0.0; }
+    __syncthreads();
+    wg_stream(k * k, [&](int q) { return theirs[q]; },
+              [&](int q, double v) { const int c = dk.div(q), a = q - c * k; gro[c * 64 + a] = v; gto[a * 64 + c] = v; });
+}
+
+// end of a sweep in a pair launch: status bits and wrap error of the two spins merged (thread 0's st)
+__device__ __noinline__ void pair_merge_state(Desc d, ChainState& st, Lds&) {
+    Lds& s = g_lds;
+    const int me = s.me, ot = 1 - me;
+    const int slot = pair_slot();
+    AS_G double* slots = gptr(d.xbuf + 4 * KB * KB);   // [spin][slot][status, wrap error]
+    if (tid() == 0) {
+        slots[(me * 2 + slot) * 2] = (double)s.status;
+        slots[(me * 2 + slot) * 2 + 1] = st.wrap_error;
+    }
+    pair_exchange(d, s);
+    if (tid() == 0) {
+        s.status |= (int)slots[(ot * 2 + slot) * 2];
+        const double w = slots[(ot * 2 + slot) * 2 + 1];
+        if (!(w <= st.wrap_error)) st.wrap_error = w;   // (a NaN from either side is kept)
+    }
+    __syncthreads();
+}
+
 // (the results come back by value: references to the caller's variables would be
 // generic pointers into private memory, read and written with flat instructions)
 __device__ __noinline__ WalkOut wg_metropolis(Desc d, int e, uint64_t rng, double sign, bool record, Lds&) {
     Lds& s = g_lds;
     const int N = __builtin_amdgcn_readfirstlane(d.N);
     const int ns = __builtin_amdgcn_readfirstlane(d.nspin);
+    const bool pair = uni(s.pair) != 0;
+    const int me = uni(s.me);
+    // the spins this workgroup updates: both (one-spin model: spin 0) or its own in a pair launch
+    const int lo = pair ? me : 0, hi = pair ? me + 1 : ns;
     int naccept = 0;
     const int kb = __builtin_amdgcn_readfirstlane(block_depth(d));
     auto Gs = [&](int sp) { return d.G + (size_t)sp * N * N; };
     // G0[S,S] of the first block, both halves of the walk (also in the one-spin model)
-    for (int sp = 0; sp < 2; ++sp) stage_gss(Gs(sp < ns ? sp : 0), N, 0, min(kb, N), sp, s);
+    if (pair) {
+        stage_gss(Gs(me), N, 0, min(kb, N), me, s);
+        pair_swap_gss(d, min(kb, N), s);
+    } else {
+        for (int sp = 0; sp < 2; ++sp) stage_gss(Gs(sp < ns ? sp : 0), N, 0, min(kb, N), sp, s);
+    }
     bool pending = false;   // the last spin's update of the previous block is still to be applied
     int pk = 0;
     for (int b0 = 0; b0 < N; b0 += kb) {
@@ -2324,7 +2447,7 @@ __device__ __noinline__ WalkOut wg_metropolis(Desc d, int e, uint64_t rng, doubl
             sign = o.sign;
             if (lane() == 0) s.ivar[2] = o.accepted;
         } else if (pending) {
-            block_apply(Gs(ns - 1), N, pk, 1, s);
+            block_apply(Gs(hi - 1), N, pk, 1, s);
         }
         __syncthreads();
         walk_clk.stop(0);
@@ -2333,11 +2456,13 @@ __device__ __noinline__ WalkOut wg_metropolis(Desc d, int e, uint64_t rng, doubl
         naccept += nblk;
         const int b1 = b0 + kb, k1 = min(kb, N - b1);   // the next block (k1 <= 0: none)
         if (nblk > 0) {
-            for (int sp = 0; sp < ns; ++sp) {
+            for (int sp = lo; sp < hi; ++sp) {
                 block_stage_w(Gs(sp), N, b0, k, sp, s);
-                if (sp == ns - 1 && k1 > 0) {
-                    // the next walk starts from the factors; the update itself runs beside it
-                    gss_from_factors(Gs(sp), N, k, b1, k1, sp, s);
+                // every spin's next G0[S',S'] comes from the factors, before they are applied (so a
+                // pair launch, where each spin's update is the pending one, computes the same numbers)
+                if (k1 > 0) gss_from_factors(Gs(sp), N, k, b1, k1, sp, s);
+                if (sp == hi - 1 && k1 > 0) {
+                    // the next walk starts from the factors; the last spin's update runs beside it
                     pending = true;
                     pk = k;
                 } else {
@@ -2345,11 +2470,16 @@ __device__ __noinline__ WalkOut wg_metropolis(Desc d, int e, uint64_t rng, doubl
                     __syncthreads();
                 }
             }
+            if (k1 > 0)
+                for (int sp = lo; sp < hi; ++sp) gss_transpose(sp, s);
         }
         if (k1 > 0) {
-            for (int sp = 0; sp < 2; ++sp) {
-                if (pending && sp == ns - 1) continue;
-                stage_gss(Gs(sp < ns ? sp : 0), N, b1, k1, sp, s);
+            if (pair) {
+                if (nblk == 0) stage_gss(Gs(me), N, b1, k1, me, s);
+                pair_swap_gss(d, k1, s);
+            } else {
+                // (after accepted flips: only the one-spin model's second half, a copy it does not use)
+                for (int sp = nblk > 0 ? ns : 0; sp < 2; ++sp) stage_gss(Gs(sp < ns ? sp : 0), N, b1, k1, sp, s);
             }
         }
     }
@@ -2434,12 +2564,14 @@ __device__ __noinline__ void wg_measure_from(Desc d, const double* G, double cs,
 __device__ void wg_measure(Desc d, double cs, Lds&) { Lds& s = g_lds; wg_measure_from(d, d.G, cs, s); }
 
 __device__ void record_slot(Desc d, int slot, double sign, bool with_sign) {
+    Lds& s = g_lds;
     const int N = d.N;
     const size_t n2 = (size_t)N * N;
-    AS_G double* dst = gptr(d.recG + (size_t)slot * d.nspin * n2);
-    const AS_G double* src = gptr(d.G);
-    wg_stream((int)(n2 * d.nspin), [&](int e) { return src[e]; }, [&](int e, double v) { dst[e] = v; });
-    if (with_sign && tid() == 0) gptr(d.recSign)[slot] = sign;
+    const int lo = uni(s.sp_lo), hi = uni(s.sp_hi);   // this workgroup's spins
+    AS_G double* dst = gptr(d.recG + ((size_t)slot * d.nspin + lo) * n2);
+    const AS_G double* src = gptr(d.G + lo * n2);
+    wg_stream((int)(n2 * (hi - lo)), [&](int e) { return src[e]; }, [&](int e, double v) { dst[e] = v; });
+    if (with_sign && tid() == 0 && lo == 0) gptr(d.recSign)[slot] = sign;
     __syncthreads();
 }
 
@@ -2458,8 +2590,9 @@ __device__ __noinline__ void stabilize_green(Desc d, int lslot, int rslot, bool 
     Lds& s = g_lds;
     const int N = d.N;
     const size_t n2 = (size_t)N * N;
+    const int lo = uni(s.sp_lo), hi = uni(s.sp_hi);   // this workgroup's spins
     double err = 0.0;
-    for (int sp = 0; sp < d.nspin; ++sp) {
+    for (int sp = lo; sp < hi; ++sp) {
         double* Gf = d.Gw + sp * n2;
         wg_fresh_green(d, sp, lslot, rslot, Gf, s);
         const AS_G double* Gfg = gptr(Gf);
@@ -2468,7 +2601,7 @@ __device__ __noinline__ void stabilize_green(Desc d, int lslot, int rslot, bool 
     }
     err = wg_max(err, s);
     if (tid() == 0) note_error(d, st, err, track_error);
-    wg_copy(d.G, d.Gw, (int)(n2 * d.nspin));
+    wg_copy(d.G + lo * n2, d.Gw + lo * n2, (int)(n2 * (hi - lo)));
 }
 
 // ---------------------------------------------------------------------------
@@ -2797,6 +2930,16 @@ __device__ void record_dynamic(Desc d, int slot) {
     wg_copy(d.recG0t + slot * n, d.G0t, (int)n);
 }
 
+// the equal-time measurement of the current slice; in a pair launch the partners meet while
+// spin 0's workgroup reads both spins' G (and before either changes its G again)
+__device__ void measure_slice(Desc d, double cs, Lds&) {
+    Lds& s = g_lds;
+    if (!uni(s.pair)) { wg_measure(d, cs, s); return; }
+    pair_exchange(d, s);
+    if (uni(s.me) == 0) wg_measure(d, cs, s);
+    pair_exchange(d, s);
+}
+
 // per-phase device time (thread 0 reads the 100 MHz wall clock at barriers)
 struct PhaseClock {
     bool on;
@@ -2818,12 +2961,13 @@ __device__ void sweep_up(Desc d, ChainState& st, uint64_t& rng, double& sign, in
     const int N = d.N, L = d.L;
     const size_t n2 = (size_t)N * N;
     const bool rec = flags & DQMC_FLAG_RECORD, meas = flags & DQMC_FLAG_MEASURE;
+    const int lo = uni(s.sp_lo), hi = uni(s.sp_hi);   // this workgroup's spins
     PhaseClock clk(flags & DQMC_FLAG_PROFILE);
-    for (int sp = 0; sp < d.nspin; ++sp) wg_identity(d.Tmp + sp * n2, N);
+    for (int sp = lo; sp < hi; ++sp) wg_identity(d.Tmp + sp * n2, N);
     for (int t = 1; t <= L; ++t) {
         const int e = t - 1;
         load_field_row(d, e, s);
-        for (int sp = 0; sp < d.nspin; ++sp) wg_wrap(d, sp, true, d.G + sp * n2, d.Gw + sp * n2, s);
+        for (int sp = lo; sp < hi; ++sp) wg_wrap(d, sp, true, d.G + sp * n2, d.Gw + sp * n2, s);
         clk.lap(st, DQMC_PH_WRAP);
         if (!(flags & DQMC_FLAG_FROZEN)) {
             const WalkOut o = wg_metropolis(d, e, rng, sign, rec, s);
@@ -2833,20 +2977,20 @@ __device__ void sweep_up(Desc d, ChainState& st, uint64_t& rng, double& sign, in
         }
         clk.lap(st, DQMC_PH_METROPOLIS);
         refresh_scalings(d, s);
-        for (int sp = 0; sp < d.nspin; ++sp) wg_bmul(d, 0, sp, d.Tmp + sp * n2, d.Gw + sp * n2, s);
+        for (int sp = lo; sp < hi; ++sp) wg_bmul(d, 0, sp, d.Tmp + sp * n2, d.Gw + sp * n2, s);
         clk.lap(st, DQMC_PH_PRODUCT);
         if (t % d.pace == 0 || t == L) {
             st.n_right -= 1;
             const int slot = st.n_right;
-            for (int sp = 0; sp < d.nspin; ++sp)
+            for (int sp = lo; sp < hi; ++sp)
                 wg_push(d, sp, slot, slot + 1 < d.n_stack ? slot + 1 : -1, d.Tmp + sp * n2, s);
             clk.lap(st, DQMC_PH_PUSH);
             stabilize_green(d, slot, slot > 0 ? slot - 1 : -1, true, st, s);
-            for (int sp = 0; sp < d.nspin; ++sp) wg_identity(d.Tmp + sp * n2, N);
+            for (int sp = lo; sp < hi; ++sp) wg_identity(d.Tmp + sp * n2, N);
             clk.lap(st, DQMC_PH_FRESH);
         }
         if (rec) record_slot(d, e, sign, true);
-        if (meas) wg_measure(d, sign, s);
+        if (meas) measure_slice(d, sign, s);
         clk.lap(st, DQMC_PH_MEASURE);
     }
     st.pos = L;
@@ -2857,19 +3001,20 @@ __device__ void sweep_down(Desc d, ChainState& st, uint64_t& rng, double& sign, 
     const int N = d.N, L = d.L;
     const size_t n2 = (size_t)N * N;
     const bool rec = flags & DQMC_FLAG_RECORD, meas = flags & DQMC_FLAG_MEASURE;
+    const int lo = uni(s.sp_lo), hi = uni(s.sp_hi);   // this workgroup's spins
     PhaseClock clk(flags & DQMC_FLAG_PROFILE);
-    for (int sp = 0; sp < d.nspin; ++sp) wg_identity(d.Tmp + sp * n2, N);
+    for (int sp = lo; sp < hi; ++sp) wg_identity(d.Tmp + sp * n2, N);
     double sign_top = sign;
     for (int t = L; t >= 1; --t) {
         const int e = t - 1;
         if (t % d.pace == 0 && t != L) {
             const int slot = st.n_right;
-            for (int sp = 0; sp < d.nspin; ++sp)
+            for (int sp = lo; sp < hi; ++sp)
                 wg_push(d, sp, slot, slot > 0 ? slot - 1 : -1, d.Tmp + sp * n2, s);
             st.n_right += 1;
             clk.lap(st, DQMC_PH_PUSH);
             stabilize_green(d, slot + 1 < d.n_stack ? slot + 1 : -1, slot, true, st, s);
-            for (int sp = 0; sp < d.nspin; ++sp) wg_identity(d.Tmp + sp * n2, N);
+            for (int sp = lo; sp < hi; ++sp) wg_identity(d.Tmp + sp * n2, N);
             clk.lap(st, DQMC_PH_FRESH);
         }
         load_field_row(d, e, s);
@@ -2882,22 +3027,22 @@ __device__ void sweep_down(Desc d, ChainState& st, uint64_t& rng, double& sign, 
         clk.lap(st, DQMC_PH_METROPOLIS);
         if (rec) record_slot(d, e, sign, true);
         if (t == L) sign_top = sign;
-        else if (meas) wg_measure(d, sign, s);
+        else if (meas) measure_slice(d, sign, s);
         clk.lap(st, DQMC_PH_MEASURE);
         refresh_scalings(d, s);
-        for (int sp = 0; sp < d.nspin; ++sp) wg_bmul(d, 4, sp, d.Tmp + sp * n2, d.Gw + sp * n2, s);
+        for (int sp = lo; sp < hi; ++sp) wg_bmul(d, 4, sp, d.Tmp + sp * n2, d.Gw + sp * n2, s);
         clk.lap(st, DQMC_PH_PRODUCT);
-        for (int sp = 0; sp < d.nspin; ++sp) wg_wrap(d, sp, false, d.G + sp * n2, d.Gw + sp * n2, s);
+        for (int sp = lo; sp < hi; ++sp) wg_wrap(d, sp, false, d.G + sp * n2, d.Gw + sp * n2, s);
         clk.lap(st, DQMC_PH_WRAP);
     }
     {
         const int slot = st.n_right;
-        for (int sp = 0; sp < d.nspin; ++sp) wg_push(d, sp, slot, slot > 0 ? slot - 1 : -1, d.Tmp + sp * n2, s);
+        for (int sp = lo; sp < hi; ++sp) wg_push(d, sp, slot, slot > 0 ? slot - 1 : -1, d.Tmp + sp * n2, s);
         st.n_right += 1;
         stabilize_green(d, -1, slot, false, st, s);
     }
     if (rec) record_slot(d, L - 1, sign, false);
-    if (meas) wg_measure(d, sign_top, s);
+    if (meas) measure_slice(d, sign_top, s);
     st.pos = 0;
 }
 
@@ -2965,7 +3110,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1)))
     ChainState st = *d.st;
     const int N = d.N, L = d.L;
     const size_t n2 = (size_t)N * N;
-    if (tid() == 0) { s.prof_on = 0; s.status = 0; }
+    if (tid() == 0) { s.prof_on = 0; s.status = 0; s.pair = 0; s.me = 0; s.sp_lo = 0; s.sp_hi = d.nspin; }
     __syncthreads();
     for (int sp = 0; sp < d.nspin; ++sp) wg_identity(d.Tmp + sp * n2, N);
     st.n_right = 0;
@@ -3010,9 +3155,21 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1)))
     }
 }
 
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) k_sweep(const ChainDesc* __restrict__ descs, int n_sweeps, int flags) {
+// Workgroup -> (chain, spin).  One workgroup per chain, or in a pair launch
+// (pair != 0, grid 16 x ceil(n_chains / 8)) chain 8 g + r and spin h at workgroup
+// 16 g + 8 h + r (r < 8): the two workgroups of a chain are 8 apart, so under the
+// round-robin dispatch over the 8 XCDs they share an XCD and its L2.  In a pair
+// launch only spin 0's workgroup writes the chain state.
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) k_sweep(const ChainDesc* __restrict__ descs, int n_sweeps, int flags,
+                                                                                        int n_chains, int pair) {
     Lds& s = g_lds;
-    Desc d = ((const AS_C ChainDesc*)descs)[blockIdx.x];
+    int c = blockIdx.x, me = 0;
+    if (pair) {
+        c = (blockIdx.x >> 4) * 8 + (blockIdx.x & 7);
+        me = (blockIdx.x >> 3) & 1;
+        if (c >= n_chains) return;
+    }
+    Desc d = ((const AS_C ChainDesc*)descs)[c];
     ChainState st = *d.st;
     uint64_t rng = st.rng;
     double sign = st.sign;
@@ -3023,17 +3180,28 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1)))
         s.prof[0] = s.prof[1] = 0;
         s.prof_on = (flags & DQMC_FLAG_PROFILE) ? 1 : 0;
         s.status = st.status;
+        s.pair = pair;
+        s.me = me;
+        s.sp_lo = pair ? me : 0;
+        s.sp_hi = pair ? me + 1 : d.nspin;
+        s.xseq = 0;
+        s.xbroken = 0;
+        // the time-displaced sweep runs in one workgroup per chain only (the host never asks otherwise)
+        if (pair && (flags & DQMC_FLAG_DYNAMIC)) s.status |= DQMC_STATUS_PAIR_SYNC;
     }
     __syncthreads();
     // a chain that met a fatal numerical failure sweeps no further (every thread reads the
-    // status after a barrier, so the whole workgroup leaves the loop together)
+    // status after a barrier, so the whole workgroup leaves the loop together; the partners of
+    // a pair launch merge their status bits at the end of every sweep, so they leave together)
     for (int k = 0; k < n_sweeps && !(s.status & DQMC_STATUS_FATAL); ++k) {
         if (st.pos == 0) {
             if (flags & DQMC_FLAG_DYNAMIC) sweep_dynamic(d, st, sign, flags, s);
             else sweep_up(d, st, rng, sign, flags, nacc, s);
         } else sweep_down(d, st, rng, sign, flags, nacc, s);
         __syncthreads();
+        if (pair) pair_merge_state(d, st, s);
     }
+    if (pair && me != 0) return;
     if (tid() == 0) {
         st.status = s.status;
         st.rng = rng;
@@ -3175,7 +3343,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1)))
                                              double* X, double* Y, UnitPtrs ext) {
     Lds& s = g_lds;
     Desc d = ((const AS_C ChainDesc*)descs)[0];
-    if (tid() == 0) { s.prof_on = 0; s.status = 0; }
+    if (tid() == 0) { s.prof_on = 0; s.status = 0; s.pair = 0; s.me = 0; s.sp_lo = 0; s.sp_hi = d.nspin; }
     __syncthreads();
     // a non-finite entry among the first n of M sets DQMC_STATUS_NONFINITE
     auto check_finite = [&](const double* M, int n) {
@@ -3294,7 +3462,30 @@ hipError_t dqmc_launch_init(const ChainDesc* d_descs, int n_chains, hipStream_t 
     return hipGetLastError();
 }
 hipError_t dqmc_launch_sweep(const ChainDesc* d_descs, int n_chains, int n_sweeps, int flags, hipStream_t stream) {
-    hipLaunchKernelGGL(dqmc::k_sweep, dim3(n_chains), dim3(NT), 0, stream, d_descs, n_sweeps, flags);
+    int pair = 0;
+    hipLaunchKernelGGL(dqmc::k_sweep, dim3(n_chains), dim3(NT), 0, stream, d_descs, n_sweeps, flags, n_chains, pair);
     return hipGetLastError();
+}
+// pair launch: the chains' exchange flags (n_chains x DQMC_PAIR_FLAG_WORDS, contiguous) zeroed on
+// the stream, then a cooperative launch of 16 x ceil(n_chains / 8) workgroups (all resident, or
+// the launch fails)
+hipError_t dqmc_launch_sweep_pair(const ChainDesc* d_descs, int n_chains, int n_sweeps, int flags, unsigned* xflags,
+                                  hipStream_t stream) {
+    hipError_t e = hipMemsetAsync(xflags, 0, sizeof(unsigned) * DQMC_PAIR_FLAG_WORDS * (size_t)n_chains, stream);
+    if (e != hipSuccess) return e;
+    int pair = 1;
+    void* args[] = {(void*)&d_descs, (void*)&n_sweeps, (void*)&flags, (void*)&n_chains, (void*)&pair};
+    return hipLaunchCooperativeKernel((const void*)dqmc::k_sweep, dim3(16 * ((n_chains + 7) / 8)), dim3(NT), args, 0,
+                                      stream);
+}
+// workgroups of k_sweep that can be resident at once on a device (the pair launch's limit)
+hipError_t dqmc_sweep_resident_limit(int device, int* limit) {
+    int ncu = 0, per_cu = 0;
+    hipError_t e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dqmc::k_sweep, NT, 0);
+    int coop = 0;
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device);
+    *limit = (e == hipSuccess && coop) ? ncu * per_cu : 0;
+    return e;
 }
 }

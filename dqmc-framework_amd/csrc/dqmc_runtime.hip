@@ -119,8 +119,8 @@ Layout chain_layout(int N, int L, int n_stack, int nspin, int kdelay, bool dense
     l.stkU = take((size_t)nspin * n_stack * n2);
     l.stkT = take((size_t)nspin * n_stack * n2);
     l.stkD = take((size_t)nspin * n_stack * N * sizeof(double));
-    l.W = take(3 * n2);
-    l.Wq = take((size_t)3 * 16 * (N + 16) * sizeof(double));
+    l.W = take((size_t)nspin * 3 * n2);   // one stabilization scratch per spin
+    l.Wq = take((size_t)nspin * 3 * 16 * (N + 16) * sizeof(double));
     l.expK = dense ? take(n2) : o;
     l.invExpK = dense ? take(n2) : o;
     l.cosq = take(N * sizeof(double));
@@ -137,6 +137,7 @@ Layout chain_layout(int N, int L, int n_stack, int nspin, int kdelay, bool dense
     l.recAcc = record ? take((size_t)L * N) : o;
     l.recGt0 = record ? take((size_t)L * nspin * n2) : o;
     l.recG0t = record ? take((size_t)L * nspin * n2) : o;
+    l.xbuf = nspin == 2 ? take(DQMC_PAIR_XBUF_DOUBLES * sizeof(double)) : o;   // spin-pair exchange blocks
     l.total = o;
     return l;
 }
@@ -190,7 +191,7 @@ const char* dqmc_context_error(dqmc_ctx_t ctx) { return ctx ? ctx->err.c_str() :
 
 int dqmc_batch_footprint(int n_chains, const dqmc_chain_params* params, int record, size_t* bytes) {
     if (n_chains < 1 || !params || !bytes) return DQMC_ERR_ARG;
-    size_t total = align256(sizeof(ChainDesc) * (size_t)n_chains);
+    size_t total = batch_head_bytes(n_chains);
     for (int c = 0; c < n_chains; ++c) {
         const dqmc_chain_params& p = params[c];
         std::string why;
@@ -217,7 +218,7 @@ int batch_prepare(dqmc_context* ctx, int n_chains, const dqmc_chain_params* para
     b->layouts.resize(n_chains);
     b->base.resize(n_chains);
     b->descs.resize(n_chains);
-    size_t off = align256(sizeof(ChainDesc) * n_chains);  // the descriptor array heads the arena
+    size_t off = batch_head_bytes(n_chains);  // the descriptor array and the pair flags head the arena
     for (int c = 0; c < n_chains; ++c) {
         HostChain& h = b->chains[c];
         h.p = params[c];
@@ -273,6 +274,7 @@ int batch_prepare(dqmc_context* ctx, int n_chains, const dqmc_chain_params* para
 int batch_bind(dqmc_batch* b) {
     dqmc_context* ctx = b->ctx;
     b->d_descs = (ChainDesc*)b->arena;
+    b->d_xflags = (unsigned*)(b->arena + align256(sizeof(ChainDesc) * (size_t)b->n));
     HIPCHK(ctx, hipMemsetAsync(b->arena, 0, b->arena_bytes, b->stream));
     for (int c = 0; c < b->n; ++c) {
         HostChain& h = b->chains[c];
@@ -331,6 +333,8 @@ int batch_bind(dqmc_batch* b) {
         d.rhoq = (const double*)(base + l.rhoq);
         d.recGt0 = b->record ? (double*)(base + l.recGt0) : nullptr;
         d.recG0t = b->record ? (double*)(base + l.recG0t) : nullptr;
+        d.xbuf = h.nspin == 2 ? (double*)(base + l.xbuf) : nullptr;
+        d.xflag = h.nspin == 2 ? b->d_xflags + (size_t)c * DQMC_PAIR_FLAG_WORDS : nullptr;
         if (!p.checkerboard) {
             HIPCHK(ctx, hipMemcpyAsync((void*)d.expK, h.expK.data(), h.expK.size() * 8, hipMemcpyHostToDevice, b->stream));
             HIPCHK(ctx, hipMemcpyAsync((void*)d.invExpK, h.invExpK.data(), h.invExpK.size() * 8, hipMemcpyHostToDevice, b->stream));
@@ -507,6 +511,39 @@ int dqmc_init(dqmc_batch_t b) {
     return DQMC_OK;
 }
 
+// a pair launch fits: every chain repulsive, and 16 x ceil(n / 8) sweep workgroups resident at once
+static bool pair_qualifies(dqmc_batch* b) {
+    for (const auto& h : b->chains)
+        if (h.nspin != 2) return false;
+    dqmc_context* ctx = b->ctx;
+    if (ctx->resident_limit < 0) {
+        int lim = 0;
+        if (dqmc_sweep_resident_limit(ctx->device, &lim) != hipSuccess) {
+            lim = 0;
+            (void)hipGetLastError();
+        }
+        ctx->resident_limit = lim;
+    }
+    return 16 * (((size_t)b->n + 7) / 8) <= (size_t)ctx->resident_limit;
+}
+
+int dqmc_batch_set_launch_mode(dqmc_batch_t b, int mode) {
+    if (!b || mode < DQMC_LAUNCH_AUTO || mode > DQMC_LAUNCH_PAIR) return DQMC_ERR_ARG;
+    HIPCHK(b->ctx, hipSetDevice(b->ctx->device));
+    if (mode == DQMC_LAUNCH_PAIR && !pair_qualifies(b)) {
+        b->ctx->err = "pair launch needs every chain repulsive and 16 x ceil(chains / 8) resident workgroups";
+        return DQMC_ERR_ARG;
+    }
+    b->launch_mode = mode;
+    return DQMC_OK;
+}
+
+int dqmc_batch_last_launch_mode(dqmc_batch_t b, int* mode) {
+    if (!b || !mode) return DQMC_ERR_ARG;
+    *mode = b->last_mode;
+    return DQMC_OK;
+}
+
 int dqmc_sweep(dqmc_batch_t b, int n_sweeps, int flags) {
     if (!b || n_sweeps < 0) return DQMC_ERR_ARG;
     if (!b->initialized) { b->ctx->err = "dqmc_sweep before dqmc_init"; return DQMC_ERR_STATE; }
@@ -517,7 +554,14 @@ int dqmc_sweep(dqmc_batch_t b, int n_sweeps, int flags) {
         HIPCHK(b->ctx, hipEventCreate(&b->ev0));
         HIPCHK(b->ctx, hipEventCreate(&b->ev1));
     }
-    if (b->use_graph && (b->graph_n != n_sweeps || b->graph_flags != flags)) {
+    // the time-displaced sweep runs one workgroup per chain; a pair launch is not replayed from a graph
+    const bool pair = b->launch_mode != DQMC_LAUNCH_SINGLE && !(flags & DQMC_SWEEP_DYNAMIC) && pair_qualifies(b);
+    if (pair) {
+        HIPCHK(b->ctx, hipEventRecord(b->ev0, b->stream));
+        HIPCHK(b->ctx, dqmc_launch_sweep_pair(b->d_descs, b->n, n_sweeps, flags, b->d_xflags, b->stream));
+        HIPCHK(b->ctx, hipEventRecord(b->ev1, b->stream));
+    }
+    if (!pair && b->use_graph && (b->graph_n != n_sweeps || b->graph_flags != flags)) {
         if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
         if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
         HIPCHK(b->ctx, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -528,10 +572,13 @@ int dqmc_sweep(dqmc_batch_t b, int n_sweeps, int flags) {
         b->graph_n = n_sweeps;
         b->graph_flags = flags;
     }
-    HIPCHK(b->ctx, hipEventRecord(b->ev0, b->stream));
-    if (b->use_graph) HIPCHK(b->ctx, hipGraphLaunch(b->graph_exec, b->stream));
-    else HIPCHK(b->ctx, dqmc_launch_sweep(b->d_descs, b->n, n_sweeps, flags, b->stream));
-    HIPCHK(b->ctx, hipEventRecord(b->ev1, b->stream));
+    if (!pair) {
+        HIPCHK(b->ctx, hipEventRecord(b->ev0, b->stream));
+        if (b->use_graph) HIPCHK(b->ctx, hipGraphLaunch(b->graph_exec, b->stream));
+        else HIPCHK(b->ctx, dqmc_launch_sweep(b->d_descs, b->n, n_sweeps, flags, b->stream));
+        HIPCHK(b->ctx, hipEventRecord(b->ev1, b->stream));
+    }
+    b->last_mode = pair ? DQMC_LAUNCH_PAIR : DQMC_LAUNCH_SINGLE;
     b->timed = true;
     for (auto& h : b->chains) {
         h.sweeps += n_sweeps;

@@ -15,6 +15,9 @@ extern "C" hipError_t dqmc_launch_init(const ChainDesc* d_descs, int n_chains, h
 extern "C" hipError_t dqmc_launch_measure_recorded(const ChainDesc* d_descs, int n_chains, hipStream_t stream);
 extern "C" hipError_t dqmc_launch_sweep(const ChainDesc* d_descs, int n_chains, int n_sweeps, int flags,
                                         hipStream_t stream);
+extern "C" hipError_t dqmc_launch_sweep_pair(const ChainDesc* d_descs, int n_chains, int n_sweeps, int flags,
+                                             unsigned* xflags, hipStream_t stream);
+extern "C" hipError_t dqmc_sweep_resident_limit(int device, int* limit);
 extern "C" hipError_t dqmc_launch_unit_ext(const ChainDesc* desc, ChainDesc* d_desc, int unit, int a, int b, int c,
                                            double* X, double* Y, const DqmcUnitPtrs* ptrs, hipStream_t stream);
 
@@ -22,6 +25,7 @@ struct dqmc_context {
     int device = 0;
     std::string err;
     double* noalloc_scratch = nullptr;   // the device's address of the no-allocation entry point's scratch
+    int resident_limit = -1;             // sweep workgroups resident at once (0: no cooperative launch); -1 unknown
 };
 
 struct HostChain {
@@ -39,7 +43,7 @@ struct HostChain {
 
 struct Layout {
     size_t G, Gw, Tmp, stkU, stkT, stkD, W, Wq, expK, invExpK, cosq, field, st, acc, Gt0, G0t, dacc, dscr, rhoq, recG,
-        recSign, recAcc, recGt0, recG0t, total;
+        recSign, recAcc, recGt0, recG0t, xbuf, total;
 };
 
 struct dqmc_batch {
@@ -63,6 +67,10 @@ struct dqmc_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_n = -1, graph_flags = -1;
+    // chain -> workgroup mapping of the sweep launches (DQMC_LAUNCH_*), the one the last launch used,
+    // and the spin-pair exchange flags (n x DQMC_PAIR_FLAG_WORDS, after the descriptor array)
+    int launch_mode = DQMC_LAUNCH_AUTO, last_mode = 0;
+    unsigned* d_xflags = nullptr;
 };
 
 // scratch of the building blocks on caller-provided device buffers: a one-chain batch (host
@@ -82,6 +90,11 @@ struct dqmc_workspace {
     } while (0)
 
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+// head of a batch arena: the descriptor array, then the spin-pair flags of every chain
+inline size_t batch_head_bytes(int n_chains) {
+    return align256(sizeof(ChainDesc) * (size_t)n_chains) +
+           align256(sizeof(unsigned) * DQMC_PAIR_FLAG_WORDS * (size_t)n_chains);
+}
 
 // minstd_rand0 (std::default_random_engine of libstdc++) and the canonical double of
 // std::bernoulli_distribution (C++ standard [rand.util.canonical], two engine draws per

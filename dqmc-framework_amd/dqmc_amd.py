@@ -15,8 +15,9 @@ LIB_PATH = os.path.join(HERE, "lib", "libdqmc_amd.so")
 OK, ERR_ARG, ERR_HIP, ERR_MEMORY, ERR_STATE, ERR_NUMERIC = 0, -1, -2, -3, -4, -5
 REPULSIVE, ATTRACTIVE = 0, 1
 PREC_FP64, PREC_MIXED, PREC_FP32_WRAP, PREC_FP32_PRODUCT = 0, 1, 2, 3
-STATUS_NONFINITE, STATUS_SINGULAR, STATUS_WRAP_LIMIT = 1, 2, 4
-STATUS_FATAL = STATUS_NONFINITE | STATUS_SINGULAR
+STATUS_NONFINITE, STATUS_SINGULAR, STATUS_WRAP_LIMIT, STATUS_PAIR_SYNC = 1, 2, 4, 8
+STATUS_FATAL = STATUS_NONFINITE | STATUS_SINGULAR | STATUS_PAIR_SYNC
+LAUNCH_AUTO, LAUNCH_SINGLE, LAUNCH_PAIR = 0, 1, 2
 DEFAULT_WRAP_LIMIT = 1e-3
 SWEEP_MEASURE, SWEEP_RECORD, SWEEP_FROZEN, SWEEP_PROFILE, SWEEP_DYNAMIC = 1, 2, 4, 8, 16
 PHASES = ["wrap", "metropolis", "product", "push", "fresh", "measure", "walk", "panels"]
@@ -25,7 +26,8 @@ OBS_NAMES = ["sign", "filling_number", "double_occupancy", "kinetic_energy", "mo
              "s_wave_pairing_corr"]
 
 EXPORTS = ["dqmc_context_create", "dqmc_context_destroy", "dqmc_context_error", "dqmc_batch_footprint",
-           "dqmc_batch_create", "dqmc_batch_destroy", "dqmc_batch_size", "dqmc_batch_use_graph", "dqmc_fields_random", "dqmc_set_fields",
+           "dqmc_batch_create", "dqmc_batch_destroy", "dqmc_batch_size", "dqmc_batch_use_graph",
+           "dqmc_batch_set_launch_mode", "dqmc_batch_last_launch_mode", "dqmc_fields_random", "dqmc_set_fields",
            "dqmc_get_fields", "dqmc_set_rng_state", "dqmc_set_expk", "dqmc_get_expk", "dqmc_init", "dqmc_sweep",
            "dqmc_set_wrap_error_limit",
            "dqmc_synchronize", "dqmc_last_sweep_ms", "dqmc_get_green", "dqmc_get_slice_green", "dqmc_get_slice_signs",
@@ -103,6 +105,8 @@ def lib():
                   "dqmc_clear_measure", "dqmc_measure_recorded"]:
             getattr(L, n).argtypes = [vp]
         L.dqmc_batch_use_graph.argtypes = [vp, ip]
+        L.dqmc_batch_set_launch_mode.argtypes = [vp, ip]
+        L.dqmc_batch_last_launch_mode.argtypes = [vp, C.POINTER(C.c_int)]
         L.dqmc_set_fields.argtypes = [vp, ip, vp]
         L.dqmc_get_fields.argtypes = [vp, ip, vp]
         L.dqmc_set_rng_state.argtypes = [vp, ip, C.c_ulonglong]
@@ -313,6 +317,16 @@ class Batch:
 
     def use_graph(self, enable=True):
         self.ctx.check(lib().dqmc_batch_use_graph(self.h, int(enable)), "dqmc_batch_use_graph")
+
+    def set_launch_mode(self, mode):
+        """LAUNCH_AUTO (default), LAUNCH_SINGLE (one workgroup per chain) or LAUNCH_PAIR (one per
+        spin of every repulsive chain); raises when the batch does not qualify for LAUNCH_PAIR."""
+        self.ctx.check(lib().dqmc_batch_set_launch_mode(self.h, int(mode)), "dqmc_batch_set_launch_mode")
+
+    def last_launch_mode(self):
+        m = C.c_int(0)
+        self.ctx.check(lib().dqmc_batch_last_launch_mode(self.h, C.byref(m)), "dqmc_batch_last_launch_mode")
+        return m.value
 
     def fields_random(self):
         self.ctx.check(lib().dqmc_fields_random(self.h), "dqmc_fields_random")

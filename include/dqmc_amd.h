@@ -54,7 +54,8 @@ extern "C" {
 #define DQMC_STATUS_NONFINITE 1   /* a recomputed or wrapped G held inf or NaN (fatal) */
 #define DQMC_STATUS_SINGULAR 2    /* an LU pivot of the recompute was zero, subnormal or non-finite (fatal) */
 #define DQMC_STATUS_WRAP_LIMIT 4  /* the wrap error exceeded the batch's limit (dqmc_set_wrap_error_limit) */
-#define DQMC_STATUS_FATAL (DQMC_STATUS_NONFINITE | DQMC_STATUS_SINGULAR)
+#define DQMC_STATUS_PAIR_SYNC 8   /* a spin-pair launch's partner workgroup did not answer within 2 s (fatal) */
+#define DQMC_STATUS_FATAL (DQMC_STATUS_NONFINITE | DQMC_STATUS_SINGULAR | DQMC_STATUS_PAIR_SYNC)
 #define DQMC_DEFAULT_WRAP_LIMIT 1e-3
 
 /* models (Model::RepulsiveHubbard / Model::AttractiveHubbard) */
@@ -157,6 +158,26 @@ int dqmc_batch_size(dqmc_batch_t b);
 /* replay the persistent sweep launch from a captured HIP graph (re-captured
  * when n_sweeps or flags change); no counterpart in the reference */
 int dqmc_batch_use_graph(dqmc_batch_t b, int enable);
+
+/* How a sweep launch maps chains onto workgroups; no counterpart in the reference (its
+ * walker runs one chain per process, src/dqmc_walker.cpp:212-216 runs the two spins one
+ * after the other).
+ *   DQMC_LAUNCH_SINGLE  one workgroup per chain (both spins in it)
+ *   DQMC_LAUNCH_PAIR    two co-resident workgroups per repulsive chain, one per spin: wraps,
+ *                       slice products, stack pushes and recomputes of the two spins run side
+ *                       by side; the Metropolis walk runs in both, exchanging the k x k block of
+ *                       each spin per delayed-update block.  Needs every chain repulsive and
+ *                       2 x (chains rounded up to 8) workgroups resident at once (cooperative
+ *                       launch); time-displaced sweeps always run SINGLE.  Not replayed from a
+ *                       graph.  Results equal SINGLE's to rounding (the same decisions).
+ *   DQMC_LAUNCH_AUTO    PAIR when the batch qualifies, else SINGLE (the default) */
+#define DQMC_LAUNCH_AUTO 0
+#define DQMC_LAUNCH_SINGLE 1
+#define DQMC_LAUNCH_PAIR 2
+/* DQMC_ERR_ARG when PAIR is asked of a batch that does not qualify */
+int dqmc_batch_set_launch_mode(dqmc_batch_t b, int mode);
+/* the mapping the last sweep launch used (DQMC_LAUNCH_SINGLE or DQMC_LAUNCH_PAIR; 0 before any) */
+int dqmc_batch_last_launch_mode(dqmc_batch_t b, int* mode);
 
 /* fields: random (seed, reference draw order), or explicit */
 int dqmc_fields_random(dqmc_batch_t b);

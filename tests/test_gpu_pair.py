@@ -1,0 +1,171 @@
+"""Spin-pair launches (DQMC_LAUNCH_PAIR): each repulsive chain on two co-resident workgroups,
+one per spin, exchanging the k x k block of the walk per delayed-update block.
+
+A pair launch computes the same numbers as the one-workgroup launch: every spin's next
+G0[S,S] is formed from the block's factors in both, each 16x16 block of an update is one
+wavefront's product whichever wavefronts run it, and the walk gets identical inputs in both
+workgroups.  So the tests demand bit-for-bit equality with DQMC_LAUNCH_SINGLE (fields, accept
+bits, random state, sign, G of every recorded slot, measurement sums, wrap error), and the
+one-workgroup launch is itself pinned to the oracle by the parity tests.  Shapes: dense and
+checkerboard, the headline 12x12 chains, the 16x16 precision-workload chains, an uneven
+batch of four sizes, a chain count that is not a multiple of 8 (idle padding workgroups),
+and a failing chain beside healthy ones.  The reference runs the two spins one after the
+other in one process (src/dqmc_walker.cpp:212-216).
+"""
+import numpy as np
+import pytest
+
+import _oracle as O
+import dqmc_amd as D
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return D.Context(0)
+
+
+def run(ctx, ps, mode, sweeps, measure_sweeps=2, record=True, expk=None):
+    b = D.Batch(ctx, ps, record=record)
+    b.set_launch_mode(mode)
+    b.fields_random()
+    if expk is not None:
+        for c, (e, ie) in expk.items():
+            b.set_expk(c, e, ie)
+    b.init()
+    if sweeps:
+        b.sweep(sweeps)
+    if measure_sweeps:
+        b.sweep(measure_sweeps, measure=True, record=record)
+    b.synchronize()
+    return b
+
+
+def assert_same(a, b, n, record=True):
+    for c in range(n):
+        ia, ib = a.info(c), b.info(c)
+        assert ia.rng == ib.rng and ia.sign == ib.sign and ia.pos == ib.pos, c
+        assert ia.status == ib.status and ia.wrap_error == ib.wrap_error, (c, ia.wrap_error, ib.wrap_error)
+        assert np.array_equal(a.fields(c), b.fields(c)), c
+        for spin in (1, -1):
+            assert np.array_equal(a.green(c, spin), b.green(c, spin), equal_nan=True), (c, spin)
+        sa, ca = a.measure_accum(c)
+        sb, cb = b.measure_accum(c)
+        assert np.array_equal(sa, sb) and np.array_equal(ca, cb), c
+        if record:
+            assert np.array_equal(a.accepts(c), b.accepts(c)), c
+            assert np.array_equal(a.slice_signs(c), b.slice_signs(c)), c
+            for slot in (0, 1, a.L[c] - 1):
+                for spin in (1, -1):
+                    assert np.array_equal(a.slice_green(c, slot, spin), b.slice_green(c, slot, spin), equal_nan=True), (c, slot, spin)
+
+
+CASES = {
+    "6x6-dense": [D.make_params(6, 40, 4.0, pace=8, U=4.0, mu=-0.3, delay=8, seed=s) for s in range(8)],
+    "12x12-checkerboard": [D.make_params(12, 60, 6.0, pace=10, checkerboard=True, delay=32, U=4.0, mu=0.0, seed=s)
+                           for s in range(8)],
+    "uneven-13": [D.make_params(side, 24 + 8 * (c % 3), 3.0, pace=5 + c % 3, U=5.0, mu=-0.2 * (c % 4), delay=(8, 16, 32)[c % 3],
+                                checkerboard=(side % 2 == 0 and c % 2 == 1), seed=100 + c)
+                  for c, side in enumerate([4, 6, 8, 10, 4, 6, 8, 10, 5, 7, 9, 3, 6])],
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_pair_equals_single_bit_for_bit(ctx, name):
+    ps = CASES[name]
+    print(f"[gpu-pair] {name}: {len(ps)} chains, pair vs single", flush=True)
+    single = run(ctx, ps, D.LAUNCH_SINGLE, 3)
+    pair = run(ctx, ps, D.LAUNCH_PAIR, 3)
+    assert single.last_launch_mode() == D.LAUNCH_SINGLE and pair.last_launch_mode() == D.LAUNCH_PAIR
+    assert_same(single, pair, len(ps))
+    print(f"[gpu-pair] {name}: identical; wrap error max {max(pair.info(c).wrap_error for c in range(len(ps))):.2e}",
+          flush=True)
+    single.close(); pair.close()
+
+
+def test_pair_16x16_precision_workload_chains(ctx):
+    """Eight chains of the 16x16 workload (U 6, beta 10, L 200, pace 8, dense, delay 32, four mu)
+    over one sweep each way: identical in both launch forms, and AUTO picks the pair form."""
+    print("[gpu-pair] 16x16 precision-workload chains", flush=True)
+    mus = [0.0, -0.2, -0.4, -0.6]
+    ps = [D.make_params(16, 200, 10.0, pace=8, U=6.0, mu=mus[c % 4], delay=32, seed=c) for c in range(8)]
+    single = run(ctx, ps, D.LAUNCH_SINGLE, 1, measure_sweeps=1)
+    auto = run(ctx, ps, D.LAUNCH_AUTO, 1, measure_sweeps=1)
+    assert auto.last_launch_mode() == D.LAUNCH_PAIR
+    assert_same(single, auto, len(ps))
+    single.close(); auto.close()
+
+
+def test_pair_chain_matches_the_oracle(ctx):
+    """One 6x6 chain in a pair launch against the oracle over a forward and a backward sweep."""
+    print("[gpu-pair] 6x6 pair launch vs oracle", flush=True)
+    p = D.make_params(6, 40, 4.0, pace=8, U=4.0, mu=-0.3, delay=8, seed=5)
+    b = D.Batch(ctx, [p], record=True)
+    b.set_launch_mode(D.LAUNCH_PAIR)
+    b.fields_random()
+    orc = O.Chain(6, 40, 4.0, pace=8, U=4.0, mu=-0.3)
+    orc.seed(5); orc.fields_random()
+    orc.set_expK(b.expk(0)[0], b.expk(0)[1])
+    b.init(); orc.init()
+    for up in (True, False):
+        b.sweep(1, record=True)
+        orc.sweep_forward() if up else orc.sweep_backward()
+        assert b.last_launch_mode() == D.LAUNCH_PAIR
+        assert np.array_equal(b.fields(0).astype(np.float64), orc.fields())
+        assert np.array_equal(b.accepts(0), orc.accepts())
+        assert b.info(0).sign == orc.sign and b.info(0).rng == orc.rng_state
+        for spin in (1, -1):
+            assert np.max(np.abs(b.green(0, spin) - orc.green(spin))) < 1e-10
+    b.close()
+
+
+def test_failing_chain_in_a_pair_launch(ctx):
+    """An overflowing exp(-dtau K) fails its chain at init (DQMC_STATUS_NONFINITE): neither of its
+    workgroups sweeps or waits, and the healthy chains are the chains the one-workgroup launch gives."""
+    print("[gpu-pair] failing chain beside healthy ones", flush=True)
+    side = 4
+    ps = [D.make_params(side, 20, 2.0, pace=5, U=4.0, seed=s) for s in (3, 11, 12)]
+    huge = np.full((side * side, side * side), 1e200, order="F")
+    single = run(ctx, ps, D.LAUNCH_SINGLE, 2, expk={0: (huge, huge)})
+    pair = run(ctx, ps, D.LAUNCH_PAIR, 2, expk={0: (huge, huge)})
+    assert pair.info(0).status & D.STATUS_FATAL and pair.info(0).pos == 0
+    assert not pair.info(0).status & D.STATUS_PAIR_SYNC
+    for c in (1, 2):
+        assert pair.info(c).status == 0
+    assert_same(single, pair, 3)
+    single.close(); pair.close()
+
+
+def test_launch_mode_rules(ctx):
+    """PAIR needs every chain repulsive and the pair grid resident at once; the time-displaced
+    sweep falls back to one workgroup per chain."""
+    print("[gpu-pair] launch-mode rules", flush=True)
+    att = D.Batch(ctx, [D.make_params(4, 16, 2.0, pace=4, model=D.ATTRACTIVE, U=4.0, seed=1)])
+    with pytest.raises(D.DqmcError):
+        att.set_launch_mode(D.LAUNCH_PAIR)
+    att.fields_random(); att.init(); att.sweep(1); att.synchronize()
+    assert att.last_launch_mode() == D.LAUNCH_SINGLE
+    att.close()
+    many = D.Batch(ctx, [D.make_params(4, 8, 1.0, pace=4, U=4.0, seed=s) for s in range(256)])
+    with pytest.raises(D.DqmcError):
+        many.set_launch_mode(D.LAUNCH_PAIR)
+    many.fields_random(); many.init(); many.sweep(1); many.synchronize()
+    assert many.last_launch_mode() == D.LAUNCH_SINGLE   # AUTO: 512 workgroups do not fit
+    many.close()
+    ps = [D.make_params(6, 24, 3.0, pace=6, U=4.0, mu=-0.2, delay=8, seed=s) for s in range(3)]
+    out = {}
+    for mode in (D.LAUNCH_SINGLE, D.LAUNCH_PAIR):
+        b = D.Batch(ctx, ps)
+        b.set_launch_mode(mode)
+        b.fields_random(); b.init()
+        b.sweep(2)
+        b.sweep(2, measure=True, dynamic=True)
+        b.synchronize()
+        assert b.last_launch_mode() == D.LAUNCH_SINGLE   # the time-displaced sweeps ran unpaired
+        out[mode] = b
+    assert_same(out[D.LAUNCH_SINGLE], out[D.LAUNCH_PAIR], 3, record=False)
+    for c in range(3):
+        assert np.array_equal(out[D.LAUNCH_SINGLE].dynamic_accum(c)["gk"], out[D.LAUNCH_PAIR].dynamic_accum(c)["gk"])
+    for b in out.values():
+        b.close()
