@@ -113,10 +113,14 @@ def autotune(w, candidates=(8, 16, 32)):
     return best, ms
 
 
-def run_gpu(w, steps, warmup, chains=None, graph=True):
+LAUNCH_MODES = {"auto": D.LAUNCH_AUTO, "single": D.LAUNCH_SINGLE, "pair": D.LAUNCH_PAIR}
+
+
+def run_gpu(w, steps, warmup, chains=None, graph=True, launch="auto"):
     ctx = D.Context(0)
     b = D.Batch(ctx, chain_params(w, chains))
-    b.use_graph(graph)   # the sweep launch replayed from a captured HIP graph
+    b.use_graph(graph)   # the one-workgroup-per-chain launch replayed from a captured HIP graph
+    b.set_launch_mode(LAUNCH_MODES[launch])   # AUTO: spin pairs when the batch leaves CUs idle
     b.fields_random()
     b.init()
     for _ in range(warmup):
@@ -135,16 +139,21 @@ def run_gpu(w, steps, warmup, chains=None, graph=True):
     acc_per_chain_sweep = acc0 / (min(n, 8) * steps)
     wrap_err = max(b.info(c).wrap_error for c in range(n))
     statuses = [b.info(c).status for c in range(n)]
+    pair = b.last_launch_mode() == D.LAUNCH_PAIR
     b.close()
     ctx.close()
+    mode = ("spin pairs (two workgroups per chain)" if pair else
+            "one workgroup per chain, " + ("hipGraph replay" if graph else "direct"))
     return dict(wall=wall, kernel_ms=kernel_ms, chains=n, accepted=acc_per_chain_sweep, wrap_error=wrap_err,
-                status=max(statuses))
+                status=max(statuses), launch=mode)
 
 
-def phase_breakdown(w, chains, sweeps=2):
-    """Device time per sweep phase (ms per chain-sweep), from the engine's phase clock."""
+def phase_breakdown(w, chains, sweeps=2, launch="auto"):
+    """Device time per sweep phase (ms per chain-sweep), from the engine's phase clock (in a
+    spin-pair launch, the clock of spin 0's workgroup)."""
     ctx = D.Context(0)
     b = D.Batch(ctx, chain_params(w, chains))
+    b.set_launch_mode(LAUNCH_MODES[launch])
     b.fields_random()
     b.init()
     b.sweep(1)
@@ -302,6 +311,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the single-chain comparison run")
     ap.add_argument("--phases", action="store_true", help="also report device time per sweep phase")
+    ap.add_argument("--launch", default="auto", choices=sorted(LAUNCH_MODES),
+                    help="chain -> workgroup mapping: auto (spin pairs when the batch leaves CUs idle), single, pair")
     ap.add_argument("--precision", choices=["workload", "fp64", "mixed", "fp32_wrap", "fp32_product"],
                     default="workload", help="override the workload's precision mode")
     ap.add_argument("--dynamic-steps", type=int, default=3,
@@ -340,7 +351,7 @@ def main():
         w["delay"] = best
         print(f"[bench] delay depth autotuned: {best} (ms per 2 sweeps {tune})", file=sys.stderr, flush=True)
     print(f"[bench] {w['desc']}: warmup {args.warmup}, steps {args.steps}", file=sys.stderr, flush=True)
-    r = run_gpu(w, args.steps, args.warmup, graph=not args.no_graph)
+    r = run_gpu(w, args.steps, args.warmup, graph=not args.no_graph, launch=args.launch)
     chain_sweeps = r["chains"] * args.steps
     value = chain_sweeps / r["wall"]
     ms_per_step = 1000.0 * r["wall"] / args.steps
@@ -359,7 +370,7 @@ def main():
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "kernel": "k_sweep", "kernel_ms": kms, "algorithmic_gflop_per_launch": flops_launch / 1e9},
         "accepted_per_chain_sweep": r["accepted"], "max_wrap_error": r["wrap_error"],
-        "launch": "hipGraph replay" if not args.no_graph else "direct",
+        "launch": r["launch"],
     }
     if tune is not None:
         out["config"]["delay_autotune_ms"] = tune
@@ -368,14 +379,14 @@ def main():
         out["roofline"]["traffic_tbs_at_kernel_ms"] = traffic / (kms * 1e-3) / 1e12
     if not args.no_plain and r["chains"] > 1:
         print("[bench] plain path: the same chain alone, one chain per launch", file=sys.stderr, flush=True)
-        p = run_gpu(w, max(2, args.steps // 4), 1, chains=1)
+        p = run_gpu(w, max(2, args.steps // 4), 1, chains=1, launch=args.launch)
         plain = p["chains"] * max(2, args.steps // 4) / p["wall"]
         out["plain_chain_sweeps_per_s"] = plain
         out["batched_over_plain"] = value / plain
     if args.phases:
         print("[bench] phase breakdown", file=sys.stderr, flush=True)
-        out["phases_ms_per_chain_sweep"] = {"batched": phase_breakdown(w, r["chains"]),
-                                            "single": phase_breakdown(w, 1)}
+        out["phases_ms_per_chain_sweep"] = {"batched": phase_breakdown(w, r["chains"], launch=args.launch),
+                                            "single": phase_breakdown(w, 1, launch=args.launch)}
         out["phase_rooflines"] = phase_rooflines(w, out["phases_ms_per_chain_sweep"]["batched"], r["chains"],
                                                  r["accepted"])
     if w.get("precision", D.PREC_FP64) != D.PREC_FP64:

@@ -2322,7 +2322,8 @@ __device__ __forceinline__ WalkOut walk_block(Desc d, int e, int b0, int k, uint
 
 // ---------------------------------------------------------------------------
 // Spin pairs.  In a pair launch a repulsive chain runs on two workgroups, one
-// per spin, launched cooperatively so that both are resident.  The spins are
+// per spin; the launch is no larger than the device holds at once and pair
+// launches never overlap each other, so the partners are resident together.  The spins are
 // independent except in the walk (the acceptance ratio is the product of the
 // two spins' ratios), so per delayed-update block each workgroup publishes its
 // spin's G0[S,S] and takes the partner's; both then walk the block with
@@ -2333,14 +2334,15 @@ __device__ __forceinline__ WalkOut walk_block(Desc d, int e, int b0, int k, uint
 // An exchange point: every wavefront's stores reach the L2 (vmcnt(0)), thread
 // 0 releases the running exchange number on this spin's flag (agent scope, so
 // the partner may sit on another XCD) and acquires the partner's flag.  The wait
-// is bounded by the 100 MHz wall clock: a partner silent for 2 s sets
+// is bounded by the 100 MHz wall clock: a partner silent for 30 s (e.g. kept
+// off the device by another process's long kernel) sets
 // DQMC_STATUS_PAIR_SYNC (fatal) and later waits are skipped, so the kernel always
 // ends.  The flags are zeroed by the host before each pair launch and compared
 // by signed difference.  Data published at exchange q lives in buffer slot
 // q & 1: a workgroup writes slot q & 1 again only at exchange q + 2, after the
 // partner has arrived at q + 1, which it does only after reading slot q & 1.
 // ---------------------------------------------------------------------------
-#define PAIR_TIMEOUT_TICKS 200000000ull   // 2 s of the 100 MHz wall clock
+#define PAIR_TIMEOUT_TICKS 3000000000ull   // 30 s of the 100 MHz wall clock
 
 __device__ __forceinline__ int pair_slot() { return (int)((g_lds.xseq + 1u) & 1u); }   // slot of the next exchange
 
@@ -3466,26 +3468,21 @@ hipError_t dqmc_launch_sweep(const ChainDesc* d_descs, int n_chains, int n_sweep
     hipLaunchKernelGGL(dqmc::k_sweep, dim3(n_chains), dim3(NT), 0, stream, d_descs, n_sweeps, flags, n_chains, pair);
     return hipGetLastError();
 }
-// pair launch: the chains' exchange flags (n_chains x DQMC_PAIR_FLAG_WORDS, contiguous) zeroed on
-// the stream, then a cooperative launch of 16 x ceil(n_chains / 8) workgroups (all resident, or
-// the launch fails)
-hipError_t dqmc_launch_sweep_pair(const ChainDesc* d_descs, int n_chains, int n_sweeps, int flags, unsigned* xflags,
-                                  hipStream_t stream) {
-    hipError_t e = hipMemsetAsync(xflags, 0, sizeof(unsigned) * DQMC_PAIR_FLAG_WORDS * (size_t)n_chains, stream);
-    if (e != hipSuccess) return e;
+// pair launch: 16 x ceil(n_chains / 8) workgroups, no more than the device holds at once (the
+// caller checks), so all of them are resident together once earlier kernels have drained; the
+// chains' exchange flags must be zero (the caller clears them on the stream)
+hipError_t dqmc_launch_sweep_pair(const ChainDesc* d_descs, int n_chains, int n_sweeps, int flags, hipStream_t stream) {
     int pair = 1;
-    void* args[] = {(void*)&d_descs, (void*)&n_sweeps, (void*)&flags, (void*)&n_chains, (void*)&pair};
-    return hipLaunchCooperativeKernel((const void*)dqmc::k_sweep, dim3(16 * ((n_chains + 7) / 8)), dim3(NT), args, 0,
-                                      stream);
+    hipLaunchKernelGGL(dqmc::k_sweep, dim3(16 * ((n_chains + 7) / 8)), dim3(NT), 0, stream, d_descs, n_sweeps, flags,
+                       n_chains, pair);
+    return hipGetLastError();
 }
 // workgroups of k_sweep that can be resident at once on a device (the pair launch's limit)
 hipError_t dqmc_sweep_resident_limit(int device, int* limit) {
     int ncu = 0, per_cu = 0;
     hipError_t e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
     if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dqmc::k_sweep, NT, 0);
-    int coop = 0;
-    if (e == hipSuccess) e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device);
-    *limit = (e == hipSuccess && coop) ? ncu * per_cu : 0;
+    *limit = e == hipSuccess ? ncu * per_cu : 0;
     return e;
 }
 }

@@ -511,6 +511,12 @@ int dqmc_init(dqmc_batch_t b) {
     return DQMC_OK;
 }
 
+// Pair launches of a device never overlap one another (any batch or context of the process): each
+// waits on its stream for the one before, so two of them can never hold parts of the device while
+// waiting for partner workgroups the other keeps off it.  One event per device, kept to the end.
+static std::mutex g_pair_mu;
+static std::map<int, hipEvent_t> g_pair_done;
+
 // a pair launch fits: every chain repulsive, and 16 x ceil(n / 8) sweep workgroups resident at once
 static bool pair_qualifies(dqmc_batch* b) {
     for (const auto& h : b->chains)
@@ -557,9 +563,20 @@ int dqmc_sweep(dqmc_batch_t b, int n_sweeps, int flags) {
     // the time-displaced sweep runs one workgroup per chain; a pair launch is not replayed from a graph
     const bool pair = b->launch_mode != DQMC_LAUNCH_SINGLE && !(flags & DQMC_SWEEP_DYNAMIC) && pair_qualifies(b);
     if (pair) {
+        std::lock_guard<std::mutex> lk(g_pair_mu);
+        auto it = g_pair_done.find(b->ctx->device);
+        if (it == g_pair_done.end()) {
+            hipEvent_t e = nullptr;
+            HIPCHK(b->ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            it = g_pair_done.emplace(b->ctx->device, e).first;
+        } else {
+            HIPCHK(b->ctx, hipStreamWaitEvent(b->stream, it->second, 0));
+        }
+        HIPCHK(b->ctx, hipMemsetAsync(b->d_xflags, 0, sizeof(unsigned) * DQMC_PAIR_FLAG_WORDS * (size_t)b->n, b->stream));
         HIPCHK(b->ctx, hipEventRecord(b->ev0, b->stream));
-        HIPCHK(b->ctx, dqmc_launch_sweep_pair(b->d_descs, b->n, n_sweeps, flags, b->d_xflags, b->stream));
+        HIPCHK(b->ctx, dqmc_launch_sweep_pair(b->d_descs, b->n, n_sweeps, flags, b->stream));
         HIPCHK(b->ctx, hipEventRecord(b->ev1, b->stream));
+        HIPCHK(b->ctx, hipEventRecord(it->second, b->stream));
     }
     if (!pair && b->use_graph && (b->graph_n != n_sweeps || b->graph_flags != flags)) {
         if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }

@@ -16,7 +16,7 @@ extern "C" hipError_t dqmc_launch_measure_recorded(const ChainDesc* d_descs, int
 extern "C" hipError_t dqmc_launch_sweep(const ChainDesc* d_descs, int n_chains, int n_sweeps, int flags,
                                         hipStream_t stream);
 extern "C" hipError_t dqmc_launch_sweep_pair(const ChainDesc* d_descs, int n_chains, int n_sweeps, int flags,
-                                             unsigned* xflags, hipStream_t stream);
+                                             hipStream_t stream);
 extern "C" hipError_t dqmc_sweep_resident_limit(int device, int* limit);
 extern "C" hipError_t dqmc_launch_unit_ext(const ChainDesc* desc, ChainDesc* d_desc, int unit, int a, int b, int c,
                                            double* X, double* Y, const DqmcUnitPtrs* ptrs, hipStream_t stream);

@@ -54,7 +54,7 @@ extern "C" {
 #define DQMC_STATUS_NONFINITE 1   /* a recomputed or wrapped G held inf or NaN (fatal) */
 #define DQMC_STATUS_SINGULAR 2    /* an LU pivot of the recompute was zero, subnormal or non-finite (fatal) */
 #define DQMC_STATUS_WRAP_LIMIT 4  /* the wrap error exceeded the batch's limit (dqmc_set_wrap_error_limit) */
-#define DQMC_STATUS_PAIR_SYNC 8   /* a spin-pair launch's partner workgroup did not answer within 2 s (fatal) */
+#define DQMC_STATUS_PAIR_SYNC 8   /* a spin-pair launch's partner workgroup did not answer within 30 s (fatal) */
 #define DQMC_STATUS_FATAL (DQMC_STATUS_NONFINITE | DQMC_STATUS_SINGULAR | DQMC_STATUS_PAIR_SYNC)
 #define DQMC_DEFAULT_WRAP_LIMIT 1e-3
 
@@ -167,9 +167,12 @@ int dqmc_batch_use_graph(dqmc_batch_t b, int enable);
  *                       slice products, stack pushes and recomputes of the two spins run side
  *                       by side; the Metropolis walk runs in both, exchanging the k x k block of
  *                       each spin per delayed-update block.  Needs every chain repulsive and
- *                       2 x (chains rounded up to 8) workgroups resident at once (cooperative
- *                       launch); time-displaced sweeps always run SINGLE.  Not replayed from a
- *                       graph.  Results equal SINGLE's to rounding (the same decisions).
+ *                       2 x (chains rounded up to 8) workgroups resident at once; the pair
+ *                       launches of a device run one after another (any batch of the process),
+ *                       and a partner kept off the device for 30 s (by another process's
+ *                       kernels) fails its chain with DQMC_STATUS_PAIR_SYNC.  Time-displaced
+ *                       sweeps always run SINGLE.  Not replayed from a graph.  Results equal
+ *                       SINGLE's bit for bit.
  *   DQMC_LAUNCH_AUTO    PAIR when the batch qualifies, else SINGLE (the default) */
 #define DQMC_LAUNCH_AUTO 0
 #define DQMC_LAUNCH_SINGLE 1
