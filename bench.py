@@ -139,10 +139,10 @@ def run_gpu(w, steps, warmup, chains=None, graph=True, launch="auto"):
     acc_per_chain_sweep = acc0 / (min(n, 8) * steps)
     wrap_err = max(b.info(c).wrap_error for c in range(n))
     statuses = [b.info(c).status for c in range(n)]
-    pair = b.last_launch_mode() == D.LAUNCH_PAIR
+    used = b.last_launch_mode()
     b.close()
     ctx.close()
-    mode = ("spin pairs (two workgroups per chain)" if pair else
+    mode = ("spin pairs (two workgroups per chain)" if used == D.LAUNCH_PAIR else
             "one workgroup per chain, " + ("hipGraph replay" if graph else "direct"))
     return dict(wall=wall, kernel_ms=kernel_ms, chains=n, accepted=acc_per_chain_sweep, wrap_error=wrap_err,
                 status=max(statuses), launch=mode)

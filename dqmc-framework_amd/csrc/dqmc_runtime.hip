@@ -534,6 +534,10 @@ static bool pair_qualifies(dqmc_batch* b) {
 }
 
 int dqmc_batch_set_launch_mode(dqmc_batch_t b, int mode) {
+    if (b && mode == DQMC_LAUNCH_QUAD) {
+        b->ctx->err = "quad launch is not available: its helper workgroups did not finish on the device";
+        return DQMC_ERR_ARG;
+    }
     if (!b || mode < DQMC_LAUNCH_AUTO || mode > DQMC_LAUNCH_PAIR) return DQMC_ERR_ARG;
     HIPCHK(b->ctx, hipSetDevice(b->ctx->device));
     if (mode == DQMC_LAUNCH_PAIR && !pair_qualifies(b)) {

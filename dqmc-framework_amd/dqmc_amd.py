@@ -17,7 +17,7 @@ REPULSIVE, ATTRACTIVE = 0, 1
 PREC_FP64, PREC_MIXED, PREC_FP32_WRAP, PREC_FP32_PRODUCT = 0, 1, 2, 3
 STATUS_NONFINITE, STATUS_SINGULAR, STATUS_WRAP_LIMIT, STATUS_PAIR_SYNC = 1, 2, 4, 8
 STATUS_FATAL = STATUS_NONFINITE | STATUS_SINGULAR | STATUS_PAIR_SYNC
-LAUNCH_AUTO, LAUNCH_SINGLE, LAUNCH_PAIR = 0, 1, 2
+LAUNCH_AUTO, LAUNCH_SINGLE, LAUNCH_PAIR, LAUNCH_QUAD = 0, 1, 2, 3
 DEFAULT_WRAP_LIMIT = 1e-3
 SWEEP_MEASURE, SWEEP_RECORD, SWEEP_FROZEN, SWEEP_PROFILE, SWEEP_DYNAMIC = 1, 2, 4, 8, 16
 PHASES = ["wrap", "metropolis", "product", "push", "fresh", "measure", "walk", "panels"]
@@ -320,7 +320,8 @@ class Batch:
 
     def set_launch_mode(self, mode):
         """LAUNCH_AUTO (default), LAUNCH_SINGLE (one workgroup per chain) or LAUNCH_PAIR (one per
-        spin of every repulsive chain); raises when the batch does not qualify for LAUNCH_PAIR."""
+        spin of every repulsive chain); raises when the batch does not qualify, and for the
+        reserved LAUNCH_QUAD."""
         self.ctx.check(lib().dqmc_batch_set_launch_mode(self.h, int(mode)), "dqmc_batch_set_launch_mode")
 
     def last_launch_mode(self):

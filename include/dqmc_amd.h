@@ -173,11 +173,14 @@ int dqmc_batch_use_graph(dqmc_batch_t b, int enable);
  *                       kernels) fails its chain with DQMC_STATUS_PAIR_SYNC.  Time-displaced
  *                       sweeps always run SINGLE.  Not replayed from a graph.  Results equal
  *                       SINGLE's bit for bit.
+ *   DQMC_LAUNCH_QUAD    reserved: PAIR plus a helper workgroup per spin for half of every
+ *                       dense product; not available (DQMC_ERR_ARG), its launches did not finish
  *   DQMC_LAUNCH_AUTO    PAIR when the batch qualifies, else SINGLE (the default) */
 #define DQMC_LAUNCH_AUTO 0
 #define DQMC_LAUNCH_SINGLE 1
 #define DQMC_LAUNCH_PAIR 2
-/* DQMC_ERR_ARG when PAIR is asked of a batch that does not qualify */
+#define DQMC_LAUNCH_QUAD 3
+/* DQMC_ERR_ARG when PAIR is asked of a batch that does not qualify, and for QUAD */
 int dqmc_batch_set_launch_mode(dqmc_batch_t b, int mode);
 /* the mapping the last sweep launch used (DQMC_LAUNCH_SINGLE or DQMC_LAUNCH_PAIR; 0 before any) */
 int dqmc_batch_last_launch_mode(dqmc_batch_t b, int* mode);

@@ -1,10 +1,13 @@
 """Spin-pair launches (DQMC_LAUNCH_PAIR): each repulsive chain on two co-resident workgroups,
-one per spin, exchanging the k x k block of the walk per delayed-update block.
+one per spin, exchanging the k x k block of the walk per delayed-update block; and quad
+launches (DQMC_LAUNCH_QUAD) that add a helper workgroup per spin computing half of the output
+columns of every dense wrap and slice product.
 
 A pair launch computes the same numbers as the one-workgroup launch: every spin's next
 G0[S,S] is formed from the block's factors in both, each 16x16 block of an update is one
-wavefront's product whichever wavefronts run it, and the walk gets identical inputs in both
-workgroups.  So the tests demand bit-for-bit equality with DQMC_LAUNCH_SINGLE (fields, accept
+wavefront's product whichever wavefronts run it, the walk gets identical inputs in both
+workgroups, and the helpers split the GEMMs at 64-column tile boundaries (each element summed in
+the same order).  So the tests demand bit-for-bit equality with DQMC_LAUNCH_SINGLE (fields, accept
 bits, random state, sign, G of every recorded slot, measurement sums, wrap error), and the
 one-workgroup launch is itself pinned to the oracle by the parity tests.  Shapes: dense and
 checkerboard, the headline 12x12 chains, the 16x16 precision-workload chains, an uneven
@@ -71,6 +74,28 @@ CASES = {
 }
 
 
+DENSE = {
+    "6x6-dense": CASES["6x6-dense"],   # N = 36: the helpers get no columns
+    "12x12-dense": [D.make_params(12, 40, 4.0, pace=8, U=4.0, mu=-0.1 * s, delay=32, seed=s) for s in range(5)],
+    "uneven-dense": [D.make_params(side, 24 + 8 * (c % 3), 3.0, pace=6, U=5.0, mu=-0.2 * (c % 4), delay=(8, 16, 32)[c % 3],
+                                   seed=200 + c) for c, side in enumerate([4, 8, 10, 11, 12, 16, 9, 14, 13])],
+}
+
+
+@pytest.mark.skip(reason="quad launches (a helper workgroup per spin for half of every dense product) did not "
+                         "finish on the device in two runs (the second after making the helpers' flag polls "
+                         "relaxed); the cause is not found, so dqmc_batch_set_launch_mode refuses DQMC_LAUNCH_QUAD")
+@pytest.mark.parametrize("name", sorted(DENSE))
+def test_quad_equals_single_bit_for_bit(ctx, name):
+    ps = DENSE[name]
+    print(f"[gpu-pair] {name}: {len(ps)} chains, quad vs single", flush=True)
+    single = run(ctx, ps, D.LAUNCH_SINGLE, 3)
+    quad = run(ctx, ps, D.LAUNCH_QUAD, 3)
+    assert quad.last_launch_mode() == D.LAUNCH_QUAD
+    assert_same(single, quad, len(ps))
+    single.close(); quad.close()
+
+
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_pair_equals_single_bit_for_bit(ctx, name):
     ps = CASES[name]
@@ -91,10 +116,12 @@ def test_pair_16x16_precision_workload_chains(ctx):
     mus = [0.0, -0.2, -0.4, -0.6]
     ps = [D.make_params(16, 200, 10.0, pace=8, U=6.0, mu=mus[c % 4], delay=32, seed=c) for c in range(8)]
     single = run(ctx, ps, D.LAUNCH_SINGLE, 1, measure_sweeps=1)
+    pair = run(ctx, ps, D.LAUNCH_PAIR, 1, measure_sweeps=1)
     auto = run(ctx, ps, D.LAUNCH_AUTO, 1, measure_sweeps=1)
-    assert auto.last_launch_mode() == D.LAUNCH_PAIR
+    assert pair.last_launch_mode() == D.LAUNCH_PAIR and auto.last_launch_mode() == D.LAUNCH_PAIR
+    assert_same(single, pair, len(ps))
     assert_same(single, auto, len(ps))
-    single.close(); auto.close()
+    single.close(); pair.close(); auto.close()
 
 
 def test_pair_chain_matches_the_oracle(ctx):
@@ -147,6 +174,16 @@ def test_launch_mode_rules(ctx):
     att.fields_random(); att.init(); att.sweep(1); att.synchronize()
     assert att.last_launch_mode() == D.LAUNCH_SINGLE
     att.close()
+    cb = D.Batch(ctx, [D.make_params(4, 8, 1.0, pace=4, checkerboard=True, U=4.0, seed=s) for s in range(4)])
+    with pytest.raises(D.DqmcError):
+        cb.set_launch_mode(D.LAUNCH_QUAD)   # refused (reserved)
+    cb.set_launch_mode(D.LAUNCH_PAIR)
+    cb.close()
+    mid = D.Batch(ctx, [D.make_params(4, 8, 1.0, pace=4, U=4.0, seed=s) for s in range(72)])
+    with pytest.raises(D.DqmcError):
+        mid.set_launch_mode(D.LAUNCH_QUAD)   # refused (reserved)
+    mid.set_launch_mode(D.LAUNCH_PAIR)
+    mid.close()
     many = D.Batch(ctx, [D.make_params(4, 8, 1.0, pace=4, U=4.0, seed=s) for s in range(256)])
     with pytest.raises(D.DqmcError):
         many.set_launch_mode(D.LAUNCH_PAIR)
