@@ -14,7 +14,7 @@
 // spin-pair exchange area of a repulsive chain: the k x k walk blocks [spin][parity][KMAX^2], then
 // the per-sweep state slots [spin][parity][2] (status bits, wrap error)
 #define DQMC_PAIR_XBUF_DOUBLES (4 * DQMC_KMAX * DQMC_KMAX + 8)
-#define DQMC_PAIR_FLAG_WORDS 64    // per chain: the two spins' sequence flags, 128 bytes apart
+#define DQMC_PAIR_FLAG_WORDS 96    // per chain: the two spins' sequence flags and the join word, 128 bytes apart
 
 enum {
     DQMC_FLAG_MEASURE = 1,   // accumulate equal-time observables after each sweep
@@ -64,6 +64,7 @@ struct ChainState {
     int status;          // DQMC_STATUS_* bits (0 ok)
     int accepted;        // accepted flips in the last launch (for reporting)
     unsigned long long phase[8];  // device ticks per phase (DQMC_FLAG_PROFILE)
+    int paired;          // the last sweep launch ran the two spins on two workgroups
 };
 
 struct ChainDesc {

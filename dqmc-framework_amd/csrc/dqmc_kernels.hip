@@ -67,6 +67,7 @@ struct Lds {
     int pair, me;                 // pair launch; this workgroup's spin in it
     unsigned xseq;                // exchanges of this launch so far (pair launch)
     int xbroken;                  // the partner stopped answering: no further waits
+    int join;                     // pair launch: 1 paired, 0 this workgroup runs the chain alone, -1 it leaves
 };
 
 // one LDS object per workgroup, at namespace scope so that separately
@@ -3157,11 +3158,49 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1)))
     }
 }
 
+// Joining a pair launch.  Its two workgroups are not guaranteed to be resident
+// together (another stream's or process's kernels may hold the CUs, or the grid
+// may exceed what the device holds), so they meet on the chain's join word first
+// (0 nobody, 1 one arrived, 2 paired, 3 taken alone; zeroed by the host before
+// the launch): the first to arrive waits up to PAIR_JOIN_TICKS for its partner
+// and then claims the chain for itself; the late partner finds the claim and
+// leaves.  A pair therefore starts only when both workgroups run, and once both
+// run they stay resident to the end, so every later wait ends.  A chain alone
+// computes the same numbers as a pair (the one-workgroup path), so results never
+// depend on how the join went.  Returns 1 (paired), 0 (alone) or -1 (leave).
+#define PAIR_JOIN_TICKS 10000ull   // 100 us of the 100 MHz wall clock
+__device__ int pair_join(Desc d) {
+    unsigned* w = d.xflag + 64;
+    unsigned seen = 0u;
+    if (__hip_atomic_compare_exchange_strong(w, &seen, 1u, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        const unsigned long long t0 = wall_clock64();   // first: wait for the partner
+        while (__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 2u) {
+            if (wall_clock64() - t0 > PAIR_JOIN_TICKS) {
+                unsigned one = 1u;
+                if (__hip_atomic_compare_exchange_strong(w, &one, 3u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT))
+                    return 0;   // claimed: the chain runs here alone
+                return 1;       // the partner arrived meanwhile
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        return 1;
+    }
+    if (seen == 1u) {   // second: join unless the first has claimed the chain
+        unsigned one = 1u;
+        if (__hip_atomic_compare_exchange_strong(w, &one, 2u, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            return 1;
+    }
+    return -1;
+}
+
 // Workgroup -> (chain, spin).  One workgroup per chain, or in a pair launch
 // (pair != 0, grid 16 x ceil(n_chains / 8)) chain 8 g + r and spin h at workgroup
 // 16 g + 8 h + r (r < 8): the two workgroups of a chain are 8 apart, so under the
-// round-robin dispatch over the 8 XCDs they share an XCD and its L2.  In a pair
-// launch only spin 0's workgroup writes the chain state.
+// round-robin dispatch over the 8 XCDs they share an XCD and its L2.  A chain that
+// cannot pair (one spin, a time-displaced sweep) or whose join failed runs in one
+// workgroup, which then writes the chain state; a paired chain's state is written
+// by spin 0's workgroup.
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) k_sweep(const ChainDesc* __restrict__ descs, int n_sweeps, int flags,
                                                                                         int n_chains, int pair) {
     Lds& s = g_lds;
@@ -3178,6 +3217,16 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1)))
     int nacc = 0;
     if (d.launch_sweeps > 0) n_sweeps = d.launch_sweeps;
     if (d.launch_flags >= 0) flags = d.launch_flags;
+    if (pair) {
+        // one spin or a time-displaced sweep: spin 0's workgroup runs the chain alone
+        const bool can = d.nspin == 2 && !(flags & DQMC_FLAG_DYNAMIC);
+        if (tid() == 0) s.join = can ? pair_join(d) : (me == 0 ? 0 : -1);
+        __syncthreads();
+        const int j = s.join;
+        if (j < 0) return;
+        pair = j;
+        if (!pair) me = 0;
+    }
     if (tid() == 0) {
         s.prof[0] = s.prof[1] = 0;
         s.prof_on = (flags & DQMC_FLAG_PROFILE) ? 1 : 0;
@@ -3188,8 +3237,6 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1)))
         s.sp_hi = pair ? me + 1 : d.nspin;
         s.xseq = 0;
         s.xbroken = 0;
-        // the time-displaced sweep runs in one workgroup per chain only (the host never asks otherwise)
-        if (pair && (flags & DQMC_FLAG_DYNAMIC)) s.status |= DQMC_STATUS_PAIR_SYNC;
     }
     __syncthreads();
     // a chain that met a fatal numerical failure sweeps no further (every thread reads the
@@ -3209,6 +3256,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1)))
         st.rng = rng;
         st.sign = sign;
         st.accepted = nacc;
+        st.paired = pair;
         st.phase[DQMC_PH_WALK] += s.prof[0];
         st.phase[DQMC_PH_PANEL] += s.prof[1];
         *d.st = st;
@@ -3468,9 +3516,8 @@ hipError_t dqmc_launch_sweep(const ChainDesc* d_descs, int n_chains, int n_sweep
     hipLaunchKernelGGL(dqmc::k_sweep, dim3(n_chains), dim3(NT), 0, stream, d_descs, n_sweeps, flags, n_chains, pair);
     return hipGetLastError();
 }
-// pair launch: 16 x ceil(n_chains / 8) workgroups, no more than the device holds at once (the
-// caller checks), so all of them are resident together once earlier kernels have drained; the
-// chains' exchange flags must be zero (the caller clears them on the stream)
+// pair launch: 16 x ceil(n_chains / 8) workgroups; the chains' exchange flags and join words must
+// be zero (the caller clears them on the stream)
 hipError_t dqmc_launch_sweep_pair(const ChainDesc* d_descs, int n_chains, int n_sweeps, int flags, hipStream_t stream) {
     int pair = 1;
     hipLaunchKernelGGL(dqmc::k_sweep, dim3(16 * ((n_chains + 7) / 8)), dim3(NT), 0, stream, d_descs, n_sweeps, flags,

@@ -511,16 +511,12 @@ int dqmc_init(dqmc_batch_t b) {
     return DQMC_OK;
 }
 
-// Pair launches of a device never overlap one another (any batch or context of the process): each
-// waits on its stream for the one before, so two of them can never hold parts of the device while
-// waiting for partner workgroups the other keeps off it.  One event per device, kept to the end.
-static std::mutex g_pair_mu;
-static std::map<int, hipEvent_t> g_pair_done;
-
-// a pair launch fits: every chain repulsive, and 16 x ceil(n / 8) sweep workgroups resident at once
+// a pair launch pays: some chain is repulsive, and 16 x ceil(n / 8) sweep workgroups fit on the
+// device at once (a chain whose workgroups do not meet runs alone, so this is about speed only)
 static bool pair_qualifies(dqmc_batch* b) {
-    for (const auto& h : b->chains)
-        if (h.nspin != 2) return false;
+    bool any = false;
+    for (const auto& h : b->chains) any = any || h.nspin == 2;
+    if (!any) return false;
     dqmc_context* ctx = b->ctx;
     if (ctx->resident_limit < 0) {
         int lim = 0;
@@ -541,7 +537,7 @@ int dqmc_batch_set_launch_mode(dqmc_batch_t b, int mode) {
     if (!b || mode < DQMC_LAUNCH_AUTO || mode > DQMC_LAUNCH_PAIR) return DQMC_ERR_ARG;
     HIPCHK(b->ctx, hipSetDevice(b->ctx->device));
     if (mode == DQMC_LAUNCH_PAIR && !pair_qualifies(b)) {
-        b->ctx->err = "pair launch needs every chain repulsive and 16 x ceil(chains / 8) resident workgroups";
+        b->ctx->err = "pair launch needs a repulsive chain and 16 x ceil(chains / 8) resident workgroups";
         return DQMC_ERR_ARG;
     }
     b->launch_mode = mode;
@@ -567,20 +563,10 @@ int dqmc_sweep(dqmc_batch_t b, int n_sweeps, int flags) {
     // the time-displaced sweep runs one workgroup per chain; a pair launch is not replayed from a graph
     const bool pair = b->launch_mode != DQMC_LAUNCH_SINGLE && !(flags & DQMC_SWEEP_DYNAMIC) && pair_qualifies(b);
     if (pair) {
-        std::lock_guard<std::mutex> lk(g_pair_mu);
-        auto it = g_pair_done.find(b->ctx->device);
-        if (it == g_pair_done.end()) {
-            hipEvent_t e = nullptr;
-            HIPCHK(b->ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
-            it = g_pair_done.emplace(b->ctx->device, e).first;
-        } else {
-            HIPCHK(b->ctx, hipStreamWaitEvent(b->stream, it->second, 0));
-        }
         HIPCHK(b->ctx, hipMemsetAsync(b->d_xflags, 0, sizeof(unsigned) * DQMC_PAIR_FLAG_WORDS * (size_t)b->n, b->stream));
         HIPCHK(b->ctx, hipEventRecord(b->ev0, b->stream));
         HIPCHK(b->ctx, dqmc_launch_sweep_pair(b->d_descs, b->n, n_sweeps, flags, b->stream));
         HIPCHK(b->ctx, hipEventRecord(b->ev1, b->stream));
-        HIPCHK(b->ctx, hipEventRecord(it->second, b->stream));
     }
     if (!pair && b->use_graph && (b->graph_n != n_sweeps || b->graph_flags != flags)) {
         if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
@@ -694,6 +680,7 @@ int dqmc_get_info(dqmc_batch_t b, int c, dqmc_chain_info* out) {
     out->pos = st.pos;
     out->status = st.status;
     out->accepted = st.accepted;
+    out->paired = st.paired;
     out->sweeps = b->chains[c].sweeps;
     for (int q = 0; q < DQMC_PH_COUNT; ++q) out->phase_seconds[q] = (double)st.phase[q] * 1e-8;  // 100 MHz clock
     return DQMC_OK;

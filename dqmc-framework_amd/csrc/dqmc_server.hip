@@ -108,6 +108,7 @@ struct Lane {
     double* h_stage = nullptr;   // pinned, max_active x STAGE_DOUBLES
     ChainDesc* d_descs = nullptr;
     ChainDesc* d_init = nullptr;
+    unsigned* d_xflags = nullptr;   // spin-pair flags of a launch's chains (max_active x DQMC_PAIR_FLAG_WORDS)
     std::vector<ChainDesc> h_descs, h_init;
     std::vector<Job*> in_flight;
     std::deque<Job*> queue;
@@ -367,12 +368,19 @@ bool launch_lane(dqmc_server* s, Lane& ln, int lane_index) {
         ChainDesc d = j->b->descs[0];
         d.launch_sweeps = n;
         d.launch_flags = flags;
+        // this launch's flags for the chain (its exchange blocks stay in the job's own memory)
+        if (d.xflag) d.xflag = ln.d_xflags + ln.h_descs.size() * DQMC_PAIR_FLAG_WORDS;
         ln.h_descs.push_back(d);
         ln.launch_flop += n * j->cost;
         ln.in_flight.push_back(j);
     }
     hipMemcpyAsync(ln.d_descs, ln.h_descs.data(), sizeof(ChainDesc) * ln.h_descs.size(), hipMemcpyHostToDevice, ln.stream);
-    if (dqmc_launch_sweep(ln.d_descs, (int)ln.h_descs.size(), 1, 0, ln.stream) != hipSuccess) s->status = DQMC_ERR_HIP;
+    // spin pairs: each repulsive chain's spins on two workgroups when both get a CU (a chain whose
+    // partner is kept off the device by the other lanes runs alone, with the same result)
+    const int nl = (int)ln.h_descs.size();
+    hipError_t le = hipMemsetAsync(ln.d_xflags, 0, sizeof(unsigned) * DQMC_PAIR_FLAG_WORDS * nl, ln.stream);
+    if (le == hipSuccess) le = dqmc_launch_sweep_pair(ln.d_descs, nl, 1, 0, ln.stream);
+    if (le != hipSuccess) s->status = DQMC_ERR_HIP;
     // the readbacks this launch ends in, behind it on the stream: a bin's accumulators (then
     // cleared) and the state of a job that finishes
     for (size_t q = 0; q < ln.in_flight.size(); ++q) {
@@ -455,6 +463,7 @@ int dqmc_server_create(dqmc_ctx_t ctx, size_t hbm_budget, int max_active, int n_
         if (e == hipSuccess) e = hipEventCreateWithFlags(&ln.ev, hipEventDisableTiming);
         if (e == hipSuccess) e = hipMalloc((void**)&ln.d_descs, sizeof(ChainDesc) * max_active);
         if (e == hipSuccess) e = hipMalloc((void**)&ln.d_init, sizeof(ChainDesc) * max_active);
+        if (e == hipSuccess) e = hipMalloc((void**)&ln.d_xflags, sizeof(unsigned) * DQMC_PAIR_FLAG_WORDS * max_active);
         if (e == hipSuccess) e = hipHostMalloc((void**)&ln.h_stage, sizeof(double) * STAGE_DOUBLES * max_active, 0);
         if (e != hipSuccess) { ctx->err = hipGetErrorString(e); dqmc_server_destroy(s); return DQMC_ERR_HIP; }
     }
@@ -585,6 +594,7 @@ int dqmc_server_destroy(dqmc_server_t s) {
         if (ln.stream) hipStreamSynchronize(ln.stream);
         if (ln.d_descs) hipFree(ln.d_descs);
         if (ln.d_init) hipFree(ln.d_init);
+        if (ln.d_xflags) hipFree(ln.d_xflags);
         if (ln.h_stage) hipHostFree(ln.h_stage);
         if (ln.ev) hipEventDestroy(ln.ev);
         if (ln.stream) hipStreamDestroy(ln.stream);

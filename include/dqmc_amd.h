@@ -54,7 +54,7 @@ extern "C" {
 #define DQMC_STATUS_NONFINITE 1   /* a recomputed or wrapped G held inf or NaN (fatal) */
 #define DQMC_STATUS_SINGULAR 2    /* an LU pivot of the recompute was zero, subnormal or non-finite (fatal) */
 #define DQMC_STATUS_WRAP_LIMIT 4  /* the wrap error exceeded the batch's limit (dqmc_set_wrap_error_limit) */
-#define DQMC_STATUS_PAIR_SYNC 8   /* a spin-pair launch's partner workgroup did not answer within 30 s (fatal) */
+#define DQMC_STATUS_PAIR_SYNC 8   /* a paired chain's partner workgroup did not answer within 30 s (fatal) */
 #define DQMC_STATUS_FATAL (DQMC_STATUS_NONFINITE | DQMC_STATUS_SINGULAR | DQMC_STATUS_PAIR_SYNC)
 #define DQMC_DEFAULT_WRAP_LIMIT 1e-3
 
@@ -141,6 +141,7 @@ typedef struct {
                                 and two parts of those: the Metropolis in-block walk, the QR/LU
                                 panel factorizations (accumulated over DQMC_SWEEP_PROFILE sweeps
                                 since init) */
+    int paired;              /* 1 if the last sweep launch ran the chain's spins on two workgroups */
 } dqmc_chain_info;
 
 /* context: device selection, error string */
@@ -166,13 +167,14 @@ int dqmc_batch_use_graph(dqmc_batch_t b, int enable);
  *   DQMC_LAUNCH_PAIR    two co-resident workgroups per repulsive chain, one per spin: wraps,
  *                       slice products, stack pushes and recomputes of the two spins run side
  *                       by side; the Metropolis walk runs in both, exchanging the k x k block of
- *                       each spin per delayed-update block.  Needs every chain repulsive and
- *                       2 x (chains rounded up to 8) workgroups resident at once; the pair
- *                       launches of a device run one after another (any batch of the process),
- *                       and a partner kept off the device for 30 s (by another process's
- *                       kernels) fails its chain with DQMC_STATUS_PAIR_SYNC.  Time-displaced
- *                       sweeps always run SINGLE.  Not replayed from a graph.  Results equal
- *                       SINGLE's bit for bit.
+ *                       each spin per delayed-update block.  The two workgroups of a chain
+ *                       meet at the start; one that finds no partner within 100 us runs the
+ *                       chain alone, so a pair launch never waits on workgroups that are not
+ *                       running.  Attractive chains run alone in one workgroup (the other
+ *                       leaves); time-displaced sweeps always run SINGLE.  Needs a repulsive
+ *                       chain and 2 x (chains rounded up to 8) workgroups resident at once.  Not
+ *                       replayed from a graph.  Results equal SINGLE's bit for bit however the
+ *                       chains paired.
  *   DQMC_LAUNCH_QUAD    reserved: PAIR plus a helper workgroup per spin for half of every
  *                       dense product; not available (DQMC_ERR_ARG), its launches did not finish
  *   DQMC_LAUNCH_AUTO    PAIR when the batch qualifies, else SINGLE (the default) */

@@ -147,6 +147,47 @@ def test_pair_chain_matches_the_oracle(ctx):
     b.close()
 
 
+def test_pair_join_falls_back_on_a_busy_device(ctx):
+    """A pair launch while another stream's 250-chain launch holds 250 of the 256 CUs: the chains
+    whose partner workgroup cannot get a CU within 100 us run alone, the others pair once CUs free
+    up, and every chain equals the one-workgroup launch bit for bit, with no synchronization
+    failure (dqmc_chain_info.paired tells which way each went)."""
+    print("[gpu-pair] pair launch beside a 250-chain launch on another stream", flush=True)
+    busy = D.Batch(ctx, [D.make_params(12, 60, 6.0, pace=10, checkerboard=True, delay=32, U=4.0, seed=s)
+                         for s in range(250)])
+    busy.set_launch_mode(D.LAUNCH_SINGLE)
+    busy.fields_random(); busy.init(); busy.synchronize()
+    ps = [D.make_params(8, 40, 4.0, pace=8, U=4.0, mu=-0.2, delay=16, seed=s) for s in range(16)]
+    ref = run(ctx, ps, D.LAUNCH_SINGLE, 2, measure_sweeps=0, record=False)
+    b = D.Batch(ctx, ps)
+    b.set_launch_mode(D.LAUNCH_PAIR)
+    b.fields_random(); b.init(); b.synchronize()
+    busy.sweep(6)   # asynchronous, on the busy batch's stream
+    b.sweep(2)
+    b.synchronize(); busy.synchronize()
+    paired = [b.info(c).paired for c in range(16)]
+    print(f"[gpu-pair] {sum(paired)} of 16 chains paired, {16 - sum(paired)} ran alone", flush=True)
+    assert b.last_launch_mode() == D.LAUNCH_PAIR
+    assert all(b.info(c).status == 0 for c in range(16))
+    assert all(busy.info(c).status == 0 for c in range(250))
+    assert_same(ref, b, 16, record=False)
+    ref.close(); b.close(); busy.close()
+
+
+def test_pair_launch_with_attractive_chains(ctx):
+    """Attractive chains in a pair launch run alone (the second workgroup leaves); the batch equals
+    the one-workgroup launch bit for bit and AUTO takes the pair form."""
+    print("[gpu-pair] mixed repulsive / attractive batch", flush=True)
+    ps = [D.make_params(6, 24, 3.0, pace=6, model=(D.ATTRACTIVE if c % 3 == 0 else D.REPULSIVE), U=4.0, mu=-0.3,
+                        delay=8, seed=300 + c) for c in range(9)]
+    single = run(ctx, ps, D.LAUNCH_SINGLE, 2)
+    auto = run(ctx, ps, D.LAUNCH_AUTO, 2)
+    assert auto.last_launch_mode() == D.LAUNCH_PAIR
+    assert [auto.info(c).paired for c in range(9)] == [0 if c % 3 == 0 else 1 for c in range(9)]
+    assert_same(single, auto, 9)
+    single.close(); auto.close()
+
+
 def test_failing_chain_in_a_pair_launch(ctx):
     """An overflowing exp(-dtau K) fails its chain at init (DQMC_STATUS_NONFINITE): neither of its
     workgroups sweeps or waits, and the healthy chains are the chains the one-workgroup launch gives."""
@@ -165,8 +206,8 @@ def test_failing_chain_in_a_pair_launch(ctx):
 
 
 def test_launch_mode_rules(ctx):
-    """PAIR needs every chain repulsive and the pair grid resident at once; the time-displaced
-    sweep falls back to one workgroup per chain."""
+    """PAIR needs a repulsive chain and the pair grid resident at once; the time-displaced sweep
+    falls back to one workgroup per chain."""
     print("[gpu-pair] launch-mode rules", flush=True)
     att = D.Batch(ctx, [D.make_params(4, 16, 2.0, pace=4, model=D.ATTRACTIVE, U=4.0, seed=1)])
     with pytest.raises(D.DqmcError):
