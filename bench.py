@@ -105,12 +105,16 @@ def algorithmic_flops_per_dynamic_sweep(w):
 
 
 def autotune(w, candidates=(8, 16, 32)):
-    """Delayed-update depth for the workload's chain shape, chosen by the engine's
-    autotuner (each candidate timed on a full batch of the workload's size)."""
+    """Delayed-update depth and chain -> workgroup mapping for the workload's chain shape, chosen
+    by the engine's autotuners (each candidate timed on a full batch of the workload's size)."""
     ctx = D.Context(0)
     best, ms = ctx.autotune_delay(chain_params(w, 1)[0], w["chains"], candidates=candidates, sweeps=2)
+    p = chain_params(w, 1)[0]
+    p.delay = best
+    mode, mode_ms = ctx.autotune_launch(p, w["chains"], sweeps=2)
     ctx.close()
-    return best, ms
+    names = {D.LAUNCH_SINGLE: "single", D.LAUNCH_PAIR: "pair"}
+    return best, ms, names[mode], {names[m]: t for m, t in mode_ms.items()}
 
 
 LAUNCH_MODES = {"auto": D.LAUNCH_AUTO, "single": D.LAUNCH_SINGLE, "pair": D.LAUNCH_PAIR}
@@ -345,11 +349,14 @@ def main():
         w["precision"] = {"fp64": D.PREC_FP64, "mixed": D.PREC_MIXED, "fp32_wrap": D.PREC_FP32_WRAP,
                           "fp32_product": D.PREC_FP32_PRODUCT}[args.precision]
         w["desc"] += f" (precision override: {args.precision})"
-    tune = None
+    tune = launch_tune = None
     if not args.no_autotune and w["chains"] > 1:
-        best, tune = autotune(w)
+        best, tune, mapping, launch_tune = autotune(w)
         w["delay"] = best
-        print(f"[bench] delay depth autotuned: {best} (ms per 2 sweeps {tune})", file=sys.stderr, flush=True)
+        if args.launch == "auto":
+            args.launch = mapping
+        print(f"[bench] autotuned: delay depth {best} (ms per 2 sweeps {tune}), mapping {mapping} ({launch_tune})",
+              file=sys.stderr, flush=True)
     print(f"[bench] {w['desc']}: warmup {args.warmup}, steps {args.steps}", file=sys.stderr, flush=True)
     r = run_gpu(w, args.steps, args.warmup, graph=not args.no_graph, launch=args.launch)
     chain_sweeps = r["chains"] * args.steps
@@ -374,6 +381,7 @@ def main():
     }
     if tune is not None:
         out["config"]["delay_autotune_ms"] = tune
+        out["config"]["launch_autotune_ms"] = launch_tune
     if traffic is not None:
         out["roofline"]["traffic_source"] = f"{traffic_src} (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated)"
         out["roofline"]["traffic_tbs_at_kernel_ms"] = traffic / (kms * 1e-3) / 1e12

@@ -1263,4 +1263,34 @@ int dqmc_autotune_delay(dqmc_ctx_t ctx, const dqmc_chain_params* shape, int n_ch
     return DQMC_OK;
 }
 
+int dqmc_autotune_launch(dqmc_ctx_t ctx, const dqmc_chain_params* shape, int n_chains, int sweeps, int* best,
+                         float* ms_out) {
+    if (!ctx || !shape || !best || n_chains < 1 || sweeps < 1) return DQMC_ERR_ARG;
+    const int modes[2] = {DQMC_LAUNCH_SINGLE, DQMC_LAUNCH_PAIR};
+    float best_ms = -1.0f;
+    for (int m = 0; m < 2; ++m) {
+        std::vector<dqmc_chain_params> ps(n_chains, *shape);
+        for (int k = 0; k < n_chains; ++k) ps[k].seed = shape->seed + k;
+        dqmc_batch_t b = nullptr;
+        int rc = dqmc_batch_create(ctx, n_chains, ps.data(), 0, &b);
+        if (rc != DQMC_OK) return rc;
+        float ms = -1.0f;
+        rc = dqmc_batch_set_launch_mode(b, modes[m]);
+        if (rc == DQMC_ERR_ARG && m == 1) {   // the batch does not qualify for pairs
+            dqmc_batch_destroy(b);
+            break;
+        }
+        if (rc == DQMC_OK) rc = dqmc_fields_random(b);
+        if (rc == DQMC_OK) rc = dqmc_init(b);
+        if (rc == DQMC_OK) rc = dqmc_sweep(b, 1, 0);
+        if (rc == DQMC_OK) rc = dqmc_sweep(b, sweeps, 0);
+        if (rc == DQMC_OK) rc = dqmc_last_sweep_ms(b, &ms);
+        dqmc_batch_destroy(b);
+        if (rc != DQMC_OK) return rc;
+        if (ms_out) ms_out[m] = ms;
+        if (best_ms < 0.0f || ms < best_ms) { best_ms = ms; *best = modes[m]; }
+    }
+    return DQMC_OK;
+}
+
 }  // extern "C"

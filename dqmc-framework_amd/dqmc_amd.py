@@ -27,7 +27,7 @@ OBS_NAMES = ["sign", "filling_number", "double_occupancy", "kinetic_energy", "mo
 
 EXPORTS = ["dqmc_context_create", "dqmc_context_destroy", "dqmc_context_error", "dqmc_batch_footprint",
            "dqmc_batch_create", "dqmc_batch_destroy", "dqmc_batch_size", "dqmc_batch_use_graph",
-           "dqmc_batch_set_launch_mode", "dqmc_batch_last_launch_mode", "dqmc_fields_random", "dqmc_set_fields",
+           "dqmc_batch_set_launch_mode", "dqmc_batch_last_launch_mode", "dqmc_autotune_launch", "dqmc_fields_random", "dqmc_set_fields",
            "dqmc_get_fields", "dqmc_set_rng_state", "dqmc_set_expk", "dqmc_get_expk", "dqmc_init", "dqmc_sweep",
            "dqmc_set_wrap_error_limit",
            "dqmc_synchronize", "dqmc_last_sweep_ms", "dqmc_get_green", "dqmc_get_slice_green", "dqmc_get_slice_signs",
@@ -137,6 +137,7 @@ def lib():
         L.dqmc_k_udv_push.argtypes = [vp, ip, vp, vp, vp, vp, vp, vp, vp]
         L.dqmc_k_greens_recompute.argtypes = [vp, ip, vp, vp, vp, vp, vp, vp, vp]
         L.dqmc_autotune_delay.argtypes = [vp, C.POINTER(ChainParams), ip, vp, ip, ip, C.POINTER(ip), vp]
+        L.dqmc_autotune_launch.argtypes = [vp, C.POINTER(ChainParams), ip, ip, C.POINTER(ip), vp]
         L.dqmc_server_create.argtypes = [vp, C.c_size_t, ip, ip, C.POINTER(vp)]
         L.dqmc_server_submit.argtypes = [vp, C.POINTER(Job), C.POINTER(C.c_longlong)]
         L.dqmc_server_cancel.argtypes = [vp, C.c_longlong]
@@ -277,6 +278,15 @@ class Context:
         self.check(lib().dqmc_autotune_delay(self.h, C.byref(params), int(n_chains), _p(cand), len(cand), int(sweeps),
                                              C.byref(best), _p(ms)), "dqmc_autotune_delay")
         return best.value, dict(zip([int(c) for c in cand], [float(x) for x in ms]))
+
+    def autotune_launch(self, params, n_chains, sweeps=2):
+        """Faster chain -> workgroup mapping (LAUNCH_SINGLE or LAUNCH_PAIR) for chains of this shape, and the
+        launch times ({mode: ms}; pairs are left out when the batch does not qualify)."""
+        ms = np.full(2, -1.0, dtype=np.float32)
+        best = C.c_int(0)
+        self.check(lib().dqmc_autotune_launch(self.h, C.byref(params), int(n_chains), int(sweeps), C.byref(best), _p(ms)),
+                   "dqmc_autotune_launch")
+        return best.value, {m: float(t) for m, t in zip((LAUNCH_SINGLE, LAUNCH_PAIR), ms) if t >= 0}
 
     def error(self):
         return lib().dqmc_context_error(self.h).decode()

@@ -271,6 +271,12 @@ int dqmc_write_observable_bins_file(const char* path, int rank, int n_bins, int 
  * (ms_out[c] = device time of candidate c, may be NULL). */
 int dqmc_autotune_delay(dqmc_ctx_t ctx, const dqmc_chain_params* shape, int n_chains, const int* candidates,
                         int n_candidates, int sweeps, int* best, float* ms_out);
+/* the faster chain -> workgroup mapping for n_chains chains of this shape: times `sweeps` sweeps
+ * with DQMC_LAUNCH_SINGLE and, when the batch qualifies, DQMC_LAUNCH_PAIR; *best receives the
+ * faster, ms_out[0..1] (may be NULL) the launch times (ms_out[1] = -1 when pairs do not apply).
+ * No counterpart in the reference (one chain per process). */
+int dqmc_autotune_launch(dqmc_ctx_t ctx, const dqmc_chain_params* shape, int n_chains, int sweeps, int* best,
+                         float* ms_out);
 
 /* ------------------------------------------------------------------------
  * Serving: continuous batching of DQMC jobs.  No counterpart in the

@@ -247,3 +247,17 @@ def test_launch_mode_rules(ctx):
         assert np.array_equal(out[D.LAUNCH_SINGLE].dynamic_accum(c)["gk"], out[D.LAUNCH_PAIR].dynamic_accum(c)["gk"])
     for b in out.values():
         b.close()
+
+
+def test_autotune_launch(ctx):
+    """dqmc_autotune_launch times both mappings where pairs apply (8 chains of 12x12: pairs are
+    faster, the CUs being idle) and only the one-workgroup form where they do not (256 chains)."""
+    print("[gpu-pair] launch autotuner", flush=True)
+    p = D.make_params(12, 60, 6.0, pace=10, checkerboard=True, delay=32, U=4.0, seed=1)
+    best, ms = ctx.autotune_launch(p, 8, sweeps=1)
+    print(f"[gpu-pair] 8 x 12x12: {ms}", flush=True)
+    assert set(ms) == {D.LAUNCH_SINGLE, D.LAUNCH_PAIR} and best == D.LAUNCH_PAIR
+    assert ms[D.LAUNCH_PAIR] < ms[D.LAUNCH_SINGLE]
+    q = D.make_params(4, 8, 1.0, pace=4, U=4.0, seed=1)
+    best, ms = ctx.autotune_launch(q, 256, sweeps=1)
+    assert set(ms) == {D.LAUNCH_SINGLE} and best == D.LAUNCH_SINGLE
