@@ -683,6 +683,216 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int Nc, int K, const double* A,
     if (uni(M) <= 16) gemm_body<lowp, aL, bL, 16>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, alpha);
     else gemm_body<lowp, aL, bL, 64>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, alpha);
 }
+
+// ---------------------------------------------------------------------------
+// Wide tiles for the N x N x N products (FP64, A and B in global memory).  The
+// 64x64 tiles above give each wavefront 2x2 blocks, i.e. 16 MFMAs per slab step
+// between two workgroup barriers, and re-read A and B once per tile row and
+// column.  Here each wavefront keeps RB x CB blocks of 16x16 in accumulators
+// (a (32 RB) x (32 CB) workgroup tile: 160 x 160 with RB = CB = 5 covers an
+// N = 144 matrix in one tile, 128 x 128 tiles cover N = 256 in four), so a step
+// issues RB CB KW / 4 MFMAs per wavefront and A and B are read once per tile.  The
+// slabs (KW = 16 deep, 8 for the 160 x 160 tile, whose accumulators leave fewer
+// registers for staging) are double-buffered in the LDS panel, which the callers
+// leave free.
+// Every output element sums its products in the same order as gemm_body (16-deep
+// slabs in k order, four 16x16x4 MFMAs each, formed transposed), and the
+// epilogue applies alpha, cs, rs and beta C in the same order, so the two
+// GEMMs give bit-identical results.
+// ---------------------------------------------------------------------------
+template <int RB, int CB, int KW>
+__device__ __noinline__ void wg_gemm_wide(int M, int Nc, int K, const double* A_, int lda, bool tA, const double* B_,
+                                          int ldb, bool tB, double* C, int ldc, const double* rs, const double* ms_,
+                                          const double* cs, double beta, double alpha) {
+    Lds& s = g_lds;
+    M = uni(M); Nc = uni(Nc); K = uni(K); lda = uni(lda); ldb = uni(ldb); ldc = uni(ldc);
+    tA = uni(tA); tB = uni(tB); beta = uni(beta); alpha = uni(alpha);
+    const AS_G double* const A = (const AS_G double*)uni(A_);
+    const AS_G double* const B = (const AS_G double*)uni(B_);
+    AS_G double* const Cg = (AS_G double*)uni(C);
+    rs = uni(rs); ms_ = uni(ms_); cs = uni(cs);
+    const bool has_ms = ms_ != nullptr, has_rs = rs != nullptr, has_cs = cs != nullptr;
+    // the scalings live in LDS (null tests above on the generic pointers)
+    const AS_L double* ms = has_ms ? (const AS_L double*)ms_ : (const AS_L double*)s.panel;
+    const AS_L double* rsl = (const AS_L double*)rs;
+    const AS_L double* csl = (const AS_L double*)cs;
+    constexpr int TM = 32 * RB, TN = 32 * CB;      // workgroup tile
+    constexpr int SA = TM + 16, SB = TN + 16;      // LDS row strides of the k-major slabs (16 mod 32 doubles)
+    constexpr int SQA = TM * KW / NT, SQB = TN * KW / NT;   // slab elements per thread (KW-deep slabs)
+    constexpr int BUF = KW * (SA + SB);            // one slab buffer (A then B)
+    static_assert(KW == 8 || KW == 16, "slab depth");
+    static_assert(SQA == RB * (KW / 8) && SQB == CB * (KW / 8), "slab elements per thread");
+    static_assert(2 * BUF <= PANEL_DOUBLES, "two wide slabs exceed the LDS panel");
+    static_assert(SQA + SQB <= 32, "slab range bits");
+    if (M <= 0 || Nc <= 0 || K <= 0) { __syncthreads(); return; }
+    const int l = lane();
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wr = (w & 1) * 16 * RB, wc = (w >> 1) * 16 * CB;   // this wavefront's quarter of the tile
+    const int t = tid();
+    // Slab element q of this thread, branch-free for either layout: along the operand's
+    // contiguous dimension in memory the lanes read consecutive doubles.
+    //   rows contiguous (A, B^T):  row = t%32 + 32 (q % R),   depth = t/32 + 8 (q / R)
+    //   depth contiguous (A^T, B): row = t/KW + (NT/KW) q,   depth = t%KW
+    // i.e. row = rb0 + dr1 (q % R) + dr2 (q / R), depth = kb0 + dk2 (q / R), R = RB (A) or CB (B)
+    constexpr int RK = NT / KW;   // rows per thread group in the depth-contiguous map
+    const int ra0 = tA ? t / KW : t & 31, ka0 = tA ? t % KW : t >> 5;
+    const int dra1 = tA ? RK : 32, dra2 = tA ? RK * RB : 0, dka2 = tA ? 0 : 8;
+    const int nb0 = tB ? t & 31 : t / KW, kb0 = tB ? t >> 5 : t % KW;
+    const int dnb1 = tB ? 32 : RK, dnb2 = tB ? 0 : RK * CB, dkb2 = tB ? 8 : 0;
+    // memory strides of a row and of a depth step: op(A)(m, k) at A[m sar + k sak], op(B)(k, n) at B[n sbn + k sbk]
+    const int sar = tA ? lda : 1, sak = tA ? 1 : lda, sbn = tB ? 1 : ldb, sbk = tB ? ldb : 1;
+    const int oa0 = ra0 * sar + ka0 * sak, ob0 = nb0 * sbn + kb0 * sbk;
+    const int oa1 = dra1 * sar, oa2 = dra2 * sar + dka2 * sak;
+    const int ob1 = dnb1 * sbn, ob2 = dnb2 * sbn + dkb2 * sbk;
+    const int la0 = ka0 * SA + ra0, la1 = dra1, la2 = dra2 + dka2 * SA;
+    const int lb0 = kb0 * SB + nb0, lb1 = dnb1, lb2 = dnb2 + dkb2 * SB;
+    double va[SQA], vb[SQB], msv[SQA];
+    unsigned okm = 0;   // bit q: A element q of the loaded slab in range; bit SQA + q: B element q
+    auto load = [&](int r0, int c0, int k0) {
+        const int mr = M - r0, nr = Nc - c0, kr = K - k0;
+        const int ba = r0 * sar + k0 * sak, bb = c0 * sbn + k0 * sbk;
+        unsigned ok = 0;
+#pragma unroll
+        for (int q = 0; q < SQA; ++q) {
+            const int i1 = q % RB, i2 = q / RB;
+            const bool okA = (ra0 + i1 * dra1 + i2 * dra2 < mr) & (ka0 + i2 * dka2 < kr);
+            va[q] = A[okA ? ba + oa0 + i1 * oa1 + i2 * oa2 : 0];
+            ok |= (unsigned)okA << q;
+        }
+#pragma unroll
+        for (int q = 0; q < SQB; ++q) {
+            const int i1 = q % CB, i2 = q / CB;
+            const bool okB = (nb0 + i1 * dnb1 + i2 * dnb2 < nr) & (kb0 + i2 * dkb2 < kr);
+            vb[q] = B[okB ? bb + ob0 + i1 * ob1 + i2 * ob2 : 0];
+            ok |= (unsigned)okB << (SQA + q);
+        }
+        okm = ok;
+    };
+    // the K scalings of the loaded slab (depth origin k0s), read ahead of its store
+    auto load_ms = [&](int k0s) {
+#pragma unroll
+        for (int q = 0; q < SQA; ++q) msv[q] = ms[has_ms ? min(k0s + ka0 + (q / RB) * dka2, K - 1) : 0];
+    };
+    auto store = [&](AS_L double* buf) {
+#pragma unroll
+        for (int q = 0; q < SQA; ++q) {
+            const int i1 = q % RB, i2 = q / RB;
+            double v = ((okm >> q) & 1u) ? va[q] : 0.0;
+            v = has_ms ? v * msv[q] : v;
+            buf[la0 + i1 * la1 + i2 * la2] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < SQB; ++q) {
+            const int i1 = q % CB, i2 = q / CB;
+            buf[KW * SA + lb0 + i1 * lb1 + i2 * lb2] = ((okm >> (SQA + q)) & 1u) ? vb[q] : 0.0;
+        }
+    };
+    AS_L double* const buf0 = (AS_L double*)s.panel;
+    int r0 = 0, c0 = 0, k0 = 0;
+    int lr = 0, lc = 0, lk = 0;
+    auto advance = [&](int& r, int& c, int& k) {
+        k += KW;
+        if (k >= K) { k = 0; r += TM; if (r >= M) { r = 0; c += TN; } }
+    };
+    // double-buffered slabs: a step reads buffer (step & 1) while it stores the next slab into
+    // the other one (read in the previous step, which every wavefront has finished at the
+    // step's barrier); the slab after that is loaded into registers meanwhile
+    load(lr, lc, lk);
+    advance(lr, lc, lk);
+    load_ms(0);
+    store(buf0);
+    int sk = lk;
+    load(lr, lc, lk);
+    advance(lr, lc, lk);
+    int par = 0;
+    dbl4 acc[RB][CB];
+    while (c0 < Nc) {
+        if (k0 == 0) {
+#pragma unroll
+            for (int a = 0; a < RB; ++a)
+#pragma unroll
+                for (int b = 0; b < CB; ++b) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+        }
+        const bool last = k0 + KW >= K;
+        AS_L double* const cur = buf0 + par * BUF;
+        AS_L double* const nxt = buf0 + (par ^ 1) * BUF;
+        load_ms(sk);
+        __syncthreads();   // this step's slab is in `cur`; `nxt` is no longer read
+#pragma unroll
+        for (int kk = 0; kk < KW / 4; ++kk) {
+            const int k = kk * 4 + (l >> 4);
+            double oa[RB], ob[CB];
+#pragma unroll
+            for (int h = 0; h < RB; ++h) oa[h] = cur[k * SA + wr + 16 * h + (l & 15)];
+#pragma unroll
+            for (int h = 0; h < CB; ++h) ob[h] = cur[KW * SA + k * SB + wc + 16 * h + (l & 15)];
+#pragma unroll
+            for (int a = 0; a < RB; ++a)
+#pragma unroll
+                for (int b = 0; b < CB; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ob[b], oa[a], acc[a][b], 0, 0, 0);
+            if (kk == 0) store(nxt);   // the next slab, under the matrix cores
+        }
+        sk = lk;
+        load(lr, lc, lk);
+        advance(lr, lc, lk);
+        par ^= 1;
+        if (last) {
+            // tile done: element e of acc[rb][cb] is C(r0 + wr + 16 rb + l%16, c0 + wc + 16 cb + l/16 + 4e)
+            const int rows = M - r0 - wr, cols = Nc - c0 - wc;
+            const int cbase = (c0 + wc) * ldc + r0 + wr + (l & 15) + (l >> 4) * ldc;
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) {
+                const int row = rb * 16 + (l & 15);
+                const double rsv = has_rs ? rsl[row < rows ? r0 + wr + row : 0] : 1.0;
+                double cv[CB][4];
+                if (beta != 0.0) {
+#pragma unroll
+                    for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const int col = cb * 16 + (l >> 4) + 4 * e;
+                            const bool ok = (row < rows) & (col < cols);
+                            cv[cb][e] = Cg[ok ? cbase + (cb * 16 + 4 * e) * ldc + rb * 16 : 0];
+                        }
+                }
+#pragma unroll
+                for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int col = cb * 16 + (l >> 4) + 4 * e;
+                        if (row >= rows || col >= cols) continue;
+                        double v = acc[rb][cb][e] * alpha;
+                        if (has_cs) v *= csl[c0 + wc + col];
+                        if (has_rs) v *= rsv;
+                        Cg[cbase + (cb * 16 + 4 * e) * ldc + rb * 16] = (beta != 0.0) ? v + beta * cv[cb][e] : v;
+                    }
+            }
+        }
+        advance(r0, c0, k0);
+    }
+    __syncthreads();
+}
+
+// the FP64 product of two global operands: wide tiles when the output and the depth are large
+// enough to fill them, the 64x64 tiles otherwise (both give the same bits)
+__device__ __forceinline__ void gemm64_global(int M, int Nc, int K, const double* A, int lda, bool tA, const double* B,
+                                              int ldb, bool tB, double* C, int ldc, const double* rs,
+                                              const double* ms, const double* cs, double beta, double alpha) {
+    Lds& s = g_lds;
+    M = uni(M); Nc = uni(Nc); K = uni(K);
+    if (M >= 96 && Nc >= 96 && K >= 32) {
+        // the tile shape that pads the output least (measured on one CU: N = 144 122 -> 69 us with
+        // 160 x 160 tiles, N = 256 394 -> 273 us and N = 196 304 -> 228 us with 128 x 128 tiles,
+        // N = 400 1783 -> 1400 us with 160 x 160 tiles)
+        const int p128 = ((M + 127) / 128) * ((Nc + 127) / 128) * 128 * 128;
+        const int p160 = ((M + 159) / 160) * ((Nc + 159) / 160) * 160 * 160;
+        if (p160 < p128) wg_gemm_wide<5, 5, 8>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, alpha);
+        else wg_gemm_wide<4, 4, 16>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, alpha);
+        return;
+    }
+    wg_gemm_mfma<false, false, false>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, s, alpha);
+}
 // C <- C - V op(T) (V^T C) for a block reflector (V: m x jb at ld ldv, T: jb x jb at ld NB,
 // op(T) = T^T when tT; C: m x n at ld ldc; W1, W2: jb x n scratch at ld NB), the three
 // products in one call so that the register save and restore of a call is paid once
@@ -711,8 +921,7 @@ __device__ __noinline__ void wg_reflect(const double* V, int ldv, const double* 
 __device__ __forceinline__ void gemm64(int M, int Nc, int K, const double* A, int lda, bool tA, const double* B,
                                        int ldb, bool tB, double* C, int ldc, const double* rs, const double* ms,
                                        const double* cs, double beta, Lds&, double alpha = 1.0) {
-    Lds& s = g_lds;
-    wg_gemm_mfma<false, false, false>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, s, alpha);
+    gemm64_global(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, alpha);
 }
 __device__ __forceinline__ void gemm64_lg(int M, int Nc, int K, const double* A, int lda, bool tA, const double* B,
                                           int ldb, bool tB, double* C, int ldc, double beta, Lds&, double alpha = 1.0) {
@@ -740,7 +949,7 @@ __device__ __forceinline__ void gemmB(Desc d, bool wrap, int M, int Nc, int K, c
     Lds& s = g_lds;
     if (lowp_product(d, wrap))
         wg_gemm_mfma<true, false, false>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, 0.0, s);
-    else wg_gemm_mfma<false, false, false>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, 0.0, s);
+    else gemm64_global(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, 0.0, 1.0);
 }
 
 // Elementwise streams over e in [0, n): st(e, ld(e)), with U loads of each
@@ -3272,7 +3481,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1)))
     for (int slot = 0; slot < d.L; ++slot) wg_measure_from(d, d.recG + slot * slot_size, d.recSign[slot], s);
 }
 
-// one workgroup runs one GEMM variant (0 vector FP64, 1 MFMA FP64, 2 MFMA FP32)
+// one workgroup runs one GEMM variant (0 vector FP64, 1 MFMA FP64, 2 MFMA FP32, 3/4 MFMA FP64 with A / A and B
+// staged in LDS, 5/6 MFMA FP64 on wide 128x128 / 160x160 tiles)
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) k_gemm(int variant, int M, int Nc, int K, const double* A, int lda, int tA,
                                              const double* B, int ldb, int tB, double* C, int ldc, const double* rs,
                                              const double* ms, const double* cs, double beta) {
@@ -3293,7 +3503,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1)))
     }
     // variants 3 and 4 stage A (and B) in LDS first, as the engine's in-LDS operands are
     const size_t na = (size_t)M * K, nb = (size_t)K * Nc;
-    if (variant >= 3) {
+    if (variant == 3 || variant == 4) {
         for (size_t e = tid(); e < na; e += NT) s.panel[e] = A[e];
         A = s.panel;
     }
@@ -3305,7 +3515,9 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1)))
     if (variant == 1) wg_gemm_mfma<false, false, false>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, s);
     else if (variant == 2) wg_gemm_mfma<true, false, false>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, s);
     else if (variant == 3) wg_gemm_mfma<false, true, false>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, s);
-    else wg_gemm_mfma<false, true, true>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, s);
+    else if (variant == 4) wg_gemm_mfma<false, true, true>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, s);
+    else if (variant == 5) wg_gemm_wide<4, 4, 16>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, 1.0);
+    else wg_gemm_wide<5, 5, 8>(M, Nc, K, A, lda, tA, B, ldb, tB, C, ldc, rs, ms, cs, beta, 1.0);
 }
 
 // The single-flip update of the equal-time Green's functions after the field at

@@ -783,7 +783,7 @@ int dqmc_num_kstars(dqmc_batch_t b, int c, double* kxy) {
 
 int dqmc_k_gemm(dqmc_ctx_t ctx, int variant, int M, int Nc, int K, int tA, int tB, const double* A, const double* B,
                 double* C, double beta, const double* rs, const double* ms, const double* cs) {
-    if (!ctx || !A || !B || !C || M < 1 || Nc < 1 || K < 1 || variant < 0 || variant > 4) return DQMC_ERR_ARG;
+    if (!ctx || !A || !B || !C || M < 1 || Nc < 1 || K < 1 || variant < 0 || variant > 6) return DQMC_ERR_ARG;
     // the scalings are staged in LDS
     if (variant >= 1 && ((ms && K > DQMC_NMAX) || (rs && M > DQMC_NMAX) || (cs && Nc > DQMC_NMAX))) return DQMC_ERR_ARG;
     if (variant == 3 && (size_t)M * K > DQMC_PANEL_DOUBLES) return DQMC_ERR_ARG;

@@ -17,7 +17,7 @@ def ctx():
     return D.Context(0)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 5, 6])
 @pytest.mark.parametrize("shape", SHAPES, ids=[f"{m}x{n}x{k}" for m, n, k in SHAPES])
 @pytest.mark.parametrize("tA,tB", [(False, False), (True, False), (False, True)])
 def test_gemm_variants(ctx, variant, shape, tA, tB):
@@ -62,6 +62,31 @@ def test_gemm_lds_operands(ctx, variant, shape, tA, tB):
     got = ctx.gemm(variant, A, B, C0, transA=tA, transB=tB, beta=-1.5, ms=ms)
     scale = np.abs(opA) @ np.diag(ms) @ np.abs(opB) + 1.5 * np.abs(C0)
     assert np.max(np.abs(got - ref) / scale) < 1e-14 * K
+
+
+WIDE_SHAPES = [(144, 144, 144), (256, 256, 256), (100, 121, 144), (160, 160, 37), (161, 150, 200), (400, 400, 400),
+               (196, 196, 196), (17, 300, 5), (129, 128, 16)]
+
+
+@pytest.mark.parametrize("variant", [5, 6])
+@pytest.mark.parametrize("shape", WIDE_SHAPES, ids=[f"{m}x{n}x{k}" for m, n, k in WIDE_SHAPES])
+@pytest.mark.parametrize("tA,tB", [(False, False), (True, False), (False, True), (True, True)])
+@pytest.mark.parametrize("beta", [0.0, -0.75])
+def test_wide_gemm_equals_tiled(ctx, variant, shape, tA, tB, beta):
+    """The wide-tile GEMM sums every output element in the same order as the 64 x 64 tiles and
+    scales it the same way, so the engine may use either: the results are equal bit for bit
+    (with and without scalings, transposes and a C term)."""
+    print(f"[gpu-gemm] wide variant {variant} shape {shape} tA {tA} tB {tB} beta {beta}", flush=True)
+    M, Nc, K = shape
+    rng = np.random.default_rng(M * 13 + Nc * 5 + K + variant)
+    A = rng.standard_normal((K, M) if tA else (M, K))
+    B = rng.standard_normal((Nc, K) if tB else (K, Nc))
+    C0 = rng.standard_normal((M, Nc))
+    for scal in (False, True):
+        kw = dict(rs=rng.uniform(0.5, 2, M), ms=rng.uniform(0.5, 2, K), cs=rng.uniform(0.5, 2, Nc)) if scal else {}
+        ref = ctx.gemm(1, A, B, C0, transA=tA, transB=tB, beta=beta, **kw)
+        got = ctx.gemm(variant, A, B, C0, transA=tA, transB=tB, beta=beta, **kw)
+        assert np.array_equal(got, ref), np.max(np.abs(got - ref))
 
 
 def test_gemm_plain(ctx):
