@@ -2300,54 +2300,77 @@ __device__ __noinline__ void block_stage_w(const double* G_, int N, int b0, int 
 
 // block_apply:  G += ACP W on wavefronts wave_lo..3 (no workgroup barrier inside, so it can run
 // beside the walk of wavefront 0).  The product is formed transposed (W^T ACP^T), so 16 lanes
-// run down one column of G; 4 blocks per wavefront advance together with all their loads in flight.
+// run down one column of G.  A wavefront takes 2x2 groups of 16x16 blocks (block rows 2gr, 2gr+1,
+// columns 2gc, 2gc+1), so each depth step reads two ACP and two W operands for four MFMAs, and
+// the depth loop is unrolled for the block's padded depth K4 (a template parameter), which keeps
+// the operand reads ahead of the matrix cores; the G loads of the group are in flight together.
+// Each block sums its K4/4 MFMAs in depth order whatever the grouping.
+template <int K4>
+__device__ __forceinline__ void block_apply_k(AS_G double* G, int N, int wave_lo, const AS_L double* acp,
+                                              const AS_L double* gsw) {
+    const int l = lane(), w = wave();
+    const int nw = 4 - wave_lo, wi = w - wave_lo;
+    const int lr = l & 15, lk = l >> 4;
+    const int nbr = (N + 15) >> 4, ngr = (nbr + 1) >> 1, ng = ngr * ngr;
+    const FastDiv dg(ngr);
+    for (int g = wi; g < ng; g += nw) {
+        const int gc = dg.div(g), gr = g - gc * ngr;
+        int oa[2], ob[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            oa[a] = min((2 * gr + a) * 16 + lr, N - 1);   // (rows and columns past N repeat the last; not stored)
+            ob[a] = min((2 * gc + a) * 16 + lr, N - 1);
+        }
+        double cv[2][2][4];
+        dbl4 acc[2][2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) cv[a][b][i] = G[(size_t)min((2 * gc + b) * 16 + lk + 4 * i, N - 1) * N + oa[a]];
+                acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+            }
+#pragma unroll
+        for (int kk = 0; kk < K4; kk += 4) {
+            const int kr = (kk + lk) * N;
+            const double a0 = acp[kr + oa[0]], a1 = acp[kr + oa[1]];
+            const double b0 = gsw[kr + ob[0]], b1 = gsw[kr + ob[1]];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int row = (2 * gr + a) * 16 + lr;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int col = (2 * gc + b) * 16 + lk + 4 * i;
+                    if (row < N && col < N) G[(size_t)col * N + row] = cv[a][b][i] + acc[a][b][i];
+                }
+            }
+    }
+}
 __device__ __noinline__ void block_apply(double* G_, int N, int k, int wave_lo, Lds&) {
     Lds& s = g_lds;
     N = uni(N); k = uni(k); wave_lo = uni(wave_lo);
     AS_G double* G = gptr(uni(G_));
     const int k4 = (k + 3) & ~3;
-    const int l = lane(), w = wave();
-    if (w < wave_lo) return;
-    const int nw = 4 - wave_lo, wi = w - wave_lo;
-    const double* const acp = s.panel;
-    const double* const gsw = acp + (size_t)N * k4;
-    const int lr = l & 15, lk = l >> 4;
-    const int nbr = (N + 15) >> 4, nblk = nbr * nbr;
-    constexpr int CH = 4;
-    for (int q0 = wi; q0 < nblk; q0 += nw * CH) {
-        double cv[CH][4];
-        dbl4 acc[CH];
-        int oa[CH], ob[CH];
-#pragma unroll
-        for (int u = 0; u < CH; ++u) {
-            const int q = min(q0 + nw * u, nblk - 1);   // (a block past the end repeats the last; not stored)
-            const int rb = q % nbr, cb = q / nbr;
-            oa[u] = rb * 16 + lr;
-            ob[u] = cb * 16 + lr;
-            const int og = min(rb * 16 + lr, N - 1);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) cv[u][i] = G[(size_t)min(cb * 16 + lk + 4 * i, N - 1) * N + og];
-            acc[u] = dbl4{0.0, 0.0, 0.0, 0.0};
-        }
-        for (int kk = 0; kk < k4; kk += 4) {
-            const int kr = (kk + lk) * N;
-            double av[CH], bv[CH];
-#pragma unroll
-            for (int u = 0; u < CH; ++u) { av[u] = acp[kr + oa[u]]; bv[u] = gsw[kr + ob[u]]; }
-#pragma unroll
-            for (int u = 0; u < CH; ++u) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[u], av[u], acc[u], 0, 0, 0);
-        }
-#pragma unroll
-        for (int u = 0; u < CH; ++u) {
-            const int q = q0 + nw * u;
-            const int rb = q % nbr, cb = q / nbr;
-            const int row = rb * 16 + lr;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int col = cb * 16 + lk + 4 * i;
-                if (q < nblk && row < N && col < N) G[(size_t)col * N + row] = cv[u][i] + acc[u][i];
-            }
-        }
+    if (wave() < wave_lo) return;
+    const AS_L double* const acp = (const AS_L double*)s.panel;
+    const AS_L double* const gsw = acp + (size_t)N * k4;
+    switch (k4) {
+        case 4: block_apply_k<4>(G, N, wave_lo, acp, gsw); break;
+        case 8: block_apply_k<8>(G, N, wave_lo, acp, gsw); break;
+        case 12: block_apply_k<12>(G, N, wave_lo, acp, gsw); break;
+        case 16: block_apply_k<16>(G, N, wave_lo, acp, gsw); break;
+        case 20: block_apply_k<20>(G, N, wave_lo, acp, gsw); break;
+        case 24: block_apply_k<24>(G, N, wave_lo, acp, gsw); break;
+        case 28: block_apply_k<28>(G, N, wave_lo, acp, gsw); break;
+        default: block_apply_k<32>(G, N, wave_lo, acp, gsw); break;
     }
 }
 
