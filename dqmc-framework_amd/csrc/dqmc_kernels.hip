@@ -2229,27 +2229,13 @@ __device__ void stage_gss(const double* G_, int N, int b0, int k, int sp, Lds&) 
               [&](int q, double v) { const int c = dk.div(q), a = q - c * k; gr[c * 64 + a] = v; gt[a * 64 + c] = v; });
 }
 
-// The block's update of G for one spin, on the matrix cores with operands in LDS.
-// block_stage_w:  ACP = G0[:,S], GSW = G0[S,:], then W = C (G0[S,:] - E_S) over GSW
-__device__ __noinline__ void block_stage_w(const double* G_, int N, int b0, int k, int sp, Lds&) {
-    Lds& s = g_lds;
-    N = uni(N); b0 = uni(b0); k = uni(k); sp = uni(sp);
-    const AS_G double* G = gptr(uni(G_));
-    const int k4 = (k + 3) & ~3;
-    const FastDiv dk4(k4);
+// W = C (G0[S,:] - E_S) over GSW in place, for a block of padded depth K4: a wavefront advances up
+// to 4 of its column blocks together; all loads are unconditional (clamped indices) so the
+// compiler keeps them in flight.  Rows 16..31 of C are zero when K4 <= 16 (not multiplied).
+template <int K4>
+__device__ __forceinline__ void stage_w_product(int N, int b0, int k, const double* cr, double* gsw) {
+    constexpr bool hi = K4 > 16;
     const int l = lane(), w = wave();
-    double* const acp = s.panel;
-    double* const gsw = acp + (size_t)N * k4;
-    const double* const cr = MP_CR(sp);
-    // ACP and GSW together (GSW element e = c k4 + b: consecutive threads read down a column of G)
-    wg_stream2(N * k4, [&](int e) { return G[(size_t)b0 * N + min(e, N * k - 1)]; },
-               [&](int e, double v) { acp[e] = e < N * k ? v : 0.0; },
-               [&](int e) { const int c = dk4.div(e), b = e - c * k4; return G[(size_t)c * N + b0 + min(b, k - 1)]; },
-               [&](int e, double v) { const int c = dk4.div(e), b = e - c * k4; gsw[b * N + c] = b < k ? v : 0.0; });
-    __syncthreads();
-    // W = C G0[S,:] - C E_S, column block by column block (each wavefront owns whole column blocks,
-    // so W may overwrite GSW in place); a wavefront advances up to 4 of its column blocks together.
-    // All loads are unconditional (clamped indices) so the compiler can keep them in flight.
     const int nbc = (N + 15) >> 4;
     const int lr = l & 15, lk = l >> 4;
     for (int j0 = w; j0 < nbc; j0 += 16) {
@@ -2261,19 +2247,20 @@ __device__ __noinline__ void block_stage_w(const double* G_, int N, int b0, int 
             acc1[u] = dbl4{0.0, 0.0, 0.0, 0.0};
             colq[u] = min((j0 + 4 * u) * 16 + lr, N - 1);   // a column past N repeats column N-1; not stored
         }
-        for (int kk = 0; kk < k4; kk += 4) {
+#pragma unroll
+        for (int kk = 0; kk < K4; kk += 4) {
             const int kr = kk + lk;
-            const double a0 = cr[lr * KP + kr], a1 = cr[(16 + lr) * KP + kr];
+            const double a0 = cr[lr * KP + kr];
+            const double a1 = hi ? cr[(16 + lr) * KP + kr] : 0.0;
             double bv[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) bv[u] = gsw[kr * N + colq[u]];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 acc0[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv[u], acc0[u], 0, 0, 0);
-                acc1[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv[u], acc1[u], 0, 0, 0);
+                if (hi) acc1[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv[u], acc1[u], 0, 0, 0);
             }
         }
-        // rows 16..31 of C are zero when k <= 16, so acc1 is then zero and not stored
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int col = (j0 + 4 * u) * 16 + lr;
@@ -2285,15 +2272,46 @@ __device__ __noinline__ void block_stage_w(const double* G_, int N, int b0, int 
             for (int i = 0; i < 4; ++i) {
                 const int a = lk + 4 * i;
                 corr0[i] = cr[a * KP + cs];
-                corr1[i] = cr[(16 + a) * KP + cs];
+                corr1[i] = hi ? cr[(16 + a) * KP + cs] : 0.0;
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int a = lk + 4 * i;
-                if (colok && a < k4) gsw[a * N + col] = acc0[u][i] - ((in_s && a < k) ? corr0[i] : 0.0);
-                if (colok && 16 + a < k4) gsw[(16 + a) * N + col] = acc1[u][i] - ((in_s && 16 + a < k) ? corr1[i] : 0.0);
+                if (colok && a < K4) gsw[a * N + col] = acc0[u][i] - ((in_s && a < k) ? corr0[i] : 0.0);
+                if (hi && colok && 16 + a < K4) gsw[(16 + a) * N + col] = acc1[u][i] - ((in_s && 16 + a < k) ? corr1[i] : 0.0);
             }
         }
+    }
+}
+
+// The block's update of G for one spin, on the matrix cores with operands in LDS.
+// block_stage_w:  ACP = G0[:,S], GSW = G0[S,:], then W = C (G0[S,:] - E_S) over GSW
+__device__ __noinline__ void block_stage_w(const double* G_, int N, int b0, int k, int sp, Lds&) {
+    Lds& s = g_lds;
+    N = uni(N); b0 = uni(b0); k = uni(k); sp = uni(sp);
+    const AS_G double* G = gptr(uni(G_));
+    const int k4 = (k + 3) & ~3;
+    const FastDiv dk4(k4);
+    double* const acp = s.panel;
+    double* const gsw = acp + (size_t)N * k4;
+    const double* const cr = MP_CR(sp);
+    // ACP and GSW together (GSW element e = c k4 + b: consecutive threads read down a column of G)
+    wg_stream2(N * k4, [&](int e) { return G[(size_t)b0 * N + min(e, N * k - 1)]; },
+               [&](int e, double v) { acp[e] = e < N * k ? v : 0.0; },
+               [&](int e) { const int c = dk4.div(e), b = e - c * k4; return G[(size_t)c * N + b0 + min(b, k - 1)]; },
+               [&](int e, double v) { const int c = dk4.div(e), b = e - c * k4; gsw[b * N + c] = b < k ? v : 0.0; });
+    __syncthreads();
+    // W = C G0[S,:] - C E_S, column block by column block (each wavefront owns whole column blocks,
+    // so W may overwrite GSW in place), with the depth loop unrolled for the padded depth k4
+    switch (k4) {
+        case 4: stage_w_product<4>(N, b0, k, cr, gsw); break;
+        case 8: stage_w_product<8>(N, b0, k, cr, gsw); break;
+        case 12: stage_w_product<12>(N, b0, k, cr, gsw); break;
+        case 16: stage_w_product<16>(N, b0, k, cr, gsw); break;
+        case 20: stage_w_product<20>(N, b0, k, cr, gsw); break;
+        case 24: stage_w_product<24>(N, b0, k, cr, gsw); break;
+        case 28: stage_w_product<28>(N, b0, k, cr, gsw); break;
+        default: stage_w_product<32>(N, b0, k, cr, gsw); break;
     }
     __syncthreads();
 }
@@ -2390,11 +2408,28 @@ __device__ __noinline__ void gss_from_factors(const double* G_, int N, int k, in
     for (int q = tid(); q < KB * KB; q += NT) gr[(q >> 5) * 64 + (q & 31)] = 0.0;
     __syncthreads();
     const FastDiv dk1(k1);
-    for (int q = tid(); q < k1 * k1; q += NT) {
-        const int c = dk1.div(q), a = q - c * k1;
-        double v = G[(size_t)(b1 + c) * N + b1 + a];
-        for (int j = 0; j < k4; ++j) v += acp[j * N + b1 + a] * gsw[j * N + b1 + c];
-        gr[c * 64 + a] = v;
+    // the (at most 4) elements of a thread advance together: four independent sums per depth step
+    constexpr int U = KB * KB / NT;
+    for (int q0 = tid(); q0 < k1 * k1; q0 += U * NT) {
+        int oa[U], oc[U];
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = min(q0 + u * NT, k1 * k1 - 1);   // (clamped; not stored)
+            const int c = dk1.div(q), a = q - c * k1;
+            oa[u] = b1 + a;
+            oc[u] = b1 + c;
+            v[u] = G[(size_t)(b1 + c) * N + b1 + a];
+        }
+        for (int j = 0; j < k4; ++j) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] += acp[j * N + oa[u]] * gsw[j * N + oc[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = q0 + u * NT;
+            if (q < k1 * k1) gr[(oc[u] - b1) * 64 + oa[u] - b1] = v[u];
+        }
     }
 }
 
