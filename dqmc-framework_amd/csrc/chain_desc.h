@@ -105,4 +105,9 @@ struct ChainDesc {
     double* recG0t;      // L x nspin x N*N (RECORD): G(0,t)
     double* xbuf;        // DQMC_PAIR_XBUF_DOUBLES (repulsive chains): spin-pair exchange blocks
     unsigned* xflag;     // DQMC_PAIR_FLAG_WORDS (repulsive chains): spin-pair flags, zeroed before a pair launch
+    // dense chains whose exp(K) factors over the two lattice directions (side <= 16):
+    // exp(-dtau K)[(x,y),(x',y')] = X[x][x'] Y[y][y'], site x + side y; four 16 x 16 column-major
+    // matrices, zero padded: X, Y of exp(K), then X, Y of exp(K)^-1.  Null: dense N x N products.
+    const double* kron;
 };
+#define DQMC_KRON_MAX_SIDE 16

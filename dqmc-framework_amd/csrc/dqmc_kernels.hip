@@ -1238,6 +1238,174 @@ __device__ __forceinline__ const double* vneg(Desc d, int sp, Lds&) {
 // B(t) products on an N x N matrix M (in place; W is N*N scratch for the dense path).
 // op: 0 B_left, 1 B_right, 2 invB_left, 3 invB_right, 4 transB_left
 // (wrap: the product propagates G, which selects its arithmetic in the FP32 modes)
+// ---------------------------------------------------------------------------
+// Dense B products in Kronecker form.  On the periodic square lattice the
+// hopping matrix is a sum over the two directions, K = Kx (+) Ky, so
+// F = exp(-s K) factors: F[(x,y),(x',y')] = X[x][x'] Y[y][y'] (site x + side y; the
+// chemical potential is a scalar that the host folds into X).  A column v of M,
+// read as the side x side matrix V[x'][y'] = v[x' + side y'], maps to
+// (F v) = X V Y^T: two 16 x 16 x 16 products (8 MFMAs, factors zero padded to 16)
+// instead of a length-N dot product per element, which cuts an N x N x N
+// product by a factor N / (2 side) in work.  The host takes X and Y from the
+// chain's exp(K) itself and uses this form only when X (x) Y reproduces every
+// entry of it within 1e-13 of its largest, so the products equal the dense
+// ones to rounding.
+// MFMA layouts (16x16x4 FP64): A operand lane l holds A[l%16][4s + l/16], B operand
+// B[4s + l/16][l%16], result element e of lane l is D[l/16 + 4e][l%16], so a result
+// is the B operand of the next product directly (k step s = element s).
+// ---------------------------------------------------------------------------
+// fragment of a padded 16 x 16 factor for depth step s: as an A operand (row l%16, depth 4s + l/16)
+// when tr is false, or of its transpose; the same index gives the B operand of the transpose
+__device__ __forceinline__ double kron_frag(const AS_G double* F, int s, bool tr) {
+    const int l = lane();
+    return tr ? F[(l & 15) * 16 + 4 * s + (l >> 4)] : F[(4 * s + (l >> 4)) * 16 + (l & 15)];
+}
+
+// W = diag(rpost) op(F) diag(rpre) M, op(F) = X (x) Y or its transpose; one wavefront per column
+// (columns w, w + 4, ...; four of them in flight together).  rpre / rpost live in LDS or are null.
+__device__ __noinline__ void kron_left(const double* M_, double* W_, int N, int side, const double* X_,
+                                       const double* Y_, bool tr, const double* rpre_, const double* rpost_) {
+    N = uni(N); side = uni(side); tr = uni(tr);
+    const AS_G double* const M = gptr(uni(M_));
+    AS_G double* const W = gptr(uni(W_));
+    const AS_G double* const X = gptr(uni(X_));
+    const AS_G double* const Y = gptr(uni(Y_));
+    rpre_ = uni(rpre_); rpost_ = uni(rpost_);
+    const bool has_pre = rpre_ != nullptr, has_post = rpost_ != nullptr;
+    const AS_L double* const rpre = (const AS_L double*)rpre_;
+    const AS_L double* const rpost = (const AS_L double*)rpost_;
+    const int l = lane(), w = wave();
+    const int i = l & 15, kq = l >> 4;
+    // step 1, U = V Y'^T: B operand (Y'^T)[4s + l/16][l%16] = Y'[l%16][4s + l/16]; step 2, W = X' U: A operand X'
+    double xf[4], yf[4];
+#pragma unroll
+    for (int s4 = 0; s4 < 4; ++s4) { xf[s4] = kron_frag(X, s4, tr); yf[s4] = kron_frag(Y, s4, tr); }
+    // this lane's element of V for depth step s: V[i][4s + kq] = v[i + side (4s + kq)]
+    int vo[4];
+    bool vok[4];
+#pragma unroll
+    for (int s4 = 0; s4 < 4; ++s4) {
+        vok[s4] = i < side && 4 * s4 + kq < side;
+        vo[s4] = vok[s4] ? i + side * (4 * s4 + kq) : 0;
+    }
+    double vpre[4];
+#pragma unroll
+    for (int s4 = 0; s4 < 4; ++s4) vpre[s4] = has_pre ? rpre[vo[s4]] : 1.0;
+    // this lane's output sites: W[x = kq + 4e][y = i] at x + side y
+    int wo[4];
+    bool wok[4];
+    double wpost[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int x = kq + 4 * e;
+        wok[e] = x < side && i < side;
+        wo[e] = wok[e] ? x + side * i : 0;
+        wpost[e] = has_post ? rpost[wo[e]] : 1.0;
+    }
+    constexpr int CH = 4;
+    for (int j0 = w; j0 < N; j0 += 4 * CH) {
+        double v[CH][4];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            const int j = min(j0 + 4 * c, N - 1);   // (a column past N repeats the last; not stored)
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) v[c][s4] = M[(size_t)j * N + vo[s4]];
+        }
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            dbl4 u = dbl4{0.0, 0.0, 0.0, 0.0}, r = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                const double a = vok[s4] ? (has_pre ? v[c][s4] * vpre[s4] : v[c][s4]) : 0.0;
+                u = __builtin_amdgcn_mfma_f64_16x16x4f64(a, yf[s4], u, 0, 0, 0);
+            }
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) r = __builtin_amdgcn_mfma_f64_16x16x4f64(xf[s4], u[s4], r, 0, 0, 0);
+            const int j = j0 + 4 * c;
+            if (j < N) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (wok[e]) W[(size_t)j * N + wo[e]] = has_post ? r[e] * wpost[e] : r[e];
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// W = M diag(cpre) (X (x) Y) diag(cpost), by blocks of 16 rows of M: T = M_R (X (x) 1) from global
+// memory into LDS (wavefront w takes the column groups y = w, w + 4, ...), then W_R = T (1 (x) Y)
+// (wavefront w takes x' = w, w + 4, ...) into LDS, written out along columns.  Row r of a block
+// block sits at stride LDR = 17 doubles in LDS (16 rows plus one, so a wavefront's lanes that
+// read one row at different columns fall on different banks).
+#define KRON_LDR 17
+__device__ __noinline__ void kron_right(const double* M_, double* W_, int N, int side, const double* X_,
+                                        const double* Y_, const double* cpre_, const double* cpost_, Lds&) {
+    Lds& s = g_lds;
+    N = uni(N); side = uni(side);
+    const AS_G double* const M = gptr(uni(M_));
+    AS_G double* const W = gptr(uni(W_));
+    const AS_G double* const X = gptr(uni(X_));
+    const AS_G double* const Y = gptr(uni(Y_));
+    cpre_ = uni(cpre_); cpost_ = uni(cpost_);
+    const bool has_pre = cpre_ != nullptr, has_post = cpost_ != nullptr;
+    const AS_L double* const cpre = (const AS_L double*)cpre_;
+    const AS_L double* const cpost = (const AS_L double*)cpost_;
+    AS_L double* const T = (AS_L double*)s.panel;            // 16 rows x N, row stride... column c at c LDR
+    AS_L double* const Wl = T + (size_t)N * KRON_LDR;
+    const int l = lane(), w = wave();
+    const int i = l & 15, kq = l >> 4;
+    // B operands: X[4s + l/16][l%16] (pass 1) and Y[4s + l/16][l%16] (pass 2)
+    double xb[4], yb[4];
+#pragma unroll
+    for (int s4 = 0; s4 < 4; ++s4) { xb[s4] = kron_frag(X, s4, true); yb[s4] = kron_frag(Y, s4, true); }
+    const bool dok[4] = {4 * 0 + kq < side, 4 * 1 + kq < side, 4 * 2 + kq < side, 4 * 3 + kq < side};
+    for (int r0 = 0; r0 < N; r0 += 16) {
+        const int row = min(r0 + i, N - 1);   // (rows past N repeat the last; not stored)
+        // pass 1: T[r][(x', y)] = sum_x M[r][(x, y)] cpre[(x, y)] X[x][x']
+        for (int y = w; y < side; y += 4) {
+            double a[4];
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                const int c = dok[s4] ? 4 * s4 + kq + side * y : 0;
+                const double v = M[(size_t)c * N + row];
+                a[s4] = dok[s4] ? (has_pre ? v * cpre[c] : v) : 0.0;
+            }
+            dbl4 t = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) t = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s4], xb[s4], t, 0, 0, 0);
+            // element e: T[r0 + kq + 4e][(x' = i, y)]
+            if (i < side) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) T[(i + side * y) * KRON_LDR + kq + 4 * e] = t[e];
+            }
+        }
+        __syncthreads();
+        // pass 2: W[r][(x', y')] = sum_y T[r][(x', y)] Y[y][y'] cpost[(x', y')]
+        for (int xp = w; xp < side; xp += 4) {
+            double a[4];
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) a[s4] = dok[s4] ? T[(xp + side * (4 * s4 + kq)) * KRON_LDR + i] : 0.0;
+            dbl4 r = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) r = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s4], yb[s4], r, 0, 0, 0);
+            // element e: W[r0 + kq + 4e][(x', y' = i)]
+            if (i < side) {
+                const int c = xp + side * i;
+                const double cp = has_post ? cpost[c] : 1.0;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) Wl[c * KRON_LDR + kq + 4 * e] = has_post ? r[e] * cp : r[e];
+            }
+        }
+        __syncthreads();
+        const int nr = min(16, N - r0);
+        for (int q = tid(); q < 16 * N; q += NT) {
+            const int c = q >> 4, rr = q & 15;
+            if (rr < nr) W[(size_t)c * N + r0 + rr] = Wl[c * KRON_LDR + rr];
+        }
+    }
+    __syncthreads();
+}
+
 __device__ void wg_bmul(Desc d, int op, int sp, double* M, double* W, Lds&, bool wrap = false) {
     Lds& s = g_lds;
     const int N = d.N;
@@ -1251,6 +1419,22 @@ __device__ void wg_bmul(Desc d, int op, int sp, double* M, double* W, Lds&, bool
             case 3: wg_cb_right_fused(M, N, d.side, d.Pi, 1, 0, nullptr, en, s); break;
             case 4: wg_cb_left_fused(M, N, d.side, d.P, 0, 1, ep, nullptr, s); break;
         }
+        return;
+    }
+    if (d.kron && !lowp_product(d, wrap)) {
+        // X, Y of exp(K) at kron, of exp(K)^-1 at kron + 512
+        const double* kx = d.kron;
+        const double* ky = d.kron + 256;
+        const double* ix = d.kron + 512;
+        const double* iy = d.kron + 768;
+        switch (op) {
+            case 0: kron_left(M, W, N, d.side, kx, ky, false, nullptr, ep); break;
+            case 1: kron_right(M, W, N, d.side, kx, ky, ep, nullptr, s); break;
+            case 2: kron_left(M, W, N, d.side, ix, iy, false, en, nullptr); break;
+            case 3: kron_right(M, W, N, d.side, ix, iy, nullptr, en, s); break;
+            case 4: kron_left(M, W, N, d.side, kx, ky, true, ep, nullptr); break;
+        }
+        wg_copy(M, W, N * N);
         return;
     }
     switch (op) {
@@ -1270,6 +1454,20 @@ __device__ void wg_wrap(Desc d, int sp, bool up, double* G, double* W, Lds&) {
     if (!d.cb) {
         const double* ep = vpos(d, sp, s);
         const double* en = vneg(d, sp, s);
+        if (d.kron && !lowp_product(d, true)) {
+            const double* kx = d.kron;
+            const double* ky = d.kron + 256;
+            const double* ix = d.kron + 512;
+            const double* iy = d.kron + 768;
+            if (up) {   // G <- diag(ep) F G F^-1 diag(en)
+                kron_left(G, W, N, d.side, kx, ky, false, nullptr, ep);
+                kron_right(W, G, N, d.side, ix, iy, nullptr, en, s);
+            } else {    // G <- F^-1 diag(en) G diag(ep) F
+                kron_right(G, W, N, d.side, kx, ky, ep, nullptr, s);
+                kron_left(W, G, N, d.side, ix, iy, false, en, nullptr);
+            }
+            return;
+        }
         if (up) {
             gemmB(d, true, N, N, N, d.expK, N, false, G, N, false, W, N, ep, nullptr, nullptr, s);
             gemmB(d, true, N, N, N, W, N, false, d.invExpK, N, false, G, N, nullptr, nullptr, en, s);

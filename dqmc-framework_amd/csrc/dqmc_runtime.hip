@@ -3,6 +3,7 @@
 // checkerboard and HS constants, random fields) and the C-ABI of dqmc_amd.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -65,7 +66,43 @@ void expk_closed_form(int side, double t, double mu, double s, std::vector<doubl
         }
 }
 
+// Kronecker factors of a dense exp(K) on the side x side lattice (site x + side y):
+// F[(x,y),(x',y')] = X[x][x'] Y[y][y'] with X the block y = y' = 0 of F and Y the entries
+// x = x' = 0 divided by F(0,0), both written 16 x 16 column-major and zero padded.  They
+// are accepted when X (x) Y reproduces every entry of F within 1e-13 of its largest (true
+// of the periodic square lattice, whose hopping is a sum over the two directions).
+bool kron_factors(int side, const std::vector<double>& F, double* X, double* Y) {
+    if (side > DQMC_KRON_MAX_SIDE) return false;
+    const int N = side * side;
+    const double f00 = F[0];
+    if (!(std::fabs(f00) > 0.0)) return false;
+    std::fill(X, X + 256, 0.0);
+    std::fill(Y, Y + 256, 0.0);
+    for (int a = 0; a < side; ++a)
+        for (int b = 0; b < side; ++b) {
+            X[b * 16 + a] = F[(size_t)b * N + a];                             // F[(a,0),(b,0)]
+            Y[b * 16 + a] = F[(size_t)(side * b) * N + side * a] / f00;       // F[(0,a),(0,b)] / F(0,0)
+        }
+    double fmax = 0.0, err = 0.0;
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < N; ++i) {
+            const double f = F[(size_t)j * N + i];
+            fmax = std::max(fmax, std::fabs(f));
+            const double p = X[(j % side) * 16 + i % side] * Y[(j / side) * 16 + i / side];
+            err = std::max(err, std::fabs(p - f));
+        }
+    return err <= 1e-13 * fmax;
+}
 
+// the chain's Kronecker factors (of expK and invExpK), or none
+void kron_setup(HostChain& h) {
+    h.kron.clear();
+    if (h.p.checkerboard) return;
+    std::vector<double> k(4 * 256);
+    if (kron_factors(h.p.side, h.expK, k.data(), k.data() + 256) &&
+        kron_factors(h.p.side, h.invExpK, k.data() + 512, k.data() + 768))
+        h.kron.swap(k);
+}
 
 // ---------------------------------------------------------------------------
 // Scratch of dqmc_k_greens_recompute_noalloc: one file-scope device array (each device holds its
@@ -123,6 +160,7 @@ Layout chain_layout(int N, int L, int n_stack, int nspin, int kdelay, bool dense
     l.Wq = take((size_t)nspin * 3 * 16 * (N + 16) * sizeof(double));
     l.expK = dense ? take(n2) : o;
     l.invExpK = dense ? take(n2) : o;
+    l.kron = dense ? take(4 * 256 * sizeof(double)) : o;
     l.cosq = take(N * sizeof(double));
     l.field = take((size_t)L * N);
     l.st = take(sizeof(ChainState));
@@ -241,6 +279,7 @@ int batch_prepare(dqmc_context* ctx, int n_chains, const dqmc_chain_params* para
             expk_closed_form(p.side, p.hopping_t, p.chemical_potential, h.dtau, h.expK);
             expk_closed_form(p.side, p.hopping_t, p.chemical_potential, -h.dtau, h.invExpK);
         }
+        kron_setup(h);
         // Re exp(-i Q r_d) of the measured momentum (src/lattice/square.cpp:42-66, :148-161)
         const int side = p.side;
         std::vector<double> kx, ky;
@@ -319,6 +358,7 @@ int batch_bind(dqmc_batch* b) {
         d.W = (double*)(base + l.W); d.Wq = (double*)(base + l.Wq);
         d.expK = p.checkerboard ? nullptr : (const double*)(base + l.expK);
         d.invExpK = p.checkerboard ? nullptr : (const double*)(base + l.invExpK);
+        d.kron = h.kron.empty() ? nullptr : (const double*)(base + l.kron);
         d.cosq = (const double*)(base + l.cosq);
         d.field = (int8_t*)(base + l.field);
         d.st = (ChainState*)(base + l.st);
@@ -338,6 +378,7 @@ int batch_bind(dqmc_batch* b) {
         if (!p.checkerboard) {
             HIPCHK(ctx, hipMemcpyAsync((void*)d.expK, h.expK.data(), h.expK.size() * 8, hipMemcpyHostToDevice, b->stream));
             HIPCHK(ctx, hipMemcpyAsync((void*)d.invExpK, h.invExpK.data(), h.invExpK.size() * 8, hipMemcpyHostToDevice, b->stream));
+            if (d.kron) HIPCHK(ctx, hipMemcpyAsync((void*)d.kron, h.kron.data(), h.kron.size() * 8, hipMemcpyHostToDevice, b->stream));
         }
         HIPCHK(ctx, hipMemcpyAsync((void*)d.cosq, h.cosq.data(), h.cosq.size() * 8, hipMemcpyHostToDevice, b->stream));
         HIPCHK(ctx, hipMemcpyAsync((void*)d.rhoq, h.rhoq.data(), h.rhoq.size() * 8, hipMemcpyHostToDevice, b->stream));
@@ -477,8 +518,13 @@ int dqmc_set_expk(dqmc_batch_t b, int c, const double* expK, const double* invEx
         if (!is_finite(expK[e]) || !is_finite(invExpK[e])) { b->ctx->err = "non-finite exp(K)"; return DQMC_ERR_NUMERIC; }
     h.expK.assign(expK, expK + n2);
     h.invExpK.assign(invExpK, invExpK + n2);
-    HIPCHK(b->ctx, hipMemcpyAsync((void*)b->descs[c].expK, expK, n2 * 8, hipMemcpyHostToDevice, b->stream));
-    HIPCHK(b->ctx, hipMemcpyAsync((void*)b->descs[c].invExpK, invExpK, n2 * 8, hipMemcpyHostToDevice, b->stream));
+    kron_setup(h);
+    ChainDesc& d = b->descs[c];
+    d.kron = h.kron.empty() ? nullptr : (const double*)(b->arena + b->base[c] + b->layouts[c].kron);
+    HIPCHK(b->ctx, hipMemcpyAsync((void*)d.expK, expK, n2 * 8, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(b->ctx, hipMemcpyAsync((void*)d.invExpK, invExpK, n2 * 8, hipMemcpyHostToDevice, b->stream));
+    if (d.kron) HIPCHK(b->ctx, hipMemcpyAsync((void*)d.kron, h.kron.data(), h.kron.size() * 8, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(b->ctx, hipMemcpyAsync(b->d_descs + c, &d, sizeof(ChainDesc), hipMemcpyHostToDevice, b->stream));
     HIPCHK(b->ctx, hipStreamSynchronize(b->stream));
     b->initialized = false;
     return DQMC_OK;
@@ -681,6 +727,7 @@ int dqmc_get_info(dqmc_batch_t b, int c, dqmc_chain_info* out) {
     out->status = st.status;
     out->accepted = st.accepted;
     out->paired = st.paired;
+    out->kron = b->descs[c].kron != nullptr;
     out->sweeps = b->chains[c].sweeps;
     for (int q = 0; q < DQMC_PH_COUNT; ++q) out->phase_seconds[q] = (double)st.phase[q] * 1e-8;  // 100 MHz clock
     return DQMC_OK;
@@ -1051,7 +1098,7 @@ int dqmc_kd_wrap(dqmc_workspace_t w, const int8_t* fields, const double* expK, c
     if (t < 0 || t > w->time_size) { w->b->ctx->err = "time index out of range"; return DQMC_ERR_ARG; }
     ChainDesc d = w->b->descs[0];
     if ((expK != nullptr) != (invExpK != nullptr)) { w->b->ctx->err = "exp(K) needs its inverse"; return DQMC_ERR_ARG; }
-    if (expK && !d.cb) { d.expK = expK; d.invExpK = invExpK; }
+    if (expK && !d.cb) { d.expK = expK; d.invExpK = invExpK; d.kron = nullptr; }   // (caller's matrices: dense products)
     d.field = const_cast<int8_t*>(fields);
     const int row = t == 0 ? w->time_size - 1 : t - 1;   // time index t acts with field row t - 1
     const int sp = (spin == 1 || d.nspin == 1) ? 0 : 1;

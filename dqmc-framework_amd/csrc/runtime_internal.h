@@ -34,6 +34,7 @@ struct HostChain {
     double dtau = 0, alpha = 0;
     std::vector<int8_t> fields;
     std::vector<double> expK, invExpK, cosq, rhoq;
+    std::vector<double> kron;     // Kronecker factors of expK and invExpK (4 x 16 x 16), empty: dense products
     std::vector<double> kx, ky;   // k-stars of the lattice (src/lattice/square.cpp:42-66)
     long long sweeps = 0;
     double counts = 0;  // measured slices (same count for every observable)
@@ -42,8 +43,8 @@ struct HostChain {
 };
 
 struct Layout {
-    size_t G, Gw, Tmp, stkU, stkT, stkD, W, Wq, expK, invExpK, cosq, field, st, acc, Gt0, G0t, dacc, dscr, rhoq, recG,
-        recSign, recAcc, recGt0, recG0t, xbuf, total;
+    size_t G, Gw, Tmp, stkU, stkT, stkD, W, Wq, expK, invExpK, kron, cosq, field, st, acc, Gt0, G0t, dacc, dscr, rhoq,
+        recG, recSign, recAcc, recGt0, recG0t, xbuf, total;
 };
 
 struct dqmc_batch {

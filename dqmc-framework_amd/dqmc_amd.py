@@ -55,7 +55,7 @@ class ChainParams(C.Structure):
 class ChainInfo(C.Structure):
     _fields_ = [("sign", C.c_double), ("wrap_error", C.c_double), ("rng", C.c_ulonglong), ("pos", C.c_int),
                 ("status", C.c_int), ("accepted", C.c_longlong), ("sweeps", C.c_longlong),
-                ("phase_seconds", C.c_double * 8), ("paired", C.c_int)]
+                ("phase_seconds", C.c_double * 8), ("paired", C.c_int), ("kron", C.c_int)]
 
 
 JOB_DYNAMIC = 1

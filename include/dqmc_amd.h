@@ -142,6 +142,10 @@ typedef struct {
                                 panel factorizations (accumulated over DQMC_SWEEP_PROFILE sweeps
                                 since init) */
     int paired;              /* 1 if the last sweep launch ran the chain's spins on two workgroups */
+    int kron;                /* 1 if the chain's dense B products run in Kronecker form: exp(-dtau K) =
+                                X (x) Y over the two lattice directions (side <= 16, checked against
+                                the chain's exp(K) within 1e-13 of its largest entry); 0 for dense
+                                N x N products or the checkerboard break-up */
 } dqmc_chain_info;
 
 /* context: device selection, error string */

@@ -44,6 +44,8 @@ WRAP_CASES = [
     (4, 8, 1.0, 1, True, 3.0, 0.1),
     (12, 20, 2.0, 0, False, 4.0, 0.0),
     (12, 20, 2.0, 0, True, 4.0, 0.0),
+    (16, 8, 1.0, 0, False, 4.0, -0.2),   # N = 256 dense: the Kronecker form at its largest tile
+    (8, 8, 1.0, 1, False, 4.0, -0.5),
 ]
 
 
@@ -248,3 +250,55 @@ def test_unit_argument_errors(ctx):
     cb = D.make_params(4, 8, 1.0, checkerboard=True)
     with pytest.raises(D.DqmcError):
         ctx.k_wrap(cb, f, 1, 1, 0, m, np.eye(16), np.eye(16))   # exp(K) given to a checkerboard chain
+
+
+def test_kron_form_selected(ctx):
+    """Dense chains on lattices up to 16 x 16 multiply in Kronecker form (exp(-dtau K) = X (x) Y
+    over the two lattice directions, side 2 with its doubled hopping included); larger dense
+    lattices and the checkerboard break-up do not."""
+    sides = [2, 4, 8, 12, 16, 18]
+    ps = [D.make_params(s, 8, 1.0, pace=4, U=4.0, mu=-0.3, seed=s) for s in sides]
+    ps.append(D.make_params(8, 8, 1.0, pace=4, checkerboard=True, U=4.0, seed=1))
+    b = D.Batch(ctx, ps)
+    got = [b.info(c).kron for c in range(len(ps))]
+    b.close()
+    assert got == [1, 1, 1, 1, 1, 0, 0], got
+
+
+def test_kron_falls_back_for_a_nonseparable_exp_k(ctx):
+    """An exp(K) that does not factor over the lattice directions (here perturbed by 1e-6) is
+    multiplied densely, and the products still equal the oracle's with that matrix."""
+    side, L = 6, 8
+    orc, p, f = chain(side, L, 1.0, U=4.0, mu=-0.2)
+    n = side * side
+    rng = np.random.default_rng(5)
+    pert = rng.standard_normal((n, n)) * 1e-6
+    ek = orc.expK(0) + (pert + pert.T)
+    ik = np.linalg.inv(ek)
+    orc.set_expK(np.asfortranarray(ek), np.asfortranarray(ik))
+    b = D.Batch(ctx, [p])
+    b.set_expk(0, ek, ik)
+    assert b.info(0).kron == 0
+    b.set_expk(0, *[np.asfortranarray(x) for x in b.expk(0)])   # (unchanged matrices: still dense)
+    assert b.info(0).kron == 0
+    b.close()
+    for op in range(7):
+        m = np.asfortranarray(rng.standard_normal((n, n)))
+        want = oracle_op(orc, op, 3, 1, m.copy(order="F"))
+        got = ctx.k_wrap(p, f, 3, 1, op, m, ek, ik)
+        assert np.max(np.abs(got - want)) < 1e-12 * np.max(np.abs(want)), op
+
+
+def test_kron_set_expk_switches_form(ctx):
+    """Setting the closed-form exp(K) back on a chain that held a non-separable one restores the
+    Kronecker form, and a sweep of the chain in either form agrees with the oracle's."""
+    side, L = 4, 12
+    b = D.Batch(ctx, [D.make_params(side, L, 2.0, pace=4, U=4.0, mu=-0.3, seed=3)])
+    ek, ik = b.expk(0)
+    assert b.info(0).kron == 1
+    pert = ek + 1e-6 * np.eye(side * side)[::-1]
+    b.set_expk(0, pert, np.linalg.inv(pert))
+    assert b.info(0).kron == 0
+    b.set_expk(0, ek, ik)
+    assert b.info(0).kron == 1
+    b.close()
