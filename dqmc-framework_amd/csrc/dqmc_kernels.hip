@@ -2522,14 +2522,21 @@ __device__ __noinline__ void block_stage_w(const double* G_, int N, int b0, int 
 // the operand reads ahead of the matrix cores; the G loads of the group are in flight together.
 // Each block sums its K4/4 MFMAs in depth order whatever the grouping.
 template <int K4>
-__device__ __forceinline__ void block_apply_k(AS_G double* G, int N, int wave_lo, const AS_L double* acp,
-                                              const AS_L double* gsw) {
+__device__ __forceinline__ void block_apply_k(AS_G double* G, int N, int wave_lo, AS_L int* next,
+                                              const AS_L double* acp, const AS_L double* gsw) {
     const int l = lane(), w = wave();
     const int nw = 4 - wave_lo, wi = w - wave_lo;
     const int lr = l & 15, lk = l >> 4;
     const int nbr = (N + 15) >> 4, ngr = (nbr + 1) >> 1, ng = ngr * ngr;
     const FastDiv dg(ngr);
-    for (int g = wi; g < ng; g += nw) {
+    // the groups in turn (wavefronts wave_lo..3), or handed out by the LDS counter `next` to
+    // whichever wavefront asks (a wavefront that joins late takes what is left)
+    auto claim = [&]() {
+        int g = 0;
+        if (l == 0) g = __hip_atomic_fetch_add(next, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        return __builtin_amdgcn_readfirstlane(g);
+    };
+    for (int g = next ? claim() : wi; g < ng; g = next ? claim() : g + nw) {
         const int gc = dg.div(g), gr = g - gc * ngr;
         int oa[2], ob[2];
 #pragma unroll
@@ -2570,23 +2577,28 @@ __device__ __forceinline__ void block_apply_k(AS_G double* G, int N, int wave_lo
             }
     }
 }
-__device__ __noinline__ void block_apply(double* G_, int N, int k, int wave_lo, Lds&) {
+// dyn: the groups are handed out by the LDS counter ivar[3] (zeroed by the caller before a
+// barrier) to every wavefront that calls, whenever it calls; otherwise wavefronts wave_lo..3 take
+// them in turn
+__device__ __noinline__ void block_apply(double* G_, int N, int k, int wave_lo, bool dyn, Lds&) {
     Lds& s = g_lds;
-    N = uni(N); k = uni(k); wave_lo = uni(wave_lo);
+    N = uni(N); k = uni(k); wave_lo = uni(wave_lo); dyn = uni(dyn);
     AS_G double* G = gptr(uni(G_));
     const int k4 = (k + 3) & ~3;
-    if (wave() < wave_lo) return;
+    if (!dyn && wave() < wave_lo) return;
+    AS_L int* const next = dyn ? (AS_L int*)&s.ivar[3] : nullptr;
+    if (dyn) wave_lo = 0;
     const AS_L double* const acp = (const AS_L double*)s.panel;
     const AS_L double* const gsw = acp + (size_t)N * k4;
     switch (k4) {
-        case 4: block_apply_k<4>(G, N, wave_lo, acp, gsw); break;
-        case 8: block_apply_k<8>(G, N, wave_lo, acp, gsw); break;
-        case 12: block_apply_k<12>(G, N, wave_lo, acp, gsw); break;
-        case 16: block_apply_k<16>(G, N, wave_lo, acp, gsw); break;
-        case 20: block_apply_k<20>(G, N, wave_lo, acp, gsw); break;
-        case 24: block_apply_k<24>(G, N, wave_lo, acp, gsw); break;
-        case 28: block_apply_k<28>(G, N, wave_lo, acp, gsw); break;
-        default: block_apply_k<32>(G, N, wave_lo, acp, gsw); break;
+        case 4: block_apply_k<4>(G, N, wave_lo, next, acp, gsw); break;
+        case 8: block_apply_k<8>(G, N, wave_lo, next, acp, gsw); break;
+        case 12: block_apply_k<12>(G, N, wave_lo, next, acp, gsw); break;
+        case 16: block_apply_k<16>(G, N, wave_lo, next, acp, gsw); break;
+        case 20: block_apply_k<20>(G, N, wave_lo, next, acp, gsw); break;
+        case 24: block_apply_k<24>(G, N, wave_lo, next, acp, gsw); break;
+        case 28: block_apply_k<28>(G, N, wave_lo, next, acp, gsw); break;
+        default: block_apply_k<32>(G, N, wave_lo, next, acp, gsw); break;
     }
 }
 
@@ -2905,18 +2917,19 @@ __device__ __noinline__ WalkOut wg_metropolis(Desc d, int e, uint64_t rng, doubl
     int pk = 0;
     for (int b0 = 0; b0 < N; b0 += kb) {
         const int k = min(kb, N - b0);
-        if (tid() == 0) s.ivar[2] = 0;
+        if (tid() == 0) { s.ivar[2] = 0; s.ivar[3] = 0; }
         __syncthreads();
         SubClock walk_clk;
         walk_clk.start();
+        // the pending update runs beside the walk on wavefronts 1..3; wavefront 0 joins it when
+        // its walk is done (the update's block groups are handed out through ivar[3])
         if (wave() == 0) {
             const WalkOut o = walk_block(d, e, b0, k, rng, sign, record, s);
             rng = o.rng;
             sign = o.sign;
             if (lane() == 0) s.ivar[2] = o.accepted;
-        } else if (pending) {
-            block_apply(Gs(hi - 1), N, pk, 1, s);
         }
+        if (pending) block_apply(Gs(hi - 1), N, pk, 1, true, s);
         __syncthreads();
         walk_clk.stop(0);
         pending = false;
@@ -2934,7 +2947,7 @@ __device__ __noinline__ WalkOut wg_metropolis(Desc d, int e, uint64_t rng, doubl
                     pending = true;
                     pk = k;
                 } else {
-                    block_apply(Gs(sp), N, k, 0, s);
+                    block_apply(Gs(sp), N, k, 0, false, s);
                     __syncthreads();
                 }
             }
