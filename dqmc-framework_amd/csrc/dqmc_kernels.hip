@@ -2139,6 +2139,80 @@ __device__ __noinline__ void qr_panel(int m, int jb, int k0, int N, double* M_, 
     __syncthreads();
 }
 
+// The same block reflector with C held in registers, for m <= 4 MQ rows: each wavefront takes
+// whole 16-column blocks of C (all m rows, so the reduction V^T C stays in the wavefront; no
+// workgroup barrier between blocks), reads C once and writes it once.  Lane l holds
+// C[4s + l/16][l%16] of its block for depth step s, which is both the B operand of
+// W1 = V^T C and, for s = 4 rb + e, element e of the 16x16 result block rb of V W2, so the
+// update C - V W2 is elementwise in registers.  V (m x jb) is staged once in the LDS panel
+// at an odd leading dimension (rows padded to a multiple of 16 with zeros, columns to 16),
+// so that lanes reading one row at 16 different columns fall on different banks.
+template <int MQ>
+__device__ __noinline__ void wg_reflect_regs(const double* V_, int ldv, const double* T_, bool tT, double* C_, int ldc,
+                                             int m, int n, int jb) {
+    Lds& s = g_lds;
+    ldv = uni(ldv); tT = uni(tT); ldc = uni(ldc); m = uni(m); n = uni(n); jb = uni(jb);
+    const AS_G double* const V = gptr(uni(V_));
+    const AS_G double* const T = gptr(uni(T_));
+    AS_G double* const C = gptr(uni(C_));
+    const int mb = (m + 15) >> 4, mp = mb * 16, ldp = mp + 1;
+    AS_L double* const Vl = (AS_L double*)s.panel;   // 16 x ldp
+    {
+        const FastDiv dmp(mp);
+        for (int q = tid(); q < 16 * mp; q += NT) {
+            const int c = dmp.div(q), r = q - c * mp;
+            Vl[c * ldp + r] = (c < jb && r < m) ? V[(size_t)c * ldv + r] : 0.0;
+        }
+    }
+    const int l = lane(), w = wave();
+    const int li = l & 15, lk = l >> 4;
+    // op(T) as the A operand: op(T)[l%16][4s + l/16] (T is zero outside jb x jb)
+    double tf[4];
+#pragma unroll
+    for (int s4 = 0; s4 < 4; ++s4) tf[s4] = tT ? T[li * NB + 4 * s4 + lk] : T[(4 * s4 + lk) * NB + li];
+    __syncthreads();
+    const int ncb = (n + 15) >> 4;
+    for (int cb = w; cb < ncb; cb += 4) {
+        const int col = cb * 16 + li;
+        const bool colok = col < n;
+        const AS_G double* const cc = C + (size_t)(colok ? col : 0) * ldc;
+        double c[MQ];
+#pragma unroll
+        for (int q = 0; q < MQ; ++q) {
+            const int r = 4 * q + lk;
+            c[q] = (colok && r < m) ? cc[r] : 0.0;
+        }
+        // W1 = V^T C: A operand V^T[l%16][4s + l/16] = V[4s + l/16][l%16]
+        dbl4 w1 = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int q = 0; q < MQ; ++q)
+            if (q < 4 * mb) w1 = __builtin_amdgcn_mfma_f64_16x16x4f64(Vl[li * ldp + 4 * q + lk], c[q], w1, 0, 0, 0);
+        // W2 = op(T) W1
+        dbl4 w2 = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) w2 = __builtin_amdgcn_mfma_f64_16x16x4f64(tf[s4], w1[s4], w2, 0, 0, 0);
+        // C -= V W2 by 16-row blocks: A operand V[16 rb + l%16][4s + l/16]
+#pragma unroll
+        for (int rb = 0; rb < MQ / 4; ++rb) {
+            if (rb < mb) {
+                dbl4 pv = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int s4 = 0; s4 < 4; ++s4)
+                    pv = __builtin_amdgcn_mfma_f64_16x16x4f64(Vl[(4 * s4 + lk) * ldp + 16 * rb + li], w2[s4], pv, 0, 0, 0);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) c[4 * rb + e] = c[4 * rb + e] - pv[e];
+            }
+        }
+        AS_G double* const co = C + (size_t)(colok ? col : 0) * ldc;
+#pragma unroll
+        for (int q = 0; q < MQ; ++q) {
+            const int r = 4 * q + lk;
+            if (colok && r < m) co[r] = c[q];
+        }
+    }
+    __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // UDV by pre-pivoted blocked Householder QR of M (N x N):
 //   columns sorted once by decreasing norm (P), then M P = Q R without
@@ -2206,7 +2280,9 @@ __device__ __noinline__ void wg_qr_udv(double* M, double* Mp, int N, double* U, 
         if (n2 > 0) {
             const double* V = M + (size_t)k0 * N + k0;
             double* A2 = Mp + (size_t)(k0 + jb) * N + k0;
-            wg_reflect(V, N, Tg + (size_t)(k0 / NB) * NB * NB, true, A2, N, m, n2, jb, Wsm, Wsm2, s);
+            if (m <= 144) wg_reflect_regs<36>(V, N, Tg + (size_t)(k0 / NB) * NB * NB, true, A2, N, m, n2, jb);
+            else if (m <= 256) wg_reflect_regs<64>(V, N, Tg + (size_t)(k0 / NB) * NB * NB, true, A2, N, m, n2, jb);
+            else wg_reflect(V, N, Tg + (size_t)(k0 / NB) * NB * NB, true, A2, N, m, n2, jb, Wsm, Wsm2, s);
         }
     }
     // D and Tl = D^-1 R P^T (D also kept in LDS for the division)
@@ -2229,7 +2305,9 @@ __device__ __noinline__ void wg_qr_udv(double* M, double* Mp, int N, double* U, 
         const double* V = M + (size_t)k0 * N + k0;
         const double* T = Tg + (size_t)(k0 / NB) * NB * NB;
         double* U2 = U + (size_t)k0 * N + k0;
-        wg_reflect(V, N, T, false, U2, N, m, m, jb, Wsm, Wsm2, s);
+        if (m <= 144) wg_reflect_regs<36>(V, N, T, false, U2, N, m, m, jb);
+        else if (m <= 256) wg_reflect_regs<64>(V, N, T, false, U2, N, m, m, jb);
+        else wg_reflect(V, N, T, false, U2, N, m, m, jb, Wsm, Wsm2, s);
     }
 }
 
