@@ -326,6 +326,10 @@ def main():
     ap.add_argument("--serve-scale", type=float, default=0.1, help="sweep-count scale of the serving stream")
     ap.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrivals per second")
     ap.add_argument("--no-graph", action="store_true", help="launch the sweeps directly instead of replaying a graph")
+    ap.add_argument("--gemm-timing", default="", metavar="SIZES",
+                    help="only run the workgroup GEMM of the N x N x N products in its 64x64-tile form (variant 1) "
+                         "and its wide-tile forms (5: 128x128, 6: 160x160) three times each per size (comma-separated "
+                         "N), for a kernel trace of k_gemm, and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only fill and copy a device buffer of this size and exit (a known-traffic reference for "
                          "profiler byte counters)")
@@ -339,6 +343,22 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"fill_bytes_written": n * 8, "copy_bytes_read": n * 8, "copy_bytes_written": n * 8}))
+        return
+    if args.gemm_timing:
+        ctx = D.Context(0)
+        rng = np.random.default_rng(0)
+        runs = []
+        for n in [int(x) for x in args.gemm_timing.split(",")]:
+            A = rng.standard_normal((n, n))
+            B = rng.standard_normal((n, n))
+            ref = ctx.gemm(1, A, B)
+            for v in (1, 5, 6):
+                for _ in range(3):
+                    assert np.array_equal(ctx.gemm(v, A, B), ref)   # the forms give the same bits
+                runs.append({"N": n, "variant": v, "launches": 3})
+        ctx.close()
+        print(json.dumps({"gemm_timing": runs, "order": "k_gemm dispatches: per N, one reference launch (variant 1), "
+                                                        "then 3 launches of each variant in the listed order"}))
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: chains are independent and would be sharded by chain across GPUs, "
