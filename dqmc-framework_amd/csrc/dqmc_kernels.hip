@@ -1303,14 +1303,23 @@ __device__ __noinline__ void kron_left(const double* M_, double* W_, int N, int 
         wpost[e] = has_post ? rpost[wo[e]] : 1.0;
     }
     constexpr int CH = 4;
-    for (int j0 = w; j0 < N; j0 += 4 * CH) {
-        double v[CH][4];
+    double v[CH][4], vn[CH][4];
+    // the loads of the next group of CH columns are issued before this group's products
+    auto load_group = [&](int j0, double (&dst)[CH][4]) {
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
             const int j = min(j0 + 4 * c, N - 1);   // (a column past N repeats the last; not stored)
 #pragma unroll
-            for (int s4 = 0; s4 < 4; ++s4) v[c][s4] = M[(size_t)j * N + vo[s4]];
+            for (int s4 = 0; s4 < 4; ++s4) dst[c][s4] = M[(size_t)j * N + vo[s4]];
         }
+    };
+    load_group(w, vn);
+    for (int j0 = w; j0 < N; j0 += 4 * CH) {
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) v[c][s4] = vn[c][s4];
+        if (j0 + 4 * CH < N) load_group(j0 + 4 * CH, vn);
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
             dbl4 u = dbl4{0.0, 0.0, 0.0, 0.0}, r = dbl4{0.0, 0.0, 0.0, 0.0};
@@ -1333,10 +1342,12 @@ __device__ __noinline__ void kron_left(const double* M_, double* W_, int N, int 
 }
 
 // W = M diag(cpre) (X (x) Y) diag(cpost), by blocks of 16 rows of M: T = M_R (X (x) 1) from global
-// memory into LDS (wavefront w takes the column groups y = w, w + 4, ...), then W_R = T (1 (x) Y)
-// (wavefront w takes x' = w, w + 4, ...) into LDS, written out along columns.  Row r of a block
-// block sits at stride LDR = 17 doubles in LDS (16 rows plus one, so a wavefront's lanes that
-// read one row at different columns fall on different banks).
+// memory into LDS (wavefront w takes the column groups y = w, w + 4, w + 8, w + 12), then
+// W_R = T (1 (x) Y) (wavefront w takes x' = w, w + 4, ...) into LDS, written out along columns.
+// The global loads of a block (16 per lane) are issued together, and those of the next block
+// while the current one runs its second pass and write-out.  Row r of a block sits at stride
+// LDR = 17 doubles in LDS (16 rows plus one, so a wavefront's lanes that read one row at
+// different columns fall on different banks).
 #define KRON_LDR 17
 __device__ __noinline__ void kron_right(const double* M_, double* W_, int N, int side, const double* X_,
                                         const double* Y_, const double* cpre_, const double* cpost_, Lds&) {
@@ -1350,7 +1361,7 @@ __device__ __noinline__ void kron_right(const double* M_, double* W_, int N, int
     const bool has_pre = cpre_ != nullptr, has_post = cpost_ != nullptr;
     const AS_L double* const cpre = (const AS_L double*)cpre_;
     const AS_L double* const cpost = (const AS_L double*)cpost_;
-    AS_L double* const T = (AS_L double*)s.panel;            // 16 rows x N, row stride... column c at c LDR
+    AS_L double* const T = (AS_L double*)s.panel;            // 16 rows x N: column c at c LDR
     AS_L double* const Wl = T + (size_t)N * KRON_LDR;
     const int l = lane(), w = wave();
     const int i = l & 15, kq = l >> 4;
@@ -1359,41 +1370,67 @@ __device__ __noinline__ void kron_right(const double* M_, double* W_, int N, int
 #pragma unroll
     for (int s4 = 0; s4 < 4; ++s4) { xb[s4] = kron_frag(X, s4, true); yb[s4] = kron_frag(Y, s4, true); }
     const bool dok[4] = {4 * 0 + kq < side, 4 * 1 + kq < side, 4 * 2 + kq < side, 4 * 3 + kq < side};
-    for (int r0 = 0; r0 < N; r0 += 16) {
+    // this lane's pass-1 elements: column group y = w + 4 g, depth step s: column 4s + kq + side y
+    int co[4][4];
+    bool cok[4][4];
+    double cp[4][4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+            const int y = w + 4 * g;
+            cok[g][s4] = y < side && dok[s4];
+            co[g][s4] = cok[g][s4] ? 4 * s4 + kq + side * y : 0;
+            cp[g][s4] = has_pre ? cpre[co[g][s4]] : 1.0;
+        }
+    double v[4][4];
+    auto load_block = [&](int r0) {
         const int row = min(r0 + i, N - 1);   // (rows past N repeat the last; not stored)
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) v[g][s4] = M[(size_t)co[g][s4] * N + row];
+    };
+    load_block(0);
+    for (int r0 = 0; r0 < N; r0 += 16) {
         // pass 1: T[r][(x', y)] = sum_x M[r][(x, y)] cpre[(x, y)] X[x][x']
-        for (int y = w; y < side; y += 4) {
-            double a[4];
 #pragma unroll
-            for (int s4 = 0; s4 < 4; ++s4) {
-                const int c = dok[s4] ? 4 * s4 + kq + side * y : 0;
-                const double v = M[(size_t)c * N + row];
-                a[s4] = dok[s4] ? (has_pre ? v * cpre[c] : v) : 0.0;
-            }
-            dbl4 t = dbl4{0.0, 0.0, 0.0, 0.0};
+        for (int g = 0; g < 4; ++g) {
+            const int y = w + 4 * g;
+            if (y < side) {
+                dbl4 t = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int s4 = 0; s4 < 4; ++s4) t = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s4], xb[s4], t, 0, 0, 0);
-            // element e: T[r0 + kq + 4e][(x' = i, y)]
-            if (i < side) {
+                for (int s4 = 0; s4 < 4; ++s4) {
+                    const double a = cok[g][s4] ? (has_pre ? v[g][s4] * cp[g][s4] : v[g][s4]) : 0.0;
+                    t = __builtin_amdgcn_mfma_f64_16x16x4f64(a, xb[s4], t, 0, 0, 0);
+                }
+                // element e: T[r0 + kq + 4e][(x' = i, y)]
+                if (i < side) {
 #pragma unroll
-                for (int e = 0; e < 4; ++e) T[(i + side * y) * KRON_LDR + kq + 4 * e] = t[e];
+                    for (int e = 0; e < 4; ++e) T[(i + side * y) * KRON_LDR + kq + 4 * e] = t[e];
+                }
             }
         }
+        if (r0 + 16 < N) load_block(r0 + 16);   // the next block's loads run under pass 2
         __syncthreads();
         // pass 2: W[r][(x', y')] = sum_y T[r][(x', y)] Y[y][y'] cpost[(x', y')]
-        for (int xp = w; xp < side; xp += 4) {
-            double a[4];
 #pragma unroll
-            for (int s4 = 0; s4 < 4; ++s4) a[s4] = dok[s4] ? T[(xp + side * (4 * s4 + kq)) * KRON_LDR + i] : 0.0;
-            dbl4 r = dbl4{0.0, 0.0, 0.0, 0.0};
+        for (int g = 0; g < 4; ++g) {
+            const int xp = w + 4 * g;
+            if (xp < side) {
+                double a[4];
 #pragma unroll
-            for (int s4 = 0; s4 < 4; ++s4) r = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s4], yb[s4], r, 0, 0, 0);
-            // element e: W[r0 + kq + 4e][(x', y' = i)]
-            if (i < side) {
-                const int c = xp + side * i;
-                const double cp = has_post ? cpost[c] : 1.0;
+                for (int s4 = 0; s4 < 4; ++s4) a[s4] = dok[s4] ? T[(xp + side * (4 * s4 + kq)) * KRON_LDR + i] : 0.0;
+                dbl4 r = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                for (int e = 0; e < 4; ++e) Wl[c * KRON_LDR + kq + 4 * e] = has_post ? r[e] * cp : r[e];
+                for (int s4 = 0; s4 < 4; ++s4) r = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s4], yb[s4], r, 0, 0, 0);
+                // element e: W[r0 + kq + 4e][(x', y' = i)]
+                if (i < side) {
+                    const int c = xp + side * i;
+                    const double cq = has_post ? cpost[c] : 1.0;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) Wl[c * KRON_LDR + kq + 4 * e] = has_post ? r[e] * cq : r[e];
+                }
             }
         }
         __syncthreads();
@@ -1427,14 +1464,15 @@ __device__ void wg_bmul(Desc d, int op, int sp, double* M, double* W, Lds&, bool
         const double* ky = d.kron + 256;
         const double* ix = d.kron + 512;
         const double* iy = d.kron + 768;
+        // in place: a column (left) or block of rows (right) is read whole before it is written,
+        // and the loads issued ahead read only columns or rows not yet written
         switch (op) {
-            case 0: kron_left(M, W, N, d.side, kx, ky, false, nullptr, ep); break;
-            case 1: kron_right(M, W, N, d.side, kx, ky, ep, nullptr, s); break;
-            case 2: kron_left(M, W, N, d.side, ix, iy, false, en, nullptr); break;
-            case 3: kron_right(M, W, N, d.side, ix, iy, nullptr, en, s); break;
-            case 4: kron_left(M, W, N, d.side, kx, ky, true, ep, nullptr); break;
+            case 0: kron_left(M, M, N, d.side, kx, ky, false, nullptr, ep); break;
+            case 1: kron_right(M, M, N, d.side, kx, ky, ep, nullptr, s); break;
+            case 2: kron_left(M, M, N, d.side, ix, iy, false, en, nullptr); break;
+            case 3: kron_right(M, M, N, d.side, ix, iy, nullptr, en, s); break;
+            case 4: kron_left(M, M, N, d.side, kx, ky, true, ep, nullptr); break;
         }
-        wg_copy(M, W, N * N);
         return;
     }
     switch (op) {
