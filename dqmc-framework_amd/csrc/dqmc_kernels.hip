@@ -2610,7 +2610,7 @@ __device__ __noinline__ void block_stage_w(const double* G_, int N, int b0, int 
     double* const gsw = acp + (size_t)N * k4;
     const double* const cr = MP_CR(sp);
     // ACP and GSW together (GSW element e = c k4 + b: consecutive threads read down a column of G)
-    wg_stream2(N * k4, [&](int e) { return G[(size_t)b0 * N + min(e, N * k - 1)]; },
+    wg_stream2<16>(N * k4, [&](int e) { return G[(size_t)b0 * N + min(e, N * k - 1)]; },
                [&](int e, double v) { acp[e] = e < N * k ? v : 0.0; },
                [&](int e) { const int c = dk4.div(e), b = e - c * k4; return G[(size_t)c * N + b0 + min(b, k - 1)]; },
                [&](int e, double v) { const int c = dk4.div(e), b = e - c * k4; gsw[b * N + c] = b < k ? v : 0.0; });
@@ -2652,24 +2652,44 @@ __device__ __forceinline__ void block_apply_k(AS_G double* G, int N, int wave_lo
         if (l == 0) g = __hip_atomic_fetch_add(next, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         return __builtin_amdgcn_readfirstlane(g);
     };
-    for (int g = next ? claim() : wi; g < ng; g = next ? claim() : g + nw) {
+    // the G loads of the next group are issued before this group's products, so their latency
+    // runs under the matrix cores
+    double cvn[2][2][4];
+    auto load_group = [&](int g) {
         const int gc = dg.div(g), gr = g - gc * ngr;
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const int orow = min((2 * gr + a) * 16 + lr, N - 1);
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) cvn[a][b][i] = G[(size_t)min((2 * gc + b) * 16 + lk + 4 * i, N - 1) * N + orow];
+        }
+    };
+    int g = next ? claim() : wi;
+    if (g < ng) load_group(g);
+    while (g < ng) {
+        const int gc = dg.div(g), gr = g - gc * ngr;
+        double cv[2][2][4];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) cv[a][b][i] = cvn[a][b][i];
+        const int gn = next ? claim() : g + nw;
+        if (gn < ng) load_group(gn);
         int oa[2], ob[2];
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
             oa[a] = min((2 * gr + a) * 16 + lr, N - 1);   // (rows and columns past N repeat the last; not stored)
             ob[a] = min((2 * gc + a) * 16 + lr, N - 1);
         }
-        double cv[2][2][4];
         dbl4 acc[2][2];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
-#pragma unroll
-                for (int i = 0; i < 4; ++i) cv[a][b][i] = G[(size_t)min((2 * gc + b) * 16 + lk + 4 * i, N - 1) * N + oa[a]];
-                acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
-            }
+            for (int b = 0; b < 2; ++b) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int kk = 0; kk < K4; kk += 4) {
             const int kr = (kk + lk) * N;
@@ -2691,11 +2711,9 @@ __device__ __forceinline__ void block_apply_k(AS_G double* G, int N, int wave_lo
                     if (row < N && col < N) G[(size_t)col * N + row] = cv[a][b][i] + acc[a][b][i];
                 }
             }
+        g = gn;
     }
 }
-// dyn: the groups are handed out by the LDS counter ivar[3] (zeroed by the caller before a
-// barrier) to every wavefront that calls, whenever it calls; otherwise wavefronts wave_lo..3 take
-// them in turn
 __device__ __noinline__ void block_apply(double* G_, int N, int k, int wave_lo, bool dyn, Lds&) {
     Lds& s = g_lds;
     N = uni(N); k = uni(k); wave_lo = uni(wave_lo); dyn = uni(dyn);
