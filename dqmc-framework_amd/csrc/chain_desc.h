@@ -10,7 +10,7 @@
 #define DQMC_NMAX 448        // largest lattice (sites) a chain may have (side <= 21)
 #define DQMC_KMAX 32         // largest delayed-update depth (a Metropolis block holds its sites in the 32
                              // lanes of a half-wavefront per spin)
-#define DQMC_PANEL_DOUBLES 11776   // LDS panel of a workgroup (92 KB)
+#define DQMC_PANEL_DOUBLES 13824   // LDS panel of a workgroup (108 KB)
 // spin-pair exchange area of a repulsive chain: the k x k walk blocks [spin][parity][KMAX^2], then
 // the per-sweep state slots [spin][parity][2] (status bits, wrap error)
 #define DQMC_PAIR_XBUF_DOUBLES (4 * DQMC_KMAX * DQMC_KMAX + 8)

@@ -345,8 +345,8 @@ int dqmc_server_destroy(dqmc_server_t s);
  *   C = diag(rs) op(A) diag(ms) op(B) diag(cs) + beta C,  column-major,
  *   op(A) M x K, op(B) K x Nc; rs/ms/cs may be NULL.
  * variant 0: vector FP64, 1: MFMA FP64, 2: MFMA FP32 (inputs rounded to FP32),
- * 3: MFMA FP64 with A staged in LDS (M*K <= 11776), 4: A and B staged in LDS
- * (M*K + K*Nc <= 11776), 5 and 6: MFMA FP64 on wide tiles (128 x 128 and
+ * 3: MFMA FP64 with A staged in LDS (M*K <= 13824), 4: A and B staged in LDS
+ * (M*K + K*Nc <= 13824), 5 and 6: MFMA FP64 on wide tiles (128 x 128 and
  * 160 x 160 per workgroup, the form the engine uses for its N x N x N products;
  * bit-identical to variant 1).  The scalings rs, ms, cs are staged in LDS too,
  * so M, K, Nc <= 448 where they are given (variants 1-6). */

@@ -48,7 +48,7 @@ def test_gemm_lds_operands(ctx, variant, shape, tA, tB):
     """The MFMA GEMM with A (3) or A and B (4) in LDS, as the LU, QR and
     block-update callers use it."""
     M, Nc, K = shape
-    if variant == 4 and M * K + K * Nc > 11776:
+    if variant == 4 and M * K + K * Nc > 13824:
         pytest.skip("operands exceed the LDS panel")
     print(f"[gpu-gemm] variant {variant} shape {shape} tA {tA} tB {tB}", flush=True)
     rng = np.random.default_rng(M * 5 + Nc * 11 + K + variant)
