@@ -2791,11 +2791,12 @@ __device__ void gss_transpose(int sp, Lds&) {
 // The in-block walk, run by wavefront 0 only.  Lanes 0..31 carry spin up and
 // lanes 32..63 spin down; lane h*32 + a owns site s_a of the block in spin h
 // and keeps row a and column a of that spin's C in registers for the whole
-// block, so a flip costs LDS reads only (the loops run over all KB rows with
-// fixed trip counts and unroll; G0[S,S] and C are zero outside the k x k block,
-// so the padding terms vanish exactly).  The decisions are computed by every
-// lane on identical data, in the reference's site order.  On return, row a of
-// C is in CR(h) for the block update.
+// block, so a flip costs LDS reads only (the loops run over the KW >= k rows
+// of the block's padded depth with fixed trip counts and unroll; G0[S,S] and C
+// are zero outside the k x k block, so the padding terms vanish exactly).  The
+// decisions are computed by every lane on identical data, in the reference's
+// site order.  On return, the first KW entries of row a of C are in CR(h) for
+// the block update (its W product reads no further).
 struct WalkOut {
     uint64_t rng;
     double sign;
@@ -2810,6 +2811,7 @@ __device__ __forceinline__ const AS_L double* lds_row(const double* p) {
     asm volatile("" : "+v"(v));
     return (const AS_L double*)(uintptr_t)v;
 }
+template <int KW>
 __device__ __forceinline__ WalkOut walk_block(Desc d, int e, int b0, int k, uint64_t rng, double sign,
                                            bool record, Lds&) {
     Lds& s = g_lds;
@@ -2827,9 +2829,9 @@ __device__ __forceinline__ WalkOut walk_block(Desc d, int e, int b0, int k, uint
     const double* const gt = MP_GT;
     double* const ush = MP_UV + h * 32;        // u at [lane], v at [64 + lane]
     double* const vsh = MP_UV + 64 + h * 32;
-    double crr[KB], ccr[KB];
+    double crr[KW], ccr[KW];
 #pragma unroll
-    for (int b = 0; b < KB; ++b) { crr[b] = 0.0; ccr[b] = 0.0; }
+    for (int b = 0; b < KW; ++b) { crr[b] = 0.0; ccr[b] = 0.0; }
     double dq = gr[a * 64 + lane()];
     int nacc = 0;
     // the block's k uniform draws, one per lane (site a), bit-identical to drawing them in order
@@ -2867,9 +2869,9 @@ __device__ __forceinline__ WalkOut walk_block(Desc d, int e, int b0, int k, uint
 #pragma unroll
             for (int j = 0; j < 4; ++j) { xc[0][j] = colp[64 * j]; yr[0][j] = rowp[64 * j]; }
 #pragma unroll
-            for (int c = 0; c < KB / 4; ++c) {
+            for (int c = 0; c < KW / 4; ++c) {
                 const int cur = c & 1, nxt = cur ^ 1;
-                if (c + 1 < KB / 4) {
+                if (c + 1 < KW / 4) {
 #pragma unroll
                     for (int j = 0; j < 4; ++j) { xc[nxt][j] = colp[64 * (4 * c + 4 + j)]; yr[nxt][j] = rowp[64 * (4 * c + 4 + j)]; }
                 }
@@ -2901,9 +2903,9 @@ __device__ __forceinline__ WalkOut walk_block(Desc d, int e, int b0, int k, uint
 #pragma unroll
             for (int j = 0; j < 4; ++j) { ub[0][j] = uo[j]; vb[0][j] = vo[j]; ga[0][j] = rowa[64 * j]; gc[0][j] = cola[64 * j]; }
 #pragma unroll
-            for (int c = 0; c < KB / 4; ++c) {
+            for (int c = 0; c < KW / 4; ++c) {
                 const int cur = c & 1, nxt = cur ^ 1;
-                if (c + 1 < KB / 4) {
+                if (c + 1 < KW / 4) {
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const int b = 4 * c + 4 + j;
@@ -2927,7 +2929,7 @@ __device__ __forceinline__ WalkOut walk_block(Desc d, int e, int b0, int k, uint
     }
     double* const cro = MP_CR(h) + a * KP;
 #pragma unroll
-    for (int b = 0; b < KB; ++b) cro[b] = crr[b];
+    for (int b = 0; b < KW; ++b) cro[b] = crr[b];
     rng = mulmod_m31(rng, kMinstdPow.v[2 * k]);   // the block consumed 2k draws
     return WalkOut{rng, sign, nacc};
 }
@@ -3058,7 +3060,13 @@ __device__ __noinline__ WalkOut wg_metropolis(Desc d, int e, uint64_t rng, doubl
         // the pending update runs beside the walk on wavefronts 1..3; wavefront 0 joins it when
         // its walk is done (the update's block groups are handed out through ivar[3])
         if (wave() == 0) {
-            const WalkOut o = walk_block(d, e, b0, k, rng, sign, record, s);
+            WalkOut o;
+            switch ((k + 7) & ~7) {   // the walk's loops run over the block's depth rounded up to 8
+                case 8: o = walk_block<8>(d, e, b0, k, rng, sign, record, s); break;
+                case 16: o = walk_block<16>(d, e, b0, k, rng, sign, record, s); break;
+                case 24: o = walk_block<24>(d, e, b0, k, rng, sign, record, s); break;
+                default: o = walk_block<32>(d, e, b0, k, rng, sign, record, s); break;
+            }
             rng = o.rng;
             sign = o.sign;
             if (lane() == 0) s.ivar[2] = o.accepted;
