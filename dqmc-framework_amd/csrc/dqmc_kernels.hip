@@ -1738,13 +1738,105 @@ __device__ __noinline__ double lu_panel(int m, int jb, int k0, Lds&) {
 }
 
 // ---------------------------------------------------------------------------
-// Blocked LU with partial pivoting: A X = B, X overwrites B (N x nrhs); A is
-// destroyed.  Panels of NB columns are factored in LDS (pivot search, swaps
-// and rank-1 updates never leave the CU); the row interchanges and L11^-1 are
-// applied to the columns right of the panel and to B one column per thread;
-// the trailing matrix and B are updated with one FP64 MFMA GEMM each
-// (right-looking, Golub & Van Loan Alg. 3.4.2).  Back substitution runs by
-// NB-row blocks with a GEMM for the rows above.  Returns the sign of det(A).
+// C +/-= A W with both operands in LDS: C is M x Nc in global memory (ld ldc); A is M x K4 with
+// element (r, k) at A[k lda + r]; W is K4 x Nc with element (k, c) at W[k ldw + c]; K4 is the
+// padded depth (a multiple of 4; both operands zero beyond the true depth).  Wavefronts
+// wave_lo..3 take 2x2 groups of 16x16 blocks of C (block rows 2gr, 2gr+1, block columns 2gc,
+// 2gc+1) in turn or, with `next`, as an LDS counter hands them out to whichever wavefront asks
+// (a wavefront that joins late takes what is left).  No workgroup barrier inside.  The product
+// is formed transposed (W^T A^T), so 16 lanes run down one column of C; each depth step reads
+// two A and two W operands for four MFMAs with the depth loop unrolled, and the C loads of a
+// wavefront's next group are issued before the current group's products, so their latency runs
+// under the matrix cores.  Each block sums its K4/4 MFMAs in depth order whatever the grouping.
+// ---------------------------------------------------------------------------
+template <int K4, bool SUB>
+__device__ __forceinline__ void lds_update(AS_G double* C, int ldc, int M, int Nc, const AS_L double* A, int lda,
+                                           const AS_L double* W, int ldw, int wave_lo, AS_L int* next) {
+    const int l = lane(), w = wave();
+    const int nw = 4 - wave_lo, wi = w - wave_lo;
+    const int lr = l & 15, lk = l >> 4;
+    const int ngr = (M + 31) >> 5, ng = ngr * ((Nc + 31) >> 5);
+    const FastDiv dg(ngr);
+    auto claim = [&]() {
+        int g = 0;
+        if (l == 0) g = __hip_atomic_fetch_add(next, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        return __builtin_amdgcn_readfirstlane(g);
+    };
+    double cvn[2][2][4];
+    auto load_group = [&](int g) {
+        const int gc = dg.div(g), gr = g - gc * ngr;
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const int orow = min((2 * gr + a) * 16 + lr, M - 1);
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    cvn[a][b][i] = C[(size_t)min((2 * gc + b) * 16 + lk + 4 * i, Nc - 1) * ldc + orow];
+        }
+    };
+    int g = next ? claim() : wi;
+    if (g < ng) load_group(g);
+    while (g < ng) {
+        const int gc = dg.div(g), gr = g - gc * ngr;
+        double cv[2][2][4];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) cv[a][b][i] = cvn[a][b][i];
+        const int gn = next ? claim() : g + nw;
+        if (gn < ng) load_group(gn);
+        int oa[2], ob[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            oa[a] = min((2 * gr + a) * 16 + lr, M - 1);   // (rows and columns past the end repeat the last; not stored)
+            ob[a] = min((2 * gc + a) * 16 + lr, Nc - 1);
+        }
+        dbl4 acc[2][2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < K4; kk += 4) {
+            const int ka = (kk + lk) * lda, kw = (kk + lk) * ldw;
+            const double a0 = A[ka + oa[0]], a1 = A[ka + oa[1]];
+            const double b0 = W[kw + ob[0]], b1 = W[kw + ob[1]];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int row = (2 * gr + a) * 16 + lr;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int col = (2 * gc + b) * 16 + lk + 4 * i;
+                    if (row < M && col < Nc) C[(size_t)col * ldc + row] = SUB ? cv[a][b][i] - acc[a][b][i] : cv[a][b][i] + acc[a][b][i];
+                }
+            }
+        g = gn;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Blocked LU with partial pivoting: A X = B, X overwrites B (N x nrhs; B must
+// follow A in memory, B = A + N N, when nrhs > 0); A is destroyed (only the
+// diagonal blocks of U and the U12 rows are kept: L is not read again).  Panels
+// of NB columns are factored in LDS (pivot search, swaps and rank-1 updates never
+// leave the CU); the row interchanges and L11^-1 are applied to the columns right
+// of the panel and to B one column per thread, which also leaves the panel's U12
+// rows in LDS; the rows below are then updated, C -= L21 U12, by lds_update with
+// both operands in LDS (right-looking, Golub & Van Loan Alg. 3.4.2).  Back
+// substitution runs by NB-row blocks: each block's rows of X are solved one column
+// per thread against U11 and left in LDS, and the rows above are updated the same
+// way with U's column block.  Columns go in chunks that fit the LDS panel.
+// Returns the sign of det(A).
 // ---------------------------------------------------------------------------
 __device__ __noinline__ double wg_lu_solve(double* A, double* B, int N, int nrhs, Lds&) {
     Lds& s = g_lds;
@@ -1754,8 +1846,8 @@ __device__ __noinline__ double wg_lu_solve(double* A, double* B, int N, int nrhs
     N = uni(N);
     nrhs = uni(nrhs);
     A = uni(A);
-    B = uni(B);
     const AS_G double* Ag = gptr(A);
+    const int ncol = N + nrhs;                // columns of [A B]
     for (int k0 = 0; k0 < N; k0 += NB) {
         const int jb = min(NB, N - k0), m = N - k0;
         const FastDiv dm(m);
@@ -1766,9 +1858,9 @@ __device__ __noinline__ double wg_lu_solve(double* A, double* B, int N, int nrhs
         __syncthreads();
         detsign *= lu_panel(m, jb, k0, s);
         pclk.stop(1);
-        for (int e = tid(); e < m * jb; e += NT) {
-            const int c = dm.div(e), r = e - c * m;
-            gptr(A)[(size_t)(k0 + c) * N + k0 + r] = P[c * m + r];
+        {   // U11 back to A (one element per thread)
+            const int r = tid() & (NB - 1), c = tid() >> 4;
+            if (r < jb && c < jb) gptr(A)[(size_t)(k0 + c) * N + k0 + r] = P[c * m + r];
         }
         // the panel's interchanges (row k0+j <-> perm[k0+j], j = 0..jb-1 in order) composed into
         // one gather: row swp_row[i] receives the old row swp_src[i]; rows k0..k0+jb-1 come first
@@ -1796,71 +1888,87 @@ __device__ __noinline__ double wg_lu_solve(double* A, double* B, int N, int nrhs
             }
         }
         __syncthreads();
-        // interchanges + L11^-1 on the columns right of the panel and on B, one column per thread
-        const int nright = N - k0 - jb;
-        for (int c = tid(); c < nright + nrhs; c += NT) {
-            AS_G double* col = gptr(c < nright ? A + (size_t)(k0 + jb + c) * N : B + (size_t)(c - nright) * N);
-            double x[2 * NB];
+        // columns right of the panel, then B, in chunks of cw: interchanges + L11^-1 one column per
+        // thread (U12 to A and to Wl), then C -= L21 U12 on the rows below the panel
+        const int c0 = k0 + jb, nr = ncol - c0, mt = m - jb;
+        double* const Wl = P + (size_t)m * NB;          // NB x cw, ld cw
+        const int cwmax = ((PANEL_DOUBLES - m * NB) / NB) & ~31;
+        for (int q0 = 0; q0 < nr; q0 += cwmax) {
+            const int cw = min(cwmax, nr - q0);
+            for (int c = tid(); c < cw; c += NT) {
+                AS_G double* col = gptr(A + (size_t)(c0 + q0 + c) * N);
+                double x[2 * NB];
 #pragma unroll
-            for (int i = 0; i < 2 * NB; ++i) x[i] = s.swp_row[i] >= 0 ? col[s.swp_src[i]] : 0.0;
+                for (int i = 0; i < 2 * NB; ++i) x[i] = s.swp_row[i] >= 0 ? col[s.swp_src[i]] : 0.0;
 #pragma unroll
-            for (int i = 0; i < 2 * NB; ++i)
-                if (i >= jb && s.swp_row[i] >= 0) col[s.swp_row[i]] = x[i];
+                for (int i = 0; i < 2 * NB; ++i)
+                    if (i >= jb && s.swp_row[i] >= 0) col[s.swp_row[i]] = x[i];
 #pragma unroll
-            for (int j = 0; j < NB; ++j)
+                for (int j = 0; j < NB; ++j)
 #pragma unroll
-                for (int i = j + 1; i < NB; ++i)
-                    if (i < jb) x[i] -= P[j * m + i] * x[j];
+                    for (int i = j + 1; i < NB; ++i)
+                        if (i < jb) x[i] -= P[j * m + i] * x[j];
 #pragma unroll
-            for (int j = 0; j < NB; ++j)
-                if (j < jb) col[k0 + j] = x[j];
-        }
-        __syncthreads();
-        if (jb < m) {
-            if (B == A + (size_t)N * N) {
-                // B continues A in memory: one product over the columns right of the panel and B
-                if (nright + nrhs > 0)
-                    gemm64_lg(m - jb, nright + nrhs, jb, P + jb, m, false, A + (size_t)(k0 + jb) * N + k0, N, false,
-                              A + (size_t)(k0 + jb) * N + k0 + jb, N, 1.0, s, -1.0);
-            } else {
-                if (nright > 0)
-                    gemm64_lg(m - jb, nright, jb, P + jb, m, false, A + (size_t)(k0 + jb) * N + k0, N, false,
-                              A + (size_t)(k0 + jb) * N + k0 + jb, N, 1.0, s, -1.0);
-                if (nrhs > 0)
-                    gemm64_lg(m - jb, nrhs, jb, P + jb, m, false, B + k0, N, false, B + k0 + jb, N, 1.0, s, -1.0);
+                for (int j = 0; j < NB; ++j)
+                    if (j < jb) {
+                        col[k0 + j] = x[j];
+                        Wl[j * cw + c] = x[j];
+                    }
             }
+            __syncthreads();
+            // (rows below the panel exist only when jb = NB, so the depth is NB exactly)
+            if (mt > 0)
+                lds_update<NB, true>(gptr(A + (size_t)(c0 + q0) * N + k0 + jb), N, mt, cw, (const AS_L double*)(P + jb), m,
+                                     (const AS_L double*)Wl, cw, 0, nullptr);
+            __syncthreads();
         }
-        __syncthreads();
     }
     if (nrhs == 0) return detsign;
+    // back substitution: Ucb = U[0:k0, k0:k0+NB] (element (r, c) at c k0 + r, zero for c >= jb), the
+    // block's rows of X in Xl (NB x cw, ld cw, zero rows past jb)
+    double* const Ucb = s.panel;
+    double* const Xl = T11 + NB * NB;
+    const int cwb = ((PANEL_DOUBLES - NMAX * NB - NB * NB) / NB) & ~31;
+    B = uni(B);
     for (int k0 = ((N - 1) / NB) * NB; k0 >= 0; k0 -= NB) {
         const int jb = min(NB, N - k0);
-        for (int e = tid(); e < jb * jb; e += NT) {
-            const int r = e % jb, c = e / jb;
-            T11[c * NB + r] = Ag[(size_t)(k0 + c) * N + k0 + r];
+        {
+            const int r = tid() & (NB - 1), c = tid() >> 4;
+            if (r < jb && c < jb) T11[c * NB + r] = Ag[(size_t)(k0 + c) * N + k0 + r];
+        }
+        if (k0 > 0) {
+            const FastDiv dk(k0);
+            wg_stream(k0 * NB, [&](int e) { const int c = dk.div(e), r = e - c * k0; return Ag[(size_t)(k0 + min(c, jb - 1)) * N + r]; },
+                      [&](int e, double v) { Ucb[e] = dk.div(e) < jb ? v : 0.0; });
         }
         __syncthreads();
-        for (int c = tid(); c < nrhs; c += NT) {
-            AS_G double* col = gptr(B + (size_t)c * N);
-            double x[NB];
+        for (int q0 = 0; q0 < nrhs; q0 += cwb) {
+            const int cw = min(cwb, nrhs - q0);
+            for (int c = tid(); c < cw; c += NT) {
+                AS_G double* col = gptr(B + (size_t)(q0 + c) * N);
+                double x[NB];
 #pragma unroll
-            for (int j = 0; j < NB; ++j) x[j] = (j < jb) ? col[k0 + j] : 0.0;
+                for (int j = 0; j < NB; ++j) x[j] = (j < jb) ? col[k0 + j] : 0.0;
 #pragma unroll
-            for (int j = NB - 1; j >= 0; --j) {
-                if (j < jb) {
-                    x[j] /= T11[j * NB + j];
+                for (int j = NB - 1; j >= 0; --j) {
+                    if (j < jb) {
+                        x[j] /= T11[j * NB + j];
 #pragma unroll
-                    for (int i = 0; i < j; ++i) x[i] -= T11[j * NB + i] * x[j];
+                        for (int i = 0; i < j; ++i) x[i] -= T11[j * NB + i] * x[j];
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
+                    if (j < jb) col[k0 + j] = x[j];
+                    Xl[j * cw + c] = x[j];
                 }
             }
-#pragma unroll
-            for (int j = 0; j < NB; ++j)
-                if (j < jb) col[k0 + j] = x[j];
+            __syncthreads();
+            if (k0 > 0)
+                lds_update<NB, true>(gptr(B + (size_t)q0 * N), N, k0, cw, (const AS_L double*)Ucb, k0,
+                                     (const AS_L double*)Xl, cw, 0, nullptr);
+            __syncthreads();
         }
-        __syncthreads();
-        if (k0 > 0)
-            gemm64(k0, nrhs, jb, A + (size_t)k0 * N, N, false, B + k0, N, false, B, N, nullptr, nullptr, nullptr, 1.0, s,
-                   -1.0);
     }
     return detsign;
 }
@@ -2631,88 +2739,12 @@ __device__ __noinline__ void block_stage_w(const double* G_, int N, int b0, int 
 }
 
 // block_apply:  G += ACP W on wavefronts wave_lo..3 (no workgroup barrier inside, so it can run
-// beside the walk of wavefront 0).  The product is formed transposed (W^T ACP^T), so 16 lanes
-// run down one column of G.  A wavefront takes 2x2 groups of 16x16 blocks (block rows 2gr, 2gr+1,
-// columns 2gc, 2gc+1), so each depth step reads two ACP and two W operands for four MFMAs, and
-// the depth loop is unrolled for the block's padded depth K4 (a template parameter), which keeps
-// the operand reads ahead of the matrix cores; the G loads of the group are in flight together.
-// Each block sums its K4/4 MFMAs in depth order whatever the grouping.
+// beside the walk of wavefront 0), by lds_update with the depth loop unrolled for the block's
+// padded depth K4 (a template parameter).
 template <int K4>
 __device__ __forceinline__ void block_apply_k(AS_G double* G, int N, int wave_lo, AS_L int* next,
                                               const AS_L double* acp, const AS_L double* gsw) {
-    const int l = lane(), w = wave();
-    const int nw = 4 - wave_lo, wi = w - wave_lo;
-    const int lr = l & 15, lk = l >> 4;
-    const int nbr = (N + 15) >> 4, ngr = (nbr + 1) >> 1, ng = ngr * ngr;
-    const FastDiv dg(ngr);
-    // the groups in turn (wavefronts wave_lo..3), or handed out by the LDS counter `next` to
-    // whichever wavefront asks (a wavefront that joins late takes what is left)
-    auto claim = [&]() {
-        int g = 0;
-        if (l == 0) g = __hip_atomic_fetch_add(next, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        return __builtin_amdgcn_readfirstlane(g);
-    };
-    // the G loads of the next group are issued before this group's products, so their latency
-    // runs under the matrix cores
-    double cvn[2][2][4];
-    auto load_group = [&](int g) {
-        const int gc = dg.div(g), gr = g - gc * ngr;
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            const int orow = min((2 * gr + a) * 16 + lr, N - 1);
-#pragma unroll
-            for (int b = 0; b < 2; ++b)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) cvn[a][b][i] = G[(size_t)min((2 * gc + b) * 16 + lk + 4 * i, N - 1) * N + orow];
-        }
-    };
-    int g = next ? claim() : wi;
-    if (g < ng) load_group(g);
-    while (g < ng) {
-        const int gc = dg.div(g), gr = g - gc * ngr;
-        double cv[2][2][4];
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) cv[a][b][i] = cvn[a][b][i];
-        const int gn = next ? claim() : g + nw;
-        if (gn < ng) load_group(gn);
-        int oa[2], ob[2];
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            oa[a] = min((2 * gr + a) * 16 + lr, N - 1);   // (rows and columns past N repeat the last; not stored)
-            ob[a] = min((2 * gc + a) * 16 + lr, N - 1);
-        }
-        dbl4 acc[2][2];
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int kk = 0; kk < K4; kk += 4) {
-            const int kr = (kk + lk) * N;
-            const double a0 = acp[kr + oa[0]], a1 = acp[kr + oa[1]];
-            const double b0 = gsw[kr + ob[0]], b1 = gsw[kr + ob[1]];
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
-        }
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                const int row = (2 * gr + a) * 16 + lr;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int col = (2 * gc + b) * 16 + lk + 4 * i;
-                    if (row < N && col < N) G[(size_t)col * N + row] = cv[a][b][i] + acc[a][b][i];
-                }
-            }
-        g = gn;
-    }
+    lds_update<K4, false>(G, N, N, N, acp, N, gsw, N, wave_lo, next);
 }
 __device__ __noinline__ void block_apply(double* G_, int N, int k, int wave_lo, bool dyn, Lds&) {
     Lds& s = g_lds;
