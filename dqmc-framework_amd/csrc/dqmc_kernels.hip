@@ -2616,10 +2616,14 @@ __device__ __noinline__ double wg_fresh_green(Desc d, int sp, int lslot, int rsl
 //             after block_stage_w has read CR and read by the walk before the
 //             walk writes CR
 //   UV      = the walk's u, v broadcast vectors
-// The panel holds, for one spin at a time, ACP = G0[:,S] (N x k4, ld N) and
-// GSW = G0[S,:] (k4 x N, ld N), which becomes W in place.  Because the walk
-// does not touch the panel, the last spin's G += ACP W of a block runs on
-// wavefronts 1..3 while wavefront 0 walks the next block.
+//   ACPS    = rows S' (the next block's sites) of G0[:,S] for the next block's
+//             G0[S',S'] (element (a, j) at j*KB + a), one spin at a time
+// The panel holds GSW = G0[S,:] (k4 x N, ld N) of each spin of the workgroup
+// side by side (when two fit), which becomes W in place; the block update reads
+// its other operand, G0[:,S], from G itself (each wavefront loads the rows of a
+// 16-row strip before it writes any of them).  Because the walk touches neither,
+// the block updates of the workgroup's spins run on wavefronts 1..3 while
+// wavefront 0 walks the next block.
 constexpr int KP = KB + 1;
 #define MP_BASE (reinterpret_cast<double*>(&g_lds))
 #define MP_CR(sp) (MP_BASE + (sp) * KB * KP)
@@ -2627,17 +2631,20 @@ constexpr int KP = KB + 1;
 #define MP_GT MP_BASE
 static_assert(64 * KB <= 2 * KB * KP, "the lane-interleaved G0[S,S] must fit in two C blocks");
 #define MP_UV (MP_BASE + 4 * KB * KP)
-#define MP_SLACK 16   // reads of partial 16x16 blocks run up to 16 doubles past GSW
-static_assert(offsetof(Lds, As) == 0 && offsetof(Lds, perm) >= (4 * KB * (KB + 1) + 128) * sizeof(double),
+#define MP_ACPS (MP_BASE + 4 * KB * KP + 128)
+#define MP_SLACK 16   // reads of partial 16x16 blocks run up to 16 doubles past the last GSW
+static_assert(offsetof(Lds, As) == 0 && offsetof(Lds, perm) >= (4 * KB * (KB + 1) + 128 + KB * KB) * sizeof(double),
               "the walk's arrays must fit in front of Lds::perm");
 
-// block size: the delay depth, at most KB, and small enough that ACP and GSW fit
+// block size: the delay depth, at most KB, and small enough that one spin's GSW fits in the
+// panel (the same in every launch mapping, so a chain's numbers do not depend on it)
 __device__ __forceinline__ int block_depth(Desc d) {
     const int N = d.N;
     int kb = min(d.kdelay, KB);
-    const int cap = ((PANEL_DOUBLES - MP_SLACK) / (2 * N)) & ~3;
+    const int cap = ((PANEL_DOUBLES - MP_SLACK) / N) & ~3;
     return max(1, min(kb, cap));
 }
+
 
 // G0[S,S] of block (b0, k) of one spin into GSS, zero outside the k x k block
 __device__ void stage_gss(const double* G_, int N, int b0, int k, int sp, Lds&) {
@@ -2706,22 +2713,19 @@ __device__ __forceinline__ void stage_w_product(int N, int b0, int k, const doub
     }
 }
 
-// The block's update of G for one spin, on the matrix cores with operands in LDS.
-// block_stage_w:  ACP = G0[:,S], GSW = G0[S,:], then W = C (G0[S,:] - E_S) over GSW
-__device__ __noinline__ void block_stage_w(const double* G_, int N, int b0, int k, int sp, Lds&) {
+// The block's update of G for one spin, on the matrix cores.
+// block_stage_w:  GSW = G0[S,:] to LDS slot `slot` of the panel, then W = C (G0[S,:] - E_S) over GSW
+__device__ __noinline__ void block_stage_w(const double* G_, int N, int b0, int k, int sp, int slot, Lds&) {
     Lds& s = g_lds;
-    N = uni(N); b0 = uni(b0); k = uni(k); sp = uni(sp);
+    N = uni(N); b0 = uni(b0); k = uni(k); sp = uni(sp); slot = uni(slot);
     const AS_G double* G = gptr(uni(G_));
     const int k4 = (k + 3) & ~3;
     const FastDiv dk4(k4);
-    double* const acp = s.panel;
-    double* const gsw = acp + (size_t)N * k4;
+    double* const gsw = s.panel + (size_t)slot * k4 * N;
     const double* const cr = MP_CR(sp);
-    // ACP and GSW together (GSW element e = c k4 + b: consecutive threads read down a column of G)
-    wg_stream2<16>(N * k4, [&](int e) { return G[(size_t)b0 * N + min(e, N * k - 1)]; },
-               [&](int e, double v) { acp[e] = e < N * k ? v : 0.0; },
-               [&](int e) { const int c = dk4.div(e), b = e - c * k4; return G[(size_t)c * N + b0 + min(b, k - 1)]; },
-               [&](int e, double v) { const int c = dk4.div(e), b = e - c * k4; gsw[b * N + c] = b < k ? v : 0.0; });
+    // GSW element e = c k4 + b: consecutive threads read down a column of G
+    wg_stream<16>(N * k4, [&](int e) { const int c = dk4.div(e), b = e - c * k4; return G[(size_t)c * N + b0 + min(b, k - 1)]; },
+                  [&](int e, double v) { const int c = dk4.div(e), b = e - c * k4; gsw[b * N + c] = b < k ? v : 0.0; });
     __syncthreads();
     // W = C G0[S,:] - C E_S, column block by column block (each wavefront owns whole column blocks,
     // so W may overwrite GSW in place), with the depth loop unrolled for the padded depth k4
@@ -2738,34 +2742,137 @@ __device__ __noinline__ void block_stage_w(const double* G_, int N, int b0, int 
     __syncthreads();
 }
 
-// block_apply:  G += ACP W on wavefronts wave_lo..3 (no workgroup barrier inside, so it can run
-// beside the walk of wavefront 0), by lds_update with the depth loop unrolled for the block's
-// padded depth K4 (a template parameter).
-template <int K4>
-__device__ __forceinline__ void block_apply_k(AS_G double* G, int N, int wave_lo, AS_L int* next,
-                                              const AS_L double* acp, const AS_L double* gsw) {
-    lds_update<K4, false>(G, N, N, N, acp, N, gsw, N, wave_lo, next);
+// block_apply:  G_m += ACP_m W_m (ACP_m = G0_m[:,S], S = b0..b0+k-1) for the nmat (1 or 2) spins
+// of the workgroup, on wavefronts wave_lo..3 (no workgroup barrier inside, so it can run beside
+// the walk of wavefront 0).  The work goes in strips of 16 rows of G (the spins' strips in
+// turn), handed out in turn or by the LDS counter `next`; the wavefront that owns a strip loads
+// the strip's rows of ACP from G into registers before it writes any of its elements, then runs
+// along the strip by steps of CB 16-column blocks, loading each step's G under the previous
+// step's products (and the next strip's ACP and first step under the strip's last).  The
+// product is formed transposed (W^T ACP^T), so 16 lanes run down one column of G; W_m (K4 x N,
+// ld N) is in LDS.  Each block sums its K4/4 MFMAs in depth order.
+template <int K4, int CB>
+__device__ __forceinline__ void block_apply_k(int nmat, AS_G double* G0, AS_G double* G1, const AS_L double* W0,
+                                              const AS_L double* W1, int N, int b0, int k, int wave_lo, AS_L int* next) {
+    constexpr int KQ = K4 / 4;
+    const int l = lane(), w = wave();
+    const int nw = 4 - wave_lo, wi = w - wave_lo;
+    const int lr = l & 15, lk = l >> 4;
+    const int nrb = (N + 15) >> 4, nu = nmat * nrb, ncp = (N + 16 * CB - 1) / (16 * CB);
+    auto claim = [&]() {
+        int g = 0;
+        if (l == 0) g = __hip_atomic_fetch_add(next, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        return __builtin_amdgcn_readfirstlane(g);
+    };
+    double avn[KQ], cvn[CB][4];
+    // ACP of strip u (row rbase + lr, depth 4q + lk; zero past the block's k sites)
+    auto load_a = [&](int u) {
+        const bool m1 = u >= nrb;
+        const AS_G double* const Gm = m1 ? G1 : G0;
+        const int row = min((u - (m1 ? nrb : 0)) * 16 + lr, N - 1);
+#pragma unroll
+        for (int q = 0; q < KQ; ++q) {
+            const int j = 4 * q + lk;
+            const double v = Gm[(size_t)(b0 + min(j, k - 1)) * N + row];
+            avn[q] = j < k ? v : 0.0;
+        }
+    };
+    // G of column step cp of strip u
+    auto load_c = [&](int u, int cp) {
+        const bool m1 = u >= nrb;
+        const AS_G double* const Gm = m1 ? G1 : G0;
+        const int row = min((u - (m1 ? nrb : 0)) * 16 + lr, N - 1);
+#pragma unroll
+        for (int b = 0; b < CB; ++b)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) cvn[b][i] = Gm[(size_t)min((CB * cp + b) * 16 + lk + 4 * i, N - 1) * N + row];
+    };
+    int u = next ? claim() : wi;
+    if (u < nu) {
+        load_a(u);
+        load_c(u, 0);
+    }
+    int cp = 0;
+    double av[KQ];
+    while (u < nu) {
+        if (cp == 0)
+#pragma unroll
+            for (int q = 0; q < KQ; ++q) av[q] = avn[q];
+        double cv[CB][4];
+#pragma unroll
+        for (int b = 0; b < CB; ++b)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) cv[b][i] = cvn[b][i];
+        // the next step: the next column step of this strip, or the first step of the next strip
+        int un = u, cpn = cp + 1;
+        if (cpn == ncp) {
+            un = next ? claim() : u + nw;
+            cpn = 0;
+            if (un < nu) load_a(un);
+        }
+        if (un < nu) load_c(un, cpn);
+        const bool m1 = u >= nrb;
+        AS_G double* const Gm = m1 ? G1 : G0;
+        const AS_L double* const Wm = m1 ? W1 : W0;
+        const int row = (u - (m1 ? nrb : 0)) * 16 + lr;
+        int ob[CB];
+#pragma unroll
+        for (int b = 0; b < CB; ++b) ob[b] = min((CB * cp + b) * 16 + lr, N - 1);   // (columns past N repeat the last; not stored)
+        dbl4 acc[CB];
+#pragma unroll
+        for (int b = 0; b < CB; ++b) acc[b] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int q = 0; q < KQ; ++q) {
+            const int kw = (4 * q + lk) * N;
+            double wv[CB];
+#pragma unroll
+            for (int b = 0; b < CB; ++b) wv[b] = Wm[kw + ob[b]];
+#pragma unroll
+            for (int b = 0; b < CB; ++b) acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(wv[b], av[q], acc[b], 0, 0, 0);
+        }
+#pragma unroll
+        for (int b = 0; b < CB; ++b)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int col = (CB * cp + b) * 16 + lk + 4 * i;
+                if (row < N && col < N) Gm[(size_t)col * N + row] = cv[b][i] + acc[b][i];
+            }
+        u = un;
+        cp = cpn;
+    }
 }
-__device__ __noinline__ void block_apply(double* G_, int N, int k, int wave_lo, bool dyn, Lds&) {
+template <int CB>
+__device__ __forceinline__ void apply_depth(int k4, int nmat, AS_G double* G0, AS_G double* G1, const AS_L double* W0,
+                                            const AS_L double* W1, int N, int b0, int k, int wave_lo, AS_L int* next) {
+    switch (k4) {
+        case 4: block_apply_k<4, CB>(nmat, G0, G1, W0, W1, N, b0, k, wave_lo, next); break;
+        case 8: block_apply_k<8, CB>(nmat, G0, G1, W0, W1, N, b0, k, wave_lo, next); break;
+        case 12: block_apply_k<12, CB>(nmat, G0, G1, W0, W1, N, b0, k, wave_lo, next); break;
+        case 16: block_apply_k<16, CB>(nmat, G0, G1, W0, W1, N, b0, k, wave_lo, next); break;
+        case 20: block_apply_k<20, CB>(nmat, G0, G1, W0, W1, N, b0, k, wave_lo, next); break;
+        case 24: block_apply_k<24, CB>(nmat, G0, G1, W0, W1, N, b0, k, wave_lo, next); break;
+        case 28: block_apply_k<28, CB>(nmat, G0, G1, W0, W1, N, b0, k, wave_lo, next); break;
+        default: block_apply_k<32, CB>(nmat, G0, G1, W0, W1, N, b0, k, wave_lo, next); break;
+    }
+}
+// the block updates (block b0..b0+k-1) of spins lo..hi-1 (their W in panel slots 0.. in order)
+__device__ __noinline__ void block_apply(Desc d, int lo, int hi, int b0, int k, int wave_lo, bool dyn, Lds&) {
     Lds& s = g_lds;
-    N = uni(N); k = uni(k); wave_lo = uni(wave_lo); dyn = uni(dyn);
-    AS_G double* G = gptr(uni(G_));
-    const int k4 = (k + 3) & ~3;
+    const int N = uni(d.N);
+    lo = uni(lo); hi = uni(hi); b0 = uni(b0); k = uni(k); wave_lo = uni(wave_lo); dyn = uni(dyn);
+    const int k4 = (k + 3) & ~3, nmat = hi - lo;
     if (!dyn && wave() < wave_lo) return;
     AS_L int* const next = dyn ? (AS_L int*)&s.ivar[3] : nullptr;
     if (dyn) wave_lo = 0;
-    const AS_L double* const acp = (const AS_L double*)s.panel;
-    const AS_L double* const gsw = acp + (size_t)N * k4;
-    switch (k4) {
-        case 4: block_apply_k<4>(G, N, wave_lo, next, acp, gsw); break;
-        case 8: block_apply_k<8>(G, N, wave_lo, next, acp, gsw); break;
-        case 12: block_apply_k<12>(G, N, wave_lo, next, acp, gsw); break;
-        case 16: block_apply_k<16>(G, N, wave_lo, next, acp, gsw); break;
-        case 20: block_apply_k<20>(G, N, wave_lo, next, acp, gsw); break;
-        case 24: block_apply_k<24>(G, N, wave_lo, next, acp, gsw); break;
-        case 28: block_apply_k<28>(G, N, wave_lo, next, acp, gsw); break;
-        default: block_apply_k<32>(G, N, wave_lo, next, acp, gsw); break;
-    }
+    AS_G double* const G0 = gptr(uni(d.G) + (size_t)lo * N * N);
+    AS_G double* const G1 = gptr(uni(d.G) + (size_t)(hi - 1) * N * N);
+    const AS_L double* const W0 = (const AS_L double*)s.panel;
+    const AS_L double* const W1 = W0 + (size_t)(nmat - 1) * k4 * N;
+    // steps of 4, 3 or 2 column blocks: the widest that pads the fewest columns
+    const int p4 = (N + 63) / 64 * 64, p3 = (N + 47) / 48 * 48, p2 = (N + 31) / 32 * 32;
+    if (p4 <= p3 && p4 <= p2) apply_depth<4>(k4, nmat, G0, G1, W0, W1, N, b0, k, wave_lo, next);
+    else if (p3 <= p2) apply_depth<3>(k4, nmat, G0, G1, W0, W1, N, b0, k, wave_lo, next);
+    else apply_depth<2>(k4, nmat, G0, G1, W0, W1, N, b0, k, wave_lo, next);
 }
 
 // G0[S',S'] of the next block (b1, k1) of one spin, formed from the current
@@ -2773,17 +2880,21 @@ __device__ __noinline__ void block_apply(double* G_, int N, int k, int wave_lo, 
 // (the same update block_apply makes to those elements), into the spin's half
 // of GR only: GT shares its space with the C rows that block_stage_w of the
 // other spin may still read, and is filled by gss_transpose afterwards
-__device__ __noinline__ void gss_from_factors(const double* G_, int N, int k, int b1, int k1, int sp, Lds&) {
+__device__ __noinline__ void gss_from_factors(const double* G_, int N, int b0, int k, int b1, int k1, int sp, int slot,
+                                              Lds&) {
     Lds& s = g_lds;
-    N = uni(N); k = uni(k); b1 = uni(b1); k1 = uni(k1); sp = uni(sp);
+    N = uni(N); b0 = uni(b0); k = uni(k); b1 = uni(b1); k1 = uni(k1); sp = uni(sp); slot = uni(slot);
     const AS_G double* G = gptr(uni(G_));
     const int k4 = (k + 3) & ~3;
-    const double* const acp = s.panel;
-    const double* const gsw = acp + (size_t)N * k4;
+    double* const acps = MP_ACPS;
+    const double* const gsw = s.panel + (size_t)slot * k4 * N;
     double* gr = MP_GR + 32 * sp;
     for (int q = tid(); q < KB * KB; q += NT) gr[(q >> 5) * 64 + (q & 31)] = 0.0;
-    __syncthreads();
     const FastDiv dk1(k1);
+    // ACPS = G0[S',S] (zero past the block's k sites)
+    wg_stream(k1 * k4, [&](int e) { const int j = dk1.div(e), a = e - j * k1; return G[(size_t)(b0 + min(j, k - 1)) * N + b1 + a]; },
+              [&](int e, double v) { const int j = dk1.div(e), a = e - j * k1; acps[j * KB + a] = j < k ? v : 0.0; });
+    __syncthreads();
     // the (at most 4) elements of a thread advance together: four independent sums per depth step
     constexpr int U = KB * KB / NT;
     for (int q0 = tid(); q0 < k1 * k1; q0 += U * NT) {
@@ -2799,7 +2910,7 @@ __device__ __noinline__ void gss_from_factors(const double* G_, int N, int k, in
         }
         for (int j = 0; j < k4; ++j) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) v[u] += acp[j * N + oa[u]] * gsw[j * N + oc[u]];
+            for (int u = 0; u < U; ++u) v[u] += acps[j * KB + oa[u] - b1] * gsw[j * N + oc[u]];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -2807,6 +2918,7 @@ __device__ __noinline__ void gss_from_factors(const double* G_, int N, int k, in
             if (q < k1 * k1) gr[(oc[u] - b1) * 64 + oa[u] - b1] = v[u];
         }
     }
+    __syncthreads();   // the next spin's staging overwrites ACPS
 }
 
 // GT half of one spin from its GR half (the whole 32 x 32, zeros included)
@@ -3073,6 +3185,10 @@ __device__ __noinline__ WalkOut wg_metropolis(Desc d, int e, uint64_t rng, doubl
     const int lo = pair ? me : 0, hi = pair ? me + 1 : ns;
     int naccept = 0;
     const int kb = __builtin_amdgcn_readfirstlane(block_depth(d));
+    // both spins' GSW in the panel together (then both updates run beside the next walk), or one
+    // at a time (the first spin's update is applied before the second is staged)
+    const bool both = hi - lo == 2 && 2 * N * ((kb + 3) & ~3) + MP_SLACK <= PANEL_DOUBLES;
+    const int plo = both ? lo : hi - 1;   // the spins whose updates wait for the next walk: plo..hi-1
     auto Gs = [&](int sp) { return d.G + (size_t)sp * N * N; };
     // G0[S,S] of the first block, both halves of the walk (also in the one-spin model)
     if (pair) {
@@ -3081,16 +3197,16 @@ __device__ __noinline__ WalkOut wg_metropolis(Desc d, int e, uint64_t rng, doubl
     } else {
         for (int sp = 0; sp < 2; ++sp) stage_gss(Gs(sp < ns ? sp : 0), N, 0, min(kb, N), sp, s);
     }
-    bool pending = false;   // the last spin's update of the previous block is still to be applied
-    int pk = 0;
+    bool pending = false;   // the updates of the previous block (pb0, pk) are still to be applied
+    int pb0 = 0, pk = 0;
     for (int b0 = 0; b0 < N; b0 += kb) {
         const int k = min(kb, N - b0);
         if (tid() == 0) { s.ivar[2] = 0; s.ivar[3] = 0; }
         __syncthreads();
         SubClock walk_clk;
         walk_clk.start();
-        // the pending update runs beside the walk on wavefronts 1..3; wavefront 0 joins it when
-        // its walk is done (the update's block groups are handed out through ivar[3])
+        // the pending updates run beside the walk on wavefronts 1..3; wavefront 0 joins them when
+        // its walk is done (the updates' block groups are handed out through ivar[3])
         if (wave() == 0) {
             WalkOut o;
             switch ((k + 7) & ~7) {   // the walk's loops run over the block's depth rounded up to 8
@@ -3103,7 +3219,7 @@ __device__ __noinline__ WalkOut wg_metropolis(Desc d, int e, uint64_t rng, doubl
             sign = o.sign;
             if (lane() == 0) s.ivar[2] = o.accepted;
         }
-        if (pending) block_apply(Gs(hi - 1), N, pk, 1, true, s);
+        if (pending) block_apply(d, plo, hi, pb0, pk, 1, true, s);
         __syncthreads();
         walk_clk.stop(0);
         pending = false;
@@ -3112,21 +3228,26 @@ __device__ __noinline__ WalkOut wg_metropolis(Desc d, int e, uint64_t rng, doubl
         const int b1 = b0 + kb, k1 = min(kb, N - b1);   // the next block (k1 <= 0: none)
         if (nblk > 0) {
             for (int sp = lo; sp < hi; ++sp) {
-                block_stage_w(Gs(sp), N, b0, k, sp, s);
+                const int slot = both ? sp - lo : 0;
+                block_stage_w(Gs(sp), N, b0, k, sp, slot, s);
                 // every spin's next G0[S',S'] comes from the factors, before they are applied (so a
                 // pair launch, where each spin's update is the pending one, computes the same numbers)
-                if (k1 > 0) gss_from_factors(Gs(sp), N, k, b1, k1, sp, s);
-                if (sp == hi - 1 && k1 > 0) {
-                    // the next walk starts from the factors; the last spin's update runs beside it
-                    pending = true;
-                    pk = k;
-                } else {
-                    block_apply(Gs(sp), N, k, 0, false, s);
+                if (k1 > 0) gss_from_factors(Gs(sp), N, b0, k, b1, k1, sp, slot, s);
+                if (sp < plo) {
+                    block_apply(d, sp, sp + 1, b0, k, 0, false, s);
                     __syncthreads();
                 }
             }
-            if (k1 > 0)
+            if (k1 > 0) {
+                // the next walk starts from the factors; the updates of spins plo.. run beside it
+                pending = true;
+                pb0 = b0;
+                pk = k;
                 for (int sp = lo; sp < hi; ++sp) gss_transpose(sp, s);
+            } else {
+                block_apply(d, plo, hi, b0, k, 0, false, s);
+                __syncthreads();
+            }
         }
         if (k1 > 0) {
             if (pair) {
