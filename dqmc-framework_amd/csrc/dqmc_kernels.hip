@@ -1887,33 +1887,77 @@ __device__ __noinline__ double wg_lu_solve(double* A, double* B, int N, int nrhs
                 s.swp_src[i] = src;
             }
         }
+        // L11^-1 (unit lower, zero outside jb x jb) into Li, row-major 16 x 16: lane j of wavefront
+        // 0 forms column j by forward substitution
+        double* const Li = P + (size_t)m * NB;
+        if (wave() == 0 && lane() < NB) {
+            const int j = lane();
+            double y[NB];
+#pragma unroll
+            for (int i = 0; i < NB; ++i) {
+                double v = (i == j && i < jb) ? 1.0 : 0.0;
+                if (i > j && i < jb)
+#pragma unroll
+                    for (int q = 0; q < NB; ++q)
+                        if (q >= j && q < i) v -= P[q * m + i] * y[q];
+                y[i] = v;
+            }
+#pragma unroll
+            for (int i = 0; i < NB; ++i) Li[i * NB + j] = y[i];
+        }
         __syncthreads();
-        // columns right of the panel, then B, in chunks of cw: interchanges + L11^-1 one column per
-        // thread (U12 to A and to Wl), then C -= L21 U12 on the rows below the panel
+        // columns right of the panel, then B, in chunks of cw: the interchanges and U12 = L11^-1 X
+        // by 16-column blocks (lane l holds rows 4s + l/16 of column l%16 of a block: the B operand
+        // of the product with L11^-1; each wavefront takes four blocks at a time with all their
+        // loads in flight), U12 to A and to Wl, then C -= L21 U12 on the rows below the panel
         const int c0 = k0 + jb, nr = ncol - c0, mt = m - jb;
-        double* const Wl = P + (size_t)m * NB;          // NB x cw, ld cw
-        const int cwmax = ((PANEL_DOUBLES - m * NB) / NB) & ~31;
+        double* const Wl = Li + NB * NB;                // NB x cw, ld cw
+        const int cwmax = ((PANEL_DOUBLES - m * NB - NB * NB) / NB) & ~31;
+        const int lr = lane() & 15, lk = lane() >> 4;
+        double la[4];   // L11^-1 as the A operand: Li[l%16][4s + l/16]
+#pragma unroll
+        for (int q = 0; q < 4; ++q) la[q] = Li[lr * NB + 4 * q + lk];
+        int rw[8], rs[8];   // interchange entries i = 4s + l/16 (s < 4) and 16 + 4s + l/16
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int i = (q < 4 ? 4 * q : 16 + 4 * (q - 4)) + lk;
+            rw[q] = s.swp_row[i];
+            rs[q] = max(s.swp_src[i], 0);
+        }
         for (int q0 = 0; q0 < nr; q0 += cwmax) {
-            const int cw = min(cwmax, nr - q0);
-            for (int c = tid(); c < cw; c += NT) {
-                AS_G double* col = gptr(A + (size_t)(c0 + q0 + c) * N);
-                double x[2 * NB];
+            const int cw = min(cwmax, nr - q0), nbk = (cw + 15) >> 4;
+            for (int bb = wave() * 4; bb < nbk; bb += 16) {
+                double xv[4][8];
+                int cl[4];
 #pragma unroll
-                for (int i = 0; i < 2 * NB; ++i) x[i] = s.swp_row[i] >= 0 ? col[s.swp_src[i]] : 0.0;
+                for (int b = 0; b < 4; ++b) {
+                    cl[b] = (bb + b) * 16 + lr;
+                    const AS_G double* col = gptr(A + (size_t)(c0 + q0 + min(cl[b], cw - 1)) * N);
 #pragma unroll
-                for (int i = 0; i < 2 * NB; ++i)
-                    if (i >= jb && s.swp_row[i] >= 0) col[s.swp_row[i]] = x[i];
+                    for (int q = 0; q < 8; ++q) xv[b][q] = col[rs[q]];
+                }
 #pragma unroll
-                for (int j = 0; j < NB; ++j)
+                for (int b = 0; b < 4; ++b) {
+                    if (cl[b] >= cw) continue;
+                    AS_G double* col = gptr(A + (size_t)(c0 + q0 + cl[b]) * N);
+                    dbl4 u = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                    for (int i = j + 1; i < NB; ++i)
-                        if (i < jb) x[i] -= P[j * m + i] * x[j];
-#pragma unroll
-                for (int j = 0; j < NB; ++j)
-                    if (j < jb) {
-                        col[k0 + j] = x[j];
-                        Wl[j * cw + c] = x[j];
+                    for (int q = 0; q < 8; ++q) {
+                        const int i = (q < 4 ? 4 * q : 16 + 4 * (q - 4)) + lk;
+                        if (i >= jb && rw[q] >= 0) col[rw[q]] = xv[b][q];   // rows leaving the panel's rows
                     }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const double x = (4 * q + lk < jb) ? xv[b][q] : 0.0;
+                        u = __builtin_amdgcn_mfma_f64_16x16x4f64(la[q], x, u, 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int r = lk + 4 * e;
+                        if (r < jb) col[k0 + r] = u[e];
+                        Wl[r * cw + cl[b]] = u[e];
+                    }
+                }
             }
             __syncthreads();
             // (rows below the panel exist only when jb = NB, so the depth is NB exactly)
