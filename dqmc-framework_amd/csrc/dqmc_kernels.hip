@@ -2657,7 +2657,7 @@ __device__ __noinline__ double wg_fresh_green(Desc d, int sp, int lslot, int rsl
 //             512-byte access per element) and row p of a spin is a broadcast
 //   GT      = the same for G0[S,S]^T: element (b, a) at b*64 + 32 sp + a (the
 //             columns of G0[S,S]), in the space of CR(0) and CR(1): it is staged
-//             after block_stage_w has read CR and read by the walk before the
+//             after block_stage has read CR and read by the walk before the
 //             walk writes CR
 //   UV      = the walk's u, v broadcast vectors
 //   ACPS    = rows S' (the next block's sites) of G0[:,S] for the next block's
@@ -2757,19 +2757,49 @@ __device__ __forceinline__ void stage_w_product(int N, int b0, int k, const doub
     }
 }
 
-// The block's update of G for one spin, on the matrix cores.
-// block_stage_w:  GSW = G0[S,:] to LDS slot `slot` of the panel, then W = C (G0[S,:] - E_S) over GSW
-__device__ __noinline__ void block_stage_w(const double* G_, int N, int b0, int k, int sp, int slot, Lds&) {
+// The block's update of G for one spin, on the matrix cores, and the next block's G0[S',S'].
+// block_stage:  GSW = G0[S,:] to LDS slot `slot` of the panel, then W = C (G0[S,:] - E_S) over
+// GSW; with a next block (b1, k1 > 0), also its G0[S',S'] = G[S',S'] + ACP[S',:] W[:,S'] formed
+// from these factors before they are applied (the same update block_apply makes to those
+// elements), into this spin's half of GR only (GT shares its space with the C rows that the other
+// spin's staging may still read, and is filled by gss_transpose afterwards).  The loads of
+// G[S',S'] and of ACP[S',:] = G0[S',S] (staged in ACPS) are issued ahead of the GSW stream, so
+// all of the stage's loads are in flight together.
+__device__ __noinline__ void block_stage(const double* G_, int N, int b0, int k, int b1, int k1, int sp, int slot, Lds&) {
     Lds& s = g_lds;
-    N = uni(N); b0 = uni(b0); k = uni(k); sp = uni(sp); slot = uni(slot);
+    N = uni(N); b0 = uni(b0); k = uni(k); b1 = uni(b1); k1 = uni(k1); sp = uni(sp); slot = uni(slot);
     const AS_G double* G = gptr(uni(G_));
     const int k4 = (k + 3) & ~3;
     const FastDiv dk4(k4);
     double* const gsw = s.panel + (size_t)slot * k4 * N;
+    double* const acps = MP_ACPS;
+    double* const gr = MP_GR + 32 * sp;
     const double* const cr = MP_CR(sp);
+    const bool nxt = k1 > 0;
+    const int kn = nxt ? k1 : 1;
+    const FastDiv dkn(kn);
+    // the thread's (at most 4) elements of G0[S',S'] (q = c k1 + a) and of ACPS (q = j k1 + a)
+    constexpr int U = KB * KB / NT;
+    double gv[U], av[U];
+    int ga[U], gc[U], aj[U], aa[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int q = tid() + u * NT;
+        const int qg = min(q, kn * kn - 1), c = dkn.div(qg), a = qg - c * kn;   // (clamped; not stored)
+        const int qa = min(q, kn * k4 - 1), j = dkn.div(qa), a2 = qa - j * kn;
+        ga[u] = a; gc[u] = c; aj[u] = j; aa[u] = a2;
+        gv[u] = nxt ? G[(size_t)(b1 + c) * N + b1 + a] : 0.0;
+        av[u] = nxt ? G[(size_t)(b0 + min(j, k - 1)) * N + b1 + a2] : 0.0;
+    }
+    if (nxt)
+        for (int q = tid(); q < KB * KB; q += NT) gr[(q >> 5) * 64 + (q & 31)] = 0.0;
     // GSW element e = c k4 + b: consecutive threads read down a column of G
     wg_stream<16>(N * k4, [&](int e) { const int c = dk4.div(e), b = e - c * k4; return G[(size_t)c * N + b0 + min(b, k - 1)]; },
                   [&](int e, double v) { const int c = dk4.div(e), b = e - c * k4; gsw[b * N + c] = b < k ? v : 0.0; });
+    if (nxt)
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (tid() + u * NT < k1 * k4) acps[aj[u] * KB + aa[u]] = aj[u] < k ? av[u] : 0.0;
     __syncthreads();
     // W = C G0[S,:] - C E_S, column block by column block (each wavefront owns whole column blocks,
     // so W may overwrite GSW in place), with the depth loop unrolled for the padded depth k4
@@ -2784,6 +2814,17 @@ __device__ __noinline__ void block_stage_w(const double* G_, int N, int b0, int 
         default: stage_w_product<32>(N, b0, k, cr, gsw); break;
     }
     __syncthreads();
+    if (nxt) {
+        // G0[S',S']: the (at most 4) elements of a thread advance together, four independent sums
+        for (int j = 0; j < k4; ++j) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) gv[u] += acps[j * KB + ga[u]] * gsw[j * N + b1 + gc[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (tid() + u * NT < k1 * k1) gr[gc[u] * 64 + ga[u]] = gv[u];
+        __syncthreads();   // the next spin's staging overwrites ACPS
+    }
 }
 
 // block_apply:  G_m += ACP_m W_m (ACP_m = G0_m[:,S], S = b0..b0+k-1) for the nmat (1 or 2) spins
@@ -2917,52 +2958,6 @@ __device__ __noinline__ void block_apply(Desc d, int lo, int hi, int b0, int k, 
     if (p4 <= p3 && p4 <= p2) apply_depth<4>(k4, nmat, G0, G1, W0, W1, N, b0, k, wave_lo, next);
     else if (p3 <= p2) apply_depth<3>(k4, nmat, G0, G1, W0, W1, N, b0, k, wave_lo, next);
     else apply_depth<2>(k4, nmat, G0, G1, W0, W1, N, b0, k, wave_lo, next);
-}
-
-// G0[S',S'] of the next block (b1, k1) of one spin, formed from the current
-// block's factors before they are applied: G[S',S'] + ACP[S',:] W[:,S']
-// (the same update block_apply makes to those elements), into the spin's half
-// of GR only: GT shares its space with the C rows that block_stage_w of the
-// other spin may still read, and is filled by gss_transpose afterwards
-__device__ __noinline__ void gss_from_factors(const double* G_, int N, int b0, int k, int b1, int k1, int sp, int slot,
-                                              Lds&) {
-    Lds& s = g_lds;
-    N = uni(N); b0 = uni(b0); k = uni(k); b1 = uni(b1); k1 = uni(k1); sp = uni(sp); slot = uni(slot);
-    const AS_G double* G = gptr(uni(G_));
-    const int k4 = (k + 3) & ~3;
-    double* const acps = MP_ACPS;
-    const double* const gsw = s.panel + (size_t)slot * k4 * N;
-    double* gr = MP_GR + 32 * sp;
-    for (int q = tid(); q < KB * KB; q += NT) gr[(q >> 5) * 64 + (q & 31)] = 0.0;
-    const FastDiv dk1(k1);
-    // ACPS = G0[S',S] (zero past the block's k sites)
-    wg_stream(k1 * k4, [&](int e) { const int j = dk1.div(e), a = e - j * k1; return G[(size_t)(b0 + min(j, k - 1)) * N + b1 + a]; },
-              [&](int e, double v) { const int j = dk1.div(e), a = e - j * k1; acps[j * KB + a] = j < k ? v : 0.0; });
-    __syncthreads();
-    // the (at most 4) elements of a thread advance together: four independent sums per depth step
-    constexpr int U = KB * KB / NT;
-    for (int q0 = tid(); q0 < k1 * k1; q0 += U * NT) {
-        int oa[U], oc[U];
-        double v[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int q = min(q0 + u * NT, k1 * k1 - 1);   // (clamped; not stored)
-            const int c = dk1.div(q), a = q - c * k1;
-            oa[u] = b1 + a;
-            oc[u] = b1 + c;
-            v[u] = G[(size_t)(b1 + c) * N + b1 + a];
-        }
-        for (int j = 0; j < k4; ++j) {
-#pragma unroll
-            for (int u = 0; u < U; ++u) v[u] += acps[j * KB + oa[u] - b1] * gsw[j * N + oc[u]];
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int q = q0 + u * NT;
-            if (q < k1 * k1) gr[(oc[u] - b1) * 64 + oa[u] - b1] = v[u];
-        }
-    }
-    __syncthreads();   // the next spin's staging overwrites ACPS
 }
 
 // GT half of one spin from its GR half (the whole 32 x 32, zeros included)
@@ -3272,11 +3267,9 @@ __device__ __noinline__ WalkOut wg_metropolis(Desc d, int e, uint64_t rng, doubl
         const int b1 = b0 + kb, k1 = min(kb, N - b1);   // the next block (k1 <= 0: none)
         if (nblk > 0) {
             for (int sp = lo; sp < hi; ++sp) {
-                const int slot = both ? sp - lo : 0;
-                block_stage_w(Gs(sp), N, b0, k, sp, slot, s);
-                // every spin's next G0[S',S'] comes from the factors, before they are applied (so a
+                // (every spin's next G0[S',S'] comes from the factors, before they are applied, so a
                 // pair launch, where each spin's update is the pending one, computes the same numbers)
-                if (k1 > 0) gss_from_factors(Gs(sp), N, b0, k, b1, k1, sp, slot, s);
+                block_stage(Gs(sp), N, b0, k, b1, k1, sp, both ? sp - lo : 0, s);
                 if (sp < plo) {
                     block_apply(d, sp, sp + 1, b0, k, 0, false, s);
                     __syncthreads();
