@@ -119,7 +119,9 @@ typedef struct {
     int checkerboard;        /* 0 dense exp(-dtau K), 1 checkerboard break-up (even side only) */
     int delay;               /* delayed-update depth k, 1..32 (a block of the submatrix walk takes at most 32
                                 sites: one per lane of a half-wavefront and spin); larger k is DQMC_ERR_ARG;
-                                1 applies every flip at once */
+                                1 applies every flip at once.  Blocks take fewer sites where one spin's
+                                k x N block of G does not fit in the workgroup's LDS (28 at 21 x 21); the
+                                cap depends on N only, never on the launch mapping */
     int momentum_index;      /* k-star index for momentum_distribution/SDW/CDW; -1 = M point (pi,pi) */
     int precision;           /* DQMC_PREC_* */
     double beta;
