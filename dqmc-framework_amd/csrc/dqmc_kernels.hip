@@ -2662,7 +2662,7 @@ __device__ __noinline__ double wg_fresh_green(Desc d, int sp, int lslot, int rsl
 //   UV      = the walk's u, v broadcast vectors
 //   ACPS    = rows S' (the next block's sites) of G0[:,S] for the next block's
 //             G0[S',S'] (element (a, j) at j*KB + a), one spin at a time
-// The panel holds GSW = G0[S,:] (k4 x N, ld N) of each spin of the workgroup
+// The panel holds GSW = G0[S,:] (k4 x N, ld N + 1) of each spin of the workgroup
 // side by side (when two fit), which becomes W in place; the block update reads
 // its other operand, G0[:,S], from G itself (each wavefront loads the rows of a
 // 16-row strip before it writes any of them).  Because the walk touches neither,
@@ -2677,6 +2677,9 @@ static_assert(64 * KB <= 2 * KB * KP, "the lane-interleaved G0[S,S] must fit in 
 #define MP_UV (MP_BASE + 4 * KB * KP)
 #define MP_ACPS (MP_BASE + 4 * KB * KP + 128)
 #define MP_SLACK 16   // reads of partial 16x16 blocks run up to 16 doubles past the last GSW
+// leading dimension of GSW / W: odd, so that the 32 lanes storing one column of G0[S,:] (rows
+// b = 0..31 of GSW) fall on different LDS banks
+__device__ __forceinline__ int gsw_ld(int N) { return N + 1; }
 static_assert(offsetof(Lds, As) == 0 && offsetof(Lds, perm) >= (4 * KB * (KB + 1) + 128 + KB * KB) * sizeof(double),
               "the walk's arrays must fit in front of Lds::perm");
 
@@ -2685,7 +2688,7 @@ static_assert(offsetof(Lds, As) == 0 && offsetof(Lds, perm) >= (4 * KB * (KB + 1
 __device__ __forceinline__ int block_depth(Desc d) {
     const int N = d.N;
     int kb = min(d.kdelay, KB);
-    const int cap = ((PANEL_DOUBLES - MP_SLACK) / N) & ~3;
+    const int cap = ((PANEL_DOUBLES - MP_SLACK) / gsw_ld(N)) & ~3;
     return max(1, min(kb, cap));
 }
 
@@ -2706,7 +2709,7 @@ __device__ void stage_gss(const double* G_, int N, int b0, int k, int sp, Lds&) 
 // to 4 of its column blocks together; all loads are unconditional (clamped indices) so the
 // compiler keeps them in flight.  Rows 16..31 of C are zero when K4 <= 16 (not multiplied).
 template <int K4>
-__device__ __forceinline__ void stage_w_product(int N, int b0, int k, const double* cr, double* gsw) {
+__device__ __forceinline__ void stage_w_product(int N, int ldw, int b0, int k, const double* cr, double* gsw) {
     constexpr bool hi = K4 > 16;
     const int l = lane(), w = wave();
     const int nbc = (N + 15) >> 4;
@@ -2727,7 +2730,7 @@ __device__ __forceinline__ void stage_w_product(int N, int b0, int k, const doub
             const double a1 = hi ? cr[(16 + lr) * KP + kr] : 0.0;
             double bv[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) bv[u] = gsw[kr * N + colq[u]];
+            for (int u = 0; u < 4; ++u) bv[u] = gsw[kr * ldw + colq[u]];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 acc0[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv[u], acc0[u], 0, 0, 0);
@@ -2750,8 +2753,8 @@ __device__ __forceinline__ void stage_w_product(int N, int b0, int k, const doub
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int a = lk + 4 * i;
-                if (colok && a < K4) gsw[a * N + col] = acc0[u][i] - ((in_s && a < k) ? corr0[i] : 0.0);
-                if (hi && colok && 16 + a < K4) gsw[(16 + a) * N + col] = acc1[u][i] - ((in_s && 16 + a < k) ? corr1[i] : 0.0);
+                if (colok && a < K4) gsw[a * ldw + col] = acc0[u][i] - ((in_s && a < k) ? corr0[i] : 0.0);
+                if (hi && colok && 16 + a < K4) gsw[(16 + a) * ldw + col] = acc1[u][i] - ((in_s && 16 + a < k) ? corr1[i] : 0.0);
             }
         }
     }
@@ -2771,7 +2774,8 @@ __device__ __noinline__ void block_stage(const double* G_, int N, int b0, int k,
     const AS_G double* G = gptr(uni(G_));
     const int k4 = (k + 3) & ~3;
     const FastDiv dk4(k4);
-    double* const gsw = s.panel + (size_t)slot * k4 * N;
+    const int ldw = gsw_ld(N);
+    double* const gsw = s.panel + (size_t)slot * k4 * ldw;
     double* const acps = MP_ACPS;
     double* const gr = MP_GR + 32 * sp;
     const double* const cr = MP_CR(sp);
@@ -2795,7 +2799,7 @@ __device__ __noinline__ void block_stage(const double* G_, int N, int b0, int k,
         for (int q = tid(); q < KB * KB; q += NT) gr[(q >> 5) * 64 + (q & 31)] = 0.0;
     // GSW element e = c k4 + b: consecutive threads read down a column of G
     wg_stream<16>(N * k4, [&](int e) { const int c = dk4.div(e), b = e - c * k4; return G[(size_t)c * N + b0 + min(b, k - 1)]; },
-                  [&](int e, double v) { const int c = dk4.div(e), b = e - c * k4; gsw[b * N + c] = b < k ? v : 0.0; });
+                  [&](int e, double v) { const int c = dk4.div(e), b = e - c * k4; gsw[b * ldw + c] = b < k ? v : 0.0; });
     if (nxt)
 #pragma unroll
         for (int u = 0; u < U; ++u)
@@ -2804,21 +2808,21 @@ __device__ __noinline__ void block_stage(const double* G_, int N, int b0, int k,
     // W = C G0[S,:] - C E_S, column block by column block (each wavefront owns whole column blocks,
     // so W may overwrite GSW in place), with the depth loop unrolled for the padded depth k4
     switch (k4) {
-        case 4: stage_w_product<4>(N, b0, k, cr, gsw); break;
-        case 8: stage_w_product<8>(N, b0, k, cr, gsw); break;
-        case 12: stage_w_product<12>(N, b0, k, cr, gsw); break;
-        case 16: stage_w_product<16>(N, b0, k, cr, gsw); break;
-        case 20: stage_w_product<20>(N, b0, k, cr, gsw); break;
-        case 24: stage_w_product<24>(N, b0, k, cr, gsw); break;
-        case 28: stage_w_product<28>(N, b0, k, cr, gsw); break;
-        default: stage_w_product<32>(N, b0, k, cr, gsw); break;
+        case 4: stage_w_product<4>(N, ldw, b0, k, cr, gsw); break;
+        case 8: stage_w_product<8>(N, ldw, b0, k, cr, gsw); break;
+        case 12: stage_w_product<12>(N, ldw, b0, k, cr, gsw); break;
+        case 16: stage_w_product<16>(N, ldw, b0, k, cr, gsw); break;
+        case 20: stage_w_product<20>(N, ldw, b0, k, cr, gsw); break;
+        case 24: stage_w_product<24>(N, ldw, b0, k, cr, gsw); break;
+        case 28: stage_w_product<28>(N, ldw, b0, k, cr, gsw); break;
+        default: stage_w_product<32>(N, ldw, b0, k, cr, gsw); break;
     }
     __syncthreads();
     if (nxt) {
         // G0[S',S']: the (at most 4) elements of a thread advance together, four independent sums
         for (int j = 0; j < k4; ++j) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) gv[u] += acps[j * KB + ga[u]] * gsw[j * N + b1 + gc[u]];
+            for (int u = 0; u < U; ++u) gv[u] += acps[j * KB + ga[u]] * gsw[j * ldw + b1 + gc[u]];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u)
@@ -2835,10 +2839,11 @@ __device__ __noinline__ void block_stage(const double* G_, int N, int b0, int k,
 // along the strip by steps of CB 16-column blocks, loading each step's G under the previous
 // step's products (and the next strip's ACP and first step under the strip's last).  The
 // product is formed transposed (W^T ACP^T), so 16 lanes run down one column of G; W_m (K4 x N,
-// ld N) is in LDS.  Each block sums its K4/4 MFMAs in depth order.
+// ld N + 1) is in LDS.  Each block sums its K4/4 MFMAs in depth order.
 template <int K4, int CB>
 __device__ __forceinline__ void block_apply_k(int nmat, AS_G double* G0, AS_G double* G1, const AS_L double* W0,
                                               const AS_L double* W1, int N, int b0, int k, int wave_lo, AS_L int* next) {
+    const int ldw = gsw_ld(N);
     constexpr int KQ = K4 / 4;
     const int l = lane(), w = wave();
     const int nw = 4 - wave_lo, wi = w - wave_lo;
@@ -2908,7 +2913,7 @@ __device__ __forceinline__ void block_apply_k(int nmat, AS_G double* G0, AS_G do
         for (int b = 0; b < CB; ++b) acc[b] = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int q = 0; q < KQ; ++q) {
-            const int kw = (4 * q + lk) * N;
+            const int kw = (4 * q + lk) * ldw;
             double wv[CB];
 #pragma unroll
             for (int b = 0; b < CB; ++b) wv[b] = Wm[kw + ob[b]];
@@ -2952,7 +2957,7 @@ __device__ __noinline__ void block_apply(Desc d, int lo, int hi, int b0, int k, 
     AS_G double* const G0 = gptr(uni(d.G) + (size_t)lo * N * N);
     AS_G double* const G1 = gptr(uni(d.G) + (size_t)(hi - 1) * N * N);
     const AS_L double* const W0 = (const AS_L double*)s.panel;
-    const AS_L double* const W1 = W0 + (size_t)(nmat - 1) * k4 * N;
+    const AS_L double* const W1 = W0 + (size_t)(nmat - 1) * k4 * gsw_ld(N);
     // steps of 4, 3 or 2 column blocks: the widest that pads the fewest columns
     const int p4 = (N + 63) / 64 * 64, p3 = (N + 47) / 48 * 48, p2 = (N + 31) / 32 * 32;
     if (p4 <= p3 && p4 <= p2) apply_depth<4>(k4, nmat, G0, G1, W0, W1, N, b0, k, wave_lo, next);
@@ -3226,7 +3231,7 @@ __device__ __noinline__ WalkOut wg_metropolis(Desc d, int e, uint64_t rng, doubl
     const int kb = __builtin_amdgcn_readfirstlane(block_depth(d));
     // both spins' GSW in the panel together (then both updates run beside the next walk), or one
     // at a time (the first spin's update is applied before the second is staged)
-    const bool both = hi - lo == 2 && 2 * N * ((kb + 3) & ~3) + MP_SLACK <= PANEL_DOUBLES;
+    const bool both = hi - lo == 2 && 2 * gsw_ld(N) * ((kb + 3) & ~3) + MP_SLACK <= PANEL_DOUBLES;
     const int plo = both ? lo : hi - 1;   // the spins whose updates wait for the next walk: plo..hi-1
     auto Gs = [&](int sp) { return d.G + (size_t)sp * N * N; };
     // G0[S,S] of the first block, both halves of the walk (also in the one-spin model)
