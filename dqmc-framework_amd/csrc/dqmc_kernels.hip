@@ -2970,8 +2970,10 @@ __device__ void gss_transpose(int sp, Lds&) {
     __syncthreads();
     const double* gr = MP_GR + 32 * sp;
     double* gt = MP_GT + 32 * sp;
+    // diagonal order (lane t of group g takes a = t, c = (t + g) mod 32), so that neither the reads
+    // nor the writes of 32 lanes fall on one LDS bank (both arrays have rows 64 doubles apart)
     for (int q = tid(); q < KB * KB; q += NT) {
-        const int a = q & 31, c = q >> 5;
+        const int a = q & 31, c = ((q >> 5) + a) & 31;
         gt[a * 64 + c] = gr[c * 64 + a];
     }
 }
