@@ -1362,7 +1362,11 @@ __device__ __noinline__ void kron_right(const double* M_, double* W_, int N, int
     const AS_L double* const cpre = (const AS_L double*)cpre_;
     const AS_L double* const cpost = (const AS_L double*)cpost_;
     AS_L double* const T = (AS_L double*)s.panel;            // 16 rows x N: column c at c LDR
+    // the pass-2 output, 16 rows x N: column c = x' + side y' at y' bw + x' LDR, with the stride bw
+    // between the side-column groups 4 mod 32 doubles (a lane group stores 16 columns side apart)
+    const int bw = side * KRON_LDR + ((4 - side * KRON_LDR) & 31);
     AS_L double* const Wl = T + (size_t)N * KRON_LDR;
+    const FastDiv dside(side);
     const int l = lane(), w = wave();
     const int i = l & 15, kq = l >> 4;
     // B operands: X[4s + l/16][l%16] (pass 1) and Y[4s + l/16][l%16] (pass 2)
@@ -1429,15 +1433,15 @@ __device__ __noinline__ void kron_right(const double* M_, double* W_, int N, int
                     const int c = xp + side * i;
                     const double cq = has_post ? cpost[c] : 1.0;
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) Wl[c * KRON_LDR + kq + 4 * e] = has_post ? r[e] * cq : r[e];
+                    for (int e = 0; e < 4; ++e) Wl[i * bw + xp * KRON_LDR + kq + 4 * e] = has_post ? r[e] * cq : r[e];
                 }
             }
         }
         __syncthreads();
         const int nr = min(16, N - r0);
         for (int q = tid(); q < 16 * N; q += NT) {
-            const int c = q >> 4, rr = q & 15;
-            if (rr < nr) W[(size_t)c * N + r0 + rr] = Wl[c * KRON_LDR + rr];
+            const int c = q >> 4, rr = q & 15, yq = dside.div(c);
+            if (rr < nr) W[(size_t)c * N + r0 + rr] = Wl[yq * bw + (c - yq * side) * KRON_LDR + rr];
         }
     }
     __syncthreads();
