@@ -2561,26 +2561,30 @@ __device__ __noinline__ void wg_transpose_scaled(double* dst1_, double* dst2_, c
     // LDS views of the scalings; the null tests below use the generic pointers
     const AS_L double* sc = (const AS_L double*)uni(sc_);
     const AS_L double* sa = (const AS_L double*)uni(sa_);
-    const int nb = max(1, min(N, PANEL_DOUBLES / N));
+    // (the block's columns sit N + 1 doubles apart in LDS, so the transposed reads of consecutive
+    // threads fall on different banks)
+    const int ldb = N + 1;
+    const FastDiv dN(N);
+    const int nb = max(1, min(N, PANEL_DOUBLES / ldb));
     for (int j0 = 0; j0 < N; j0 += nb) {
         const int nc = min(nb, N - j0);
         const FastDiv dnc(nc);
         const AS_G double* blk = src + (size_t)j0 * N;
-        wg_stream(nc * N, [&](int e) { return blk[e]; }, [&](int e, double v) { s.panel[e] = v; });
+        wg_stream(nc * N, [&](int e) { return blk[e]; }, [&](int e, double v) { s.panel[e + dN.div(e)] = v; });
         __syncthreads();
-        // element (r, c) of dst with r = j0 + e % nc, c = e / nc is src(c, r) = panel[c + (r - j0) N]
+        // element (r, c) of dst with r = j0 + e % nc, c = e / nc is src(c, r) = panel[c + (r - j0) ldb]
         if (dst2_) {
             wg_stream(nc * N, [&](int e) { const int c = dnc.div(e), r = j0 + e - c * nc; return add[(size_t)c * N + r]; },
                       [&](int e, double a) {
                           const int c = dnc.div(e), rr = e - c * nc, r = j0 + rr;
-                          const double v = (sc_ ? sc[r] : 1.0) * s.panel[c + rr * N];
+                          const double v = (sc_ ? sc[r] : 1.0) * s.panel[c + rr * ldb];
                           dst1[(size_t)c * N + r] = v;
                           dst2[(size_t)c * N + r] = v + sa[r] * a;
                       });
         } else {
             for (int e = tid(); e < nc * N; e += NT) {
                 const int c = dnc.div(e), rr = e - c * nc, r = j0 + rr;
-                dst1[(size_t)c * N + r] = (sc_ ? sc[r] : 1.0) * s.panel[c + rr * N];
+                dst1[(size_t)c * N + r] = (sc_ ? sc[r] : 1.0) * s.panel[c + rr * ldb];
             }
         }
         __syncthreads();
