@@ -2058,16 +2058,16 @@ __device__ __forceinline__ void qr_panel_wave(int m, int jb, int k0, int N, AS_G
         double part[NB];
 #pragma unroll
         for (int q = 0; q < NB; ++q) part[q] = 0.0;
+        // (rows outside r > j enter as zero: the same sums for finite data, and no branches, so
+        // the rows' dependence chains can interleave)
 #pragma unroll
         for (int i = 0; i < RPL; ++i) {
             const int r = L + 64 * i;
-            if (r > j && r < m) {
-                const double xj = x[i][j];
-                part[0] += xj * xj;
+            const double xj = (r > j && r < m) ? x[i][j] : 0.0;
+            part[0] += xj * xj;
 #pragma unroll
-                for (int q = 1; q < NB; ++q)
-                    if (j + q < NB) part[q] += xj * x[i][j + q];
-            }
+            for (int q = 1; q < NB; ++q)
+                if (j + q < NB) part[q] += xj * x[i][j + q];
         }
         if (L == j)
 #pragma unroll
@@ -2077,13 +2077,10 @@ __device__ __forceinline__ void qr_panel_wave(int m, int jb, int k0, int N, AS_G
         __builtin_amdgcn_wave_barrier();
         const double x0 = red[RED_ROWJ];
         const double nrm = sqrt(x0 * x0 + red[0]);
-        double tk = 0.0, inv_v0 = 1.0, alpha = x0;
-        if (nrm != 0.0) {
-            alpha = x0 > 0 ? -nrm : nrm;
-            const double v0 = x0 - alpha;
-            tk = -v0 / alpha;
-            inv_v0 = 1.0 / v0;
-        }
+        const bool nz = nrm != 0.0;
+        const double alpha_r = x0 > 0 ? -nrm : nrm, v0 = x0 - alpha_r;
+        const double tk_r = -v0 / alpha_r, inv_r = 1.0 / v0;   // (discarded for a zero column)
+        const double tk = nz ? tk_r : 0.0, inv_v0 = nz ? inv_r : 1.0, alpha = nz ? alpha_r : x0;
         if (L == 0) tau[j] = tk;
         double wq[NB];
 #pragma unroll
@@ -2093,20 +2090,18 @@ __device__ __forceinline__ void qr_panel_wave(int m, int jb, int k0, int N, AS_G
 #pragma unroll
         for (int i = 0; i < RPL; ++i) {
             const int r = L + 64 * i;
-            if (r > j && r < m && tk != 0.0) {
-                const double v = x[i][j] * inv_v0;
-                x[i][j] = v;
-#pragma unroll
-                for (int q = 1; q < NB; ++q)
-                    if (j + q < NB) x[i][j + q] -= wq[q] * v;
-            }
-        }
-        if (L == j) {
-            x[0][j] = alpha;
+            const bool act = r > j && r < m && tk != 0.0;
+            const double v = x[i][j] * inv_v0, va = act ? v : 0.0;
+            x[i][j] = act ? v : x[i][j];
 #pragma unroll
             for (int q = 1; q < NB; ++q)
-                if (j + q < NB) x[0][j + q] -= wq[q];
+                if (j + q < NB) x[i][j + q] -= wq[q] * va;
         }
+        const bool lj = L == j;   // lane j holds row j in x[0]
+        x[0][j] = lj ? alpha : x[0][j];
+#pragma unroll
+        for (int q = 1; q < NB; ++q)
+            if (j + q < NB) x[0][j + q] -= lj ? wq[q] : 0.0;
       }
     }
     // R to Mp, explicit V to M
@@ -2123,29 +2118,32 @@ __device__ __forceinline__ void qr_panel_wave(int m, int jb, int k0, int N, AS_G
             }
         }
     }
-    // Gram matrix G(l, j) = V(j, l) + sum_{r > j} V(r, l) V(r, j) (l < j), 8 groups of 16 pairs
+    // Gram matrix G(l, j) = V(j, l) + sum_{r > j} V(r, l) V(r, j) (l < j), pair j (j - 1) / 2 + l,
+    // in groups of 16 pairs (the pair counter is a constant of the unrolled loops, so every
+    // operand is a fixed register)
     double* const gram = red + RED_GRAM;
+    double part[16];
 #pragma unroll
-    for (int g = 0; g < 8; ++g) {
-        double part[16];
+    for (int j = 1; j < NB; ++j) {
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            part[q] = 0.0;
-            const int pidx = g * 16 + q;
-            int j = 1;
-            while ((j + 1) * j / 2 <= pidx) ++j;
-            const int l = pidx - j * (j - 1) / 2;
-            if (j < NB && j < jb) {
+        for (int l = 0; l < j; ++l) {
+            const int pidx = j * (j - 1) / 2 + l;
+            // (pairs with j >= jb are formed too and not read: T takes j < jb only)
+            double acc = 0.0;
 #pragma unroll
-                for (int i = 0; i < RPL; ++i) {
-                    const int r = L + 64 * i;
-                    if (r < m && r > j) part[q] += x[i][l] * x[i][j];
-                    else if (r < m && r == j) part[q] += x[i][l];
-                }
+            for (int i = 0; i < RPL; ++i) {
+                const int r = L + 64 * i;
+                const double f = (r < m && r > j) ? x[i][j] : ((r < m && r == j) ? 1.0 : 0.0);
+                acc += x[i][l] * f;   // (V(r, l) V(r, j) below the diagonal, V(j, l) on it)
+            }
+            part[pidx & 15] = acc;
+            if ((pidx & 15) == 15 || pidx == NB * (NB - 1) / 2 - 1) {
+#pragma unroll
+                for (int q = (pidx & 15) + 1; q < 16; ++q) part[q] = 0.0;
+                const double wsum = wave_reduce16(part);
+                if ((L & 3) == 0) gram[4 * 128 + (pidx & ~15) + ((L >> 2) & 15)] = wsum;
             }
         }
-        const double wsum = wave_reduce16(part);
-        if ((L & 3) == 0) gram[4 * 128 + g * 16 + ((L >> 2) & 15)] = wsum;
     }
     __builtin_amdgcn_wave_barrier();
     // T: lane i builds row i, T(i, j) = -tau_j sum_{l=i}^{j-1} T(i, l) G(l, j), T(j, j) = tau_j
