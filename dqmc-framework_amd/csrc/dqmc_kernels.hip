@@ -1894,17 +1894,17 @@ __device__ __noinline__ double wg_lu_solve(double* A, double* B, int N, int nrhs
         // L11^-1 (unit lower, zero outside jb x jb) into Li, row-major 16 x 16: lane j of wavefront
         // 0 forms column j by forward substitution
         double* const Li = P + (size_t)m * NB;
+        // (the sums run over every q < i: y[q] = 0 for q < j adds exact zeros, and the loops keep no
+        // per-lane branch; rows past jb read past the panel and are discarded)
         if (wave() == 0 && lane() < NB) {
             const int j = lane();
             double y[NB];
 #pragma unroll
             for (int i = 0; i < NB; ++i) {
-                double v = (i == j && i < jb) ? 1.0 : 0.0;
-                if (i > j && i < jb)
+                double v = i == j ? 1.0 : 0.0;
 #pragma unroll
-                    for (int q = 0; q < NB; ++q)
-                        if (q >= j && q < i) v -= P[q * m + i] * y[q];
-                y[i] = v;
+                for (int q = 0; q < i; ++q) v -= P[q * m + i] * y[q];
+                y[i] = i < jb ? v : 0.0;
             }
 #pragma unroll
             for (int i = 0; i < NB; ++i) Li[i * NB + j] = y[i];
@@ -2147,21 +2147,19 @@ __device__ __forceinline__ void qr_panel_wave(int m, int jb, int k0, int N, AS_G
     }
     __builtin_amdgcn_wave_barrier();
     // T: lane i builds row i, T(i, j) = -tau_j sum_{l=i}^{j-1} T(i, l) G(l, j), T(j, j) = tau_j
+    // (the sums start at l = 0: T(i, l) = 0 for l < i adds exact zeros, and the loops keep no
+    // per-lane branch)
     double* const Tb = s.panel + PANEL_BLK;
     if (L < NB) {
         const int i = L;
         double trow[NB];
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
-            trow[j] = 0.0;
-            if (j < jb && i == j) trow[j] = tau[j];
-            else if (j < jb && i < j) {
-                double acc = 0.0;
+            double acc = 0.0;
 #pragma unroll
-                for (int l = 0; l < NB; ++l)
-                    if (l >= i && l < j) acc += trow[l] * gram[4 * 128 + j * (j - 1) / 2 + l];
-                trow[j] = -tau[j] * acc;
-            }
+            for (int l = 0; l < j; ++l) acc += trow[l] * gram[4 * 128 + j * (j - 1) / 2 + l];
+            const double tj = tau[min(j, jb - 1)];
+            trow[j] = (j < jb && i == j) ? tj : ((j < jb && i < j) ? -tj * acc : 0.0);
         }
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
