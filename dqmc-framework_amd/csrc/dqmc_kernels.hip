@@ -215,6 +215,7 @@ __device__ __forceinline__ double wave_reduce16(const double (&v)[16]) {
     return a1;
 }
 
+
 // uniform values in scalar registers: function arguments of device functions
 // arrive in vector registers and would otherwise count as divergent
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -422,6 +423,15 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
 #define AS_G __attribute__((address_space(1)))
 #define AS_L __attribute__((address_space(3)))
+// An LDS row address as an opaque register value: the element offsets of the
+// reads along the row then fold into the LDS instructions' offset fields
+// (otherwise the compiler reassociates the row base with the array base and
+// materializes one address register per read).
+__device__ __forceinline__ const AS_L double* lds_row(const double* p) {
+    unsigned v = (unsigned)(uintptr_t)(const AS_L double*)p;
+    asm volatile("" : "+v"(v));
+    return (const AS_L double*)(uintptr_t)v;
+}
 #define AS_C __attribute__((address_space(4)))
 // A chain's descriptor, read through the constant address space: its fields are
 // uniform, so they load through the scalar cache (a generic reference would make
@@ -2336,16 +2346,18 @@ __device__ __noinline__ void qr_panel(int m, int jb, int k0, int N, double* M_, 
 // W1 = V^T C and, for s = 4 rb + e, element e of the 16x16 result block rb of V W2, so the
 // update C - V W2 is elementwise in registers.  V (m x jb) is staged once in the LDS panel
 // at an odd leading dimension (rows padded to a multiple of 16 with zeros, columns to 16),
-// so that lanes reading one row at 16 different columns fall on different banks.
-template <int MQ>
+// so that lanes reading one row at 16 different columns fall on different banks.  The instance
+// takes MB = ceil(m / 16) 16-row blocks exactly, so its loops carry no branch on the row count.
+template <int MB>
 __device__ __noinline__ void wg_reflect_regs(const double* V_, int ldv, const double* T_, bool tT, double* C_, int ldc,
                                              int m, int n, int jb) {
+    constexpr int MQ = 4 * MB;
     Lds& s = g_lds;
     ldv = uni(ldv); tT = uni(tT); ldc = uni(ldc); m = uni(m); n = uni(n); jb = uni(jb);
     const AS_G double* const V = gptr(uni(V_));
     const AS_G double* const T = gptr(uni(T_));
     AS_G double* const C = gptr(uni(C_));
-    const int mb = (m + 15) >> 4, mp = mb * 16, ldp = mp + 1;
+    constexpr int mp = MB * 16, ldp = mp + 1;
     AS_L double* const Vl = (AS_L double*)s.panel;   // 16 x ldp
     {
         const FastDiv dmp(mp);
@@ -2365,42 +2377,94 @@ __device__ __noinline__ void wg_reflect_regs(const double* V_, int ldv, const do
     for (int cb = w; cb < ncb; cb += 4) {
         const int col = cb * 16 + li;
         const bool colok = col < n;
-        const AS_G double* const cc = C + (size_t)(colok ? col : 0) * ldc;
+        // (rows below the last 16-row block are all inside m: one base, constant offsets; the last
+        // block's rows past m repeat row m - 1.  Nothing is zeroed: a lane's column of W1, W2 and
+        // C - V W2 depends on that column of C only, a column past n is not stored, and the rows
+        // of V past m are zero, so the repeated rows add nothing)
+        AS_G double* const cc = C + (size_t)(colok ? col : 0) * ldc;
         double c[MQ];
 #pragma unroll
         for (int q = 0; q < MQ; ++q) {
             const int r = 4 * q + lk;
-            c[q] = (colok && r < m) ? cc[r] : 0.0;
+            c[q] = q < MQ - 4 ? cc[r] : cc[min(r, m - 1)];
         }
-        // W1 = V^T C: A operand V^T[l%16][4s + l/16] = V[4s + l/16][l%16]
+        // W1 = V^T C: A operand V^T[l%16][4s + l/16] = V[4s + l/16][l%16] (the operands of the next
+        // four steps read under the current four's MFMAs; the fences keep the loads from all being
+        // hoisted to the top)
         dbl4 w1 = dbl4{0.0, 0.0, 0.0, 0.0};
+        {
+            const AS_L double* const vr = lds_row((const double*)(Vl + li * ldp + lk));
+            double vo[2][4];
 #pragma unroll
-        for (int q = 0; q < MQ; ++q)
-            if (q < 4 * mb) w1 = __builtin_amdgcn_mfma_f64_16x16x4f64(Vl[li * ldp + 4 * q + lk], c[q], w1, 0, 0, 0);
+            for (int j = 0; j < 4; ++j) vo[0][j] = vr[4 * j];
+#pragma unroll
+            for (int g = 0; g < MB; ++g) {
+                const int cur = g & 1;
+                if (g + 1 < MB)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) vo[cur ^ 1][j] = vr[4 * (4 * g + 4 + j)];
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) w1 = __builtin_amdgcn_mfma_f64_16x16x4f64(vo[cur][j], c[4 * g + j], w1, 0, 0, 0);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
         // W2 = op(T) W1
         dbl4 w2 = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int s4 = 0; s4 < 4; ++s4) w2 = __builtin_amdgcn_mfma_f64_16x16x4f64(tf[s4], w1[s4], w2, 0, 0, 0);
         // C -= V W2 by 16-row blocks: A operand V[16 rb + l%16][4s + l/16]
+        {
+            const AS_L double* const vc = lds_row((const double*)(Vl + lk * ldp + li));
+            double vo[2][4];
 #pragma unroll
-        for (int rb = 0; rb < MQ / 4; ++rb) {
-            if (rb < mb) {
+            for (int s4 = 0; s4 < 4; ++s4) vo[0][s4] = vc[4 * s4 * ldp];
+#pragma unroll
+            for (int rb = 0; rb < MB; ++rb) {
+                const int cur = rb & 1;
+                if (rb + 1 < MB)
+#pragma unroll
+                    for (int s4 = 0; s4 < 4; ++s4) vo[cur ^ 1][s4] = vc[4 * s4 * ldp + 16 * (rb + 1)];
+                __builtin_amdgcn_sched_barrier(0);
                 dbl4 pv = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                for (int s4 = 0; s4 < 4; ++s4)
-                    pv = __builtin_amdgcn_mfma_f64_16x16x4f64(Vl[(4 * s4 + lk) * ldp + 16 * rb + li], w2[s4], pv, 0, 0, 0);
+                for (int s4 = 0; s4 < 4; ++s4) pv = __builtin_amdgcn_mfma_f64_16x16x4f64(vo[cur][s4], w2[s4], pv, 0, 0, 0);
 #pragma unroll
                 for (int e = 0; e < 4; ++e) c[4 * rb + e] = c[4 * rb + e] - pv[e];
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
-        AS_G double* const co = C + (size_t)(colok ? col : 0) * ldc;
+        if (colok) {
 #pragma unroll
-        for (int q = 0; q < MQ; ++q) {
-            const int r = 4 * q + lk;
-            if (colok && r < m) co[r] = c[q];
+            for (int q = 0; q < MQ; ++q) {
+                const int r = 4 * q + lk;
+                if (q < MQ - 4 || r < m) cc[r] = c[q];
+            }
         }
     }
     __syncthreads();
+}
+// the instance for m <= 256 rows (16 * MB >= m)
+__device__ __forceinline__ void reflect_regs(const double* V, int ldv, const double* T, bool tT, double* C, int ldc, int m,
+                                             int n, int jb) {
+    switch ((m + 15) >> 4) {
+        case 1: wg_reflect_regs<1>(V, ldv, T, tT, C, ldc, m, n, jb); break;
+        case 2: wg_reflect_regs<2>(V, ldv, T, tT, C, ldc, m, n, jb); break;
+        case 3: wg_reflect_regs<3>(V, ldv, T, tT, C, ldc, m, n, jb); break;
+        case 4: wg_reflect_regs<4>(V, ldv, T, tT, C, ldc, m, n, jb); break;
+        case 5: wg_reflect_regs<5>(V, ldv, T, tT, C, ldc, m, n, jb); break;
+        case 6: wg_reflect_regs<6>(V, ldv, T, tT, C, ldc, m, n, jb); break;
+        case 7: wg_reflect_regs<7>(V, ldv, T, tT, C, ldc, m, n, jb); break;
+        case 8: wg_reflect_regs<8>(V, ldv, T, tT, C, ldc, m, n, jb); break;
+        case 9: wg_reflect_regs<9>(V, ldv, T, tT, C, ldc, m, n, jb); break;
+        case 10: wg_reflect_regs<10>(V, ldv, T, tT, C, ldc, m, n, jb); break;
+        case 11: wg_reflect_regs<11>(V, ldv, T, tT, C, ldc, m, n, jb); break;
+        case 12: wg_reflect_regs<12>(V, ldv, T, tT, C, ldc, m, n, jb); break;
+        case 13: wg_reflect_regs<13>(V, ldv, T, tT, C, ldc, m, n, jb); break;
+        case 14: wg_reflect_regs<14>(V, ldv, T, tT, C, ldc, m, n, jb); break;
+        case 15: wg_reflect_regs<15>(V, ldv, T, tT, C, ldc, m, n, jb); break;
+        default: wg_reflect_regs<16>(V, ldv, T, tT, C, ldc, m, n, jb); break;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2470,8 +2534,7 @@ __device__ __noinline__ void wg_qr_udv(double* M, double* Mp, int N, double* U, 
         if (n2 > 0) {
             const double* V = M + (size_t)k0 * N + k0;
             double* A2 = Mp + (size_t)(k0 + jb) * N + k0;
-            if (m <= 144) wg_reflect_regs<36>(V, N, Tg + (size_t)(k0 / NB) * NB * NB, true, A2, N, m, n2, jb);
-            else if (m <= 256) wg_reflect_regs<64>(V, N, Tg + (size_t)(k0 / NB) * NB * NB, true, A2, N, m, n2, jb);
+            if (m <= 256) reflect_regs(V, N, Tg + (size_t)(k0 / NB) * NB * NB, true, A2, N, m, n2, jb);
             else wg_reflect(V, N, Tg + (size_t)(k0 / NB) * NB * NB, true, A2, N, m, n2, jb, Wsm, Wsm2, s);
         }
     }
@@ -2495,8 +2558,7 @@ __device__ __noinline__ void wg_qr_udv(double* M, double* Mp, int N, double* U, 
         const double* V = M + (size_t)k0 * N + k0;
         const double* T = Tg + (size_t)(k0 / NB) * NB * NB;
         double* U2 = U + (size_t)k0 * N + k0;
-        if (m <= 144) wg_reflect_regs<36>(V, N, T, false, U2, N, m, m, jb);
-        else if (m <= 256) wg_reflect_regs<64>(V, N, T, false, U2, N, m, m, jb);
+        if (m <= 256) reflect_regs(V, N, T, false, U2, N, m, m, jb);
         else wg_reflect(V, N, T, false, U2, N, m, m, jb, Wsm, Wsm2, s);
     }
 }
@@ -2996,15 +3058,6 @@ struct WalkOut {
     double sign;
     int accepted;
 };
-// An LDS row address as an opaque register value: the element offsets of the
-// reads along the row then fold into the LDS instructions' offset fields
-// (otherwise the compiler reassociates the row base with the array base and
-// materializes one address register per read).
-__device__ __forceinline__ const AS_L double* lds_row(const double* p) {
-    unsigned v = (unsigned)(uintptr_t)(const AS_L double*)p;
-    asm volatile("" : "+v"(v));
-    return (const AS_L double*)(uintptr_t)v;
-}
 template <int KW>
 __device__ __forceinline__ WalkOut walk_block(Desc d, int e, int b0, int k, uint64_t rng, double sign,
                                            bool record, Lds&) {
