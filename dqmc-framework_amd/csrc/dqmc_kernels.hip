@@ -1546,15 +1546,14 @@ __device__ void wg_wrap(Desc d, int sp, bool up, double* G, double* W, Lds&) {
 // and pivots contribute to det.
 // ---------------------------------------------------------------------------
 // One wavefront factors a panel of m <= 64 RPL rows: lane L holds rows L + 64 i (i < RPL) in
-// registers, the pivot is a wavefront argmax (DPP and row swaps) and rows j and p are broadcast
-// through LDS inside the wavefront, so a column step needs no workgroup barrier.  Same
+// registers, the pivot is a wavefront argmax (DPP and row swaps) and rows j and p are read out of
+// their lanes into uniform registers, so a column step needs no barrier at all.  Same
 // operations in the same order as the four-wavefront form below (bit-identical factors).
 template <int RPL>
 __device__ __forceinline__ double lu_panel_wave(int m, int jb, int k0) {
     Lds& s = g_lds;
     const int L = lane();
     double* const P = s.panel;
-    double* const rowbuf = s.panel + PANEL_RED + RED_CAND;   // row j at [0, NB), row p at [NB, 2 NB)
     double x[RPL][NB];
 #pragma unroll
     for (int i = 0; i < RPL; ++i)
@@ -1581,24 +1580,21 @@ __device__ __forceinline__ double lu_panel_wave(int m, int jb, int k0) {
         const int p = __builtin_amdgcn_readfirstlane(bi);
         const int pl = p & 63, ps = p >> 6;
         if (L == 0) s.perm[k0 + j] = k0 + p;
-        if (L == j)
+        // row p (lane pl, register row ps) read out lane by lane into uniform registers, and row j
+        // (lane j, register row 0) only for an interchange
+        double rowp[NB];
 #pragma unroll
-            for (int c = 0; c < NB; ++c) rowbuf[c] = x[0][c];
-        if (L == pl)
+        for (int c = 0; c < NB; ++c) {
+            double v = x[0][c];
 #pragma unroll
-            for (int c = 0; c < NB; ++c) {
-                double v = x[0][c];
-#pragma unroll
-                for (int i = 1; i < RPL; ++i) v = (ps == i) ? x[i][c] : v;
-                rowbuf[NB + c] = v;
-            }
-        __builtin_amdgcn_wave_barrier();
-        double rowj[NB], rowp[NB];
-#pragma unroll
-        for (int c = 0; c < NB; ++c) { rowj[c] = rowbuf[c]; rowp[c] = rowbuf[NB + c]; }
-        __builtin_amdgcn_wave_barrier();
+            for (int i = 1; i < RPL; ++i) v = (ps == i) ? x[i][c] : v;
+            rowp[c] = readlane_d(v, pl);
+        }
         if (p != j) {
             detsign = -detsign;
+            double rowj[NB];
+#pragma unroll
+            for (int c = 0; c < NB; ++c) rowj[c] = readlane_d(x[0][c], j);
             if (L == pl)
 #pragma unroll
                 for (int i = 0; i < RPL; ++i)
