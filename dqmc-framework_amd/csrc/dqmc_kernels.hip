@@ -2039,8 +2039,8 @@ __device__ __noinline__ double wg_lu_solve(double* A, double* B, int N, int nrhs
 // s.v1[k0..]; T to Tb (LDS, ld NB) and Tg.
 // ---------------------------------------------------------------------------
 // One wavefront factors a panel of m <= 64 RPL rows (lane L holds rows L + 64 i): the column's
-// norm and dot products are reduce-scattered within the wavefront and broadcast through LDS,
-// so a column step needs no workgroup barrier.  Same Householder steps, Gram matrix and T
+// norm and dot products are reduce-scattered within the wavefront and read out of their lanes
+// into uniform registers, so a column step needs no barrier at all.  Same Householder steps, Gram matrix and T
 // factor as the four-wavefront form below (sums in a different order).
 template <int RPL>
 __device__ __forceinline__ void qr_panel_wave(int m, int jb, int k0, int N, AS_G double* M, AS_G double* Mp,
@@ -2075,14 +2075,11 @@ __device__ __forceinline__ void qr_panel_wave(int m, int jb, int k0, int N, AS_G
             for (int q = 1; q < NB; ++q)
                 if (j + q < NB) part[q] += xj * x[i][j + q];
         }
-        if (L == j)
-#pragma unroll
-            for (int q = 0; q < NB; ++q) red[RED_ROWJ + q] = (j + q < NB) ? x[0][j + q] : 0.0;
+        // row j (lane j, register row 0) and the slot totals (slot q on lane 4q) read out of their
+        // lanes into uniform registers
         const double wsum = wave_reduce16(part);
-        if ((L & 3) == 0) red[(L >> 2) & 15] = wsum;   // slot totals at the front of the scratch
-        __builtin_amdgcn_wave_barrier();
-        const double x0 = red[RED_ROWJ];
-        const double nrm = sqrt(x0 * x0 + red[0]);
+        const double x0 = readlane_d(x[0][j], j);
+        const double nrm = sqrt(x0 * x0 + readlane_d(wsum, 0));
         const bool nz = nrm != 0.0;
         const double alpha_r = x0 > 0 ? -nrm : nrm, v0 = x0 - alpha_r;
         const double tk_r = -v0 / alpha_r, inv_r = 1.0 / v0;   // (discarded for a zero column)
@@ -2091,8 +2088,7 @@ __device__ __forceinline__ void qr_panel_wave(int m, int jb, int k0, int N, AS_G
         double wq[NB];
 #pragma unroll
         for (int q = 1; q < NB; ++q)
-            wq[q] = (j + q < NB) ? tk * (red[RED_ROWJ + q] + inv_v0 * red[q]) : 0.0;
-        __builtin_amdgcn_wave_barrier();
+            wq[q] = (j + q < NB) ? tk * (readlane_d(x[0][j + q], j) + inv_v0 * readlane_d(wsum, 4 * q)) : 0.0;
 #pragma unroll
         for (int i = 0; i < RPL; ++i) {
             const int r = L + 64 * i;
