@@ -2855,9 +2855,12 @@ __device__ __noinline__ void block_stage(const double* G_, int N, int b0, int k,
     }
     if (nxt)
         for (int q = tid(); q < KB * KB; q += NT) gr[(q >> 5) * 64 + (q & 31)] = 0.0;
-    // GSW element e = c k4 + b: consecutive threads read down a column of G
-    wg_stream<16>(N * k4, [&](int e) { const int c = dk4.div(e), b = e - c * k4; return G[(size_t)c * N + b0 + min(b, k - 1)]; },
-                  [&](int e, double v) { const int c = dk4.div(e), b = e - c * k4; gsw[b * ldw + c] = b < k ? v : 0.0; });
+    // GSW element e = c k4 + b: consecutive threads read down a column of G (in one round of loads
+    // when the block has at most 18 elements per thread, e.g. 32 x 144)
+    auto gld = [&](int e) { const int c = dk4.div(e), b = e - c * k4; return G[(size_t)c * N + b0 + min(b, k - 1)]; };
+    auto gst = [&](int e, double v) { const int c = dk4.div(e), b = e - c * k4; gsw[b * ldw + c] = b < k ? v : 0.0; };
+    if (N * k4 > 16 * NT && N * k4 <= 18 * NT) wg_stream<18>(N * k4, gld, gst);
+    else wg_stream<16>(N * k4, gld, gst);
     if (nxt)
 #pragma unroll
         for (int u = 0; u < U; ++u)
