@@ -2972,14 +2972,23 @@ __device__ __forceinline__ void block_apply_k(int nmat, AS_G double* G0, AS_G do
         dbl4 acc[CB];
 #pragma unroll
         for (int b = 0; b < CB; ++b) acc[b] = dbl4{0.0, 0.0, 0.0, 0.0};
+        // (W operands double-buffered: step q + 1's LDS reads are issued before step q's MFMAs, so
+        // their latency runs under the matrix cores instead of between the steps)
+        double wv[2][CB];
+#pragma unroll
+        for (int b = 0; b < CB; ++b) wv[0][b] = Wm[lk * ldw + ob[b]];
 #pragma unroll
         for (int q = 0; q < KQ; ++q) {
-            const int kw = (4 * q + lk) * ldw;
-            double wv[CB];
+            const int cur = q & 1;
+            if (q + 1 < KQ) {
+                const int kw = (4 * q + 4 + lk) * ldw;
 #pragma unroll
-            for (int b = 0; b < CB; ++b) wv[b] = Wm[kw + ob[b]];
+                for (int b = 0; b < CB; ++b) wv[cur ^ 1][b] = Wm[kw + ob[b]];
+            }
+            __builtin_amdgcn_sched_barrier(0);   // (keeps the reads ahead of the MFMAs)
 #pragma unroll
-            for (int b = 0; b < CB; ++b) acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(wv[b], av[q], acc[b], 0, 0, 0);
+            for (int b = 0; b < CB; ++b) acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(wv[cur][b], av[q], acc[b], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
         for (int b = 0; b < CB; ++b)
