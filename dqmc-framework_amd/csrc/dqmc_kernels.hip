@@ -1809,15 +1809,25 @@ __device__ __forceinline__ void lds_update(AS_G double* C, int ldc, int M, int N
         for (int a = 0; a < 2; ++a)
 #pragma unroll
             for (int b = 0; b < 2; ++b) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+        // (operands double-buffered: the next depth step's LDS reads are issued ahead of the
+        // current step's MFMAs, so their latency runs under the matrix cores)
+        double ao[2][2], bo[2][2];
+        ao[0][0] = A[lk * lda + oa[0]]; ao[0][1] = A[lk * lda + oa[1]];
+        bo[0][0] = W[lk * ldw + ob[0]]; bo[0][1] = W[lk * ldw + ob[1]];
 #pragma unroll
         for (int kk = 0; kk < K4; kk += 4) {
-            const int ka = (kk + lk) * lda, kw = (kk + lk) * ldw;
-            const double a0 = A[ka + oa[0]], a1 = A[ka + oa[1]];
-            const double b0 = W[kw + ob[0]], b1 = W[kw + ob[1]];
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
+            const int cur = (kk >> 2) & 1;
+            if (kk + 4 < K4) {
+                const int ka = (kk + 4 + lk) * lda, kw = (kk + 4 + lk) * ldw;
+                ao[cur ^ 1][0] = A[ka + oa[0]]; ao[cur ^ 1][1] = A[ka + oa[1]];
+                bo[cur ^ 1][0] = W[kw + ob[0]]; bo[cur ^ 1][1] = W[kw + ob[1]];
+            }
+            __builtin_amdgcn_sched_barrier(0);   // (keeps the reads ahead of the MFMAs)
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(bo[cur][0], ao[cur][0], acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(bo[cur][1], ao[cur][0], acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(bo[cur][0], ao[cur][1], acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(bo[cur][1], ao[cur][1], acc[1][1], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
         for (int a = 0; a < 2; ++a)
