@@ -2050,8 +2050,8 @@ __device__ __noinline__ double wg_lu_solve(double* A, double* B, int N, int nrhs
 // ---------------------------------------------------------------------------
 // One wavefront factors a panel of m <= 64 RPL rows (lane L holds rows L + 64 i): the column's
 // norm and dot products are reduce-scattered within the wavefront and read out of their lanes
-// into uniform registers, so a column step needs no barrier at all.  Same Householder steps, Gram matrix and T
-// factor as the four-wavefront form below (sums in a different order).
+// into uniform registers, so a column step needs no barrier at all.  Same Householder steps,
+// Gram matrix and T factor as the four-wavefront form below (sums in a different order).
 template <int RPL>
 __device__ __forceinline__ void qr_panel_wave(int m, int jb, int k0, int N, AS_G double* M, AS_G double* Mp,
                                               AS_G double* Tg) {
